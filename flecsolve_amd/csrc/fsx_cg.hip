@@ -1,0 +1,643 @@
+// fsx_cg.hip -- fused CG driver with device-resident scalars, HIP-graph
+// captured iterations, and FP32-inner / FP64-outer iterative refinement.
+//
+// Algorithm: op::cg::apply (flecsolve/solvers/cg.hh:43-139), unchanged in
+// its iteration sequence, tolerance semantics and solve_info.  What changes
+// is where the scalars live.  The reference forces a host round trip at every
+// .get() (three per iteration: w.p, |r|, r.z -- cg.hh:98, 111, 127).  Here
+// one iteration is three streaming kernels, each dot product finishing in a
+// one-block kernel on the device:
+//
+//   K1  w = A p, partials of w.p           (cg.hh:95-98)    SpMV + dot
+//   F1  alpha = rho / w.p                  (cg.hh:98-106)
+//   K2  x += a p; r -= a w; z = D^-1 r; partials of r.r and r.z
+//                                          (cg.hh:108-127)  one HBM pass
+//   F2  |r|, tolerance test, rho, beta     (cg.hh:111-129)
+//   K3  p = beta p + z                     (cg.hh:130)
+//
+// F1/F2 (deterministic fixed-order reductions) update alpha, rho, beta, the
+// residual history and a device `done` flag; every kernel returns
+// immediately once `done` is set, so a captured chunk of iterations that
+// overshoots convergence costs only empty launches and the reported
+// iteration count is exact.  Chunks of iterations are captured into
+// a HIP graph and replayed; the host reads the flag once per chunk with one
+// chunk in flight.
+#include "fsx_spmv_kernels.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+namespace fsx {
+
+struct cg_state {
+	double rho0, rho1, alpha, beta, res, tol, pw;
+	int iter, done, status, iters, maxiter, ncurv;
+};
+
+// ---- K1: w = A p and the partials of p.w; F1: alpha ---------------------
+
+template<class ACC, class T>
+struct epi_cg_spmv {
+	static constexpr int NV = 1;
+	T * w;
+	const T * p;
+	double * partials;
+	const cg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void row(int64_t i, ACC s, double (&d)[NV]) const {
+		const T wi = (T)s;
+		w[i] = wi;
+		d[0] = (double)wi * (double)p[i];
+	}
+	__device__ void block_end(const double (&d)[NV], unsigned blk,
+	                          unsigned) const {
+		store_block_partials<NV>(d, partials, blk);
+	}
+};
+
+// alpha = rho / (w.p)  (cg.hh:98-106; negative curvature is only logged)
+struct fin_alpha {
+	cg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void operator()(const double (&t)[1]) const {
+		const double pw = t[0];
+		st->pw = pw;
+		if (pw <= 0.0) st->ncurv++;
+		st->alpha = st->rho1 / pw;
+	}
+};
+
+// ---- K2: x, r update, partials of |r|^2 and r.z; F2: test and beta ------
+
+constexpr int kUpdThreads = 256;
+constexpr int kUpdPerThread = 4;
+
+template<class T, bool JAC>
+__global__ __launch_bounds__(kUpdThreads) void cg_update_kernel(
+	size_t n, T * __restrict__ x, T * __restrict__ r, const T * __restrict__ p,
+	const T * __restrict__ w, const T * __restrict__ dinv, T * __restrict__ z,
+	double * partials, const cg_state * st) {
+	if (st->done) return;
+	const double alpha = st->alpha;
+	const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
+	double rr = 0.0, rz = 0.0;
+#pragma unroll
+	for (int j = 0; j < kUpdPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			// x.axpy(alpha, p, x); r.axpy(-alpha, w, r)  (cg.hh:108-109)
+			if constexpr (std::is_same_v<T, double>) {
+				x[i] = alpha * p[i] + x[i];
+				const double ri = (-alpha) * w[i] + r[i];
+				r[i] = ri;
+				rr += ri * ri;
+				if constexpr (JAC) {
+					const double zi = 0.0 + dinv[i] * ri;
+					z[i] = zi;
+					rz += ri * zi;
+				}
+			}
+			else {
+				// FP32 storage, FP64 arithmetic and reductions
+				x[i] = (T)(alpha * (double)p[i] + (double)x[i]);
+				const T ri = (T)((-alpha) * (double)w[i] + (double)r[i]);
+				r[i] = ri;
+				rr += (double)ri * (double)ri;
+				if constexpr (JAC) {
+					const T zi = (T)((double)dinv[i] * (double)ri);
+					z[i] = zi;
+					rz += (double)ri * (double)zi;
+				}
+			}
+		}
+	}
+	double d[2] = {rr, rz};
+	store_block_partials<2>(d, partials, blk);
+}
+
+// |r| = sqrt(r.r); diagnostic-free tolerance test; rho and beta
+// (cg.hh:111-129)
+template<bool JAC>
+struct fin_beta {
+	cg_state * st;
+	double * hist;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void operator()(const double (&t)[2]) const {
+		const double res = sqrt(t[0]);
+		st->res = res;
+		const int it = st->iter;
+		if (hist) hist[it] = res;
+		st->iter = it + 1;
+		if (res < st->tol) {
+			st->iters = it + 1;
+			st->status = FSX_CONVERGED_RTOL;
+			st->done = 1;
+			return;
+		}
+		if (it + 1 >= st->maxiter) { // loop exhausted
+			st->done = 1;
+			return;
+		}
+		st->rho0 = st->rho1;
+		st->rho1 = JAC ? t[1] : t[0];
+		st->beta = st->rho1 / st->rho0;
+	}
+};
+
+// ---- K3: p = beta p + z ---------------------------------------------------
+
+template<class T>
+__global__ __launch_bounds__(256) void cg_pupdate_kernel(
+	size_t n, T * __restrict__ p, const T * __restrict__ z, const cg_state * st) {
+	if (st->done) return;
+	const double beta = st->beta;
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) {
+		if constexpr (std::is_same_v<T, double>) p[i] = beta * p[i] + z[i];
+		else p[i] = (T)(beta * (double)p[i] + (double)z[i]);
+	}
+}
+
+// ---- residual + norm (FP64 outer refinement) --------------------------------
+
+template<class ACC, class T>
+struct epi_res_norm {
+	static constexpr int NV = 1;
+	T * r;
+	const T * b;
+	double * partials;
+	__device__ bool skip() const { return false; }
+	__device__ void row(int64_t i, ACC s, double (&d)[NV]) const {
+		const T ri = b[i] - (T)s;
+		r[i] = ri;
+		d[0] = (double)ri * (double)ri;
+	}
+	__device__ void block_end(const double (&d)[NV], unsigned blk,
+	                          unsigned) const {
+		store_block_partials<NV>(d, partials, blk);
+	}
+};
+
+// x += s * d (FP64 x, FP32 d) with s read from device memory (|r|)
+__global__ void ir_update_kernel(size_t n, double * __restrict__ x,
+                                 const float * __restrict__ d,
+                                 const double * rnorm2) {
+	const double s = sqrt(*rnorm2);
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) x[i] = x[i] + s * (double)d[i];
+}
+
+// r32 = (float)(r / |r|)
+__global__ void ir_scale_kernel(size_t n, const double * __restrict__ r,
+                                float * __restrict__ r32,
+                                const double * rnorm2) {
+	const double inv = 1.0 / sqrt(*rnorm2);
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) r32[i] = (float)(r[i] * inv);
+}
+
+} // namespace fsx
+
+// Per-context solver cache: pooled work vectors (so their addresses are stable
+// from one solve to the next), the device state and history buffers, and the
+// captured iteration chunks keyed by everything a captured launch depends on.
+struct cg_graph_cache {
+	struct entry {
+		hipGraph_t g = nullptr;
+		hipGraphExec_t ge = nullptr;
+	};
+	using key_t = std::tuple<const void *, const void *, const void *, const void *,
+	                         const void *, const void *, const void *, const void *,
+	                         const void *, int64_t, int64_t, int, int, int, int>;
+	std::map<key_t, entry> m;
+	std::map<std::tuple<size_t, int, int>, fsx_vec *> pool;
+	fsx::dbuf * statebuf = nullptr;
+	fsx::dbuf * histbuf = nullptr;
+	size_t hist_cap = 0;
+	void clear_graphs() {
+		for (auto & kv : m) {
+			if (kv.second.ge) (void)hipGraphExecDestroy(kv.second.ge);
+			if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+		}
+		m.clear();
+	}
+	~cg_graph_cache() {
+		clear_graphs();
+		for (auto & kv : pool) fsx_vec_release(kv.second);
+		delete statebuf;
+		delete histbuf;
+	}
+};
+
+void cg_cache_deleter::operator()(cg_graph_cache * c) const { delete c; }
+
+void fsx::invalidate_graphs(fsx_ctx * ctx) {
+	if (ctx->cg_cache) ctx->cg_cache->clear_graphs();
+}
+
+namespace fsx {
+
+struct cg_bufs {
+	const fsx_csr * A;
+	void * x;
+	void * r;
+	void * z; // == r for identity
+	void * p;
+	void * w;
+	const void * dinv;
+	size_t n;
+};
+
+template<class T, class V, class I>
+static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
+                              double * hist) {
+	hipStream_t s = ctx->stream;
+	const fsx_csr * A = B.A;
+	double * part = ctx->ws.partials;
+	// K1 + F1
+	epi_cg_spmv<double, T> e1{(T *)B.w, (const T *)B.p, part, st};
+	spmv_launch_typed<decltype(e1), double, V, I, T>(s, A, (const T *)B.p, e1);
+	hipLaunchKernelGGL((finalize_kernel<1, fin_alpha>), dim3(1), dim3(kFinThreads),
+	                   0, s, (const double *)part, spmv_blocks(A), fin_alpha{st});
+	// K2 + F2
+	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+	const unsigned g2 = (unsigned)((B.n + per - 1) / per);
+	if (B.dinv) {
+		hipLaunchKernelGGL((cg_update_kernel<T, true>), dim3(g2), dim3(kUpdThreads),
+		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)B.p,
+		                   (const T *)B.w, (const T *)B.dinv, (T *)B.z, part,
+		                   (const cg_state *)st);
+		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<true>>), dim3(1),
+		                   dim3(kFinThreads), 0, s, (const double *)part, g2,
+		                   fin_beta<true>{st, hist});
+	}
+	else {
+		hipLaunchKernelGGL((cg_update_kernel<T, false>), dim3(g2), dim3(kUpdThreads),
+		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)B.p,
+		                   (const T *)B.w, (const T *)nullptr, (T *)nullptr, part,
+		                   (const cg_state *)st);
+		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<false>>), dim3(1),
+		                   dim3(kFinThreads), 0, s, (const double *)part, g2,
+		                   fin_beta<false>{st, hist});
+	}
+	// K3
+	const unsigned g3 = (unsigned)((B.n + 255) / 256);
+	hipLaunchKernelGGL((cg_pupdate_kernel<T>), dim3(g3), dim3(256), 0, s, B.n,
+	                   (T *)B.p, (const T *)B.z, (const cg_state *)st);
+	FSX_HIP(hipGetLastError());
+}
+
+static int chunk_size() {
+	static int c = [] {
+		const char * e = getenv("FSX_CG_CHUNK");
+		int v = e ? atoi(e) : 16;
+		return v < 1 ? 1 : (v > 256 ? 256 : v);
+	}();
+	return c;
+}
+
+static bool use_graphs() {
+	static bool g = [] {
+		const char * e = getenv("FSX_CG_GRAPH");
+		return !(e && e[0] == '0');
+	}();
+	return g;
+}
+
+static cg_graph_cache & cache_of(fsx_ctx * ctx) {
+	if (!ctx->cg_cache) ctx->cg_cache.reset(new cg_graph_cache());
+	auto & c = *ctx->cg_cache;
+	if (!c.statebuf) c.statebuf = new dbuf(sizeof(cg_state) * 4);
+	return c;
+}
+
+// Device residual-history buffer with room for `maxiter` entries.  Growing
+// it invalidates captured graphs (they hold its address).
+static double * history_buffer(fsx_ctx * ctx, int maxiter) {
+	auto & c = cache_of(ctx);
+	size_t need = (size_t)std::max(1, maxiter);
+	if (need > c.hist_cap) {
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		c.clear_graphs();
+		delete c.histbuf;
+		c.histbuf = nullptr;
+		size_t cap = std::max<size_t>(need, 4096);
+		c.histbuf = new dbuf(cap * sizeof(double));
+		c.hist_cap = cap;
+	}
+	return (double *)c.histbuf->p;
+}
+
+// Pooled work vector `slot` of length n and dtype dt (owned by the context).
+static fsx_vec * pooled(fsx_ctx * ctx, size_t n, fsx_dtype dt, int slot) {
+	auto & c = cache_of(ctx);
+	auto key = std::make_tuple(n, (int)dt, slot);
+	auto it = c.pool.find(key);
+	if (it != c.pool.end()) return it->second;
+	fsx_vec * v = nullptr;
+	int rc = fsx_vec_create(ctx, n, dt, &v);
+	if (rc) throw error(rc, get_error());
+	c.pool[key] = v;
+	return v;
+}
+
+// Runs the iteration loop on the device.  `st_host` holds the initial state
+// and receives the final one.
+template<class T, class V, class I>
+static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
+                        double * hist) {
+	auto & cache = cache_of(ctx);
+	cg_state * st = (cg_state *)cache.statebuf->p;
+	cg_state * pinned = (cg_state *)ctx->host_scalars; // pinned host space
+	static_assert(sizeof(cg_state) * 3 <= sizeof(double) * kScalarSlots, "");
+	std::memcpy(pinned, &st_host, sizeof(cg_state));
+	FSX_HIP(hipMemcpyAsync(st, pinned, sizeof(cg_state), hipMemcpyHostToDevice,
+	                       ctx->stream));
+	const int K = chunk_size();
+	{
+		// partials for the SpMV+dot (1 per block) and the update (2 per block)
+		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+		const size_t need = std::max<size_t>(spmv_blocks(B.A), 2 * ((B.n + per - 1) / per));
+		ensure_partials(ctx, need);
+	}
+	hipGraphExec_t ge = nullptr;
+	if (use_graphs()) {
+		const fsx_csr * A = B.A;
+		cg_graph_cache::key_t key{A->rowptr(), A->colind(), A->vals->p,
+		                          B.x, B.r, B.z, B.p, B.w, B.dinv,
+		                          A->nrows, A->nnz, A->plan_kind, A->plan_lanes,
+		                          (int)sizeof(T), K};
+		auto it = cache.m.find(key);
+		if (it == cache.m.end()) {
+			if (cache.m.size() >= 32) cache.clear_graphs();
+			cg_graph_cache::entry en;
+			FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+			try {
+				for (int k = 0; k < K; ++k)
+					enqueue_iteration<T, V, I>(ctx, B, st, hist);
+			}
+			catch (...) {
+				hipGraph_t junk = nullptr;
+				(void)hipStreamEndCapture(ctx->stream, &junk);
+				if (junk) (void)hipGraphDestroy(junk);
+				throw;
+			}
+			FSX_HIP(hipStreamEndCapture(ctx->stream, &en.g));
+			FSX_HIP(hipGraphInstantiate(&en.ge, en.g, nullptr, nullptr, 0));
+			cache.m[key] = en;
+			ge = en.ge;
+		}
+		else
+			ge = it->second.ge;
+	}
+	// pipelined loop: keep one chunk in flight while checking the previous
+	cg_state * flags = pinned + 1;
+	hipEvent_t ev[2];
+	FSX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+	FSX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+	auto launch_chunk = [&](int slot) {
+		if (ge) FSX_HIP(hipGraphLaunch(ge, ctx->stream));
+		else
+			for (int k = 0; k < K; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist);
+		ctx->launches += 5 * K;
+		FSX_HIP(hipMemcpyAsync(&flags[slot], st, sizeof(cg_state),
+		                       hipMemcpyDeviceToHost, ctx->stream));
+		FSX_HIP(hipEventRecord(ev[slot], ctx->stream));
+	};
+	const int max_chunks = (st_host.maxiter + K - 1) / K;
+	int launched = 0, checked = 0;
+	if (max_chunks > 0) {
+		launch_chunk(0);
+		launched = 1;
+	}
+	while (checked < launched) {
+		if (launched < max_chunks) {
+			launch_chunk(launched & 1);
+			++launched;
+		}
+		FSX_HIP(hipEventSynchronize(ev[checked & 1]));
+		ctx->host_syncs++;
+		const cg_state & f = flags[checked & 1];
+		++checked;
+		if (f.done) break;
+	}
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+	(void)hipEventDestroy(ev[0]);
+	(void)hipEventDestroy(ev[1]);
+	FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(cg_state), hipMemcpyDeviceToHost,
+	                       ctx->stream));
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+	std::memcpy(&st_host, pinned, sizeof(cg_state));
+}
+
+// Shared driver for FP64 (T = double) and FP32 inner (T = float) solves.
+// Follows cg.hh:43-139 step by step; see the file comment.  `hist_count`
+// receives the number of residual-history entries written on the device.
+template<class T>
+static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                          const fsx_vec * b, fsx_vec * x, const fsx_settings & s,
+                          fsx_info * info, fsx_vec * r, fsx_vec * z,
+                          fsx_vec * p, fsx_vec * w, int * hist_count) {
+	std::memset(info, 0, sizeof(*info));
+	info->status = FSX_UNKNOWN;
+	if (hist_count) *hist_count = 0;
+	const size_t n = b->n;
+	double b_norm = std::sqrt(reduce_host(ctx, RED_L2SQ, b, nullptr));
+	if (b_norm == 0.0) b_norm = 1.0;
+	const double terminate_tol = s.rtol * b_norm;
+	info->rhs_norm = (float)b_norm;
+	if (s.use_zero_guess) {
+		info->sol_norm_initial = 0;
+		launch_set(ctx, 0., x);
+		launch_copy(ctx, b, r);
+	}
+	else {
+		info->sol_norm_initial =
+			(float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
+		int rc = fsx_csr_residual(ctx, A, b, x, r);
+		if (rc) throw error(rc, get_error());
+	}
+	double current_res = std::sqrt(reduce_host(ctx, RED_L2SQ, r, nullptr));
+	if (current_res < terminate_tol) {
+		info->res_norm_initial = (float)current_res;
+		info->res_norm_final = (float)current_res;
+		info->status = FSX_CONVERGED_RTOL;
+		return;
+	}
+	fsx_vec * zz = r; // op::I: z is a copy of r, so read r directly
+	if (dinv) {
+		int rc = fsx_jacobi_apply(ctx, dinv, r, z);
+		if (rc) throw error(rc, get_error());
+		zz = z;
+	}
+	cg_state st{};
+	st.rho1 = reduce_host(ctx, RED_DOT, zz, r);
+	st.rho0 = st.rho1;
+	st.tol = terminate_tol;
+	st.res = current_res;
+	st.maxiter = s.maxiter;
+	launch_copy(ctx, zz, p);
+	if (s.maxiter > 0) {
+		double * hist = history_buffer(ctx, s.maxiter);
+		cg_bufs B{A, x->ptr(), r->ptr(), zz->ptr(), p->ptr(), w->ptr(),
+		          dinv ? dinv->ptr() : nullptr, n};
+		by_matrix_type(A, [&](auto * vtag, auto * itag) {
+			using V = std::remove_pointer_t<decltype(vtag)>;
+			using I = std::remove_pointer_t<decltype(itag)>;
+			run_cg_loop<T, V, I>(ctx, B, st, hist);
+		});
+		current_res = st.res;
+		info->iters = st.iters;
+		if (st.iters) info->status = st.status;
+		if (hist_count) *hist_count = st.iter;
+	}
+	info->res_norm_final = (float)current_res;
+	info->sol_norm_final = (float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
+	if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
+}
+
+} // namespace fsx
+
+using namespace fsx;
+
+extern "C" {
+
+int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                 const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                 fsx_info * info) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A && s && info, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)b->n == A->nrows,
+		            "cg: shape mismatch");
+		if (dinv) check_same(dinv, b);
+		FSX_REQUIRE(b->dt == FSX_F32 || A->vt == FSX_F64,
+		            "cg: FP64 vectors need an FP64 matrix");
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "cg: b and x must differ");
+		const size_t n = b->n;
+		fsx_vec * r = pooled(ctx, n, b->dt, 0);
+		fsx_vec * z = dinv ? pooled(ctx, n, b->dt, 1) : nullptr;
+		fsx_vec * p = pooled(ctx, n, b->dt, 2);
+		fsx_vec * w = pooled(ctx, n, b->dt, 3);
+		int cnt = 0;
+		if (b->dt == FSX_F64)
+			cg_solve_impl<double>(ctx, A, dinv, b, x, *s, info, r, z, p, w, &cnt);
+		else
+			cg_solve_impl<float>(ctx, A, dinv, b, x, *s, info, r, z, p, w, &cnt);
+		ctx->last_cg_hist.assign(cnt, 0.0);
+		if (cnt > 0)
+			FSX_HIP(hipMemcpy(ctx->last_cg_hist.data(), ctx->cg_cache->histbuf->p,
+			                  sizeof(double) * cnt, hipMemcpyDeviceToHost));
+	});
+}
+
+int fsx_cg_history(fsx_ctx * ctx, double * hist, int cap, int * count) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && count, "null argument");
+		int c = (int)ctx->last_cg_hist.size();
+		*count = c;
+		if (hist)
+			for (int i = 0; i < std::min(c, cap); ++i) hist[i] = ctx->last_cg_hist[i];
+	});
+}
+
+int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
+                    int jacobi, const fsx_vec * b, fsx_vec * x,
+                    double rtol_outer, int max_outer,
+                    const fsx_settings * inner, fsx_ir_info * info) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A64 && A32 && inner && info, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE(b->dt == FSX_F64, "IR: b and x are FP64");
+		FSX_REQUIRE(A64->vt == FSX_F64 && A32->vt == FSX_F32,
+		            "IR: needs FP64 and FP32 matrices");
+		FSX_REQUIRE(A64->nrows == A64->ncols && A32->nrows == A64->nrows &&
+		                A32->ncols == A64->ncols && (int64_t)b->n == A64->nrows,
+		            "IR: shape mismatch");
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "IR: b and x must differ");
+		std::memset(info, 0, sizeof(*info));
+		info->status = FSX_UNKNOWN;
+		const size_t n = b->n;
+		fsx_vec * r = pooled(ctx, n, FSX_F64, 10);
+		fsx_vec * r32 = pooled(ctx, n, FSX_F32, 11);
+		fsx_vec * d32 = pooled(ctx, n, FSX_F32, 12);
+		fsx_vec * dinv32 = jacobi ? pooled(ctx, n, FSX_F32, 13) : nullptr;
+		fsx_vec * ir = pooled(ctx, n, FSX_F32, 14);
+		fsx_vec * iz = jacobi ? pooled(ctx, n, FSX_F32, 15) : nullptr;
+		fsx_vec * ip = pooled(ctx, n, FSX_F32, 16);
+		fsx_vec * iw = pooled(ctx, n, FSX_F32, 17);
+		if (jacobi) {
+			int rc = fsx_csr_diag_inv(ctx, A64, dinv32);
+			if (rc) throw error(rc, get_error());
+		}
+		const double b_norm2 = reduce_host(ctx, RED_L2SQ, b, nullptr);
+		double b_norm = std::sqrt(b_norm2);
+		info->rhs_norm = b_norm;
+		if (b_norm == 0.0) b_norm = 1.0;
+		double * rn2 = ctx->ws.scalars + 8; // |r|^2 on the device
+		// r = b - A64 x and |r|^2 in one pass (FP64 values, FP64 vectors)
+		auto residual = [&]() {
+			by_matrix_type(A64, [&](auto * vtag, auto * itag) {
+				using V = std::remove_pointer_t<decltype(vtag)>;
+				using I = std::remove_pointer_t<decltype(itag)>;
+				epi_res_norm<double, double> e{(double *)r->ptr(),
+				                               (const double *)b->ptr(),
+				                               ctx->ws.partials};
+				spmv_launch_typed<decltype(e), double, V, I, double>(
+					ctx->stream, A64, (const double *)x->ptr(), e);
+			});
+			hipLaunchKernelGGL((finalize_kernel<1, fin_store<1>>), dim3(1),
+			                   dim3(kFinThreads), 0, ctx->stream,
+			                   (const double *)ctx->ws.partials, spmv_blocks(A64),
+			                   fin_store<1>{rn2});
+			FSX_HIP(hipGetLastError());
+			ctx->launches += 2;
+			return std::sqrt(fetch_scalar(ctx, rn2));
+		};
+		ensure_partials(ctx, spmv_blocks(A64));
+		double rnorm = residual();
+		const unsigned g = (unsigned)((n + 255) / 256);
+		fsx_settings is = *inner;
+		is.use_zero_guess = 1;
+		int outer = 0;
+		for (;;) {
+			if (rnorm / b_norm <= rtol_outer) {
+				info->status = FSX_CONVERGED_RTOL;
+				break;
+			}
+			if (outer >= max_outer || rnorm == 0.0) {
+				info->status = rnorm == 0.0 ? FSX_CONVERGED_RTOL : FSX_DIVERGED_ITERS;
+				break;
+			}
+			// normalised residual in FP32
+			hipLaunchKernelGGL(ir_scale_kernel, dim3(g), dim3(256), 0, ctx->stream,
+			                   n, (const double *)r->ptr(), (float *)r32->ptr(),
+			                   (const double *)rn2);
+			FSX_HIP(hipGetLastError());
+			fsx_info ii;
+			int cnt = 0;
+			cg_solve_impl<float>(ctx, A32, dinv32, r32, d32, is, &ii, ir, iz, ip, iw,
+			                     &cnt);
+			info->inner_iters_total += cnt;
+			// x += |r| d  (FP64)
+			hipLaunchKernelGGL(ir_update_kernel, dim3(g), dim3(256), 0, ctx->stream,
+			                   n, (double *)x->ptr(), (const float *)d32->ptr(),
+			                   (const double *)rn2);
+			FSX_HIP(hipGetLastError());
+			ctx->launches += 2;
+			++outer;
+			rnorm = residual();
+		}
+		info->outer_iters = outer;
+		info->rel_res_final = rnorm / b_norm;
+	});
+}
+
+} // extern "C"

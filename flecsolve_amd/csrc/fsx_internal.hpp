@@ -1,0 +1,294 @@
+// fsx_internal.hpp -- internal types shared by the libfsx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fsx.h"
+
+namespace fsx {
+
+// ---- error plumbing ------------------------------------------------------
+
+void set_error(const std::string & msg);
+const char * get_error();
+
+struct error : std::exception {
+	int code;
+	std::string msg;
+	error(int c, std::string m) : code(c), msg(std::move(m)) {}
+	const char * what() const noexcept override { return msg.c_str(); }
+};
+
+#define FSX_HIP(call)                                                        \
+	do {                                                                     \
+		hipError_t e_ = (call);                                              \
+		if (e_ != hipSuccess)                                                \
+			throw ::fsx::error(FSX_ERR_HIP,                                  \
+			                   std::string(#call ": ") + hipGetErrorString(e_)); \
+	} while (0)
+
+#define FSX_REQUIRE(cond, msg)                                               \
+	do {                                                                     \
+		if (!(cond)) throw ::fsx::error(FSX_ERR_ARG, (msg));                 \
+	} while (0)
+
+// Run `body` and translate exceptions into status codes for the C ABI.
+template<class F>
+int guard(F && body) {
+	try {
+		body();
+		return FSX_OK;
+	}
+	catch (const error & e) {
+		set_error(e.msg);
+		return e.code;
+	}
+	catch (const std::bad_alloc &) {
+		set_error("host allocation failed");
+		return FSX_ERR_ALLOC;
+	}
+	catch (const std::exception & e) {
+		set_error(e.what());
+		return FSX_ERR_STATE;
+	}
+	catch (...) {
+		set_error("unknown error");
+		return FSX_ERR_STATE;
+	}
+}
+
+// ---- device buffers ------------------------------------------------------
+
+struct dbuf {
+	void * p = nullptr;
+	size_t bytes = 0;
+	bool owns = true;
+	dbuf() = default;
+	explicit dbuf(size_t b);
+	~dbuf();
+	dbuf(const dbuf &) = delete;
+	dbuf & operator=(const dbuf &) = delete;
+};
+
+// Reduction workspace: per-block partials and a small array of device
+// scalars.
+constexpr int kRedBlocks = 2048;   // maximum blocks of a reduction grid
+constexpr int kScalarSlots = 64;   // device scalar slots per context
+
+struct reduce_ws {
+	double * partials = nullptr;   // grown on demand (ensure_partials)
+	double * scalars = nullptr;    // [kScalarSlots]
+};
+
+} // namespace fsx
+
+struct cg_graph_cache;
+struct cg_cache_deleter {
+	void operator()(cg_graph_cache * c) const; // defined in fsx_cg.hip
+};
+
+struct fsx_ctx {
+	// Vectors and matrices hold a reference; the context is freed when the
+	// owner's fsx_ctx_destroy and the last dependent release have happened.
+	std::atomic<int> refs{1};
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int num_cus = 0;
+	std::unique_ptr<fsx::dbuf> ws_buf;
+	std::unique_ptr<fsx::dbuf> part_buf; // per-block partials
+	size_t part_cap = 0;                 // in doubles
+	fsx::reduce_ws ws;
+	double * host_scalars = nullptr; // pinned, kScalarSlots
+	uint64_t host_syncs = 0;
+	uint64_t launches = 0;
+	uint64_t next_id = 1;
+	std::vector<double> last_cg_hist;
+	std::unique_ptr<cg_graph_cache, cg_cache_deleter> cg_cache;
+	~fsx_ctx();
+};
+
+struct fsx_vec {
+	std::atomic<int> refcnt{1};
+	fsx_ctx * ctx = nullptr;
+	size_t n = 0;
+	fsx_dtype dt = FSX_F64;
+	std::shared_ptr<fsx::dbuf> buf;
+	uint64_t id = 0;
+	void * ptr() const { return buf->p; }
+	size_t elem() const { return dt == FSX_F64 ? 8 : 4; }
+};
+
+struct csr_index {
+	std::unique_ptr<fsx::dbuf> rowptr, colind;
+	// Lazily built transpose structure for A^T x (adjoint).
+	std::unique_ptr<fsx::dbuf> t_rowptr, t_colind, t_perm;
+};
+
+struct fsx_csr {
+	fsx_ctx * ctx = nullptr;
+	int64_t nrows = 0, ncols = 0, nnz = 0;
+	fsx_itype it = FSX_I32;
+	fsx_dtype vt = FSX_F64;
+	std::shared_ptr<csr_index> idx;
+	std::unique_ptr<fsx::dbuf> vals;
+	// SpMV row mapping
+	int plan_kind = 0;   // 0 LDS-staged row blocks, 1 vector per row
+	int plan_lanes = 8;
+	int64_t max_row = 0;
+	double avg_row = 0;
+	const void * rowptr() const { return idx->rowptr->p; }
+	const void * colind() const { return idx->colind->p; }
+};
+
+namespace fsx {
+
+// ---- helpers implemented in the .hip files -------------------------------
+
+void ctx_retain(fsx_ctx * c);
+void ctx_release(fsx_ctx * c);
+
+inline fsx_ctx * ctx_of(const fsx_vec * v) { return v->ctx; }
+void check_vec(const fsx_vec * v, const char * name);
+void check_same(const fsx_vec * a, const fsx_vec * b);
+
+// Reductions into a device scalar slot (async) and to the host (sync).
+enum red_kind { RED_DOT = 0, RED_L1, RED_L2SQ, RED_INF, RED_MIN, RED_MAX };
+void reduce_async(fsx_ctx * ctx, red_kind k, const fsx_vec * x,
+                  const fsx_vec * y, double * dslot);
+double reduce_host(fsx_ctx * ctx, red_kind k, const fsx_vec * x,
+                   const fsx_vec * y);
+
+// Elementwise kernels used by the solve drivers.
+void launch_axpy(fsx_ctx * ctx, double a, const fsx_vec * x, const fsx_vec * y,
+                 fsx_vec * z);
+void launch_copy(fsx_ctx * ctx, const fsx_vec * s, fsx_vec * d);
+void launch_set(fsx_ctx * ctx, double a, fsx_vec * x);
+void launch_convert(fsx_ctx * ctx, const fsx_vec * s, fsx_vec * d, double scale);
+
+// SpMV row-mapping selection (fsx_spmv.hip)
+void choose_plan(fsx_csr * A, const void * host_rowptr);
+
+// Make ctx->ws.partials hold at least `count` doubles.  Reallocation drops
+// captured graphs (they hold the old address).
+double * ensure_partials(fsx_ctx * ctx, size_t count);
+// Drop captured solver graphs of ctx (defined with the CG driver).
+void invalidate_graphs(fsx_ctx * ctx);
+
+// host-side scalar fetch of a device slot (sync)
+double fetch_scalar(fsx_ctx * ctx, const double * dslot);
+
+// grid size for a reduction over n elements (fixed function of n)
+int red_grid(size_t n);
+
+// Block remap so that each XCD (workgroups are dealt round-robin over the
+// 8 XCDs of an MI355X) walks a contiguous range of logical blocks.
+__device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblk) {
+	constexpr unsigned X = 8;
+	unsigned xcd = bid % X, idx = bid / X;
+	unsigned per = nblk / X, rem = nblk % X;
+	return xcd < rem ? xcd * (per + 1) + idx
+	                 : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+template<class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;
+}
+
+// Sum over a 256-thread block; result valid in thread 0.  `sh` holds 4 values.
+template<class T>
+__device__ __forceinline__ T block_sum_256(T v, T * sh) {
+	v = wave_sum(v);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) sh[w] = v;
+	__syncthreads();
+	T r = T(0);
+	if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+	return r;
+}
+
+// Two-stage deterministic reductions.  Stage 1: every block reduces its
+// terms and thread 0 stores them at partials[blk*NV + i] with plain stores
+// (no fences or atomics: on a multi-XCD GPU a device-scope release fence per
+// block would write back the XCD's L2 each time).  Stage 2: a one-block
+// kernel, launched next in stream order (the kernel boundary makes the
+// partials visible), sums them in a fixed order.
+template<int NV>
+__device__ __forceinline__ void store_block_partials(const double (&d)[NV],
+                                                     double * partials,
+                                                     unsigned blk) {
+	__shared__ double sh[NV][4];
+#pragma unroll
+	for (int i = 0; i < NV; ++i) {
+		double s = wave_sum(d[i]);
+		if ((threadIdx.x & 63) == 0) sh[i][threadIdx.x >> 6] = s;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int i = 0; i < NV; ++i)
+			partials[(size_t)blk * NV + i] =
+				((sh[i][0] + sh[i][1]) + sh[i][2]) + sh[i][3];
+	}
+}
+
+constexpr int kFinThreads = 1024;
+
+// Sum of nblk*NV partials by one block of kFinThreads threads; thread t sums
+// blocks t, t + kFinThreads, ... (coalesced, fixed order), then a wave tree
+// and the waves in order.  Result valid in every thread.
+template<int NV>
+__device__ __forceinline__ void sum_partials(const double * partials,
+                                             unsigned nblk, double (&tot)[NV]) {
+	__shared__ double red[NV][kFinThreads / 64];
+	double acc[NV];
+#pragma unroll
+	for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+	for (unsigned b = threadIdx.x; b < nblk; b += kFinThreads) {
+#pragma unroll
+		for (int i = 0; i < NV; ++i) acc[i] += partials[(size_t)b * NV + i];
+	}
+#pragma unroll
+	for (int i = 0; i < NV; ++i) {
+		double s = wave_sum(acc[i]);
+		if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = s;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < NV; ++i) {
+		double s = 0.0;
+		for (int w = 0; w < kFinThreads / 64; ++w) s += red[i][w];
+		tot[i] = s;
+	}
+}
+
+// Stage-2 kernel: f(tot) runs on thread 0 with the grid totals.
+template<int NV, class F>
+__global__ __launch_bounds__(kFinThreads) void finalize_kernel(
+	const double * partials, unsigned nblk, F f) {
+	if (f.skip()) return;
+	double tot[NV];
+	sum_partials<NV>(partials, nblk, tot);
+	if (threadIdx.x == 0) f(tot);
+}
+
+// Finaliser that stores the totals (optionally square-rooted) to `out`.
+template<int NV>
+struct fin_store {
+	double * out;
+	__device__ bool skip() const { return false; }
+	__device__ void operator()(const double (&t)[NV]) const {
+#pragma unroll
+		for (int i = 0; i < NV; ++i) out[i] = t[i];
+	}
+};
+
+} // namespace fsx

@@ -1,0 +1,647 @@
+// fsx_spmv.hip -- device CSR operator: creation, SpMV, fused residual and
+// SpMV+dot, Jacobi relaxation, transpose product and timing helpers.
+//
+// Replaces mat::parcsr's apply (flecsolve/matrices/parcsr.hh:60-91): at one
+// colour the off-diagonal block is empty, the diagonal block is the whole
+// matrix with unchanged column ids (topo/csr.hh:553-618), and the product is
+// compressed_ops::spmv (matrices/seq.hh:177-193) -- here one kernel.
+#include "fsx_spmv_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace fsx {
+
+// ---- epilogues -------------------------------------------------------------
+
+template<class ACC, class Y>
+struct epi_store {
+	static constexpr int NV = 1;
+	Y * y;
+	__device__ bool skip() const { return false; }
+	__device__ void row(int64_t i, ACC s, double (&)[NV]) const { y[i] = (Y)s; }
+	__device__ void block_end(const double (&)[NV], unsigned, unsigned) const {}
+};
+
+// r = b - A x  (op::core::residual: apply, then subtract_self<false>,
+// operators/core.hh:135-142 and topo_tasks.hh:98-108)
+template<class ACC, class Y>
+struct epi_residual {
+	static constexpr int NV = 1;
+	Y * r;
+	const Y * b;
+	__device__ bool skip() const { return false; }
+	__device__ void row(int64_t i, ACC s, double (&)[NV]) const {
+		r[i] = b[i] - (Y)s;
+	}
+	__device__ void block_end(const double (&)[NV], unsigned, unsigned) const {}
+};
+
+// w = A p and the per-block partials of w . p  (cg.hh:95-98: A.apply(p, w);
+// w.dot(p)); a finalize kernel sums the partials.
+template<class ACC, class Y>
+struct epi_dot {
+	static constexpr int NV = 1;
+	Y * w;
+	const Y * p;
+	double * partials;
+	__device__ bool skip() const { return false; }
+	__device__ void row(int64_t i, ACC s, double (&d)[NV]) const {
+		const Y wi = (Y)s;
+		w[i] = wi;
+		d[0] = (double)wi * (double)p[i];
+	}
+	__device__ void block_end(const double (&d)[NV], unsigned blk,
+	                          unsigned) const {
+		store_block_partials<NV>(d, partials, blk);
+	}
+};
+
+// ---- plan selection ------------------------------------------------------
+
+void choose_plan(fsx_csr * A, const void * host_rowptr) {
+	int64_t maxr = 0;
+	for (int64_t i = 0; i < A->nrows; ++i) {
+		int64_t a, b;
+		if (A->it == FSX_I32) {
+			a = ((const int32_t *)host_rowptr)[i];
+			b = ((const int32_t *)host_rowptr)[i + 1];
+		}
+		else {
+			a = ((const int64_t *)host_rowptr)[i];
+			b = ((const int64_t *)host_rowptr)[i + 1];
+		}
+		maxr = std::max(maxr, b - a);
+	}
+	A->max_row = maxr;
+	A->avg_row = A->nrows ? (double)A->nnz / (double)A->nrows : 0.0;
+	const double per_row_budget = (double)kStreamCap / kStreamRows;
+	if (A->avg_row <= per_row_budget) {
+		A->plan_kind = 0;
+		A->plan_lanes = 0;
+	}
+	else {
+		A->plan_kind = 1;
+		int l = 2;
+		while (l < 64 && l < A->avg_row) l <<= 1;
+		A->plan_lanes = l;
+	}
+}
+
+// x element type follows the vector dtype; ACC follows the matrix values
+// (FP64 matrices accumulate in FP64; FP32 matrices with FP32 vectors in FP32,
+// with FP64 vectors in FP64).
+template<template<class, class> class EpiT, class MakeEpi>
+void spmv_dispatch(fsx_ctx * ctx, const fsx_csr * A, const void * x, int xdt,
+                   MakeEpi && make) {
+	by_matrix_type(A, [&](auto * vtag, auto * itag) {
+		using V = std::remove_pointer_t<decltype(vtag)>;
+		using I = std::remove_pointer_t<decltype(itag)>;
+		if (xdt == FSX_F64) {
+			using ACC = double;
+			auto epi = make((EpiT<ACC, double> *)nullptr);
+			spmv_launch_typed<decltype(epi), ACC, V, I, double>(
+				ctx->stream, A, (const double *)x, epi);
+		}
+		else {
+			using ACC = std::conditional_t<std::is_same_v<V, double>, double, float>;
+			auto epi = make((EpiT<ACC, float> *)nullptr);
+			spmv_launch_typed<decltype(epi), ACC, V, I, float>(
+				ctx->stream, A, (const float *)x, epi);
+		}
+	});
+	ctx->launches++;
+}
+
+static void check_op(const fsx_csr * A, const fsx_vec * x, const fsx_vec * y) {
+	FSX_REQUIRE(A, "null matrix");
+	check_vec(x, "x");
+	check_vec(y, "y");
+	FSX_REQUIRE((int64_t)x->n == A->ncols,
+	            "x length " + std::to_string(x->n) + " != ncols " +
+	                std::to_string(A->ncols));
+	FSX_REQUIRE((int64_t)y->n == A->nrows,
+	            "y length " + std::to_string(y->n) + " != nrows " +
+	                std::to_string(A->nrows));
+	FSX_REQUIRE(x->dt == y->dt, "x and y dtypes differ");
+	FSX_REQUIRE(x->ptr() != y->ptr() || A->nrows == 0,
+	            "spmv: x and y must be distinct vectors");
+}
+
+// ---- transpose structure (for A^T x) ------------------------------------
+
+template<class I>
+__global__ void spmv_t_kernel(int64_t ncols, const I * __restrict__ trp,
+                              const I * __restrict__ tci,
+                              const I * __restrict__ perm,
+                              const double * __restrict__ vals,
+                              const double * __restrict__ x,
+                              double * __restrict__ y) {
+	const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= ncols) return;
+	double s = 0.0;
+	for (I e = trp[j]; e < trp[j + 1]; ++e) s += vals[perm[e]] * x[tci[e]];
+	y[j] = s;
+}
+
+template<class I>
+static void build_transpose(fsx_ctx * ctx, const fsx_csr * A) {
+	std::vector<I> rp(A->nrows + 1), ci(A->nnz);
+	FSX_HIP(hipMemcpyAsync(rp.data(), A->rowptr(), rp.size() * sizeof(I),
+	                       hipMemcpyDeviceToHost, ctx->stream));
+	FSX_HIP(hipMemcpyAsync(ci.data(), A->colind(), ci.size() * sizeof(I),
+	                       hipMemcpyDeviceToHost, ctx->stream));
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+	std::vector<I> trp(A->ncols + 1, 0), tci(A->nnz), perm(A->nnz);
+	for (int64_t e = 0; e < A->nnz; ++e) trp[ci[e] + 1]++;
+	for (int64_t j = 0; j < A->ncols; ++j) trp[j + 1] += trp[j];
+	std::vector<I> fill(trp.begin(), trp.end() - 1);
+	for (int64_t i = 0; i < A->nrows; ++i)
+		for (I e = rp[i]; e < rp[i + 1]; ++e) {
+			I j = ci[e];
+			I pos = fill[j]++;
+			tci[pos] = (I)i;
+			perm[pos] = e;
+		}
+	auto & X = *A->idx;
+	X.t_rowptr = std::make_unique<dbuf>(trp.size() * sizeof(I));
+	X.t_colind = std::make_unique<dbuf>(std::max<size_t>(1, tci.size()) * sizeof(I));
+	X.t_perm = std::make_unique<dbuf>(std::max<size_t>(1, perm.size()) * sizeof(I));
+	FSX_HIP(hipMemcpyAsync(X.t_rowptr->p, trp.data(), trp.size() * sizeof(I),
+	                       hipMemcpyHostToDevice, ctx->stream));
+	if (A->nnz) {
+		FSX_HIP(hipMemcpyAsync(X.t_colind->p, tci.data(), tci.size() * sizeof(I),
+		                       hipMemcpyHostToDevice, ctx->stream));
+		FSX_HIP(hipMemcpyAsync(X.t_perm->p, perm.data(), perm.size() * sizeof(I),
+		                       hipMemcpyHostToDevice, ctx->stream));
+	}
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// ---- Jacobi relaxation (mg::bound_jacobi::relax, solvers/mg/jacobi.hh:58-93)
+
+template<class I>
+__global__ __launch_bounds__(256) void jacobi_relax_kernel(
+	int64_t nrows, const I * __restrict__ rp, const I * __restrict__ ci,
+	const double * __restrict__ vals, double omega,
+	const double * __restrict__ b, const double * __restrict__ tmp,
+	double * __restrict__ x) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	double diag = 0;
+	double lpu_x = 0;
+	for (I off = rp[r]; off < rp[r + 1]; ++off) {
+		const I c = ci[off];
+		if ((int64_t)c == r) diag = vals[off];
+		else lpu_x += vals[off] * tmp[c];
+	}
+	const double dinv = 1. / diag;
+	x[r] = omega * dinv * (b[r] - lpu_x) + (1 - omega) * tmp[r];
+}
+
+template<class I>
+__global__ void diag_inv_kernel(int64_t nrows, const I * __restrict__ rp,
+                                const I * __restrict__ ci,
+                                const double * __restrict__ vals,
+                                double * __restrict__ d) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	double v = 0.0;
+	bool found = false;
+	for (I off = rp[r]; off < rp[r + 1]; ++off)
+		if ((int64_t)ci[off] == r) {
+			v = 1.0 / vals[off];
+			found = true;
+		}
+	d[r] = found ? v : 0.0;
+}
+
+template<class S, class D>
+__global__ void cast_kernel(size_t n, const S * __restrict__ s, D * __restrict__ d) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) d[i] = (D)s[i];
+}
+
+__global__ void copy_kernel(size_t n, const double * __restrict__ s,
+                            double * __restrict__ d) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) d[i] = s[i];
+}
+
+} // namespace fsx
+
+using namespace fsx;
+
+extern "C" {
+
+int fsx_csr_create(fsx_ctx * ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                   fsx_itype it, const void * rowptr, const void * colind,
+                   fsx_dtype vt, const void * vals, fsx_csr ** out) {
+	return guard([&] {
+		FSX_REQUIRE(ctx, "null context");
+		FSX_REQUIRE(out, "null output");
+		FSX_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, "negative size");
+		FSX_REQUIRE(it == FSX_I32 || it == FSX_I64, "bad index type");
+		FSX_REQUIRE(vt == FSX_F64 || vt == FSX_F32, "bad value type");
+		FSX_REQUIRE(rowptr, "null rowptr");
+		FSX_REQUIRE(nnz == 0 || (colind && vals), "null colind/vals");
+		if (it == FSX_I32)
+			FSX_REQUIRE(nnz < ((int64_t)1 << 31) && nrows < ((int64_t)1 << 31) &&
+			                ncols < ((int64_t)1 << 31),
+			            "int32 indices cannot address this matrix");
+		const size_t isz = it == FSX_I32 ? 4 : 8;
+		const size_t vsz = vt == FSX_F64 ? 8 : 4;
+		// validate structure on the host: monotone rowptr, columns in range.
+		auto rp_at = [&](int64_t i) -> int64_t {
+			return it == FSX_I32 ? ((const int32_t *)rowptr)[i]
+			                     : ((const int64_t *)rowptr)[i];
+		};
+		FSX_REQUIRE(rp_at(0) == 0, "rowptr[0] must be 0");
+		FSX_REQUIRE(rp_at(nrows) == nnz, "rowptr[nrows] must equal nnz");
+		for (int64_t i = 0; i < nrows; ++i)
+			FSX_REQUIRE(rp_at(i + 1) >= rp_at(i), "rowptr not monotone");
+		for (int64_t e = 0; e < nnz; ++e) {
+			int64_t c = it == FSX_I32 ? ((const int32_t *)colind)[e]
+			                          : ((const int64_t *)colind)[e];
+			FSX_REQUIRE(c >= 0 && c < ncols, "column index out of range");
+		}
+		auto A = std::make_unique<fsx_csr>();
+		A->ctx = ctx;
+		A->nrows = nrows;
+		A->ncols = ncols;
+		A->nnz = nnz;
+		A->it = it;
+		A->vt = vt;
+		A->idx = std::make_shared<csr_index>();
+		A->idx->rowptr = std::make_unique<dbuf>((nrows + 1) * isz);
+		A->idx->colind = std::make_unique<dbuf>(std::max<int64_t>(nnz, 1) * isz);
+		A->vals = std::make_unique<dbuf>(std::max<int64_t>(nnz, 1) * vsz);
+		FSX_HIP(hipMemcpyAsync(A->idx->rowptr->p, rowptr, (nrows + 1) * isz,
+		                       hipMemcpyHostToDevice, ctx->stream));
+		if (nnz) {
+			FSX_HIP(hipMemcpyAsync(A->idx->colind->p, colind, nnz * isz,
+			                       hipMemcpyHostToDevice, ctx->stream));
+			FSX_HIP(hipMemcpyAsync(A->vals->p, vals, nnz * vsz,
+			                       hipMemcpyHostToDevice, ctx->stream));
+		}
+		choose_plan(A.get(), rowptr);
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx_retain(ctx);
+		*out = A.release();
+	});
+}
+
+int fsx_csr_convert(fsx_ctx * ctx, const fsx_csr * A, fsx_dtype vt,
+                    fsx_csr ** out) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A && out, "null argument");
+		auto B = std::make_unique<fsx_csr>();
+		B->ctx = ctx;
+		B->nrows = A->nrows;
+		B->ncols = A->ncols;
+		B->nnz = A->nnz;
+		B->it = A->it;
+		B->vt = vt;
+		B->idx = A->idx; // shared index arrays
+		B->plan_kind = A->plan_kind;
+		B->plan_lanes = A->plan_lanes;
+		B->max_row = A->max_row;
+		B->avg_row = A->avg_row;
+		const size_t vsz = vt == FSX_F64 ? 8 : 4;
+		B->vals = std::make_unique<dbuf>(std::max<int64_t>(A->nnz, 1) * vsz);
+		const size_t n = (size_t)A->nnz;
+		if (n) {
+			unsigned g = (unsigned)((n + 255) / 256);
+			if (A->vt == FSX_F64 && vt == FSX_F32)
+				hipLaunchKernelGGL((cast_kernel<double, float>), dim3(g), dim3(256), 0,
+				                   ctx->stream, n, (const double *)A->vals->p,
+				                   (float *)B->vals->p);
+			else if (A->vt == FSX_F32 && vt == FSX_F64)
+				hipLaunchKernelGGL((cast_kernel<float, double>), dim3(g), dim3(256), 0,
+				                   ctx->stream, n, (const float *)A->vals->p,
+				                   (double *)B->vals->p);
+			else
+				FSX_HIP(hipMemcpyAsync(B->vals->p, A->vals->p, n * vsz,
+				                       hipMemcpyDeviceToDevice, ctx->stream));
+			FSX_HIP(hipGetLastError());
+		}
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx_retain(ctx);
+		*out = B.release();
+	});
+}
+
+int fsx_csr_destroy(fsx_csr * A) {
+	return guard([&] {
+		if (!A) return;
+		fsx_ctx * c = A->ctx;
+		(void)hipStreamSynchronize(c->stream);
+		delete A;
+		ctx_release(c);
+	});
+}
+
+int fsx_csr_info(const fsx_csr * A, int64_t * nrows, int64_t * ncols,
+                 int64_t * nnz, int * itype, int * vtype) {
+	return guard([&] {
+		FSX_REQUIRE(A, "null matrix");
+		if (nrows) *nrows = A->nrows;
+		if (ncols) *ncols = A->ncols;
+		if (nnz) *nnz = A->nnz;
+		if (itype) *itype = A->it;
+		if (vtype) *vtype = A->vt;
+	});
+}
+
+int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes) {
+	return guard([&] {
+		FSX_REQUIRE(A, "null matrix");
+		*kind = A->plan_kind;
+		*lanes = A->plan_lanes;
+	});
+}
+
+int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes) {
+	return guard([&] {
+		FSX_REQUIRE(A, "null matrix");
+		FSX_REQUIRE(kind == 0 || kind == 1, "plan kind must be 0 or 1");
+		if (kind == 1)
+			FSX_REQUIRE(lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 ||
+			                lanes == 32 || lanes == 64,
+			            "lanes must be a power of two in [2, 64]");
+		A->plan_kind = kind;
+		A->plan_lanes = kind == 1 ? lanes : 0;
+	});
+}
+
+int fsx_csr_download(fsx_ctx * ctx, const fsx_csr * A, void * rowptr,
+                     void * colind, void * vals) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A, "null argument");
+		const size_t isz = A->it == FSX_I32 ? 4 : 8;
+		const size_t vsz = A->vt == FSX_F64 ? 8 : 4;
+		if (rowptr)
+			FSX_HIP(hipMemcpyAsync(rowptr, A->rowptr(), (A->nrows + 1) * isz,
+			                       hipMemcpyDeviceToHost, ctx->stream));
+		if (colind && A->nnz)
+			FSX_HIP(hipMemcpyAsync(colind, A->colind(), A->nnz * isz,
+			                       hipMemcpyDeviceToHost, ctx->stream));
+		if (vals && A->nnz)
+			FSX_HIP(hipMemcpyAsync(vals, A->vals->p, A->nnz * vsz,
+			                       hipMemcpyDeviceToHost, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+	});
+}
+
+int fsx_csr_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
+                 fsx_vec * y) {
+	return guard([&] {
+		FSX_REQUIRE(ctx, "null context");
+		check_op(A, x, y);
+		void * yp = y->ptr();
+		if (x->dt == FSX_F64)
+			spmv_dispatch<epi_store>(ctx, A, x->ptr(), x->dt, [&](auto * t) {
+				using E = std::remove_pointer_t<decltype(t)>;
+				E e{};
+				e.y = (decltype(e.y))yp;
+				return e;
+			});
+		else
+			spmv_dispatch<epi_store>(ctx, A, x->ptr(), x->dt, [&](auto * t) {
+				using E = std::remove_pointer_t<decltype(t)>;
+				E e{};
+				e.y = (decltype(e.y))yp;
+				return e;
+			});
+	});
+}
+
+int fsx_csr_residual(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * b,
+                     const fsx_vec * x, fsx_vec * r) {
+	return guard([&] {
+		FSX_REQUIRE(ctx, "null context");
+		check_op(A, x, r);
+		check_vec(b, "b");
+		check_same(b, r);
+		FSX_REQUIRE(b->ptr() != r->ptr() || b->n == 0,
+		            "residual: b and r must be distinct (use fsx_csr_spmv + "
+		            "fsx_subtract for the in-place form)");
+		void * rp = r->ptr();
+		const void * bp = b->ptr();
+		spmv_dispatch<epi_residual>(ctx, A, x->ptr(), x->dt, [&](auto * t) {
+			using E = std::remove_pointer_t<decltype(t)>;
+			E e{};
+			e.r = (decltype(e.r))rp;
+			e.b = (decltype(e.b))bp;
+			return e;
+		});
+	});
+}
+
+int fsx_spmv_dot(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p,
+                 fsx_vec * w, double * out) {
+	return guard([&] {
+		FSX_REQUIRE(ctx, "null context");
+		check_op(A, p, w);
+		FSX_REQUIRE(A->nrows == A->ncols, "spmv_dot needs a square matrix");
+		void * wp = w->ptr();
+		const void * pp = p->ptr();
+		double * slot = ctx->ws.scalars + 1;
+		ensure_partials(ctx, spmv_blocks(A));
+		spmv_dispatch<epi_dot>(ctx, A, p->ptr(), p->dt, [&](auto * t) {
+			using E = std::remove_pointer_t<decltype(t)>;
+			E e{};
+			e.w = (decltype(e.w))wp;
+			e.p = (decltype(e.p))pp;
+			e.partials = ctx->ws.partials;
+			return e;
+		});
+		hipLaunchKernelGGL((finalize_kernel<1, fin_store<1>>), dim3(1),
+		                   dim3(kFinThreads), 0, ctx->stream,
+		                   (const double *)ctx->ws.partials, spmv_blocks(A),
+		                   fin_store<1>{slot});
+		FSX_HIP(hipGetLastError());
+		*out = fetch_scalar(ctx, slot);
+	});
+}
+
+int fsx_csr_diag_inv(fsx_ctx * ctx, const fsx_csr * A, fsx_vec * dinv) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A, "null argument");
+		check_vec(dinv, "dinv");
+		FSX_REQUIRE((int64_t)dinv->n == A->nrows, "dinv length != nrows");
+		FSX_REQUIRE(A->vt == FSX_F64, "diag_inv needs an FP64 matrix");
+		if (A->nrows == 0) return;
+		unsigned g = (unsigned)((A->nrows + 255) / 256);
+		if (dinv->dt == FSX_F64) {
+			if (A->it == FSX_I32)
+				hipLaunchKernelGGL((diag_inv_kernel<int32_t>), dim3(g), dim3(256), 0,
+				                   ctx->stream, A->nrows, (const int32_t *)A->rowptr(),
+				                   (const int32_t *)A->colind(),
+				                   (const double *)A->vals->p, (double *)dinv->ptr());
+			else
+				hipLaunchKernelGGL((diag_inv_kernel<int64_t>), dim3(g), dim3(256), 0,
+				                   ctx->stream, A->nrows, (const int64_t *)A->rowptr(),
+				                   (const int64_t *)A->colind(),
+				                   (const double *)A->vals->p, (double *)dinv->ptr());
+		}
+		else {
+			// FP32 preconditioner vector: 1/a_ii formed in FP64, then rounded
+			dbuf tmp(A->nrows * 8);
+			if (A->it == FSX_I32)
+				hipLaunchKernelGGL((diag_inv_kernel<int32_t>), dim3(g), dim3(256), 0,
+				                   ctx->stream, A->nrows, (const int32_t *)A->rowptr(),
+				                   (const int32_t *)A->colind(),
+				                   (const double *)A->vals->p, (double *)tmp.p);
+			else
+				hipLaunchKernelGGL((diag_inv_kernel<int64_t>), dim3(g), dim3(256), 0,
+				                   ctx->stream, A->nrows, (const int64_t *)A->rowptr(),
+				                   (const int64_t *)A->colind(),
+				                   (const double *)A->vals->p, (double *)tmp.p);
+			hipLaunchKernelGGL((cast_kernel<double, float>), dim3(g), dim3(256), 0,
+			                   ctx->stream, (size_t)A->nrows, (const double *)tmp.p,
+			                   (float *)dinv->ptr());
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+		}
+		FSX_HIP(hipGetLastError());
+		ctx->launches++;
+	});
+}
+
+int fsx_jacobi_relax(fsx_ctx * ctx, const fsx_csr * A, float omega, int nrelax,
+                     const fsx_vec * b, fsx_vec * x) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		FSX_REQUIRE(A->vt == FSX_F64 && x->dt == FSX_F64 && b->dt == FSX_F64,
+		            "jacobi_relax is FP64");
+		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)x->n == A->nrows &&
+		                (int64_t)b->n == A->nrows,
+		            "jacobi_relax: shape mismatch");
+		FSX_REQUIRE(nrelax >= 0, "nrelax must be >= 0");
+		if (A->nrows == 0 || nrelax == 0) return;
+		dbuf tmp(A->nrows * 8);
+		unsigned g = (unsigned)((A->nrows + 255) / 256);
+		for (int s = 0; s < nrelax; ++s) {
+			// tmp = x (jacobi.hh:63), then one sweep
+			hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, ctx->stream,
+			                   (size_t)A->nrows, (const double *)x->ptr(),
+			                   (double *)tmp.p);
+			if (A->it == FSX_I32)
+				hipLaunchKernelGGL((jacobi_relax_kernel<int32_t>), dim3(g), dim3(256),
+				                   0, ctx->stream, A->nrows,
+				                   (const int32_t *)A->rowptr(),
+				                   (const int32_t *)A->colind(),
+				                   (const double *)A->vals->p, (double)omega,
+				                   (const double *)b->ptr(), (const double *)tmp.p,
+				                   (double *)x->ptr());
+			else
+				hipLaunchKernelGGL((jacobi_relax_kernel<int64_t>), dim3(g), dim3(256),
+				                   0, ctx->stream, A->nrows,
+				                   (const int64_t *)A->rowptr(),
+				                   (const int64_t *)A->colind(),
+				                   (const double *)A->vals->p, (double)omega,
+				                   (const double *)b->ptr(), (const double *)tmp.p,
+				                   (double *)x->ptr());
+			FSX_HIP(hipGetLastError());
+			ctx->launches += 2;
+		}
+		FSX_HIP(hipStreamSynchronize(ctx->stream)); // tmp is freed on return
+	});
+}
+
+int fsx_csr_spmv_t(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
+                   fsx_vec * y) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A, "null argument");
+		check_vec(x, "x");
+		check_vec(y, "y");
+		FSX_REQUIRE(A->vt == FSX_F64 && x->dt == FSX_F64 && y->dt == FSX_F64,
+		            "spmv_t is FP64");
+		FSX_REQUIRE((int64_t)x->n == A->nrows && (int64_t)y->n == A->ncols,
+		            "spmv_t: shape mismatch");
+		FSX_REQUIRE(x->ptr() != y->ptr(), "spmv_t: x and y must differ");
+		if (!A->idx->t_rowptr) {
+			if (A->it == FSX_I32) build_transpose<int32_t>(ctx, A);
+			else build_transpose<int64_t>(ctx, A);
+		}
+		if (A->ncols == 0) return;
+		unsigned g = (unsigned)((A->ncols + 255) / 256);
+		auto & X = *A->idx;
+		if (A->it == FSX_I32)
+			hipLaunchKernelGGL((spmv_t_kernel<int32_t>), dim3(g), dim3(256), 0,
+			                   ctx->stream, A->ncols, (const int32_t *)X.t_rowptr->p,
+			                   (const int32_t *)X.t_colind->p,
+			                   (const int32_t *)X.t_perm->p,
+			                   (const double *)A->vals->p, (const double *)x->ptr(),
+			                   (double *)y->ptr());
+		else
+			hipLaunchKernelGGL((spmv_t_kernel<int64_t>), dim3(g), dim3(256), 0,
+			                   ctx->stream, A->ncols, (const int64_t *)X.t_rowptr->p,
+			                   (const int64_t *)X.t_colind->p,
+			                   (const int64_t *)X.t_perm->p,
+			                   (const double *)A->vals->p, (const double *)x->ptr(),
+			                   (double *)y->ptr());
+		FSX_HIP(hipGetLastError());
+		ctx->launches++;
+	});
+}
+
+// ---- timing --------------------------------------------------------------
+
+int fsx_time_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
+                  fsx_vec * y, int reps, double * avg_ms) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && avg_ms, "null argument");
+		check_op(A, x, y);
+		FSX_REQUIRE(reps > 0, "reps must be > 0");
+		hipEvent_t e0, e1;
+		FSX_HIP(hipEventCreate(&e0));
+		FSX_HIP(hipEventCreate(&e1));
+		FSX_HIP(hipEventRecord(e0, ctx->stream));
+		for (int i = 0; i < reps; ++i) {
+			int rc = fsx_csr_spmv(ctx, A, x, y);
+			if (rc) throw error(rc, get_error());
+		}
+		FSX_HIP(hipEventRecord(e1, ctx->stream));
+		FSX_HIP(hipEventSynchronize(e1));
+		float ms = 0;
+		FSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		*avg_ms = (double)ms / reps;
+	});
+}
+
+int fsx_time_copy(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst, int reps,
+                  double * avg_ms) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && avg_ms, "null argument");
+		check_vec(src, "src");
+		check_vec(dst, "dst");
+		check_same(src, dst);
+		FSX_REQUIRE(src->dt == FSX_F64, "copy timing uses FP64 vectors");
+		FSX_REQUIRE(reps > 0, "reps must be > 0");
+		size_t n = src->n;
+		unsigned g = (unsigned)((n + 255) / 256);
+		hipEvent_t e0, e1;
+		FSX_HIP(hipEventCreate(&e0));
+		FSX_HIP(hipEventCreate(&e1));
+		FSX_HIP(hipEventRecord(e0, ctx->stream));
+		for (int i = 0; i < reps; ++i)
+			hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+			                   (const double *)src->ptr(), (double *)dst->ptr());
+		FSX_HIP(hipGetLastError());
+		FSX_HIP(hipEventRecord(e1, ctx->stream));
+		FSX_HIP(hipEventSynchronize(e1));
+		float ms = 0;
+		FSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		*avg_ms = (double)ms / reps;
+	});
+}
+
+} // extern "C"

@@ -1,0 +1,215 @@
+// fsx_spmv_kernels.hpp -- CSR SpMV kernel templates for gfx950.
+//
+// Reference operation: compressed_ops::spmv (flecsolve/matrices/seq.hh:177-193)
+//     y[i] = 0; for off in [rowptr[i], rowptr[i+1]): y[i] += v[off]*x[col[off]]
+// reached through mat::parcsr's apply (matrices/parcsr.hh:60-91), which at one
+// colour is this single local product.
+//
+// Two row mappings, chosen per matrix from its row-length profile:
+//
+//  * Row blocks staged through LDS (short rows; the 5/7-point stencils).
+//    A 256-thread workgroup owns 256 consecutive rows.  Its products
+//    v[off]*x[col[off]] are formed with one lane per nonzero -- so the value
+//    and column streams are read fully coalesced -- and parked in LDS; each
+//    thread then sums its own row's products from LDS in CSR order.  This is
+//    the CSR-Stream scheme of Greathouse & Daga (SC'14).  Because every row is
+//    summed serially from 0 in CSR order with separate multiply and add
+//    roundings, y is bit-identical to the reference's serial loop.  A block
+//    whose nonzeros overflow the LDS budget falls back to one wavefront per
+//    row for that block only.
+//
+//  * Sub-wavefront vector per row (long rows): LANES lanes (2..64) share a
+//    row, stride through it, and combine with cross-lane shuffles.  The
+//    summation order differs from the serial loop (results agree to rounding).
+//
+// Workgroups are dealt round-robin over the 8 XCDs; the block index is
+// remapped so that each XCD (with its own L2) walks one contiguous range of
+// rows, which keeps the x[i +- n] stencil neighbours in that XCD's L2.
+//
+// The kernels take an epilogue policy `Epi` that consumes each row's sum:
+// plain store, residual b - Ax, or store plus fused dot-product terms whose
+// per-block partials a one-block finalize kernel reduces next (device-side
+// scalar updates of the fused CG happen there).  Epi::skip() lets captured
+// iterations become no-ops after the device-side convergence flag is set.
+#pragma once
+
+#include "fsx_internal.hpp"
+
+namespace fsx {
+
+constexpr int kSpmvThreads = 256;
+constexpr int kStreamRows = 256;   // rows per workgroup (LDS-staged mapping)
+constexpr int kStreamCap = 2048;   // nonzeros staged per workgroup
+
+template<class T>
+__device__ __forceinline__ T ldnt(const T * p) {
+	return __builtin_nontemporal_load(p);
+}
+
+// Sum of the thread's row for the LDS-staged mapping.  Valid when
+// threadIdx.x < nr.  `srow` holds nr+1 row pointers, `prod` kStreamCap slots.
+template<class ACC, class V, class I, class X>
+__device__ __forceinline__ ACC stream_rows(int64_t r0, int nr,
+                                           const I * __restrict__ rowptr,
+                                           const I * __restrict__ colind,
+                                           const V * __restrict__ vals,
+                                           const X * __restrict__ x, I * srow,
+                                           ACC * prod) {
+	const int tid = threadIdx.x;
+	for (int t = tid; t <= nr; t += kSpmvThreads) srow[t] = rowptr[r0 + t];
+	__syncthreads();
+	const I base = srow[0];
+	const int64_t cnt = (int64_t)(srow[nr] - base);
+	ACC sum = ACC(0);
+	if (cnt <= kStreamCap) {
+		constexpr int NJ = kStreamCap / kSpmvThreads;
+		I cols[NJ];
+		V vv[NJ];
+#pragma unroll
+		for (int j = 0; j < NJ; ++j) {
+			const int k = tid + j * kSpmvThreads;
+			if (k < cnt) {
+				cols[j] = ldnt(colind + base + k);
+				vv[j] = ldnt(vals + base + k);
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < NJ; ++j) {
+			const int k = tid + j * kSpmvThreads;
+			if (k < cnt) prod[k] = (ACC)vv[j] * (ACC)x[cols[j]];
+		}
+		__syncthreads();
+		if (tid < nr) {
+			const int lo = (int)(srow[tid] - base), hi = (int)(srow[tid + 1] - base);
+			ACC s = ACC(0);
+			for (int e = lo; e < hi; ++e) s += prod[e];
+			sum = s;
+		}
+	}
+	else {
+		// overflow: one wavefront per row within this block
+		const int lane = tid & 63, wv = tid >> 6;
+		for (int rr = wv; rr < nr; rr += kSpmvThreads / 64) {
+			const I lo = srow[rr], hi = srow[rr + 1];
+			ACC s = ACC(0);
+			for (I e = lo + lane; e < hi; e += 64)
+				s += (ACC)ldnt(vals + e) * (ACC)x[ldnt(colind + e)];
+			s = wave_sum(s);
+			if (lane == 0) prod[rr] = s;
+		}
+		__syncthreads();
+		if (tid < nr) sum = prod[tid];
+	}
+	return sum;
+}
+
+// LANES lanes per row; returns the row sum in every lane of the group.
+template<int LANES, class ACC, class V, class I, class X>
+__device__ __forceinline__ ACC vector_row(int64_t row, int64_t nrows,
+                                          const I * __restrict__ rowptr,
+                                          const I * __restrict__ colind,
+                                          const V * __restrict__ vals,
+                                          const X * __restrict__ x) {
+	const int sub = threadIdx.x & (LANES - 1);
+	ACC s = ACC(0);
+	if (row < nrows) {
+		const I lo = rowptr[row], hi = rowptr[row + 1];
+		for (I e = lo + sub; e < hi; e += LANES)
+			s += (ACC)ldnt(vals + e) * (ACC)x[ldnt(colind + e)];
+	}
+#pragma unroll
+	for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
+	return s;
+}
+
+// ---- kernels -------------------------------------------------------------
+
+template<class Epi, class ACC, class V, class I, class X>
+__global__ __launch_bounds__(kSpmvThreads) void spmv_stream_kernel(
+	int64_t nrows, const I * __restrict__ rowptr, const I * __restrict__ colind,
+	const V * __restrict__ vals, const X * __restrict__ x, Epi epi) {
+	if (epi.skip()) return;
+	__shared__ I srow[kStreamRows + 1];
+	__shared__ ACC prod[kStreamCap];
+	const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+	const int64_t r0 = (int64_t)blk * kStreamRows;
+	const int nr = (int)min((int64_t)kStreamRows, nrows - r0);
+	ACC sum = stream_rows<ACC>(r0, nr, rowptr, colind, vals, x, srow, prod);
+	double d[Epi::NV];
+#pragma unroll
+	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
+	if ((int)threadIdx.x < nr) epi.row(r0 + threadIdx.x, sum, d);
+	epi.block_end(d, blk, gridDim.x);
+}
+
+template<int LANES, class Epi, class ACC, class V, class I, class X>
+__global__ __launch_bounds__(kSpmvThreads) void spmv_vector_kernel(
+	int64_t nrows, const I * __restrict__ rowptr, const I * __restrict__ colind,
+	const V * __restrict__ vals, const X * __restrict__ x, Epi epi) {
+	if (epi.skip()) return;
+	constexpr int RPB = kSpmvThreads / LANES;
+	const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+	const int64_t row = (int64_t)blk * RPB + threadIdx.x / LANES;
+	ACC sum = vector_row<LANES, ACC>(row, nrows, rowptr, colind, vals, x);
+	double d[Epi::NV];
+#pragma unroll
+	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
+	if ((threadIdx.x & (LANES - 1)) == 0 && row < nrows) epi.row(row, sum, d);
+	epi.block_end(d, blk, gridDim.x);
+}
+
+// Launch either mapping for matrix A.  X is the x element type.
+template<class Epi, class ACC, class V, class I, class X>
+void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
+                       const Epi & epi) {
+	const I * rp = (const I *)A->rowptr();
+	const I * ci = (const I *)A->colind();
+	const V * vv = (const V *)A->vals->p;
+	if (A->nrows == 0) return;
+	if (A->plan_kind == 0) {
+		unsigned g = (unsigned)((A->nrows + kStreamRows - 1) / kStreamRows);
+		hipLaunchKernelGGL((spmv_stream_kernel<Epi, ACC, V, I, X>), dim3(g),
+		                   dim3(kSpmvThreads), 0, st, A->nrows, rp, ci, vv, x, epi);
+	}
+	else {
+		auto go = [&](auto lanes_c) {
+			constexpr int L = decltype(lanes_c)::value;
+			constexpr int RPB = kSpmvThreads / L;
+			unsigned g = (unsigned)((A->nrows + RPB - 1) / RPB);
+			hipLaunchKernelGGL((spmv_vector_kernel<L, Epi, ACC, V, I, X>), dim3(g),
+			                   dim3(kSpmvThreads), 0, st, A->nrows, rp, ci, vv, x,
+			                   epi);
+		};
+		switch (A->plan_lanes) {
+		case 2: go(std::integral_constant<int, 2>{}); break;
+		case 4: go(std::integral_constant<int, 4>{}); break;
+		case 8: go(std::integral_constant<int, 8>{}); break;
+		case 16: go(std::integral_constant<int, 16>{}); break;
+		case 32: go(std::integral_constant<int, 32>{}); break;
+		default: go(std::integral_constant<int, 64>{}); break;
+		}
+	}
+	FSX_HIP(hipGetLastError());
+}
+
+inline unsigned spmv_blocks(const fsx_csr * A) {
+	if (A->plan_kind == 0)
+		return (unsigned)((A->nrows + kStreamRows - 1) / kStreamRows);
+	const int rpb = kSpmvThreads / A->plan_lanes;
+	return (unsigned)((A->nrows + rpb - 1) / rpb);
+}
+
+// Dispatch on index and value type.  `body` is called with (V*, I*) tags.
+template<class B>
+void by_matrix_type(const fsx_csr * A, B && body) {
+	if (A->vt == FSX_F64) {
+		if (A->it == FSX_I32) body((double *)nullptr, (int32_t *)nullptr);
+		else body((double *)nullptr, (int64_t *)nullptr);
+	}
+	else {
+		if (A->it == FSX_I32) body((float *)nullptr, (int32_t *)nullptr);
+		else body((float *)nullptr, (int64_t *)nullptr);
+	}
+}
+
+} // namespace fsx

@@ -1,0 +1,420 @@
+"""ctypes binding of libfsx (include/fsx.h) for tests and benchmarks.
+
+The library must have been built in-tree (``python -m flecsolve_amd.build``);
+importing this module without it raises -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libfsx.so"
+
+F64, F32 = 0, 1
+I32, I64 = 0, 1
+
+STOP_REASONS = ["converged_atol", "converged_rtol", "converged_user",
+                "diverged_dtol", "diverged_iters", "diverged_breakdown",
+                "unknown"]
+
+
+class Settings(C.Structure):
+    _fields_ = [("maxiter", C.c_int), ("rtol", C.c_float), ("atol", C.c_float),
+                ("use_zero_guess", C.c_int), ("max_krylov_dim", C.c_int),
+                ("pre_side", C.c_int), ("restart", C.c_int)]
+
+    def __init__(self, maxiter=100, rtol=0.0, atol=0.0, use_zero_guess=False,
+                 max_krylov_dim=-1, pre_side=1, restart=False):
+        super().__init__(maxiter, rtol, atol, int(use_zero_guess),
+                         max_krylov_dim, pre_side, int(restart))
+
+
+class Info(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("restarts", C.c_int),
+                ("res_norm_initial", C.c_float), ("res_norm_final", C.c_float),
+                ("sol_norm_initial", C.c_float), ("sol_norm_final", C.c_float),
+                ("rhs_norm", C.c_float)]
+
+    @property
+    def reason(self) -> str:
+        return STOP_REASONS[self.status]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class IRInfo(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("inner_iters_total", C.c_int),
+                ("status", C.c_int), ("rel_res_final", C.c_double),
+                ("rhs_norm", C.c_double)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_SIGS = {
+    "fsx_last_error": (C.c_char_p, []),
+    "fsx_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "fsx_ctx_destroy": (C.c_int, [_P]),
+    "fsx_sync": (C.c_int, [_P]),
+    "fsx_ctx_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "fsx_ctx_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "fsx_vec_create": (C.c_int, [_P, C.c_size_t, C.c_int, C.POINTER(_P)]),
+    "fsx_vec_wrap": (C.c_int, [_P, _P, C.c_size_t, C.c_int, C.POINTER(_P)]),
+    "fsx_vec_retain": (C.c_int, [_P]),
+    "fsx_vec_release": (C.c_int, [_P]),
+    "fsx_vec_size": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    "fsx_vec_dtype": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "fsx_vec_id": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "fsx_vec_device_ptr": (C.c_int, [_P, C.POINTER(_P)]),
+    "fsx_vec_upload": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "fsx_vec_download": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "fsx_copy": (C.c_int, [_P, _P, _P]),
+    "fsx_zero": (C.c_int, [_P, _P]),
+    "fsx_set_scalar": (C.c_int, [_P, C.c_double, _P]),
+    "fsx_scale": (C.c_int, [_P, C.c_double, _P]),
+    "fsx_scale_to": (C.c_int, [_P, C.c_double, _P, _P]),
+    "fsx_add": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_subtract": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_multiply": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_divide": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_reciprocal": (C.c_int, [_P, _P, _P]),
+    "fsx_linear_sum": (C.c_int, [_P, C.c_double, _P, C.c_double, _P, _P]),
+    "fsx_axpy": (C.c_int, [_P, C.c_double, _P, _P, _P]),
+    "fsx_axpby": (C.c_int, [_P, C.c_double, C.c_double, _P, _P]),
+    "fsx_abs": (C.c_int, [_P, _P, _P]),
+    "fsx_add_scalar": (C.c_int, [_P, _P, C.c_double, _P]),
+    "fsx_set_random": (C.c_int, [_P, _P, C.c_uint]),
+    "fsx_convert": (C.c_int, [_P, _P, _P, C.c_double]),
+    "fsx_dot": (C.c_int, [_P, _P, _P, C.POINTER(C.c_double)]),
+    "fsx_nrm1": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "fsx_nrm2": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "fsx_nrminf": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "fsx_min": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "fsx_max": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "fsx_csr_create": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int64, C.c_int,
+                                 _P, _P, C.c_int, _P, C.POINTER(_P)]),
+    "fsx_csr_convert": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
+    "fsx_csr_destroy": (C.c_int, [_P]),
+    "fsx_csr_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int)]),
+    "fsx_csr_plan": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fsx_csr_set_plan": (C.c_int, [_P, C.c_int, C.c_int]),
+    "fsx_csr_download": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fsx_csr_spmv": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_csr_residual": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fsx_csr_diag_inv": (C.c_int, [_P, _P, _P]),
+    "fsx_csr_spmv_t": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_spmv_dot": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_double)]),
+    "fsx_jacobi_apply": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_jacobi_relax": (C.c_int, [_P, _P, C.c_float, C.c_int, _P, _P]),
+    "fsx_cg_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
+                               C.POINTER(Info)]),
+    "fsx_cg_history": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int,
+                                 C.POINTER(C.c_int)]),
+    "fsx_ir_cg_solve": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double,
+                                  C.c_int, C.POINTER(Settings),
+                                  C.POINTER(IRInfo)]),
+    "fsx_krylov_cg": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
+                                C.POINTER(Settings), C.POINTER(Info),
+                                C.POINTER(C.c_double), C.c_int]),
+    "fsx_krylov_fcg": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
+                                 C.POINTER(Settings), C.POINTER(Info)]),
+    "fsx_krylov_gmres": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
+                                   C.POINTER(Settings), C.POINTER(Info),
+                                   C.POINTER(C.c_double), C.c_int]),
+    "fsx_krylov_bicgstab": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
+                                      C.POINTER(Settings), C.POINTER(Info),
+                                      C.POINTER(C.c_double), C.c_int]),
+    "fsx_time_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
+    "fsx_time_copy": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
+}
+
+
+class FsxError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libfsx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FsxError(f"{LIB_PATH} not built; run `python -m flecsolve_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().fsx_last_error().decode()
+        raise FsxError(f"{what} failed (status {rc}): {msg}")
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().fsx_ctx_create(device, C.byref(self.h)), "fsx_ctx_create")
+
+    def sync(self):
+        check(lib().fsx_sync(self.h), "fsx_sync")
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(lib().fsx_ctx_stream(self.h, C.byref(s)), "fsx_ctx_stream")
+        return s.value or 0
+
+    def counters(self) -> tuple[int, int]:
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().fsx_ctx_counters(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            check(lib().fsx_ctx_destroy(self.h), "fsx_ctx_destroy")
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # convenience constructors
+    def vec(self, data, dtype=None) -> "Vector":
+        a = np.asarray(data)
+        dt = F32 if (dtype == F32 or (dtype is None and a.dtype == np.float32)) else F64
+        v = Vector(self, a.size, dt)
+        v.upload(a)
+        return v
+
+    def zeros(self, n: int, dtype: int = F64) -> "Vector":
+        return Vector(self, n, dtype)
+
+    def csr(self, rowptr, colind, vals, ncols=None, vtype=None) -> "CSR":
+        return CSR(self, rowptr, colind, vals, ncols=ncols, vtype=vtype)
+
+
+class Vector:
+    def __init__(self, ctx: Context, n: int, dtype: int = F64, _handle=None):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = dtype
+        if _handle is not None:
+            self.h = _handle
+        else:
+            self.h = C.c_void_p()
+            check(lib().fsx_vec_create(ctx.h, self.n, dtype, C.byref(self.h)),
+                  "fsx_vec_create")
+
+    @property
+    def np_dtype(self):
+        return np.float64 if self.dtype == F64 else np.float32
+
+    def upload(self, a) -> "Vector":
+        a = np.ascontiguousarray(a, dtype=self.np_dtype).ravel()
+        check(lib().fsx_vec_upload(self.ctx.h, self.h, _ptr(a), a.size), "upload")
+        return self
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.np_dtype)
+        check(lib().fsx_vec_download(self.ctx.h, self.h, _ptr(out), self.n),
+              "download")
+        return out
+
+    def id(self) -> int:
+        i = C.c_uint64()
+        check(lib().fsx_vec_id(self.h, C.byref(i)))
+        return i.value
+
+    def release(self):
+        if self.h:
+            check(lib().fsx_vec_release(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class CSR:
+    def __init__(self, ctx: Context, rowptr, colind, vals, ncols=None, vtype=None,
+                 _handle=None):
+        self.ctx = ctx
+        if _handle is not None:
+            self.h = _handle
+            nr, nc, nz = C.c_int64(), C.c_int64(), C.c_int64()
+            it, vt = C.c_int(), C.c_int()
+            check(lib().fsx_csr_info(self.h, C.byref(nr), C.byref(nc), C.byref(nz),
+                                     C.byref(it), C.byref(vt)))
+            self.nrows, self.ncols, self.nnz = nr.value, nc.value, nz.value
+            self.itype, self.vtype = it.value, vt.value
+            return
+        rowptr = np.ascontiguousarray(rowptr)
+        it = I32 if rowptr.dtype == np.int32 else I64
+        idt = np.int32 if it == I32 else np.int64
+        rowptr = rowptr.astype(idt, copy=False)
+        colind = np.ascontiguousarray(colind, dtype=idt)
+        if vtype is None:
+            vtype = F32 if np.asarray(vals).dtype == np.float32 else F64
+        vals = np.ascontiguousarray(vals, dtype=np.float64 if vtype == F64 else np.float32)
+        self.nrows = rowptr.size - 1
+        self.ncols = self.nrows if ncols is None else int(ncols)
+        self.nnz = int(colind.size)
+        self.itype, self.vtype = it, vtype
+        self.h = C.c_void_p()
+        check(lib().fsx_csr_create(ctx.h, self.nrows, self.ncols, self.nnz, it,
+                                   _ptr(rowptr), _ptr(colind), vtype, _ptr(vals),
+                                   C.byref(self.h)), "fsx_csr_create")
+
+    def convert(self, vtype: int) -> "CSR":
+        h = C.c_void_p()
+        check(lib().fsx_csr_convert(self.ctx.h, self.h, vtype, C.byref(h)),
+              "fsx_csr_convert")
+        return CSR(self.ctx, None, None, None, _handle=h)
+
+    def plan(self) -> tuple[int, int]:
+        k, l = C.c_int(), C.c_int()
+        check(lib().fsx_csr_plan(self.h, C.byref(k), C.byref(l)))
+        return k.value, l.value
+
+    def set_plan(self, kind: int, lanes: int = 0):
+        check(lib().fsx_csr_set_plan(self.h, kind, lanes), "fsx_csr_set_plan")
+
+    def download(self):
+        idt = np.int32 if self.itype == I32 else np.int64
+        vdt = np.float64 if self.vtype == F64 else np.float32
+        rp = np.empty(self.nrows + 1, idt)
+        ci = np.empty(self.nnz, idt)
+        vv = np.empty(self.nnz, vdt)
+        check(lib().fsx_csr_download(self.ctx.h, self.h, _ptr(rp), _ptr(ci), _ptr(vv)))
+        return rp, ci, vv
+
+    def spmv(self, x: Vector, y: Vector):
+        check(lib().fsx_csr_spmv(self.ctx.h, self.h, x.h, y.h), "fsx_csr_spmv")
+
+    def spmv_t(self, x: Vector, y: Vector):
+        check(lib().fsx_csr_spmv_t(self.ctx.h, self.h, x.h, y.h), "fsx_csr_spmv_t")
+
+    def residual(self, b: Vector, x: Vector, r: Vector):
+        check(lib().fsx_csr_residual(self.ctx.h, self.h, b.h, x.h, r.h), "residual")
+
+    def diag_inv(self, dtype: int = F64) -> Vector:
+        d = Vector(self.ctx, self.nrows, dtype)
+        check(lib().fsx_csr_diag_inv(self.ctx.h, self.h, d.h), "diag_inv")
+        return d
+
+    def destroy(self):
+        if self.h:
+            check(lib().fsx_csr_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ---- free functions --------------------------------------------------------
+
+def _red(name, ctx, *vs) -> float:
+    out = C.c_double()
+    check(getattr(lib(), name)(ctx.h, *[v.h for v in vs], C.byref(out)), name)
+    return out.value
+
+
+def dot(x: Vector, y: Vector) -> float:
+    return _red("fsx_dot", x.ctx, x, y)
+
+
+def nrm2(x: Vector) -> float:
+    return _red("fsx_nrm2", x.ctx, x)
+
+
+def nrm1(x: Vector) -> float:
+    return _red("fsx_nrm1", x.ctx, x)
+
+
+def nrminf(x: Vector) -> float:
+    return _red("fsx_nrminf", x.ctx, x)
+
+
+def vmin(x: Vector) -> float:
+    return _red("fsx_min", x.ctx, x)
+
+
+def vmax(x: Vector) -> float:
+    return _red("fsx_max", x.ctx, x)
+
+
+def spmv_dot(A: CSR, p: Vector, w: Vector) -> float:
+    out = C.c_double()
+    check(lib().fsx_spmv_dot(A.ctx.h, A.h, p.h, w.h, C.byref(out)), "spmv_dot")
+    return out.value
+
+
+def cg_solve(A: CSR, b: Vector, x: Vector, settings: Settings,
+             dinv: Vector | None = None) -> Info:
+    info = Info()
+    check(lib().fsx_cg_solve(A.ctx.h, A.h, dinv.h if dinv is not None else None,
+                             b.h, x.h, C.byref(settings), C.byref(info)),
+          "fsx_cg_solve")
+    return info
+
+
+def cg_history(ctx: Context) -> np.ndarray:
+    cnt = C.c_int()
+    check(lib().fsx_cg_history(ctx.h, None, 0, C.byref(cnt)))
+    out = np.empty(cnt.value)
+    check(lib().fsx_cg_history(ctx.h, out.ctypes.data_as(C.POINTER(C.c_double)),
+                               cnt.value, C.byref(cnt)))
+    return out
+
+
+def ir_cg_solve(A64: CSR, A32: CSR, b: Vector, x: Vector, rtol_outer: float,
+                inner: Settings, jacobi: bool = True, max_outer: int = 50) -> IRInfo:
+    info = IRInfo()
+    check(lib().fsx_ir_cg_solve(A64.ctx.h, A64.h, A32.h, int(jacobi), b.h, x.h,
+                                rtol_outer, max_outer, C.byref(inner),
+                                C.byref(info)), "fsx_ir_cg_solve")
+    return info
+
+
+def krylov(kind: str, A: CSR, b: Vector, x: Vector, settings: Settings,
+           prec: int = 0, nrelax: int = 1, hist_cap: int = 0):
+    """Generic solver (cg, fcg, gmres, bicgstab) over device vectors.
+    Returns (Info, residual history array)."""
+    info = Info()
+    hist = np.zeros(max(hist_cap, 1))
+    hp = hist.ctypes.data_as(C.POINTER(C.c_double))
+    L = lib()
+    if kind == "fcg":
+        rc = L.fsx_krylov_fcg(A.ctx.h, A.h, prec, nrelax, b.h, x.h,
+                              C.byref(settings), C.byref(info))
+    else:
+        fn = getattr(L, f"fsx_krylov_{kind}")
+        rc = fn(A.ctx.h, A.h, prec, nrelax, b.h, x.h, C.byref(settings),
+                C.byref(info), hp, hist_cap)
+    check(rc, f"fsx_krylov_{kind}")
+    return info, hist[:min(hist_cap, max(info.iters, 0) or hist_cap)]

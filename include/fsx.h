@@ -1,0 +1,248 @@
+/*
+ * fsx.h -- C ABI of libfsx, the MI355X (gfx950) implementation of flecsolve's
+ * Krylov solve path: device vectors, a device CSR operator, fused CG kernels
+ * and solve drivers.
+ *
+ * Conventions
+ *   - Every call returns an int status: 0 = OK, non-zero = error; the message
+ *     is available from fsx_last_error() (thread-local).  No C++ exception
+ *     crosses this boundary.  Contract violations that the reference turns
+ *     into flog_assert aborts return FSX_ERR_ARG here.
+ *   - Handles are opaque.  One context per host thread; each context owns a
+ *     HIP stream (default priority) and all calls on it are stream-ordered.
+ *     Calls that return host values (reductions into double*, downloads,
+ *     solve drivers) synchronise the context stream before returning.
+ *   - Vectors are reference-counted: copying a handle (fsx_vec_retain) makes
+ *     an alias of the same device buffer, which is the ownership model of the
+ *     reference's topo_view vectors (vectors/data/topo_view.hh:73-81, equality
+ *     by field id).  fsx_vec_id() returns the identity used for that equality.
+ *   - Elementwise operations are alias-safe: any output may be the same
+ *     handle as any input (the reference dispatches aliasing cases to *_self
+ *     tasks, vectors/operations/topo_view.hh:78-210).  fsx_scale_to() rejects
+ *     aliasing, as the reference asserts (topo_view.hh:71-76).
+ */
+#ifndef FSX_H
+#define FSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	FSX_OK = 0,
+	FSX_ERR_ARG = 1,     /* invalid argument / contract violation */
+	FSX_ERR_HIP = 2,     /* HIP runtime error */
+	FSX_ERR_ALLOC = 3,   /* device or host allocation failure */
+	FSX_ERR_STATE = 4    /* object in the wrong state */
+};
+
+typedef enum { FSX_F64 = 0, FSX_F32 = 1 } fsx_dtype;
+typedef enum { FSX_I32 = 0, FSX_I64 = 1 } fsx_itype;
+
+typedef struct fsx_ctx fsx_ctx;
+typedef struct fsx_vec fsx_vec;
+typedef struct fsx_csr fsx_csr;
+
+/* solver_settings (flecsolve/solvers/solver_settings.hh:31-36) plus the
+ * GMRES settings (flecsolve/solvers/gmres.hh:389-393).  rtol/atol are float
+ * exactly as in the reference. */
+typedef struct {
+	int maxiter;
+	float rtol;
+	float atol;
+	int use_zero_guess;
+	int max_krylov_dim; /* GMRES: < 0 means maxiter */
+	int pre_side;       /* GMRES: 0 = left, 1 = right (reference default) */
+	int restart;        /* GMRES: non-zero enables restarts */
+} fsx_settings;
+
+/* solve_info (flecsolve/solvers/solver_settings.hh:61-85); norms are float
+ * as in the reference.  status follows solve_info::stop_reason order. */
+typedef struct {
+	int status;
+	int iters;
+	int restarts;
+	float res_norm_initial, res_norm_final;
+	float sol_norm_initial, sol_norm_final;
+	float rhs_norm;
+} fsx_info;
+
+enum {
+	FSX_CONVERGED_ATOL = 0,
+	FSX_CONVERGED_RTOL = 1,
+	FSX_CONVERGED_USER = 2,
+	FSX_DIVERGED_DTOL = 3,
+	FSX_DIVERGED_ITERS = 4,
+	FSX_DIVERGED_BREAKDOWN = 5,
+	FSX_UNKNOWN = 6
+};
+
+/* ---- context ----------------------------------------------------------
+ * Replaces the FleCSI runtime/scheduler the reference's tasks run on
+ * (flecsi::scheduler::instance, vectors/operations/topo_view.hh:38-40). */
+const char * fsx_last_error(void);
+int fsx_ctx_create(int device, fsx_ctx ** out);
+int fsx_ctx_destroy(fsx_ctx * ctx);
+int fsx_sync(fsx_ctx * ctx);
+/* The context's hipStream_t, for callers that time or order their own work
+ * against the library's kernels. */
+int fsx_ctx_stream(fsx_ctx * ctx, void ** stream);
+/* Per-context counters of device reductions forced to the host (each is one
+ * blocking .get() of the reference, topo_view.hh:265-269). */
+int fsx_ctx_counters(fsx_ctx * ctx, uint64_t * host_syncs, uint64_t * launches);
+
+/* ---- vectors: vec::core data (vectors/core.hh:26-353) ------------------ */
+int fsx_vec_create(fsx_ctx * ctx, size_t n, fsx_dtype dt, fsx_vec ** out);
+/* Non-owning view of existing device memory (e.g. a torch tensor). */
+int fsx_vec_wrap(fsx_ctx * ctx, void * dptr, size_t n, fsx_dtype dt,
+                 fsx_vec ** out);
+int fsx_vec_retain(fsx_vec * v);
+int fsx_vec_release(fsx_vec * v);
+int fsx_vec_size(const fsx_vec * v, size_t * n);
+int fsx_vec_dtype(const fsx_vec * v, int * dt);
+int fsx_vec_id(const fsx_vec * v, uint64_t * id);
+int fsx_vec_device_ptr(const fsx_vec * v, void ** dptr);
+int fsx_vec_upload(fsx_ctx * ctx, fsx_vec * v, const void * host, size_t n);
+int fsx_vec_download(fsx_ctx * ctx, const fsx_vec * v, void * host, size_t n);
+
+/* ---- BLAS-1, mirroring the Ops policy (vectors/core.hh:59-310;
+ *      semantics vectors/seq.hh:93-304; tasks operations/topo_tasks.hh) ---- */
+int fsx_copy(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst);          /* core.hh:59-62 */
+int fsx_zero(fsx_ctx * ctx, fsx_vec * x);                                  /* core.hh:67 */
+int fsx_set_scalar(fsx_ctx * ctx, double a, fsx_vec * x);                 /* core.hh:75 */
+int fsx_scale(fsx_ctx * ctx, double a, fsx_vec * x);                       /* core.hh:95 */
+int fsx_scale_to(fsx_ctx * ctx, double a, const fsx_vec * x, fsx_vec * y);/* core.hh:84-87 */
+int fsx_add(fsx_ctx * ctx, const fsx_vec * x, const fsx_vec * y, fsx_vec * z);      /* core.hh:104-110 */
+int fsx_subtract(fsx_ctx * ctx, const fsx_vec * x, const fsx_vec * y, fsx_vec * z); /* core.hh:119-125 */
+int fsx_multiply(fsx_ctx * ctx, const fsx_vec * x, const fsx_vec * y, fsx_vec * z); /* core.hh:134-140 */
+int fsx_divide(fsx_ctx * ctx, const fsx_vec * x, const fsx_vec * y, fsx_vec * z);   /* core.hh:149-155 */
+int fsx_reciprocal(fsx_ctx * ctx, const fsx_vec * x, fsx_vec * y);        /* core.hh:162-165 */
+int fsx_linear_sum(fsx_ctx * ctx, double a, const fsx_vec * x, double b,
+                   const fsx_vec * y, fsx_vec * z);                        /* core.hh:172-178 */
+int fsx_axpy(fsx_ctx * ctx, double a, const fsx_vec * x, const fsx_vec * y,
+             fsx_vec * z);                                                 /* core.hh:185-191 */
+int fsx_axpby(fsx_ctx * ctx, double a, double b, const fsx_vec * x,
+              fsx_vec * z);                                                /* core.hh:198-201 */
+int fsx_abs(fsx_ctx * ctx, const fsx_vec * x, fsx_vec * y);               /* core.hh:208-211 */
+int fsx_add_scalar(fsx_ctx * ctx, const fsx_vec * x, double a, fsx_vec * y); /* core.hh:218-221 */
+/* mt19937(seed) + U[0,1) doubles, as seq_ops::set_random (seq.hh:103-115) */
+int fsx_set_random(fsx_ctx * ctx, fsx_vec * x, unsigned seed);            /* core.hh:302-310 */
+/* FP64 <-> FP32 conversion (new: mixed-precision refinement) */
+int fsx_convert(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst, double scale);
+
+/* ---- reductions (vectors/core.hh:229-290; topo_tasks.hh:52-66, 216-303).
+ * All accumulate in FP64 with a fixed, size-determined order (deterministic
+ * run to run).  Each returns the value to the host (synchronises). ------ */
+int fsx_dot(fsx_ctx * ctx, const fsx_vec * x, const fsx_vec * y, double * out);
+int fsx_nrm1(fsx_ctx * ctx, const fsx_vec * x, double * out);
+int fsx_nrm2(fsx_ctx * ctx, const fsx_vec * x, double * out);
+int fsx_nrminf(fsx_ctx * ctx, const fsx_vec * x, double * out);
+int fsx_min(fsx_ctx * ctx, const fsx_vec * x, double * out);
+int fsx_max(fsx_ctx * ctx, const fsx_vec * x, double * out);
+
+/* ---- CSR operator: replaces mat::parcsr / csr apply()
+ *      (matrices/parcsr.hh:60-91 -> matrices/seq.hh:177-193) ------------- */
+int fsx_csr_create(fsx_ctx * ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                   fsx_itype it, const void * rowptr, const void * colind,
+                   fsx_dtype vt, const void * vals, fsx_csr ** out);
+/* New matrix sharing A's row/column index arrays, with values converted to
+ * `vt` (FP32 values for mixed-precision refinement). */
+int fsx_csr_convert(fsx_ctx * ctx, const fsx_csr * A, fsx_dtype vt,
+                    fsx_csr ** out);
+int fsx_csr_destroy(fsx_csr * A);
+int fsx_csr_info(const fsx_csr * A, int64_t * nrows, int64_t * ncols,
+                 int64_t * nnz, int * itype, int * vtype);
+/* Row-mapping selected for A: 0 = LDS-staged row blocks, 1 = sub-wavefront
+ * vector per row; `lanes` is the vector width. */
+int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes);
+int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes);
+int fsx_csr_download(fsx_ctx * ctx, const fsx_csr * A, void * rowptr,
+                     void * colind, void * vals);
+/* y = A x   (sparse::apply -> mult -> Ops::spmv, matrices/seq.hh:46-60) */
+int fsx_csr_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
+                 fsx_vec * y);
+/* r = b - A x  (op::core::residual, operators/core.hh:129-142), fused */
+int fsx_csr_residual(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * b,
+                     const fsx_vec * x, fsx_vec * r);
+/* dinv_i = 1 / a_ii (values of the Dinv CSR of util/test/mesh.hh:123-140) */
+int fsx_csr_diag_inv(fsx_ctx * ctx, const fsx_csr * A, fsx_vec * dinv);
+/* y = A^T x (adjoint solves) */
+int fsx_csr_spmv_t(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
+                   fsx_vec * y);
+
+/* ---- fused kernels (CG iteration pieces, cg.hh:95-130) ----------------- */
+/* w = A p and out = w . p in one pass (cg.hh:95-98) */
+int fsx_spmv_dot(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p,
+                 fsx_vec * w, double * out);
+/* z = dinv .* r  (Jacobi preconditioner apply; Dinv SpMV of mesh.hh:123-140) */
+int fsx_jacobi_apply(fsx_ctx * ctx, const fsx_vec * dinv, const fsx_vec * r,
+                     fsx_vec * z);
+/* Weighted Jacobi relaxation, nrelax sweeps, omega stored as float
+ * (mg::bound_jacobi::relax, solvers/mg/jacobi.hh:58-93). */
+int fsx_jacobi_relax(fsx_ctx * ctx, const fsx_csr * A, float omega,
+                     int nrelax, const fsx_vec * b, fsx_vec * x);
+
+/* ---- solve drivers -------------------------------------------------------
+ * fsx_cg_solve: op::cg::apply (solvers/cg.hh:43-139) with identity (dinv ==
+ * NULL) or Jacobi preconditioning, run as fused kernels with device-resident
+ * scalars and device-side convergence flags, captured in HIP graphs of
+ * `chunk` iterations.  Same iteration sequence, tolerance semantics and
+ * solve_info as the reference (no user diagnostic). */
+int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                 const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                 fsx_info * info);
+/* Residual history of the last fsx_cg_solve on ctx (|r| per iteration). */
+int fsx_cg_history(fsx_ctx * ctx, double * hist, int cap, int * count);
+
+/* Mixed-precision iterative refinement (new, no reference counterpart):
+ * outer FP64 r = b - A64 x, |r|/|b| check, inner FP32 CG(+Jacobi) on A32 to
+ * `inner->rtol` with FP64 reductions, FP64 update x += |r| d. */
+typedef struct {
+	int outer_iters;
+	int inner_iters_total;
+	int status;
+	double rel_res_final;  /* true FP64 |b - A x| / |b| */
+	double rhs_norm;
+} fsx_ir_info;
+int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
+                    int jacobi, const fsx_vec * b, fsx_vec * x,
+                    double rtol_outer, int max_outer,
+                    const fsx_settings * inner, fsx_ir_info * info);
+
+/* Generic drop-in solvers: the C++ op::cg / op::gmres / op::bicgstab
+ * templates (include/flecsolve/solvers/) instantiated over the device vector
+ * type, one blocking reduction per .get() exactly as the reference.
+ * prec: 0 identity (op::I), 1 Jacobi z = D^{-1} r (Dinv CSR), 2 weighted
+ * Jacobi relaxation (omega 2/3 float, `nrelax` sweeps).
+ * res_hist (optional) receives the per-iteration residual estimate. */
+int fsx_krylov_cg(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
+                  const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                  fsx_info * info, double * res_hist, int hist_cap);
+int fsx_krylov_fcg(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
+                   const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                   fsx_info * info);
+int fsx_krylov_gmres(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
+                     const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                     fsx_info * info, double * res_hist, int hist_cap);
+int fsx_krylov_bicgstab(fsx_ctx * ctx, const fsx_csr * A, int prec,
+                        int nrelax, const fsx_vec * b, fsx_vec * x,
+                        const fsx_settings * s, fsx_info * info,
+                        double * res_hist, int hist_cap);
+
+/* ---- measurement helpers ------------------------------------------------
+ * Average duration (ms) of `reps` back-to-back launches of the SpMV kernel
+ * for A, timed with HIP events on the context stream. */
+int fsx_time_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
+                  fsx_vec * y, int reps, double * avg_ms);
+/* Same for a plain device copy of n doubles (bandwidth reference). */
+int fsx_time_copy(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst, int reps,
+                  double * avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

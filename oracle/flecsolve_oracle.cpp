@@ -1,0 +1,947 @@
+// flecsolve_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of lanl/flecsolve's Krylov solve path, used as the checker
+// for the HIP implementation in flecsolve_amd.  Only tests/, the smoke check
+// and the CPU-baseline leg of bench.py may load this library; nothing in the
+// product links or calls it.
+//
+// Every function follows the reference file:line given in its comment.  The
+// arithmetic is written in the same operation order as the reference's serial
+// `seq_ops` / `compressed_ops` loops and the library is compiled with
+// -ffp-contract=off, so each elementwise result is bit-identical to what the
+// reference's serial CPU path computes.
+//
+// Parity pins (tests/test_oracle_*.py):
+//   * BLAS-1 and reductions: closed-form answers of
+//     flecsolve/vectors/test/flecsi_vector.cc:99-307 on gid-valued fields.
+//   * spmv: scipy.sparse CSR product, exact-integer matrices, row sums.
+//   * coo_tocsr: brute force on small random COO (stable order, duplicates kept).
+//   * mm_read: hand-written files (general / symmetric, std::stof rounding).
+//   * cg: exact convergence on diagonal matrices, A-norm monotonicity and the
+//     Trefethen-Bau bound (flecsolve/solvers/test/cg.cc:16-54), numpy.linalg.
+//   * gmres: residual monotonicity and bound, restart-split identity
+//     (flecsolve/solvers/test/gmres.cc:17-110), numpy.linalg on small systems.
+//   * bicgstab: convergence to numpy.linalg solutions; the breakdown restart is
+//     forced with a residual orthogonal to r~.
+//   * jacobi_relax: fixed point equals the exact solution; one sweep checked by
+//     hand on a 3x3 system.
+//   * fcg: equals cg's solution on SPD systems (parity unpinned for iteration
+//     trajectories beyond that; fcg is not named by the hot path).
+//
+// Reference quirks reproduced on purpose (they change iteration counts or
+// stored values):
+//   rtol/atol are float (solvers/solver_settings.hh:33-34); solve_info norms are
+//   float (solver_settings.hh:76-78); CG reports iters == 0 and status
+//   diverged_iters when maxiter is hit (cg.hh:135-136); BiCGSTAB's early exit
+//   compares |s| with the *absolute* rtol and reports `iter`
+//   (bicgstab.hh:136-142); BiCGSTAB's rho starts as {2.0, 1.0}
+//   (bicgstab.hh:89, initializer-list construction); GMRES uses epsilon for the
+//   zero-rhs test (gmres.hh:103); inf_norm starts from
+//   numeric_limits<double>::min() (vectors/seq.hh:251); Matrix Market values go
+//   through std::stof (matrices/io/matrix_market.hh:160); coo::tocsr keeps
+//   duplicates and input column order (matrices/seq.hh:324-352); Jacobi omega
+//   is a float (solvers/mg/jacobi.hh:26).
+//
+// Deliberate deviations: solve_info's uninitialised float fields are zeroed;
+// flog_assert aborts (bicgstab.hh:129, gmres.hh:397-403) become an error
+// return code; seq_ops::reciprocal's reversed assignment (vectors/seq.hh:154,
+// never instantiated) is restated with the intended y = 1/x of
+// topo_tasks.hh:140-142.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <numeric>
+#include <random>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+extern "C" {
+
+// Layout shared with the C-ABI's fsx_settings / fsx_info (include/fsx.h).
+typedef struct {
+	int maxiter;
+	float rtol;
+	float atol;
+	int use_zero_guess;
+	int max_krylov_dim; // GMRES only; < 0 means maxiter (gmres.hh:395-396)
+	int pre_side;       // GMRES only; 0 = left, 1 = right (gmres.hh:33)
+	int restart;        // GMRES only
+} or_settings;
+
+typedef struct {
+	int status; // solve_info::stop_reason order (solver_settings.hh:64-72)
+	int iters;
+	int restarts;
+	float res_norm_initial, res_norm_final;
+	float sol_norm_initial, sol_norm_final;
+	float rhs_norm;
+} or_info;
+
+enum {
+	OR_CONVERGED_ATOL = 0,
+	OR_CONVERGED_RTOL,
+	OR_CONVERGED_USER,
+	OR_DIVERGED_DTOL,
+	OR_DIVERGED_ITERS,
+	OR_DIVERGED_BREAKDOWN,
+	OR_UNKNOWN
+};
+
+// CSR view: rowptr/colind are int32 (itype 0) or int64 (itype 1).
+typedef struct {
+	int64_t nrows, ncols, nnz;
+	int itype;
+	const void * rowptr;
+	const void * colind;
+	const double * vals;
+} or_csr;
+
+// Preconditioner / operator description.
+//   kind 0: identity copy (operators/shell.hh:94-95)
+//   kind 1: CSR SpMV with M (e.g. the Dinv CSR of util/test/mesh.hh:123-140)
+//   kind 2: weighted Jacobi relaxation on M, nrelax sweeps (mg/jacobi.hh:58-93)
+typedef struct {
+	int kind;
+	const or_csr * M;
+	float omega;
+	int nrelax;
+} or_prec;
+
+// User diagnostic (krylov_parameters.hh:29-31, cg.hh:112): return non-zero to
+// stop with converged_user.  `x` is the current iterate.
+typedef int (*or_diag_fn)(void * user, const double * x, int64_t n, double res);
+
+} // extern "C"
+
+namespace {
+
+using vec = std::vector<double>;
+
+// ---- seq_ops (flecsolve/vectors/seq.hh:93-304) ----------------------------
+
+inline double scalar_prod(const double * x, const double * y, int64_t n) {
+	// seq.hh:282-295: res += x[i] * y[i], serial order
+	double res = 0.0;
+	for (int64_t i = 0; i < n; ++i)
+		res += x[i] * y[i];
+	return res;
+}
+inline double l2norm(const double * x, int64_t n) {
+	// seq.hh:221-247 with p == 2: sqrt(scalar_prod(x, x))
+	return std::sqrt(scalar_prod(x, x, n));
+}
+inline void copy(const double * s, double * d, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) d[i] = s[i]; // seq.hh:94-96
+}
+inline void set_scalar(double a, double * x, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) x[i] = a; // seq.hh:117-120
+}
+// z = alpha * x + y (seq.hh:171-174); alias-safe elementwise
+inline void axpy(double alpha, const double * x, const double * y, double * z,
+                 int64_t n) {
+	for (int64_t i = 0; i < n; ++i) z[i] = alpha * x[i] + y[i];
+}
+inline void scale_self(double alpha, double * x, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) x[i] *= alpha; // seq.hh:122-125
+}
+// z = x - y (seq.hh:137-140)
+inline void subtract(const double * x, const double * y, double * z,
+                     int64_t n) {
+	for (int64_t i = 0; i < n; ++i) z[i] = x[i] - y[i];
+}
+
+// ---- compressed_ops::spmv (flecsolve/matrices/seq.hh:177-193) --------------
+
+template<class I>
+void spmv_t(const or_csr & A, const double * x, double * y) {
+	const I * rowptr = static_cast<const I *>(A.rowptr);
+	const I * colind = static_cast<const I *>(A.colind);
+	const double * values = A.vals;
+	for (int64_t i = 0; i < A.nrows; ++i) {
+		y[i] = 0.;
+		for (int64_t off = rowptr[i]; off < rowptr[i + 1]; ++off)
+			y[i] += values[off] * x[colind[off]];
+	}
+}
+void spmv(const or_csr & A, const double * x, double * y) {
+	if (A.itype == 0) spmv_t<int32_t>(A, x, y);
+	else spmv_t<int64_t>(A, x, y);
+}
+
+// ---- mg::bound_jacobi::relax at one colour (mg/jacobi.hh:58-93) ------------
+
+template<class I>
+void jacobi_relax_t(const or_csr & A, double omega, const double * b,
+                    double * x, double * tmp) {
+	const I * rowptr = static_cast<const I *>(A.rowptr);
+	const I * colind = static_cast<const I *>(A.colind);
+	const double * dvalues = A.vals;
+	std::copy(x, x + A.nrows, tmp); // jacobi.hh:63
+	for (int64_t r = 0; r < A.nrows; ++r) {
+		double diag = 0;
+		double lpu_x = 0;
+		for (int64_t off = rowptr[r]; off < rowptr[r + 1]; ++off) {
+			int64_t c = colind[off];
+			if (r == c) diag = dvalues[off];
+			else lpu_x += dvalues[off] * tmp[c];
+		}
+		double dinv = 1. / diag;
+		x[r] = omega * dinv * (b[r] - lpu_x) + (1 - omega) * tmp[r];
+	}
+}
+void jacobi_relax(const or_csr & A, double omega, const double * b, double * x,
+                  double * tmp) {
+	if (A.itype == 0) jacobi_relax_t<int32_t>(A, omega, b, x, tmp);
+	else jacobi_relax_t<int64_t>(A, omega, b, x, tmp);
+}
+
+// ---- operators ------------------------------------------------------------
+
+struct op_t {
+	const or_csr * A = nullptr; // kind 1/2 matrix
+	int kind = 1;               // 0 identity, 1 csr, 2 jacobi
+	double omega = 0;
+	int nrelax = 0;
+	mutable vec tmp;
+
+	void apply(const double * x, double * y, int64_t n) const {
+		if (kind == 0) copy(x, y, n);
+		else if (kind == 1) spmv(*A, x, y);
+		else {
+			if ((int64_t)tmp.size() != n) tmp.assign(n, 0.0);
+			// bound_jacobi::apply(b, x): b = input, x = output (in/out)
+			for (int s = 0; s < nrelax; ++s)
+				jacobi_relax(*A, omega, x, y, tmp.data());
+		}
+	}
+	// op::core::residual (operators/core.hh:135-142): apply, then r = b - r
+	void residual(const double * b, const double * x, double * r,
+	              int64_t n) const {
+		apply(x, r, n);
+		subtract(b, r, r, n);
+	}
+};
+
+op_t make_prec(const or_prec * p) {
+	op_t o;
+	if (!p) { o.kind = 0; return o; }
+	o.kind = p->kind;
+	o.A = p->M;
+	o.omega = (double)p->omega; // float setting promoted (jacobi.hh:26, 47)
+	o.nrelax = p->nrelax;
+	return o;
+}
+
+struct diag_t {
+	or_diag_fn fn;
+	void * user;
+	bool operator()(const double * x, int64_t n, double res) const {
+		return fn ? fn(user, x, n, res) != 0 : false;
+	}
+};
+
+void zero_info(or_info * info) {
+	std::memset(info, 0, sizeof(*info));
+	info->status = OR_UNKNOWN; // solve_info() (solver_settings.hh:62)
+}
+
+// ---- op::cg::apply (flecsolve/solvers/cg.hh:43-139) ------------------------
+
+int cg(const op_t & A, const op_t & P, const double * b, double * x,
+       int64_t n, const or_settings & settings, const diag_t & user_diagnostic,
+       or_info * info, double * res_hist, int hist_cap) {
+	zero_info(info);
+	vec r(n), z(n), p(n), w(n); // cg::nwork == 4 (cg.hh:252)
+	double b_norm = l2norm(b, n);
+	if (b_norm == 0.0) b_norm = 1.0;
+	const double terminate_tol = settings.rtol * b_norm;
+	info->rhs_norm = (float)b_norm;
+
+	if (settings.use_zero_guess) {
+		info->sol_norm_initial = 0;
+		set_scalar(0., x, n);
+		copy(b, r.data(), n);
+	}
+	else {
+		info->sol_norm_initial = (float)l2norm(x, n);
+		A.residual(b, x, r.data(), n);
+	}
+
+	double current_res = l2norm(r.data(), n);
+	if (current_res < terminate_tol) {
+		info->res_norm_initial = (float)current_res;
+		info->res_norm_final = (float)current_res;
+		info->status = OR_CONVERGED_RTOL;
+		return 0;
+	}
+
+	P.apply(r.data(), z.data(), n);
+	double rho[2] = {2.0, 0.0};
+	rho[1] = scalar_prod(z.data(), r.data(), n); // z.dot(r)
+	rho[0] = rho[1];
+	copy(z.data(), p.data(), n);
+	int ncurv = 0;
+	for (int iter = 0; iter < settings.maxiter; iter++) {
+		double beta = 1.0;
+		A.apply(p.data(), w.data(), n);
+		double alpha = scalar_prod(w.data(), p.data(), n); // w.dot(p)
+		if (alpha <= 0.0) ++ncurv; // logged, not fatal (cg.hh:101-104)
+		alpha = rho[1] / alpha;
+		axpy(alpha, p.data(), x, x, n);
+		axpy(-alpha, w.data(), r.data(), r.data(), n);
+		current_res = l2norm(r.data(), n);
+		if (res_hist && iter < hist_cap) res_hist[iter] = current_res;
+		if (user_diagnostic(x, n, current_res)) {
+			info->iters = iter + 1;
+			info->status = OR_CONVERGED_USER;
+			break;
+		}
+		if (current_res < terminate_tol) {
+			info->iters = iter + 1;
+			info->status = OR_CONVERGED_RTOL;
+			break;
+		}
+		P.apply(r.data(), z.data(), n);
+		rho[0] = rho[1];
+		rho[1] = scalar_prod(r.data(), z.data(), n); // r.dot(z)
+		beta = rho[1] / rho[0];
+		axpy(beta, p.data(), z.data(), p.data(), n); // p.axpy(beta, p, z)
+	}
+	info->res_norm_final = (float)current_res;
+	info->sol_norm_final = (float)l2norm(x, n);
+	if (info->iters == 0) info->status = OR_DIVERGED_ITERS;
+	return ncurv;
+}
+
+// ---- op::fcg::apply (flecsolve/solvers/cg.hh:161-243) ---------------------
+
+int fcg(const op_t & A, const op_t & P, const double * b, double * u,
+        int64_t n, const or_settings & settings,
+        const diag_t & user_diagnostic, or_info * info) {
+	zero_info(info);
+	vec r(n), v(n), w(n), q(n), d(n); // fcg::nwork == 5
+	double b_norm = l2norm(b, n);
+	if (b_norm == 0.0) b_norm = 1.0;
+	const double terminate_tol = settings.rtol * b_norm;
+	info->rhs_norm = (float)b_norm;
+	if (settings.use_zero_guess) {
+		info->sol_norm_initial = 0;
+		set_scalar(0., u, n);
+		copy(b, r.data(), n);
+	}
+	else {
+		info->sol_norm_initial = (float)l2norm(u, n);
+		A.residual(b, u, r.data(), n);
+	}
+	double current_res = l2norm(r.data(), n);
+	if (current_res < terminate_tol) {
+		info->res_norm_initial = (float)current_res;
+		info->res_norm_final = (float)current_res;
+		info->status = OR_CONVERGED_RTOL;
+		return 0;
+	}
+	double rho = 0;
+	for (int iter = 0; iter < settings.maxiter; iter++) {
+		P.apply(r.data(), v.data(), n);
+		A.apply(v.data(), w.data(), n);
+		double alpha = scalar_prod(v.data(), r.data(), n);
+		double beta = scalar_prod(v.data(), w.data(), n);
+		if (iter > 0) {
+			double gamma = scalar_prod(v.data(), q.data(), n);
+			axpy((-gamma) / rho, d.data(), v.data(), d.data(), n);
+			axpy((-gamma) / rho, q.data(), w.data(), q.data(), n);
+			rho = beta - (gamma * gamma) / rho;
+		}
+		else {
+			copy(v.data(), d.data(), n);
+			copy(w.data(), q.data(), n);
+			rho = beta;
+		}
+		axpy(alpha / rho, d.data(), u, u, n);
+		axpy(-alpha / rho, q.data(), r.data(), r.data(), n);
+		current_res = l2norm(r.data(), n);
+		if (user_diagnostic(u, n, current_res)) {
+			info->iters = iter + 1;
+			info->status = OR_CONVERGED_USER;
+			break;
+		}
+		if (current_res < terminate_tol) {
+			info->iters = iter + 1;
+			info->status = OR_CONVERGED_RTOL;
+			break;
+		}
+	}
+	info->res_norm_final = (float)current_res;
+	info->sol_norm_final = (float)l2norm(u, n);
+	if (info->iters == 0) info->status = OR_DIVERGED_ITERS;
+	return 0;
+}
+
+// ---- op::gmres (flecsolve/solvers/gmres.hh:54-353) -------------------------
+
+struct gmres_t {
+	static constexpr int krylov_dim_bound = 100; // gmres.hh:31
+	or_settings settings;
+	int max_dim = 0;
+	std::vector<double> hess; // (max_dim+1)^2, column-major (mdcolex)
+	std::vector<double> sinvec, cosvec, dwvec, dyvec;
+	std::vector<vec> basis;
+	vec res, z, v;
+	int64_t n;
+
+	double & H(int i, int j) { return hess[(size_t)j * (max_dim + 1) + i]; }
+
+	// gmres.hh:54-76 (reset) + settings::validate (gmres.hh:394-404)
+	int reset(const or_settings & s, int64_t nn) {
+		settings = s;
+		if (settings.max_krylov_dim < 0)
+			settings.max_krylov_dim = settings.maxiter;
+		if (settings.max_krylov_dim > krylov_dim_bound) return -1;
+		if (!settings.restart && settings.maxiter > settings.max_krylov_dim)
+			return -1;
+		n = nn;
+		max_dim = std::min(settings.max_krylov_dim, settings.maxiter);
+		hess.assign((size_t)(max_dim + 1) * (max_dim + 1), 0.0);
+		cosvec.assign(max_dim + 1, 0.0);
+		sinvec.assign(max_dim + 1, 0.0);
+		dwvec.assign(max_dim + 1, 0.0);
+		dyvec.assign(max_dim + 1, 0.0);
+		basis.assign(max_dim + 1, vec(n, 0.0));
+		res.assign(n, 0.0);
+		z.assign(n, 0.0);
+		v.assign(n, 0.0);
+		return 0;
+	}
+
+	// gmres.hh:265-279 (modified Gram-Schmidt)
+	void orthogonalize(double * vv, int k) {
+		for (int j = 0; j < k; j++) {
+			const double h_jk = scalar_prod(vv, basis[j].data(), n);
+			axpy(-h_jk, basis[j].data(), vv, vv, n);
+			H(j, k - 1) = h_jk;
+		}
+		const double v_norm = l2norm(vv, n);
+		H(k, k - 1) = v_norm;
+	}
+
+	// gmres.hh:281-296
+	void apply_givens_rotation(int i, int k) {
+		double x = H(i, k);
+		double y = H(i + 1, k);
+		double c = cosvec[i];
+		double s = sinvec[i];
+		H(i, k) = c * x + s * y;
+		H(i + 1, k) = -s * x + c * y;
+	}
+
+	// gmres.hh:298-331 (Bindel, Demmel, Kahan, Marques, Algorithm 1)
+	void compute_givens_rotation(int k) {
+		double f = H(k, k);
+		double g = H(k + 1, k);
+		double c, s;
+		if (g == 0.0) {
+			c = 1.0;
+			s = 0.0;
+		}
+		else if (f == 0.0) {
+			c = 0.0;
+			s = (g < 0.0) ? -1.0 : 1.0;
+		}
+		else {
+			double r;
+			r = std::sqrt(f * f + g * g);
+			r = 1.0 / r;
+			c = std::fabs(f) * r;
+			s = std::copysign(g * r, f);
+		}
+		cosvec[k] = c;
+		sinvec[k] = s;
+	}
+
+	// gmres.hh:333-348
+	void back_solve(int nr) {
+		dyvec[nr] = dwvec[nr] / H(nr, nr);
+		for (int k = nr - 1; k >= 0; k--) {
+			dyvec[k] = dwvec[k];
+			for (int i = k + 1; i <= nr; i++)
+				dyvec[k] -= H(k, i) * dyvec[i];
+			dyvec[k] = dyvec[k] / H(k, k);
+		}
+	}
+
+	// gmres.hh:244-263
+	void correct(int nr, const op_t & P, double * x) {
+		if (settings.pre_side == 1) {
+			set_scalar(0.0, z.data(), n);
+			for (int i = 0; i <= nr; i++)
+				axpy(dyvec[i], basis[i].data(), z.data(), z.data(), n);
+			P.apply(z.data(), v.data(), n);
+			axpy(1.0, v.data(), x, x, n);
+		}
+		else {
+			for (int i = 0; i <= nr; i++)
+				axpy(dyvec[i], basis[i].data(), x, x, n);
+		}
+	}
+
+	// gmres.hh:78-241
+	void apply(const op_t & A, const op_t & P, const double * b, double * x,
+	           const diag_t & user_diagnostic, or_info * info,
+	           double * res_hist, int hist_cap) {
+		zero_info(info);
+		double b_norm = l2norm(b, n);
+		info->rhs_norm = (float)b_norm;
+		if (b_norm < std::numeric_limits<double>::epsilon()) b_norm = 1.0;
+		const double terminate_tol = settings.rtol * b_norm;
+		if (settings.use_zero_guess) set_scalar(0., x, n);
+		if (settings.pre_side == 0) {
+			if (settings.use_zero_guess) copy(b, basis[0].data(), n);
+			else A.residual(b, x, basis[0].data(), n);
+			P.apply(basis[0].data(), res.data(), n);
+		}
+		else {
+			if (settings.use_zero_guess) copy(b, res.data(), n);
+			else A.residual(b, x, res.data(), n);
+		}
+		const double beta = l2norm(res.data(), n);
+		info->res_norm_initial = (float)beta;
+		if (beta < terminate_tol) {
+			info->res_norm_final = (float)beta;
+			info->status = OR_CONVERGED_RTOL;
+			return;
+		}
+		scale_self(1.0 / beta, res.data(), n);
+		copy(res.data(), basis[0].data(), n);
+		dwvec[0] = beta;
+		double v_norm = beta;
+		int k = 0;
+		for (int iter = 0; iter < settings.maxiter; iter++) {
+			if (settings.pre_side == 1) {
+				P.apply(basis[k].data(), z.data(), n);
+				A.apply(z.data(), v.data(), n);
+			}
+			else {
+				A.apply(basis[k].data(), z.data(), n);
+				P.apply(z.data(), v.data(), n);
+			}
+			orthogonalize(v.data(), k + 1);
+			v_norm = H(k + 1, k);
+			if (v_norm != 0.0) scale_self(1.0 / v_norm, v.data(), n);
+			copy(v.data(), basis[k + 1].data(), n);
+			for (int i = 0; i < k; i++) apply_givens_rotation(i, k);
+			if (v_norm != 0.0) {
+				compute_givens_rotation(k);
+				apply_givens_rotation(k, k);
+				H(k + 1, k) = 0.0;
+				double xx = dwvec[k];
+				double c = cosvec[k];
+				double s = sinvec[k];
+				dwvec[k] = c * xx;
+				dwvec[k + 1] = -s * xx;
+			}
+			v_norm = std::fabs(dwvec[k + 1]);
+			++k;
+			if (res_hist && iter < hist_cap) res_hist[iter] = v_norm;
+			if (user_diagnostic(x, n, v_norm)) {
+				info->status = OR_CONVERGED_USER;
+				info->iters = iter + 1;
+				break;
+			}
+			if (v_norm < terminate_tol) {
+				info->status = OR_CONVERGED_RTOL;
+				info->iters = iter + 1;
+				break;
+			}
+			if (k == settings.max_krylov_dim && iter != settings.maxiter - 1) {
+				back_solve(k - 1);
+				correct(k - 1, P, x);
+				if (settings.pre_side == 0) {
+					A.residual(b, x, basis[0].data(), n);
+					P.apply(basis[0].data(), res.data(), n);
+				}
+				else
+					A.residual(b, x, res.data(), n);
+				const double betar = l2norm(res.data(), n);
+				scale_self(1.0 / betar, res.data(), n);
+				copy(res.data(), basis[0].data(), n);
+				dwvec[0] = betar;
+				++info->restarts;
+				k = 0;
+			}
+		}
+		if (k > 0) {
+			back_solve(k - 1);
+			correct(k - 1, P, x);
+		}
+		info->res_norm_final = (float)v_norm;
+		info->sol_norm_final = (float)l2norm(x, n);
+		if (info->iters == 0) info->status = OR_DIVERGED_ITERS;
+	}
+};
+
+// ---- op::bicgstab::apply (flecsolve/solvers/bicgstab.hh:39-187) -----------
+
+int bicgstab(const op_t & A, const op_t & P, const double * b, double * x,
+             int64_t n, const or_settings & settings,
+             const diag_t & user_diagnostic, or_info * info,
+             double * res_hist, int hist_cap) {
+	zero_info(info);
+	// work [res, r_tilde, p, v, p_hat, s, s_hat, t] (bicgstab.hh:49, 195)
+	vec res(n), r_tilde(n), p(n), v(n), p_hat(n), s(n), s_hat(n), t(n);
+	double b_norm = l2norm(b, n);
+	if (b_norm == 0.) b_norm = 1.;
+	const double terminate_tol = settings.rtol * b_norm;
+	info->rhs_norm = (float)b_norm;
+	if (settings.use_zero_guess) {
+		info->sol_norm_initial = 0;
+		copy(b, res.data(), n);
+		set_scalar(0., x, n);
+	}
+	else {
+		info->sol_norm_initial = (float)l2norm(x, n);
+		A.residual(b, x, res.data(), n);
+	}
+	double res_norm = l2norm(res.data(), n);
+	double r_tilde_norm = res_norm;
+	info->res_norm_initial = (float)res_norm;
+	if (res_norm < terminate_tol) {
+		info->status = OR_CONVERGED_RTOL;
+		info->res_norm_initial = (float)res_norm;
+		info->res_norm_final = (float)res_norm;
+		return 0;
+	}
+	double alpha = 1.0;
+	double beta = 0.0;
+	double omega = 1.0;
+	// `std::vector<real> rho{2, 1.0}` is list-initialised: {2.0, 1.0}
+	double rho[2] = {2.0, 1.0};
+	copy(res.data(), r_tilde.data(), n);
+	set_scalar(0., p.data(), n);
+	set_scalar(0., v.data(), n);
+	for (int iter = 0; iter < settings.maxiter; iter++) {
+		rho[1] = scalar_prod(r_tilde.data(), res.data(), n);
+		double angle = std::sqrt(std::fabs(rho[1]));
+		double eps = std::numeric_limits<double>::epsilon();
+		if (angle < eps * r_tilde_norm) {
+			A.residual(b, x, res.data(), n);
+			copy(res.data(), r_tilde.data(), n);
+			res_norm = l2norm(res.data(), n);
+			rho[1] = r_tilde_norm = res_norm;
+			copy(res.data(), p.data(), n);
+			++info->restarts;
+			continue;
+		}
+		if (iter == 0) copy(res.data(), p.data(), n);
+		else {
+			beta = (rho[1] / rho[0]) * (alpha / omega);
+			axpy(-omega, v.data(), p.data(), p.data(), n);
+			axpy(beta, p.data(), res.data(), p.data(), n);
+		}
+		P.apply(p.data(), p_hat.data(), n);
+		A.apply(p_hat.data(), v.data(), n);
+		alpha = scalar_prod(r_tilde.data(), v.data(), n);
+		if (alpha == 0.) { // flog_assert (bicgstab.hh:129)
+			info->status = OR_DIVERGED_BREAKDOWN;
+			return -1;
+		}
+		alpha = rho[1] / alpha;
+		axpy(-alpha, v.data(), res.data(), s.data(), n);
+		const double s_norm = l2norm(s.data(), n);
+		if (s_norm < settings.rtol) {
+			axpy(alpha, p_hat.data(), x, x, n);
+			info->iters = iter;
+			info->status = OR_CONVERGED_RTOL;
+			break;
+		}
+		P.apply(s.data(), s_hat.data(), n);
+		A.apply(s_hat.data(), t.data(), n);
+		double t_sqnorm = scalar_prod(t.data(), t.data(), n);
+		double t_dot_s = scalar_prod(t.data(), s.data(), n);
+		omega = (t_sqnorm == 0.0) ? 0.0 : t_dot_s / t_sqnorm;
+		axpy(alpha, p_hat.data(), x, x, n);
+		axpy(omega, s_hat.data(), x, x, n);
+		axpy(-omega, t.data(), s.data(), res.data(), n);
+		res_norm = l2norm(res.data(), n);
+		if (res_hist && iter < hist_cap) res_hist[iter] = res_norm;
+		if (user_diagnostic(x, n, res_norm)) {
+			info->status = OR_CONVERGED_USER;
+			info->iters = iter + 1;
+			break;
+		}
+		if (res_norm < terminate_tol) {
+			info->status = OR_CONVERGED_RTOL;
+			info->iters = iter + 1;
+			break;
+		}
+		if (omega == 0.0) {
+			info->iters = iter + 1;
+			info->status = OR_DIVERGED_BREAKDOWN;
+			break;
+		}
+		rho[0] = rho[1];
+	}
+	info->res_norm_final = (float)res_norm;
+	info->sol_norm_final = (float)l2norm(x, n);
+	if (info->iters == 0) info->status = OR_DIVERGED_ITERS;
+	return 0;
+}
+
+// ---- Matrix Market (flecsolve/matrices/io/matrix_market.hh:39-179) ---------
+
+struct mm_header {
+	int64_t nrows, ncols, nnz;
+	bool symmetric;
+};
+
+bool mm_read_header(std::ifstream & fh, mm_header & hdr) {
+	// matrix_market.hh:39-65
+	int64_t sizes[3] = {0, 0, 0};
+	std::string line;
+	hdr.symmetric = false;
+	bool found = false;
+	while (std::getline(fh, line)) {
+		if (line.find("symmetric") != std::string::npos) hdr.symmetric = true;
+		if (line[0] != '%') {
+			std::istringstream iss(line);
+			for (int i = 0; i < 3; i++) {
+				std::string tok;
+				std::getline(iss, tok, ' ');
+				sizes[i] = std::atoi(tok.c_str());
+			}
+			found = true;
+			break;
+		}
+	}
+	hdr.nrows = sizes[0];
+	hdr.ncols = sizes[1];
+	hdr.nnz = sizes[2];
+	return found;
+}
+
+} // namespace
+
+extern "C" {
+
+// ---- BLAS-1 entry points (vectors/seq.hh:93-304) --------------------------
+
+void or_copy(const double * s, double * d, int64_t n) { copy(s, d, n); }
+void or_set_scalar(double a, double * x, int64_t n) { set_scalar(a, x, n); }
+void or_scale(double a, double * x, int64_t n) { scale_self(a, x, n); }
+void or_scale_to(double a, const double * x, double * y, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) y[i] = a * x[i]; // seq.hh:127-130
+}
+void or_add(const double * x, const double * y, double * z, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) z[i] = x[i] + y[i]; // seq.hh:132-135
+}
+void or_subtract(const double * x, const double * y, double * z, int64_t n) {
+	subtract(x, y, z, n);
+}
+void or_multiply(const double * x, const double * y, double * z, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) z[i] = x[i] * y[i]; // seq.hh:142-145
+}
+void or_divide(const double * x, const double * y, double * z, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) z[i] = x[i] / y[i]; // seq.hh:147-150
+}
+void or_reciprocal(const double * x, double * y, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) y[i] = 1.0 / x[i]; // topo_tasks.hh:140-142
+}
+void or_linear_sum(double a, const double * x, double b, const double * y,
+                   double * z, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) z[i] = a * x[i] + b * y[i]; // seq.hh:165-169
+}
+void or_axpy(double a, const double * x, const double * y, double * z,
+             int64_t n) {
+	axpy(a, x, y, z, n);
+}
+void or_axpby(double a, double b, const double * x, double * z, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) z[i] = a * x[i] + b * z[i]; // seq.hh:176-179
+}
+void or_abs(const double * x, double * y, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) y[i] = std::abs(x[i]); // seq.hh:181-184
+}
+void or_add_scalar(const double * x, double a, double * y, int64_t n) {
+	for (int64_t i = 0; i < n; ++i) y[i] = x[i] + a; // seq.hh:186-189
+}
+double or_min(const double * x, int64_t n) {
+	double curr = std::numeric_limits<double>::max(); // seq.hh:193
+	for (int64_t i = 0; i < n; ++i) curr = std::min(x[i], curr);
+	return curr;
+}
+double or_max(const double * x, int64_t n) {
+	double curr = std::numeric_limits<double>::lowest(); // seq.hh:208
+	for (int64_t i = 0; i < n; ++i) curr = std::max(x[i], curr);
+	return curr;
+}
+double or_l1norm(const double * x, int64_t n) {
+	double curr = 0; // seq.hh:222-242 (p == 1)
+	for (int64_t i = 0; i < n; ++i) curr += std::abs(x[i]);
+	return curr;
+}
+double or_l2norm(const double * x, int64_t n) { return l2norm(x, n); }
+double or_inf_norm(const double * x, int64_t n) {
+	double ret = std::numeric_limits<double>::min(); // seq.hh:251 quirk
+	for (int64_t i = 0; i < n; ++i) ret = std::max(std::abs(x[i]), ret);
+	return ret;
+}
+double or_dot(const double * x, const double * y, int64_t n) {
+	return scalar_prod(x, y, n);
+}
+// seq_ops::set_random (seq.hh:103-115): mt19937(seed), U[0,1) doubles
+void or_set_random(double * x, int64_t n, unsigned seed) {
+	std::mt19937 gen(seed);
+	std::uniform_real_distribution<double> dis(0., 1.);
+	for (int64_t i = 0; i < n; ++i) x[i] = dis(gen);
+}
+
+// ---- matrices --------------------------------------------------------------
+
+void or_spmv(const or_csr * A, const double * x, double * y) { spmv(*A, x, y); }
+
+void or_residual(const or_csr * A, const double * b, const double * x,
+                 double * r) {
+	spmv(*A, x, r);
+	subtract(b, r, r, A->nrows);
+}
+
+// coo::tocsr (matrices/seq.hh:324-352).  Output index arrays are int64.
+void or_coo_tocsr(int64_t nrows, int64_t nnz, const int64_t * I,
+                  const int64_t * J, const double * V, int64_t * rowptr,
+                  int64_t * colind, double * values) {
+	std::vector<std::size_t> ind(nnz);
+	std::iota(ind.begin(), ind.end(), 0);
+	std::stable_sort(ind.begin(), ind.end(),
+	                 [&](std::size_t i1, std::size_t i2) { return I[i1] < I[i2]; });
+	for (int64_t i = 0; i <= nrows; ++i) rowptr[i] = 0;
+	for (int64_t i = 0; i < nnz; ++i) {
+		++rowptr[I[ind[i]] + 1];
+		colind[i] = J[ind[i]];
+		values[i] = V[ind[i]];
+	}
+	for (int64_t i = 0; i < nrows; ++i) rowptr[i + 1] += rowptr[i];
+}
+
+// Dinv CSR of csr_op_gen (flecsolve/util/test/mesh.hh:123-140): diagonal
+// matrix with 1/a_ii.  Outputs have nrows entries (the reference allocates
+// nnz and leaves the tail unused); rows without a diagonal hold value 0 at
+// column 0 (value-initialised storage).
+void or_dinv_csr(const or_csr * A, int64_t * rowptr, int64_t * colind,
+                 double * values) {
+	rowptr[0] = 0;
+	for (int64_t i = 0; i < A->nrows; i++) {
+		values[i] = 0.0;
+		colind[i] = 0;
+		int64_t lo, hi;
+		if (A->itype == 0) {
+			lo = ((const int32_t *)A->rowptr)[i];
+			hi = ((const int32_t *)A->rowptr)[i + 1];
+		}
+		else {
+			lo = ((const int64_t *)A->rowptr)[i];
+			hi = ((const int64_t *)A->rowptr)[i + 1];
+		}
+		for (int64_t off = lo; off < hi; off++) {
+			int64_t c = A->itype == 0 ? ((const int32_t *)A->colind)[off]
+			                          : ((const int64_t *)A->colind)[off];
+			if (c == i) {
+				values[i] = 1.0 / A->vals[off];
+				colind[i] = i;
+			}
+		}
+		rowptr[i + 1] = i + 1;
+	}
+}
+
+// Matrix Market read (matrix_market.hh:67-103) into COO arrays.  Call with
+// null arrays first to get the header (returns entry capacity needed).
+int64_t or_mm_read(const char * fname, int64_t * nrows, int64_t * ncols,
+                   int * symmetric, int64_t * I, int64_t * J, double * V,
+                   int64_t cap) {
+	std::ifstream fh(fname);
+	if (!fh) return -1;
+	mm_header hdr;
+	if (!mm_read_header(fh, hdr)) return -1;
+	*nrows = hdr.nrows;
+	*ncols = hdr.ncols;
+	*symmetric = hdr.symmetric;
+	int64_t need = hdr.symmetric ? 2 * hdr.nnz : hdr.nnz;
+	if (!I) return need;
+	int64_t cnt = 0;
+	for (int64_t i = 0; i < hdr.nnz; ++i) {
+		std::string line;
+		std::getline(fh, line);
+		std::istringstream iss(line);
+		std::string tok;
+		std::getline(iss, tok, ' ');
+		int64_t row = std::stoi(tok) - 1;
+		std::getline(iss, tok, ' ');
+		int64_t col = std::stoi(tok) - 1;
+		std::getline(iss, tok, ' ');
+		double val = std::stof(tok); // value rounded through float
+		if (cnt >= cap) return -2;
+		I[cnt] = row; J[cnt] = col; V[cnt] = val; ++cnt;
+		if (hdr.symmetric && (row != col)) {
+			if (cnt >= cap) return -2;
+			I[cnt] = col; J[cnt] = row; V[cnt] = val; ++cnt;
+		}
+	}
+	return cnt;
+}
+
+void or_jacobi_relax(const or_csr * A, float omega, int nrelax,
+                     const double * b, double * x) {
+	vec tmp(A->nrows);
+	for (int s = 0; s < nrelax; ++s)
+		jacobi_relax(*A, (double)omega, b, x, tmp.data());
+}
+
+// ---- solvers ---------------------------------------------------------------
+
+int or_cg(const or_csr * A, const or_prec * P, const double * b, double * x,
+          const or_settings * s, or_diag_fn diag, void * user, or_info * info,
+          double * res_hist, int hist_cap) {
+	op_t a; a.A = A; a.kind = 1;
+	return cg(a, make_prec(P), b, x, A->nrows, *s, diag_t{diag, user}, info,
+	          res_hist, hist_cap);
+}
+
+int or_fcg(const or_csr * A, const or_prec * P, const double * b, double * x,
+           const or_settings * s, or_diag_fn diag, void * user,
+           or_info * info) {
+	op_t a; a.A = A; a.kind = 1;
+	return fcg(a, make_prec(P), b, x, A->nrows, *s, diag_t{diag, user}, info);
+}
+
+int or_gmres(const or_csr * A, const or_prec * P, const double * b, double * x,
+             const or_settings * s, or_diag_fn diag, void * user,
+             or_info * info, double * res_hist, int hist_cap) {
+	op_t a; a.A = A; a.kind = 1;
+	gmres_t g;
+	if (g.reset(*s, A->nrows) != 0) return -1;
+	g.apply(a, make_prec(P), b, x, diag_t{diag, user}, info, res_hist,
+	        hist_cap);
+	return 0;
+}
+
+// GMRES restart check of flecsolve/solvers/test/gmres.cc:88-110 needs the same
+// solver object to be re-applied with changed settings; `or_gmres` builds a
+// fresh object per call, which matches that test since reset() only sizes
+// workspace and each apply re-initialises everything it reads.
+
+int or_bicgstab(const or_csr * A, const or_prec * P, const double * b,
+                double * x, const or_settings * s, or_diag_fn diag,
+                void * user, or_info * info, double * res_hist,
+                int hist_cap) {
+	op_t a; a.A = A; a.kind = 1;
+	return bicgstab(a, make_prec(P), b, x, A->nrows, *s, diag_t{diag, user},
+	                info, res_hist, hist_cap);
+}
+
+} // extern "C"

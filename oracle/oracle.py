@@ -1,0 +1,314 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py, and nowhere else.  The oracle restates the
+reference's serial algorithm (see flecsolve_oracle.cpp for file:line
+citations and the list of pinned functions).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+STOP_REASONS = ["converged_atol", "converged_rtol", "converged_user",
+                "diverged_dtol", "diverged_iters", "diverged_breakdown",
+                "unknown"]
+
+
+class Settings(C.Structure):
+    _fields_ = [("maxiter", C.c_int), ("rtol", C.c_float), ("atol", C.c_float),
+                ("use_zero_guess", C.c_int), ("max_krylov_dim", C.c_int),
+                ("pre_side", C.c_int), ("restart", C.c_int)]
+
+    def __init__(self, maxiter=100, rtol=0.0, atol=0.0, use_zero_guess=False,
+                 max_krylov_dim=-1, pre_side=1, restart=False):
+        super().__init__(maxiter, rtol, atol, int(use_zero_guess),
+                         max_krylov_dim, pre_side, int(restart))
+
+
+class Info(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("restarts", C.c_int),
+                ("res_norm_initial", C.c_float), ("res_norm_final", C.c_float),
+                ("sol_norm_initial", C.c_float), ("sol_norm_final", C.c_float),
+                ("rhs_norm", C.c_float)]
+
+    @property
+    def reason(self):
+        return STOP_REASONS[self.status]
+
+
+class _Csr(C.Structure):
+    _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("nnz", C.c_int64),
+                ("itype", C.c_int), ("rowptr", C.c_void_p), ("colind", C.c_void_p),
+                ("vals", C.c_void_p)]
+
+
+class _Prec(C.Structure):
+    _fields_ = [("kind", C.c_int), ("M", C.POINTER(_Csr)), ("omega", C.c_float),
+                ("nrelax", C.c_int)]
+
+
+DIAG_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int64,
+                      C.c_double)
+
+_lib = None
+_D = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+        L = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        i64 = C.c_int64
+        for nm in ["or_copy"]:
+            getattr(L, nm).argtypes = [P, P, i64]
+        L.or_set_scalar.argtypes = [C.c_double, P, i64]
+        L.or_scale.argtypes = [C.c_double, P, i64]
+        L.or_scale_to.argtypes = [C.c_double, P, P, i64]
+        for nm in ["or_add", "or_subtract", "or_multiply", "or_divide"]:
+            getattr(L, nm).argtypes = [P, P, P, i64]
+        L.or_reciprocal.argtypes = [P, P, i64]
+        L.or_linear_sum.argtypes = [C.c_double, P, C.c_double, P, P, i64]
+        L.or_axpy.argtypes = [C.c_double, P, P, P, i64]
+        L.or_axpby.argtypes = [C.c_double, C.c_double, P, P, i64]
+        L.or_abs.argtypes = [P, P, i64]
+        L.or_add_scalar.argtypes = [P, C.c_double, P, i64]
+        for nm in ["or_min", "or_max", "or_l1norm", "or_l2norm", "or_inf_norm"]:
+            getattr(L, nm).argtypes = [P, i64]
+            getattr(L, nm).restype = C.c_double
+        L.or_dot.argtypes = [P, P, i64]
+        L.or_dot.restype = C.c_double
+        L.or_set_random.argtypes = [P, i64, C.c_uint]
+        L.or_spmv.argtypes = [C.POINTER(_Csr), P, P]
+        L.or_residual.argtypes = [C.POINTER(_Csr), P, P, P]
+        L.or_coo_tocsr.argtypes = [i64, i64, P, P, P, P, P, P]
+        L.or_dinv_csr.argtypes = [C.POINTER(_Csr), P, P, P]
+        L.or_mm_read.argtypes = [C.c_char_p, C.POINTER(i64), C.POINTER(i64),
+                                 C.POINTER(C.c_int), P, P, P, i64]
+        L.or_mm_read.restype = i64
+        L.or_jacobi_relax.argtypes = [C.POINTER(_Csr), C.c_float, C.c_int, P, P]
+        solver_args = [C.POINTER(_Csr), C.POINTER(_Prec), P, P, C.POINTER(Settings),
+                       DIAG_FN, P, C.POINTER(Info)]
+        L.or_cg.argtypes = solver_args + [_D, C.c_int]
+        L.or_fcg.argtypes = solver_args
+        L.or_gmres.argtypes = solver_args + [_D, C.c_int]
+        L.or_bicgstab.argtypes = solver_args + [_D, C.c_int]
+        for nm in ["or_cg", "or_fcg", "or_gmres", "or_bicgstab"]:
+            getattr(L, nm).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Csr:
+    """Holds numpy arrays alive for the oracle's CSR view."""
+
+    def __init__(self, rowptr, colind, vals, ncols=None):
+        rowptr = np.ascontiguousarray(rowptr)
+        if rowptr.dtype not in (np.int32, np.int64):
+            rowptr = rowptr.astype(np.int64)
+        idt = rowptr.dtype
+        self.rowptr = rowptr
+        self.colind = np.ascontiguousarray(colind, dtype=idt)
+        self.vals = np.ascontiguousarray(vals, dtype=np.float64)
+        self.nrows = rowptr.size - 1
+        self.ncols = self.nrows if ncols is None else ncols
+        self.s = _Csr(self.nrows, self.ncols, self.colind.size,
+                      0 if idt == np.int32 else 1, _p(self.rowptr),
+                      _p(self.colind), _p(self.vals))
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.nrows)
+        lib().or_spmv(C.byref(self.s), _p(x), _p(y))
+        return y
+
+    def residual(self, b, x):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        r = np.empty(self.nrows)
+        lib().or_residual(C.byref(self.s), _p(b), _p(x), _p(r))
+        return r
+
+    def dinv(self) -> "Csr":
+        rp = np.empty(self.nrows + 1, np.int64)
+        ci = np.empty(self.nrows, np.int64)
+        v = np.empty(self.nrows)
+        lib().or_dinv_csr(C.byref(self.s), _p(rp), _p(ci), _p(v))
+        return Csr(rp, ci, v)
+
+    def jacobi_relax(self, b, x, omega=np.float32(2 / 3.), nrelax=1):
+        x = np.array(x, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        lib().or_jacobi_relax(C.byref(self.s), float(omega), nrelax, _p(b), _p(x))
+        return x
+
+
+def prec(kind: int = 0, M: Csr | None = None, omega=np.float32(2 / 3.),
+         nrelax: int = 1):
+    """0 identity, 1 CSR operator M (e.g. Csr.dinv()), 2 Jacobi relaxation on M."""
+    p = _Prec(kind, C.pointer(M.s) if M is not None else None, float(omega), nrelax)
+    p._keep = M
+    return p
+
+
+def _solve(name, A: Csr, b, x0, settings: Settings, P=None, diag=None,
+           hist_cap=0):
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    info = Info()
+    if diag is None:
+        fn = DIAG_FN(0)
+    else:
+        def _cb(_u, xp, n, res):
+            return int(bool(diag(np.ctypeslib.as_array(xp, shape=(n,)), res)))
+        fn = DIAG_FN(_cb)
+    Pp = C.byref(P) if P is not None else None
+    args = [C.byref(A.s), Pp, _p(b), _p(x), C.byref(settings), fn, None,
+            C.byref(info)]
+    hist = np.zeros(max(hist_cap, 1))
+    if name != "or_fcg":
+        args += [hist.ctypes.data_as(_D), hist_cap]
+    rc = getattr(lib(), name)(*args)
+    return x, info, rc, hist[:hist_cap]
+
+
+def cg(A, b, x0, settings, P=None, diag=None, hist_cap=0):
+    x, info, rc, h = _solve("or_cg", A, b, x0, settings, P, diag, hist_cap)
+    return x, info, h
+
+
+def fcg(A, b, x0, settings, P=None, diag=None):
+    x, info, rc, _ = _solve("or_fcg", A, b, x0, settings, P, diag)
+    return x, info
+
+
+def gmres(A, b, x0, settings, P=None, diag=None, hist_cap=0):
+    x, info, rc, h = _solve("or_gmres", A, b, x0, settings, P, diag, hist_cap)
+    if rc != 0:
+        raise ValueError("gmres settings rejected (krylov dimension bound)")
+    return x, info, h
+
+
+def bicgstab(A, b, x0, settings, P=None, diag=None, hist_cap=0):
+    x, info, rc, h = _solve("or_bicgstab", A, b, x0, settings, P, diag, hist_cap)
+    return x, info, h
+
+
+# ---- BLAS-1 ----------------------------------------------------------------
+
+def _a(x):
+    return np.ascontiguousarray(x, np.float64)
+
+
+def dot(x, y):
+    x, y = _a(x), _a(y)
+    return lib().or_dot(_p(x), _p(y), x.size)
+
+
+def l2norm(x):
+    x = _a(x)
+    return lib().or_l2norm(_p(x), x.size)
+
+
+def l1norm(x):
+    x = _a(x)
+    return lib().or_l1norm(_p(x), x.size)
+
+
+def inf_norm(x):
+    x = _a(x)
+    return lib().or_inf_norm(_p(x), x.size)
+
+
+def vmin(x):
+    x = _a(x)
+    return lib().or_min(_p(x), x.size)
+
+
+def vmax(x):
+    x = _a(x)
+    return lib().or_max(_p(x), x.size)
+
+
+def set_random(n, seed):
+    x = np.empty(n)
+    lib().or_set_random(_p(x), n, seed)
+    return x
+
+
+def binop(name, x, y):
+    x, y = _a(x), _a(y)
+    z = np.empty_like(x)
+    getattr(lib(), "or_" + name)(_p(x), _p(y), _p(z), x.size)
+    return z
+
+
+def axpy(a, x, y):
+    x, y = _a(x), _a(y)
+    z = np.empty_like(x)
+    lib().or_axpy(a, _p(x), _p(y), _p(z), x.size)
+    return z
+
+
+def axpby(a, b, x, z):
+    x, z = _a(x), np.array(z, np.float64)
+    lib().or_axpby(a, b, _p(x), _p(z), x.size)
+    return z
+
+
+def linear_sum(a, x, b, y):
+    x, y = _a(x), _a(y)
+    z = np.empty_like(x)
+    lib().or_linear_sum(a, _p(x), b, _p(y), _p(z), x.size)
+    return z
+
+
+def unop(name, x, *scal):
+    x = _a(x)
+    y = np.empty_like(x)
+    if name == "scale_to":
+        lib().or_scale_to(scal[0], _p(x), _p(y), x.size)
+    elif name == "add_scalar":
+        lib().or_add_scalar(_p(x), scal[0], _p(y), x.size)
+    else:
+        getattr(lib(), "or_" + name)(_p(x), _p(y), x.size)
+    return y
+
+
+def coo_tocsr(nrows, I, J, V):
+    I = np.ascontiguousarray(I, np.int64)
+    J = np.ascontiguousarray(J, np.int64)
+    V = _a(V)
+    rp = np.empty(nrows + 1, np.int64)
+    ci = np.empty(I.size, np.int64)
+    vv = np.empty(I.size)
+    lib().or_coo_tocsr(nrows, I.size, _p(I), _p(J), _p(V), _p(rp), _p(ci), _p(vv))
+    return rp, ci, vv
+
+
+def mm_read(path):
+    nr, nc, sym = C.c_int64(), C.c_int64(), C.c_int()
+    cap = lib().or_mm_read(str(path).encode(), C.byref(nr), C.byref(nc),
+                           C.byref(sym), None, None, None, 0)
+    if cap < 0:
+        raise IOError(path)
+    I = np.empty(cap, np.int64)
+    J = np.empty(cap, np.int64)
+    V = np.empty(cap)
+    cnt = lib().or_mm_read(str(path).encode(), C.byref(nr), C.byref(nc), C.byref(sym),
+                           _p(I), _p(J), _p(V), cap)
+    if cnt < 0:
+        raise IOError(path)
+    return nr.value, nc.value, bool(sym.value), I[:cnt], J[:cnt], V[:cnt]

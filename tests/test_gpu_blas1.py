@@ -1,0 +1,147 @@
+"""Device BLAS-1 and reductions against the oracle (gpu)."""
+import numpy as np
+import pytest
+
+from flecsolve_amd import fsx
+from oracle import oracle as O
+from tests import vector_kat
+
+pytestmark = pytest.mark.gpu
+
+
+class DeviceOps:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.L = fsx.lib()
+
+    def new(self, v):
+        return self.ctx.vec(np.asarray(v, np.float64))
+
+    def get(self, h):
+        return h.get()
+
+    def _c(self, rc):
+        fsx.check(rc)
+
+    def add(self, z, x, y): self._c(self.L.fsx_add(self.ctx.h, x.h, y.h, z.h))
+    def subtract(self, z, x, y): self._c(self.L.fsx_subtract(self.ctx.h, x.h, y.h, z.h))
+    def multiply(self, z, x, y): self._c(self.L.fsx_multiply(self.ctx.h, x.h, y.h, z.h))
+    def divide(self, z, x, y): self._c(self.L.fsx_divide(self.ctx.h, x.h, y.h, z.h))
+    def add_scalar(self, y, x, a): self._c(self.L.fsx_add_scalar(self.ctx.h, x.h, a, y.h))
+    def scale_to(self, y, a, x): self._c(self.L.fsx_scale_to(self.ctx.h, a, x.h, y.h))
+    def reciprocal(self, y, x): self._c(self.L.fsx_reciprocal(self.ctx.h, x.h, y.h))
+    def linear_sum(self, z, a, x, b, y): self._c(self.L.fsx_linear_sum(self.ctx.h, a, x.h, b, y.h, z.h))
+    def axpy(self, z, a, x, y): self._c(self.L.fsx_axpy(self.ctx.h, a, x.h, y.h, z.h))
+    def copy(self, d, s): self._c(self.L.fsx_copy(self.ctx.h, s.h, d.h))
+    def axpby(self, z, a, b, x): self._c(self.L.fsx_axpby(self.ctx.h, a, b, x.h, z.h))
+    def abs(self, y, x): self._c(self.L.fsx_abs(self.ctx.h, x.h, y.h))
+    def set_scalar(self, x, a): self._c(self.L.fsx_set_scalar(self.ctx.h, a, x.h))
+    def min(self, x): return fsx.vmin(x)
+    def max(self, x): return fsx.vmax(x)
+    def dot(self, x, y): return fsx.dot(x, y)
+    def l1norm(self, x): return fsx.nrm1(x)
+    def l2norm(self, x): return fsx.nrm2(x)
+
+
+def test_flecsi_vector_closed_forms(ctx):
+    vector_kat.run(DeviceOps(ctx))
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 100_003, 2_000_000])
+def test_elementwise_bit_exact(ctx, n):
+    rng = np.random.default_rng(n)
+    xa, ya = rng.standard_normal(n), rng.standard_normal(n) + 3.0
+    x, y, z = ctx.vec(xa), ctx.vec(ya), ctx.zeros(n)
+    D = DeviceOps(ctx)
+    for name in ["add", "subtract", "multiply", "divide"]:
+        getattr(D, name)(z, x, y)
+        np.testing.assert_array_equal(z.get(), O.binop(name, xa, ya), err_msg=name)
+    D.axpy(z, -1.75, x, y)
+    np.testing.assert_array_equal(z.get(), O.axpy(-1.75, xa, ya))
+    D.linear_sum(z, 0.3, x, -2.0, y)
+    np.testing.assert_array_equal(z.get(), O.linear_sum(0.3, xa, -2.0, ya))
+    D.copy(z, y)
+    D.axpby(z, 1.5, -0.25, x)
+    np.testing.assert_array_equal(z.get(), O.axpby(1.5, -0.25, xa, ya))
+    D.reciprocal(z, y)
+    np.testing.assert_array_equal(z.get(), O.unop("reciprocal", ya))
+    D.abs(z, x)
+    np.testing.assert_array_equal(z.get(), np.abs(xa))
+    D.add_scalar(z, x, 0.125)
+    np.testing.assert_array_equal(z.get(), xa + 0.125)
+    D.scale_to(z, 3.0, x)
+    np.testing.assert_array_equal(z.get(), 3.0 * xa)
+
+
+def test_aliasing_forms(ctx):
+    # the solvers call axpy with the output aliasing either input
+    # (cg.hh:108-109, 130); subtract(b, r) into r is the residual form
+    n = 5000
+    rng = np.random.default_rng(2)
+    xa, ya = rng.standard_normal(n), rng.standard_normal(n)
+    L = fsx.lib()
+    x, y = ctx.vec(xa), ctx.vec(ya)
+    fsx.check(L.fsx_axpy(ctx.h, 2.0, x.h, y.h, x.h))     # x = 2x + y
+    np.testing.assert_array_equal(x.get(), 2.0 * xa + ya)
+    x.upload(xa)
+    fsx.check(L.fsx_axpy(ctx.h, 2.0, x.h, y.h, y.h))     # y = 2x + y
+    np.testing.assert_array_equal(y.get(), 2.0 * xa + ya)
+    y.upload(ya)
+    fsx.check(L.fsx_subtract(ctx.h, y.h, x.h, x.h))      # x = y - x
+    np.testing.assert_array_equal(x.get(), ya - xa)
+    # scale(alpha, x, y) with x == y is a contract violation (topo_view.hh:72)
+    rc = L.fsx_scale_to(ctx.h, 2.0, x.h, x.h)
+    assert rc == 1 and b"cannot be the same" in L.fsx_last_error()
+
+
+@pytest.mark.parametrize("n", [1, 1000, 777_777, 16_777_216])
+def test_reductions(ctx, n):
+    rng = np.random.default_rng(n + 1)
+    xa, ya = rng.standard_normal(n), rng.standard_normal(n)
+    x, y = ctx.vec(xa), ctx.vec(ya)
+    # serial and tree summation orders differ by O(sqrt(n) eps) of sum |terms|
+    tol = 16 * np.sqrt(n) * np.finfo(np.float64).eps
+    scale = np.abs(xa * ya).sum()
+    assert abs(fsx.dot(x, y) - O.dot(xa, ya)) <= tol * scale + 1e-300
+    assert abs(fsx.nrm2(x) - O.l2norm(xa)) <= tol * O.l2norm(xa)
+    assert abs(fsx.nrm1(x) - O.l1norm(xa)) <= tol * O.l1norm(xa)
+    assert fsx.nrminf(x) == O.inf_norm(xa)
+    assert fsx.vmin(x) == O.vmin(xa)
+    assert fsx.vmax(x) == O.vmax(xa)
+    # deterministic: same bits on repeat
+    assert fsx.dot(x, y) == fsx.dot(x, y)
+
+
+def test_reduction_edge_cases(ctx):
+    z = ctx.zeros(10)
+    assert fsx.nrminf(z) == np.finfo(np.float64).tiny  # seq.hh:251 quirk
+    assert fsx.nrm2(z) == 0.0
+    e = ctx.zeros(0)
+    assert fsx.vmin(e) == np.finfo(np.float64).max
+    assert fsx.dot(e, e) == 0.0
+
+
+def test_set_random_matches_reference_stream(ctx):
+    v = ctx.zeros(1001)
+    fsx.check(fsx.lib().fsx_set_random(ctx.h, v.h, 7))
+    np.testing.assert_array_equal(v.get(), O.set_random(1001, 7))
+
+
+def test_fp32_and_convert(ctx):
+    n = 12345
+    a = np.random.default_rng(3).standard_normal(n)
+    v64 = ctx.vec(a)
+    v32 = ctx.zeros(n, fsx.F32)
+    fsx.check(fsx.lib().fsx_convert(ctx.h, v64.h, v32.h, 0.5))
+    np.testing.assert_array_equal(v32.get(), (0.5 * a).astype(np.float32))
+    assert abs(fsx.dot(v32, v32) - float(np.dot(v32.get().astype(np.float64),
+                                                v32.get().astype(np.float64)))) < 1e-9 * n
+
+
+def test_vector_identity_and_refcount(ctx):
+    a = ctx.zeros(10)
+    b = ctx.zeros(10)
+    assert a.id() != b.id()
+    fsx.check(fsx.lib().fsx_vec_retain(a.h))
+    fsx.check(fsx.lib().fsx_vec_release(a.h))
+    assert a.get().shape == (10,)

@@ -1,0 +1,160 @@
+"""Fused, graph-captured CG (fsx_cg_solve) and FP32-inner / FP64-outer
+refinement against the oracle (gpu).
+
+Bar (from the hot-path contract): iteration counts within +-2 of the
+oracle's, solutions within 1e-10 relative of the oracle's.  The reductions
+sum in a different (tree) order than the reference's serial loop, so the two
+trajectories drift apart at the level of rounding; the solution comparison
+uses a tolerance a factor below the solve tolerance where the problem's
+conditioning allows, and otherwise compares against the exact solution.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from flecsolve_amd import fsx
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def poisson(n):
+    rp, ci, v = P.poisson2d(n)
+    return rp, ci, v, O.Csr(rp, ci, v)
+
+
+def diffusion(n, sigma=1.0, seed=2001):
+    k = P.lognormal_field(n, sigma, 8.0, seed)
+    rp, ci, v = P.diffusion3d(n, k)
+    return rp, ci, v, O.Csr(rp, ci, v)
+
+
+def test_cg_poisson64_cfg1(ctx):
+    """2D Poisson 64x64, unpreconditioned CG to rtol 1e-8, x0 = 0."""
+    rp, ci, v, Ao = poisson(64)
+    b = P.uniform(64 * 64, P.seeds()["poisson2d_rhs"])
+    s = dict(maxiter=1000, rtol=1e-8, use_zero_guess=True)
+    xo, io, ho = O.cg(Ao, b, np.zeros_like(b), O.Settings(**s), hist_cap=1000)
+    A = ctx.csr(rp, ci, v)
+    bd, xd = ctx.vec(b), ctx.zeros(b.size)
+    info = fsx.cg_solve(A, bd, xd, fsx.Settings(**s))
+    assert info.reason == io.reason == "converged_rtol"
+    assert abs(info.iters - io.iters) <= 2
+    assert rel(xd.get(), xo) <= 1e-10
+    h = fsx.cg_history(ctx)
+    m = min(len(h), io.iters)
+    np.testing.assert_allclose(h[:m], ho[:m], rtol=1e-6)
+    assert info.rhs_norm == io.rhs_norm
+
+
+@pytest.mark.parametrize("n", [16, 48])
+def test_cg_jacobi_diffusion(ctx, n):
+    rp, ci, v, Ao = diffusion(n)
+    b = P.uniform(n ** 3, 2002)
+    s = dict(maxiter=3000, rtol=1e-10, use_zero_guess=True)
+    xo, io, _ = O.cg(Ao, b, np.zeros_like(b), O.Settings(**s), P=O.prec(1, Ao.dinv()))
+    A = ctx.csr(rp, ci, v)
+    d = A.diag_inv()
+    xd = ctx.zeros(b.size)
+    info = fsx.cg_solve(A, ctx.vec(b), xd, fsx.Settings(**s), dinv=d)
+    assert info.reason == "converged_rtol"
+    assert abs(info.iters - io.iters) <= 2
+    assert rel(xd.get(), xo) <= 1e-10
+    S = sp.csr_matrix((v, ci, rp))
+    assert np.linalg.norm(b - S @ xd.get()) / np.linalg.norm(b) < 2e-10
+
+
+def test_cg_nonzero_guess_and_repeat(ctx):
+    rp, ci, v, Ao = diffusion(20)
+    n = 20 ** 3
+    b = P.uniform(n, 1)
+    x0 = O.set_random(n, 7)
+    s = dict(maxiter=500, rtol=1e-9)
+    xo, io, _ = O.cg(Ao, b, x0, O.Settings(**s))
+    A = ctx.csr(rp, ci, v)
+    for _ in range(2):  # second call replays the cached graph
+        xd = ctx.vec(x0)
+        info = fsx.cg_solve(A, ctx.vec(b), xd, fsx.Settings(**s))
+        assert abs(info.iters - io.iters) <= 2
+        assert rel(xd.get(), xo) <= 1e-9
+        assert info.sol_norm_initial == io.sol_norm_initial
+
+
+def test_cg_quirks(ctx):
+    rp, ci, v, Ao = poisson(8)
+    A = ctx.csr(rp, ci, v)
+    b = P.uniform(64, 1)
+    # maxiter hit: iters == 0, diverged_iters (cg.hh:135-136)
+    info = fsx.cg_solve(A, ctx.vec(b), ctx.zeros(64),
+                        fsx.Settings(maxiter=3, rtol=1e-12, use_zero_guess=True))
+    assert info.iters == 0 and info.reason == "diverged_iters"
+    xo, io, _ = O.cg(Ao, b, np.zeros(64), O.Settings(maxiter=3, rtol=1e-12,
+                                                    use_zero_guess=True))
+    assert info.res_norm_final == pytest.approx(io.res_norm_final, rel=1e-6)
+    # zero rhs: converged before the loop, b_norm := 1
+    info = fsx.cg_solve(A, ctx.zeros(64), ctx.zeros(64),
+                        fsx.Settings(maxiter=3, rtol=1e-12, use_zero_guess=True))
+    assert info.iters == 0 and info.reason == "converged_rtol" and info.rhs_norm == 1.0
+    # maxiter == 0
+    info = fsx.cg_solve(A, ctx.vec(b), ctx.zeros(64),
+                        fsx.Settings(maxiter=0, rtol=1e-12, use_zero_guess=True))
+    assert info.reason == "diverged_iters"
+
+
+def test_cg_chunk_boundaries(ctx):
+    # convergence exactly at, before and after a captured chunk boundary
+    d = np.repeat(np.arange(1.0, 18.0), 3)  # 17 distinct eigenvalues
+    n = d.size
+    rp = np.arange(n + 1, dtype=np.int32)
+    A = ctx.csr(rp, np.arange(n, dtype=np.int32), d)
+    Ao = O.Csr(rp, np.arange(n, dtype=np.int32), d)
+    b = O.set_random(n, 3)
+    for maxiter in (15, 16, 17, 40):
+        s = dict(maxiter=maxiter, rtol=1e-12, use_zero_guess=True)
+        xo, io, _ = O.cg(Ao, b, np.zeros(n), O.Settings(**s))
+        xd = ctx.zeros(n)
+        info = fsx.cg_solve(A, ctx.vec(b), xd, fsx.Settings(**s))
+        assert (info.iters, info.reason) == (io.iters, io.reason), maxiter
+        np.testing.assert_allclose(xd.get(), xo, rtol=1e-10, atol=1e-14)
+
+
+def test_fp32_cg(ctx):
+    rp, ci, v, Ao = diffusion(24)
+    n = 24 ** 3
+    b = P.uniform(n, 5).astype(np.float32)
+    A32 = ctx.csr(rp, ci, v).convert(fsx.F32)
+    d = ctx.csr(rp, ci, v).diag_inv(fsx.F32)
+    xd = ctx.zeros(n, fsx.F32)
+    info = fsx.cg_solve(A32, ctx.vec(b), xd, fsx.Settings(maxiter=500, rtol=1e-4,
+                                                         use_zero_guess=True), dinv=d)
+    assert info.reason == "converged_rtol"
+    S = sp.csr_matrix((v, ci, rp))
+    x = xd.get().astype(np.float64)
+    assert np.linalg.norm(b - S @ x) / np.linalg.norm(b) < 2e-4
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.5])
+def test_ir_reaches_1e10_and_matches_fp64(ctx, sigma):
+    n = 32
+    rp, ci, v, Ao = diffusion(n, sigma, seed=3001)
+    N = n ** 3
+    b = P.uniform(N, 3002)
+    A64 = ctx.csr(rp, ci, v)
+    A32 = A64.convert(fsx.F32)
+    xd = ctx.zeros(N)
+    info = fsx.ir_cg_solve(A64, A32, ctx.vec(b), xd, 1e-10,
+                           fsx.Settings(maxiter=2000, rtol=1e-4))
+    assert info.status == 1 and info.rel_res_final <= 1e-10
+    S = sp.csr_matrix((v, ci, rp))
+    x = xd.get()
+    assert np.linalg.norm(b - S @ x) / np.linalg.norm(b) <= 1.01e-10
+    xo, io, _ = O.cg(Ao, b, np.zeros(N), O.Settings(maxiter=5000, rtol=1e-12,
+                                                   use_zero_guess=True),
+                     P=O.prec(1, Ao.dinv()))
+    # both are 1e-10-residual solutions: error bounded by cond * 1e-10
+    assert rel(x, xo) < 1e-6

@@ -1,0 +1,160 @@
+"""Generic C++ solvers (include/flecsolve/solvers) over device vectors,
+called through fsx_krylov_*, against the oracle (gpu).
+
+Same algorithm on both sides; only the reduction order differs (device tree
+vs serial), so iteration counts agree within +-2 and solutions to 1e-10.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from flecsolve_amd import fsx
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def convdiff(n):
+    rp, ci, v = P.convdiff3d(n, (10.0, 5.0, 2.5))
+    return rp, ci, v, O.Csr(rp, ci, v)
+
+
+def _problem(name):
+    if name == "poisson64":
+        rp, ci, v = P.poisson2d(64)
+        return rp, ci, v, P.uniform(64 * 64, P.seeds()["poisson2d_rhs"]), 1e-8
+    k = P.lognormal_field(20, 1.0, 8.0, 2001)
+    rp, ci, v = P.diffusion3d(20, k)
+    return rp, ci, v, P.uniform(8000, 2002), 1e-10
+
+
+@pytest.mark.parametrize("name,prec,slack", [
+    ("poisson64", 0, 2),      # 2D Poisson 64^2, unpreconditioned, rtol 1e-8
+    ("diffusion20", 1, 2),    # lognormal-k diffusion, CG + Jacobi, rtol 1e-10
+    # Stress case outside the configurations: unpreconditioned CG on the
+    # lognormal-k operator needs ~400 iterations; the serial-vs-tree order of
+    # the dot products moves the count by up to ~1% there.
+    ("diffusion20", 0, 4),
+])
+def test_generic_cg(ctx, name, prec, slack):
+    rp, ci, v, b, rtol = _problem(name)
+    n = b.size
+    Ao = O.Csr(rp, ci, v)
+    s = dict(maxiter=2000, rtol=rtol, use_zero_guess=True)
+    xo, io, ho = O.cg(Ao, b, np.zeros(n), O.Settings(**s),
+                      P=O.prec(1, Ao.dinv()) if prec else None, hist_cap=2000)
+    A = ctx.csr(rp, ci, v)
+    xd = ctx.zeros(n)
+    info, h = fsx.krylov("cg", A, ctx.vec(b), xd, fsx.Settings(**s), prec=prec,
+                         hist_cap=2000)
+    assert info.reason == io.reason == "converged_rtol"
+    assert abs(info.iters - io.iters) <= slack
+    assert rel(xd.get(), xo) <= 1e-10
+    if slack <= 2:
+        # residual histories coincide; in the stress case the (oscillating)
+        # unpreconditioned residual drifts by tens of percent mid-solve
+        # although the final iterates agree to 1e-10
+        m = min(info.iters, io.iters) - 5
+        np.testing.assert_allclose(h[:m], ho[:m], rtol=1e-4)
+
+
+def test_generic_fcg(ctx):
+    rp, ci, v = P.poisson2d(32)
+    Ao = O.Csr(rp, ci, v)
+    b = P.uniform(1024, 3)
+    s = dict(maxiter=500, rtol=1e-10, use_zero_guess=True)
+    xo, io = O.fcg(Ao, b, np.zeros(1024), O.Settings(**s))
+    xd = ctx.zeros(1024)
+    info, _ = fsx.krylov("fcg", ctx.csr(rp, ci, v), ctx.vec(b), xd, fsx.Settings(**s))
+    assert abs(info.iters - io.iters) <= 2
+    assert rel(xd.get(), xo) <= 1e-10
+
+
+@pytest.mark.parametrize("side,prec", [(1, 0), (1, 1), (0, 1)])
+def test_generic_gmres30(ctx, side, prec):
+    rp, ci, v, Ao = convdiff(16)
+    n = 16 ** 3
+    b = P.uniform(n, 2101)
+    s = dict(maxiter=400, rtol=1e-10, use_zero_guess=True, max_krylov_dim=30,
+             restart=True, pre_side=side)
+    xo, io, _ = O.gmres(Ao, b, np.zeros(n), O.Settings(**s),
+                        P=O.prec(1, Ao.dinv()) if prec else None)
+    xd = ctx.zeros(n)
+    info, _ = fsx.krylov("gmres", ctx.csr(rp, ci, v), ctx.vec(b), xd,
+                         fsx.Settings(**s), prec=prec)
+    assert info.reason == io.reason == "converged_rtol"
+    assert abs(info.iters - io.iters) <= 2
+    assert info.restarts == io.restarts or abs(info.iters - io.iters) > 0
+    assert rel(xd.get(), xo) <= 1e-10
+
+
+def test_generic_gmres_restart_split(ctx):
+    # flecsolve/solvers/test/gmres.cc:88-110 on the device path
+    rp, ci, v = P.poisson2d(24)
+    A = ctx.csr(rp, ci, v)
+    n = 576
+    b = ctx.vec(O.set_random(n, 0))
+    x0 = O.set_random(n, 1)
+    x = ctx.vec(x0)
+    ir, _ = fsx.krylov("gmres", A, b, x, fsx.Settings(maxiter=100, rtol=1e-6,
+                                                     max_krylov_dim=50, restart=True))
+    x = ctx.vec(x0)
+    fsx.krylov("gmres", A, b, x, fsx.Settings(maxiter=50, rtol=1e-6,
+                                              max_krylov_dim=50, restart=True))
+    i2, _ = fsx.krylov("gmres", A, b, x, fsx.Settings(maxiter=100, rtol=1e-6,
+                                                     max_krylov_dim=100, restart=False))
+    assert 50 + i2.iters == ir.iters
+    assert i2.res_norm_final == ir.res_norm_final
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_generic_bicgstab(ctx, prec):
+    rp, ci, v, Ao = convdiff(16)
+    n = 16 ** 3
+    b = P.uniform(n, 2101)
+    s = dict(maxiter=400, rtol=1e-10, use_zero_guess=True)
+    xo, io, _ = O.bicgstab(Ao, b, np.zeros(n), O.Settings(**s),
+                           P=O.prec(1, Ao.dinv()) if prec else None)
+    xd = ctx.zeros(n)
+    info, _ = fsx.krylov("bicgstab", ctx.csr(rp, ci, v), ctx.vec(b), xd,
+                         fsx.Settings(**s), prec=prec)
+    assert info.reason == io.reason == "converged_rtol"
+    assert abs(info.iters - io.iters) <= 2
+    assert rel(xd.get(), xo) <= 1e-10
+
+
+def test_generic_jacobi_relax_preconditioner(ctx):
+    # mg::bound_jacobi relaxes in place from the output vector's current
+    # contents (solvers/mg/jacobi.hh:44-53), so as a GMRES preconditioner it
+    # is affine in stale work-vector state rather than linear.  The rotated
+    # residual estimate then converges while the true residual does not (the
+    # reference behaves the same way: the oracle's x has the same large true
+    # residual), and rounding differences are amplified; the two sides agree
+    # on the iteration count and on x to 1e-6.
+    rp, ci, v = P.poisson2d(16)
+    Ao = O.Csr(rp, ci, v)
+    S = sp.csr_matrix((v, ci, rp))
+    b = P.uniform(256, 4)
+    s = dict(maxiter=100, rtol=1e-9, use_zero_guess=True)
+    xo, io, _ = O.gmres(Ao, b, np.zeros(256), O.Settings(**s, max_krylov_dim=100),
+                        P=O.prec(2, Ao, nrelax=2))
+    xd = ctx.zeros(256)
+    info, _ = fsx.krylov("gmres", ctx.csr(rp, ci, v), ctx.vec(b), xd,
+                         fsx.Settings(**s, max_krylov_dim=100), prec=2, nrelax=2)
+    assert info.reason == io.reason == "converged_rtol"
+    assert abs(info.iters - io.iters) <= 2
+    assert rel(xd.get(), xo) <= 1e-6
+    tr = [np.linalg.norm(b - S @ x) / np.linalg.norm(b) for x in (xd.get(), xo)]
+    assert abs(tr[0] - tr[1]) <= 1e-6 * tr[1]
+
+
+def test_generic_rejects_bad_settings(ctx):
+    A = ctx.csr(*P.poisson2d(4))
+    with pytest.raises(fsx.FsxError):
+        fsx.krylov("gmres", A, ctx.vec(np.ones(16)), ctx.zeros(16),
+                   fsx.Settings(maxiter=200, max_krylov_dim=-1))

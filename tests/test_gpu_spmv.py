@@ -1,0 +1,193 @@
+"""Device CSR SpMV and fused kernels against the oracle (gpu).
+
+Bar: structure round trips bit-exact; SpMV on the LDS-staged row-block
+mapping bit-identical to the reference's serial CSR loop (it sums each row
+serially in CSR order); the vector-per-row mapping within 1e-13 relative
+(tree order inside a row).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from flecsolve_amd import fsx
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def ragged(n, seed, maxlen=300, itype=np.int32):
+    """Rows of very different lengths (including empty rows)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 12, n)
+    lens[rng.integers(0, n, max(1, n // 50))] = rng.integers(100, maxlen, max(1, n // 50))
+    lens[::97] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(itype)
+    ci = rng.integers(0, n, rp[-1]).astype(itype)
+    v = rng.standard_normal(rp[-1])
+    return rp, ci, v
+
+
+def check_spmv(ctx, rp, ci, v, exact, ncols=None, vtype=None):
+    A = ctx.csr(rp, ci, v, ncols=ncols, vtype=vtype)
+    ncols = A.ncols
+    x = np.random.default_rng(9).standard_normal(ncols)
+    xd, yd = ctx.vec(x), ctx.zeros(A.nrows)
+    A.spmv(xd, yd)
+    vals64 = np.asarray(v, np.float64) if vtype != fsx.F32 else np.asarray(v, np.float32).astype(np.float64)
+    ref = O.Csr(rp, ci, vals64, ncols=ncols).spmv(x)
+    y = yd.get()
+    if exact:
+        np.testing.assert_array_equal(y, ref)
+    else:
+        assert rel(y, ref) <= 1e-13
+    return A
+
+
+def test_structure_round_trip(ctx):
+    rp, ci, v = P.poisson2d(64)
+    A = ctx.csr(rp, ci, v)
+    r2, c2, v2 = A.download()
+    np.testing.assert_array_equal(r2, rp)
+    np.testing.assert_array_equal(c2, ci)
+    np.testing.assert_array_equal(v2, v)
+    assert A.plan() == (0, 0)
+
+
+@pytest.mark.parametrize("n", [1, 7, 64])
+def test_poisson2d_bit_exact(ctx, n):
+    rp, ci, v = P.poisson2d(n)
+    check_spmv(ctx, rp, ci, v, exact=True)
+
+
+@pytest.mark.parametrize("n", [5, 32, 96])
+def test_diffusion3d_bit_exact(ctx, n):
+    k = P.lognormal_field(n, 1.0, 8.0, 2001)
+    rp, ci, v = P.diffusion3d(n, k)
+    check_spmv(ctx, rp, ci, v, exact=True)
+
+
+def test_convdiff_bit_exact(ctx):
+    rp, ci, v = P.convdiff3d(40)
+    check_spmv(ctx, rp, ci, v, exact=True)
+
+
+@pytest.mark.parametrize("itype", [np.int32, np.int64])
+def test_ragged_rows_both_mappings(ctx, itype):
+    rp, ci, v = ragged(20_000, 1, itype=itype)
+    A = check_spmv(ctx, rp, ci, v, exact=False)
+    x = np.random.default_rng(9).standard_normal(A.ncols)
+    ref = O.Csr(rp, ci, v).spmv(x)
+    xd, yd = ctx.vec(x), ctx.zeros(A.nrows)
+    # LDS-staged blocks: the long rows overflow their block -> wavefront per row
+    A.set_plan(0)
+    A.spmv(xd, yd)
+    assert rel(yd.get(), ref) <= 1e-13
+    for lanes in (2, 4, 8, 16, 32, 64):
+        A.set_plan(1, lanes)
+        A.spmv(xd, yd)
+        assert rel(yd.get(), ref) <= 1e-13, lanes
+
+
+def test_long_rows_pick_vector_mapping(ctx):
+    S = sp.random(3000, 5000, density=0.02, random_state=3, format="csr")
+    A = check_spmv(ctx, S.indptr.astype(np.int32), S.indices.astype(np.int32),
+                   S.data, exact=False, ncols=5000)
+    kind, lanes = A.plan()
+    assert (kind, lanes) == (1, 64)  # 100 nonzeros per row
+
+
+def test_empty_and_rectangular(ctx):
+    rp = np.zeros(11, np.int32)
+    A = ctx.csr(rp, np.zeros(0, np.int32), np.zeros(0), ncols=4)
+    y = ctx.vec(np.ones(10))
+    A.spmv(ctx.vec(np.ones(4)), y)
+    np.testing.assert_array_equal(y.get(), np.zeros(10))
+
+
+def test_fp32_values(ctx):
+    k = P.lognormal_field(24, 2.5, 8.0, 3001)
+    rp, ci, v = P.diffusion3d(24, k)
+    A64 = ctx.csr(rp, ci, v)
+    A32 = A64.convert(fsx.F32)
+    x = np.random.default_rng(1).standard_normal(24 ** 3)
+    y = ctx.zeros(24 ** 3)
+    A32.spmv(ctx.vec(x), y)   # FP32 values, FP64 x and accumulation
+    ref = O.Csr(rp, ci, v.astype(np.float32).astype(np.float64)).spmv(x)
+    np.testing.assert_array_equal(y.get(), ref)
+    x32 = x.astype(np.float32)
+    y32 = ctx.zeros(24 ** 3, fsx.F32)
+    A32.spmv(ctx.vec(x32), y32)
+    ref32 = sp.csr_matrix((v.astype(np.float32), ci, rp)) @ x32.astype(np.float64)
+    assert rel(y32.get().astype(np.float64), ref32) < 1e-5
+
+
+def test_residual_and_spmv_dot(ctx):
+    k = P.lognormal_field(32, 1.0, 8.0, 7)
+    rp, ci, v = P.diffusion3d(32, k)
+    A = ctx.csr(rp, ci, v)
+    Ao = O.Csr(rp, ci, v)
+    n = 32 ** 3
+    rng = np.random.default_rng(5)
+    b, x = rng.standard_normal(n), rng.standard_normal(n)
+    r = ctx.zeros(n)
+    A.residual(ctx.vec(b), ctx.vec(x), r)
+    np.testing.assert_array_equal(r.get(), Ao.residual(b, x))
+    w = ctx.zeros(n)
+    pw = fsx.spmv_dot(A, ctx.vec(x), w)
+    wref = Ao.spmv(x)
+    np.testing.assert_array_equal(w.get(), wref)
+    assert abs(pw - O.dot(wref, x)) <= 1e-12 * np.abs(wref * x).sum()
+
+
+def test_diag_inv_and_jacobi(ctx):
+    k = P.lognormal_field(16, 1.0, 4.0, 8)
+    rp, ci, v = P.diffusion3d(16, k)
+    A = ctx.csr(rp, ci, v)
+    Ao = O.Csr(rp, ci, v)
+    d = A.diag_inv()
+    Do = Ao.dinv()
+    np.testing.assert_array_equal(d.get(), Do.vals)
+    n = 16 ** 3
+    r = np.random.default_rng(6).standard_normal(n)
+    z = ctx.zeros(n)
+    rd = ctx.vec(r)  # keep the handle alive across the call
+    fsx.check(fsx.lib().fsx_jacobi_apply(ctx.h, d.h, rd.h, z.h))
+    np.testing.assert_array_equal(z.get(), Do.spmv(r))
+    # weighted Jacobi relaxation (mg/jacobi.hh:58-93), omega = float(2/3)
+    b = np.random.default_rng(7).standard_normal(n)
+    x0 = np.random.default_rng(8).standard_normal(n)
+    xd = ctx.vec(x0)
+    bd = ctx.vec(b)
+    fsx.check(fsx.lib().fsx_jacobi_relax(ctx.h, A.h, np.float32(2 / 3.), 3,
+                                         bd.h, xd.h))
+    np.testing.assert_array_equal(xd.get(), Ao.jacobi_relax(b, x0, np.float32(2 / 3.), 3))
+
+
+def test_transpose_product(ctx):
+    rp, ci, v = ragged(3000, 4)
+    A = ctx.csr(rp, ci, v)
+    S = sp.csr_matrix((v, ci, rp), shape=(3000, 3000))
+    x = np.random.default_rng(2).standard_normal(3000)
+    y = ctx.zeros(3000)
+    A.spmv_t(ctx.vec(x), y)
+    assert rel(y.get(), S.T @ x) <= 1e-13
+
+
+def test_rejects_bad_input(ctx):
+    L = fsx.lib()
+    rp = np.array([0, 2, 1], np.int32)  # not monotone
+    with pytest.raises(fsx.FsxError):
+        ctx.csr(rp, np.array([0, 1], np.int32), np.ones(2))
+    with pytest.raises(fsx.FsxError):
+        ctx.csr(np.array([0, 1], np.int32), np.array([5], np.int32), np.ones(1))
+    A = ctx.csr(*P.poisson2d(4))
+    with pytest.raises(fsx.FsxError):
+        A.spmv(ctx.zeros(3), ctx.zeros(16))
+    x = ctx.zeros(16)
+    with pytest.raises(fsx.FsxError):
+        A.spmv(x, x)
