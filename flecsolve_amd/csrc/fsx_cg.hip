@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kUpdThreads) void cg_update_kernel(
 	double * partials, const cg_state * st) {
 	if (st->done) return;
 	const double alpha = st->alpha;
-	const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+	const unsigned blk = blockIdx.x;
 	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
 	double rr = 0.0, rz = 0.0;
 #pragma unroll
@@ -366,8 +366,8 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 	hipGraphExec_t ge = nullptr;
 	if (use_graphs()) {
 		const fsx_csr * A = B.A;
-		cg_graph_cache::key_t key{A->rowptr(), A->colind(), A->vals->p,
-		                          B.x, B.r, B.z, B.p, B.w, B.dinv,
+		cg_graph_cache::key_t key{A->rowptr(), A->sl_vals ? A->sl_vals->p : nullptr,
+		                          A->vals->p, B.x, B.r, B.z, B.p, B.w, B.dinv,
 		                          A->nrows, A->nnz, A->plan_kind, A->plan_lanes,
 		                          (int)sizeof(T), K};
 		auto it = cache.m.find(key);

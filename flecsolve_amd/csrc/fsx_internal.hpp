@@ -126,6 +126,13 @@ struct fsx_vec {
 
 struct csr_index {
 	std::unique_ptr<fsx::dbuf> rowptr, colind;
+	// Sliced-row layout (built for short-row matrices): rows grouped in
+	// slices of 64 (one wavefront), each slice padded to its longest row and
+	// stored entry-major, so lane l of a wavefront reads entry j of row
+	// 64*s + l at sl_ptr[s] + 64*j + l.  Entries keep their CSR order within
+	// each row; padding carries column -1.  Shared by all value copies.
+	std::unique_ptr<fsx::dbuf> sl_ptr, sl_col;
+	int64_t sl_nnz = 0;
 	// Lazily built transpose structure for A^T x (adjoint).
 	std::unique_ptr<fsx::dbuf> t_rowptr, t_colind, t_perm;
 };
@@ -137,9 +144,10 @@ struct fsx_csr {
 	fsx_dtype vt = FSX_F64;
 	std::shared_ptr<csr_index> idx;
 	std::unique_ptr<fsx::dbuf> vals;
-	// SpMV row mapping
-	int plan_kind = 0;   // 0 LDS-staged row blocks, 1 vector per row
-	int plan_lanes = 8;
+	std::unique_ptr<fsx::dbuf> sl_vals; // values in the sliced-row layout
+	// SpMV row mapping: 0 LDS-staged row blocks, 1 vector per row, 2 sliced rows
+	int plan_kind = 0;
+	int plan_lanes = 0;
 	int64_t max_row = 0;
 	double avg_row = 0;
 	const void * rowptr() const { return idx->rowptr->p; }
@@ -171,8 +179,10 @@ void launch_copy(fsx_ctx * ctx, const fsx_vec * s, fsx_vec * d);
 void launch_set(fsx_ctx * ctx, double a, fsx_vec * x);
 void launch_convert(fsx_ctx * ctx, const fsx_vec * s, fsx_vec * d, double scale);
 
-// SpMV row-mapping selection (fsx_spmv.hip)
-void choose_plan(fsx_csr * A, const void * host_rowptr);
+// SpMV row-mapping selection (fsx_spmv.hip); may build the sliced layout
+void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr);
+// Build (or complete) the sliced-row layout of A; host_rowptr may be null.
+void ensure_sliced(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr);
 
 // Make ctx->ws.partials hold at least `count` doubles.  Reallocation drops
 // captured graphs (they hold the old address).
@@ -185,16 +195,6 @@ double fetch_scalar(fsx_ctx * ctx, const double * dslot);
 
 // grid size for a reduction over n elements (fixed function of n)
 int red_grid(size_t n);
-
-// Block remap so that each XCD (workgroups are dealt round-robin over the
-// 8 XCDs of an MI355X) walks a contiguous range of logical blocks.
-__device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblk) {
-	constexpr unsigned X = 8;
-	unsigned xcd = bid % X, idx = bid / X;
-	unsigned per = nblk / X, rem = nblk % X;
-	return xcd < rem ? xcd * (per + 1) + idx
-	                 : rem * (per + 1) + (xcd - rem) * per + idx;
-}
 
 template<class T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -242,23 +242,40 @@ __device__ __forceinline__ void store_block_partials(const double (&d)[NV],
 
 constexpr int kFinThreads = 1024;
 
-// Sum of nblk*NV partials by one block of kFinThreads threads; thread t sums
-// blocks t, t + kFinThreads, ... (coalesced, fixed order), then a wave tree
-// and the waves in order.  Result valid in every thread.
+// Sum of nblk*NV partials by one block of kFinThreads threads.  Thread t
+// owns blocks t, t + kFinThreads, ...; it keeps kFinUnroll independent
+// accumulators so that many loads are in flight (the reads are latency
+// bound), folds them in a fixed order, then a wave tree and the waves in
+// order.  The order depends only on nblk: deterministic.  Result valid in
+// every thread.
+constexpr int kFinUnroll = 8;
+
 template<int NV>
 __device__ __forceinline__ void sum_partials(const double * partials,
                                              unsigned nblk, double (&tot)[NV]) {
 	__shared__ double red[NV][kFinThreads / 64];
-	double acc[NV];
+	double acc[NV][kFinUnroll];
 #pragma unroll
-	for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-	for (unsigned b = threadIdx.x; b < nblk; b += kFinThreads) {
+	for (int i = 0; i < NV; ++i)
 #pragma unroll
-		for (int i = 0; i < NV; ++i) acc[i] += partials[(size_t)b * NV + i];
+		for (int u = 0; u < kFinUnroll; ++u) acc[i][u] = 0.0;
+	unsigned b = threadIdx.x;
+	for (; b + (kFinUnroll - 1) * kFinThreads < nblk; b += kFinUnroll * kFinThreads) {
+#pragma unroll
+		for (int u = 0; u < kFinUnroll; ++u)
+#pragma unroll
+			for (int i = 0; i < NV; ++i)
+				acc[i][u] += partials[(size_t)(b + u * kFinThreads) * NV + i];
 	}
+	for (; b < nblk; b += kFinThreads)
+#pragma unroll
+		for (int i = 0; i < NV; ++i) acc[i][0] += partials[(size_t)b * NV + i];
 #pragma unroll
 	for (int i = 0; i < NV; ++i) {
-		double s = wave_sum(acc[i]);
+		double a = 0.0;
+#pragma unroll
+		for (int u = 0; u < kFinUnroll; ++u) a += acc[i][u];
+		double s = wave_sum(a);
 		if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = s;
 	}
 	__syncthreads();

@@ -58,9 +58,117 @@ struct epi_dot {
 	}
 };
 
-// ---- plan selection ------------------------------------------------------
+// ---- plan selection and the sliced-row layout ---------------------------
 
-void choose_plan(fsx_csr * A, const void * host_rowptr) {
+// Scatter CSR rows into the sliced layout (one thread per slot row, padding
+// rows of the last slice included).  With cols == nullptr only values.
+template<class I, class V>
+__global__ void sliced_build_kernel(int64_t nrows, int64_t nslots,
+                                    const I * __restrict__ rp,
+                                    const I * __restrict__ ci,
+                                    const V * __restrict__ v,
+                                    const I * __restrict__ sl_ptr,
+                                    I * __restrict__ sl_col,
+                                    V * __restrict__ sl_val) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nslots) return;
+	const int64_t s = r / kSliceRows;
+	const int lane = (int)(r % kSliceRows);
+	const I base = sl_ptr[s];
+	const int w = (int)((sl_ptr[s + 1] - base) / kSliceRows);
+	int64_t lo = 0, len = 0;
+	if (r < nrows) {
+		lo = rp[r];
+		len = rp[r + 1] - lo;
+	}
+	for (int j = 0; j < w; ++j) {
+		const int64_t e = (int64_t)base + (int64_t)j * kSliceRows + lane;
+		if (j < len) {
+			if (sl_col) sl_col[e] = ci[lo + j];
+			sl_val[e] = v[lo + j];
+		}
+		else {
+			if (sl_col) sl_col[e] = (I)-1;
+			sl_val[e] = V(0);
+		}
+	}
+}
+
+// Slice offsets (64-bit) from a host row-pointer array of index type I.
+template<class I>
+static std::vector<int64_t> slice_offsets(int64_t nrows, const I * hrp) {
+	const int64_t ns = (nrows + kSliceRows - 1) / kSliceRows;
+	std::vector<int64_t> sp(ns + 1);
+	int64_t tot = 0;
+	for (int64_t s = 0; s < ns; ++s) {
+		sp[s] = tot;
+		int64_t w = 0;
+		const int64_t hi = std::min<int64_t>(nrows, (s + 1) * kSliceRows);
+		for (int64_t r = s * kSliceRows; r < hi; ++r)
+			w = std::max<int64_t>(w, (int64_t)hrp[r + 1] - hrp[r]);
+		tot += w * kSliceRows;
+	}
+	sp[ns] = tot;
+	return sp;
+}
+
+template<class I>
+static void build_sliced_t(fsx_ctx * ctx, fsx_csr * A, const void * host_rp) {
+	auto & X = *A->idx;
+	const int64_t ns = (A->nrows + kSliceRows - 1) / kSliceRows;
+	const int64_t nslots = ns * kSliceRows;
+	const bool need_index = !X.sl_ptr;
+	if (need_index) {
+		std::vector<I> hrp;
+		const I * rp = (const I *)host_rp;
+		if (!rp) {
+			hrp.resize(A->nrows + 1);
+			FSX_HIP(hipMemcpyAsync(hrp.data(), A->rowptr(), hrp.size() * sizeof(I),
+			                       hipMemcpyDeviceToHost, ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			rp = hrp.data();
+		}
+		std::vector<int64_t> sp64 = slice_offsets<I>(A->nrows, rp);
+		FSX_REQUIRE(sizeof(I) == 8 || sp64[ns] < ((int64_t)1 << 31),
+		            "sliced layout not addressable with int32 indices");
+		std::vector<I> sp(sp64.begin(), sp64.end());
+		X.sl_nnz = sp64[ns];
+		X.sl_ptr = std::make_unique<dbuf>(sp.size() * sizeof(I));
+		X.sl_col = std::make_unique<dbuf>(std::max<int64_t>(1, X.sl_nnz) * sizeof(I));
+		FSX_HIP(hipMemcpyAsync(X.sl_ptr->p, sp.data(), sp.size() * sizeof(I),
+		                       hipMemcpyHostToDevice, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+	}
+	const size_t vsz = A->vt == FSX_F64 ? 8 : 4;
+	A->sl_vals = std::make_unique<dbuf>(std::max<int64_t>(1, X.sl_nnz) * vsz);
+	if (nslots == 0) return;
+	const unsigned g = (unsigned)((nslots + 255) / 256);
+	I * col = need_index ? (I *)X.sl_col->p : nullptr;
+	if (A->vt == FSX_F64)
+		hipLaunchKernelGGL((sliced_build_kernel<I, double>), dim3(g), dim3(256), 0,
+		                   ctx->stream, A->nrows, nslots, (const I *)A->rowptr(),
+		                   (const I *)A->colind(), (const double *)A->vals->p,
+		                   (const I *)X.sl_ptr->p, col, (double *)A->sl_vals->p);
+	else
+		hipLaunchKernelGGL((sliced_build_kernel<I, float>), dim3(g), dim3(256), 0,
+		                   ctx->stream, A->nrows, nslots, (const I *)A->rowptr(),
+		                   (const I *)A->colind(), (const float *)A->vals->p,
+		                   (const I *)X.sl_ptr->p, col, (float *)A->sl_vals->p);
+	FSX_HIP(hipGetLastError());
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void ensure_sliced(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr) {
+	if (A->idx->sl_ptr && A->sl_vals) return;
+	if (A->it == FSX_I32) build_sliced_t<int32_t>(ctx, A, host_rowptr);
+	else build_sliced_t<int64_t>(ctx, A, host_rowptr);
+}
+
+// Short rows of similar length -> sliced rows (padding <= 10%, addressable
+// with the index type); other short-row matrices -> LDS-staged row blocks;
+// long rows -> LANES lanes per row with LANES the next power of two of the
+// mean row length (2..64).
+void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr) {
 	int64_t maxr = 0;
 	for (int64_t i = 0; i < A->nrows; ++i) {
 		int64_t a, b;
@@ -78,6 +186,17 @@ void choose_plan(fsx_csr * A, const void * host_rowptr) {
 	A->avg_row = A->nrows ? (double)A->nnz / (double)A->nrows : 0.0;
 	const double per_row_budget = (double)kStreamCap / kStreamRows;
 	if (A->avg_row <= per_row_budget) {
+		const int64_t padded =
+			A->it == FSX_I32
+				? slice_offsets<int32_t>(A->nrows, (const int32_t *)host_rowptr).back()
+				: slice_offsets<int64_t>(A->nrows, (const int64_t *)host_rowptr).back();
+		const bool fits = A->it == FSX_I64 || padded < ((int64_t)1 << 31);
+		if (A->nrows > 0 && fits && padded <= A->nnz + A->nnz / 10 + kSliceRows) {
+			A->plan_kind = 2;
+			A->plan_lanes = 0;
+			ensure_sliced(ctx, A, host_rowptr);
+			return;
+		}
 		A->plan_kind = 0;
 		A->plan_lanes = 0;
 	}
@@ -285,8 +404,8 @@ int fsx_csr_create(fsx_ctx * ctx, int64_t nrows, int64_t ncols, int64_t nnz,
 			FSX_HIP(hipMemcpyAsync(A->vals->p, vals, nnz * vsz,
 			                       hipMemcpyHostToDevice, ctx->stream));
 		}
-		choose_plan(A.get(), rowptr);
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		choose_plan(ctx, A.get(), rowptr);
 		ctx_retain(ctx);
 		*out = A.release();
 	});
@@ -326,6 +445,26 @@ int fsx_csr_convert(fsx_ctx * ctx, const fsx_csr * A, fsx_dtype vt,
 				                       hipMemcpyDeviceToDevice, ctx->stream));
 			FSX_HIP(hipGetLastError());
 		}
+		if (A->sl_vals) {
+			// same sliced layout, values converted (padding stays zero)
+			const size_t m = (size_t)std::max<int64_t>(1, A->idx->sl_nnz);
+			B->sl_vals = std::make_unique<dbuf>(m * vsz);
+			const unsigned gs = (unsigned)((m + 255) / 256);
+			if (A->vt == FSX_F64 && vt == FSX_F32)
+				hipLaunchKernelGGL((cast_kernel<double, float>), dim3(gs), dim3(256), 0,
+				                   ctx->stream, m, (const double *)A->sl_vals->p,
+				                   (float *)B->sl_vals->p);
+			else if (A->vt == FSX_F32 && vt == FSX_F64)
+				hipLaunchKernelGGL((cast_kernel<float, double>), dim3(gs), dim3(256), 0,
+				                   ctx->stream, m, (const float *)A->sl_vals->p,
+				                   (double *)B->sl_vals->p);
+			else
+				FSX_HIP(hipMemcpyAsync(B->sl_vals->p, A->sl_vals->p, m * vsz,
+				                       hipMemcpyDeviceToDevice, ctx->stream));
+			FSX_HIP(hipGetLastError());
+		}
+		else if (B->plan_kind == 2)
+			B->plan_kind = 0;
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
 		ctx_retain(ctx);
 		*out = B.release();
@@ -365,11 +504,15 @@ int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes) {
 int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes) {
 	return guard([&] {
 		FSX_REQUIRE(A, "null matrix");
-		FSX_REQUIRE(kind == 0 || kind == 1, "plan kind must be 0 or 1");
+		FSX_REQUIRE(kind == 0 || kind == 1 || kind == 2, "plan kind must be 0, 1 or 2");
 		if (kind == 1)
 			FSX_REQUIRE(lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 ||
 			                lanes == 32 || lanes == 64,
 			            "lanes must be a power of two in [2, 64]");
+		if (kind == 2) {
+			FSX_REQUIRE(A->nrows > 0, "sliced rows need a non-empty matrix");
+			ensure_sliced(A->ctx, A, nullptr);
+		}
 		A->plan_kind = kind;
 		A->plan_lanes = kind == 1 ? lanes : 0;
 	});

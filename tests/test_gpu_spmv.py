@@ -55,7 +55,7 @@ def test_structure_round_trip(ctx):
     np.testing.assert_array_equal(r2, rp)
     np.testing.assert_array_equal(c2, ci)
     np.testing.assert_array_equal(v2, v)
-    assert A.plan() == (0, 0)
+    assert A.plan() == (2, 0)  # sliced rows for the short, uniform rows
 
 
 @pytest.mark.parametrize("n", [1, 7, 64])
@@ -76,6 +76,35 @@ def test_convdiff_bit_exact(ctx):
     check_spmv(ctx, rp, ci, v, exact=True)
 
 
+@pytest.mark.parametrize("n", [3, 33, 70])
+def test_every_plan_on_stencil(ctx, n):
+    # plans 0 (LDS-staged) and 2 (sliced) sum rows serially: bit-exact;
+    # plan 1 (lanes per row) sums in a tree: within rounding
+    k = P.lognormal_field(n, 1.0, 8.0, 11)
+    rp, ci, v = P.diffusion3d(n, k)
+    A = ctx.csr(rp, ci, v)
+    x = np.random.default_rng(3).standard_normal(n ** 3)
+    ref = O.Csr(rp, ci, v).spmv(x)
+    xd, yd = ctx.vec(x), ctx.zeros(n ** 3)
+    for plan in [(0, 0), (2, 0), (1, 2), (1, 8), (1, 64)]:
+        A.set_plan(*plan)
+        A.spmv(xd, yd)
+        if plan[0] != 1:
+            np.testing.assert_array_equal(yd.get(), ref, err_msg=str(plan))
+        else:
+            assert rel(yd.get(), ref) <= 1e-13, plan
+    # fused residual and SpMV+dot on the sliced layout
+    A.set_plan(2)
+    b = np.random.default_rng(4).standard_normal(n ** 3)
+    r = ctx.zeros(n ** 3)
+    bd = ctx.vec(b)
+    A.residual(bd, xd, r)
+    np.testing.assert_array_equal(r.get(), b - ref)
+    pw = fsx.spmv_dot(A, xd, yd)
+    np.testing.assert_array_equal(yd.get(), ref)
+    assert abs(pw - O.dot(ref, x)) <= 1e-12 * np.abs(ref * x).sum()
+
+
 @pytest.mark.parametrize("itype", [np.int32, np.int64])
 def test_ragged_rows_both_mappings(ctx, itype):
     rp, ci, v = ragged(20_000, 1, itype=itype)
@@ -87,6 +116,10 @@ def test_ragged_rows_both_mappings(ctx, itype):
     A.set_plan(0)
     A.spmv(xd, yd)
     assert rel(yd.get(), ref) <= 1e-13
+    # sliced rows (heavily padded here, but exact: rows are summed serially)
+    A.set_plan(2)
+    A.spmv(xd, yd)
+    np.testing.assert_array_equal(yd.get(), ref)
     for lanes in (2, 4, 8, 16, 32, 64):
         A.set_plan(1, lanes)
         A.spmv(xd, yd)
