@@ -36,20 +36,26 @@ struct cg_state {
 	int iter, done, status, iters, maxiter, ncurv;
 };
 
-// ---- K1: w = A p and the partials of p.w; F1: alpha ---------------------
+// ---- K1: w = A p and the partials of w.p; F1: alpha ----------------------
+//
+// The row's own p_i comes from the SpMV's gather of the diagonal entry when
+// the row has one (no second read of p).  Measured alternative, not used:
+// folding K3's p = beta p + z into this gather (two gathered streams) was
+// slower than the separate pass on the 256^3 problem.
 
 template<class ACC, class T>
 struct epi_cg_spmv {
 	static constexpr int NV = 1;
+	static constexpr bool reduces = true;
 	T * w;
-	const T * p;
 	double * partials;
 	const cg_state * st;
 	__device__ bool skip() const { return st->done != 0; }
-	__device__ void row(int64_t i, ACC s, double (&d)[NV]) const {
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI && pi) const {
 		const T wi = (T)s;
 		w[i] = wi;
-		d[0] = (double)wi * (double)p[i];
+		d[0] += (double)wi * (double)pi(); // w.dot(p)
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk,
 	                          unsigned) const {
@@ -141,13 +147,14 @@ struct fin_beta {
 			st->done = 1;
 			return;
 		}
+		// P.apply(r, z) happened in K2; rho and beta (cg.hh:124-129)
 		st->rho0 = st->rho1;
 		st->rho1 = JAC ? t[1] : t[0];
 		st->beta = st->rho1 / st->rho0;
 	}
 };
 
-// ---- K3: p = beta p + z ---------------------------------------------------
+// ---- K3: p = beta p + z  (cg.hh:130) -------------------------------------
 
 template<class T>
 __global__ __launch_bounds__(256) void cg_pupdate_kernel(
@@ -166,14 +173,16 @@ __global__ __launch_bounds__(256) void cg_pupdate_kernel(
 template<class ACC, class T>
 struct epi_res_norm {
 	static constexpr int NV = 1;
+	static constexpr bool reduces = true;
 	T * r;
 	const T * b;
 	double * partials;
 	__device__ bool skip() const { return false; }
-	__device__ void row(int64_t i, ACC s, double (&d)[NV]) const {
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI &&) const {
 		const T ri = b[i] - (T)s;
 		r[i] = ri;
-		d[0] = (double)ri * (double)ri;
+		d[0] += (double)ri * (double)ri;
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk,
 	                          unsigned) const {
@@ -211,7 +220,8 @@ struct cg_graph_cache {
 	};
 	using key_t = std::tuple<const void *, const void *, const void *, const void *,
 	                         const void *, const void *, const void *, const void *,
-	                         const void *, int64_t, int64_t, int, int, int, int>;
+	                         const void *, const void *, int64_t, int64_t, int, int,
+	                         int, int>;
 	std::map<key_t, entry> m;
 	std::map<std::tuple<size_t, int, int>, fsx_vec *> pool;
 	fsx::dbuf * statebuf = nullptr;
@@ -245,21 +255,24 @@ struct cg_bufs {
 	void * x;
 	void * r;
 	void * z; // == r for identity
-	void * p;
+	void * p;  // search direction, buffer 0
+	void * p2; // search direction, buffer 1
 	void * w;
 	const void * dinv;
 	size_t n;
 };
 
+// One CG iteration (K1, F1, K2, F2, K3).
 template<class T, class V, class I>
 static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
-                              double * hist) {
+                              double * hist, int) {
 	hipStream_t s = ctx->stream;
 	const fsx_csr * A = B.A;
 	double * part = ctx->ws.partials;
+	T * p = (T *)B.p;
 	// K1 + F1
-	epi_cg_spmv<double, T> e1{(T *)B.w, (const T *)B.p, part, st};
-	spmv_launch_typed<decltype(e1), double, V, I, T>(s, A, (const T *)B.p, e1);
+	epi_cg_spmv<double, T> e1{(T *)B.w, part, st};
+	spmv_launch_typed<decltype(e1), double, V, I, T>(s, A, (const T *)p, e1);
 	hipLaunchKernelGGL((finalize_kernel<1, fin_alpha>), dim3(1), dim3(kFinThreads),
 	                   0, s, (const double *)part, spmv_blocks(A), fin_alpha{st});
 	// K2 + F2
@@ -267,7 +280,7 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	const unsigned g2 = (unsigned)((B.n + per - 1) / per);
 	if (B.dinv) {
 		hipLaunchKernelGGL((cg_update_kernel<T, true>), dim3(g2), dim3(kUpdThreads),
-		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)B.p,
+		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)p,
 		                   (const T *)B.w, (const T *)B.dinv, (T *)B.z, part,
 		                   (const cg_state *)st);
 		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<true>>), dim3(1),
@@ -276,7 +289,7 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	}
 	else {
 		hipLaunchKernelGGL((cg_update_kernel<T, false>), dim3(g2), dim3(kUpdThreads),
-		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)B.p,
+		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)p,
 		                   (const T *)B.w, (const T *)nullptr, (T *)nullptr, part,
 		                   (const cg_state *)st);
 		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<false>>), dim3(1),
@@ -285,8 +298,8 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	}
 	// K3
 	const unsigned g3 = (unsigned)((B.n + 255) / 256);
-	hipLaunchKernelGGL((cg_pupdate_kernel<T>), dim3(g3), dim3(256), 0, s, B.n,
-	                   (T *)B.p, (const T *)B.z, (const cg_state *)st);
+	hipLaunchKernelGGL((cg_pupdate_kernel<T>), dim3(g3), dim3(256), 0, s, B.n, p,
+	                   (const T *)B.z, (const cg_state *)st);
 	FSX_HIP(hipGetLastError());
 }
 
@@ -367,7 +380,7 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 	if (use_graphs()) {
 		const fsx_csr * A = B.A;
 		cg_graph_cache::key_t key{A->rowptr(), A->sl_vals ? A->sl_vals->p : nullptr,
-		                          A->vals->p, B.x, B.r, B.z, B.p, B.w, B.dinv,
+		                          A->vals->p, B.x, B.r, B.z, B.p, B.p2, B.w, B.dinv,
 		                          A->nrows, A->nnz, A->plan_kind, A->plan_lanes,
 		                          (int)sizeof(T), K};
 		auto it = cache.m.find(key);
@@ -377,7 +390,7 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 			FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
 			try {
 				for (int k = 0; k < K; ++k)
-					enqueue_iteration<T, V, I>(ctx, B, st, hist);
+					enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
 			}
 			catch (...) {
 				hipGraph_t junk = nullptr;
@@ -401,7 +414,7 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 	auto launch_chunk = [&](int slot) {
 		if (ge) FSX_HIP(hipGraphLaunch(ge, ctx->stream));
 		else
-			for (int k = 0; k < K; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist);
+			for (int k = 0; k < K; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
 		ctx->launches += 5 * K;
 		FSX_HIP(hipMemcpyAsync(&flags[slot], st, sizeof(cg_state),
 		                       hipMemcpyDeviceToHost, ctx->stream));
@@ -440,7 +453,8 @@ template<class T>
 static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
                           const fsx_vec * b, fsx_vec * x, const fsx_settings & s,
                           fsx_info * info, fsx_vec * r, fsx_vec * z,
-                          fsx_vec * p, fsx_vec * w, int * hist_count) {
+                          fsx_vec * p, fsx_vec * p2, fsx_vec * w,
+                          int * hist_count) {
 	std::memset(info, 0, sizeof(*info));
 	info->status = FSX_UNKNOWN;
 	if (hist_count) *hist_count = 0;
@@ -482,7 +496,7 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 	launch_copy(ctx, zz, p);
 	if (s.maxiter > 0) {
 		double * hist = history_buffer(ctx, s.maxiter);
-		cg_bufs B{A, x->ptr(), r->ptr(), zz->ptr(), p->ptr(), w->ptr(),
+		cg_bufs B{A, x->ptr(), r->ptr(), zz->ptr(), p->ptr(), p2->ptr(), w->ptr(),
 		          dinv ? dinv->ptr() : nullptr, n};
 		by_matrix_type(A, [&](auto * vtag, auto * itag) {
 			using V = std::remove_pointer_t<decltype(vtag)>;
@@ -523,12 +537,13 @@ int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 		fsx_vec * r = pooled(ctx, n, b->dt, 0);
 		fsx_vec * z = dinv ? pooled(ctx, n, b->dt, 1) : nullptr;
 		fsx_vec * p = pooled(ctx, n, b->dt, 2);
+		fsx_vec * p2 = pooled(ctx, n, b->dt, 4);
 		fsx_vec * w = pooled(ctx, n, b->dt, 3);
 		int cnt = 0;
 		if (b->dt == FSX_F64)
-			cg_solve_impl<double>(ctx, A, dinv, b, x, *s, info, r, z, p, w, &cnt);
+			cg_solve_impl<double>(ctx, A, dinv, b, x, *s, info, r, z, p, p2, w, &cnt);
 		else
-			cg_solve_impl<float>(ctx, A, dinv, b, x, *s, info, r, z, p, w, &cnt);
+			cg_solve_impl<float>(ctx, A, dinv, b, x, *s, info, r, z, p, p2, w, &cnt);
 		ctx->last_cg_hist.assign(cnt, 0.0);
 		if (cnt > 0)
 			FSX_HIP(hipMemcpy(ctx->last_cg_hist.data(), ctx->cg_cache->histbuf->p,
@@ -572,6 +587,7 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 		fsx_vec * ir = pooled(ctx, n, FSX_F32, 14);
 		fsx_vec * iz = jacobi ? pooled(ctx, n, FSX_F32, 15) : nullptr;
 		fsx_vec * ip = pooled(ctx, n, FSX_F32, 16);
+		fsx_vec * ip2 = pooled(ctx, n, FSX_F32, 18);
 		fsx_vec * iw = pooled(ctx, n, FSX_F32, 17);
 		if (jacobi) {
 			int rc = fsx_csr_diag_inv(ctx, A64, dinv32);
@@ -623,8 +639,8 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 			FSX_HIP(hipGetLastError());
 			fsx_info ii;
 			int cnt = 0;
-			cg_solve_impl<float>(ctx, A32, dinv32, r32, d32, is, &ii, ir, iz, ip, iw,
-			                     &cnt);
+			cg_solve_impl<float>(ctx, A32, dinv32, r32, d32, is, &ii, ir, iz, ip, ip2,
+			                     iw, &cnt);
 			info->inner_iters_total += cnt;
 			// x += |r| d  (FP64)
 			hipLaunchKernelGGL(ir_update_kernel, dim3(g), dim3(256), 0, ctx->stream,

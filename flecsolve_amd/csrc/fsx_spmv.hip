@@ -18,9 +18,11 @@ namespace fsx {
 template<class ACC, class Y>
 struct epi_store {
 	static constexpr int NV = 1;
+	static constexpr bool reduces = false;
 	Y * y;
 	__device__ bool skip() const { return false; }
-	__device__ void row(int64_t i, ACC s, double (&)[NV]) const { y[i] = (Y)s; }
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&)[NV], XI &&) const { y[i] = (Y)s; }
 	__device__ void block_end(const double (&)[NV], unsigned, unsigned) const {}
 };
 
@@ -29,10 +31,12 @@ struct epi_store {
 template<class ACC, class Y>
 struct epi_residual {
 	static constexpr int NV = 1;
+	static constexpr bool reduces = false;
 	Y * r;
 	const Y * b;
 	__device__ bool skip() const { return false; }
-	__device__ void row(int64_t i, ACC s, double (&)[NV]) const {
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&)[NV], XI &&) const {
 		r[i] = b[i] - (Y)s;
 	}
 	__device__ void block_end(const double (&)[NV], unsigned, unsigned) const {}
@@ -43,14 +47,16 @@ struct epi_residual {
 template<class ACC, class Y>
 struct epi_dot {
 	static constexpr int NV = 1;
+	static constexpr bool reduces = true;
 	Y * w;
 	const Y * p;
 	double * partials;
 	__device__ bool skip() const { return false; }
-	__device__ void row(int64_t i, ACC s, double (&d)[NV]) const {
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI && pi) const {
 		const Y wi = (Y)s;
 		w[i] = wi;
-		d[0] = (double)wi * (double)p[i];
+		d[0] += (double)wi * (double)pi();
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk,
 	                          unsigned) const {

@@ -37,7 +37,8 @@
 // that default order is faster than remapping blocks to give each XCD a
 // contiguous row range (all XCDs then stream neighbouring rows at once).
 //
-// The kernels take an epilogue policy `Epi` that consumes each row's sum:
+// The kernels take an epilogue policy `Epi` that consumes each row's sum
+// (row() *adds* its dot-product terms to d: a thread may own several rows):
 // plain store, residual b - Ax, or store plus fused dot-product terms whose
 // per-block partials a one-block finalize kernel reduces next (device-side
 // scalar updates of the fused CG happen there).  Epi::skip() lets captured
@@ -45,6 +46,8 @@
 #pragma once
 
 #include "fsx_internal.hpp"
+
+#include <algorithm>
 
 namespace fsx {
 
@@ -57,14 +60,23 @@ __device__ __forceinline__ T ldnt(const T * p) {
 	return __builtin_nontemporal_load(p);
 }
 
+// Gather policies: the value of the multiplied vector at column c.
+template<class X>
+struct gather_x {
+	using value_type = X;
+	const X * x;
+	__device__ __forceinline__ void init() {}
+	__device__ __forceinline__ X operator()(int64_t c) const { return x[c]; }
+};
+
 // Sum of the thread's row for the LDS-staged mapping.  Valid when
 // threadIdx.x < nr.  `srow` holds nr+1 row pointers, `prod` kStreamCap slots.
-template<class ACC, class V, class I, class X>
+template<class ACC, class V, class I, class G>
 __device__ __forceinline__ ACC stream_rows(int64_t r0, int nr,
                                            const I * __restrict__ rowptr,
                                            const I * __restrict__ colind,
                                            const V * __restrict__ vals,
-                                           const X * __restrict__ x, I * srow,
+                                           const G & x, I * srow,
                                            ACC * prod) {
 	const int tid = threadIdx.x;
 	for (int t = tid; t <= nr; t += kSpmvThreads) srow[t] = rowptr[r0 + t];
@@ -87,7 +99,7 @@ __device__ __forceinline__ ACC stream_rows(int64_t r0, int nr,
 #pragma unroll
 		for (int j = 0; j < NJ; ++j) {
 			const int k = tid + j * kSpmvThreads;
-			if (k < cnt) prod[k] = (ACC)vv[j] * (ACC)x[cols[j]];
+			if (k < cnt) prod[k] = (ACC)vv[j] * (ACC)x(cols[j]);
 		}
 		__syncthreads();
 		if (tid < nr) {
@@ -104,7 +116,7 @@ __device__ __forceinline__ ACC stream_rows(int64_t r0, int nr,
 			const I lo = srow[rr], hi = srow[rr + 1];
 			ACC s = ACC(0);
 			for (I e = lo + lane; e < hi; e += 64)
-				s += (ACC)ldnt(vals + e) * (ACC)x[ldnt(colind + e)];
+				s += (ACC)ldnt(vals + e) * (ACC)x(ldnt(colind + e));
 			s = wave_sum(s);
 			if (lane == 0) prod[rr] = s;
 		}
@@ -115,18 +127,18 @@ __device__ __forceinline__ ACC stream_rows(int64_t r0, int nr,
 }
 
 // LANES lanes per row; returns the row sum in every lane of the group.
-template<int LANES, class ACC, class V, class I, class X>
+template<int LANES, class ACC, class V, class I, class G>
 __device__ __forceinline__ ACC vector_row(int64_t row, int64_t nrows,
                                           const I * __restrict__ rowptr,
                                           const I * __restrict__ colind,
                                           const V * __restrict__ vals,
-                                          const X * __restrict__ x) {
+                                          const G & x) {
 	const int sub = threadIdx.x & (LANES - 1);
 	ACC s = ACC(0);
 	if (row < nrows) {
 		const I lo = rowptr[row], hi = rowptr[row + 1];
 		for (I e = lo + sub; e < hi; e += LANES)
-			s += (ACC)ldnt(vals + e) * (ACC)x[ldnt(colind + e)];
+			s += (ACC)ldnt(vals + e) * (ACC)x(ldnt(colind + e));
 	}
 #pragma unroll
 	for (int o = LANES / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LANES);
@@ -135,11 +147,12 @@ __device__ __forceinline__ ACC vector_row(int64_t row, int64_t nrows,
 
 // ---- kernels -------------------------------------------------------------
 
-template<class Epi, class ACC, class V, class I, class X>
+template<class Epi, class ACC, class V, class I, class G>
 __global__ __launch_bounds__(kSpmvThreads) void spmv_stream_kernel(
 	int64_t nrows, const I * __restrict__ rowptr, const I * __restrict__ colind,
-	const V * __restrict__ vals, const X * __restrict__ x, Epi epi) {
+	const V * __restrict__ vals, G x, Epi epi) {
 	if (epi.skip()) return;
+	x.init();
 	__shared__ I srow[kStreamRows + 1];
 	__shared__ ACC prod[kStreamCap];
 	const unsigned blk = blockIdx.x;
@@ -149,15 +162,19 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_stream_kernel(
 	double d[Epi::NV];
 #pragma unroll
 	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
-	if ((int)threadIdx.x < nr) epi.row(r0 + threadIdx.x, sum, d);
+	if ((int)threadIdx.x < nr) {
+		const int64_t i = r0 + threadIdx.x;
+		epi.row(i, sum, d, [&] { return x(i); });
+	}
 	epi.block_end(d, blk, gridDim.x);
 }
 
-template<int LANES, class Epi, class ACC, class V, class I, class X>
+template<int LANES, class Epi, class ACC, class V, class I, class G>
 __global__ __launch_bounds__(kSpmvThreads) void spmv_vector_kernel(
 	int64_t nrows, const I * __restrict__ rowptr, const I * __restrict__ colind,
-	const V * __restrict__ vals, const X * __restrict__ x, Epi epi) {
+	const V * __restrict__ vals, G x, Epi epi) {
 	if (epi.skip()) return;
+	x.init();
 	constexpr int RPB = kSpmvThreads / LANES;
 	const unsigned blk = blockIdx.x;
 	const int64_t row = (int64_t)blk * RPB + threadIdx.x / LANES;
@@ -165,30 +182,41 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_vector_kernel(
 	double d[Epi::NV];
 #pragma unroll
 	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
-	if ((threadIdx.x & (LANES - 1)) == 0 && row < nrows) epi.row(row, sum, d);
+	if ((threadIdx.x & (LANES - 1)) == 0 && row < nrows)
+		epi.row(row, sum, d, [&] { return x(row); });
 	epi.block_end(d, blk, gridDim.x);
 }
 
 constexpr int kSliceRows = 64;   // slice height = wavefront width
 constexpr int kSliceUnroll = 8;  // entries loaded ahead of the gathers
 
-template<class Epi, class ACC, class V, class I, class X>
+// Grid-stride over rows: at most kSlicedMaxBlocks workgroups, so a fused
+// dot product leaves few per-block partials for its finalize kernel.
+constexpr unsigned kSlicedMaxBlocks = 8192;
+
+template<class Epi, class ACC, class V, class I, class G>
 __global__ __launch_bounds__(kSpmvThreads) void spmv_sliced_kernel(
 	int64_t nrows, const I * __restrict__ sl_ptr, const I * __restrict__ sl_col,
-	const V * __restrict__ sl_val, const X * __restrict__ x, Epi epi) {
+	const V * __restrict__ sl_val, G x, Epi epi) {
 	if (epi.skip()) return;
-	const int64_t r = (int64_t)blockIdx.x * kSpmvThreads + threadIdx.x;
+	x.init();
 	const int lane = threadIdx.x & (kSliceRows - 1);
 	double d[Epi::NV];
 #pragma unroll
 	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
-	if (r < nrows) {
+	const int64_t stride = (int64_t)gridDim.x * kSpmvThreads;
+	for (int64_t r = (int64_t)blockIdx.x * kSpmvThreads + threadIdx.x; r < nrows;
+	     r += stride) {
 		const int64_t s = r / kSliceRows;
 		const I base = sl_ptr[s];
 		const int w = (int)((sl_ptr[s + 1] - base) / kSliceRows);
 		const I * cp = sl_col + base + lane;
 		const V * vp = sl_val + base + lane;
 		ACC sum = ACC(0);
+		// x at the row's own column, captured from the diagonal entry's gather
+		// (epilogues that need it then do not read x again)
+		typename G::value_type xself{};
+		bool has_self = false;
 		for (int j0 = 0; j0 < w; j0 += kSliceUnroll) {
 			I c[kSliceUnroll];
 			V v[kSliceUnroll];
@@ -200,31 +228,45 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sliced_kernel(
 				}
 #pragma unroll
 			for (int k = 0; k < kSliceUnroll; ++k)
-				if (j0 + k < w && c[k] >= 0) sum += (ACC)v[k] * (ACC)x[c[k]];
+				if (j0 + k < w && c[k] >= 0) {
+					const auto xv = x(c[k]);
+					if ((int64_t)c[k] == r) {
+						xself = xv;
+						has_self = true;
+					}
+					sum += (ACC)v[k] * (ACC)xv;
+				}
 		}
-		epi.row(r, sum, d);
+		epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
 	}
 	epi.block_end(d, blockIdx.x, gridDim.x);
 }
 
-// Launch the selected mapping for matrix A.  X is the x element type.
-template<class Epi, class ACC, class V, class I, class X>
-void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
-                       const Epi & epi) {
+// Epilogues that leave per-block partials cap the grid (fewer partials for
+// the finalize kernel); plain products use one row per thread.
+inline unsigned sliced_blocks(int64_t nrows, bool reduces = true) {
+	const int64_t g = (nrows + kSpmvThreads - 1) / kSpmvThreads;
+	return (unsigned)(reduces ? std::min<int64_t>(g, kSlicedMaxBlocks) : g);
+}
+
+// Launch the selected mapping for matrix A with gather policy G.
+template<class Epi, class ACC, class V, class I, class G>
+void spmv_launch_gather(hipStream_t st, const fsx_csr * A, const G & x,
+                        const Epi & epi) {
 	const I * rp = (const I *)A->rowptr();
 	const I * ci = (const I *)A->colind();
 	const V * vv = (const V *)A->vals->p;
 	if (A->nrows == 0) return;
 	if (A->plan_kind == 2) {
-		unsigned g = (unsigned)((A->nrows + kSpmvThreads - 1) / kSpmvThreads);
-		hipLaunchKernelGGL((spmv_sliced_kernel<Epi, ACC, V, I, X>), dim3(g),
+		const unsigned g = sliced_blocks(A->nrows, Epi::reduces);
+		hipLaunchKernelGGL((spmv_sliced_kernel<Epi, ACC, V, I, G>), dim3(g),
 		                   dim3(kSpmvThreads), 0, st, A->nrows,
 		                   (const I *)A->idx->sl_ptr->p, (const I *)A->idx->sl_col->p,
 		                   (const V *)A->sl_vals->p, x, epi);
 	}
 	else if (A->plan_kind == 0) {
 		unsigned g = (unsigned)((A->nrows + kStreamRows - 1) / kStreamRows);
-		hipLaunchKernelGGL((spmv_stream_kernel<Epi, ACC, V, I, X>), dim3(g),
+		hipLaunchKernelGGL((spmv_stream_kernel<Epi, ACC, V, I, G>), dim3(g),
 		                   dim3(kSpmvThreads), 0, st, A->nrows, rp, ci, vv, x, epi);
 	}
 	else {
@@ -232,7 +274,7 @@ void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
 			constexpr int L = decltype(lanes_c)::value;
 			constexpr int RPB = kSpmvThreads / L;
 			unsigned g = (unsigned)((A->nrows + RPB - 1) / RPB);
-			hipLaunchKernelGGL((spmv_vector_kernel<L, Epi, ACC, V, I, X>), dim3(g),
+			hipLaunchKernelGGL((spmv_vector_kernel<L, Epi, ACC, V, I, G>), dim3(g),
 			                   dim3(kSpmvThreads), 0, st, A->nrows, rp, ci, vv, x,
 			                   epi);
 		};
@@ -248,9 +290,15 @@ void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
 	FSX_HIP(hipGetLastError());
 }
 
+// y = A x with x a plain vector of element type X.
+template<class Epi, class ACC, class V, class I, class X>
+void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
+                       const Epi & epi) {
+	spmv_launch_gather<Epi, ACC, V, I>(st, A, gather_x<X>{x}, epi);
+}
+
 inline unsigned spmv_blocks(const fsx_csr * A) {
-	if (A->plan_kind == 2)
-		return (unsigned)((A->nrows + kSpmvThreads - 1) / kSpmvThreads);
+	if (A->plan_kind == 2) return sliced_blocks(A->nrows);
 	if (A->plan_kind == 0)
 		return (unsigned)((A->nrows + kStreamRows - 1) / kStreamRows);
 	const int rpb = kSpmvThreads / A->plan_lanes;

@@ -158,3 +158,30 @@ def test_ir_reaches_1e10_and_matches_fp64(ctx, sigma):
                      P=O.prec(1, Ao.dinv()))
     # both are 1e-10-residual solutions: error bounded by cond * 1e-10
     assert rel(x, xo) < 1e-6
+
+
+def test_grid_stride_sizes_iteration_matched(ctx):
+    """More rows than one sliced-SpMV grid covers (> 8192 x 256), so each
+    thread owns several rows: SpMV+dot and 25 CG iterations against the
+    oracle, iteration-matched (both stop at maxiter)."""
+    n = 132
+    rp, ci, v, Ao = diffusion(n)
+    N = n ** 3
+    assert N > 8192 * 256
+    b = P.uniform(N, 2002)
+    A = ctx.csr(rp, ci, v)
+    assert A.plan() == (2, 0)
+    bd = ctx.vec(b)
+    w = ctx.zeros(N)
+    pw = fsx.spmv_dot(A, bd, w)
+    wref = Ao.spmv(b)
+    np.testing.assert_array_equal(w.get(), wref)
+    assert abs(pw - O.dot(wref, b)) <= 1e-12 * np.abs(wref * b).sum()
+    s = dict(maxiter=25, rtol=1e-12, use_zero_guess=True)
+    xo, io, ho = O.cg(Ao, b, np.zeros(N), O.Settings(**s), P=O.prec(1, Ao.dinv()),
+                      hist_cap=25)
+    xd = ctx.zeros(N)
+    info = fsx.cg_solve(A, bd, xd, fsx.Settings(**s), dinv=A.diag_inv())
+    assert info.reason == io.reason == "diverged_iters"
+    np.testing.assert_allclose(fsx.cg_history(ctx), ho, rtol=1e-10)
+    assert rel(xd.get(), xo) <= 1e-12
