@@ -125,6 +125,13 @@ def main():
     gbs32 = B32 / (ms32 * 1e-3) / 1e9
     log(f"spmv fp32: {ms32:.4f} ms, {gbs32:.0f} GB/s")
     del x32, y32, y
+    # HBM bytes per launch from the committed rocprofv3 PMC passes (FETCH_SIZE,
+    # WRITE_SIZE scaled by the copy-kernel calibration) for this workload
+    traffic, traffic_src = None, None
+    pmc = ROOT / "profiles" / "spmv_sliced_256cube_pmc.json"
+    if n == 256 and abs(a.sigma - 1.0) < 1e-12 and pmc.exists():
+        traffic = round(json.loads(pmc.read_text())["spmv_fp64"]["traffic_bytes"])
+        traffic_src = "profiles/spmv_sliced_256cube_pmc.md"
     roofline = {
         "bound": "hbm",
         "kernel": "spmv_sliced_kernel (FP64 values, int32 indices)",
@@ -132,7 +139,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "algorithmic_bytes": B64,
         "avg_ms": round(ms, 5),
         "fp32_values": {"achieved": round(gbs32, 1), "avg_ms": round(ms32, 5),

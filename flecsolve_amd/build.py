@@ -87,6 +87,18 @@ def build_problems() -> Path:
     return out
 
 
+def build_cpp_tests() -> Path:
+    """C++ drop-in check against the header API (tests/cpp)."""
+    src = ROOT / "tests" / "cpp" / "cg_dropin.cpp"
+    out = ROOT / "tests" / "cpp" / "cg_dropin"
+    tmp = ROOT / "tests" / "cpp" / "cg_dropin.tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(INC), str(src), "-o", str(tmp),
+          "-L", str(PKG), "-lfsx", "-lfsx_problems", f"-Wl,-rpath,{PKG}",
+          "-Wl,-rpath,$ORIGIN/../../flecsolve_amd"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
     return ROOT / "oracle" / "liboracle.so"
@@ -97,6 +109,7 @@ def build_all(oracle: bool = True) -> None:
     if oracle:
         build_oracle()
     build_fsx()
+    build_cpp_tests()
 
 
 def main() -> None:

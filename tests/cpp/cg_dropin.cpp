@@ -1,0 +1,120 @@
+// C++ drop-in check: the reference's CG test pattern
+// (flecsolve/solvers/test/cg.cc:16-91) written against the device vector
+// type and device CSR operator.  b = 0, x0 = set_random(7), and a user
+// diagnostic that tracks the A-norm of the error (the solution is 0):
+// monotone decrease and the Trefethen-Bau bound 2((sqrt k - 1)/(sqrt k + 1))^i.
+// Also runs GMRES and BiCGSTAB through the same API.  Exit status 0 = pass.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <vector>
+
+#include "flecsolve/matrices/device_csr.hh"
+#include "flecsolve/solvers/bicgstab.hh"
+#include "flecsolve/solvers/cg.hh"
+#include "flecsolve/solvers/gmres.hh"
+#include "fsx_problems.h"
+
+using namespace flecsolve;
+
+template<class Op, class Vec>
+struct diagnostic {
+	diagnostic(const Op & A, const Vec & x0, double cond)
+		: cond(cond), A(A), Ax(x0.context(), x0.data.size()) {
+		A.apply(x0, Ax);
+		e0 = std::sqrt(x0.dot(Ax).get());
+		eprev = e0;
+	}
+	template<class T>
+	bool operator()(const T & x, double) {
+		A.apply(x, Ax);
+		const double ea = std::sqrt(x.dot(Ax).get());
+		const double frac = (std::sqrt(cond) - 1) / (std::sqrt(cond) + 1);
+		const double bnd = 2 * std::pow(frac, ++iter) * e0;
+		monotonic_fail = monotonic_fail || ea > eprev;
+		convergence_fail = convergence_fail || ea >= bnd;
+		eprev = ea;
+		return false;
+	}
+	int iter = 0;
+	double cond, e0, eprev;
+	const Op & A;
+	Vec Ax;
+	bool monotonic_fail = false, convergence_fail = false;
+};
+
+int main() {
+	const int n = 48;
+	std::vector<int32_t> rp(n * n + 1), ci(fsx_gen_poisson2d_nnz(n));
+	std::vector<double> v(ci.size());
+	fsx_gen_poisson2d(n, rp.data(), ci.data(), v.data());
+	double lmin = 1e300, lmax = 0;
+	for (int i = 1; i <= n; ++i)
+		for (int j = 1; j <= n; ++j) {
+			double l = 4 - 2 * std::cos(M_PI * i / (n + 1)) - 2 * std::cos(M_PI * j / (n + 1));
+			lmin = std::min(lmin, l);
+			lmax = std::max(lmax, l);
+		}
+	fsx_ctx * ctx;
+	fsx_check(fsx_ctx_create(0, &ctx), "ctx");
+	fsx_csr * Ah;
+	fsx_check(fsx_csr_create(ctx, n * n, n * n, (int64_t)ci.size(), FSX_I32, rp.data(),
+	                         ci.data(), FSX_F64, v.data(), &Ah),
+	          "csr");
+	int fails = 0;
+	{
+		op::core<mat::device_csr> A(ctx, Ah);
+		vec::device<double> b(ctx, n * n), x(ctx, n * n);
+		b.set_scalar(0.0);
+		x.set_random(7);
+		diagnostic diag{A, x, lmax / lmin};
+		cg::settings s;
+		s.maxiter = 2000;
+		s.rtol = 1e-9;
+		s.use_zero_guess = false;
+		auto slv = cg::solver(s, vec::make_device_work<cg::nwork>(b))(
+			op::ref(A), op::I, std::ref(diag));
+		solve_info info = slv(b, x);
+		std::printf("cg: %d iterations, %s, monotone %s, bound %s\n", info.iters,
+		            info.success() ? "converged" : "not converged",
+		            diag.monotonic_fail ? "FAIL" : "ok",
+		            diag.convergence_fail ? "FAIL" : "ok");
+		fails += !info.success() || diag.monotonic_fail || diag.convergence_fail ||
+		         info.iters != diag.iter;
+
+		// GMRES(30) with the Dinv preconditioner, BiCGSTAB, from x0 = 0
+		vec::device<double> rhs(ctx, n * n), y(ctx, n * n);
+		rhs.set_random(0);
+		gmres::settings gs;
+		gs.maxiter = 400;
+		gs.rtol = 1e-10;
+		gs.use_zero_guess = true;
+		gs.max_krylov_dim = 30;
+		gs.restart = true;
+		op::core<mat::device_dinv> D(A);
+		auto gm = gmres::solver(gs, vec::make_device_work<gmres::nwork>(rhs))(
+			op::ref(A), op::ref(D));
+		solve_info gi = gm(rhs, y);
+		vec::device<double> r(ctx, n * n);
+		A.residual(rhs, y, r);
+		const double grel = r.l2norm().get() / rhs.l2norm().get();
+		std::printf("gmres(30)+dinv: %d iterations, %d restarts, true rel res %.3e\n",
+		            gi.iters, gi.restarts, grel);
+		fails += !gi.success() || grel > 1e-9;
+		bicgstab::settings bs;
+		bs.maxiter = 400;
+		bs.rtol = 1e-10;
+		bs.use_zero_guess = true;
+		auto bi = bicgstab::solver(bs, vec::make_device_work<bicgstab::nwork>(rhs))(
+			op::ref(A));
+		solve_info bsi = bi(rhs, y);
+		A.residual(rhs, y, r);
+		const double brel = r.l2norm().get() / rhs.l2norm().get();
+		std::printf("bicgstab: %d iterations, true rel res %.3e\n", bsi.iters, brel);
+		fails += !bsi.success() || brel > 1e-9;
+	}
+	fsx_csr_destroy(Ah);
+	fsx_ctx_destroy(ctx);
+	std::printf(fails ? "FAIL\n" : "PASS\n");
+	return fails ? 1 : 0;
+}
