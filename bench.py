@@ -4,11 +4,17 @@ with the SpMV kernel's fraction of the HBM roofline.
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--sigma 1.0]
                     [--no-cpu-baseline]
+    python bench.py --workload batched [--nsys 2048 --emin 8 --emax 48]
 
 A step is one complete solve from x0 = 0 to |b - A x| / |b| <= 1e-10 with the
 matrix and right-hand side already resident in HBM.  The timed steps use
 the refined solver (the method the headline targets); plain FP64 CG + Jacobi
 is timed the same way beside it.  One JSON line is printed at the end.
+
+`--workload batched`: 2048 independent lognormal-k diffusion systems with
+edge lengths uniform in 8..48, solved in one persistent launch (FP64 CG +
+Jacobi and FP32-inner/FP64-outer refinement, autotuned row mapping); a step
+is the whole batch solved to 1e-10 per system.
 """
 from __future__ import annotations
 
@@ -79,6 +85,11 @@ def main():
     ap.add_argument("--max-outer", type=int, default=10)
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the SpMV and copy kernels only (profiling runs)")
+    ap.add_argument("--workload", choices=["diffusion", "batched"], default="diffusion")
+    ap.add_argument("--nsys", type=int, default=2048)
+    ap.add_argument("--emin", type=int, default=8)
+    ap.add_argument("--emax", type=int, default=48)
+    ap.add_argument("--cpu-sample-systems", type=int, default=16)
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -89,6 +100,9 @@ def main():
 
     from flecsolve_amd import fsx
     from flecsolve_amd import problems as P
+
+    if a.workload == "batched":
+        return bench_batched(a, fsx, P)
 
     seeds = P.seeds()
     n = a.n
@@ -233,6 +247,156 @@ def main():
         out["cpu_baseline"] = cpu_baseline(rp, ci, v, b, info64.iters,
                                            a.cpu_sample_iters)
     print(json.dumps(out), flush=True)
+
+
+def batch_cg_bytes(rows_s, nnz_s, vbytes, xbytes):
+    """Algorithmic bytes of one CG + Jacobi iteration of one system: the SpMV
+    (values + column ids + row pointers, p read once, w written) and the
+    vector passes (x, r read+written; p, w, dinv read; z written; then p
+    read+written and z read)."""
+    spmv = nnz_s * (vbytes + 4) + (rows_s + 1) * 4 + rows_s * xbytes * 2
+    vec = rows_s * xbytes * (4 + 3 + 1 + 3)
+    return spmv + vec
+
+
+def bench_batched(a, fsx, P):
+    seeds = P.seeds()
+    t0 = time.perf_counter()
+    e, rows, rp, ci, v = P.batch(a.nsys, a.emin, a.emax, seeds["batched_edges"],
+                                 seeds["batched_k"], a.sigma)
+    N, nnz = int(rows[-1]), int(ci.size)
+    b = P.uniform(N, seeds["batched_rhs"])
+    sizes = np.diff(rows)
+    nnz_s = np.diff(rp[rows].astype(np.int64))
+    log(f"batch {a.nsys} systems, edges {a.emin}..{a.emax}: {N} rows, {nnz} nnz, "
+        f"generated in {time.perf_counter()-t0:.1f} s")
+    ctx = fsx.Context(0)
+    A64 = ctx.csr(rp, ci, v)
+    A32 = A64.convert(fsx.F32)
+    dinv = A64.diag_inv()
+    B = fsx.Batch(A64, rows)
+    bd, x = ctx.vec(b), ctx.zeros(N)
+    log(f"uploaded; matrix plan {A64.plan()}")
+    tune_ms = B.tune(bd, x, dinv, sample=64, iters=20)
+    cfg = B.config()
+    log(f"autotuned: workgroup {cfg[0]}, mapping {'sliced' if cfg[1] == 0 else 'lanes'}, "
+        f"lanes {cfg[2]} ({tune_ms:.2f} ms per sample run)")
+    L = fsx.lib()
+    s64 = fsx.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True)
+    inner = fsx.Settings(maxiter=a.maxiter, rtol=a.inner_rtol)
+    r = ctx.zeros(N)
+
+    def true_rel():
+        A64.residual(bd, x, r)
+        rh = r.get()
+        num = np.sqrt(np.add.reduceat(rh * rh, rows[:-1]))
+        den = np.sqrt(np.add.reduceat(b * b, rows[:-1]))
+        return num / den
+
+    def timed(fn, label):
+        ts = []
+        for i in range(a.warmup + a.steps):
+            fsx.check(L.fsx_zero(ctx.h, x.h))
+            ctx.sync()
+            t = time.perf_counter()
+            out = fn()
+            ctx.sync()
+            dt = time.perf_counter() - t
+            log(f"{label} {'warmup' if i < a.warmup else 'step'} {i}: {dt:.3f} s")
+            if i >= a.warmup:
+                ts.append(dt)
+        return out, sum(ts) / len(ts)
+
+    (it64, st64, _), t64 = timed(lambda: B.cg(bd, x, s64, dinv=dinv), "fp64 batched cg")
+    if not (st64 == 1).all():
+        sys.exit(f"{int((st64 != 1).sum())} systems did not converge (FP64)")
+    rel64 = true_rel()
+    x64 = x.get()
+    (oit, iit, stir, _), tir = timed(
+        lambda: B.ir_cg(A32, bd, x, 1e-10, inner, max_outer=a.max_outer), "ir batched cg")
+    if not (stir == 1).all():
+        sys.exit(f"{int((stir != 1).sum())} systems did not reach 1e-10 (IR)")
+    relir = true_rel()
+    xir = x.get()
+    err_ir = float(np.max(np.sqrt(np.add.reduceat((xir - x64) ** 2, rows[:-1])) /
+                          np.sqrt(np.add.reduceat(x64 * x64, rows[:-1]))))
+    bytes64 = float(np.sum(it64.astype(np.float64) * batch_cg_bytes(sizes, nnz_s, 8, 8)))
+    gbs = bytes64 / t64 / 1e9
+    log(f"fp64: {t64:.3f} s, iters {it64.min()}..{it64.max()} (sum {int(it64.sum())}), "
+        f"{gbs:.0f} GB/s algorithmic; ir: {tir:.3f} s, outer {oit.max()}, "
+        f"inner sum {int(iit.sum())}")
+    out = {
+        "metric": "batched CG time-to-1e-10, 2048 diffusion systems (edges 8..48)",
+        "value": round(tir, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(tir * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-outer",
+        "data": "synthetic (seeded per-system lognormal k, uniform rhs)",
+        "config": {
+            "workload": f"batched_diffusion_{a.nsys}x_edges{a.emin}-{a.emax}_rtol1e-10",
+            "systems": a.nsys, "rows": N, "nnz": nnz,
+            "autotuned": {"workgroup": cfg[0], "mapping": "sliced" if cfg[1] == 0 else "lanes",
+                          "lanes": cfg[2]},
+            "fp64_cg": {"seconds": round(t64, 4), "iters_min": int(it64.min()),
+                        "iters_max": int(it64.max()), "iters_sum": int(it64.sum()),
+                        "max_true_rel_residual": float(rel64.max())},
+            "ir": {"seconds": round(tir, 4), "outer_max": int(oit.max()),
+                   "inner_sum": int(iit.sum()),
+                   "max_true_rel_residual": float(relir.max()),
+                   "max_rel_err_vs_fp64": err_ir},
+            "ir_speedup_vs_fp64_cg": round(t64 / tir, 3),
+        },
+        "roofline": {"bound": "hbm", "kernel": "batched_cg_kernel<jacobi, fp64>",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": bytes64,
+                     "note": "algorithmic bytes of every system's CG iterations over "
+                             "the FP64 batch's wall time; small systems run from L2"},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batched(a, rows, rp, ci, v, b, it64, x64)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_batched(a, rows, rp, ci, v, b, it64, x64):
+    """The oracle's CG + Jacobi on a sample of the batch's systems spread over
+    the size range, each solved completely; the time is scaled to the whole
+    batch by the GPU's per-system rows x iterations."""
+    from oracle import oracle as O
+    order = np.argsort(-np.diff(rows), kind="stable")
+    m = min(a.cpu_sample_systems, order.size)
+    samp = order[(np.arange(m) * order.size) // m]
+    t_total, work_s, worst = 0.0, 0.0, 0.0
+    for s in samp:
+        lo, hi = rows[s], rows[s + 1]
+        e0, e1 = rp[lo], rp[hi]
+        A = O.Csr((rp[lo:hi + 1] - e0).astype(np.int64), (ci[e0:e1] - lo).astype(np.int64),
+                  v[e0:e1])
+        t = time.perf_counter()
+        xo, info, _ = O.cg(A, b[lo:hi], np.zeros(hi - lo),
+                           O.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True),
+                           P=O.prec(1, A.dinv()))
+        t_total += time.perf_counter() - t
+        work_s += float(hi - lo) * max(1, info.iters)
+        worst = max(worst, float(np.linalg.norm(x64[lo:hi] - xo) / np.linalg.norm(xo)))
+    work_all = float(np.sum(np.diff(rows).astype(np.float64) * np.maximum(it64, 1)))
+    log(f"cpu sample: {m} systems in {t_total:.1f} s; max rel err of GPU x vs oracle {worst:.2e}")
+    return {
+        "value": t_total * work_all / work_s,
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{m} systems spread over the size range solved completely by the "
+                   f"oracle's FP64 CG + Jacobi on 1 thread ({t_total:.1f} s), scaled to "
+                   f"the whole batch by rows x iterations; max rel error of the GPU's "
+                   f"FP64 x vs the oracle on the sample {worst:.1e}"),
+    }
 
 
 def fsx_time(fsx, name, ctx, A, x, y, reps):

@@ -234,6 +234,48 @@ int fsx_gen_convdiff3d(int n, double cx, double cy, double cz,
 	return 0;
 }
 
+int fsx_gen_batch_sizes(int nsys, int emin, int emax, uint64_t seed_e,
+                        int32_t * edges, int64_t * rows, int64_t * nnzs) {
+	if (nsys < 0 || emin < 1 || emax < emin) return 1;
+	rows[0] = 0;
+	nnzs[0] = 0;
+	for (int s = 0; s < nsys; ++s) {
+		const uint64_t span = (uint64_t)(emax - emin + 1);
+		const int e = emin + (int)(hash2(seed_e, (uint64_t)s) % span);
+		edges[s] = e;
+		rows[s + 1] = rows[s] + (int64_t)e * e * e;
+		nnzs[s + 1] = nnzs[s] + fsx_gen_diffusion3d_nnz(e);
+	}
+	return 0;
+}
+
+int fsx_gen_batch(int nsys, const int32_t * edges, const int64_t * rows,
+                  const int64_t * nnzs, double sigma, double ell,
+                  uint64_t seed_k, int32_t * rowptr, int32_t * colind,
+                  double * vals) {
+	if (nsys < 0) return 1;
+	if (nnzs[nsys] >= ((int64_t)1 << 31)) return 2;
+	int rc = 0;
+	// one system per iteration; inner generators run single-threaded here
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : rc)
+	for (int s = 0; s < nsys; ++s) {
+		const int e = edges[s];
+		const int64_t L = e + 2;
+		std::vector<double> k((size_t)(L * L * L));
+		std::vector<int32_t> rp(rows[s + 1] - rows[s] + 1);
+		int r1 = fsx_gen_grf_lognormal(e, sigma, ell, seed_k + (uint64_t)s, k.data());
+		int r2 = fsx_gen_diffusion3d(e, k.data(), 1.0, 0.0, rp.data(),
+		                             colind + nnzs[s], vals + nnzs[s]);
+		rc |= r1 | r2;
+		for (int64_t i = 0; i < (int64_t)rp.size() - 1; ++i)
+			rowptr[rows[s] + i] = (int32_t)(nnzs[s] + rp[i]);
+		for (int64_t q = nnzs[s]; q < nnzs[s + 1]; ++q)
+			colind[q] += (int32_t)rows[s];
+	}
+	rowptr[rows[nsys]] = (int32_t)nnzs[nsys];
+	return rc;
+}
+
 int fsx_gen_uniform(double * x, int64_t n, uint64_t seed, double lo,
                     double hi) {
 #pragma omp parallel for schedule(static)

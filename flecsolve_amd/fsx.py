@@ -131,6 +131,16 @@ _SIGS = {
     "fsx_krylov_bicgstab": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
                                       C.POINTER(Settings), C.POINTER(Info),
                                       C.POINTER(C.c_double), C.c_int]),
+    "fsx_batch_create": (C.c_int, [_P, _P, C.c_int64, _P, C.POINTER(_P)]),
+    "fsx_batch_destroy": (C.c_int, [_P]),
+    "fsx_batch_config": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]),
+    "fsx_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "fsx_batch_tune": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.c_int,
+                                 C.POINTER(C.c_double)]),
+    "fsx_batched_cg": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings), _P, _P, _P]),
+    "fsx_batched_ir_cg": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double, C.c_int,
+                                    C.POINTER(Settings), _P, _P, _P, _P]),
     "fsx_time_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
     "fsx_time_copy": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
 }
@@ -418,3 +428,63 @@ def krylov(kind: str, A: CSR, b: Vector, x: Vector, settings: Settings,
                 C.byref(info), hp, hist_cap)
     check(rc, f"fsx_krylov_{kind}")
     return info, hist[:min(hist_cap, max(info.iters, 0) or hist_cap)]
+
+
+class Batch:
+    """Block-diagonal batch of independent systems (fsx_batch_*)."""
+
+    def __init__(self, A: CSR, sys_rows):
+        self.ctx = A.ctx
+        self.A = A
+        self.rows = np.ascontiguousarray(sys_rows, np.int64)
+        self.nsys = self.rows.size - 1
+        self.h = C.c_void_p()
+        check(lib().fsx_batch_create(A.ctx.h, A.h, self.nsys, _ptr(self.rows),
+                                     C.byref(self.h)), "fsx_batch_create")
+
+    def config(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().fsx_batch_config(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_config(self, wg, mapping, lanes=1):
+        check(lib().fsx_batch_set_config(self.h, wg, mapping, lanes), "set_config")
+
+    def tune(self, b: Vector, x: Vector, dinv: Vector | None = None, sample=32,
+             iters=20) -> float:
+        ms = C.c_double()
+        check(lib().fsx_batch_tune(self.ctx.h, self.h, dinv.h if dinv else None,
+                                   b.h, x.h, sample, iters, C.byref(ms)), "tune")
+        return ms.value
+
+    def cg(self, b: Vector, x: Vector, settings: Settings, dinv: Vector | None = None):
+        it = np.zeros(self.nsys, np.int32)
+        st = np.zeros(self.nsys, np.int32)
+        rs = np.zeros(self.nsys, np.float64)
+        check(lib().fsx_batched_cg(self.ctx.h, self.h, dinv.h if dinv else None,
+                                   b.h, x.h, C.byref(settings), _ptr(it), _ptr(st),
+                                   _ptr(rs)), "fsx_batched_cg")
+        return it, st, rs
+
+    def ir_cg(self, A32: CSR, b: Vector, x: Vector, rtol_outer: float,
+              inner: Settings, jacobi=True, max_outer=20):
+        outer = np.zeros(self.nsys, np.int32)
+        inn = np.zeros(self.nsys, np.int32)
+        st = np.zeros(self.nsys, np.int32)
+        rs = np.zeros(self.nsys, np.float64)
+        check(lib().fsx_batched_ir_cg(self.ctx.h, self.h, A32.h, int(jacobi), b.h, x.h,
+                                      rtol_outer, max_outer, C.byref(inner),
+                                      _ptr(outer), _ptr(inn), _ptr(st), _ptr(rs)),
+              "fsx_batched_ir_cg")
+        return outer, inn, st, rs
+
+    def destroy(self):
+        if self.h:
+            check(lib().fsx_batch_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -37,6 +37,9 @@ def lib():
                                          P, P, P]
         L.fsx_gen_uniform.argtypes = [P, C.c_int64, C.c_uint64, C.c_double, C.c_double]
         L.fsx_gen_bumps.argtypes = [C.c_int, C.c_int, C.c_uint64, P]
+        L.fsx_gen_batch_sizes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, P, P, P]
+        L.fsx_gen_batch.argtypes = [C.c_int, P, P, P, C.c_double, C.c_double,
+                                    C.c_uint64, P, P, P]
         _lib = L
     return _lib
 
@@ -94,3 +97,18 @@ def bumps(n: int, nb: int, seed: int) -> np.ndarray:
     u = np.empty(n ** 3, np.float64)
     assert lib().fsx_gen_bumps(n, nb, seed, _p(u)) == 0
     return u
+
+
+def batch(nsys: int, emin: int, emax: int, seed_e: int, seed_k: int,
+          sigma: float = 1.0, ell: float = 4.0):
+    """Block-diagonal batch of 3D lognormal-k diffusion systems.
+    Returns (edges, sys_rows, rowptr, colind, vals)."""
+    edges = np.empty(nsys, np.int32)
+    rows = np.empty(nsys + 1, np.int64)
+    nnzs = np.empty(nsys + 1, np.int64)
+    assert lib().fsx_gen_batch_sizes(nsys, emin, emax, seed_e, _p(edges), _p(rows),
+                                     _p(nnzs)) == 0
+    rp, ci, v = _alloc(int(rows[-1]), int(nnzs[-1]))
+    assert lib().fsx_gen_batch(nsys, _p(edges), _p(rows), _p(nnzs), sigma, ell, seed_k,
+                               _p(rp), _p(ci), _p(v)) == 0
+    return edges, rows, rp, ci, v

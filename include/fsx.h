@@ -232,6 +232,37 @@ int fsx_krylov_bicgstab(fsx_ctx * ctx, const fsx_csr * A, int prec,
                         const fsx_settings * s, fsx_info * info,
                         double * res_hist, int hist_cap);
 
+/* ---- batched solves (new: many independent systems, one persistent launch)
+ * A is the block-diagonal concatenation (int32 indices, FP64 values, global
+ * column ids) of nsys systems; system s owns rows [sys_rows[s],
+ * sys_rows[s+1]).  Each system is solved with op::cg::apply's iteration
+ * (flecsolve/solvers/cg.hh:43-139) by one workgroup; per-system results are
+ * written to the host arrays (any may be NULL). */
+typedef struct fsx_batch fsx_batch;
+int fsx_batch_create(fsx_ctx * ctx, const fsx_csr * A, int64_t nsys,
+                     const int64_t * sys_rows, fsx_batch ** out);
+int fsx_batch_destroy(fsx_batch * B);
+/* Execution configuration: workgroup size, row mapping (0 sliced rows,
+ * 1 `lanes` lanes per CSR row). */
+int fsx_batch_config(const fsx_batch * B, int * wg, int * mapping, int * lanes);
+int fsx_batch_set_config(fsx_batch * B, int wg, int mapping, int lanes);
+/* Autotuner: times the candidate configurations on `sample` systems for
+ * `iters` CG iterations each and keeps the fastest (cached per sparsity
+ * profile).  Overwrites x on the sampled systems. */
+int fsx_batch_tune(fsx_ctx * ctx, fsx_batch * B, const fsx_vec * dinv,
+                   const fsx_vec * b, fsx_vec * x, int sample, int iters,
+                   double * best_ms);
+int fsx_batched_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_vec * dinv,
+                   const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                   int * iters, int * status, double * res);
+/* FP32-inner / FP64-outer refinement per system; A32 =
+ * fsx_csr_convert(A, FSX_F32).  outer_iters/inner_iters/status/res per system. */
+int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
+                      int jacobi, const fsx_vec * b, fsx_vec * x,
+                      double rtol_outer, int max_outer, const fsx_settings * inner,
+                      int * outer_iters, int * inner_iters, int * status,
+                      double * res);
+
 /* ---- measurement helpers ------------------------------------------------
  * Average duration (ms) of `reps` back-to-back launches of the SpMV kernel
  * for A, timed with HIP events on the context stream. */

@@ -50,6 +50,20 @@ int fsx_gen_diffusion3d(int n, const double * k, double scale, double shift,
 int fsx_gen_convdiff3d(int n, double cx, double cy, double cz,
                        int32_t * rowptr, int32_t * colind, double * vals);
 
+/* Batch of independent 3D diffusion systems (block-diagonal concatenation).
+ * System s has edge e_s = emin + (splitmix64(seed_e, s) mod (emax-emin+1)),
+ * its own lognormal field (fsx_gen_grf_lognormal with seed seed_k + s) and
+ * A_s = L_{k_s} as in fsx_gen_diffusion3d.  Rows of system s are
+ * [rows[s], rows[s+1]); column ids are global.
+ * Call 1: fsx_gen_batch_sizes fills edges[nsys], rows[nsys+1], nnzs[nsys+1].
+ * Call 2: fsx_gen_batch fills the concatenated CSR (int32). */
+int fsx_gen_batch_sizes(int nsys, int emin, int emax, uint64_t seed_e,
+                        int32_t * edges, int64_t * rows, int64_t * nnzs);
+int fsx_gen_batch(int nsys, const int32_t * edges, const int64_t * rows,
+                  const int64_t * nnzs, double sigma, double ell,
+                  uint64_t seed_k, int32_t * rowptr, int32_t * colind,
+                  double * vals);
+
 /* x_i ~ U[lo, hi) from splitmix64(seed, i). */
 int fsx_gen_uniform(double * x, int64_t n, uint64_t seed, double lo, double hi);
 

@@ -86,7 +86,7 @@ int main() {
 		vec::device<double> rhs(ctx, n * n), y(ctx, n * n);
 		rhs.set_random(0);
 		gmres::settings gs;
-		gs.maxiter = 400;
+		gs.maxiter = 3000; // restarted GMRES(30) on the 48^2 Laplacian needs ~450
 		gs.rtol = 1e-10;
 		gs.use_zero_guess = true;
 		gs.max_krylov_dim = 30;

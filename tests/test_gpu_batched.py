@@ -1,0 +1,137 @@
+"""Batched persistent CG (fsx_batch_*) against the oracle, system by system (gpu).
+
+Each system is an independent lognormal-k diffusion problem; the oracle solves
+each one separately with the reference's serial CG + Dinv.  Bar: every system
+converged, iteration counts within +-2, x within 1e-10 relative.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from flecsolve_amd import fsx
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def make(nsys=24, emin=3, emax=12, sigma=1.0, seed=4001):
+    e, rows, rp, ci, v = P.batch(nsys, emin, emax, seed, 4100, sigma)
+    b = P.uniform(int(rows[-1]), 4200)
+    return e, rows, rp, ci, v, b
+
+
+def local(rows, rp, ci, v, s):
+    lo, hi = rows[s], rows[s + 1]
+    e0, e1 = rp[lo], rp[hi]
+    return (rp[lo:hi + 1] - e0).astype(np.int64), (ci[e0:e1] - lo).astype(np.int64), v[e0:e1]
+
+
+def check_against_oracle(rows, rp, ci, v, b, xg, it, st, rtol=1e-10, slack=2):
+    for s in range(rows.size - 1):
+        sl = slice(rows[s], rows[s + 1])
+        Ao = O.Csr(*local(rows, rp, ci, v, s))
+        xo, io, _ = O.cg(Ao, b[sl], np.zeros(rows[s + 1] - rows[s]),
+                         O.Settings(maxiter=1000, rtol=rtol, use_zero_guess=True),
+                         P=O.prec(1, Ao.dinv()))
+        assert st[s] == 1, (s, st[s])
+        assert abs(int(it[s]) - io.iters) <= slack, (s, it[s], io.iters)
+        assert rel(xg[sl], xo) <= 1e-10, s
+
+
+@pytest.mark.parametrize("cfg", [(256, 0, 1), (1024, 0, 1), (64, 0, 1),
+                                 (256, 1, 2), (512, 1, 4), (256, 1, 8)])
+def test_batched_cg_matches_oracle(ctx, cfg):
+    e, rows, rp, ci, v, b = make()
+    A = ctx.csr(rp, ci, v)
+    B = fsx.Batch(A, rows)
+    B.set_config(*cfg)
+    assert B.config() == cfg
+    d = A.diag_inv()
+    bd, x = ctx.vec(b), ctx.zeros(b.size)
+    it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=1000, rtol=1e-10, use_zero_guess=True),
+                      dinv=d)
+    check_against_oracle(rows, rp, ci, v, b, x.get(), it, st)
+
+
+def test_batched_same_result_any_order_or_config(ctx):
+    # a system's arithmetic is independent of which workgroup solved it
+    e, rows, rp, ci, v, b = make(40, 4, 10)
+    A = ctx.csr(rp, ci, v)
+    B = fsx.Batch(A, rows)
+    d = A.diag_inv()
+    bd = ctx.vec(b)
+    outs = []
+    for wg in (256, 1024):
+        B.set_config(wg, 0, 1)
+        x = ctx.zeros(b.size)
+        it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=500, rtol=1e-10, use_zero_guess=True),
+                          dinv=d)
+        outs.append((x.get(), it))
+    # workgroup size changes the reduction tree, so compare to rounding
+    assert np.abs(outs[0][1] - outs[1][1]).max() <= 2
+    assert rel(outs[0][0], outs[1][0]) <= 1e-10
+    # repeat with the same config: bit-identical
+    x = ctx.zeros(b.size)
+    it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=500, rtol=1e-10, use_zero_guess=True),
+                      dinv=d)
+    np.testing.assert_array_equal(x.get(), outs[1][0])
+
+
+def test_batched_autotune(ctx):
+    e, rows, rp, ci, v, b = make(64, 4, 16, seed=77)
+    A = ctx.csr(rp, ci, v)
+    B = fsx.Batch(A, rows)
+    d = A.diag_inv()
+    bd, x = ctx.vec(b), ctx.zeros(b.size)
+    ms = B.tune(bd, x, d, sample=16, iters=10)
+    assert ms > 0
+    wg, mapping, lanes = B.config()
+    assert wg in (256, 512, 1024) and mapping in (0, 1) and lanes in (1, 2, 4, 8)
+    x = ctx.zeros(b.size)
+    it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=1000, rtol=1e-10, use_zero_guess=True),
+                      dinv=d)
+    check_against_oracle(rows, rp, ci, v, b, x.get(), it, st)
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.5])
+def test_batched_ir_reaches_1e10(ctx, sigma):
+    e, rows, rp, ci, v, b = make(32, 4, 14, sigma=sigma)
+    A = ctx.csr(rp, ci, v)
+    A32 = A.convert(fsx.F32)
+    B = fsx.Batch(A, rows)
+    bd, x = ctx.vec(b), ctx.zeros(b.size)
+    outer, inner, st, rs = B.ir_cg(A32, bd, x, 1e-10, fsx.Settings(maxiter=2000, rtol=1e-4))
+    assert (st == 1).all()
+    S = sp.csr_matrix((v, ci, rp))
+    xg = x.get()
+    for s in range(rows.size - 1):
+        sl = slice(rows[s], rows[s + 1])
+        r = b[sl] - (S[sl] @ xg)
+        assert np.linalg.norm(r) / np.linalg.norm(b[sl]) <= 1.01e-10, s
+        assert outer[s] >= 1 and inner[s] >= outer[s]
+
+
+def test_batched_edge_cases(ctx):
+    e, rows, rp, ci, v, b = make(6, 2, 5)
+    b[rows[2]:rows[3]] = 0.0       # zero right-hand side: converged at once
+    A = ctx.csr(rp, ci, v)
+    B = fsx.Batch(A, rows)
+    bd, x = ctx.vec(b), ctx.zeros(b.size)
+    it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=200, rtol=1e-10, use_zero_guess=True),
+                      dinv=A.diag_inv())
+    assert it[2] == 0 and st[2] == 1
+    assert (st == 1).all()
+    # maxiter exhausted: iters 0, diverged_iters (cg.hh:135-136)
+    it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=1, rtol=1e-14, use_zero_guess=True))
+    assert ((it == 0) & (st == 4) | (it == 0) & (st == 1)).all()
+    # coupled systems are rejected
+    rp2, ci2 = rp.copy(), ci.copy()
+    ci2[rp2[rows[1]]] = rows[0]     # first row of system 1 points into system 0
+    A2 = ctx.csr(rp2, ci2, v)
+    with pytest.raises(fsx.FsxError):
+        fsx.Batch(A2, rows)
