@@ -27,7 +27,8 @@ BUILD = PKG / "_build"
 ARCH = os.environ.get("FSX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["fsx_core.hip", "fsx_spmv.hip", "fsx_cg.hip", "fsx_batched.hip"]
+HIP_SOURCES = ["fsx_core.hip", "fsx_spmv.hip", "fsx_cg.hip", "fsx_batched.hip",
+               "fsx_adjoint.hip"]
 CPP_SOURCES = ["fsx_krylov.cpp"]
 
 # -ffp-contract=off: no fused multiply-add contraction, so elementwise results

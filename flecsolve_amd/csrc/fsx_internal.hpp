@@ -183,6 +183,8 @@ void launch_convert(fsx_ctx * ctx, const fsx_vec * s, fsx_vec * d, double scale)
 void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr);
 // Build (or complete) the sliced-row layout of A; host_rowptr may be null.
 void ensure_sliced(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr);
+// After A->vals changed on the device: refill the sliced copy in place.
+void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A);
 
 // Make ctx->ws.partials hold at least `count` doubles.  Reallocation drops
 // captured graphs (they hold the old address).

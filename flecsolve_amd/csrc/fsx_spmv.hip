@@ -170,6 +170,33 @@ void ensure_sliced(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr) {
 	else build_sliced_t<int64_t>(ctx, A, host_rowptr);
 }
 
+// Re-copy A's CSR values into its existing sliced-row buffer (same address,
+// so captured graphs stay valid); stream-ordered, no sync.
+template<class I, class V>
+static void refresh_sliced_t(fsx_ctx * ctx, fsx_csr * A) {
+	const int64_t nslots = ((A->nrows + kSliceRows - 1) / kSliceRows) * kSliceRows;
+	if (nslots == 0) return;
+	const unsigned g = (unsigned)((nslots + 255) / 256);
+	hipLaunchKernelGGL((sliced_build_kernel<I, V>), dim3(g), dim3(256), 0, ctx->stream,
+	                   A->nrows, nslots, (const I *)A->rowptr(), (const I *)A->colind(),
+	                   (const V *)A->vals->p, (const I *)A->idx->sl_ptr->p, (I *)nullptr,
+	                   (V *)A->sl_vals->p);
+	FSX_HIP(hipGetLastError());
+	ctx->launches++;
+}
+
+void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A) {
+	if (!(A->idx->sl_ptr && A->sl_vals)) return;
+	if (A->it == FSX_I32) {
+		if (A->vt == FSX_F64) refresh_sliced_t<int32_t, double>(ctx, A);
+		else refresh_sliced_t<int32_t, float>(ctx, A);
+	}
+	else {
+		if (A->vt == FSX_F64) refresh_sliced_t<int64_t, double>(ctx, A);
+		else refresh_sliced_t<int64_t, float>(ctx, A);
+	}
+}
+
 // Short rows of similar length -> sliced rows (padding <= 10%, addressable
 // with the index type); other short-row matrices -> LDS-staged row blocks;
 // long rows -> LANES lanes per row with LANES the next power of two of the

@@ -53,6 +53,16 @@ class IRInfo(C.Structure):
                 ("rhs_norm", C.c_double)]
 
 
+class BEStats(C.Structure):
+    _fields_ = [("forward_solves", C.c_int), ("recompute_solves", C.c_int),
+                ("adjoint_solves", C.c_int), ("failed_step", C.c_int),
+                ("forward_iters", C.c_int64), ("recompute_iters", C.c_int64),
+                ("adjoint_iters", C.c_int64), ("checkpoint_bytes", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -141,6 +151,15 @@ _SIGS = {
     "fsx_batched_cg": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings), _P, _P, _P]),
     "fsx_batched_ir_cg": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double, C.c_int,
                                     C.POINTER(Settings), _P, _P, _P, _P]),
+    "fsx_csr_transpose": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "fsx_csr_sampled_outer": (C.c_int, [_P, _P, _P, _P, C.c_double, _P]),
+    "fsx_diffusion3d_values": (C.c_int, [_P, C.c_int, _P, C.c_double, C.c_double, _P]),
+    "fsx_diffusion3d_coeff_grad": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_double, _P]),
+    "fsx_be_forward": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(Settings), _P,
+                                 C.POINTER(BEStats)]),
+    "fsx_be_adjoint": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_int,
+                                 C.POINTER(Settings), C.POINTER(C.c_double), _P, _P, _P,
+                                 C.POINTER(BEStats)]),
     "fsx_time_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
     "fsx_time_copy": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
 }
@@ -329,6 +348,27 @@ class CSR:
     def residual(self, b: Vector, x: Vector, r: Vector):
         check(lib().fsx_csr_residual(self.ctx.h, self.h, b.h, x.h, r.h), "residual")
 
+    def transpose(self) -> "CSR":
+        h = C.c_void_p()
+        check(lib().fsx_csr_transpose(self.ctx.h, self.h, C.byref(h)), "fsx_csr_transpose")
+        return CSR(self.ctx, None, None, None, _handle=h)
+
+    def sampled_outer(self, lam: Vector, x: Vector, scale: float, out: Vector):
+        """out[e] += (scale * lam[row e]) * x[col e] on the stored entries."""
+        check(lib().fsx_csr_sampled_outer(self.ctx.h, self.h, lam.h, x.h, scale, out.h),
+              "fsx_csr_sampled_outer")
+
+    def set_diffusion3d_values(self, n: int, k: Vector, shift: float, scale: float):
+        """Values <- shift I + scale L_k (7-point, arithmetic-mean faces)."""
+        check(lib().fsx_diffusion3d_values(self.ctx.h, n, k.h, shift, scale, self.h),
+              "fsx_diffusion3d_values")
+
+    def diffusion3d_coeff_grad(self, n: int, G: Vector, k: Vector, scale: float) -> Vector:
+        g = Vector(self.ctx, (n + 2) ** 3)
+        check(lib().fsx_diffusion3d_coeff_grad(self.ctx.h, n, self.h, G.h, k.h, scale, g.h),
+              "fsx_diffusion3d_coeff_grad")
+        return g
+
     def diag_inv(self, dtype: int = F64) -> Vector:
         d = Vector(self.ctx, self.nrows, dtype)
         check(lib().fsx_csr_diag_inv(self.ctx.h, self.h, d.h), "diag_inv")
@@ -409,6 +449,32 @@ def ir_cg_solve(A64: CSR, A32: CSR, b: Vector, x: Vector, rtol_outer: float,
                                 rtol_outer, max_outer, C.byref(inner),
                                 C.byref(info)), "fsx_ir_cg_solve")
     return info
+
+
+def be_forward(A: CSR, u0: Vector, steps: int, settings: Settings,
+               dinv: Vector | None = None):
+    """Backward-Euler trajectory u^{s+1} = A^{-1} u^s; returns (u^steps, stats)."""
+    uT = Vector(A.ctx, A.nrows)
+    st = BEStats()
+    check(lib().fsx_be_forward(A.ctx.h, A.h, dinv.h if dinv is not None else None, u0.h,
+                               steps, C.byref(settings), uT.h, C.byref(st)), "fsx_be_forward")
+    return uT, st
+
+
+def be_adjoint(A: CSR, u0: Vector, u_target: Vector, steps: int, ckpt: int,
+               settings: Settings, dinv: Vector | None = None, AT: CSR | None = None):
+    """Loss |u^steps - u_target|^2 and its gradients by the discrete adjoint.
+    Returns (loss, G = dL/dA on A's pattern, u^steps, dL/du^0, stats)."""
+    G = Vector(A.ctx, A.nnz)
+    uT = Vector(A.ctx, A.nrows)
+    gu0 = Vector(A.ctx, A.nrows)
+    loss = C.c_double()
+    st = BEStats()
+    check(lib().fsx_be_adjoint(A.ctx.h, A.h, AT.h if AT is not None else None,
+                               dinv.h if dinv is not None else None, u0.h, u_target.h,
+                               steps, ckpt, C.byref(settings), C.byref(loss), G.h, uT.h,
+                               gu0.h, C.byref(st)), "fsx_be_adjoint")
+    return loss.value, G, uT, gu0, st
 
 
 def krylov(kind: str, A: CSR, b: Vector, x: Vector, settings: Settings,

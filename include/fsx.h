@@ -232,6 +232,57 @@ int fsx_krylov_bicgstab(fsx_ctx * ctx, const fsx_csr * A, int prec,
                         const fsx_settings * s, fsx_info * info,
                         double * res_hist, int hist_cap);
 
+/* ---- adjoint of backward-Euler diffusion (new: no reference counterpart;
+ * the forward step is the reference's BE integrator, time-integrators/
+ * bdf.hh:74-95, 269-272, 292-296 with operator_adapter.hh:29-35, assembled
+ * as A = I + (dt/h^2) L_k and solved with op::cg::apply + Jacobi). */
+
+/* AT = A^T as a new matrix (same index and value types; rows of AT keep
+ * ascending column order). */
+int fsx_csr_transpose(fsx_ctx * ctx, const fsx_csr * A, fsx_csr ** AT);
+
+/* out_vals[e] += (scale * lam[row(e)]) * x[col(e)] for every stored entry e
+ * of A: the sampled outer product giving dL/dA on A's pattern. */
+int fsx_csr_sampled_outer(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * lam,
+                          const fsx_vec * x, double scale, fsx_vec * out_vals);
+
+/* A's values <- shift I + scale L_k for node coefficients k on the (n+2)^3
+ * lattice, 7-point stencil with arithmetic-mean faces
+ * (physics/volume_diffusion/coefficient.hh:157-174); A must have the pattern
+ * of fsx_gen_diffusion3d(n) (values identical to that generator's). */
+int fsx_diffusion3d_values(fsx_ctx * ctx, int n, const fsx_vec * k, double shift,
+                           double scale, fsx_csr * A);
+
+/* grad_logk[p] = dL/dlog k_p given G = dL/dA on A's pattern, for A = shift I
+ * + scale L_k as above. */
+int fsx_diffusion3d_coeff_grad(fsx_ctx * ctx, int n, const fsx_csr * A,
+                               const fsx_vec * G, const fsx_vec * k, double scale,
+                               fsx_vec * grad_logk);
+
+typedef struct {
+	int forward_solves, recompute_solves, adjoint_solves;
+	int failed_step; /* -1 when every solve converged */
+	int64_t forward_iters, recompute_iters, adjoint_iters;
+	int64_t checkpoint_bytes; /* device bytes of checkpoints + segment states */
+} fsx_be_stats;
+
+/* u^{s+1} = A^{-1} u^s for s < steps (CG + Jacobi when dinv is given, each
+ * solve from the guess u^s unless s->use_zero_guess); uT = u^steps.  A solve
+ * that does not converge is an error (FSX_ERR_STATE, stats->failed_step). */
+int fsx_be_forward(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                   const fsx_vec * u0, int steps, const fsx_settings * s,
+                   fsx_vec * uT, fsx_be_stats * stats);
+
+/* Loss L = |u^steps - u_target|^2 and G = dL/dA on A's pattern (overwritten)
+ * by the discrete adjoint: checkpoints every `ckpt` steps, adjoint solves
+ * with AT (A itself when AT is NULL, for symmetric A) from the guess
+ * lambda^{s+1}.  Optional outputs: uT = u^steps, grad_u0 = dL/du^0. */
+int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT,
+                   const fsx_vec * dinv, const fsx_vec * u0,
+                   const fsx_vec * u_target, int steps, int ckpt,
+                   const fsx_settings * s, double * loss, fsx_vec * G,
+                   fsx_vec * uT, fsx_vec * grad_u0, fsx_be_stats * stats);
+
 /* ---- batched solves (new: many independent systems, one persistent launch)
  * A is the block-diagonal concatenation (int32 indices, FP64 values, global
  * column ids) of nsys systems; system s owns rows [sys_rows[s],

@@ -312,3 +312,110 @@ def mm_read(path):
     if cnt < 0:
         raise IOError(path)
     return nr.value, nc.value, bool(sym.value), I[:cnt], J[:cnt], V[:cnt]
+
+
+# ---- backward-Euler diffusion and its adjoint --------------------------------
+# The forward step restates the reference's BE integrator
+# (time-integrators/bdf.hh:86-94, 269-272, 292-296 with operator_adapter.hh:
+# 29-35): solve (I + gamma F) u^{s+1} = u^s from the guess u^s.  The adjoint
+# has no reference counterpart; it is pinned by central finite differences of
+# the forward loss (tests/test_oracle_adjoint.py) and, for the coefficient
+# chain, by finite differences of a linear functional of the assembled values.
+
+def be_forward(A: Csr, u0, steps, settings, jacobi=True):
+    """States u^0..u^steps of u^{s+1} = A^{-1} u^s (CG, Jacobi if asked)."""
+    P = prec(1, A.dinv()) if jacobi else None
+    states = [np.array(u0, np.float64)]
+    iters = 0
+    for s in range(steps):
+        x, info, _ = cg(A, states[-1], states[-1], settings, P=P)
+        if info.reason not in ("converged_rtol", "converged_atol"):
+            raise RuntimeError(f"oracle BE step {s}: {info.reason}")
+        iters += info.iters
+        states.append(x)
+    return states, iters
+
+
+def sampled_outer(rowptr, colind, lam, x, scale, out):
+    """out[e] += (scale * lam[row e]) * x[col e] over the stored entries."""
+    rows = np.repeat(np.arange(rowptr.size - 1), np.diff(rowptr))
+    out += (scale * lam[rows]) * x[colind]
+    return out
+
+
+def be_adjoint(A: Csr, u0, u_target, steps, settings, jacobi=True):
+    """Loss |u^N - u_target|^2, G = dL/dA on A's pattern and dL/du^0, keeping
+    every state (no checkpoints).  A is symmetric here, so A^{-T} = A^{-1}."""
+    states, _ = be_forward(A, u0, steps, settings, jacobi)
+    P = prec(1, A.dinv()) if jacobi else None
+    d = states[-1] - u_target
+    loss = float(np.dot(d, d))
+    lam = 2.0 * d
+    G = np.zeros(A.colind.size)
+    for s in range(steps - 1, -1, -1):
+        mu, info, _ = cg(A, lam, lam, settings, P=P)
+        if info.reason not in ("converged_rtol", "converged_atol"):
+            raise RuntimeError(f"oracle adjoint step {s}: {info.reason}")
+        sampled_outer(A.rowptr, A.colind, mu, states[s + 1], -1.0, G)
+        lam = mu
+    return loss, G, lam, states[-1]
+
+
+def _entry_pos(rowptr, colind, rows, cols, width=7):
+    """Position of column cols[q] in row rows[q] (rows of at most `width`)."""
+    pos = np.full(rows.size, -1, np.int64)
+    lo, hi = rowptr[rows], rowptr[rows + 1]
+    for j in range(width):
+        e = lo + j
+        ok = (e < hi) & (pos < 0)
+        e = np.where(ok, e, 0)
+        hit = ok & (colind[e] == cols)
+        pos[hit] = e[hit]
+    assert (pos >= 0).all(), "entry not in the pattern"
+    return pos
+
+
+def diffusion3d_coeff_grad(n, rowptr, colind, G, k, scale):
+    """dL/dlog k on the (n+2)^3 lattice from G = dL/dA for A = shift I +
+    scale L_k (7-point, k_f = (k_a + k_b)/2): face by face, g_f = ((G_ll +
+    G_hh) - G_lh) - G_hl over the face's lower/upper nodes (Dirichlet nodes
+    contribute no term), each node summing (scale/2) g_f over its faces in the
+    order -z, -y, -x, +x, +y, +z, times k at the node."""
+    L = n + 2
+    rowptr = np.asarray(rowptr, np.int64)
+    colind = np.asarray(colind, np.int64)
+    kk, jj, ii = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    inside = (ii >= 1) & (ii <= n) & (jj >= 1) & (jj <= n) & (kk >= 1) & (kk <= n)
+    R = np.where(inside, (ii - 1) + n * ((jj - 1) + n * (kk - 1)), -1)
+    gdiag = np.zeros((L, L, L))
+    rows = R[inside]
+    gdiag[inside] = G[_entry_pos(rowptr, colind, rows, rows)]
+    hs = 0.5 * scale
+
+    def face_terms(axis):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[axis] = slice(0, L - 1)
+        hi[axis] = slice(1, L)
+        lo, hi = tuple(lo), tuple(hi)
+        lin, hin = inside[lo], inside[hi]
+        both = lin & hin
+        glh = np.zeros(lin.shape)
+        ghl = np.zeros(lin.shape)
+        rl, rh = R[lo][both], R[hi][both]
+        glh[both] = G[_entry_pos(rowptr, colind, rl, rh)]
+        ghl[both] = G[_entry_pos(rowptr, colind, rh, rl)]
+        gll = np.where(lin, gdiag[lo], 0.0)
+        ghh = np.where(hin, gdiag[hi], 0.0)
+        t = hs * (((gll + ghh) - glh) - ghl)
+        return np.where(lin | hin, t, 0.0)
+
+    tz, ty, tx = face_terms(0), face_terms(1), face_terms(2)
+    acc = np.zeros((L, L, L))
+    acc[1:, :, :] += tz    # -z face of each node
+    acc[:, 1:, :] += ty    # -y
+    acc[:, :, 1:] += tx    # -x
+    acc[:, :, :-1] += tx   # +x
+    acc[:, :-1, :] += ty   # +y
+    acc[:-1, :, :] += tz   # +z
+    return (np.asarray(k).reshape(L, L, L) * acc).ravel()
