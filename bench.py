@@ -5,6 +5,8 @@ with the SpMV kernel's fraction of the HBM roofline.
     python bench.py [--steps K] [--warmup W] [--n 256] [--sigma 1.0]
                     [--no-cpu-baseline]
     python bench.py --workload batched [--nsys 2048 --emin 8 --emax 48]
+    python bench.py --workload adjoint [--n 128 --heat-steps 50 --ckpt 10]
+    python bench.py --workload convdiff [--n 128]
 
 A step is one complete solve from x0 = 0 to |b - A x| / |b| <= 1e-10 with the
 matrix and right-hand side already resident in HBM.  The timed steps use
@@ -15,6 +17,15 @@ is timed the same way beside it.  One JSON line is printed at the end.
 edge lengths uniform in 8..48, solved in one persistent launch (FP64 CG +
 Jacobi and FP32-inner/FP64-outer refinement, autotuned row mapping); a step
 is the whole batch solved to 1e-10 per system.
+
+`--workload adjoint`: backward-Euler heat on 128^3 (dt 1e-3, 50 steps, CG +
+Jacobi per step), loss |u_T - u_target|^2 and its gradient with respect to
+log k by the checkpointed discrete adjoint; a step is one loss-and-gradient
+evaluation; the gradient is checked against central finite differences
+along 20 seeded directions.
+
+`--workload convdiff`: 128^3 upwind convection-diffusion (cell Peclet 10)
+solved with BiCGSTAB and with GMRES(30) (right Jacobi) to 1e-10.
 """
 from __future__ import annotations
 
@@ -85,7 +96,12 @@ def main():
     ap.add_argument("--max-outer", type=int, default=10)
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the SpMV and copy kernels only (profiling runs)")
-    ap.add_argument("--workload", choices=["diffusion", "batched"], default="diffusion")
+    ap.add_argument("--workload", choices=["diffusion", "batched", "adjoint", "convdiff"],
+                    default="diffusion")
+    ap.add_argument("--heat-steps", type=int, default=50)
+    ap.add_argument("--ckpt", type=int, default=10)
+    ap.add_argument("--fd-directions", type=int, default=20)
+    ap.add_argument("--heat-rtol", type=float, default=1e-12)
     ap.add_argument("--nsys", type=int, default=2048)
     ap.add_argument("--emin", type=int, default=8)
     ap.add_argument("--emax", type=int, default=48)
@@ -103,6 +119,10 @@ def main():
 
     if a.workload == "batched":
         return bench_batched(a, fsx, P)
+    if a.workload == "adjoint":
+        return bench_adjoint(a, fsx, P)
+    if a.workload == "convdiff":
+        return bench_convdiff(a, fsx, P)
 
     seeds = P.seeds()
     n = a.n
@@ -397,6 +417,160 @@ def cpu_baseline_batched(a, rows, rp, ci, v, b, it64, x64):
                    f"the whole batch by rows x iterations; max rel error of the GPU's "
                    f"FP64 x vs the oracle on the sample {worst:.1e}"),
     }
+
+
+def cg_iter_bytes(N, nnz):
+    """Algorithmic bytes of one fused CG + Jacobi iteration (FP64, int32):
+    the SpMV plus 88 N of vector traffic (update 64 N, p-update 24 N)."""
+    return spmv_bytes(N, nnz, 8) + 88 * N
+
+
+def bench_adjoint(a, fsx, P):
+    from flecsolve_amd.heat import HeatProblem
+    n = a.n if a.n != 256 else 128
+    ctx = fsx.Context(0)
+    t0 = time.perf_counter()
+    prob = HeatProblem(ctx, n=n, dt=1e-3, steps=a.heat_steps, ckpt=a.ckpt,
+                       rtol=a.heat_rtol)
+    log(f"heat {n}^3, {a.heat_steps} steps, gamma {prob.gamma:.3f}: set up and target "
+        f"trajectory in {time.perf_counter()-t0:.1f} s "
+        f"({prob.target_stats.forward_iters} CG iterations)")
+    theta = prob.theta0
+    tf = []
+    for i in range(a.warmup + a.steps):
+        ctx.sync()
+        t = time.perf_counter()
+        L = prob.loss(theta)
+        ctx.sync()
+        if i >= a.warmup:
+            tf.append(time.perf_counter() - t)
+    tg, res = [], None
+    for i in range(a.warmup + a.steps):
+        ctx.sync()
+        t = time.perf_counter()
+        res = prob.loss_and_grad(theta)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        log(f"loss+gradient {'warmup' if i < a.warmup else 'step'} {i}: {dt:.3f} s")
+        if i >= a.warmup:
+            tg.append(dt)
+    loss, g, gu0, st = res
+    t_fwd, t_grad = sum(tf) / len(tf), sum(tg) / len(tg)
+    st = st.as_dict()
+    log(f"forward {t_fwd:.3f} s; loss+grad {t_grad:.3f} s; stats {st}")
+    # central finite differences along seeded directions
+    seeds = P.seeds()
+    eps, worst, t = 1e-4, 0.0, time.perf_counter()
+    for j in range(a.fd_directions):
+        d = P.uniform(theta.size, seeds["heat_directions"] + j)
+        fd = (prob.loss(theta + eps * d) - prob.loss(theta - eps * d)) / (2 * eps)
+        err = abs(fd - g @ d) / abs(fd)
+        worst = max(worst, err)
+        log(f"direction {j}: fd {fd:.10e} adjoint {g @ d:.10e} rel err {err:.2e}")
+    log(f"finite differences: {a.fd_directions} directions in {time.perf_counter()-t:.1f} s")
+    iters = st["forward_iters"] + st["recompute_iters"] + st["adjoint_iters"]
+    N, nnz = prob.N, prob.A.nnz
+    gbs = iters * cg_iter_bytes(N, nnz) / t_grad / 1e9
+    out = {
+        "metric": f"BE heat {n}^3 x {a.heat_steps} steps: loss + adjoint gradient wrt log k",
+        "value": round(t_grad, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_grad * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal k fields, Gaussian-bump u0)",
+        "config": {
+            "workload": f"heat_be_{n}^3_dt1e-3_{a.heat_steps}steps_ckpt{a.ckpt}",
+            "rows": N, "nnz": int(nnz), "solve_rtol": a.heat_rtol,
+            "forward_seconds": round(t_fwd, 4),
+            "adjoint_stats": st,
+            "fd_check": {"directions": a.fd_directions, "eps": eps,
+                         "max_rel_err": worst},
+            "loss": loss,
+        },
+        "roofline": {"bound": "hbm", "kernel": "fused CG iteration (sliced SpMV + BLAS-1)",
+                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "note": "every CG iteration of the forward, recompute and adjoint "
+                             "solves at the fused floor B_spmv + 88 N bytes, over the "
+                             "loss+gradient wall time"},
+    }
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        k = np.exp(theta)
+        rp, ci, v = P.diffusion3d(n, k, scale=prob.gamma, shift=1.0)
+        A = O.Csr(rp, ci, v)
+        u0 = prob.u0.get()
+        sample = 2
+        so = O.Settings(maxiter=5000, rtol=a.heat_rtol, use_zero_guess=False)
+        t = time.perf_counter()
+        _, it_s = O.be_forward(A, u0, sample, so)
+        dt = time.perf_counter() - t
+        per_iter = dt / max(1, it_s)
+        out["cpu_baseline"] = {
+            "value": per_iter * iters, "unit": "s", "cores": 1, "kind": "port",
+            "sample": (f"{sample} BE steps of the oracle's FP64 CG + Jacobi on 1 thread "
+                       f"({it_s} iterations, {dt:.1f} s), scaled by iterations to the "
+                       f"{iters} CG iterations of one loss + gradient evaluation")}
+    print(json.dumps(out), flush=True)
+
+
+def bench_convdiff(a, fsx, P):
+    n = a.n if a.n != 256 else 128
+    seeds = P.seeds()
+    rp, ci, v = P.convdiff3d(n, (10.0, 5.0, 2.5))
+    N, nnz = n ** 3, int(ci.size)
+    b = P.uniform(N, seeds["convdiff_rhs"])
+    ctx = fsx.Context(0)
+    A = ctx.csr(rp, ci, v)
+    bd, x = ctx.vec(b), ctx.zeros(N)
+    r = ctx.zeros(N)
+    results = {}
+    for name, kind, st, prec in [
+            ("bicgstab", "bicgstab", fsx.Settings(maxiter=a.maxiter, rtol=1e-10,
+                                                  use_zero_guess=True), 0),
+            ("gmres30_jacobi", "gmres", fsx.Settings(maxiter=a.maxiter, rtol=1e-10,
+                                                     use_zero_guess=True, max_krylov_dim=30,
+                                                     pre_side=1, restart=True), 1)]:
+        ts = []
+        for i in range(a.warmup + a.steps):
+            fsx.check(fsx.lib().fsx_zero(ctx.h, x.h))
+            ctx.sync()
+            t = time.perf_counter()
+            info, _ = fsx.krylov(kind, A, bd, x, st, prec=prec)
+            ctx.sync()
+            dt = time.perf_counter() - t
+            log(f"{name} {'warmup' if i < a.warmup else 'step'} {i}: {dt:.3f} s, "
+                f"{info.iters} iters, {info.restarts} restarts, {info.reason}")
+            if i >= a.warmup:
+                ts.append(dt)
+        A.residual(bd, x, r)
+        relres = fsx.nrm2(r) / fsx.nrm2(bd)
+        results[name] = {"seconds": round(sum(ts) / len(ts), 4), "iters": info.iters,
+                         "restarts": info.restarts, "status": info.reason,
+                         "true_rel_residual": relres}
+    out = {
+        "metric": f"convection-diffusion {n}^3 time-to-1e-10: BiCGSTAB, GMRES(30)",
+        "value": results["bicgstab"]["seconds"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(results["bicgstab"]["seconds"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (upwind convection-diffusion, uniform rhs)",
+        "config": {"workload": f"convdiff3d_{n}^3_peclet10_rtol1e-10", "rows": N,
+                   "nnz": nnz, "solvers": results},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def fsx_time(fsx, name, ctx, A, x, y, reps):
