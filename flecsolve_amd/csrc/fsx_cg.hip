@@ -10,18 +10,23 @@
 //
 //   K1  w = A p, partials of w.p           (cg.hh:95-98)    SpMV + dot
 //   F1  alpha = rho / w.p                  (cg.hh:98-106)
-//   K2  x += a p; r -= a w; z = D^-1 r; partials of r.r and r.z
-//                                          (cg.hh:108-127)  one HBM pass
+//   K2  r -= a w; z = D^-1 r; partials of r.r and r.z
+//                                          (cg.hh:109-127)  one HBM pass
 //   F2  |r|, tolerance test, rho, beta     (cg.hh:111-129)
-//   K3  p = beta p + z                     (cg.hh:130)
+//   K3  x += a p; p = beta p + z           (cg.hh:108, 130)
 //
-// F1/F2 (deterministic fixed-order reductions) update alpha, rho, beta, the
+// x += alpha p (cg.hh:108) is deferred from K2 to K3, which reads p anyway:
+// the iteration moves 80 instead of 88 bytes of vectors per row.  x is not
+// read inside the loop, and K3 applies the same expression to the same p
+// before overwriting it, so every x value is the reference's.  F1/F2
+// (deterministic fixed-order reductions) update alpha, rho, beta, the
 // residual history and a device `done` flag; every kernel returns
 // immediately once `done` is set, so a captured chunk of iterations that
 // overshoots convergence costs only empty launches and the reported
-// iteration count is exact.  Chunks of iterations are captured into
-// a HIP graph and replayed; the host reads the flag once per chunk with one
-// chunk in flight.
+// iteration count is exact.  The iteration that sets `done` skips its K3;
+// its x update is applied by one kernel after the loop (`xpend`).  Chunks of
+// iterations are captured into a HIP graph and replayed; the host reads the
+// flag once per chunk with one chunk in flight.
 #include "fsx_spmv_kernels.hpp"
 
 #include <cmath>
@@ -34,6 +39,7 @@ namespace fsx {
 struct cg_state {
 	double rho0, rho1, alpha, beta, res, tol, pw;
 	int iter, done, status, iters, maxiter, ncurv;
+	int xpend; // the last iteration's x += alpha p is still to be applied
 };
 
 // ---- K1: w = A p and the partials of w.p; F1: alpha ----------------------
@@ -75,16 +81,16 @@ struct fin_alpha {
 	}
 };
 
-// ---- K2: x, r update, partials of |r|^2 and r.z; F2: test and beta ------
+// ---- K2: r update, partials of |r|^2 and r.z; F2: test and beta ---------
 
 constexpr int kUpdThreads = 256;
 constexpr int kUpdPerThread = 4;
 
 template<class T, bool JAC>
 __global__ __launch_bounds__(kUpdThreads) void cg_update_kernel(
-	size_t n, T * __restrict__ x, T * __restrict__ r, const T * __restrict__ p,
-	const T * __restrict__ w, const T * __restrict__ dinv, T * __restrict__ z,
-	double * partials, const cg_state * st) {
+	size_t n, T * __restrict__ r, const T * __restrict__ w,
+	const T * __restrict__ dinv, T * __restrict__ z, double * partials,
+	const cg_state * st) {
 	if (st->done) return;
 	const double alpha = st->alpha;
 	const unsigned blk = blockIdx.x;
@@ -94,9 +100,8 @@ __global__ __launch_bounds__(kUpdThreads) void cg_update_kernel(
 	for (int j = 0; j < kUpdPerThread; ++j) {
 		const size_t i = base + (size_t)j * kUpdThreads;
 		if (i < n) {
-			// x.axpy(alpha, p, x); r.axpy(-alpha, w, r)  (cg.hh:108-109)
+			// r.axpy(-alpha, w, r)  (cg.hh:109)
 			if constexpr (std::is_same_v<T, double>) {
-				x[i] = alpha * p[i] + x[i];
 				const double ri = (-alpha) * w[i] + r[i];
 				r[i] = ri;
 				rr += ri * ri;
@@ -108,7 +113,6 @@ __global__ __launch_bounds__(kUpdThreads) void cg_update_kernel(
 			}
 			else {
 				// FP32 storage, FP64 arithmetic and reductions
-				x[i] = (T)(alpha * (double)p[i] + (double)x[i]);
 				const T ri = (T)((-alpha) * (double)w[i] + (double)r[i]);
 				r[i] = ri;
 				rr += (double)ri * (double)ri;
@@ -141,10 +145,12 @@ struct fin_beta {
 			st->iters = it + 1;
 			st->status = FSX_CONVERGED_RTOL;
 			st->done = 1;
+			st->xpend = 1;
 			return;
 		}
 		if (it + 1 >= st->maxiter) { // loop exhausted
 			st->done = 1;
+			st->xpend = 1;
 			return;
 		}
 		// P.apply(r, z) happened in K2; rho and beta (cg.hh:124-129)
@@ -154,17 +160,40 @@ struct fin_beta {
 	}
 };
 
-// ---- K3: p = beta p + z  (cg.hh:130) -------------------------------------
+// ---- K3: x += alpha p; p = beta p + z  (cg.hh:108, 130) -------------------
 
 template<class T>
-__global__ __launch_bounds__(256) void cg_pupdate_kernel(
-	size_t n, T * __restrict__ p, const T * __restrict__ z, const cg_state * st) {
+__global__ __launch_bounds__(256) void cg_xp_update_kernel(
+	size_t n, T * __restrict__ x, T * __restrict__ p, const T * __restrict__ z,
+	const cg_state * st) {
 	if (st->done) return;
-	const double beta = st->beta;
+	const double alpha = st->alpha, beta = st->beta;
 	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
 	if (i < n) {
-		if constexpr (std::is_same_v<T, double>) p[i] = beta * p[i] + z[i];
-		else p[i] = (T)(beta * (double)p[i] + (double)z[i]);
+		if constexpr (std::is_same_v<T, double>) {
+			const double pi = p[i];
+			x[i] = alpha * pi + x[i];
+			p[i] = beta * pi + z[i];
+		}
+		else {
+			const double pi = (double)p[i];
+			x[i] = (T)(alpha * pi + (double)x[i]);
+			p[i] = (T)(beta * pi + (double)z[i]);
+		}
+	}
+}
+
+// The final iteration's x += alpha p (its K3 was skipped by `done`).
+template<class T>
+__global__ __launch_bounds__(256) void cg_xfinal_kernel(size_t n, T * __restrict__ x,
+                                                        const T * __restrict__ p,
+                                                        const cg_state * st) {
+	if (!st->xpend) return;
+	const double alpha = st->alpha;
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) {
+		if constexpr (std::is_same_v<T, double>) x[i] = alpha * p[i] + x[i];
+		else x[i] = (T)(alpha * (double)p[i] + (double)x[i]);
 	}
 }
 
@@ -280,26 +309,24 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	const unsigned g2 = (unsigned)((B.n + per - 1) / per);
 	if (B.dinv) {
 		hipLaunchKernelGGL((cg_update_kernel<T, true>), dim3(g2), dim3(kUpdThreads),
-		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)p,
-		                   (const T *)B.w, (const T *)B.dinv, (T *)B.z, part,
-		                   (const cg_state *)st);
+		                   0, s, B.n, (T *)B.r, (const T *)B.w, (const T *)B.dinv,
+		                   (T *)B.z, part, (const cg_state *)st);
 		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<true>>), dim3(1),
 		                   dim3(kFinThreads), 0, s, (const double *)part, g2,
 		                   fin_beta<true>{st, hist});
 	}
 	else {
 		hipLaunchKernelGGL((cg_update_kernel<T, false>), dim3(g2), dim3(kUpdThreads),
-		                   0, s, B.n, (T *)B.x, (T *)B.r, (const T *)p,
-		                   (const T *)B.w, (const T *)nullptr, (T *)nullptr, part,
-		                   (const cg_state *)st);
+		                   0, s, B.n, (T *)B.r, (const T *)B.w, (const T *)nullptr,
+		                   (T *)nullptr, part, (const cg_state *)st);
 		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<false>>), dim3(1),
 		                   dim3(kFinThreads), 0, s, (const double *)part, g2,
 		                   fin_beta<false>{st, hist});
 	}
 	// K3
 	const unsigned g3 = (unsigned)((B.n + 255) / 256);
-	hipLaunchKernelGGL((cg_pupdate_kernel<T>), dim3(g3), dim3(256), 0, s, B.n, p,
-	                   (const T *)B.z, (const cg_state *)st);
+	hipLaunchKernelGGL((cg_xp_update_kernel<T>), dim3(g3), dim3(256), 0, s, B.n,
+	                   (T *)B.x, p, (const T *)B.z, (const cg_state *)st);
 	FSX_HIP(hipGetLastError());
 }
 
@@ -437,12 +464,19 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 		++checked;
 		if (f.done) break;
 	}
-	FSX_HIP(hipStreamSynchronize(ctx->stream));
-	(void)hipEventDestroy(ev[0]);
-	(void)hipEventDestroy(ev[1]);
+	if (launched > 0 && B.n > 0) {
+		// the iteration that set `done` skipped K3: apply its x update
+		const unsigned g3 = (unsigned)((B.n + 255) / 256);
+		hipLaunchKernelGGL((cg_xfinal_kernel<T>), dim3(g3), dim3(256), 0, ctx->stream,
+		                   B.n, (T *)B.x, (const T *)B.p, (const cg_state *)st);
+		FSX_HIP(hipGetLastError());
+		ctx->launches++;
+	}
 	FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(cg_state), hipMemcpyDeviceToHost,
 	                       ctx->stream));
 	FSX_HIP(hipStreamSynchronize(ctx->stream));
+	(void)hipEventDestroy(ev[0]);
+	(void)hipEventDestroy(ev[1]);
 	std::memcpy(&st_host, pinned, sizeof(cg_state));
 }
 
