@@ -135,12 +135,14 @@ def main():
     log(f"problem {n}^3: {N} rows, {nnz} nnz, generated in {time.perf_counter()-t0:.1f} s")
 
     ctx = fsx.Context(0)
+    t0 = time.perf_counter()
     A64 = ctx.csr(rp, ci, v)
     A32 = A64.convert(fsx.F32)
     dinv = A64.diag_inv()
     bd = ctx.vec(b)
     x = ctx.zeros(N)
-    log(f"uploaded; SpMV plan {A64.plan()}")
+    log(f"uploaded and layouts built in {time.perf_counter()-t0:.1f} s; "
+        f"SpMV plan {A64.plan()}")
 
     # ---- SpMV roofline (HIP events on the library stream) ----------------
     y = ctx.zeros(N)

@@ -21,7 +21,9 @@
 // Row mapping inside a workgroup (chosen by the autotuner, fsx_batch_tune):
 //   0: sliced rows -- one row per lane on the matrix's sliced layout
 //      (coalesced, rows summed serially: bit-identical SpMV);
-//   1: L lanes per row on the CSR arrays (L = 2, 4, 8), shuffle-combined.
+//   1: L lanes per row on the CSR arrays (L = 2, 4, 8), shuffle-combined;
+//   2: one row per lane on the slice-offset layout (per-slice column
+//      offsets and lane masks; serial row sums, bit-identical SpMV).
 #include "fsx_spmv_kernels.hpp"
 
 #include <algorithm>
@@ -39,7 +41,7 @@ struct fsx_batch {
 	std::unique_ptr<fsx::dbuf> rows, order, queue;
 	// execution configuration
 	int wg = 256;
-	int mapping = 0; // 0 sliced rows, 1 lanes per row
+	int mapping = 0; // 0 sliced rows, 1 lanes per row, 2 slice offsets
 	int lanes = 1;
 	bool tuned = false;
 	double tuned_ms = 0;
@@ -63,6 +65,11 @@ struct batch_args {
 	const int32_t * sl_col;
 	const double * sl_v64;
 	const float * sl_v32;
+	const int64_t * dg_ptr;
+	const int32_t * dg_off;
+	const uint64_t * dg_mask;
+	const double * dg_v64;
+	const float * dg_v32;
 	int mapping, lanes;
 	// vectors (concatenated)
 	const double * b;
@@ -126,10 +133,26 @@ __device__ __forceinline__ void wg_sum(double (&v)[NV], double * lds) {
 // first entry; columns are global ids, so gathers index xl - r0.
 template<class V, class X, class F>
 __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
-                                        const V * v, int64_t r0, int64_t n,
-                                        const X * xl, F && f) {
+                                        const V * v, const V * dg_v, int64_t r0,
+                                        int64_t n, const X * xl, F && f) {
 	const X * x = xl - r0;
-	if (a.mapping == 0) {
+	if (a.mapping == 2) {
+		for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+			const int64_t r = r0 + i;
+			const int64_t s = r / kSliceRows;
+			const int lane = (int)(r & (kSliceRows - 1));
+			const int64_t base = a.dg_ptr[s];
+			const int wd = (int)((a.dg_ptr[s + 1] - base) / kSliceRows);
+			const int64_t q0 = base / kSliceRows;
+			double sum = 0.0;
+			for (int j = 0; j < wd; ++j)
+				if ((a.dg_mask[q0 + j] >> lane) & 1)
+					sum += (double)dg_v[base + (int64_t)j * kSliceRows + lane] *
+					       (double)x[r + a.dg_off[q0 + j]];
+			f(i, sum);
+		}
+	}
+	else if (a.mapping == 0) {
 		for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
 			const int64_t r = r0 + i;
 			const int64_t s = r / kSliceRows;
@@ -169,7 +192,7 @@ __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
 // solve), V the matrix value type; scalars and reductions are FP64.
 template<bool JAC, class T, class V>
 __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
-                      int64_t r0, int64_t n, const T * b, T * x, T * r, T * z,
+                      const V * dg_v, int64_t r0, int64_t n, const T * b, T * x, T * r, T * z,
                       T * p, T * w, const T * dinv, int maxiter, double rtol,
                       bool zero_guess, double * lds, int & iters_out,
                       int & status_out, double & res_out) {
@@ -191,7 +214,7 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
 		}
 	}
 	else {
-		wg_spmv(a, sl_v, v, r0, n, (const T *)x, [&](int64_t i, double s) {
+		wg_spmv(a, sl_v, v, dg_v, r0, n, (const T *)x, [&](int64_t i, double s) {
 			r[i] = b[i] - (T)s;
 		});
 	}
@@ -225,7 +248,7 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
 	for (int it = 0; it < maxiter; ++it) {
 		// w = A p, alpha = rho / (w.p)  (cg.hh:95-106)
 		one[0] = 0.0;
-		wg_spmv(a, sl_v, v, r0, n, (const T *)p, [&](int64_t i, double s) {
+		wg_spmv(a, sl_v, v, dg_v, r0, n, (const T *)p, [&](int64_t i, double s) {
 			const T wi = (T)s;
 			w[i] = wi;
 			one[0] += (double)wi * (double)p[i];
@@ -300,7 +323,8 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batched_cg_kernel(batch_args
 		int iters = 0, status = FSX_UNKNOWN, inner = 0;
 		double res = 0.0;
 		if constexpr (!IR) {
-			wg_cg<JAC, double, double>(a, a.sl_v64, a.v64, r0, n, a.b + r0, a.x + r0,
+			wg_cg<JAC, double, double>(a, a.sl_v64, a.v64, a.dg_v64, r0, n, a.b + r0,
+			                           a.x + r0,
 			                           a.r + r0, a.z + r0, a.p + r0, a.w + r0,
 			                           a.dinv64 + r0, a.maxiter, a.rtol,
 			                           a.use_zero_guess != 0, lds, iters, status, res);
@@ -319,7 +343,8 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batched_cg_kernel(batch_args
 			for (;;) {
 				// r = b - A64 x, |r|
 				acc[0] = 0.0;
-				wg_spmv(a, a.sl_v64, a.v64, r0, n, (const double *)x, [&](int64_t i, double s) {
+				wg_spmv(a, a.sl_v64, a.v64, a.dg_v64, r0, n, (const double *)x,
+				        [&](int64_t i, double s) {
 					const double ri = b[i] - s;
 					r[i] = ri;
 					acc[0] += ri * ri;
@@ -340,7 +365,8 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batched_cg_kernel(batch_args
 				__syncthreads();
 				int it_in = 0, st_in = 0;
 				double res_in = 0.0;
-				wg_cg<JAC, float, float>(a, a.sl_v32, a.v32, r0, n, a.fb + r0, a.fd + r0,
+				wg_cg<JAC, float, float>(a, a.sl_v32, a.v32, a.dg_v32, r0, n, a.fb + r0,
+				                         a.fd + r0,
 				                         a.fr + r0, a.fz + r0, a.fp + r0, a.fw + r0,
 				                         a.dinv32 + r0, a.maxiter, a.rtol, true, lds,
 				                         it_in, st_in, res_in);
@@ -405,6 +431,9 @@ void check_batch(const fsx_batch * B) {
 	FSX_REQUIRE(B && B->A, "null batch");
 }
 
+bool has_sliced(const fsx_csr * A) { return A->idx->sl_ptr && A->sl_vals; }
+bool has_sdia(const fsx_csr * A) { return A->idx->dg_ptr && A->dg_vals; }
+
 struct batch_outputs {
 	std::unique_ptr<dbuf> iters, status, res, inner;
 	explicit batch_outputs(int64_t nsys) {
@@ -428,9 +457,14 @@ batch_args base_args(fsx_batch * B, const fsx_csr * A32) {
 	a.sl_ptr = A->idx->sl_ptr ? (const int32_t *)A->idx->sl_ptr->p : nullptr;
 	a.sl_col = A->idx->sl_col ? (const int32_t *)A->idx->sl_col->p : nullptr;
 	a.sl_v64 = A->sl_vals ? (const double *)A->sl_vals->p : nullptr;
+	a.dg_ptr = A->idx->dg_ptr ? (const int64_t *)A->idx->dg_ptr->p : nullptr;
+	a.dg_off = A->idx->dg_off ? (const int32_t *)A->idx->dg_off->p : nullptr;
+	a.dg_mask = A->idx->dg_mask ? (const uint64_t *)A->idx->dg_mask->p : nullptr;
+	a.dg_v64 = A->dg_vals ? (const double *)A->dg_vals->p : nullptr;
 	if (A32) {
 		a.v32 = (const float *)A32->vals->p;
 		a.sl_v32 = A32->sl_vals ? (const float *)A32->sl_vals->p : nullptr;
+		a.dg_v32 = A32->dg_vals ? (const float *)A32->dg_vals->p : nullptr;
 	}
 	a.mapping = B->mapping;
 	a.lanes = B->lanes;
@@ -533,8 +567,8 @@ int fsx_batch_create(fsx_ctx * ctx, const fsx_csr * A, int64_t nsys,
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
 		FSX_REQUIRE(bad == 0, "a system's rows reference columns of another system");
 		set_order(ctx, B.get(), lpt_order(B.get()));
-		B->mapping = (A->idx->sl_ptr && A->sl_vals) ? 0 : 1;
-		B->lanes = B->mapping == 0 ? 1 : 4;
+		B->mapping = has_sdia(A) ? 2 : has_sliced(A) ? 0 : 1;
+		B->lanes = B->mapping == 1 ? 4 : 1;
 		ctx_retain(ctx);
 		*out = B.release();
 	});
@@ -565,17 +599,19 @@ int fsx_batch_set_config(fsx_batch * B, int wg, int mapping, int lanes) {
 		check_batch(B);
 		FSX_REQUIRE(wg == 64 || wg == 128 || wg == 256 || wg == 512 || wg == 1024,
 		            "workgroup size must be 64, 128, 256, 512 or 1024");
-		FSX_REQUIRE(mapping == 0 || mapping == 1, "mapping must be 0 or 1");
+		FSX_REQUIRE(mapping >= 0 && mapping <= 2, "mapping must be 0, 1 or 2");
 		if (mapping == 0)
-			FSX_REQUIRE(B->A->idx->sl_ptr && B->A->sl_vals,
-			            "sliced mapping needs the matrix's sliced layout");
+			FSX_REQUIRE(has_sliced(B->A), "sliced mapping needs the matrix's sliced layout");
+		else if (mapping == 2)
+			FSX_REQUIRE(has_sdia(B->A),
+			            "slice-offset mapping needs the matrix's slice-offset layout");
 		else
 			FSX_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 ||
 			                lanes == 16,
 			            "lanes must be 1, 2, 4, 8 or 16");
 		B->wg = wg;
 		B->mapping = mapping;
-		B->lanes = mapping == 0 ? 1 : lanes;
+		B->lanes = mapping == 1 ? lanes : 1;
 	});
 }
 
@@ -640,7 +676,8 @@ int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
 		            "b and x: FP64, one entry per batch row");
 		FSX_REQUIRE(A32->vt == FSX_F32 && A32->idx == B->A->idx,
 		            "A32 must be fsx_csr_convert(A, FSX_F32) of the batch matrix");
-		FSX_REQUIRE(B->mapping == 1 || A32->sl_vals, "A32 lacks the sliced layout");
+		FSX_REQUIRE(B->mapping != 0 || A32->sl_vals, "A32 lacks the sliced layout");
+		FSX_REQUIRE(B->mapping != 2 || A32->dg_vals, "A32 lacks the slice-offset layout");
 		auto & W = work_for(B, true);
 		if (jacobi && !W.dinv32) {
 			W.dinv32 = std::make_unique<dbuf>(std::max<int64_t>(1, B->A->nrows) * 4);
@@ -739,7 +776,8 @@ int fsx_batch_tune(fsx_ctx * ctx, fsx_batch * B, const fsx_vec * dinv,
 		struct cand { int wg, mapping, lanes; };
 		std::vector<cand> cands;
 		for (int wg : {256, 512, 1024}) {
-			if (B->A->idx->sl_ptr && B->A->sl_vals) cands.push_back({wg, 0, 1});
+			if (has_sdia(B->A)) cands.push_back({wg, 2, 1});
+			if (has_sliced(B->A)) cands.push_back({wg, 0, 1});
 			for (int L : {2, 4, 8}) cands.push_back({wg, 1, L});
 		}
 		double best = 1e300;

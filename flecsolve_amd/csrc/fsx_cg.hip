@@ -406,7 +406,10 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 	hipGraphExec_t ge = nullptr;
 	if (use_graphs()) {
 		const fsx_csr * A = B.A;
-		cg_graph_cache::key_t key{A->rowptr(), A->sl_vals ? A->sl_vals->p : nullptr,
+		const void * layout_vals = A->plan_kind == 3 && A->dg_vals ? A->dg_vals->p
+		                           : A->sl_vals                   ? A->sl_vals->p
+		                                                          : nullptr;
+		cg_graph_cache::key_t key{A->rowptr(), layout_vals,
 		                          A->vals->p, B.x, B.r, B.z, B.p, B.p2, B.w, B.dinv,
 		                          A->nrows, A->nnz, A->plan_kind, A->plan_lanes,
 		                          (int)sizeof(T), K};

@@ -133,6 +133,14 @@ struct csr_index {
 	// each row; padding carries column -1.  Shared by all value copies.
 	std::unique_ptr<fsx::dbuf> sl_ptr, sl_col;
 	int64_t sl_nnz = 0;
+	// Slice-offset layout (built for matrices whose 64-row slices share a
+	// few column offsets, e.g. stencils): slice s keeps the sorted offsets
+	// off[q] = column - row present in any of its rows (q = dg_ptr[s]/64 +
+	// j), a 64-bit mask of the lanes whose row has an entry at offset j, and
+	// values entry-major at dg_ptr[s] + 64 j + lane (zero where absent).
+	// Index bytes per entry: 12/64 instead of 4; rows keep CSR order.
+	std::unique_ptr<fsx::dbuf> dg_ptr, dg_off, dg_mask;
+	int64_t dg_nnz = 0;
 	// Lazily built transpose structure for A^T x (adjoint).
 	std::unique_ptr<fsx::dbuf> t_rowptr, t_colind, t_perm;
 };
@@ -145,7 +153,9 @@ struct fsx_csr {
 	std::shared_ptr<csr_index> idx;
 	std::unique_ptr<fsx::dbuf> vals;
 	std::unique_ptr<fsx::dbuf> sl_vals; // values in the sliced-row layout
-	// SpMV row mapping: 0 LDS-staged row blocks, 1 vector per row, 2 sliced rows
+	std::unique_ptr<fsx::dbuf> dg_vals; // values in the slice-offset layout
+	// SpMV row mapping: 0 LDS-staged row blocks, 1 vector per row, 2 sliced
+	// rows, 3 slice offsets
 	int plan_kind = 0;
 	int plan_lanes = 0;
 	int64_t max_row = 0;
@@ -179,11 +189,18 @@ void launch_copy(fsx_ctx * ctx, const fsx_vec * s, fsx_vec * d);
 void launch_set(fsx_ctx * ctx, double a, fsx_vec * x);
 void launch_convert(fsx_ctx * ctx, const fsx_vec * s, fsx_vec * d, double scale);
 
-// SpMV row-mapping selection (fsx_spmv.hip); may build the sliced layout
-void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr);
+// SpMV row-mapping selection (fsx_spmv.hip); may build the sliced or the
+// slice-offset layout from the host copies of the structure
+void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
+                 const void * host_colind);
+// Build the slice-offset layout if A's structure qualifies (and, with
+// check_budget, its padding is small enough); false otherwise.
+bool ensure_sdia(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
+                 const void * host_colind, bool check_budget);
 // Build (or complete) the sliced-row layout of A; host_rowptr may be null.
 void ensure_sliced(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr);
-// After A->vals changed on the device: refill the sliced copy in place.
+// After A->vals changed on the device: refill the sliced and slice-offset
+// copies in place.
 void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A);
 
 // Make ctx->ws.partials hold at least `count` doubles.  Reallocation drops

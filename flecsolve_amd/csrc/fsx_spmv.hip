@@ -185,7 +185,155 @@ static void refresh_sliced_t(fsx_ctx * ctx, fsx_csr * A) {
 	ctx->launches++;
 }
 
+// ---- slice-offset layout ----------------------------------------------------
+
+// Values of row r (thread r of the padded slice range) into the layout:
+// zero the row's slots, then place each entry at its offset's position.
+// Rows and offsets are both ascending, so one forward scan finds them.
+template<class I, class V>
+__global__ void sdia_scatter_kernel(int64_t nrows, int64_t nslots,
+                                    const I * __restrict__ rp, const I * __restrict__ ci,
+                                    const V * __restrict__ v,
+                                    const int64_t * __restrict__ dg_ptr,
+                                    const int32_t * __restrict__ dg_off,
+                                    V * __restrict__ dg_val) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nslots) return;
+	const int64_t s = r / kSliceRows;
+	const int lane = (int)(r % kSliceRows);
+	const int64_t base = dg_ptr[s];
+	const int w = (int)((dg_ptr[s + 1] - base) / kSliceRows);
+	const int64_t q0 = base / kSliceRows;
+	for (int j = 0; j < w; ++j) dg_val[base + (int64_t)j * kSliceRows + lane] = V(0);
+	if (r >= nrows) return;
+	int j = 0;
+	for (I e = rp[r]; e < rp[r + 1]; ++e) {
+		const int64_t off = (int64_t)ci[e] - r;
+		while (j < w && (int64_t)dg_off[q0 + j] != off) ++j;
+		if (j == w) return; // not this layout's structure (cannot happen)
+		dg_val[base + (int64_t)j * kSliceRows + lane] = v[e];
+		++j;
+	}
+}
+
+// Host analysis: per slice, the sorted union of the rows' column offsets and
+// the lane masks.  Qualifies when every row's columns strictly increase, no
+// slice has more than 64 offsets and offsets fit int32; for automatic
+// selection also the padded size must be within 25% of nnz (+ one slice
+// column): a padded slot costs one value
+// (8 or 4 bytes) while every entry saves its 4-byte column id, so the layout
+// moves fewer bytes than the sliced one up to ~50% (FP64) / ~100% (FP32)
+// padding.
+template<class I>
+static bool sdia_analyze(int64_t nrows, int64_t nnz, const I * rp, const I * ci,
+                         bool check_budget, std::vector<int64_t> & ptr,
+                         std::vector<int32_t> & off, std::vector<uint64_t> & mask) {
+	const int64_t ns = (nrows + kSliceRows - 1) / kSliceRows;
+	const int64_t budget = check_budget ? nnz + nnz / 4 + kSliceRows : INT64_MAX;
+	ptr.assign(ns + 1, 0);
+	off.clear();
+	mask.clear();
+	std::vector<int64_t> U;
+	int64_t total = 0;
+	for (int64_t s = 0; s < ns; ++s) {
+		const int64_t r0 = s * kSliceRows, r1 = std::min<int64_t>(nrows, r0 + kSliceRows);
+		U.clear();
+		for (int64_t r = r0; r < r1; ++r)
+			for (I e = rp[r]; e < rp[r + 1]; ++e) {
+				if (e > rp[r] && ci[e] <= ci[e - 1]) return false;
+				const int64_t d = (int64_t)ci[e] - r;
+				if (d < INT32_MIN || d > INT32_MAX) return false;
+				U.push_back(d);
+			}
+		std::sort(U.begin(), U.end());
+		U.erase(std::unique(U.begin(), U.end()), U.end());
+		if ((int64_t)U.size() > kSliceRows) return false;
+		ptr[s] = total;
+		total += (int64_t)U.size() * kSliceRows;
+		if (total > budget) return false;
+		const size_t q0 = off.size();
+		for (int64_t d : U) off.push_back((int32_t)d);
+		mask.resize(off.size(), 0);
+		for (int64_t r = r0; r < r1; ++r) {
+			size_t j = 0;
+			for (I e = rp[r]; e < rp[r + 1]; ++e) {
+				const int64_t d = (int64_t)ci[e] - r;
+				while (U[j] != d) ++j;
+				mask[q0 + j] |= 1ull << (r - r0);
+			}
+		}
+	}
+	ptr[ns] = total;
+	return true;
+}
+
+template<class I, class V>
+static void sdia_fill_t(fsx_ctx * ctx, fsx_csr * A) {
+	const int64_t nslots = ((A->nrows + kSliceRows - 1) / kSliceRows) * kSliceRows;
+	if (nslots == 0) return;
+	const unsigned g = (unsigned)((nslots + 255) / 256);
+	hipLaunchKernelGGL((sdia_scatter_kernel<I, V>), dim3(g), dim3(256), 0, ctx->stream,
+	                   A->nrows, nslots, (const I *)A->rowptr(), (const I *)A->colind(),
+	                   (const V *)A->vals->p, (const int64_t *)A->idx->dg_ptr->p,
+	                   (const int32_t *)A->idx->dg_off->p, (V *)A->dg_vals->p);
+	FSX_HIP(hipGetLastError());
+	ctx->launches++;
+}
+
+// Fill A's slice-offset values from its CSR values (stream-ordered).
+static void sdia_fill(fsx_ctx * ctx, fsx_csr * A) {
+	if (A->it == FSX_I32) {
+		if (A->vt == FSX_F64) sdia_fill_t<int32_t, double>(ctx, A);
+		else sdia_fill_t<int32_t, float>(ctx, A);
+	}
+	else {
+		if (A->vt == FSX_F64) sdia_fill_t<int64_t, double>(ctx, A);
+		else sdia_fill_t<int64_t, float>(ctx, A);
+	}
+}
+
+bool ensure_sdia(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
+                 const void * host_colind, bool check_budget) {
+	auto & X = *A->idx;
+	if (!X.dg_ptr) {
+		if (A->nrows == 0) return false;
+		std::vector<int64_t> ptr;
+		std::vector<int32_t> off;
+		std::vector<uint64_t> mask;
+		const bool ok =
+			A->it == FSX_I32
+				? sdia_analyze<int32_t>(A->nrows, A->nnz, (const int32_t *)host_rowptr,
+				                        (const int32_t *)host_colind, check_budget, ptr,
+				                        off, mask)
+				: sdia_analyze<int64_t>(A->nrows, A->nnz, (const int64_t *)host_rowptr,
+				                        (const int64_t *)host_colind, check_budget, ptr,
+				                        off, mask);
+		if (!ok) return false;
+		X.dg_nnz = ptr.back();
+		X.dg_ptr = std::make_unique<dbuf>(ptr.size() * sizeof(int64_t));
+		X.dg_off = std::make_unique<dbuf>(std::max<size_t>(1, off.size()) * sizeof(int32_t));
+		X.dg_mask = std::make_unique<dbuf>(std::max<size_t>(1, mask.size()) * sizeof(uint64_t));
+		FSX_HIP(hipMemcpyAsync(X.dg_ptr->p, ptr.data(), ptr.size() * sizeof(int64_t),
+		                       hipMemcpyHostToDevice, ctx->stream));
+		if (!off.empty()) {
+			FSX_HIP(hipMemcpyAsync(X.dg_off->p, off.data(), off.size() * sizeof(int32_t),
+			                       hipMemcpyHostToDevice, ctx->stream));
+			FSX_HIP(hipMemcpyAsync(X.dg_mask->p, mask.data(), mask.size() * sizeof(uint64_t),
+			                       hipMemcpyHostToDevice, ctx->stream));
+		}
+		FSX_HIP(hipStreamSynchronize(ctx->stream)); // host vectors go out of scope
+	}
+	if (!A->dg_vals) {
+		const size_t vsz = A->vt == FSX_F64 ? 8 : 4;
+		A->dg_vals = std::make_unique<dbuf>(std::max<int64_t>(1, X.dg_nnz) * vsz);
+		sdia_fill(ctx, A);
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+	}
+	return true;
+}
+
 void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A) {
+	if (A->idx->dg_ptr && A->dg_vals) sdia_fill(ctx, A);
 	if (!(A->idx->sl_ptr && A->sl_vals)) return;
 	if (A->it == FSX_I32) {
 		if (A->vt == FSX_F64) refresh_sliced_t<int32_t, double>(ctx, A);
@@ -197,11 +345,13 @@ void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A) {
 	}
 }
 
-// Short rows of similar length -> sliced rows (padding <= 10%, addressable
-// with the index type); other short-row matrices -> LDS-staged row blocks;
-// long rows -> LANES lanes per row with LANES the next power of two of the
-// mean row length (2..64).
-void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr) {
+// Short rows whose slices share few column offsets (stencils) -> slice
+// offsets; other short rows of similar length -> sliced rows (padding <= 10%,
+// addressable with the index type); other short-row matrices -> LDS-staged
+// row blocks; long rows -> LANES lanes per row with LANES the next power of
+// two of the mean row length (2..64).
+void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
+                 const void * host_colind) {
 	int64_t maxr = 0;
 	for (int64_t i = 0; i < A->nrows; ++i) {
 		int64_t a, b;
@@ -219,6 +369,11 @@ void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr) {
 	A->avg_row = A->nrows ? (double)A->nnz / (double)A->nrows : 0.0;
 	const double per_row_budget = (double)kStreamCap / kStreamRows;
 	if (A->avg_row <= per_row_budget) {
+		if (host_colind && ensure_sdia(ctx, A, host_rowptr, host_colind, true)) {
+			A->plan_kind = 3;
+			A->plan_lanes = 0;
+			return;
+		}
 		const int64_t padded =
 			A->it == FSX_I32
 				? slice_offsets<int32_t>(A->nrows, (const int32_t *)host_rowptr).back()
@@ -438,7 +593,7 @@ int fsx_csr_create(fsx_ctx * ctx, int64_t nrows, int64_t ncols, int64_t nnz,
 			                       hipMemcpyHostToDevice, ctx->stream));
 		}
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
-		choose_plan(ctx, A.get(), rowptr);
+		choose_plan(ctx, A.get(), rowptr, colind);
 		ctx_retain(ctx);
 		*out = A.release();
 	});
@@ -498,6 +653,26 @@ int fsx_csr_convert(fsx_ctx * ctx, const fsx_csr * A, fsx_dtype vt,
 		}
 		else if (B->plan_kind == 2)
 			B->plan_kind = 0;
+		if (A->dg_vals) {
+			// same slice-offset layout, values converted (padding stays zero)
+			const size_t m = (size_t)std::max<int64_t>(1, A->idx->dg_nnz);
+			B->dg_vals = std::make_unique<dbuf>(m * vsz);
+			const unsigned gs = (unsigned)((m + 255) / 256);
+			if (A->vt == FSX_F64 && vt == FSX_F32)
+				hipLaunchKernelGGL((cast_kernel<double, float>), dim3(gs), dim3(256), 0,
+				                   ctx->stream, m, (const double *)A->dg_vals->p,
+				                   (float *)B->dg_vals->p);
+			else if (A->vt == FSX_F32 && vt == FSX_F64)
+				hipLaunchKernelGGL((cast_kernel<float, double>), dim3(gs), dim3(256), 0,
+				                   ctx->stream, m, (const float *)A->dg_vals->p,
+				                   (double *)B->dg_vals->p);
+			else
+				FSX_HIP(hipMemcpyAsync(B->dg_vals->p, A->dg_vals->p, m * vsz,
+				                       hipMemcpyDeviceToDevice, ctx->stream));
+			FSX_HIP(hipGetLastError());
+		}
+		else if (B->plan_kind == 3)
+			B->plan_kind = 0;
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
 		ctx_retain(ctx);
 		*out = B.release();
@@ -537,7 +712,7 @@ int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes) {
 int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes) {
 	return guard([&] {
 		FSX_REQUIRE(A, "null matrix");
-		FSX_REQUIRE(kind == 0 || kind == 1 || kind == 2, "plan kind must be 0, 1 or 2");
+		FSX_REQUIRE(kind >= 0 && kind <= 3, "plan kind must be 0, 1, 2 or 3");
 		if (kind == 1)
 			FSX_REQUIRE(lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 ||
 			                lanes == 32 || lanes == 64,
@@ -545,6 +720,20 @@ int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes) {
 		if (kind == 2) {
 			FSX_REQUIRE(A->nrows > 0, "sliced rows need a non-empty matrix");
 			ensure_sliced(A->ctx, A, nullptr);
+		}
+		if (kind == 3 && !(A->idx->dg_ptr && A->dg_vals)) {
+			FSX_REQUIRE(A->nrows > 0, "slice offsets need a non-empty matrix");
+			fsx_ctx * ctx = A->ctx;
+			const size_t isz = A->it == FSX_I32 ? 4 : 8;
+			std::vector<char> hrp((A->nrows + 1) * isz), hci(std::max<int64_t>(1, A->nnz) * isz);
+			FSX_HIP(hipMemcpyAsync(hrp.data(), A->rowptr(), hrp.size(), hipMemcpyDeviceToHost,
+			                       ctx->stream));
+			if (A->nnz)
+				FSX_HIP(hipMemcpyAsync(hci.data(), A->colind(), A->nnz * isz,
+				                       hipMemcpyDeviceToHost, ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			FSX_REQUIRE(ensure_sdia(ctx, A, hrp.data(), hci.data(), false),
+			            "matrix structure does not fit the slice-offset layout");
 		}
 		A->plan_kind = kind;
 		A->plan_lanes = kind == 1 ? lanes : 0;

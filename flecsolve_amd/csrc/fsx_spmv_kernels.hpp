@@ -242,6 +242,62 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sliced_kernel(
 	epi.block_end(d, blockIdx.x, gridDim.x);
 }
 
+// Slice-offset layout: wave = slice of 64 rows, lane = row.  The slice's
+// offsets and lane masks are wave-uniform (scalar loads); values stream
+// entry-major; the gathers x[row + off] of a wave are contiguous.  A lane
+// skips the offsets its row does not have, so each row's sum runs over its
+// entries in CSR (ascending column) order: bit-identical to the serial loop.
+template<class Epi, class ACC, class V, class G>
+__global__ __launch_bounds__(kSpmvThreads) void spmv_sdia_kernel(
+	int64_t nrows, const int64_t * __restrict__ dg_ptr,
+	const int32_t * __restrict__ dg_off, const uint64_t * __restrict__ dg_mask,
+	const V * __restrict__ dg_val, G x, Epi epi) {
+	if (epi.skip()) return;
+	x.init();
+	const int lane = threadIdx.x & (kSliceRows - 1);
+	double d[Epi::NV];
+#pragma unroll
+	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
+	constexpr int kWaves = kSpmvThreads / kSliceRows;
+	const int64_t nslices = (nrows + kSliceRows - 1) / kSliceRows;
+	const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSliceRows));
+	const int64_t sstride = (int64_t)gridDim.x * kWaves;
+	for (int64_t s = (int64_t)blockIdx.x * kWaves + wave; s < nslices; s += sstride) {
+		const int64_t r = s * kSliceRows + lane;
+		const int64_t base = dg_ptr[s];
+		const int w = (int)((dg_ptr[s + 1] - base) / kSliceRows);
+		const int64_t q0 = base / kSliceRows;
+		const V * vp = dg_val + base + lane;
+		ACC sum = ACC(0);
+		typename G::value_type xself{};
+		bool has_self = false;
+		for (int j0 = 0; j0 < w; j0 += kSliceUnroll) {
+			int32_t off[kSliceUnroll];
+			uint64_t msk[kSliceUnroll];
+			V v[kSliceUnroll];
+#pragma unroll
+			for (int k = 0; k < kSliceUnroll; ++k)
+				if (j0 + k < w) {
+					off[k] = dg_off[q0 + j0 + k];
+					msk[k] = dg_mask[q0 + j0 + k];
+					v[k] = ldnt(vp + (int64_t)(j0 + k) * kSliceRows);
+				}
+#pragma unroll
+			for (int k = 0; k < kSliceUnroll; ++k)
+				if (j0 + k < w && ((msk[k] >> lane) & 1)) {
+					const auto xv = x(r + off[k]);
+					if (off[k] == 0) {
+						xself = xv;
+						has_self = true;
+					}
+					sum += (ACC)v[k] * (ACC)xv;
+				}
+		}
+		if (r < nrows) epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
+	}
+	epi.block_end(d, blockIdx.x, gridDim.x);
+}
+
 // Epilogues that leave per-block partials cap the grid (fewer partials for
 // the finalize kernel); plain products use one row per thread.
 inline unsigned sliced_blocks(int64_t nrows, bool reduces = true) {
@@ -257,7 +313,16 @@ void spmv_launch_gather(hipStream_t st, const fsx_csr * A, const G & x,
 	const I * ci = (const I *)A->colind();
 	const V * vv = (const V *)A->vals->p;
 	if (A->nrows == 0) return;
-	if (A->plan_kind == 2) {
+	if (A->plan_kind == 3) {
+		const unsigned g = sliced_blocks(A->nrows, Epi::reduces);
+		hipLaunchKernelGGL((spmv_sdia_kernel<Epi, ACC, V, G>), dim3(g),
+		                   dim3(kSpmvThreads), 0, st, A->nrows,
+		                   (const int64_t *)A->idx->dg_ptr->p,
+		                   (const int32_t *)A->idx->dg_off->p,
+		                   (const uint64_t *)A->idx->dg_mask->p,
+		                   (const V *)A->dg_vals->p, x, epi);
+	}
+	else if (A->plan_kind == 2) {
 		const unsigned g = sliced_blocks(A->nrows, Epi::reduces);
 		hipLaunchKernelGGL((spmv_sliced_kernel<Epi, ACC, V, I, G>), dim3(g),
 		                   dim3(kSpmvThreads), 0, st, A->nrows,
@@ -298,7 +363,7 @@ void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
 }
 
 inline unsigned spmv_blocks(const fsx_csr * A) {
-	if (A->plan_kind == 2) return sliced_blocks(A->nrows);
+	if (A->plan_kind == 2 || A->plan_kind == 3) return sliced_blocks(A->nrows);
 	if (A->plan_kind == 0)
 		return (unsigned)((A->nrows + kStreamRows - 1) / kStreamRows);
 	const int rpb = kSpmvThreads / A->plan_lanes;

@@ -156,7 +156,11 @@ int fsx_csr_destroy(fsx_csr * A);
 int fsx_csr_info(const fsx_csr * A, int64_t * nrows, int64_t * ncols,
                  int64_t * nnz, int * itype, int * vtype);
 /* Row-mapping selected for A: 0 = LDS-staged row blocks, 1 = sub-wavefront
- * vector per row; `lanes` is the vector width. */
+ * vector per row (`lanes` is the vector width), 2 = sliced rows (64-row
+ * slices, entry-major, column per entry), 3 = slice offsets (64-row slices
+ * storing the distinct column offsets of their rows and lane masks instead
+ * of a column per entry; chosen for stencil-like structure).  Every mapping
+ * sums each row serially in CSR order except 1. */
 int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes);
 int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes);
 int fsx_csr_download(fsx_ctx * ctx, const fsx_csr * A, void * rowptr,
@@ -294,7 +298,7 @@ int fsx_batch_create(fsx_ctx * ctx, const fsx_csr * A, int64_t nsys,
                      const int64_t * sys_rows, fsx_batch ** out);
 int fsx_batch_destroy(fsx_batch * B);
 /* Execution configuration: workgroup size, row mapping (0 sliced rows,
- * 1 `lanes` lanes per CSR row). */
+ * 1 `lanes` lanes per CSR row, 2 slice offsets). */
 int fsx_batch_config(const fsx_batch * B, int * wg, int * mapping, int * lanes);
 int fsx_batch_set_config(fsx_batch * B, int wg, int mapping, int lanes);
 /* Autotuner: times the candidate configurations on `sample` systems for

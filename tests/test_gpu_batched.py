@@ -43,12 +43,16 @@ def check_against_oracle(rows, rp, ci, v, b, xg, it, st, rtol=1e-10, slack=2):
         assert rel(xg[sl], xo) <= 1e-10, s
 
 
-@pytest.mark.parametrize("cfg", [(256, 0, 1), (1024, 0, 1), (64, 0, 1),
-                                 (256, 1, 2), (512, 1, 4), (256, 1, 8)])
+@pytest.mark.parametrize("cfg", [(256, 2, 1), (1024, 2, 1), (256, 0, 1), (1024, 0, 1),
+                                 (64, 0, 1), (256, 1, 2), (512, 1, 4), (256, 1, 8)])
 def test_batched_cg_matches_oracle(ctx, cfg):
     e, rows, rp, ci, v, b = make()
     A = ctx.csr(rp, ci, v)
+    assert A.plan() == (3, 0)
+    A.set_plan(2)   # also build the sliced layout for mapping 0
+    A.set_plan(3)
     B = fsx.Batch(A, rows)
+    assert B.config()[1] == 2
     B.set_config(*cfg)
     assert B.config() == cfg
     d = A.diag_inv()
@@ -67,7 +71,7 @@ def test_batched_same_result_any_order_or_config(ctx):
     bd = ctx.vec(b)
     outs = []
     for wg in (256, 1024):
-        B.set_config(wg, 0, 1)
+        B.set_config(wg, 2, 1)
         x = ctx.zeros(b.size)
         it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=500, rtol=1e-10, use_zero_guess=True),
                           dinv=d)
@@ -91,7 +95,7 @@ def test_batched_autotune(ctx):
     ms = B.tune(bd, x, d, sample=16, iters=10)
     assert ms > 0
     wg, mapping, lanes = B.config()
-    assert wg in (256, 512, 1024) and mapping in (0, 1) and lanes in (1, 2, 4, 8)
+    assert wg in (256, 512, 1024) and mapping in (1, 2) and lanes in (1, 2, 4, 8)
     x = ctx.zeros(b.size)
     it, st, rs = B.cg(bd, x, fsx.Settings(maxiter=1000, rtol=1e-10, use_zero_guess=True),
                       dinv=d)

@@ -160,17 +160,20 @@ def test_ir_reaches_1e10_and_matches_fp64(ctx, sigma):
     assert rel(x, xo) < 1e-6
 
 
-def test_grid_stride_sizes_iteration_matched(ctx):
-    """More rows than one sliced-SpMV grid covers (> 8192 x 256), so each
+@pytest.mark.parametrize("plan", [3, 2])
+def test_grid_stride_sizes_iteration_matched(ctx, plan):
+    """More rows than one capped SpMV grid covers (> 8192 x 256), so each
     thread owns several rows: SpMV+dot and 25 CG iterations against the
-    oracle, iteration-matched (both stop at maxiter)."""
+    oracle, iteration-matched (both stop at maxiter), on the slice-offset
+    and the sliced layouts."""
     n = 132
     rp, ci, v, Ao = diffusion(n)
     N = n ** 3
     assert N > 8192 * 256
     b = P.uniform(N, 2002)
     A = ctx.csr(rp, ci, v)
-    assert A.plan() == (2, 0)
+    assert A.plan() == (3, 0)
+    A.set_plan(plan)
     bd = ctx.vec(b)
     w = ctx.zeros(N)
     pw = fsx.spmv_dot(A, bd, w)
@@ -185,3 +188,24 @@ def test_grid_stride_sizes_iteration_matched(ctx):
     assert info.reason == io.reason == "diverged_iters"
     np.testing.assert_allclose(fsx.cg_history(ctx), ho, rtol=1e-10)
     assert rel(xd.get(), xo) <= 1e-12
+
+
+def test_layouts_give_identical_trajectories(ctx):
+    """The sliced and slice-offset layouts both sum rows serially in CSR
+    order, so the fused CG produces bit-identical iterates and counts."""
+    n = 40
+    rp, ci, v, Ao = diffusion(n)
+    b = P.uniform(n ** 3, 77)
+    A = ctx.csr(rp, ci, v)
+    d = A.diag_inv()
+    bd = ctx.vec(b)
+    out = []
+    for plan in (3, 2, 0):
+        A.set_plan(plan)
+        x = ctx.zeros(n ** 3)
+        info = fsx.cg_solve(A, bd, x, fsx.Settings(maxiter=2000, rtol=1e-10,
+                                                   use_zero_guess=True), dinv=d)
+        out.append((info.iters, x.get()))
+    for it, x in out[1:]:
+        assert it == out[0][0]
+        np.testing.assert_array_equal(x, out[0][1])

@@ -55,7 +55,7 @@ def test_structure_round_trip(ctx):
     np.testing.assert_array_equal(r2, rp)
     np.testing.assert_array_equal(c2, ci)
     np.testing.assert_array_equal(v2, v)
-    assert A.plan() == (2, 0)  # sliced rows for the short, uniform rows
+    assert A.plan() == (3, 0)  # slice offsets for the stencil structure
 
 
 @pytest.mark.parametrize("n", [1, 7, 64])
@@ -78,31 +78,60 @@ def test_convdiff_bit_exact(ctx):
 
 @pytest.mark.parametrize("n", [3, 33, 70])
 def test_every_plan_on_stencil(ctx, n):
-    # plans 0 (LDS-staged) and 2 (sliced) sum rows serially: bit-exact;
-    # plan 1 (lanes per row) sums in a tree: within rounding
+    # plans 0 (LDS-staged), 2 (sliced) and 3 (slice offsets) sum rows
+    # serially: bit-exact; plan 1 (lanes per row) sums in a tree: rounding
     k = P.lognormal_field(n, 1.0, 8.0, 11)
     rp, ci, v = P.diffusion3d(n, k)
     A = ctx.csr(rp, ci, v)
+    assert A.plan() == ((3, 0) if n > 3 else (0, 0))  # 27 rows: too much padding
     x = np.random.default_rng(3).standard_normal(n ** 3)
     ref = O.Csr(rp, ci, v).spmv(x)
     xd, yd = ctx.vec(x), ctx.zeros(n ** 3)
-    for plan in [(0, 0), (2, 0), (1, 2), (1, 8), (1, 64)]:
+    for plan in [(0, 0), (2, 0), (3, 0), (1, 2), (1, 8), (1, 64)]:
         A.set_plan(*plan)
         A.spmv(xd, yd)
         if plan[0] != 1:
             np.testing.assert_array_equal(yd.get(), ref, err_msg=str(plan))
         else:
             assert rel(yd.get(), ref) <= 1e-13, plan
-    # fused residual and SpMV+dot on the sliced layout
-    A.set_plan(2)
+    # fused residual and SpMV+dot on both slice layouts
     b = np.random.default_rng(4).standard_normal(n ** 3)
     r = ctx.zeros(n ** 3)
     bd = ctx.vec(b)
-    A.residual(bd, xd, r)
-    np.testing.assert_array_equal(r.get(), b - ref)
-    pw = fsx.spmv_dot(A, xd, yd)
-    np.testing.assert_array_equal(yd.get(), ref)
-    assert abs(pw - O.dot(ref, x)) <= 1e-12 * np.abs(ref * x).sum()
+    for plan in (2, 3):
+        A.set_plan(plan)
+        A.residual(bd, xd, r)
+        np.testing.assert_array_equal(r.get(), b - ref)
+        pw = fsx.spmv_dot(A, xd, yd)
+        np.testing.assert_array_equal(yd.get(), ref)
+        assert abs(pw - O.dot(ref, x)) <= 1e-12 * np.abs(ref * x).sum()
+    # FP32 copy shares the layout
+    A.set_plan(3)
+    A32 = A.convert(fsx.F32)
+    assert A32.plan() == (3, 0)
+    x32, y32 = ctx.vec(x.astype(np.float32)), ctx.zeros(n ** 3, fsx.F32)
+    A32.spmv(x32, y32)
+    ref32 = O.Csr(rp, ci, v.astype(np.float32).astype(np.float64)).spmv(
+        x.astype(np.float32).astype(np.float64))
+    assert rel(y32.get().astype(np.float64), ref32) <= 1e-6
+
+
+def test_slice_offsets_fallback_and_block_diagonal(ctx):
+    # unsorted columns within a row: not the slice-offset layout, still exact
+    rp, ci, v = P.poisson2d(20)
+    ci2 = ci.copy()
+    ci2[rp[5]:rp[6]] = ci2[rp[5]:rp[6]][::-1]
+    v2 = v.copy()
+    v2[rp[5]:rp[6]] = v2[rp[5]:rp[6]][::-1]
+    A = check_spmv(ctx, rp, ci2, v2, exact=True)
+    assert A.plan()[0] != 3
+    with pytest.raises(fsx.FsxError):
+        A.set_plan(3)
+    # block-diagonal batch of stencils of different sizes: slices that
+    # straddle two systems carry both offset sets
+    e, rows, brp, bci, bv = P.batch(16, 4, 11, 4001, 4100)
+    B = check_spmv(ctx, brp, bci, bv, exact=True)
+    assert B.plan() == (3, 0)
 
 
 @pytest.mark.parametrize("itype", [np.int32, np.int64])
@@ -120,6 +149,9 @@ def test_ragged_rows_both_mappings(ctx, itype):
     A.set_plan(2)
     A.spmv(xd, yd)
     np.testing.assert_array_equal(yd.get(), ref)
+    # random columns: too many distinct offsets per slice for plan 3
+    with pytest.raises(fsx.FsxError):
+        A.set_plan(3)
     for lanes in (2, 4, 8, 16, 32, 64):
         A.set_plan(1, lanes)
         A.spmv(xd, yd)
