@@ -217,25 +217,31 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sliced_kernel(
 		// (epilogues that need it then do not read x again)
 		typename G::value_type xself{};
 		bool has_self = false;
+		// Branch-free body as in the slice-offset kernel: all loads of a tile
+		// first, padding (column -1) and slots past the width excluded by a
+		// select.
 		for (int j0 = 0; j0 < w; j0 += kSliceUnroll) {
 			I c[kSliceUnroll];
 			V v[kSliceUnroll];
+			typename G::value_type xv[kSliceUnroll];
 #pragma unroll
-			for (int k = 0; k < kSliceUnroll; ++k)
-				if (j0 + k < w) {
-					c[k] = ldnt(cp + (j0 + k) * kSliceRows);
-					v[k] = ldnt(vp + (j0 + k) * kSliceRows);
-				}
+			for (int k = 0; k < kSliceUnroll; ++k) {
+				const int j = min(j0 + k, w - 1);
+				c[k] = ldnt(cp + j * kSliceRows);
+				v[k] = ldnt(vp + j * kSliceRows);
+				if (j0 + k >= w) c[k] = (I)-1;
+			}
 #pragma unroll
-			for (int k = 0; k < kSliceUnroll; ++k)
-				if (j0 + k < w && c[k] >= 0) {
-					const auto xv = x(c[k]);
-					if ((int64_t)c[k] == r) {
-						xself = xv;
-						has_self = true;
-					}
-					sum += (ACC)v[k] * (ACC)xv;
+			for (int k = 0; k < kSliceUnroll; ++k) xv[k] = x(c[k] >= 0 ? c[k] : (I)0);
+#pragma unroll
+			for (int k = 0; k < kSliceUnroll; ++k) {
+				const ACC t = sum + (ACC)v[k] * (ACC)xv[k];
+				sum = c[k] >= 0 ? t : sum;
+				if ((int64_t)c[k] == r) {
+					xself = xv[k];
+					has_self = true;
 				}
+			}
 		}
 		epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
 	}
@@ -271,27 +277,33 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sdia_kernel(
 		ACC sum = ACC(0);
 		typename G::value_type xself{};
 		bool has_self = false;
+		// Branch-free body: every load of a tile is issued before the first
+		// use (slots past the width re-read the last column, absent lanes
+		// gather x[0]); a select keeps `sum` unchanged for absent entries,
+		// bitwise the same as skipping them.
 		for (int j0 = 0; j0 < w; j0 += kSliceUnroll) {
 			int32_t off[kSliceUnroll];
-			uint64_t msk[kSliceUnroll];
+			bool pr[kSliceUnroll];
 			V v[kSliceUnroll];
+			typename G::value_type xv[kSliceUnroll];
 #pragma unroll
-			for (int k = 0; k < kSliceUnroll; ++k)
-				if (j0 + k < w) {
-					off[k] = dg_off[q0 + j0 + k];
-					msk[k] = dg_mask[q0 + j0 + k];
-					v[k] = ldnt(vp + (int64_t)(j0 + k) * kSliceRows);
-				}
+			for (int k = 0; k < kSliceUnroll; ++k) {
+				const int j = min(j0 + k, w - 1);
+				off[k] = dg_off[q0 + j];
+				pr[k] = j0 + k < w && ((dg_mask[q0 + j] >> lane) & 1);
+				v[k] = ldnt(vp + (int64_t)j * kSliceRows);
+			}
 #pragma unroll
-			for (int k = 0; k < kSliceUnroll; ++k)
-				if (j0 + k < w && ((msk[k] >> lane) & 1)) {
-					const auto xv = x(r + off[k]);
-					if (off[k] == 0) {
-						xself = xv;
-						has_self = true;
-					}
-					sum += (ACC)v[k] * (ACC)xv;
+			for (int k = 0; k < kSliceUnroll; ++k) xv[k] = x(pr[k] ? r + off[k] : 0);
+#pragma unroll
+			for (int k = 0; k < kSliceUnroll; ++k) {
+				const ACC t = sum + (ACC)v[k] * (ACC)xv[k];
+				sum = pr[k] ? t : sum;
+				if (pr[k] && off[k] == 0) {
+					xself = xv[k];
+					has_self = true;
 				}
+			}
 		}
 		if (r < nrows) epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
 	}
