@@ -240,11 +240,13 @@ __device__ __forceinline__ T block_sum_256(T v, T * sh) {
 // block would write back the XCD's L2 each time).  Stage 2: a one-block
 // kernel, launched next in stream order (the kernel boundary makes the
 // partials visible), sums them in a fixed order.
+// Blocks of up to 1024 threads (16 waves); the waves' sums are added in
+// wave order.
 template<int NV>
 __device__ __forceinline__ void store_block_partials(const double (&d)[NV],
                                                      double * partials,
                                                      unsigned blk) {
-	__shared__ double sh[NV][4];
+	__shared__ double sh[NV][16];
 #pragma unroll
 	for (int i = 0; i < NV; ++i) {
 		double s = wave_sum(d[i]);
@@ -252,10 +254,13 @@ __device__ __forceinline__ void store_block_partials(const double (&d)[NV],
 	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
+		const int nw = (int)(blockDim.x >> 6);
 #pragma unroll
-		for (int i = 0; i < NV; ++i)
-			partials[(size_t)blk * NV + i] =
-				((sh[i][0] + sh[i][1]) + sh[i][2]) + sh[i][3];
+		for (int i = 0; i < NV; ++i) {
+			double t = sh[i][0];
+			for (int q = 1; q < nw; ++q) t += sh[i][q];
+			partials[(size_t)blk * NV + i] = t;
+		}
 	}
 }
 

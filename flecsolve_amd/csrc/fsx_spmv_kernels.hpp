@@ -253,8 +253,8 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sliced_kernel(
 // entry-major; the gathers x[row + off] of a wave are contiguous.  A lane
 // skips the offsets its row does not have, so each row's sum runs over its
 // entries in CSR (ascending column) order: bit-identical to the serial loop.
-template<class Epi, class ACC, class V, class G>
-__global__ __launch_bounds__(kSpmvThreads) void spmv_sdia_kernel(
+template<class Epi, class ACC, class V, class G, int BT>
+__global__ __launch_bounds__(BT) void spmv_sdia_kernel(
 	int64_t nrows, const int64_t * __restrict__ dg_ptr,
 	const int32_t * __restrict__ dg_off, const uint64_t * __restrict__ dg_mask,
 	const V * __restrict__ dg_val, G x, Epi epi) {
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sdia_kernel(
 	double d[Epi::NV];
 #pragma unroll
 	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
-	constexpr int kWaves = kSpmvThreads / kSliceRows;
+	constexpr int kWaves = BT / kSliceRows;
 	const int64_t nslices = (nrows + kSliceRows - 1) / kSliceRows;
 	const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSliceRows));
 	const int64_t sstride = (int64_t)gridDim.x * kWaves;
@@ -317,6 +317,28 @@ inline unsigned sliced_blocks(int64_t nrows, bool reduces = true) {
 	return (unsigned)(reduces ? std::min<int64_t>(g, kSlicedMaxBlocks) : g);
 }
 
+// Slice-offset launches.  Plain products: 256-thread blocks, one slice per
+// wave.  Reducing epilogues (one partial per block), measured on the 256^3
+// CG iteration: FP64 values are fastest with one slice per wave in
+// 512-thread blocks (249-250 us vs 278 us grid-striding); FP32 values, whose
+// slices are half the bytes, are fastest in 256-thread blocks capped at 8192
+// and striding (137 us vs 150-165 us one slice per wave: block dispatch and
+// the end-of-block barrier dominate short blocks).
+template<class V>
+struct sdia_shape {
+	static constexpr bool f64 = std::is_same_v<V, double>;
+	static constexpr int reduce_threads = f64 ? 512 : 256;
+	static constexpr unsigned max_blocks = f64 ? (1u << 17) : 8192u;
+};
+
+inline unsigned sdia_blocks(int64_t nrows, bool reduces, int reduce_threads,
+                            unsigned max_blocks) {
+	const int64_t ns = (nrows + kSliceRows - 1) / kSliceRows;
+	if (!reduces) return (unsigned)((ns * kSliceRows + kSpmvThreads - 1) / kSpmvThreads);
+	const int64_t per = reduce_threads / kSliceRows;
+	return (unsigned)std::min<int64_t>((ns + per - 1) / per, max_blocks);
+}
+
 // Launch the selected mapping for matrix A with gather policy G.
 template<class Epi, class ACC, class V, class I, class G>
 void spmv_launch_gather(hipStream_t st, const fsx_csr * A, const G & x,
@@ -326,10 +348,12 @@ void spmv_launch_gather(hipStream_t st, const fsx_csr * A, const G & x,
 	const V * vv = (const V *)A->vals->p;
 	if (A->nrows == 0) return;
 	if (A->plan_kind == 3) {
-		const unsigned g = sliced_blocks(A->nrows, Epi::reduces);
-		hipLaunchKernelGGL((spmv_sdia_kernel<Epi, ACC, V, G>), dim3(g),
-		                   dim3(kSpmvThreads), 0, st, A->nrows,
-		                   (const int64_t *)A->idx->dg_ptr->p,
+		using SH = sdia_shape<V>;
+		constexpr int BT = Epi::reduces ? SH::reduce_threads : kSpmvThreads;
+		const unsigned g = sdia_blocks(A->nrows, Epi::reduces, SH::reduce_threads,
+		                               SH::max_blocks);
+		hipLaunchKernelGGL((spmv_sdia_kernel<Epi, ACC, V, G, BT>), dim3(g), dim3(BT), 0,
+		                   st, A->nrows, (const int64_t *)A->idx->dg_ptr->p,
 		                   (const int32_t *)A->idx->dg_off->p,
 		                   (const uint64_t *)A->idx->dg_mask->p,
 		                   (const V *)A->dg_vals->p, x, epi);
@@ -375,7 +399,14 @@ void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
 }
 
 inline unsigned spmv_blocks(const fsx_csr * A) {
-	if (A->plan_kind == 2 || A->plan_kind == 3) return sliced_blocks(A->nrows);
+	if (A->plan_kind == 3) {
+		if (A->vt == FSX_F64)
+			return sdia_blocks(A->nrows, true, sdia_shape<double>::reduce_threads,
+			                   sdia_shape<double>::max_blocks);
+		return sdia_blocks(A->nrows, true, sdia_shape<float>::reduce_threads,
+		                   sdia_shape<float>::max_blocks);
+	}
+	if (A->plan_kind == 2) return sliced_blocks(A->nrows);
 	if (A->plan_kind == 0)
 		return (unsigned)((A->nrows + kStreamRows - 1) / kStreamRows);
 	const int rpb = kSpmvThreads / A->plan_lanes;

@@ -12,7 +12,8 @@ from pathlib import Path
 import numpy as np
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "libfsx.so"
+# FSX_LIB overrides the library path (e.g. to load another build of libfsx)
+LIB_PATH = Path(os.environ["FSX_LIB"]) if os.environ.get("FSX_LIB") else _PKG / "libfsx.so"
 
 F64, F32 = 0, 1
 I32, I64 = 0, 1
