@@ -162,15 +162,26 @@ def main():
     log(f"spmv fp32: {ms32:.4f} ms, {gbs32:.0f} GB/s")
     del x32, y32, y
     # HBM bytes per launch from the committed rocprofv3 PMC passes (FETCH_SIZE,
-    # WRITE_SIZE scaled by the copy-kernel calibration) for this workload
-    traffic, traffic_src = None, None
-    pmc = ROOT / "profiles" / "spmv_sliced_256cube_pmc.json"
-    if n == 256 and abs(a.sigma - 1.0) < 1e-12 and pmc.exists():
-        traffic = round(json.loads(pmc.read_text())["spmv_fp64"]["traffic_bytes"])
-        traffic_src = "profiles/spmv_sliced_256cube_pmc.md"
+    # WRITE_SIZE scaled by the copy-kernel calibration) for this workload, and
+    # the bytes the chosen layout must move (below the CSR-based algorithmic
+    # figure for the slice-offset layout, which stores no column per entry)
+    plan = A64.plan()
+    kernels = {0: "spmv_stream_kernel (LDS-staged CSR)", 1: "spmv_vector_kernel",
+               2: "spmv_sliced_kernel (sliced rows)",
+               3: "spmv_sdia_kernel (slice-offset layout)"}
+    pmc_files = {2: "spmv_sliced_256cube_pmc", 3: "spmv_sdia_256cube_pmc"}
+    traffic, traffic_src, layout64, layout32 = None, None, None, None
+    if n == 256 and abs(a.sigma - 1.0) < 1e-12 and plan[0] in pmc_files:
+        pmc = ROOT / "profiles" / (pmc_files[plan[0]] + ".json")
+        if pmc.exists():
+            d = json.loads(pmc.read_text())
+            traffic = round(d["spmv_fp64"]["traffic_bytes"])
+            traffic_src = f"profiles/{pmc_files[plan[0]]}.md"
+            layout64 = d["spmv_fp64"].get("layout_bytes")
+            layout32 = d["spmv_fp32"].get("layout_bytes")
     roofline = {
         "bound": "hbm",
-        "kernel": "spmv_sliced_kernel (FP64 values, int32 indices)",
+        "kernel": kernels[plan[0]] + ", FP64 values, int32 CSR input",
         "achieved": round(gbs64, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -179,8 +190,11 @@ def main():
         "traffic_source": traffic_src,
         "algorithmic_bytes": B64,
         "avg_ms": round(ms, 5),
+        "layout_bytes": layout64,
+        "layout_frac": (round(layout64 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                        if layout64 else None),
         "fp32_values": {"achieved": round(gbs32, 1), "avg_ms": round(ms32, 5),
-                        "algorithmic_bytes": B32},
+                        "algorithmic_bytes": B32, "layout_bytes": layout32},
         "copy_kernel_gbs": round(copy_gbs, 1),
     }
     if a.spmv_only:

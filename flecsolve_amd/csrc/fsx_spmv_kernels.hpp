@@ -35,7 +35,9 @@
 //
 // Workgroups are dealt to XCDs round-robin by block id.  Measured: keeping
 // that default order is faster than remapping blocks to give each XCD a
-// contiguous row range (all XCDs then stream neighbouring rows at once).
+// contiguous row range (all XCDs then stream neighbouring rows at once), and
+// no faster for the slice-offset kernel than grouping runs of 4-32
+// consecutive blocks on one XCD (so slices +-n rows apart share an L2).
 //
 // The kernels take an epilogue policy `Epi` that consumes each row's sum
 // (row() *adds* its dot-product terms to d: a thread may own several rows):
