@@ -130,7 +130,12 @@ __device__ __forceinline__ void wg_sum(double (&v)[NV], double * lds) {
 
 // y = A x over rows [r0, r0 + n) of the concatenation; calls f(local row,
 // row sum) once per row on exactly one thread.  `xl` points at the system's
-// first entry; columns are global ids, so gathers index xl - r0.
+// first entry; columns are global ids, so gathers index xl - r0.  The slice
+// mappings use branch-free tiles of 8 entries (all loads issued before the
+// first use; absent entries re-read the row's own x and are dropped by a
+// select, bitwise the same as skipping them).
+constexpr int kBatchTile = 4;
+
 template<class V, class X, class F>
 __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
                                         const V * v, const V * dg_v, int64_t r0,
@@ -145,10 +150,26 @@ __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
 			const int wd = (int)((a.dg_ptr[s + 1] - base) / kSliceRows);
 			const int64_t q0 = base / kSliceRows;
 			double sum = 0.0;
-			for (int j = 0; j < wd; ++j)
-				if ((a.dg_mask[q0 + j] >> lane) & 1)
-					sum += (double)dg_v[base + (int64_t)j * kSliceRows + lane] *
-					       (double)x[r + a.dg_off[q0 + j]];
+			for (int j0 = 0; j0 < wd; j0 += kBatchTile) {
+				int32_t off[kBatchTile];
+				bool pr[kBatchTile];
+				V vv[kBatchTile];
+				X xv[kBatchTile];
+#pragma unroll
+				for (int k = 0; k < kBatchTile; ++k) {
+					const int jj = min(j0 + k, wd - 1);
+					off[k] = a.dg_off[q0 + jj];
+					pr[k] = j0 + k < wd && ((a.dg_mask[q0 + jj] >> lane) & 1);
+					vv[k] = dg_v[base + (int64_t)jj * kSliceRows + lane];
+				}
+#pragma unroll
+				for (int k = 0; k < kBatchTile; ++k) xv[k] = x[pr[k] ? r + off[k] : r];
+#pragma unroll
+				for (int k = 0; k < kBatchTile; ++k) {
+					const double t = sum + (double)vv[k] * (double)xv[k];
+					sum = pr[k] ? t : sum;
+				}
+			}
 			f(i, sum);
 		}
 	}
@@ -160,9 +181,23 @@ __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
 			const int32_t base = a.sl_ptr[s];
 			const int wd = (a.sl_ptr[s + 1] - base) / kSliceRows;
 			double sum = 0.0;
-			for (int j = 0; j < wd; ++j) {
-				const int32_t c = a.sl_col[base + j * kSliceRows + lane];
-				if (c >= 0) sum += (double)sl_v[base + j * kSliceRows + lane] * (double)x[c];
+			for (int j0 = 0; j0 < wd; j0 += kBatchTile) {
+				int32_t c[kBatchTile];
+				V vv[kBatchTile];
+				X xv[kBatchTile];
+#pragma unroll
+				for (int k = 0; k < kBatchTile; ++k) {
+					const int jj = min(j0 + k, wd - 1);
+					c[k] = j0 + k < wd ? a.sl_col[base + jj * kSliceRows + lane] : -1;
+					vv[k] = sl_v[base + jj * kSliceRows + lane];
+				}
+#pragma unroll
+				for (int k = 0; k < kBatchTile; ++k) xv[k] = x[c[k] >= 0 ? (int64_t)c[k] : r];
+#pragma unroll
+				for (int k = 0; k < kBatchTile; ++k) {
+					const double t = sum + (double)vv[k] * (double)xv[k];
+					sum = c[k] >= 0 ? t : sum;
+				}
 			}
 			f(i, sum);
 		}
@@ -189,7 +224,13 @@ __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
 // ---- one CG solve of one system (cg.hh:43-139) --------------------------------
 //
 // T is the vector/storage type (double, or float for the refinement's inner
-// solve), V the matrix value type; scalars and reductions are FP64.
+// solve), V the matrix value type; scalars and reductions are FP64.  The
+// vector passes handle kBatchRows rows per thread per trip with every load
+// issued first.  x += alpha p (cg.hh:108) runs in the p pass, which reads p
+// anyway (80 instead of 88 bytes of vectors per row and iteration); the
+// iteration that ends the loop applies it in a last pass.
+constexpr int kBatchRows = 4;
+
 template<bool JAC, class T, class V>
 __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
                       const V * dg_v, int64_t r0, int64_t n, const T * b, T * x, T * r, T * z,
@@ -197,6 +238,7 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
                       bool zero_guess, double * lds, int & iters_out,
                       int & status_out, double & res_out) {
 	const int tid = threadIdx.x, nt = blockDim.x;
+	constexpr bool F64 = std::is_same_v<T, double>;
 	double acc[2];
 	double one[1];
 	// b_norm (cg.hh:55-60)
@@ -236,7 +278,7 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
 	for (int64_t i = tid; i < n; i += nt) {
 		T zi = r[i];
 		if constexpr (JAC) {
-			if constexpr (std::is_same_v<T, double>) zi = 0.0 + dinv[i] * r[i];
+			if constexpr (F64) zi = 0.0 + dinv[i] * r[i];
 			else zi = (T)((double)dinv[i] * (double)r[i]);
 			z[i] = zi;
 		}
@@ -244,8 +286,10 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
 		one[0] += (double)zi * (double)r[i];
 	}
 	wg_sum<1>(one, lds);
-	double rho = one[0], rho0;
+	double rho = one[0], rho0, alpha = 0.0;
+	bool ran = false;
 	for (int it = 0; it < maxiter; ++it) {
+		ran = true;
 		// w = A p, alpha = rho / (w.p)  (cg.hh:95-106)
 		one[0] = 0.0;
 		wg_spmv(a, sl_v, v, dg_v, r0, n, (const T *)p, [&](int64_t i, double s) {
@@ -254,28 +298,38 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
 			one[0] += (double)wi * (double)p[i];
 		});
 		wg_sum<1>(one, lds);
-		const double alpha = rho / one[0];
-		// x += alpha p; r -= alpha w; |r|; z = P r; r.z  (cg.hh:108-127)
+		alpha = rho / one[0];
+		// r -= alpha w; |r|; z = P r; r.z  (cg.hh:109-127)
 		acc[0] = 0.0;
 		acc[1] = 0.0;
-		for (int64_t i = tid; i < n; i += nt) {
-			T ri;
-			if constexpr (std::is_same_v<T, double>) {
-				x[i] = alpha * p[i] + x[i];
-				ri = (-alpha) * w[i] + r[i];
+		for (int64_t i0 = tid; i0 < n; i0 += (int64_t)kBatchRows * nt) {
+			T wv[kBatchRows], rv[kBatchRows], dv[kBatchRows];
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					wv[k] = w[i];
+					rv[k] = r[i];
+					if constexpr (JAC) dv[k] = dinv[i];
+				}
 			}
-			else {
-				x[i] = (T)(alpha * (double)p[i] + (double)x[i]);
-				ri = (T)((-alpha) * (double)w[i] + (double)r[i]);
-			}
-			r[i] = ri;
-			acc[0] += (double)ri * (double)ri;
-			if constexpr (JAC) {
-				T zi;
-				if constexpr (std::is_same_v<T, double>) zi = 0.0 + dinv[i] * ri;
-				else zi = (T)((double)dinv[i] * (double)ri);
-				z[i] = zi;
-				acc[1] += (double)ri * (double)zi;
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					T ri;
+					if constexpr (F64) ri = (-alpha) * wv[k] + rv[k];
+					else ri = (T)((-alpha) * (double)wv[k] + (double)rv[k]);
+					r[i] = ri;
+					acc[0] += (double)ri * (double)ri;
+					if constexpr (JAC) {
+						T zi;
+						if constexpr (F64) zi = 0.0 + dv[k] * ri;
+						else zi = (T)((double)dv[k] * (double)ri);
+						z[i] = zi;
+						acc[1] += (double)ri * (double)zi;
+					}
+				}
 			}
 		}
 		wg_sum<2>(acc, lds);
@@ -289,10 +343,40 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
 		rho0 = rho;
 		rho = JAC ? acc[1] : acc[0];
 		const double beta = rho / rho0;
-		// p = beta p + z  (cg.hh:130)
+		// x += alpha p; p = beta p + z  (cg.hh:108, 130)
+		for (int64_t i0 = tid; i0 < n; i0 += (int64_t)kBatchRows * nt) {
+			T pv[kBatchRows], zv[kBatchRows], xv[kBatchRows];
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					pv[k] = p[i];
+					zv[k] = zz[i];
+					xv[k] = x[i];
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					if constexpr (F64) {
+						x[i] = alpha * pv[k] + xv[k];
+						p[i] = beta * pv[k] + zv[k];
+					}
+					else {
+						x[i] = (T)(alpha * (double)pv[k] + (double)xv[k]);
+						p[i] = (T)(beta * (double)pv[k] + (double)zv[k]);
+					}
+				}
+			}
+		}
+		__syncthreads();
+	}
+	// the last iteration's x += alpha p
+	if (ran) {
 		for (int64_t i = tid; i < n; i += nt) {
-			if constexpr (std::is_same_v<T, double>) p[i] = beta * p[i] + zz[i];
-			else p[i] = (T)(beta * (double)p[i] + (double)zz[i]);
+			if constexpr (F64) x[i] = alpha * p[i] + x[i];
+			else x[i] = (T)(alpha * (double)p[i] + (double)x[i]);
 		}
 		__syncthreads();
 	}
