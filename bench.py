@@ -315,7 +315,8 @@ def bench_batched(a, fsx, P):
     log(f"uploaded; matrix plan {A64.plan()}")
     tune_ms = B.tune(bd, x, dinv, sample=64, iters=20)
     cfg = B.config()
-    log(f"autotuned: workgroup {cfg[0]}, mapping {'sliced' if cfg[1] == 0 else 'lanes'}, "
+    mapping_names = {0: "sliced", 1: "lanes", 2: "slice-offsets"}
+    log(f"autotuned: workgroup {cfg[0]}, mapping {mapping_names[cfg[1]]}, "
         f"lanes {cfg[2]} ({tune_ms:.2f} ms per sample run)")
     L = fsx.lib()
     s64 = fsx.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True)
@@ -377,7 +378,7 @@ def bench_batched(a, fsx, P):
         "config": {
             "workload": f"batched_diffusion_{a.nsys}x_edges{a.emin}-{a.emax}_rtol1e-10",
             "systems": a.nsys, "rows": N, "nnz": nnz,
-            "autotuned": {"workgroup": cfg[0], "mapping": "sliced" if cfg[1] == 0 else "lanes",
+            "autotuned": {"workgroup": cfg[0], "mapping": mapping_names[cfg[1]],
                           "lanes": cfg[2]},
             "fp64_cg": {"seconds": round(t64, 4), "iters_min": int(it64.min()),
                         "iters_max": int(it64.max()), "iters_sum": int(it64.sum()),

@@ -29,7 +29,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["fsx_core.hip", "fsx_spmv.hip", "fsx_cg.hip", "fsx_batched.hip",
                "fsx_adjoint.hip"]
-CPP_SOURCES = ["fsx_krylov.cpp"]
+CPP_SOURCES = ["fsx_krylov.cpp", "fsx_io.cpp"]
 
 # -ffp-contract=off: no fused multiply-add contraction, so elementwise results
 # round exactly like the reference's serial loops.  Packed-FP32 instructions

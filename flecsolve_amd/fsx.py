@@ -152,6 +152,12 @@ _SIGS = {
     "fsx_batched_cg": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings), _P, _P, _P]),
     "fsx_batched_ir_cg": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double, C.c_int,
                                     C.POINTER(Settings), _P, _P, _P, _P]),
+    "fsx_mm_read_header": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int64)]),
+    "fsx_mm_read_coo": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P, C.POINTER(C.c_int64)]),
+    "fsx_coo_tocsr": (C.c_int, [C.c_int64, C.c_int64, _P, _P, _P, _P, _P, _P]),
+    "fsx_csr_from_mm": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(_P)]),
     "fsx_csr_transpose": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "fsx_csr_sampled_outer": (C.c_int, [_P, _P, _P, _P, C.c_double, _P]),
     "fsx_diffusion3d_values": (C.c_int, [_P, C.c_int, _P, C.c_double, C.c_double, _P]),
@@ -450,6 +456,43 @@ def ir_cg_solve(A64: CSR, A32: CSR, b: Vector, x: Vector, rtol_outer: float,
                                 rtol_outer, max_outer, C.byref(inner),
                                 C.byref(info)), "fsx_ir_cg_solve")
     return info
+
+
+def mm_read(path):
+    """Matrix Market file -> (nrows, ncols, symmetric, I, J, V), 0-based COO."""
+    nr, nc, lines, cap = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+    sym = C.c_int()
+    p = str(path).encode()
+    check(lib().fsx_mm_read_header(p, C.byref(nr), C.byref(nc), C.byref(lines),
+                                   C.byref(sym), C.byref(cap)), "fsx_mm_read_header")
+    I = np.empty(max(cap.value, 1), np.int64)
+    J = np.empty(max(cap.value, 1), np.int64)
+    V = np.empty(max(cap.value, 1))
+    cnt = C.c_int64()
+    check(lib().fsx_mm_read_coo(p, cap.value, _ptr(I), _ptr(J), _ptr(V), C.byref(cnt)),
+          "fsx_mm_read_coo")
+    n = cnt.value
+    return nr.value, nc.value, bool(sym.value), I[:n], J[:n], V[:n]
+
+
+def coo_tocsr(nrows: int, I, J, V):
+    """Stable-by-row COO -> CSR (int64 indices), duplicates kept."""
+    I = np.ascontiguousarray(I, np.int64)
+    J = np.ascontiguousarray(J, np.int64)
+    V = np.ascontiguousarray(V, np.float64)
+    rp = np.empty(nrows + 1, np.int64)
+    ci = np.empty(max(I.size, 1), np.int64)
+    vv = np.empty(max(I.size, 1))
+    check(lib().fsx_coo_tocsr(nrows, I.size, _ptr(I), _ptr(J), _ptr(V), _ptr(rp), _ptr(ci),
+                              _ptr(vv)), "fsx_coo_tocsr")
+    return rp, ci[:I.size], vv[:I.size]
+
+
+def csr_from_mm(ctx: "Context", path, itype: int = I32) -> "CSR":
+    h = C.c_void_p()
+    check(lib().fsx_csr_from_mm(ctx.h, str(path).encode(), itype, C.byref(h)),
+          "fsx_csr_from_mm")
+    return CSR(ctx, None, None, None, _handle=h)
 
 
 def be_forward(A: CSR, u0: Vector, steps: int, settings: Settings,

@@ -236,6 +236,26 @@ int fsx_krylov_bicgstab(fsx_ctx * ctx, const fsx_csr * A, int prec,
                         const fsx_settings * s, fsx_info * info,
                         double * res_hist, int hist_cap);
 
+/* ---- host input path: Matrix Market and COO -> CSR ------------------------
+ * matrix_market::read_header / read / parse_entry
+ * (flecsolve/matrices/io/matrix_market.hh:39-103, 157-179) and coo::tocsr
+ * (flecsolve/matrices/seq.hh:324-352).  Host-only; no context needed. */
+
+/* Size line and symmetry flag; `lines` = entry lines in the file, `entries`
+ * = capacity fsx_mm_read_coo needs (2 * lines for symmetric files). */
+int fsx_mm_read_header(const char * path, int64_t * nrows, int64_t * ncols,
+                       int64_t * lines, int * symmetric, int64_t * entries);
+/* COO entries (0-based), values rounded through float as std::stof does;
+ * for symmetric files each off-diagonal entry is followed by its mirror. */
+int fsx_mm_read_coo(const char * path, int64_t cap, int64_t * I, int64_t * J,
+                    double * V, int64_t * count);
+/* Stable by row (input order inside a row), duplicates kept, columns not
+ * sorted: the reference's stable_sort order, by a counting sort. */
+int fsx_coo_tocsr(int64_t nrows, int64_t nnz, const int64_t * I, const int64_t * J,
+                  const double * V, int64_t * rowptr, int64_t * colind, double * vals);
+/* Read, convert and create the device CSR (FP64 values, index type it). */
+int fsx_csr_from_mm(fsx_ctx * ctx, const char * path, fsx_itype it, fsx_csr ** out);
+
 /* ---- adjoint of backward-Euler diffusion (new: no reference counterpart;
  * the forward step is the reference's BE integrator, time-integrators/
  * bdf.hh:74-95, 269-272, 292-296 with operator_adapter.hh:29-35, assembled

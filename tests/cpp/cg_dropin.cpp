@@ -3,13 +3,16 @@
 // type and device CSR operator.  b = 0, x0 = set_random(7), and a user
 // diagnostic that tracks the A-norm of the error (the solution is 0):
 // monotone decrease and the Trefethen-Bau bound 2((sqrt k - 1)/(sqrt k + 1))^i.
-// Also runs GMRES and BiCGSTAB through the same API.  Exit status 0 = pass.
+// Also runs GMRES and BiCGSTAB through the same API, and CG on a Matrix
+// Market file read with matrix_market<>::read(...).tocsr() (argv[1]).
+// Exit status 0 = pass.
 #include <cmath>
 #include <cstdio>
 #include <functional>
 #include <vector>
 
 #include "flecsolve/matrices/device_csr.hh"
+#include "flecsolve/matrices/io/matrix_market.hh"
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
 #include "flecsolve/solvers/gmres.hh"
@@ -43,7 +46,7 @@ struct diagnostic {
 	bool monotonic_fail = false, convergence_fail = false;
 };
 
-int main() {
+int main(int argc, char ** argv) {
 	const int n = 48;
 	std::vector<int32_t> rp(n * n + 1), ci(fsx_gen_poisson2d_nnz(n));
 	std::vector<double> v(ci.size());
@@ -112,6 +115,26 @@ int main() {
 		const double brel = r.l2norm().get() / rhs.l2norm().get();
 		std::printf("bicgstab: %d iterations, true rel res %.3e\n", bsi.iters, brel);
 		fails += !bsi.success() || brel > 1e-9;
+	}
+	if (argc > 1) {
+		// the reference's test input path: read, tocsr, device operator, CG
+		auto coo = mat::io::matrix_market<double, int32_t>::read(argv[1]);
+		auto csr = coo.tocsr();
+		op::core<mat::device_csr> M(csr.device(ctx));
+		const int m = (int)csr.rows();
+		vec::device<double> mb(ctx, m), mx(ctx, m), mr(ctx, m);
+		mb.set_scalar(1.0);
+		cg::settings ms;
+		ms.maxiter = 100;
+		ms.rtol = 1e-12;
+		ms.use_zero_guess = true;
+		auto mslv = cg::solver(ms, vec::make_device_work<cg::nwork>(mb))(op::ref(M));
+		solve_info mi = mslv(mb, mx);
+		M.residual(mb, mx, mr);
+		const double mrel = mr.l2norm().get() / mb.l2norm().get();
+		std::printf("matrix market %s: %d rows, %d entries, cg %d iterations, true rel res %.3e\n",
+		            argv[1], m, (int)csr.nnz(), mi.iters, mrel);
+		fails += !mi.success() || mrel > 1e-11 || csr.nnz() != (int32_t)coo.nnz();
 	}
 	fsx_csr_destroy(Ah);
 	fsx_ctx_destroy(ctx);

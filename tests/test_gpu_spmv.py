@@ -5,6 +5,8 @@ mapping bit-identical to the reference's serial CSR loop (it sums each row
 serially in CSR order); the vector-per-row mapping within 1e-13 relative
 (tree order inside a row).
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -256,3 +258,19 @@ def test_rejects_bad_input(ctx):
     x = ctx.zeros(16)
     with pytest.raises(fsx.FsxError):
         A.spmv(x, x)
+
+
+@pytest.mark.parametrize("name", ["sym5.mtx", "gen4.mtx"])
+def test_csr_from_matrix_market(ctx, name):
+    gold = Path(__file__).resolve().parent / "golden" / name
+    A = fsx.csr_from_mm(ctx, gold)
+    nr, nc, sym, I, J, V = O.mm_read(gold)
+    rp, ci, vv = O.coo_tocsr(nr, I, J, V)
+    r2, c2, v2 = A.download()
+    np.testing.assert_array_equal(r2, rp)
+    np.testing.assert_array_equal(c2, ci)
+    np.testing.assert_array_equal(v2, vv)
+    x = np.random.default_rng(1).standard_normal(nc)
+    y = ctx.zeros(nr)
+    A.spmv(ctx.vec(x), y)
+    np.testing.assert_array_equal(y.get(), O.Csr(rp, ci, vv, ncols=nc).spmv(x))
