@@ -325,7 +325,9 @@ inline unsigned sliced_blocks(int64_t nrows, bool reduces = true) {
 // 512-thread blocks (249-250 us vs 278 us grid-striding); FP32 values, whose
 // slices are half the bytes, are fastest in 256-thread blocks capped at 8192
 // and striding (137 us vs 150-165 us one slice per wave: block dispatch and
-// the end-of-block barrier dominate short blocks).
+// the end-of-block barrier dominate short blocks; interleaving the loads of
+// two strided slices per trip was slower still, 82 VGPRs cut occupancy to 6
+// waves per SIMD).
 template<class V>
 struct sdia_shape {
 	static constexpr bool f64 = std::is_same_v<V, double>;
