@@ -29,10 +29,12 @@
 // flag once per chunk with one chunk in flight.
 #include "fsx_spmv_kernels.hpp"
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <tuple>
+#include <vector>
 
 namespace fsx {
 
@@ -252,6 +254,7 @@ struct cg_graph_cache {
 	                         const void *, const void *, int64_t, int64_t, int, int,
 	                         int, int>;
 	std::map<key_t, entry> m;
+	std::map<std::vector<const void *>, entry> mv; // other fused drivers
 	std::map<std::tuple<size_t, int, int>, fsx_vec *> pool;
 	fsx::dbuf * statebuf = nullptr;
 	fsx::dbuf * histbuf = nullptr;
@@ -262,6 +265,11 @@ struct cg_graph_cache {
 			if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
 		}
 		m.clear();
+		for (auto & kv : mv) {
+			if (kv.second.ge) (void)hipGraphExecDestroy(kv.second.ge);
+			if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+		}
+		mv.clear();
 	}
 	~cg_graph_cache() {
 		clear_graphs();
@@ -550,6 +558,375 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 	if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
 }
 
+// ---- fused BiCGSTAB (op::bicgstab::apply, solvers/bicgstab.hh:39-187) ----
+//
+// Same operation sequence, tolerances and solve_info as the reference; the
+// scalars live on the device.  One iteration:
+//   B1  p = beta (p - omega v) + res (res on the first iteration); p^ = P p
+//                                                    (bicgstab.hh:114-124)
+//   B2  v = A p^, partials of r~.v                  (126-128)   SpMV + dot
+//   G2  alpha = rho / (r~.v)                        (128-130)
+//   B3  s = res - alpha v, partials of |s|^2; s^ = P s   (132-140)
+//   G3  |s| < rtol (absolute, as the reference compares) -> early exit with
+//       iters = iter; x += alpha p^ applied by one kernel after the loop
+//   B4  t = A s^, partials of t.t and t.s           (142-146)   SpMV + dots
+//   G4  omega = t.s / t.t (0 when t.t = 0)
+//   B5  x += alpha p^; x += omega s^; res = s - omega t; partials of |res|^2
+//       and r~.res (the next iteration's rho)       (148-153, 105)
+//   G5  |res| test, omega = 0 breakdown, rho0 = rho, iteration count, the
+//       next rho and its breakdown test, beta      (155-183, 105-111, 118)
+// A breakdown restart (r~ orthogonal to res, 107-116) ends the chunk; the
+// host restarts from the true residual and resumes.
+
+struct bcg_state {
+	double rho0, rho1, alpha, omega, beta, res, tol, rtol_abs, rt_norm;
+	int iter, done, status, iters, maxiter, first, restart, early;
+};
+
+template<bool JAC>
+__global__ __launch_bounds__(256) void bcg_p_kernel(size_t n, double * __restrict__ p,
+                                                    double * __restrict__ ph,
+                                                    const double * __restrict__ v,
+                                                    const double * __restrict__ res,
+                                                    const double * __restrict__ dinv,
+                                                    const bcg_state * st) {
+	if (st->done) return;
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	double pi;
+	if (st->first) pi = res[i];
+	else {
+		const double t = (-st->omega) * v[i] + p[i]; // p.axpy(-omega, v, p)
+		pi = st->beta * t + res[i];                  // p.axpy(beta, p, res)
+	}
+	p[i] = pi;
+	if constexpr (JAC) ph[i] = 0.0 + dinv[i] * pi;
+}
+
+template<class ACC, class T>
+struct epi_bcg_v {
+	static constexpr int NV = 1;
+	static constexpr bool reduces = true;
+	T * v;
+	const T * rt;
+	double * partials;
+	const bcg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI &&) const {
+		const T vi = (T)s;
+		v[i] = vi;
+		d[0] += (double)rt[i] * (double)vi; // r_tilde.dot(v)
+	}
+	__device__ void block_end(const double (&d)[NV], unsigned blk, unsigned) const {
+		store_block_partials<NV>(d, partials, blk);
+	}
+};
+
+struct fin_bcg_alpha {
+	bcg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void operator()(const double (&t)[1]) const {
+		if (t[0] == 0.0) { // the reference asserts alpha != 0
+			st->status = FSX_DIVERGED_BREAKDOWN;
+			st->done = 1;
+			return;
+		}
+		st->alpha = st->rho1 / t[0];
+	}
+};
+
+template<bool JAC>
+__global__ __launch_bounds__(kUpdThreads) void bcg_s_kernel(
+	size_t n, double * __restrict__ s, double * __restrict__ sh,
+	const double * __restrict__ v, const double * __restrict__ res,
+	const double * __restrict__ dinv, double * partials, const bcg_state * st) {
+	if (st->done) return;
+	const double alpha = st->alpha;
+	const unsigned blk = blockIdx.x;
+	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
+	double ss = 0.0;
+#pragma unroll
+	for (int j = 0; j < kUpdPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			const double si = (-alpha) * v[i] + res[i]; // s.axpy(-alpha, v, res)
+			s[i] = si;
+			ss += si * si;
+			if constexpr (JAC) sh[i] = 0.0 + dinv[i] * si;
+		}
+	}
+	double d[1] = {ss};
+	store_block_partials<1>(d, partials, blk);
+}
+
+struct fin_bcg_s {
+	bcg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void operator()(const double (&t)[1]) const {
+		if (sqrt(t[0]) < st->rtol_abs) { // early convergence, iters = iter
+			st->early = 1;
+			st->iters = st->iter;
+			st->status = FSX_CONVERGED_RTOL;
+			st->done = 1;
+		}
+	}
+};
+
+template<class ACC, class T>
+struct epi_bcg_t {
+	static constexpr int NV = 2;
+	static constexpr bool reduces = true;
+	T * t;
+	const T * s;
+	double * partials;
+	const bcg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	template<class XI>
+	__device__ void row(int64_t i, ACC sum, double (&d)[NV], XI &&) const {
+		const T ti = (T)sum;
+		t[i] = ti;
+		d[0] += (double)ti * (double)ti; // t.dot(t)
+		d[1] += (double)ti * (double)s[i]; // t.dot(s)
+	}
+	__device__ void block_end(const double (&d)[NV], unsigned blk, unsigned) const {
+		store_block_partials<NV>(d, partials, blk);
+	}
+};
+
+struct fin_bcg_omega {
+	bcg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void operator()(const double (&t)[2]) const {
+		st->omega = t[0] == 0.0 ? 0.0 : t[1] / t[0];
+	}
+};
+
+__global__ __launch_bounds__(kUpdThreads) void bcg_x_kernel(
+	size_t n, double * __restrict__ x, double * __restrict__ res,
+	const double * __restrict__ ph, const double * __restrict__ sh,
+	const double * __restrict__ t, const double * __restrict__ s,
+	const double * __restrict__ rt, double * partials, const bcg_state * st) {
+	if (st->done) return;
+	const double alpha = st->alpha, omega = st->omega;
+	const unsigned blk = blockIdx.x;
+	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
+	double rr = 0.0, rtr = 0.0;
+#pragma unroll
+	for (int j = 0; j < kUpdPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			double xi = alpha * ph[i] + x[i]; // x.axpy(alpha, p_hat, x)
+			xi = omega * sh[i] + xi;          // x.axpy(omega, s_hat, x)
+			x[i] = xi;
+			const double ri = (-omega) * t[i] + s[i]; // res.axpy(-omega, t, s)
+			res[i] = ri;
+			rr += ri * ri;
+			rtr += rt[i] * ri; // next rho: r_tilde.dot(res)
+		}
+	}
+	double d[2] = {rr, rtr};
+	store_block_partials<2>(d, partials, blk);
+}
+
+struct fin_bcg_res {
+	bcg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void operator()(const double (&t)[2]) const {
+		const double res = sqrt(t[0]);
+		st->res = res;
+		const int it = st->iter;
+		if (res < st->tol) {
+			st->status = FSX_CONVERGED_RTOL;
+			st->iters = it + 1;
+			st->done = 1;
+			return;
+		}
+		if (st->omega == 0.0) {
+			st->iters = it + 1;
+			st->status = FSX_DIVERGED_BREAKDOWN;
+			st->done = 1;
+			return;
+		}
+		st->rho0 = st->rho1;
+		st->iter = it + 1;
+		st->first = 0;
+		if (it + 1 >= st->maxiter) { // loop exhausted: iters stays 0
+			st->done = 1;
+			return;
+		}
+		// top of the next iteration (bicgstab.hh:105-118)
+		const double rho = t[1];
+		st->rho1 = rho;
+		if (sqrt(fabs(rho)) < DBL_EPSILON * st->rt_norm) {
+			st->restart = 1;
+			st->done = 1;
+			return;
+		}
+		st->beta = (rho / st->rho0) * (st->alpha / st->omega);
+	}
+};
+
+__global__ __launch_bounds__(256) void bcg_xearly_kernel(size_t n, double * __restrict__ x,
+                                                         const double * __restrict__ ph,
+                                                         const bcg_state * st) {
+	if (!st->early) return;
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) x[i] = st->alpha * ph[i] + x[i];
+}
+
+struct bcg_bufs {
+	const fsx_csr * A;
+	double *x, *res, *rt, *p, *v, *ph, *s, *sh, *t;
+	const double * dinv;
+	size_t n;
+};
+
+template<class V, class I>
+static void bcg_enqueue_iteration(fsx_ctx * ctx, const bcg_bufs & B, bcg_state * st) {
+	hipStream_t s = ctx->stream;
+	double * part = ctx->ws.partials;
+	const unsigned g1 = (unsigned)((B.n + 255) / 256);
+	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+	const unsigned gu = (unsigned)((B.n + per - 1) / per);
+	const bool jac = B.dinv != nullptr;
+	// B1
+	if (jac)
+		hipLaunchKernelGGL((bcg_p_kernel<true>), dim3(g1), dim3(256), 0, s, B.n, B.p, B.ph,
+		                   (const double *)B.v, (const double *)B.res, B.dinv,
+		                   (const bcg_state *)st);
+	else
+		hipLaunchKernelGGL((bcg_p_kernel<false>), dim3(g1), dim3(256), 0, s, B.n, B.p, B.ph,
+		                   (const double *)B.v, (const double *)B.res, B.dinv,
+		                   (const bcg_state *)st);
+	// B2 + G2
+	epi_bcg_v<double, double> e2{B.v, B.rt, part, st};
+	spmv_launch_typed<decltype(e2), double, V, I, double>(s, B.A, (const double *)B.ph, e2);
+	hipLaunchKernelGGL((finalize_kernel<1, fin_bcg_alpha>), dim3(1), dim3(kFinThreads), 0,
+	                   s, (const double *)part, spmv_blocks(B.A), fin_bcg_alpha{st});
+	// B3 + G3
+	if (jac)
+		hipLaunchKernelGGL((bcg_s_kernel<true>), dim3(gu), dim3(kUpdThreads), 0, s, B.n, B.s,
+		                   B.sh, (const double *)B.v, (const double *)B.res, B.dinv, part,
+		                   (const bcg_state *)st);
+	else
+		hipLaunchKernelGGL((bcg_s_kernel<false>), dim3(gu), dim3(kUpdThreads), 0, s, B.n,
+		                   B.s, B.sh, (const double *)B.v, (const double *)B.res, B.dinv,
+		                   part, (const bcg_state *)st);
+	hipLaunchKernelGGL((finalize_kernel<1, fin_bcg_s>), dim3(1), dim3(kFinThreads), 0, s,
+	                   (const double *)part, gu, fin_bcg_s{st});
+	// B4 + G4
+	epi_bcg_t<double, double> e4{B.t, B.s, part, st};
+	spmv_launch_typed<decltype(e4), double, V, I, double>(s, B.A, (const double *)B.sh, e4);
+	hipLaunchKernelGGL((finalize_kernel<2, fin_bcg_omega>), dim3(1), dim3(kFinThreads), 0,
+	                   s, (const double *)part, spmv_blocks(B.A), fin_bcg_omega{st});
+	// B5 + G5
+	hipLaunchKernelGGL(bcg_x_kernel, dim3(gu), dim3(kUpdThreads), 0, s, B.n, B.x, B.res,
+	                   (const double *)B.ph, (const double *)B.sh, (const double *)B.t,
+	                   (const double *)B.s, (const double *)B.rt, part,
+	                   (const bcg_state *)st);
+	hipLaunchKernelGGL((finalize_kernel<2, fin_bcg_res>), dim3(1), dim3(kFinThreads), 0, s,
+	                   (const double *)part, gu, fin_bcg_res{st});
+	FSX_HIP(hipGetLastError());
+}
+
+// Runs device iterations from the state in st_host until the device sets
+// `done` (converged, breakdown, early exit, exhausted, or restart needed).
+template<class V, class I>
+static void bcg_run_chunks(fsx_ctx * ctx, const bcg_bufs & B, bcg_state & st_host) {
+	auto & cache = cache_of(ctx);
+	bcg_state * st = (bcg_state *)cache.statebuf->p;
+	bcg_state * pinned = (bcg_state *)ctx->host_scalars;
+	static_assert(sizeof(bcg_state) * 3 <= sizeof(double) * kScalarSlots, "");
+	static_assert(sizeof(bcg_state) <= sizeof(cg_state) * 4, "");
+	std::memcpy(pinned, &st_host, sizeof(bcg_state));
+	FSX_HIP(hipMemcpyAsync(st, pinned, sizeof(bcg_state), hipMemcpyHostToDevice,
+	                       ctx->stream));
+	const int K = chunk_size();
+	{
+		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+		ensure_partials(ctx, std::max<size_t>(2 * (size_t)spmv_blocks(B.A),
+		                                      2 * ((B.n + per - 1) / per)));
+	}
+	hipGraphExec_t ge = nullptr;
+	if (use_graphs()) {
+		const fsx_csr * A = B.A;
+		const void * layout_vals = A->plan_kind == 3 && A->dg_vals ? A->dg_vals->p
+		                           : A->sl_vals                   ? A->sl_vals->p
+		                                                          : nullptr;
+		std::vector<const void *> key{(const void *)&bcg_enqueue_iteration<V, I>,
+		                              A->rowptr(), layout_vals, A->vals->p, B.x, B.res,
+		                              B.rt, B.p, B.v, B.ph, B.s, B.sh, B.t, B.dinv,
+		                              (const void *)ctx->ws.partials, (const void *)st,
+		                              (const void *)(intptr_t)A->plan_kind,
+		                              (const void *)(intptr_t)B.n, (const void *)(intptr_t)K};
+		auto it = cache.mv.find(key);
+		if (it == cache.mv.end()) {
+			if (cache.mv.size() >= 16) cache.clear_graphs();
+			cg_graph_cache::entry en;
+			FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+			try {
+				for (int k = 0; k < K; ++k) bcg_enqueue_iteration<V, I>(ctx, B, st);
+			}
+			catch (...) {
+				hipGraph_t junk = nullptr;
+				(void)hipStreamEndCapture(ctx->stream, &junk);
+				if (junk) (void)hipGraphDestroy(junk);
+				throw;
+			}
+			FSX_HIP(hipStreamEndCapture(ctx->stream, &en.g));
+			FSX_HIP(hipGraphInstantiate(&en.ge, en.g, nullptr, nullptr, 0));
+			cache.mv[key] = en;
+			ge = en.ge;
+		}
+		else
+			ge = it->second.ge;
+	}
+	bcg_state * flags = pinned + 1;
+	hipEvent_t ev[2];
+	FSX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+	FSX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+	auto launch_chunk = [&](int slot) {
+		if (ge) FSX_HIP(hipGraphLaunch(ge, ctx->stream));
+		else
+			for (int k = 0; k < K; ++k) bcg_enqueue_iteration<V, I>(ctx, B, st);
+		ctx->launches += 10 * K;
+		FSX_HIP(hipMemcpyAsync(&flags[slot], st, sizeof(bcg_state), hipMemcpyDeviceToHost,
+		                       ctx->stream));
+		FSX_HIP(hipEventRecord(ev[slot], ctx->stream));
+	};
+	const int remaining = st_host.maxiter - st_host.iter;
+	const int max_chunks = (remaining + K - 1) / K;
+	int launched = 0, checked = 0;
+	if (max_chunks > 0) {
+		launch_chunk(0);
+		launched = 1;
+	}
+	while (checked < launched) {
+		if (launched < max_chunks) {
+			launch_chunk(launched & 1);
+			++launched;
+		}
+		FSX_HIP(hipEventSynchronize(ev[checked & 1]));
+		ctx->host_syncs++;
+		const bcg_state & f = flags[checked & 1];
+		++checked;
+		if (f.done) break;
+	}
+	if (launched > 0 && B.n > 0) {
+		hipLaunchKernelGGL(bcg_xearly_kernel, dim3((unsigned)((B.n + 255) / 256)), dim3(256),
+		                   0, ctx->stream, B.n, B.x, (const double *)B.ph,
+		                   (const bcg_state *)st);
+		FSX_HIP(hipGetLastError());
+	}
+	FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(bcg_state), hipMemcpyDeviceToHost,
+	                       ctx->stream));
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+	(void)hipEventDestroy(ev[0]);
+	(void)hipEventDestroy(ev[1]);
+	std::memcpy(&st_host, pinned, sizeof(bcg_state));
+}
+
 } // namespace fsx
 
 using namespace fsx;
@@ -690,6 +1067,130 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 		}
 		info->outer_iters = outer;
 		info->rel_res_final = rnorm / b_norm;
+	});
+}
+
+int fsx_bicgstab_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                       const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                       fsx_info * info) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A && s && info, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)b->n == A->nrows,
+		            "bicgstab: shape mismatch");
+		FSX_REQUIRE(A->vt == FSX_F64 && b->dt == FSX_F64, "bicgstab: FP64 matrix and vectors");
+		if (dinv) check_same(dinv, b);
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "bicgstab: b and x must differ");
+		std::memset(info, 0, sizeof(*info));
+		info->status = FSX_UNKNOWN;
+		const size_t n = b->n;
+		fsx_vec * res = pooled(ctx, n, FSX_F64, 20);
+		fsx_vec * rt = pooled(ctx, n, FSX_F64, 21);
+		fsx_vec * p = pooled(ctx, n, FSX_F64, 22);
+		fsx_vec * v = pooled(ctx, n, FSX_F64, 23);
+		fsx_vec * ph = dinv ? pooled(ctx, n, FSX_F64, 24) : p;
+		fsx_vec * sv = pooled(ctx, n, FSX_F64, 25);
+		fsx_vec * sh = dinv ? pooled(ctx, n, FSX_F64, 26) : sv;
+		fsx_vec * t = pooled(ctx, n, FSX_F64, 27);
+		// bicgstab.hh:52-79
+		double b_norm = std::sqrt(reduce_host(ctx, RED_L2SQ, b, nullptr));
+		if (b_norm == 0.0) b_norm = 1.0;
+		const double tol = s->rtol * b_norm;
+		info->rhs_norm = (float)b_norm;
+		if (s->use_zero_guess) {
+			info->sol_norm_initial = 0;
+			launch_copy(ctx, b, res);
+			launch_set(ctx, 0.0, x);
+		}
+		else {
+			info->sol_norm_initial = (float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
+			int rc = fsx_csr_residual(ctx, A, b, x, res);
+			if (rc) throw error(rc, get_error());
+		}
+		double res_norm = std::sqrt(reduce_host(ctx, RED_L2SQ, res, nullptr));
+		double rt_norm = res_norm;
+		info->res_norm_initial = (float)res_norm;
+		if (res_norm < tol) {
+			info->status = FSX_CONVERGED_RTOL;
+			info->res_norm_final = (float)res_norm;
+			return;
+		}
+		// alpha = 1, omega = 1, rho = {2, 1} (list-initialised, bicgstab.hh:85-89)
+		bcg_state st{};
+		st.alpha = 1.0;
+		st.omega = 1.0;
+		st.beta = 0.0;
+		st.rho0 = 2.0;
+		st.rho1 = 1.0;
+		st.tol = tol;
+		st.rtol_abs = (double)s->rtol;
+		st.maxiter = s->maxiter;
+		st.res = res_norm;
+		launch_copy(ctx, res, rt);
+		launch_set(ctx, 0.0, p);
+		launch_set(ctx, 0.0, v);
+		bcg_bufs B{A, (double *)x->ptr(), (double *)res->ptr(), (double *)rt->ptr(),
+		           (double *)p->ptr(), (double *)v->ptr(), (double *)ph->ptr(),
+		           (double *)sv->ptr(), (double *)sh->ptr(), (double *)t->ptr(),
+		           dinv ? (const double *)dinv->ptr() : nullptr, n};
+		int it = 0;
+		bool finished = false;
+		while (!finished) {
+			// top of iteration `it` on the host: rho, breakdown restart
+			// (bicgstab.hh:105-116), beta (118)
+			bool run = false;
+			while (it < s->maxiter) {
+				const double rho = reduce_host(ctx, RED_DOT, rt, res);
+				if (std::sqrt(std::fabs(rho)) < DBL_EPSILON * rt_norm) {
+					int rc = fsx_csr_residual(ctx, A, b, x, res);
+					if (rc) throw error(rc, get_error());
+					launch_copy(ctx, res, rt);
+					res_norm = std::sqrt(reduce_host(ctx, RED_L2SQ, res, nullptr));
+					rt_norm = res_norm;
+					launch_copy(ctx, res, p);
+					++info->restarts;
+					++it;
+					continue;
+				}
+				st.rho1 = rho;
+				if (it > 0) st.beta = (rho / st.rho0) * (st.alpha / st.omega);
+				run = true;
+				break;
+			}
+			if (!run) break; // maxiter reached in the restart loop
+			st.iter = it;
+			st.first = it == 0;
+			st.rt_norm = rt_norm;
+			st.done = st.restart = st.early = 0;
+			st.status = FSX_UNKNOWN;
+			st.iters = 0;
+			by_matrix_type(A, [&](auto * vtag, auto * itag) {
+				using V = std::remove_pointer_t<decltype(vtag)>;
+				using I = std::remove_pointer_t<decltype(itag)>;
+				bcg_run_chunks<V, I>(ctx, B, st);
+			});
+			res_norm = st.res;
+			if (st.restart) {
+				// the device found r~ orthogonal to res at the top of st.iter
+				int rc = fsx_csr_residual(ctx, A, b, x, res);
+				if (rc) throw error(rc, get_error());
+				launch_copy(ctx, res, rt);
+				res_norm = std::sqrt(reduce_host(ctx, RED_L2SQ, res, nullptr));
+				rt_norm = res_norm;
+				launch_copy(ctx, res, p);
+				++info->restarts;
+				it = st.iter + 1;
+				continue;
+			}
+			finished = true;
+		}
+		info->iters = st.iters;
+		if (st.iters || st.status == FSX_DIVERGED_BREAKDOWN) info->status = st.status;
+		info->res_norm_final = (float)res_norm;
+		info->sol_norm_final = (float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
+		if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
 	});
 }
 

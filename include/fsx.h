@@ -201,6 +201,15 @@ int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 /* Residual history of the last fsx_cg_solve on ctx (|r| per iteration). */
 int fsx_cg_history(fsx_ctx * ctx, double * hist, int cap, int * count);
 
+/* fsx_bicgstab_solve: op::bicgstab::apply (solvers/bicgstab.hh:39-187) with
+ * identity (dinv == NULL) or Jacobi preconditioning, fused kernels with
+ * device-resident scalars captured in HIP graphs; same operation sequence,
+ * early exit (|s| < rtol, absolute, iters = iter), breakdown restart and
+ * solve_info as the reference (no user diagnostic).  FP64. */
+int fsx_bicgstab_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                       const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                       fsx_info * info);
+
 /* Mixed-precision iterative refinement (new, no reference counterpart):
  * outer FP64 r = b - A64 x, |r|/|b| check, inner FP32 CG(+Jacobi) on A32 to
  * `inner->rtol` with FP64 reductions, FP64 update x += |r| d. */

@@ -1,0 +1,114 @@
+"""Fused device BiCGSTAB (fsx_bicgstab_solve) against the oracle's
+op::bicgstab::apply (gpu).
+
+Bar: iteration counts within +-2 and x within 1e-10 relative on the
+configurations' operators; the reference's quirks (early exit on the absolute
+|s| < rtol with iters = iter, A = I turning into diverged_iters, maxiter
+exhaustion) reproduced exactly; the breakdown restart (r~ orthogonal to the
+residual) taken at the same iteration as the oracle's.
+"""
+import numpy as np
+import pytest
+
+from flecsolve_amd import fsx
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def dense_csr(M):
+    n = M.shape[0]
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        for j in range(n):
+            if M[i, j] != 0:
+                ci.append(j)
+                v.append(M[i, j])
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(v, np.float64)
+
+
+@pytest.mark.parametrize("n,jac", [(16, False), (16, True), (40, False), (40, True)])
+def test_convdiff_matches_oracle(ctx, n, jac):
+    rp, ci, v = P.convdiff3d(n, (10.0, 5.0, 2.5))
+    b = P.uniform(n ** 3, 2101)
+    Ao = O.Csr(rp, ci, v)
+    s = dict(maxiter=2000, rtol=1e-10, use_zero_guess=True)
+    xo, io, _ = O.bicgstab(Ao, b, np.zeros(n ** 3), O.Settings(**s),
+                           P=O.prec(1, Ao.dinv()) if jac else None)
+    A = ctx.csr(rp, ci, v)
+    x = ctx.zeros(n ** 3)
+    info = fsx.bicgstab_solve(A, ctx.vec(b), x, fsx.Settings(**s),
+                              dinv=A.diag_inv() if jac else None)
+    assert info.reason == io.reason == "converged_rtol"
+    assert abs(info.iters - io.iters) <= 2, (info.iters, io.iters)
+    assert rel(x.get(), xo) <= 1e-10
+    r = ctx.zeros(n ** 3)
+    A.residual(ctx.vec(b), x, r)
+    assert fsx.nrm2(r) / np.linalg.norm(b) < 1e-10
+
+
+def test_matches_generic_path_and_nonzero_guess(ctx):
+    n = 24
+    rp, ci, v = P.convdiff3d(n, (10.0, 5.0, 2.5))
+    b = P.uniform(n ** 3, 5)
+    x0 = P.uniform(n ** 3, 6)
+    A = ctx.csr(rp, ci, v)
+    s = fsx.Settings(maxiter=2000, rtol=1e-10, use_zero_guess=False)
+    xf = ctx.vec(x0)
+    fi = fsx.bicgstab_solve(A, ctx.vec(b), xf, s)
+    xg = ctx.vec(x0)
+    gi, _ = fsx.krylov("bicgstab", A, ctx.vec(b), xg, s)
+    Ao = O.Csr(rp, ci, v)
+    xo, io, _ = O.bicgstab(Ao, b, x0, O.Settings(maxiter=2000, rtol=1e-10))
+    assert abs(fi.iters - io.iters) <= 2 and abs(gi.iters - io.iters) <= 2
+    assert rel(xf.get(), xo) <= 1e-10 and rel(xg.get(), xo) <= 1e-10
+    assert abs(fi.sol_norm_initial - io.sol_norm_initial) <= 1e-6 * io.sol_norm_initial
+
+
+def test_identity_early_exit_quirk_and_maxiter(ctx):
+    # A = I: the first |s| is 0 < rtol, exit with iters = 0 -> diverged_iters
+    m = 50
+    rp = np.arange(m + 1, dtype=np.int32)
+    ci = np.arange(m, dtype=np.int32)
+    A = ctx.csr(rp, ci, np.ones(m))
+    b = P.uniform(m, 9)
+    x = ctx.zeros(m)
+    info = fsx.bicgstab_solve(A, ctx.vec(b), x, fsx.Settings(maxiter=10, rtol=1e-8,
+                                                             use_zero_guess=True))
+    _, io, _ = O.bicgstab(O.Csr(rp, ci, np.ones(m)), b, np.zeros(m),
+                          O.Settings(maxiter=10, rtol=1e-8, use_zero_guess=True))
+    assert (info.iters, info.reason) == (io.iters, io.reason) == (0, "diverged_iters")
+    np.testing.assert_array_equal(x.get(), b)
+    # maxiter exhausted: iters 0, diverged_iters
+    n = 16
+    rp, ci, v = P.convdiff3d(n, (10.0, 5.0, 2.5))
+    A = ctx.csr(rp, ci, v)
+    info = fsx.bicgstab_solve(A, ctx.vec(P.uniform(n ** 3, 1)), ctx.zeros(n ** 3),
+                              fsx.Settings(maxiter=3, rtol=1e-14, use_zero_guess=True))
+    assert (info.iters, info.reason) == (0, "diverged_iters")
+
+
+@pytest.mark.parametrize("M", [
+    [[3, 0, 0, 0, 0], [-3, 3, 0, 0, 0], [3, 0, 3, 0, 0], [1, 0, 0, 1, 0], [2, 1, -3, -1, 1]],
+    [[2, 0, 0, 0, 0], [-1, 3, 0, 0, 0], [-3, 1, 2, 0, 0], [1, 2, -1, 3, 0], [2, 3, -1, 1, 1]],
+    [[1, 0, 0, 0], [-1, 2, 0, 0], [3, -3, 2, 0], [-1, 1, 1, 3]],
+])
+def test_breakdown_restart(ctx, M):
+    M = np.array(M, np.float64)
+    rp, ci, v = dense_csr(M)
+    n = M.shape[0]
+    b = np.zeros(n)
+    b[0] = 1.0
+    s = dict(maxiter=50, rtol=1e-12, use_zero_guess=True)
+    xo, io, _ = O.bicgstab(O.Csr(rp, ci, v), b, np.zeros(n), O.Settings(**s))
+    assert io.restarts >= 1
+    x = ctx.zeros(n)
+    info = fsx.bicgstab_solve(ctx.csr(rp, ci, v), ctx.vec(b), x, fsx.Settings(**s))
+    assert (info.restarts, info.iters, info.reason) == (io.restarts, io.iters, io.reason)
+    np.testing.assert_allclose(x.get(), np.linalg.solve(M, b), rtol=1e-12, atol=1e-12)
