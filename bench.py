@@ -566,10 +566,31 @@ def bench_convdiff(a, fsx, P):
     results["bicgstab_fused"] = {"seconds": round(sum(ts) / len(ts), 4), "iters": info.iters,
                                  "restarts": info.restarts, "status": info.reason,
                                  "true_rel_residual": fsx.nrm2(r) / fsx.nrm2(bd)}
+    # fused device GMRES(30) + right Jacobi (fsx_gmres_solve)
+    sg = fsx.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True, max_krylov_dim=30,
+                      pre_side=1, restart=True)
+    dinv = A.diag_inv()
+    ts = []
+    for i in range(a.warmup + a.steps):
+        fsx.check(fsx.lib().fsx_zero(ctx.h, x.h))
+        ctx.sync()
+        t = time.perf_counter()
+        info = fsx.gmres_solve(A, bd, x, sg, dinv=dinv)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        log(f"fused gmres30 {'warmup' if i < a.warmup else 'step'} {i}: {dt:.4f} s, "
+            f"{info.iters} iters, {info.restarts} restarts, {info.reason}")
+        if i >= a.warmup:
+            ts.append(dt)
+    A.residual(bd, x, r)
+    results["gmres30_jacobi_fused"] = {"seconds": round(sum(ts) / len(ts), 4),
+                                       "iters": info.iters, "restarts": info.restarts,
+                                       "status": info.reason,
+                                       "true_rel_residual": fsx.nrm2(r) / fsx.nrm2(bd)}
     for name, kind, st, prec in [
             ("bicgstab_generic", "bicgstab", fsx.Settings(maxiter=a.maxiter, rtol=1e-10,
                                                   use_zero_guess=True), 0),
-            ("gmres30_jacobi", "gmres", fsx.Settings(maxiter=a.maxiter, rtol=1e-10,
+            ("gmres30_jacobi_generic", "gmres", fsx.Settings(maxiter=a.maxiter, rtol=1e-10,
                                                      use_zero_guess=True, max_krylov_dim=30,
                                                      pre_side=1, restart=True), 1)]:
         ts = []

@@ -927,6 +927,246 @@ static void bcg_run_chunks(fsx_ctx * ctx, const bcg_bufs & B, bcg_state & st_hos
 	std::memcpy(&st_host, pinned, sizeof(bcg_state));
 }
 
+// ---- fused GMRES (op::gmres::apply, solvers/gmres.hh:79-258) -------------
+//
+// Same Arnoldi sequence, modified Gram-Schmidt order, Givens rotations
+// (Bindel et al.), restart rule, back_solve and correct as the reference,
+// with the Hessenberg column, rotations and least-squares right-hand side
+// on the device.  Step k of a restart cycle:
+//   G0  v = A P b_k (right; P = D^-1 enters as a scaled gather
+//       0 + dinv[c] b_k[c], bitwise the z the reference forms) or
+//       v = P A b_k (left); partials of v.b_0
+//   then for j = 0..k: a one-block kernel stores h_jk, and
+//   Gj  v = v - h_jk b_j; partials of v.b_{j+1} (j < k) or |v|^2 (j = k)
+//   GN  h_{k+1,k} = |v|; b_{k+1} = v / |v| (v if |v| = 0)
+//   GR  one thread: previous rotations on column k, the new rotation, the
+//       rhs update, |dw_{k+1}| test, iteration count, end of cycle.
+// A cycle's m steps are captured in one graph; the host runs back_solve,
+// correct and the restart residual between cycles (every m iterations).
+
+struct gm_state {
+	double tol, vnorm;
+	int iter, k, m, maxiter, done, status, iters, cycle_end;
+};
+
+struct gather_dinv {
+	using value_type = double;
+	const double * dinv;
+	const double * x;
+	__device__ __forceinline__ void init() {}
+	__device__ __forceinline__ double operator()(int64_t c) const { return 0.0 + dinv[c] * x[c]; }
+};
+
+template<class ACC, class T, bool LEFTJ>
+struct epi_gm_v {
+	static constexpr int NV = 1;
+	static constexpr bool reduces = true;
+	T * v;
+	const T * b0;
+	const T * dinv;
+	double * partials;
+	const gm_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI &&) const {
+		T vi = (T)s;
+		if constexpr (LEFTJ) vi = 0.0 + dinv[i] * vi; // P.apply(z, v)
+		v[i] = vi;
+		d[0] += (double)vi * (double)b0[i]; // v.dot(basis[0])
+	}
+	__device__ void block_end(const double (&d)[NV], unsigned blk, unsigned) const {
+		store_block_partials<NV>(d, partials, blk);
+	}
+};
+
+// H(row, col) = sum (or its square root for the norm), column-major with
+// leading dimension ld (mdcolex of the reference)
+struct fin_gm_h {
+	const gm_state * st;
+	double * H;
+	int idx;
+	bool norm;
+	__device__ bool skip() const { return st->done != 0; }
+	__device__ void operator()(const double (&t)[1]) const {
+		H[idx] = norm ? sqrt(t[0]) : t[0];
+	}
+};
+
+// v = v - h b_j; partials of v.b_next (or |v|^2 when b_next is null)
+__global__ __launch_bounds__(kUpdThreads) void gm_mgs_kernel(
+	size_t n, double * __restrict__ v, const double * __restrict__ bj,
+	const double * __restrict__ bnext, const double * hjk, double * partials,
+	const gm_state * st) {
+	if (st->done) return;
+	const double h = *hjk;
+	const unsigned blk = blockIdx.x;
+	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
+	double acc = 0.0;
+#pragma unroll
+	for (int j = 0; j < kUpdPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			const double vi = (-h) * bj[i] + v[i]; // v.axpy(-h_jk, basis[j], v)
+			v[i] = vi;
+			acc += bnext ? vi * bnext[i] : vi * vi;
+		}
+	}
+	double d[1] = {acc};
+	store_block_partials<1>(d, partials, blk);
+}
+
+// b_{k+1} = v scaled by 1/|v| (gmres.hh:140-146)
+__global__ __launch_bounds__(256) void gm_scale_kernel(size_t n, const double * __restrict__ v,
+                                                       double * __restrict__ bnext,
+                                                       const double * hnorm,
+                                                       const gm_state * st) {
+	if (st->done) return;
+	const double vn = *hnorm;
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) bnext[i] = vn != 0.0 ? (1.0 / vn) * v[i] : v[i];
+}
+
+// Givens rotations of column k, |dw_{k+1}| test (gmres.hh:148-208, 282-331)
+__global__ void gm_givens_kernel(double * H, int ld, double * cs, double * sn, double * dw,
+                                 int k, gm_state * st) {
+	if (st->done || threadIdx.x != 0) return;
+	auto h = [&](int i, int j) -> double & { return H[i + (size_t)ld * j]; };
+	const double vnorm0 = h(k + 1, k);
+	for (int i = 0; i < k; ++i) {
+		const double x = h(i, k), y = h(i + 1, k), c = cs[i], s = sn[i];
+		h(i, k) = c * x + s * y;
+		h(i + 1, k) = -s * x + c * y;
+	}
+	if (vnorm0 != 0.0) {
+		const double f = h(k, k), g = h(k + 1, k);
+		double c, s;
+		if (g == 0.0) {
+			c = 1.0;
+			s = 0.0;
+		}
+		else if (f == 0.0) {
+			c = 0.0;
+			s = (g < 0.0) ? -1.0 : 1.0;
+		}
+		else {
+			double r = sqrt(f * f + g * g);
+			r = 1.0 / r;
+			c = fabs(f) * r;
+			s = copysign(g * r, f);
+		}
+		cs[k] = c;
+		sn[k] = s;
+		const double x = h(k, k), y = h(k + 1, k);
+		h(k, k) = c * x + s * y;
+		h(k + 1, k) = 0.0;
+		(void)y;
+		const double w = dw[k];
+		dw[k] = c * w;
+		dw[k + 1] = -s * w;
+	}
+	const double vnorm = fabs(dw[k + 1]);
+	st->vnorm = vnorm;
+	const int it = st->iter;
+	st->iter = it + 1;
+	st->k = k + 1;
+	if (vnorm < st->tol) {
+		st->status = FSX_CONVERGED_RTOL;
+		st->iters = it + 1;
+		st->done = 1;
+		return;
+	}
+	if (it + 1 >= st->maxiter) {
+		st->done = 1;
+		return;
+	}
+	if (k + 1 == st->m) {
+		st->cycle_end = 1;
+		st->done = 1;
+	}
+}
+
+// correct (gmres.hh:244-263): right: z = sum_i dy_i b_i in order, x += P z;
+// left: x += dy_i b_i in order.  One pass over the basis per element.
+template<bool RIGHT, bool JAC>
+__global__ __launch_bounds__(256) void gm_correct_kernel(size_t n, double * __restrict__ x,
+                                                         const double * const * basis,
+                                                         const double * dy, int nr,
+                                                         const double * __restrict__ dinv) {
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i >= n) return;
+	if constexpr (RIGHT) {
+		double z = 0.0;
+		for (int j = 0; j <= nr; ++j) z = dy[j] * basis[j][i] + z;
+		const double v = JAC ? 0.0 + dinv[i] * z : z;
+		x[i] = 1.0 * v + x[i];
+	}
+	else {
+		double xi = x[i];
+		for (int j = 0; j <= nr; ++j) xi = dy[j] * basis[j][i] + xi;
+		x[i] = xi;
+	}
+}
+
+struct gm_bufs {
+	const fsx_csr * A;
+	double * v;
+	const double * const * basis_dev; // device array of basis pointers
+	std::vector<double *> basis;      // host copy
+	const double * dinv;
+	bool right;
+	double *H, *cs, *sn, *dw;
+	int ld, m;
+	size_t n;
+};
+
+template<class V, class I>
+static void gm_enqueue_cycle(fsx_ctx * ctx, const gm_bufs & B, gm_state * st) {
+	hipStream_t s = ctx->stream;
+	double * part = ctx->ws.partials;
+	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+	const unsigned gu = (unsigned)((B.n + per - 1) / per);
+	const unsigned g1 = (unsigned)((B.n + 255) / 256);
+	for (int k = 0; k < B.m; ++k) {
+		const double * bk = B.basis[k];
+		// G0: v = A P b_k / P A b_k, partials of v.b_0
+		if (B.right && B.dinv) {
+			epi_gm_v<double, double, false> e{B.v, B.basis[0], nullptr, part, st};
+			spmv_launch_gather<decltype(e), double, V, I>(s, B.A, gather_dinv{B.dinv, bk}, e);
+		}
+		else if (!B.right && B.dinv) {
+			epi_gm_v<double, double, true> e{B.v, B.basis[0], B.dinv, part, st};
+			spmv_launch_typed<decltype(e), double, V, I, double>(s, B.A, bk, e);
+		}
+		else {
+			epi_gm_v<double, double, false> e{B.v, B.basis[0], nullptr, part, st};
+			spmv_launch_typed<decltype(e), double, V, I, double>(s, B.A, bk, e);
+		}
+		unsigned nblk = spmv_blocks(B.A);
+		for (int j = 0; j <= k; ++j) {
+			// h_jk from the partials just written
+			hipLaunchKernelGGL((finalize_kernel<1, fin_gm_h>), dim3(1), dim3(kFinThreads), 0,
+			                   s, (const double *)part, nblk,
+			                   fin_gm_h{st, B.H, j + B.ld * k, false});
+			hipLaunchKernelGGL(gm_mgs_kernel, dim3(gu), dim3(kUpdThreads), 0, s, B.n, B.v,
+			                   (const double *)B.basis[j],
+			                   j < k ? (const double *)B.basis[j + 1] : nullptr,
+			                   (const double *)(B.H + j + (size_t)B.ld * k), part,
+			                   (const gm_state *)st);
+			nblk = gu;
+		}
+		// GN: h_{k+1,k} = |v|; b_{k+1} = v / |v|
+		hipLaunchKernelGGL((finalize_kernel<1, fin_gm_h>), dim3(1), dim3(kFinThreads), 0, s,
+		                   (const double *)part, gu, fin_gm_h{st, B.H, k + 1 + B.ld * k, true});
+		hipLaunchKernelGGL(gm_scale_kernel, dim3(g1), dim3(256), 0, s, B.n, (const double *)B.v,
+		                   B.basis[k + 1], (const double *)(B.H + k + 1 + (size_t)B.ld * k),
+		                   (const gm_state *)st);
+		// GR
+		hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(64), 0, s, B.H, B.ld, B.cs, B.sn,
+		                   B.dw, k, st);
+	}
+	FSX_HIP(hipGetLastError());
+}
+
 } // namespace fsx
 
 using namespace fsx;
@@ -1189,6 +1429,232 @@ int fsx_bicgstab_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 		info->iters = st.iters;
 		if (st.iters || st.status == FSX_DIVERGED_BREAKDOWN) info->status = st.status;
 		info->res_norm_final = (float)res_norm;
+		info->sol_norm_final = (float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
+		if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
+	});
+}
+
+int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                    const fsx_vec * b, fsx_vec * x, const fsx_settings * s_in,
+                    fsx_info * info) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A && s_in && info, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)b->n == A->nrows,
+		            "gmres: shape mismatch");
+		FSX_REQUIRE(A->vt == FSX_F64 && b->dt == FSX_F64, "gmres: FP64 matrix and vectors");
+		if (dinv) check_same(dinv, b);
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "gmres: b and x must differ");
+		// settings::validate (gmres.hh:394-405)
+		fsx_settings s = *s_in;
+		if (s.max_krylov_dim < 0) s.max_krylov_dim = s.maxiter;
+		FSX_REQUIRE(s.max_krylov_dim <= 100, "GMRES: max_krylov_dim is larger than bound");
+		if (!s.restart)
+			FSX_REQUIRE(s.maxiter <= s.max_krylov_dim,
+			            "GMRES: maxiters must be less than or equal to max_krylov_dim when "
+			            "not using restart");
+		std::memset(info, 0, sizeof(*info));
+		info->status = FSX_UNKNOWN;
+		const size_t n = b->n;
+		const int max_dim = std::max(0, std::min(s.max_krylov_dim, s.maxiter));
+		const int ld = max_dim + 1;
+		const bool right = s.pre_side != 0;
+		// pooled vectors: res, z, v, basis[0..max_dim]; small device arrays
+		fsx_vec * res = pooled(ctx, n, FSX_F64, 30);
+		fsx_vec * z = pooled(ctx, n, FSX_F64, 31);
+		fsx_vec * v = pooled(ctx, n, FSX_F64, 32);
+		std::vector<fsx_vec *> bvec(ld);
+		std::vector<double *> basis(ld);
+		for (int i = 0; i < ld; ++i) {
+			bvec[i] = pooled(ctx, n, FSX_F64, 40 + i);
+			basis[i] = (double *)bvec[i]->ptr();
+		}
+		const size_t small = (size_t)ld * ld + 4 * (size_t)ld;
+		fsx_vec * smallv = pooled(ctx, small + (size_t)ld, FSX_F64, 39); // + basis ptrs
+		double * H = (double *)smallv->ptr();
+		double * cs = H + (size_t)ld * ld;
+		double * sn = cs + ld;
+		double * dw = sn + ld;
+		double * dy = dw + ld;
+		double ** bptr = (double **)(dy + ld);
+		// gmres reset(): H, cos, sin, dw, dy zero
+		FSX_HIP(hipMemsetAsync(H, 0, small * sizeof(double), ctx->stream));
+		FSX_HIP(hipMemcpyAsync(bptr, basis.data(), ld * sizeof(double *),
+		                       hipMemcpyHostToDevice, ctx->stream));
+		double b_norm = std::sqrt(reduce_host(ctx, RED_L2SQ, b, nullptr));
+		info->rhs_norm = (float)b_norm;
+		if (b_norm < DBL_EPSILON) b_norm = 1.0;
+		const double tol = s.rtol * b_norm;
+		if (s.use_zero_guess) launch_set(ctx, 0.0, x);
+		auto precond = [&](const fsx_vec * in, fsx_vec * out) {
+			if (dinv) {
+				int rc = fsx_jacobi_apply(ctx, dinv, in, out);
+				if (rc) throw error(rc, get_error());
+			}
+			else
+				launch_copy(ctx, in, out);
+		};
+		auto residual_into = [&](fsx_vec * out) {
+			int rc = fsx_csr_residual(ctx, A, b, x, out);
+			if (rc) throw error(rc, get_error());
+		};
+		fsx_vec * b0 = bvec[0];
+		if (!right) {
+			if (s.use_zero_guess) launch_copy(ctx, b, b0);
+			else residual_into(b0);
+			precond(b0, res);
+		}
+		else {
+			if (s.use_zero_guess) launch_copy(ctx, b, res);
+			else residual_into(res);
+		}
+		const double beta = std::sqrt(reduce_host(ctx, RED_L2SQ, res, nullptr));
+		info->res_norm_initial = (float)beta;
+		if (beta < tol) {
+			info->res_norm_final = (float)beta;
+			info->status = FSX_CONVERGED_RTOL;
+			return;
+		}
+		// res.scale(1/beta); basis[0].copy(res); dwvec[0] = beta
+		auto start_cycle = [&](double bt) {
+			fsx_vec * tmp = nullptr;
+			(void)tmp;
+			int rc = fsx_scale(ctx, 1.0 / bt, res);
+			if (rc) throw error(rc, get_error());
+			launch_copy(ctx, res, b0);
+			FSX_HIP(hipMemcpyAsync(dw, &bt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream)); // bt is a host local
+		};
+		start_cycle(beta);
+		gm_bufs B{A, (double *)v->ptr(), bptr, basis, dinv ? (const double *)dinv->ptr() : nullptr,
+		          right, H, cs, sn, dw, ld, max_dim, n};
+		auto & cache = cache_of(ctx);
+		gm_state * st = (gm_state *)cache.statebuf->p;
+		gm_state * pinned = (gm_state *)ctx->host_scalars;
+		static_assert(sizeof(gm_state) <= sizeof(cg_state) * 4, "");
+		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+		ensure_partials(ctx, std::max<size_t>(spmv_blocks(A), (n + per - 1) / per));
+		gm_state sh{};
+		sh.tol = tol;
+		sh.m = max_dim;
+		sh.maxiter = s.maxiter;
+		sh.vnorm = beta;
+		std::vector<double> hH, hdw, hdy;
+		auto back_solve_correct = [&](int nr) {
+			// back_solve (gmres.hh:333-348) on the host, correct on the device
+			hH.resize((size_t)ld * ld);
+			hdw.resize(ld);
+			hdy.assign(ld, 0.0);
+			FSX_HIP(hipMemcpyAsync(hH.data(), H, hH.size() * sizeof(double),
+			                       hipMemcpyDeviceToHost, ctx->stream));
+			FSX_HIP(hipMemcpyAsync(hdw.data(), dw, ld * sizeof(double), hipMemcpyDeviceToHost,
+			                       ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			auto h = [&](int i, int j) { return hH[i + (size_t)ld * j]; };
+			hdy[nr] = hdw[nr] / h(nr, nr);
+			for (int kk = nr - 1; kk >= 0; kk--) {
+				hdy[kk] = hdw[kk];
+				for (int i = kk + 1; i <= nr; i++) hdy[kk] -= h(kk, i) * hdy[i];
+				hdy[kk] = hdy[kk] / h(kk, kk);
+			}
+			FSX_HIP(hipMemcpyAsync(dy, hdy.data(), ld * sizeof(double), hipMemcpyHostToDevice,
+			                       ctx->stream));
+			const unsigned g1 = (unsigned)((n + 255) / 256);
+			if (right && dinv)
+				hipLaunchKernelGGL((gm_correct_kernel<true, true>), dim3(g1), dim3(256), 0,
+				                   ctx->stream, n, (double *)x->ptr(),
+				                   (const double * const *)bptr, (const double *)dy, nr,
+				                   (const double *)dinv->ptr());
+			else if (right)
+				hipLaunchKernelGGL((gm_correct_kernel<true, false>), dim3(g1), dim3(256), 0,
+				                   ctx->stream, n, (double *)x->ptr(),
+				                   (const double * const *)bptr, (const double *)dy, nr,
+				                   (const double *)nullptr);
+			else
+				hipLaunchKernelGGL((gm_correct_kernel<false, false>), dim3(g1), dim3(256), 0,
+				                   ctx->stream, n, (double *)x->ptr(),
+				                   (const double * const *)bptr, (const double *)dy, nr,
+				                   (const double *)nullptr);
+			FSX_HIP(hipGetLastError());
+			FSX_HIP(hipStreamSynchronize(ctx->stream)); // hdy is reused
+		};
+		int k = 0;
+		while (max_dim > 0) {
+			sh.k = 0;
+			sh.done = sh.cycle_end = 0;
+			std::memcpy(pinned, &sh, sizeof(gm_state));
+			FSX_HIP(hipMemcpyAsync(st, pinned, sizeof(gm_state), hipMemcpyHostToDevice,
+			                       ctx->stream));
+			by_matrix_type(A, [&](auto * vtag, auto * itag) {
+				using V = std::remove_pointer_t<decltype(vtag)>;
+				using I = std::remove_pointer_t<decltype(itag)>;
+				hipGraphExec_t ge = nullptr;
+				if (use_graphs()) {
+					const void * layout_vals = A->plan_kind == 3 && A->dg_vals ? A->dg_vals->p
+					                           : A->sl_vals ? A->sl_vals->p : nullptr;
+					std::vector<const void *> key{(const void *)&gm_enqueue_cycle<V, I>,
+					                              A->rowptr(), layout_vals, A->vals->p,
+					                              (const void *)B.v, (const void *)H,
+					                              (const void *)B.dinv,
+					                              (const void *)ctx->ws.partials,
+					                              (const void *)st,
+					                              (const void *)(intptr_t)A->plan_kind,
+					                              (const void *)(intptr_t)n,
+					                              (const void *)(intptr_t)max_dim,
+					                              (const void *)(intptr_t)right};
+					for (double * bp : basis) key.push_back(bp);
+					auto it = cache.mv.find(key);
+					if (it == cache.mv.end()) {
+						if (cache.mv.size() >= 16) cache.clear_graphs();
+						cg_graph_cache::entry en;
+						FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+						try {
+							gm_enqueue_cycle<V, I>(ctx, B, st);
+						}
+						catch (...) {
+							hipGraph_t junk = nullptr;
+							(void)hipStreamEndCapture(ctx->stream, &junk);
+							if (junk) (void)hipGraphDestroy(junk);
+							throw;
+						}
+						FSX_HIP(hipStreamEndCapture(ctx->stream, &en.g));
+						FSX_HIP(hipGraphInstantiate(&en.ge, en.g, nullptr, nullptr, 0));
+						cache.mv[key] = en;
+						ge = en.ge;
+					}
+					else
+						ge = it->second.ge;
+				}
+				if (ge) FSX_HIP(hipGraphLaunch(ge, ctx->stream));
+				else gm_enqueue_cycle<V, I>(ctx, B, st);
+			});
+			ctx->launches += (size_t)max_dim * (max_dim + 7);
+			FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(gm_state), hipMemcpyDeviceToHost,
+			                       ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			ctx->host_syncs++;
+			std::memcpy(&sh, pinned, sizeof(gm_state));
+			k = sh.k;
+			if (!sh.cycle_end) break; // converged or maxiter
+			// restart (gmres.hh:215-233)
+			back_solve_correct(k - 1);
+			if (!right) {
+				residual_into(b0);
+				precond(b0, res);
+			}
+			else
+				residual_into(res);
+			const double betar = std::sqrt(reduce_host(ctx, RED_L2SQ, res, nullptr));
+			start_cycle(betar);
+			++info->restarts;
+			k = 0;
+		}
+		if (k > 0) back_solve_correct(k - 1);
+		info->iters = sh.iters;
+		if (sh.iters) info->status = sh.status;
+		info->res_norm_final = (float)sh.vnorm;
 		info->sol_norm_final = (float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
 		if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
 	});

@@ -128,6 +128,8 @@ _SIGS = {
                                C.POINTER(Info)]),
     "fsx_bicgstab_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
                                      C.POINTER(Info)]),
+    "fsx_gmres_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
+                                  C.POINTER(Info)]),
     "fsx_cg_history": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int,
                                  C.POINTER(C.c_int)]),
     "fsx_ir_cg_solve": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double,
@@ -448,6 +450,15 @@ def bicgstab_solve(A: CSR, b: Vector, x: Vector, settings: Settings,
     check(lib().fsx_bicgstab_solve(A.ctx.h, A.h, dinv.h if dinv is not None else None,
                                    b.h, x.h, C.byref(settings), C.byref(info)),
           "fsx_bicgstab_solve")
+    return info
+
+
+def gmres_solve(A: CSR, b: Vector, x: Vector, settings: Settings,
+                dinv: Vector | None = None) -> Info:
+    info = Info()
+    check(lib().fsx_gmres_solve(A.ctx.h, A.h, dinv.h if dinv is not None else None,
+                                b.h, x.h, C.byref(settings), C.byref(info)),
+          "fsx_gmres_solve")
     return info
 
 

@@ -210,6 +210,17 @@ int fsx_bicgstab_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
                        const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
                        fsx_info * info);
 
+/* fsx_gmres_solve: op::gmres::apply (solvers/gmres.hh:79-258) with identity
+ * (dinv == NULL) or Jacobi preconditioning on the side s->pre_side, restart
+ * length s->max_krylov_dim (<= 100; < 0 means maxiter), settings validated
+ * as gmres::settings::validate (gmres.hh:394-405).  Modified Gram-Schmidt,
+ * Givens rotations and the restart rule as the reference; each restart
+ * cycle is one captured graph, back_solve and correct run between cycles.
+ * FP64. */
+int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                    const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                    fsx_info * info);
+
 /* Mixed-precision iterative refinement (new, no reference counterpart):
  * outer FP64 r = b - A64 x, |r|/|b| check, inner FP32 CG(+Jacobi) on A32 to
  * `inner->rtol` with FP64 reductions, FP64 update x += |r| d. */
