@@ -558,6 +558,35 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 	if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
 }
 
+// ---- standalone fused CG update (cg.hh:108-127 in one pass) -----------------
+
+template<bool JAC>
+__global__ __launch_bounds__(kUpdThreads) void cg_update_full_kernel(
+	size_t n, double alpha, double * __restrict__ x, double * __restrict__ r,
+	const double * __restrict__ p, const double * __restrict__ w,
+	const double * __restrict__ dinv, double * __restrict__ z, double * partials) {
+	const unsigned blk = blockIdx.x;
+	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
+	double rr = 0.0, rz = 0.0;
+#pragma unroll
+	for (int j = 0; j < kUpdPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			x[i] = alpha * p[i] + x[i];
+			const double ri = (-alpha) * w[i] + r[i];
+			r[i] = ri;
+			rr += ri * ri;
+			if constexpr (JAC) {
+				const double zi = 0.0 + dinv[i] * ri;
+				z[i] = zi;
+				rz += ri * zi;
+			}
+		}
+	}
+	double d[2] = {rr, JAC ? rz : rr};
+	store_block_partials<2>(d, partials, blk);
+}
+
 // ---- fused BiCGSTAB (op::bicgstab::apply, solvers/bicgstab.hh:39-187) ----
 //
 // Same operation sequence, tolerances and solve_info as the reference; the
@@ -1657,6 +1686,60 @@ int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 		info->res_norm_final = (float)sh.vnorm;
 		info->sol_norm_final = (float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
 		if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
+	});
+}
+
+int fsx_cg_update(fsx_ctx * ctx, double alpha, const fsx_vec * p, const fsx_vec * w,
+                  const fsx_vec * dinv, fsx_vec * x, fsx_vec * r, fsx_vec * z, double * rr,
+                  double * rz) {
+	return guard([&] {
+		FSX_REQUIRE(ctx, "null context");
+		check_vec(p, "p");
+		check_vec(w, "w");
+		check_vec(x, "x");
+		check_vec(r, "r");
+		check_same(p, w);
+		check_same(p, x);
+		check_same(p, r);
+		FSX_REQUIRE(p->dt == FSX_F64, "cg_update is FP64");
+		if (dinv) {
+			check_vec(z, "z");
+			check_same(p, dinv);
+			check_same(p, z);
+		}
+		const void * ptrs[] = {p->ptr(), w->ptr(), x->ptr(), r->ptr()};
+		for (int a = 0; a < 4; ++a)
+			for (int b2 = a + 1; b2 < 4; ++b2)
+				FSX_REQUIRE(ptrs[a] != ptrs[b2] || p->n == 0, "cg_update: p, w, x, r must differ");
+		const size_t n = p->n;
+		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+		const unsigned g = (unsigned)std::max<size_t>(1, (n + per - 1) / per);
+		ensure_partials(ctx, 2 * (size_t)g);
+		double * slot = ctx->ws.scalars + 2;
+		if (dinv)
+			hipLaunchKernelGGL((cg_update_full_kernel<true>), dim3(g), dim3(kUpdThreads), 0,
+			                   ctx->stream, n, alpha, (double *)x->ptr(), (double *)r->ptr(),
+			                   (const double *)p->ptr(), (const double *)w->ptr(),
+			                   (const double *)dinv->ptr(), (double *)z->ptr(),
+			                   ctx->ws.partials);
+		else
+			hipLaunchKernelGGL((cg_update_full_kernel<false>), dim3(g), dim3(kUpdThreads), 0,
+			                   ctx->stream, n, alpha, (double *)x->ptr(), (double *)r->ptr(),
+			                   (const double *)p->ptr(), (const double *)w->ptr(),
+			                   (const double *)nullptr, (double *)nullptr, ctx->ws.partials);
+		hipLaunchKernelGGL((finalize_kernel<2, fin_store<2>>), dim3(1), dim3(kFinThreads), 0,
+		                   ctx->stream, (const double *)ctx->ws.partials, g,
+		                   fin_store<2>{slot});
+		FSX_HIP(hipGetLastError());
+		ctx->launches += 2;
+		double out[2];
+		FSX_HIP(hipMemcpyAsync(ctx->host_scalars, slot, 2 * sizeof(double),
+		                       hipMemcpyDeviceToHost, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		out[0] = ctx->host_scalars[0];
+		out[1] = ctx->host_scalars[1];
+		if (rr) *rr = out[0];
+		if (rz) *rz = out[1];
 	});
 }
 

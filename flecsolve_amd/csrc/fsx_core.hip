@@ -325,6 +325,16 @@ extern "C" {
 
 const char * fsx_last_error(void) { return get_error(); }
 
+int fsx_reduce_async(fsx_ctx * ctx, int kind, const fsx_vec * x, const fsx_vec * y,
+                     double * d_out) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && d_out, "null argument");
+		FSX_REQUIRE(kind >= FSX_RED_DOT && kind <= FSX_RED_MAX, "unknown reduction");
+		FSX_REQUIRE(kind != FSX_RED_DOT || y, "dot needs y");
+		reduce_async(ctx, (red_kind)kind, x, kind == FSX_RED_DOT ? y : nullptr, d_out);
+	});
+}
+
 int fsx_ctx_create(int device, fsx_ctx ** out) {
 	return guard([&] {
 		FSX_REQUIRE(out, "null output");

@@ -831,6 +831,39 @@ int fsx_spmv_dot(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p,
 	});
 }
 
+int fsx_csr_set_values(fsx_ctx * ctx, fsx_csr * A, const void * vals) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A, "null argument");
+		FSX_REQUIRE(vals || A->nnz == 0, "null values");
+		const size_t vsz = A->vt == FSX_F64 ? 8 : 4;
+		if (A->nnz)
+			FSX_HIP(hipMemcpyAsync(A->vals->p, vals, (size_t)A->nnz * vsz,
+			                       hipMemcpyHostToDevice, ctx->stream));
+		refresh_sliced_values(ctx, A);
+		FSX_HIP(hipStreamSynchronize(ctx->stream)); // the host buffer may go away
+	});
+}
+
+int fsx_csr_set_values_f32_from_f64(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr * A64) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A32 && A64, "null argument");
+		FSX_REQUIRE(A32->vt == FSX_F32 && A64->vt == FSX_F64,
+		            "set_values_f32_from_f64: FP32 destination, FP64 source");
+		FSX_REQUIRE(A32->idx == A64->idx,
+		            "set_values_f32_from_f64: A32 must be fsx_csr_convert(A64, FSX_F32)");
+		const size_t n = (size_t)A64->nnz;
+		if (n) {
+			const unsigned g = (unsigned)((n + 255) / 256);
+			hipLaunchKernelGGL((cast_kernel<double, float>), dim3(g), dim3(256), 0,
+			                   ctx->stream, n, (const double *)A64->vals->p,
+			                   (float *)A32->vals->p);
+			FSX_HIP(hipGetLastError());
+			ctx->launches++;
+		}
+		refresh_sliced_values(ctx, A32);
+	});
+}
+
 int fsx_csr_diag_inv(fsx_ctx * ctx, const fsx_csr * A, fsx_vec * dinv) {
 	return guard([&] {
 		FSX_REQUIRE(ctx && A, "null argument");

@@ -163,6 +163,11 @@ _SIGS = {
     "fsx_coo_tocsr": (C.c_int, [C.c_int64, C.c_int64, _P, _P, _P, _P, _P, _P]),
     "fsx_csr_from_mm": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(_P)]),
     "fsx_csr_transpose": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "fsx_csr_set_values": (C.c_int, [_P, _P, _P]),
+    "fsx_csr_set_values_f32_from_f64": (C.c_int, [_P, _P, _P]),
+    "fsx_reduce_async": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "fsx_cg_update": (C.c_int, [_P, C.c_double, _P, _P, _P, _P, _P, _P,
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "fsx_csr_sampled_outer": (C.c_int, [_P, _P, _P, _P, C.c_double, _P]),
     "fsx_diffusion3d_values": (C.c_int, [_P, C.c_int, _P, C.c_double, C.c_double, _P]),
     "fsx_diffusion3d_coeff_grad": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_double, _P]),
@@ -358,6 +363,16 @@ class CSR:
 
     def residual(self, b: Vector, x: Vector, r: Vector):
         check(lib().fsx_csr_residual(self.ctx.h, self.h, b.h, x.h, r.h), "residual")
+
+    def set_values(self, vals):
+        v = np.ascontiguousarray(vals, np.float64 if self.vtype == F64 else np.float32)
+        assert v.size == self.nnz
+        check(lib().fsx_csr_set_values(self.ctx.h, self.h, _ptr(v)), "fsx_csr_set_values")
+
+    def set_values_from(self, A64: "CSR"):
+        """FP32 copy takes A64's current values (same structure)."""
+        check(lib().fsx_csr_set_values_f32_from_f64(self.ctx.h, self.h, A64.h),
+              "fsx_csr_set_values_f32_from_f64")
 
     def transpose(self) -> "CSR":
         h = C.c_void_p()

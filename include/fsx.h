@@ -137,6 +137,14 @@ int fsx_convert(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst, double scale)
  * All accumulate in FP64 with a fixed, size-determined order (deterministic
  * run to run).  Each returns the value to the host (synchronises). ------ */
 int fsx_dot(fsx_ctx * ctx, const fsx_vec * x, const fsx_vec * y, double * out);
+/* Device-scalar flavour of the reductions: the result goes to device memory
+ * at d_out (8 bytes), stream-ordered, no host synchronisation (the value a
+ * reference future's .get() would return; FSX_RED_L2SQ is the squared
+ * 2-norm). */
+enum { FSX_RED_DOT = 0, FSX_RED_L1 = 1, FSX_RED_L2SQ = 2, FSX_RED_INF = 3,
+       FSX_RED_MIN = 4, FSX_RED_MAX = 5 };
+int fsx_reduce_async(fsx_ctx * ctx, int kind, const fsx_vec * x, const fsx_vec * y,
+                     double * d_out);
 int fsx_nrm1(fsx_ctx * ctx, const fsx_vec * x, double * out);
 int fsx_nrm2(fsx_ctx * ctx, const fsx_vec * x, double * out);
 int fsx_nrminf(fsx_ctx * ctx, const fsx_vec * x, double * out);
@@ -163,6 +171,13 @@ int fsx_csr_info(const fsx_csr * A, int64_t * nrows, int64_t * ncols,
  * sums each row serially in CSR order except 1. */
 int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes);
 int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes);
+/* Replace A's values (nnz entries of A's value type, CSR order) from host
+ * memory; the SpMV layouts are refreshed in place (captured solver graphs
+ * stay valid). */
+int fsx_csr_set_values(fsx_ctx * ctx, fsx_csr * A, const void * vals);
+/* A32 (= fsx_csr_convert(A64, FSX_F32), shared structure) takes A64's
+ * current values rounded to FP32, on the device. */
+int fsx_csr_set_values_f32_from_f64(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr * A64);
 int fsx_csr_download(fsx_ctx * ctx, const fsx_csr * A, void * rowptr,
                      void * colind, void * vals);
 /* y = A x   (sparse::apply -> mult -> Ops::spmv, matrices/seq.hh:46-60) */
@@ -200,6 +215,11 @@ int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
                  fsx_info * info);
 /* Residual history of the last fsx_cg_solve on ctx (|r| per iteration). */
 int fsx_cg_history(fsx_ctx * ctx, double * hist, int cap, int * count);
+/* One pass of cg.hh:108-127: x += alpha p; r -= alpha w; z = D^{-1} r (when
+ * dinv is given); rr = r.r and rz = r.z (r.r without dinv). */
+int fsx_cg_update(fsx_ctx * ctx, double alpha, const fsx_vec * p, const fsx_vec * w,
+                  const fsx_vec * dinv, fsx_vec * x, fsx_vec * r, fsx_vec * z, double * rr,
+                  double * rz);
 
 /* fsx_bicgstab_solve: op::bicgstab::apply (solvers/bicgstab.hh:39-187) with
  * identity (dinv == NULL) or Jacobi preconditioning, fused kernels with

@@ -145,3 +145,53 @@ def test_vector_identity_and_refcount(ctx):
     fsx.check(fsx.lib().fsx_vec_retain(a.h))
     fsx.check(fsx.lib().fsx_vec_release(a.h))
     assert a.get().shape == (10,)
+
+
+def test_reduce_async_matches_sync(ctx):
+    import ctypes as C
+    n = 300_001
+    x = ctx.vec(O.set_random(n, 3) - 0.5)
+    y = ctx.vec(O.set_random(n, 4) - 0.5)
+    out = ctx.zeros(8)
+    base = C.c_void_p()
+    fsx.check(fsx.lib().fsx_vec_device_ptr(out.h, C.byref(base)))
+    for k in range(6):
+        fsx.check(fsx.lib().fsx_reduce_async(ctx.h, k, x.h, y.h, C.c_void_p(base.value + 8 * k)),
+                  "reduce_async")
+    got = out.get()
+    assert got[0] == fsx.dot(x, y)
+    assert got[1] == fsx.nrm1(x)
+    assert np.sqrt(got[2]) == fsx.nrm2(x)
+    assert got[3] == fsx.nrminf(x)
+    xh = x.get()
+    assert got[4] == xh.min() and got[5] == xh.max()
+    with pytest.raises(fsx.FsxError):
+        fsx.check(fsx.lib().fsx_reduce_async(ctx.h, 0, x.h, None, C.c_void_p(base.value)),
+                  "dot without y")
+
+
+@pytest.mark.parametrize("jac", [True, False])
+def test_cg_update(ctx, jac):
+    import ctypes as C
+    n = 100_003
+    rng = np.random.default_rng(8)
+    p, w, x, r = (rng.standard_normal(n) for _ in range(4))
+    dinv = rng.uniform(0.5, 2.0, n)
+    alpha = 0.37
+    pv, wv, xv, rv = ctx.vec(p), ctx.vec(w), ctx.vec(x), ctx.vec(r)
+    dv, zv = ctx.vec(dinv), ctx.zeros(n)
+    rr, rz = C.c_double(), C.c_double()
+    fsx.check(fsx.lib().fsx_cg_update(ctx.h, alpha, pv.h, wv.h, dv.h if jac else None, xv.h,
+                                      rv.h, zv.h if jac else None, C.byref(rr), C.byref(rz)),
+              "cg_update")
+    xe = O.axpy(alpha, p, x)
+    re = O.axpy(-alpha, w, r)
+    np.testing.assert_array_equal(xv.get(), xe)
+    np.testing.assert_array_equal(rv.get(), re)
+    assert abs(rr.value - O.dot(re, re)) <= 1e-12 * O.dot(re, re)
+    if jac:
+        ze = 0.0 + dinv * re
+        np.testing.assert_array_equal(zv.get(), ze)
+        assert abs(rz.value - O.dot(re, ze)) <= 1e-12 * np.abs(re * ze).sum()
+    else:
+        assert rz.value == rr.value

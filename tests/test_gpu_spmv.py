@@ -274,3 +274,33 @@ def test_csr_from_matrix_market(ctx, name):
     y = ctx.zeros(nr)
     A.spmv(ctx.vec(x), y)
     np.testing.assert_array_equal(y.get(), O.Csr(rp, ci, vv, ncols=nc).spmv(x))
+
+
+def test_set_values_refreshes_layouts(ctx):
+    n = 20
+    k1 = P.lognormal_field(n, 1.0, 3.0, 1)
+    k2 = P.lognormal_field(n, 1.0, 3.0, 2)
+    rp, ci, v1 = P.diffusion3d(n, k1)
+    _, _, v2 = P.diffusion3d(n, k2)
+    A = ctx.csr(rp, ci, v1)
+    A.set_plan(2)   # build the sliced layout too
+    A.set_plan(3)
+    A32 = A.convert(fsx.F32)
+    A.set_values(v2)
+    A32.set_values_from(A)
+    x = np.random.default_rng(3).standard_normal(n ** 3)
+    ref = O.Csr(rp, ci, v2).spmv(x)
+    xd, yd = ctx.vec(x), ctx.zeros(n ** 3)
+    for plan in (3, 2, 0):
+        A.set_plan(plan)
+        A.spmv(xd, yd)
+        np.testing.assert_array_equal(yd.get(), ref, err_msg=str(plan))
+    F = ctx.csr(rp, ci, v2).convert(fsx.F32)
+    x32 = ctx.vec(x.astype(np.float32))
+    y1, y2 = ctx.zeros(n ** 3, fsx.F32), ctx.zeros(n ** 3, fsx.F32)
+    A32.spmv(x32, y1)
+    F.spmv(x32, y2)
+    np.testing.assert_array_equal(y1.get(), y2.get())
+    with pytest.raises(fsx.FsxError):   # unrelated structure
+        B32 = ctx.csr(rp, ci, v1).convert(fsx.F32)
+        B32.set_values_from(A)
