@@ -56,26 +56,45 @@ def spmv_bytes(nrows, nnz, vbytes, ibytes=4, xbytes=8):
     return nnz * (vbytes + ibytes) + (nrows + 1) * ibytes + nrows * xbytes * 2
 
 
-def cpu_baseline(rp, ci, v, b, iters_gpu, sample_iters, threads=1):
-    """The oracle (serial restatement of the reference's CG + Dinv path) on a
+def cpu_threads():
+    """Host threads for the threaded CPU baseline: the CPUs this process may
+    use, at most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(rp, ci, v, b, iters_gpu, sample_iters):
+    """The oracle (restatement of the reference's CG + Dinv path) on a
     bounded sample of the same problem: `sample_iters` iterations, scaled to
-    the GPU solve's iteration count."""
+    the GPU solve's iteration count.  Timed twice: serial (the reference's
+    seq path, one thread) and with P colours on P threads (the reference's
+    MPI backend with one colour per rank: row blocks, block partials summed
+    in block order); `value` is the P-thread figure."""
     from oracle import oracle as O
     A = O.Csr(rp, ci, v)
     D = A.dinv()
     s = O.Settings(maxiter=sample_iters, rtol=1e-10, use_zero_guess=True)
     t0 = time.perf_counter()
     O.cg(A, b, np.zeros_like(b), s, P=O.prec(1, D))
-    dt = time.perf_counter() - t0
-    per_iter = dt / sample_iters
+    dt1 = time.perf_counter() - t0
+    P = cpu_threads()
+    dinv = D.vals
+    t0 = time.perf_counter()
+    O.cg_colours(A, dinv, b, np.zeros_like(b), s, P)
+    dtp = time.perf_counter() - t0
     return {
-        "value": per_iter * iters_gpu,
+        "value": dtp / sample_iters * iters_gpu,
         "unit": "s",
-        "cores": threads,
+        "cores": P,
         "kind": "port",
-        "sample": (f"{sample_iters} CG+Jacobi FP64 iterations of the same "
-                   f"{len(b)}-row problem on {threads} thread ({dt:.1f} s, "
-                   f"{per_iter*1e3:.0f} ms/iter), scaled to the GPU's "
+        "single_thread_value": dt1 / sample_iters * iters_gpu,
+        "sample": (f"{sample_iters} CG+Jacobi FP64 iterations of the same {len(b)}-row "
+                   f"problem: {P} colours on {P} threads {dtp:.1f} s "
+                   f"({dtp / sample_iters * 1e3:.0f} ms/iter), 1 thread {dt1:.1f} s "
+                   f"({dt1 / sample_iters * 1e3:.0f} ms/iter); scaled to the GPU's "
                    f"{iters_gpu} iterations"),
     }
 
@@ -279,7 +298,7 @@ def main():
         "roofline": roofline,
     }
     if not a.no_cpu_baseline:
-        log("cpu baseline (oracle, 1 thread) ...")
+        log(f"cpu baseline (oracle: 1 thread, then {cpu_threads()} colours/threads) ...")
         out["cpu_baseline"] = cpu_baseline(rp, ci, v, b, info64.iters,
                                            a.cpu_sample_iters)
     print(json.dumps(out), flush=True)

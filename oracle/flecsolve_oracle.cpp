@@ -49,6 +49,7 @@
 // topo_tasks.hh:140-142.
 
 #include <algorithm>
+#include <omp.h>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -900,6 +901,107 @@ void or_jacobi_relax(const or_csr * A, float omega, int nrelax,
 	vec tmp(A->nrows);
 	for (int s = 0; s < nrelax; ++s)
 		jacobi_relax(*A, (double)omega, b, x, tmp.data());
+}
+
+// ---- P-colour CPU baseline ---------------------------------------------------
+// op::cg::apply with Jacobi (z = 0 + dinv r) as the reference runs it on the
+// FleCSI MPI backend with P ranks of one colour each: rows in P contiguous
+// blocks, every vector operation and the SpMV of a block on its own thread,
+// each dot product and norm a serial partial per block, the partials summed
+// in block order.  With P = 1 it is op-for-op the serial or_cg.  Used only
+// as the threaded CPU timing baseline.
+
+} // extern "C"
+
+template<class I>
+static int cg_colours_t(const or_csr & A, const double * dinv, const double * b, double * x,
+                        const or_settings & s, int P, or_info * info) {
+	zero_info(info);
+	const int64_t n = A.nrows;
+	const I * rp = static_cast<const I *>(A.rowptr);
+	const I * ci = static_cast<const I *>(A.colind);
+	const double * va = A.vals;
+	std::vector<int64_t> lo(P + 1);
+	for (int t = 0; t <= P; ++t) lo[t] = n * t / P;
+	vec r(n), z(n), p(n), w(n);
+	std::vector<double> part(P);
+	auto reduce = [&](auto && f) {
+#pragma omp parallel num_threads(P)
+		{
+			const int t = omp_get_thread_num();
+			double acc = 0.0;
+			for (int64_t i = lo[t]; i < lo[t + 1]; ++i) acc += f(i);
+			part[t] = acc;
+		}
+		double tot = 0.0;
+		for (int t = 0; t < P; ++t) tot += part[t];
+		return tot;
+	};
+	auto each = [&](auto && f) {
+#pragma omp parallel num_threads(P)
+		{
+			const int t = omp_get_thread_num();
+			for (int64_t i = lo[t]; i < lo[t + 1]; ++i) f(i);
+		}
+	};
+	auto spmv_rows = [&](const double * in, double * out) {
+		each([&](int64_t i) {
+			out[i] = 0.;
+			for (int64_t off = rp[i]; off < rp[i + 1]; ++off) out[i] += va[off] * in[ci[off]];
+		});
+	};
+	double b_norm = std::sqrt(reduce([&](int64_t i) { return b[i] * b[i]; }));
+	if (b_norm == 0.0) b_norm = 1.0;
+	const double tol = s.rtol * b_norm;
+	info->rhs_norm = (float)b_norm;
+	if (s.use_zero_guess) {
+		each([&](int64_t i) { x[i] = 0.; r[i] = b[i]; });
+	}
+	else {
+		spmv_rows(x, r.data());
+		each([&](int64_t i) { r[i] = b[i] - r[i]; });
+	}
+	double res = std::sqrt(reduce([&](int64_t i) { return r[i] * r[i]; }));
+	if (res < tol) {
+		info->res_norm_initial = info->res_norm_final = (float)res;
+		info->status = OR_CONVERGED_RTOL;
+		return 0;
+	}
+	each([&](int64_t i) { z[i] = 0. + dinv[i] * r[i]; });
+	double rho0, rho1 = reduce([&](int64_t i) { return z[i] * r[i]; });
+	each([&](int64_t i) { p[i] = z[i]; });
+	for (int iter = 0; iter < s.maxiter; ++iter) {
+		spmv_rows(p.data(), w.data());
+		double alpha = reduce([&](int64_t i) { return w[i] * p[i]; });
+		alpha = rho1 / alpha;
+		each([&](int64_t i) {
+			x[i] = alpha * p[i] + x[i];
+			r[i] = (-alpha) * w[i] + r[i];
+		});
+		res = std::sqrt(reduce([&](int64_t i) { return r[i] * r[i]; }));
+		if (res < tol) {
+			info->iters = iter + 1;
+			info->status = OR_CONVERGED_RTOL;
+			break;
+		}
+		each([&](int64_t i) { z[i] = 0. + dinv[i] * r[i]; });
+		rho0 = rho1;
+		rho1 = reduce([&](int64_t i) { return r[i] * z[i]; });
+		const double beta = rho1 / rho0;
+		each([&](int64_t i) { p[i] = beta * p[i] + z[i]; });
+	}
+	info->res_norm_final = (float)res;
+	if (info->iters == 0) info->status = OR_DIVERGED_ITERS;
+	return 0;
+}
+
+extern "C" {
+
+int or_cg_colours(const or_csr * A, const double * dinv, const double * b, double * x,
+                  const or_settings * s, int ncolours, or_info * info) {
+	if (ncolours < 1) return -1;
+	return A->itype == 0 ? cg_colours_t<int32_t>(*A, dinv, b, x, *s, ncolours, info)
+	                     : cg_colours_t<int64_t>(*A, dinv, b, x, *s, ncolours, info);
 }
 
 // ---- solvers ---------------------------------------------------------------

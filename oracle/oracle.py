@@ -103,6 +103,9 @@ def lib():
         L.or_bicgstab.argtypes = solver_args + [_D, C.c_int]
         for nm in ["or_cg", "or_fcg", "or_gmres", "or_bicgstab"]:
             getattr(L, nm).restype = C.c_int
+        L.or_cg_colours.argtypes = [C.POINTER(_Csr), P, P, P, C.POINTER(Settings), C.c_int,
+                                    C.POINTER(Info)]
+        L.or_cg_colours.restype = C.c_int
         _lib = L
     return _lib
 
@@ -182,6 +185,21 @@ def _solve(name, A: Csr, b, x0, settings: Settings, P=None, diag=None,
         args += [hist.ctypes.data_as(_D), hist_cap]
     rc = getattr(lib(), name)(*args)
     return x, info, rc, hist[:hist_cap]
+
+
+def cg_colours(A: Csr, dinv, b, x0, settings: Settings, ncolours: int):
+    """CG + Jacobi over `ncolours` row blocks on as many threads, block
+    partials summed in block order (the reference's MPI backend with one
+    colour per rank).  CPU baseline only."""
+    b = np.ascontiguousarray(b, np.float64)
+    d = np.ascontiguousarray(dinv, np.float64)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    info = Info()
+    rc = lib().or_cg_colours(C.byref(A.s), _p(d), _p(b), _p(x), C.byref(settings),
+                             int(ncolours), C.byref(info))
+    if rc != 0:
+        raise ValueError("ncolours must be >= 1")
+    return x, info
 
 
 def cg(A, b, x0, settings, P=None, diag=None, hist_cap=0):

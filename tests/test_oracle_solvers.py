@@ -198,3 +198,23 @@ def test_bicgstab_direct_and_identity_quirk():
                                                            use_zero_guess=True))
     np.testing.assert_array_equal(x, bb)
     assert info.iters == 0 and info.reason == "diverged_iters"
+
+
+def test_cg_colours_baseline():
+    # one colour is op-for-op the serial CG + Dinv; several colours change
+    # only the summation order of the reductions
+    from flecsolve_amd import problems as P
+    n = 20
+    rp, ci, v = P.diffusion3d(n, P.lognormal_field(n, 1.0, 4.0, 3))
+    A = O.Csr(rp, ci, v)
+    b = P.uniform(n ** 3, 4)
+    dinv = 1.0 / v[rp[:-1] + np.array([np.where(ci[rp[i]:rp[i + 1]] == i)[0][0]
+                                       for i in range(n ** 3)])]
+    s = O.Settings(maxiter=1000, rtol=1e-10, use_zero_guess=True)
+    xs, io, _ = O.cg(A, b, np.zeros(n ** 3), s, P=O.prec(1, A.dinv()))
+    x1, i1 = O.cg_colours(A, dinv, b, np.zeros(n ** 3), s, 1)
+    np.testing.assert_array_equal(x1, xs)
+    assert i1.iters == io.iters
+    x4, i4 = O.cg_colours(A, dinv, b, np.zeros(n ** 3), s, 4)
+    assert abs(i4.iters - io.iters) <= 2
+    assert np.linalg.norm(x4 - xs) <= 1e-10 * np.linalg.norm(xs)
