@@ -135,6 +135,7 @@ __device__ __forceinline__ void wg_sum(double (&v)[NV], double * lds) {
 // first use; absent entries re-read the row's own x and are dropped by a
 // select, bitwise the same as skipping them).
 constexpr int kBatchTile = 4;
+static_assert(kBatchTile <= kSliceUnroll, "offset/mask padding covers one tile");
 
 template<class V, class X, class F>
 __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
@@ -158,8 +159,11 @@ __device__ __forceinline__ void wg_spmv(const batch_args & a, const V * sl_v,
 #pragma unroll
 				for (int k = 0; k < kBatchTile; ++k) {
 					const int jj = min(j0 + k, wd - 1);
-					off[k] = a.dg_off[q0 + jj];
-					pr[k] = j0 + k < wd && ((a.dg_mask[q0 + jj] >> lane) & 1);
+					// consecutive unconditional loads (the offset and mask arrays
+					// are padded): wide scalar loads behind one wait
+					off[k] = a.dg_off[q0 + j0 + k];
+					const uint64_t m = a.dg_mask[q0 + j0 + k];
+					pr[k] = (j0 + k < wd) & (((m >> lane) & 1) != 0);
 					vv[k] = dg_v[base + (int64_t)jj * kSliceRows + lane];
 				}
 #pragma unroll

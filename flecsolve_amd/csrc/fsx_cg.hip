@@ -61,15 +61,24 @@ struct epi_cg_spmv {
 	__device__ bool skip() const { return st->done != 0; }
 	template<class XI>
 	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI && pi) const {
+		// Loads before the store: the store would otherwise sit in front of
+		// the load in the wave's memory counter and be waited for.
+		const double pv = (double)pi();
 		const T wi = (T)s;
 		w[i] = wi;
-		d[0] += (double)wi * (double)pi(); // w.dot(p)
+		d[0] += (double)wi * pv; // w.dot(p)
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk,
 	                          unsigned) const {
 		store_block_partials<NV>(d, partials, blk);
 	}
 };
+
+// Row-sum type of K1: FP32 when matrix and vectors are FP32 (the product
+// compressed_ops::spmv computes for a float instantiation), else FP64.
+template<class T, class V>
+using cg_acc_t = std::conditional_t<std::is_same_v<T, float> && std::is_same_v<V, float>,
+                                    float, double>;
 
 // alpha = rho / (w.p)  (cg.hh:98-106; negative curvature is only logged)
 struct fin_alpha {
@@ -249,22 +258,14 @@ struct cg_graph_cache {
 		hipGraph_t g = nullptr;
 		hipGraphExec_t ge = nullptr;
 	};
-	using key_t = std::tuple<const void *, const void *, const void *, const void *,
-	                         const void *, const void *, const void *, const void *,
-	                         const void *, const void *, int64_t, int64_t, int, int,
-	                         int, int>;
-	std::map<key_t, entry> m;
-	std::map<std::vector<const void *>, entry> mv; // other fused drivers
+	// key: the driver, every device address a captured launch uses and the
+	// sizes and plan its launch shapes follow from
+	std::map<std::vector<const void *>, entry> mv;
 	std::map<std::tuple<size_t, int, int>, fsx_vec *> pool;
 	fsx::dbuf * statebuf = nullptr;
 	fsx::dbuf * histbuf = nullptr;
 	size_t hist_cap = 0;
 	void clear_graphs() {
-		for (auto & kv : m) {
-			if (kv.second.ge) (void)hipGraphExecDestroy(kv.second.ge);
-			if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
-		}
-		m.clear();
 		for (auto & kv : mv) {
 			if (kv.second.ge) (void)hipGraphExecDestroy(kv.second.ge);
 			if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
@@ -280,6 +281,23 @@ struct cg_graph_cache {
 };
 
 void cg_cache_deleter::operator()(cg_graph_cache * c) const { delete c; }
+
+namespace fsx {
+// Appends every device array a product on A may read (all layouts), and the
+// sizes and plan its launch shapes follow from.  A graph keyed with them is
+// replayed only when each address it captured belongs to the matrix of the
+// current call.
+static void append_layout(std::vector<const void *> & key, const fsx_csr * A) {
+	auto p = [](const std::unique_ptr<dbuf> & b) -> const void * { return b ? b->p : nullptr; };
+	const csr_index & X = *A->idx;
+	key.insert(key.end(), {p(X.rowptr), p(X.colind), p(A->vals), p(X.sl_ptr), p(X.sl_col),
+	                       p(A->sl_vals), p(X.dg_ptr), p(X.dg_off), p(X.dg_mask),
+	                       p(A->dg_vals), (const void *)(intptr_t)A->nrows,
+	                       (const void *)(intptr_t)A->nnz,
+	                       (const void *)(intptr_t)A->plan_kind,
+	                       (const void *)(intptr_t)A->plan_lanes});
+}
+} // namespace fsx
 
 void fsx::invalidate_graphs(fsx_ctx * ctx) {
 	if (ctx->cg_cache) ctx->cg_cache->clear_graphs();
@@ -308,10 +326,10 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	double * part = ctx->ws.partials;
 	T * p = (T *)B.p;
 	// K1 + F1
-	epi_cg_spmv<double, T> e1{(T *)B.w, part, st};
-	spmv_launch_typed<decltype(e1), double, V, I, T>(s, A, (const T *)p, e1);
-	hipLaunchKernelGGL((finalize_kernel<1, fin_alpha>), dim3(1), dim3(kFinThreads),
-	                   0, s, (const double *)part, spmv_blocks(A), fin_alpha{st});
+	using ACC = cg_acc_t<T, V>;
+	epi_cg_spmv<ACC, T> e1{(T *)B.w, part, st};
+	spmv_launch_typed<decltype(e1), ACC, V, I, T>(s, A, (const T *)p, e1);
+	spmv_finalize<1>(s, A, part, fin_alpha{st});
 	// K2 + F2
 	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
 	const unsigned g2 = (unsigned)((B.n + per - 1) / per);
@@ -408,22 +426,21 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 	{
 		// partials for the SpMV+dot (1 per block) and the update (2 per block)
 		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
-		const size_t need = std::max<size_t>(spmv_blocks(B.A), 2 * ((B.n + per - 1) / per));
+		const size_t need = std::max<size_t>(spmv_partials_size(B.A, 1),
+		                                     2 * ((B.n + per - 1) / per));
 		ensure_partials(ctx, need);
 	}
 	hipGraphExec_t ge = nullptr;
 	if (use_graphs()) {
-		const fsx_csr * A = B.A;
-		const void * layout_vals = A->plan_kind == 3 && A->dg_vals ? A->dg_vals->p
-		                           : A->sl_vals                   ? A->sl_vals->p
-		                                                          : nullptr;
-		cg_graph_cache::key_t key{A->rowptr(), layout_vals,
-		                          A->vals->p, B.x, B.r, B.z, B.p, B.p2, B.w, B.dinv,
-		                          A->nrows, A->nnz, A->plan_kind, A->plan_lanes,
-		                          (int)sizeof(T), K};
-		auto it = cache.m.find(key);
-		if (it == cache.m.end()) {
-			if (cache.m.size() >= 32) cache.clear_graphs();
+		std::vector<const void *> key{(const void *)&enqueue_iteration<T, V, I>,
+		                              B.x, B.r, B.z, B.p, B.p2, B.w, B.dinv,
+		                              (const void *)ctx->ws.partials, (const void *)st,
+		                              (const void *)hist, (const void *)(intptr_t)B.n,
+		                              (const void *)(intptr_t)K};
+		append_layout(key, B.A);
+		auto it = cache.mv.find(key);
+		if (it == cache.mv.end()) {
+			if (cache.mv.size() >= 32) cache.clear_graphs();
 			cg_graph_cache::entry en;
 			FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
 			try {
@@ -438,7 +455,7 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 			}
 			FSX_HIP(hipStreamEndCapture(ctx->stream, &en.g));
 			FSX_HIP(hipGraphInstantiate(&en.ge, en.g, nullptr, nullptr, 0));
-			cache.m[key] = en;
+			cache.mv[key] = en;
 			ge = en.ge;
 		}
 		else
@@ -643,9 +660,10 @@ struct epi_bcg_v {
 	__device__ bool skip() const { return st->done != 0; }
 	template<class XI>
 	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI &&) const {
+		const double rti = (double)rt[i]; // load before the store
 		const T vi = (T)s;
 		v[i] = vi;
-		d[0] += (double)rt[i] * (double)vi; // r_tilde.dot(v)
+		d[0] += rti * (double)vi; // r_tilde.dot(v)
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk, unsigned) const {
 		store_block_partials<NV>(d, partials, blk);
@@ -713,10 +731,11 @@ struct epi_bcg_t {
 	__device__ bool skip() const { return st->done != 0; }
 	template<class XI>
 	__device__ void row(int64_t i, ACC sum, double (&d)[NV], XI &&) const {
+		const double si = (double)s[i]; // load before the store
 		const T ti = (T)sum;
 		t[i] = ti;
 		d[0] += (double)ti * (double)ti; // t.dot(t)
-		d[1] += (double)ti * (double)s[i]; // t.dot(s)
+		d[1] += (double)ti * si; // t.dot(s)
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk, unsigned) const {
 		store_block_partials<NV>(d, partials, blk);
@@ -831,8 +850,7 @@ static void bcg_enqueue_iteration(fsx_ctx * ctx, const bcg_bufs & B, bcg_state *
 	// B2 + G2
 	epi_bcg_v<double, double> e2{B.v, B.rt, part, st};
 	spmv_launch_typed<decltype(e2), double, V, I, double>(s, B.A, (const double *)B.ph, e2);
-	hipLaunchKernelGGL((finalize_kernel<1, fin_bcg_alpha>), dim3(1), dim3(kFinThreads), 0,
-	                   s, (const double *)part, spmv_blocks(B.A), fin_bcg_alpha{st});
+	spmv_finalize<1>(s, B.A, part, fin_bcg_alpha{st});
 	// B3 + G3
 	if (jac)
 		hipLaunchKernelGGL((bcg_s_kernel<true>), dim3(gu), dim3(kUpdThreads), 0, s, B.n, B.s,
@@ -847,8 +865,7 @@ static void bcg_enqueue_iteration(fsx_ctx * ctx, const bcg_bufs & B, bcg_state *
 	// B4 + G4
 	epi_bcg_t<double, double> e4{B.t, B.s, part, st};
 	spmv_launch_typed<decltype(e4), double, V, I, double>(s, B.A, (const double *)B.sh, e4);
-	hipLaunchKernelGGL((finalize_kernel<2, fin_bcg_omega>), dim3(1), dim3(kFinThreads), 0,
-	                   s, (const double *)part, spmv_blocks(B.A), fin_bcg_omega{st});
+	spmv_finalize<2>(s, B.A, part, fin_bcg_omega{st});
 	// B5 + G5
 	hipLaunchKernelGGL(bcg_x_kernel, dim3(gu), dim3(kUpdThreads), 0, s, B.n, B.x, B.res,
 	                   (const double *)B.ph, (const double *)B.sh, (const double *)B.t,
@@ -874,24 +891,20 @@ static void bcg_run_chunks(fsx_ctx * ctx, const bcg_bufs & B, bcg_state & st_hos
 	const int K = chunk_size();
 	{
 		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
-		ensure_partials(ctx, std::max<size_t>(2 * (size_t)spmv_blocks(B.A),
+		ensure_partials(ctx, std::max<size_t>(spmv_partials_size(B.A, 2),
 		                                      2 * ((B.n + per - 1) / per)));
 	}
 	hipGraphExec_t ge = nullptr;
 	if (use_graphs()) {
-		const fsx_csr * A = B.A;
-		const void * layout_vals = A->plan_kind == 3 && A->dg_vals ? A->dg_vals->p
-		                           : A->sl_vals                   ? A->sl_vals->p
-		                                                          : nullptr;
 		std::vector<const void *> key{(const void *)&bcg_enqueue_iteration<V, I>,
-		                              A->rowptr(), layout_vals, A->vals->p, B.x, B.res,
-		                              B.rt, B.p, B.v, B.ph, B.s, B.sh, B.t, B.dinv,
-		                              (const void *)ctx->ws.partials, (const void *)st,
-		                              (const void *)(intptr_t)A->plan_kind,
-		                              (const void *)(intptr_t)B.n, (const void *)(intptr_t)K};
+		                              B.x, B.res, B.rt, B.p, B.v, B.ph, B.s, B.sh, B.t,
+		                              B.dinv, (const void *)ctx->ws.partials,
+		                              (const void *)st, (const void *)(intptr_t)B.n,
+		                              (const void *)(intptr_t)K};
+		append_layout(key, B.A);
 		auto it = cache.mv.find(key);
 		if (it == cache.mv.end()) {
-			if (cache.mv.size() >= 16) cache.clear_graphs();
+			if (cache.mv.size() >= 32) cache.clear_graphs();
 			cg_graph_cache::entry en;
 			FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
 			try {
@@ -998,10 +1011,11 @@ struct epi_gm_v {
 	__device__ bool skip() const { return st->done != 0; }
 	template<class XI>
 	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI &&) const {
+		const double b0i = (double)b0[i]; // load before the store
 		T vi = (T)s;
 		if constexpr (LEFTJ) vi = 0.0 + dinv[i] * vi; // P.apply(z, v)
 		v[i] = vi;
-		d[0] += (double)vi * (double)b0[i]; // v.dot(basis[0])
+		d[0] += (double)vi * b0i; // v.dot(basis[0])
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk, unsigned) const {
 		store_block_partials<NV>(d, partials, blk);
@@ -1170,18 +1184,19 @@ static void gm_enqueue_cycle(fsx_ctx * ctx, const gm_bufs & B, gm_state * st) {
 			epi_gm_v<double, double, false> e{B.v, B.basis[0], nullptr, part, st};
 			spmv_launch_typed<decltype(e), double, V, I, double>(s, B.A, bk, e);
 		}
-		unsigned nblk = spmv_blocks(B.A);
 		for (int j = 0; j <= k; ++j) {
-			// h_jk from the partials just written
-			hipLaunchKernelGGL((finalize_kernel<1, fin_gm_h>), dim3(1), dim3(kFinThreads), 0,
-			                   s, (const double *)part, nblk,
-			                   fin_gm_h{st, B.H, j + B.ld * k, false});
+			// h_jk from the partials just written (the product's, then MGS's)
+			const fin_gm_h fh{st, B.H, j + B.ld * k, false};
+			if (j == 0)
+				spmv_finalize<1>(s, B.A, part, fh);
+			else
+				hipLaunchKernelGGL((finalize_kernel<1, fin_gm_h>), dim3(1), dim3(kFinThreads),
+				                   0, s, (const double *)part, gu, fh);
 			hipLaunchKernelGGL(gm_mgs_kernel, dim3(gu), dim3(kUpdThreads), 0, s, B.n, B.v,
 			                   (const double *)B.basis[j],
 			                   j < k ? (const double *)B.basis[j + 1] : nullptr,
 			                   (const double *)(B.H + j + (size_t)B.ld * k), part,
 			                   (const gm_state *)st);
-			nblk = gu;
 		}
 		// GN: h_{k+1,k} = |v|; b_{k+1} = v / |v|
 		hipLaunchKernelGGL((finalize_kernel<1, fin_gm_h>), dim3(1), dim3(kFinThreads), 0, s,
@@ -1292,15 +1307,12 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 				spmv_launch_typed<decltype(e), double, V, I, double>(
 					ctx->stream, A64, (const double *)x->ptr(), e);
 			});
-			hipLaunchKernelGGL((finalize_kernel<1, fin_store<1>>), dim3(1),
-			                   dim3(kFinThreads), 0, ctx->stream,
-			                   (const double *)ctx->ws.partials, spmv_blocks(A64),
-			                   fin_store<1>{rn2});
+			spmv_finalize<1>(ctx->stream, A64, ctx->ws.partials, fin_store<1>{rn2});
 			FSX_HIP(hipGetLastError());
 			ctx->launches += 2;
 			return std::sqrt(fetch_scalar(ctx, rn2));
 		};
-		ensure_partials(ctx, spmv_blocks(A64));
+		ensure_partials(ctx, spmv_partials_size(A64, 1));
 		double rnorm = residual();
 		const unsigned g = (unsigned)((n + 255) / 256);
 		fsx_settings is = *inner;
@@ -1564,7 +1576,7 @@ int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 		gm_state * pinned = (gm_state *)ctx->host_scalars;
 		static_assert(sizeof(gm_state) <= sizeof(cg_state) * 4, "");
 		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
-		ensure_partials(ctx, std::max<size_t>(spmv_blocks(A), (n + per - 1) / per));
+		ensure_partials(ctx, std::max<size_t>(spmv_partials_size(A, 1), (n + per - 1) / per));
 		gm_state sh{};
 		sh.tol = tol;
 		sh.m = max_dim;
@@ -1621,22 +1633,19 @@ int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 				using I = std::remove_pointer_t<decltype(itag)>;
 				hipGraphExec_t ge = nullptr;
 				if (use_graphs()) {
-					const void * layout_vals = A->plan_kind == 3 && A->dg_vals ? A->dg_vals->p
-					                           : A->sl_vals ? A->sl_vals->p : nullptr;
 					std::vector<const void *> key{(const void *)&gm_enqueue_cycle<V, I>,
-					                              A->rowptr(), layout_vals, A->vals->p,
 					                              (const void *)B.v, (const void *)H,
 					                              (const void *)B.dinv,
 					                              (const void *)ctx->ws.partials,
 					                              (const void *)st,
-					                              (const void *)(intptr_t)A->plan_kind,
 					                              (const void *)(intptr_t)n,
 					                              (const void *)(intptr_t)max_dim,
 					                              (const void *)(intptr_t)right};
 					for (double * bp : basis) key.push_back(bp);
+					append_layout(key, A);
 					auto it = cache.mv.find(key);
 					if (it == cache.mv.end()) {
-						if (cache.mv.size() >= 16) cache.clear_graphs();
+						if (cache.mv.size() >= 32) cache.clear_graphs();
 						cg_graph_cache::entry en;
 						FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
 						try {

@@ -54,9 +54,10 @@ struct epi_dot {
 	__device__ bool skip() const { return false; }
 	template<class XI>
 	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI && pi) const {
+		const double pv = (double)pi(); // load before the store
 		const Y wi = (Y)s;
 		w[i] = wi;
-		d[0] += (double)wi * (double)pi();
+		d[0] += (double)wi * pv;
 	}
 	__device__ void block_end(const double (&d)[NV], unsigned blk,
 	                          unsigned) const {
@@ -309,6 +310,9 @@ bool ensure_sdia(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
 				                        (const int64_t *)host_colind, check_budget, ptr,
 				                        off, mask);
 		if (!ok) return false;
+		// padding: the kernels read a full tile of offsets and masks
+		off.resize(off.size() + kSliceUnroll, 0);
+		mask.resize(mask.size() + kSliceUnroll, 0);
 		X.dg_nnz = ptr.back();
 		X.dg_ptr = std::make_unique<dbuf>(ptr.size() * sizeof(int64_t));
 		X.dg_off = std::make_unique<dbuf>(std::max<size_t>(1, off.size()) * sizeof(int32_t));
@@ -684,6 +688,8 @@ int fsx_csr_destroy(fsx_csr * A) {
 		if (!A) return;
 		fsx_ctx * c = A->ctx;
 		(void)hipStreamSynchronize(c->stream);
+		// captured solver graphs may hold this matrix's arrays
+		invalidate_graphs(c);
 		delete A;
 		ctx_release(c);
 	});
@@ -813,7 +819,7 @@ int fsx_spmv_dot(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p,
 		void * wp = w->ptr();
 		const void * pp = p->ptr();
 		double * slot = ctx->ws.scalars + 1;
-		ensure_partials(ctx, spmv_blocks(A));
+		ensure_partials(ctx, spmv_partials_size(A, 1));
 		spmv_dispatch<epi_dot>(ctx, A, p->ptr(), p->dt, [&](auto * t) {
 			using E = std::remove_pointer_t<decltype(t)>;
 			E e{};
@@ -822,10 +828,7 @@ int fsx_spmv_dot(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p,
 			e.partials = ctx->ws.partials;
 			return e;
 		});
-		hipLaunchKernelGGL((finalize_kernel<1, fin_store<1>>), dim3(1),
-		                   dim3(kFinThreads), 0, ctx->stream,
-		                   (const double *)ctx->ws.partials, spmv_blocks(A),
-		                   fin_store<1>{slot});
+		spmv_finalize<1>(ctx->stream, A, ctx->ws.partials, fin_store<1>{slot});
 		FSX_HIP(hipGetLastError());
 		*out = fetch_scalar(ctx, slot);
 	});

@@ -291,8 +291,13 @@ __global__ __launch_bounds__(BT) void spmv_sdia_kernel(
 #pragma unroll
 			for (int k = 0; k < kSliceUnroll; ++k) {
 				const int j = min(j0 + k, w - 1);
-				off[k] = dg_off[q0 + j];
-				pr[k] = j0 + k < w && ((dg_mask[q0 + j] >> lane) & 1);
+				// Offsets and masks: unconditional scalar loads of 8 consecutive
+				// entries (the arrays carry kSliceUnroll entries of padding),
+				// which merge into wide loads behind one wait.  A short-circuit
+				// && or a clamped index here gives one load-and-wait per entry.
+				off[k] = dg_off[q0 + j0 + k];
+				const uint64_t m = dg_mask[q0 + j0 + k];
+				pr[k] = (j0 + k < w) & (((m >> lane) & 1) != 0);
 				v[k] = ldnt(vp + (int64_t)j * kSliceRows);
 			}
 #pragma unroll
@@ -320,20 +325,29 @@ inline unsigned sliced_blocks(int64_t nrows, bool reduces = true) {
 }
 
 // Slice-offset launches.  Plain products: 256-thread blocks, one slice per
-// wave.  Reducing epilogues (one partial per block), measured on the 256^3
-// CG iteration: FP64 values are fastest with one slice per wave in
-// 512-thread blocks (249-250 us vs 278 us grid-striding); FP32 values, whose
-// slices are half the bytes, are fastest in 256-thread blocks capped at 8192
-// and striding (137 us vs 150-165 us one slice per wave: block dispatch and
-// the end-of-block barrier dominate short blocks; interleaving the loads of
-// two strided slices per trip was slower still, 82 VGPRs cut occupancy to 6
-// waves per SIMD).
-template<class V>
-struct sdia_shape {
-	static constexpr bool f64 = std::is_same_v<V, double>;
-	static constexpr int reduce_threads = f64 ? 512 : 256;
-	static constexpr unsigned max_blocks = f64 ? (1u << 17) : 8192u;
+// wave.  Reducing epilogues leave one partial per block.  Measured on CG
+// iterations (256^3 and BiCGSTAB on 128^3):
+//  * FP64 values: one slice per wave; 512-thread blocks on the 16.7M-row
+//    matrix (0.689 s CG vs 0.705 s with 256 threads), 256-thread blocks on
+//    the 2.1M-row one (28.3 ms BiCGSTAB vs 30.5 ms with 512).  The switch is
+//    at kSdiaWideRows.
+//  * FP32 values, whose slices are half the bytes: 256-thread blocks capped
+//    at 8192 and grid-striding (ahead of one slice per wave; caps of 4096
+//    and 512-thread blocks were slower, 16384 equal).
+// Measured alternative, not used: one partial per wave (no end-of-block
+// barrier, no cap) reduced in two fixed-order stages -- the extra stage
+// cost more than the barrier saved (FP64 within 1%, FP32 slower).
+constexpr int64_t kSdiaWideRows = (int64_t)1 << 23;
+
+struct sdia_reduce_shape {
+	int threads;
+	unsigned max_blocks;
 };
+
+inline sdia_reduce_shape sdia_reduce(fsx_dtype vt, int64_t nrows) {
+	if (vt == FSX_F64) return {nrows >= kSdiaWideRows ? 512 : 256, 1u << 17};
+	return {256, 8192u};
+}
 
 inline unsigned sdia_blocks(int64_t nrows, bool reduces, int reduce_threads,
                             unsigned max_blocks) {
@@ -352,15 +366,23 @@ void spmv_launch_gather(hipStream_t st, const fsx_csr * A, const G & x,
 	const V * vv = (const V *)A->vals->p;
 	if (A->nrows == 0) return;
 	if (A->plan_kind == 3) {
-		using SH = sdia_shape<V>;
-		constexpr int BT = Epi::reduces ? SH::reduce_threads : kSpmvThreads;
-		const unsigned g = sdia_blocks(A->nrows, Epi::reduces, SH::reduce_threads,
-		                               SH::max_blocks);
-		hipLaunchKernelGGL((spmv_sdia_kernel<Epi, ACC, V, G, BT>), dim3(g), dim3(BT), 0,
-		                   st, A->nrows, (const int64_t *)A->idx->dg_ptr->p,
-		                   (const int32_t *)A->idx->dg_off->p,
-		                   (const uint64_t *)A->idx->dg_mask->p,
-		                   (const V *)A->dg_vals->p, x, epi);
+		auto go = [&](auto bt_c, unsigned g) {
+			constexpr int BT = decltype(bt_c)::value;
+			hipLaunchKernelGGL((spmv_sdia_kernel<Epi, ACC, V, G, BT>), dim3(g), dim3(BT), 0,
+			                   st, A->nrows, (const int64_t *)A->idx->dg_ptr->p,
+			                   (const int32_t *)A->idx->dg_off->p,
+			                   (const uint64_t *)A->idx->dg_mask->p,
+			                   (const V *)A->dg_vals->p, x, epi);
+		};
+		if constexpr (!Epi::reduces)
+			go(std::integral_constant<int, kSpmvThreads>{},
+			   sdia_blocks(A->nrows, false, 0, 0));
+		else {
+			const sdia_reduce_shape sh = sdia_reduce(A->vt, A->nrows);
+			const unsigned g = sdia_blocks(A->nrows, true, sh.threads, sh.max_blocks);
+			if (sh.threads == 512) go(std::integral_constant<int, 512>{}, g);
+			else go(std::integral_constant<int, 256>{}, g);
+		}
 	}
 	else if (A->plan_kind == 2) {
 		const unsigned g = sliced_blocks(A->nrows, Epi::reduces);
@@ -402,19 +424,30 @@ void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
 	spmv_launch_gather<Epi, ACC, V, I>(st, A, gather_x<X>{x}, epi);
 }
 
+// Number of partials (each NV doubles) a reducing product on A leaves.
 inline unsigned spmv_blocks(const fsx_csr * A) {
 	if (A->plan_kind == 3) {
-		if (A->vt == FSX_F64)
-			return sdia_blocks(A->nrows, true, sdia_shape<double>::reduce_threads,
-			                   sdia_shape<double>::max_blocks);
-		return sdia_blocks(A->nrows, true, sdia_shape<float>::reduce_threads,
-		                   sdia_shape<float>::max_blocks);
+		const sdia_reduce_shape sh = sdia_reduce(A->vt, A->nrows);
+		return sdia_blocks(A->nrows, true, sh.threads, sh.max_blocks);
 	}
 	if (A->plan_kind == 2) return sliced_blocks(A->nrows);
 	if (A->plan_kind == 0)
 		return (unsigned)((A->nrows + kStreamRows - 1) / kStreamRows);
 	const int rpb = kSpmvThreads / A->plan_lanes;
 	return (unsigned)((A->nrows + rpb - 1) / rpb);
+}
+
+// Doubles of partials space a reducing product on A needs.
+inline size_t spmv_partials_size(const fsx_csr * A, int NV) {
+	return (size_t)spmv_blocks(A) * NV;
+}
+
+// Sum the partials a reducing product on A left in `part` and run f on the
+// totals (thread 0 of a one-block kernel).
+template<int NV, class F>
+void spmv_finalize(hipStream_t s, const fsx_csr * A, double * part, const F & f) {
+	hipLaunchKernelGGL((finalize_kernel<NV, F>), dim3(1), dim3(kFinThreads), 0, s,
+	                   (const double *)part, spmv_blocks(A), f);
 }
 
 // Dispatch on index and value type.  `body` is called with (V*, I*) tags.

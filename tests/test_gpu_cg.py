@@ -210,3 +210,33 @@ def test_layouts_give_identical_trajectories(ctx):
     for it, x in out[1:]:
         assert abs(it - out[0][0]) <= 1
         assert rel(x, out[0][1]) <= 1e-12
+
+
+def test_graphs_after_matrix_rebuild(ctx):
+    """Captured solver graphs hold the device arrays of the matrix they were
+    captured for.  Matrices of one shape are built, solved and freed in turn
+    on one context (their buffers may come back at the old addresses), and
+    every solve must match its own residual -- no graph replays against the
+    arrays of a freed matrix."""
+    import gc
+    n = 24
+    N = n ** 3
+    for rep, sigma in enumerate((1.0, 2.0, 1.5)):
+        rp, ci, v, Ao = diffusion(n, sigma, seed=4000 + rep)
+        b = P.uniform(N, 4100 + rep)
+        A = ctx.csr(rp, ci, v)
+        A32 = A.convert(fsx.F32)
+        S = sp.csr_matrix((v, ci, rp))
+        x = ctx.zeros(N)
+        info = fsx.ir_cg_solve(A, A32, ctx.vec(b), x, 1e-10,
+                               fsx.Settings(maxiter=2000, rtol=1e-4))
+        assert info.status == 1
+        assert np.linalg.norm(b - S @ x.get()) / np.linalg.norm(b) <= 1.01e-10
+        x2 = ctx.zeros(N)
+        i2 = fsx.cg_solve(A, ctx.vec(b), x2, fsx.Settings(maxiter=3000, rtol=1e-9,
+                                                          use_zero_guess=True),
+                          dinv=A.diag_inv())
+        assert i2.reason == "converged_rtol"
+        assert np.linalg.norm(b - S @ x2.get()) / np.linalg.norm(b) <= 1e-9
+        del A, A32, x, x2
+        gc.collect()
