@@ -529,7 +529,8 @@ def bench_adjoint(a, fsx, P):
                          "max_rel_err": worst},
             "loss": loss,
         },
-        "roofline": {"bound": "hbm", "kernel": "fused CG iteration (sliced SpMV + BLAS-1)",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"fused CG iteration (SpMV plan {prob.A.plan()[0]} + BLAS-1)",
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "note": "every CG iteration of the forward, recompute and adjoint "
