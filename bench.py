@@ -115,7 +115,8 @@ def main():
     ap.add_argument("--max-outer", type=int, default=10)
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the SpMV and copy kernels only (profiling runs)")
-    ap.add_argument("--workload", choices=["diffusion", "batched", "adjoint", "convdiff"],
+    ap.add_argument("--workload", choices=["diffusion", "batched", "adjoint", "convdiff",
+                                           "smoother"],
                     default="diffusion")
     ap.add_argument("--heat-steps", type=int, default=50)
     ap.add_argument("--ckpt", type=int, default=10)
@@ -142,6 +143,8 @@ def main():
         return bench_adjoint(a, fsx, P)
     if a.workload == "convdiff":
         return bench_convdiff(a, fsx, P)
+    if a.workload == "smoother":
+        return bench_smoother(a, fsx, P)
 
     seeds = P.seeds()
     n = a.n
@@ -646,6 +649,84 @@ def bench_convdiff(a, fsx, P):
         "config": {"workload": f"convdiff3d_{n}^3_peclet10_rtol1e-10", "rows": N,
                    "nnz": nnz, "solvers": results},
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_smoother(a, fsx, P):
+    """Symmetric hybrid Gauss-Seidel sweeps (op::hybrid_gs, one colour) on the
+    256^3 diffusion matrix; the first sweep is checked bit for bit against
+    the oracle's sequential sweep on the host."""
+    n = a.n
+    seeds = P.seeds()
+    k = P.lognormal_field(n, a.sigma, 8.0, seeds["diffusion_k"])
+    rp, ci, v = P.diffusion3d(n, k)
+    del k
+    N, nnz = n ** 3, int(ci.size)
+    b = P.uniform(N, seeds["diffusion_rhs"])
+    ctx = fsx.Context(0)
+    A = ctx.csr(rp, ci, v)
+    t0 = time.perf_counter()
+    G = fsx.HybridGS(A, ncolours=1, sweep=fsx.HybridGS.SYMMETRIC, nrelax=1)
+    t_an = time.perf_counter() - t0
+    lf, lb = G.levels()
+    log(f"hybrid_gs {n}^3: level analysis {t_an:.2f} s, {lf} forward / {lb} backward levels")
+    bd, x = ctx.vec(b), ctx.zeros(N)
+    G.apply(bd, x)  # first sweep from x = 0 (captures the graph)
+    ctx.sync()
+    first = x.get()
+    ts = []
+    for i in range(a.warmup + a.steps):
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(10):
+            G.apply(bd, x)
+        ctx.sync()
+        dt = (time.perf_counter() - t) / 10
+        log(f"symmetric sweep {'warmup' if i < a.warmup else 'step'} {i}: {dt*1e3:.3f} ms")
+        if i >= a.warmup:
+            ts.append(dt)
+    t_sweep = sum(ts) / len(ts)
+    # algorithmic bytes of one direction: the SpMV's (values, column ids,
+    # row pointers, x read once, x written once) plus b read once
+    per_dir = spmv_bytes(N, nnz, 8) + 8 * N
+    gbs = 2 * per_dir / t_sweep / 1e9
+    out = {
+        "metric": f"hybrid Gauss-Seidel symmetric sweep, {n}^3 diffusion",
+        "value": round(t_sweep * 1e3, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_sweep * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal GRF coefficient, uniform rhs)",
+        "config": {"workload": f"hybrid_gs_symmetric_{n}^3_lognormal_sigma{a.sigma:g}_1colour",
+                   "rows": N, "nnz": nnz, "levels_forward": lf, "levels_backward": lb,
+                   "analysis_seconds": round(t_an, 3),
+                   "us_per_level": round(t_sweep / (lf + lb) * 1e6, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": 2 * per_dir,
+                     "note": "a sweep is a chain of (levels) dependent launches, one per "
+                             "dependency level; latency, not bandwidth, bounds it"},
+    }
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        Ao = O.Csr(rp, ci, v)
+        t = time.perf_counter()
+        ref = Ao.hybrid_gs(b, np.zeros(N), sweep=2, nrelax=1)
+        t_cpu = time.perf_counter() - t
+        same = bool(np.array_equal(ref, first))
+        log(f"oracle symmetric sweep: {t_cpu:.2f} s on 1 thread; GPU bit-identical: {same}")
+        out["config"]["bit_identical_to_oracle"] = same
+        out["cpu_baseline"] = {"value": round(t_cpu * 1e3, 1), "unit": "ms", "cores": 1,
+                               "kind": "port",
+                               "sample": f"one symmetric sweep of the same {n}^3 problem "
+                                         "from x = 0 (the sequential sweep of the serial "
+                                         "backend)"}
     print(json.dumps(out), flush=True)
 
 

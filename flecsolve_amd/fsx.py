@@ -124,6 +124,10 @@ _SIGS = {
     "fsx_spmv_dot": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_double)]),
     "fsx_jacobi_apply": (C.c_int, [_P, _P, _P, _P]),
     "fsx_jacobi_relax": (C.c_int, [_P, _P, C.c_float, C.c_int, _P, _P]),
+    "fsx_gs_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
+    "fsx_gs_relax": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
+    "fsx_gs_levels": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fsx_gs_destroy": (C.c_int, [_P]),
     "fsx_cg_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
                                C.POINTER(Info)]),
     "fsx_bicgstab_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
@@ -575,6 +579,45 @@ def krylov(kind: str, A: CSR, b: Vector, x: Vector, settings: Settings,
                 C.byref(info), hp, hist_cap)
     check(rc, f"fsx_krylov_{kind}")
     return info, hist[:min(hist_cap, max(info.iters, 0) or hist_cap)]
+
+
+class HybridGS:
+    """Hybrid Gauss-Seidel smoother (fsx_gs_*; op::hybrid_gs of
+    flecsolve/solvers/mg/gs.hh).  sweep: 0 forward, 1 backward, 2 symmetric."""
+
+    FORWARD, BACKWARD, SYMMETRIC = 0, 1, 2
+
+    def __init__(self, A: CSR, ncolours: int = 1, sweep: int = 2, nrelax: int = 1):
+        self.ctx = A.ctx
+        self.A = A  # the smoother reads A's arrays: keep A alive
+        self.sweep, self.nrelax = sweep, nrelax
+        self.h = C.c_void_p()
+        check(lib().fsx_gs_create(A.ctx.h, A.h, int(ncolours), C.byref(self.h)),
+              "fsx_gs_create")
+
+    def levels(self):
+        f, b = C.c_int64(), C.c_int64()
+        check(lib().fsx_gs_levels(self.h, C.byref(f), C.byref(b)), "fsx_gs_levels")
+        return f.value, b.value
+
+    def apply(self, b: Vector, x: Vector, sweep: int | None = None,
+              nrelax: int | None = None) -> None:
+        """Relax x in place (hybrid_gs::apply(b, x))."""
+        check(lib().fsx_gs_relax(self.ctx.h, self.h,
+                                 self.sweep if sweep is None else int(sweep),
+                                 self.nrelax if nrelax is None else int(nrelax), b.h, x.h),
+              "fsx_gs_relax")
+
+    def destroy(self):
+        if self.h:
+            check(lib().fsx_gs_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -204,6 +204,23 @@ int fsx_jacobi_apply(fsx_ctx * ctx, const fsx_vec * dinv, const fsx_vec * r,
 int fsx_jacobi_relax(fsx_ctx * ctx, const fsx_csr * A, float omega,
                      int nrelax, const fsx_vec * b, fsx_vec * x);
 
+/* ---- hybrid Gauss-Seidel smoother ----------------------------------------
+ * op::hybrid_gs (solvers/mg/gs.hh:40-130): Gauss-Seidel inside each of
+ * `ncolours` row blocks (FleCSI equal_map partition), the other blocks'
+ * values taken from before the sweep; ncolours = 1 is the sequential
+ * sweep of the serial backend.  Bit-identical to the sequential
+ * reference.  fsx_gs_create analyses the structure of the square FP64
+ * matrix A (which must outlive the smoother; its values may change in
+ * place).  sweep: 0 forward, 1 backward, 2 symmetric (mg::relax_sweep);
+ * x is relaxed in place `nrelax` times (hybrid_gs::apply, gs.hh:56-76). */
+typedef struct fsx_gs fsx_gs;
+int fsx_gs_create(fsx_ctx * ctx, const fsx_csr * A, int ncolours, fsx_gs ** out);
+int fsx_gs_relax(fsx_ctx * ctx, fsx_gs * G, int sweep, int nrelax, const fsx_vec * b,
+                 fsx_vec * x);
+/* Launch levels per forward / backward sweep (the dependency depth). */
+int fsx_gs_levels(const fsx_gs * G, int64_t * forward, int64_t * backward);
+int fsx_gs_destroy(fsx_gs * G);
+
 /* ---- solve drivers -------------------------------------------------------
  * fsx_cg_solve: op::cg::apply (solvers/cg.hh:43-139) with identity (dinv ==
  * NULL) or Jacobi preconditioning, run as fused kernels with device-resident

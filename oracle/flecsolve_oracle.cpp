@@ -25,6 +25,13 @@
 //     forced with a residual orthogonal to r~.
 //   * jacobi_relax: fixed point equals the exact solution; one sweep checked by
 //     hand on a 3x3 system.
+//   * hybrid_gs: the matrix form of a Gauss-Seidel sweep (x_new solves
+//     (D + L_own) x_new = b - (U_own + offd) x_old) on random matrices; one
+//     forward sweep solves a lower-triangular system exactly (backward:
+//     upper); a block-diagonal matrix aligned with the colours gives the
+//     one-colour result bit for bit.  The equal_map partition is FleCSI's
+//     published formula (FleCSI is not vendored): parity unpinned for P > 1
+//     beyond these checks.
 //   * fcg: equals cg's solution on SPD systems (parity unpinned for iteration
 //     trajectories beyond that; fcg is not named by the hot path).
 //
@@ -901,6 +908,76 @@ void or_jacobi_relax(const or_csr * A, float omega, int nrelax,
 	vec tmp(A->nrows);
 	for (int s = 0; s < nrelax; ++s)
 		jacobi_relax(*A, (double)omega, b, x, tmp.data());
+}
+
+} // extern "C"
+
+// ---- op::hybrid_gs::relax over P colours (mg/gs.hh:56-125) -----------------
+// Colour c (FleCSI equal_map(n, P): rows [c q + min(c, m), (c+1) q +
+// min(c+1, m))) walks its rows forward or backward; the diag block's entries
+// (own-colour columns, CSR order; the diagonal is the entry whose column is
+// the row) read the colour's x in place, then the offd block's entries
+// (other colours' columns, CSR order, init_mats keeps the row's order) read
+// the ghost copy taken before the task.  The colours run one after another
+// here, each reading the others only through that copy, which is what the
+// reference's concurrent ranks see.  sweep: 0 forward, 1 backward, 2
+// symmetric (forward then backward), each repeated nrelax times
+// (gs.hh:56-76).
+template<class I>
+static void hybrid_gs_dir(const or_csr & A, int P, bool fwd, const double * b, double * x,
+                          std::vector<double> & ghost) {
+	const int64_t n = A.nrows;
+	const I * rp = static_cast<const I *>(A.rowptr);
+	const I * ci = static_cast<const I *>(A.colind);
+	const double * va = A.vals;
+	const int64_t q = n / P, m = n % P;
+	auto start = [&](int64_t c) { return c * q + std::min<int64_t>(c, m); };
+	auto colour = [&](int64_t j) {
+		const int64_t big = m * (q + 1);
+		return j < big ? j / (q + 1) : m + (j - big) / q;
+	};
+	std::copy(x, x + n, ghost.begin());
+	for (int64_t c = 0; c < P; ++c) {
+		const int64_t lo = start(c), hi = start(c + 1);
+		auto update = [&](int64_t r) {
+			double diag = 0.;
+			double rsum = 0.;
+			for (I off = rp[r]; off < rp[r + 1]; ++off) {
+				const int64_t cid = ci[off];
+				if (colour(cid) != c) continue;
+				if (cid == r) diag = va[off];
+				else rsum += va[off] * x[cid];
+			}
+			for (I off = rp[r]; off < rp[r + 1]; ++off) {
+				const int64_t cid = ci[off];
+				if (colour(cid) != c) rsum += va[off] * ghost[cid];
+			}
+			const double dinv = 1. / diag;
+			x[r] = dinv * (b[r] - rsum);
+		};
+		if (fwd)
+			for (int64_t r = lo; r < hi; ++r) update(r);
+		else
+			for (int64_t r = hi - 1; r >= lo; --r) update(r);
+	}
+}
+
+extern "C" {
+
+int or_hybrid_gs(const or_csr * A, int ncolours, int sweep, int nrelax, const double * b,
+                 double * x) {
+	if (ncolours < 1 || (A->nrows > 0 && ncolours > A->nrows) || sweep < 0 || sweep > 2)
+		return -1;
+	std::vector<double> ghost(A->nrows);
+	auto dir = [&](bool fwd) {
+		if (A->itype == 0) hybrid_gs_dir<int32_t>(*A, ncolours, fwd, b, x, ghost);
+		else hybrid_gs_dir<int64_t>(*A, ncolours, fwd, b, x, ghost);
+	};
+	for (int it = 0; it < nrelax; ++it) {
+		if (sweep != 1) dir(true);
+		if (sweep != 0) dir(false);
+	}
+	return 0;
 }
 
 // ---- P-colour CPU baseline ---------------------------------------------------

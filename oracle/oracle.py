@@ -95,6 +95,8 @@ def lib():
                                  C.POINTER(C.c_int), P, P, P, i64]
         L.or_mm_read.restype = i64
         L.or_jacobi_relax.argtypes = [C.POINTER(_Csr), C.c_float, C.c_int, P, P]
+        L.or_hybrid_gs.argtypes = [C.POINTER(_Csr), C.c_int, C.c_int, C.c_int, P, P]
+        L.or_hybrid_gs.restype = C.c_int
         solver_args = [C.POINTER(_Csr), C.POINTER(_Prec), P, P, C.POINTER(Settings),
                        DIAG_FN, P, C.POINTER(Info)]
         L.or_cg.argtypes = solver_args + [_D, C.c_int]
@@ -155,6 +157,16 @@ class Csr:
         x = np.array(x, np.float64)
         b = np.ascontiguousarray(b, np.float64)
         lib().or_jacobi_relax(C.byref(self.s), float(omega), nrelax, _p(b), _p(x))
+        return x
+
+    def hybrid_gs(self, b, x, sweep=2, nrelax=1, ncolours=1):
+        """op::hybrid_gs relaxation (sweep 0 forward, 1 backward, 2 symmetric)
+        over `ncolours` row blocks; returns the relaxed x."""
+        x = np.array(x, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        if lib().or_hybrid_gs(C.byref(self.s), int(ncolours), int(sweep), int(nrelax),
+                              _p(b), _p(x)) != 0:
+            raise ValueError("hybrid_gs: bad colours or sweep")
         return x
 
 

@@ -16,9 +16,21 @@
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
 #include "flecsolve/solvers/gmres.hh"
+#include "flecsolve/solvers/mg/gs.hh"
 #include "fsx_problems.h"
 
 using namespace flecsolve;
+
+// z = one hybrid Gauss-Seidel application to r from z = 0
+struct sgs_precond : op::base<> {
+	explicit sgs_precond(op::core<op::hybrid_gs> g) : G(std::move(g)) {}
+	template<class D, class R>
+	void apply(const D & r, R & z) const {
+		z.set_scalar(0.);
+		G.apply(r, z);
+	}
+	op::core<op::hybrid_gs> G;
+};
 
 template<class Op, class Vec>
 struct diagnostic {
@@ -115,6 +127,28 @@ int main(int argc, char ** argv) {
 		const double brel = r.l2norm().get() / rhs.l2norm().get();
 		std::printf("bicgstab: %d iterations, true rel res %.3e\n", bsi.iters, brel);
 		fails += !bsi.success() || brel > 1e-9;
+
+		// CG preconditioned by one symmetric hybrid Gauss-Seidel sweep from a
+		// zero guess (a fixed SPD operator; the reference's multigrid solver
+		// zeroes x the same way before cycling, solvers/mg/ua.hh:140-141),
+		// built with mg::hybrid_gs::solver as in its levels (ua.hh:191-196)
+		mg::hybrid_gs::solver gsf{{1, mg::relax_sweep::symmetric}};
+		op::core<sgs_precond> SG(gsf(A));
+		cg::settings fs;
+		fs.maxiter = 500;
+		fs.rtol = 1e-10;
+		fs.use_zero_guess = true;
+		auto fc = cg::solver(fs, vec::make_device_work<cg::nwork>(rhs))(op::ref(A),
+		                                                               op::ref(SG));
+		solve_info fi = fc(rhs, y);
+		A.residual(rhs, y, r);
+		const double frel = r.l2norm().get() / rhs.l2norm().get();
+		auto pc = cg::solver(fs, vec::make_device_work<cg::nwork>(rhs))(op::ref(A));
+		vec::device<double> y2(ctx, n * n);
+		solve_info ci2 = pc(rhs, y2);
+		std::printf("cg+symmetric hybrid_gs: %d iterations (no preconditioner: %d), "
+		            "true rel res %.3e\n", fi.iters, ci2.iters, frel);
+		fails += !fi.success() || frel > 1e-9 || fi.iters >= ci2.iters;
 	}
 	if (argc > 1) {
 		// the reference's test input path: read, tocsr, device operator, CG

@@ -1,0 +1,111 @@
+"""Device hybrid Gauss-Seidel (fsx_gs_*, op::hybrid_gs of
+flecsolve/solvers/mg/gs.hh) against the oracle (gpu).
+
+Bar: bit-identical to the oracle's sequential sweep (the level-scheduled
+kernels do each row's arithmetic in the reference's order) for forward,
+backward and symmetric sweeps, one and several colours, int32 and int64
+indices, rows whose columns are not sorted; the launch-level count equals
+the dependency depth the mathematics fixes (2n - 1 for the 5-point stencil
+on n^2, 3n - 2 for the 7-point stencil on n^3, 1 for a diagonal matrix).
+"""
+import numpy as np
+import pytest
+
+from flecsolve_amd import fsx
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+from test_oracle_smoothers import random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(ctx, rp, ci, v, sweep, nrelax, ncolours, seed=1):
+    n = rp.size - 1
+    A = ctx.csr(rp, ci, v)
+    b = P.uniform(n, seed)
+    x0 = P.uniform(n, seed + 1)
+    G = fsx.HybridGS(A, ncolours=ncolours)
+    x = ctx.vec(x0)
+    G.apply(ctx.vec(b), x, sweep=sweep, nrelax=nrelax)
+    ref = O.Csr(rp, ci, v).hybrid_gs(b, x0, sweep=sweep, nrelax=nrelax, ncolours=ncolours)
+    return x.get(), ref, G
+
+
+@pytest.mark.parametrize("sweep", [0, 1, 2])
+@pytest.mark.parametrize("nrelax", [1, 2])
+def test_poisson2d_bit_exact(ctx, sweep, nrelax):
+    rp, ci, v = P.poisson2d(64)
+    got, ref, G = run_both(ctx, rp, ci, v, sweep, nrelax, 1)
+    np.testing.assert_array_equal(got, ref)
+    assert G.levels() == (2 * 64 - 1, 2 * 64 - 1)
+
+
+@pytest.mark.parametrize("ncolours", [1, 5])
+def test_diffusion3d_bit_exact(ctx, ncolours):
+    n = 16
+    rp, ci, v = P.diffusion3d(n, P.lognormal_field(n, 1.0, 4.0, 7))
+    got, ref, G = run_both(ctx, rp, ci, v, 2, 2, ncolours, seed=3)
+    np.testing.assert_array_equal(got, ref)
+    if ncolours == 1:
+        assert G.levels() == (3 * n - 2, 3 * n - 2)
+
+
+@pytest.mark.parametrize("itype", [np.int32, np.int64])
+@pytest.mark.parametrize("ncolours", [1, 3])
+def test_random_unsorted_rows_bit_exact(ctx, itype, ncolours):
+    rp, ci, v, _ = random_csr(500, 0.02, 17)
+    for sweep in (0, 1, 2):
+        got, ref, _ = run_both(ctx, rp.astype(itype), ci.astype(itype), v, sweep, 1,
+                               ncolours, seed=5 + sweep)
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_diagonal_is_one_level(ctx):
+    n = 1000
+    d = 2.0 + P.uniform(n, 4)
+    rp, ci = np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32)
+    got, ref, G = run_both(ctx, rp, ci, d, 2, 1, 1)
+    np.testing.assert_array_equal(got, ref)
+    assert G.levels() == (1, 1)
+
+
+def test_graph_replay_and_value_update(ctx):
+    """Replays of the captured sweep, on two different x vectors and after an
+    in-place change of the matrix values, each equal the oracle."""
+    n = 20
+    rp, ci, v = P.diffusion3d(n, P.lognormal_field(n, 1.0, 4.0, 9))
+    A = ctx.csr(rp, ci, v)
+    G = fsx.HybridGS(A)
+    b = P.uniform(n ** 3, 10)
+    bd = ctx.vec(b)
+    x0 = P.uniform(n ** 3, 11)
+    Ao = O.Csr(rp, ci, v)
+    for rep in range(3):
+        x1, x2 = ctx.vec(x0), ctx.vec(x0 * 2.0)
+        G.apply(bd, x1, sweep=2, nrelax=1)
+        G.apply(bd, x2, sweep=2, nrelax=1)
+        np.testing.assert_array_equal(x1.get(), Ao.hybrid_gs(b, x0, 2, 1))
+        np.testing.assert_array_equal(x2.get(), Ao.hybrid_gs(b, x0 * 2.0, 2, 1))
+    v2 = v * 1.5
+    A.set_values(v2)
+    x = ctx.vec(x0)
+    G.apply(bd, x, sweep=2, nrelax=1)
+    np.testing.assert_array_equal(x.get(), O.Csr(rp, ci, v2).hybrid_gs(b, x0, 2, 1))
+
+
+def test_errors(ctx):
+    rp, ci, v = P.poisson2d(8)
+    A = ctx.csr(rp, ci, v)
+    with pytest.raises(fsx.FsxError):
+        fsx.HybridGS(A, ncolours=0)
+    with pytest.raises(fsx.FsxError):
+        fsx.HybridGS(A, ncolours=65)
+    G = fsx.HybridGS(A)
+    with pytest.raises(fsx.FsxError):
+        G.apply(ctx.vec(np.ones(63)), ctx.zeros(64))
+    with pytest.raises(fsx.FsxError):
+        G.apply(ctx.vec(np.ones(64)), ctx.zeros(64), sweep=3)
+    rect = ctx.csr(np.array([0, 1, 2], np.int32), np.array([0, 3], np.int32),
+                   np.array([1.0, 1.0]), ncols=4)
+    with pytest.raises(fsx.FsxError):
+        fsx.HybridGS(rect)
