@@ -14,6 +14,7 @@
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
 #include "flecsolve/solvers/gmres.hh"
+#include "flecsolve/solvers/mg/ua.hh"
 #include "flecsolve/vectors/device.hh"
 
 namespace fsx {
@@ -187,6 +188,31 @@ int fsx_krylov_bicgstab(fsx_ctx * ctx, const fsx_csr * A, int prec,
 			auto op = slv(Ah, Ph, std::ref(h));
 			to_info(op(bv, xv), info);
 		});
+	});
+}
+
+// CG / FCG preconditioned by an AMG solver (mg::ua::bound_solver as P).
+int fsx_krylov_cg_amg(fsx_ctx * ctx, const fsx_csr * A, fsx_amg * M, int flexible,
+                      const fsx_vec * b, fsx_vec * x, const fsx_settings * s, fsx_info * info,
+                      double * res_hist, int hist_cap) {
+	return guarded([&] {
+		check_args(ctx, A, b, x, s, info);
+		if (!M) throw std::invalid_argument("null AMG solver");
+		dvec bv = dvec::borrow(ctx, const_cast<fsx_vec *>(b));
+		dvec xv = dvec::borrow(ctx, x);
+		history h{res_hist, hist_cap};
+		op::core<mat::device_csr> Aop(ctx, const_cast<fsx_csr *>(A));
+		op::core<mg::ua::bound_solver> Pm(static_cast<const mat::device_csr &>(Aop), M);
+		if (flexible) {
+			fcg::solver slv(to_settings<cg::settings>(s), vec::make_device_work<fcg::nwork>(bv));
+			auto op = slv(op::ref(Aop), op::ref(Pm), std::ref(h));
+			to_info(op(bv, xv), info);
+		}
+		else {
+			cg::solver slv(to_settings<cg::settings>(s), vec::make_device_work<cg::nwork>(bv));
+			auto op = slv(op::ref(Aop), op::ref(Pm), std::ref(h));
+			to_info(op(bv, xv), info);
+		}
 	});
 }
 

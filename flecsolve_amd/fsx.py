@@ -128,6 +128,13 @@ _SIGS = {
     "fsx_gs_relax": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P]),
     "fsx_gs_levels": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fsx_gs_destroy": (C.c_int, [_P]),
+    "fsx_amg_create": (C.c_int, [_P, _P, _P, C.POINTER(_P)]),
+    "fsx_amg_levels": (C.c_int, [_P, C.POINTER(C.c_int), _P, _P, C.c_int]),
+    "fsx_amg_aggregates": (C.c_int, [_P, C.c_int, _P]),
+    "fsx_amg_level_matrix": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "fsx_amg_apply": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "fsx_amg_destroy": (C.c_int, [_P]),
+    "fsx_krylov_cg_amg": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P, _P, _P, C.c_int]),
     "fsx_cg_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
                                C.POINTER(Info)]),
     "fsx_bicgstab_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
@@ -405,9 +412,9 @@ class CSR:
         return d
 
     def destroy(self):
-        if self.h:
+        if self.h and getattr(self, "_owner", None) is None:
             check(lib().fsx_csr_destroy(self.h))
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -611,6 +618,82 @@ class HybridGS:
     def destroy(self):
         if self.h:
             check(lib().fsx_gs_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class AmgSettings(C.Structure):
+    """fsx_amg_settings (mg::ua::solver_settings); defaults are the reference's
+    option defaults (ua.cc), max_levels (required there) 20."""
+    _fields_ = [("max_levels", C.c_int), ("min_coarse", C.c_int64), ("maxiter", C.c_int),
+                ("atol", C.c_float), ("beta", C.c_float), ("pairwise_passes", C.c_int),
+                ("nrelax", C.c_int), ("cycle", C.c_int), ("kappa", C.c_int),
+                ("ktol", C.c_float)]
+    V, RELAXED_W, RELAXED_KAPPA, KAPPA_K = 0, 1, 2, 3
+
+    def __init__(self, max_levels=20, min_coarse=5, maxiter=10, atol=1e-6, beta=0.25,
+                 pairwise_passes=2, nrelax=2, cycle=0, kappa=1, ktol=0.25):
+        super().__init__(max_levels, min_coarse, maxiter, atol, beta, pairwise_passes,
+                         nrelax, cycle, kappa, ktol)
+
+
+class AMG:
+    """Unsmoothed-aggregation AMG (fsx_amg_*; mg::ua::bound_solver)."""
+
+    def __init__(self, A: CSR, settings: AmgSettings | None = None):
+        self.ctx = A.ctx
+        self.A = A  # the solver reads A's arrays: keep A alive
+        self.settings = settings or AmgSettings()
+        self.h = C.c_void_p()
+        check(lib().fsx_amg_create(A.ctx.h, A.h, C.byref(self.settings), C.byref(self.h)),
+              "fsx_amg_create")
+
+    def levels(self):
+        """[(rows, stored entries)] per level."""
+        nl = C.c_int()
+        check(lib().fsx_amg_levels(self.h, C.byref(nl), None, None, 0))
+        rows = np.zeros(nl.value, np.int64)
+        nnz = np.zeros(nl.value, np.int64)
+        check(lib().fsx_amg_levels(self.h, C.byref(nl), _ptr(rows), _ptr(nnz), nl.value))
+        return list(zip(rows.tolist(), nnz.tolist()))
+
+    def aggregates(self, level: int) -> np.ndarray:
+        out = np.empty(self.levels()[level][0], np.int64)
+        check(lib().fsx_amg_aggregates(self.h, level, _ptr(out)), "fsx_amg_aggregates")
+        return out
+
+    def level_matrix(self, level: int) -> CSR:
+        h = C.c_void_p()
+        check(lib().fsx_amg_level_matrix(self.h, level, C.byref(h)), "fsx_amg_level_matrix")
+        M = CSR(self.ctx, None, None, None, _handle=h)
+        M._owner = self  # owned by the solver
+        return M
+
+    def apply(self, b: Vector, x: Vector):
+        """x = the solver applied to b; returns (cycles, |b - A x|)."""
+        cy, res = C.c_int(), C.c_double()
+        check(lib().fsx_amg_apply(self.ctx.h, self.h, b.h, x.h, C.byref(cy), C.byref(res)),
+              "fsx_amg_apply")
+        return cy.value, res.value
+
+    def cg(self, b: Vector, x: Vector, settings: Settings, flexible=False, hist_cap=0):
+        """CG (or FCG) on the fine matrix with this solver as preconditioner."""
+        info = Info()
+        hist = np.zeros(max(hist_cap, 1))
+        check(lib().fsx_krylov_cg_amg(self.ctx.h, self.A.h, self.h, int(flexible), b.h, x.h,
+                                      C.byref(settings), C.byref(info),
+                                      _ptr(hist) if hist_cap else None, hist_cap),
+              "fsx_krylov_cg_amg")
+        return info
+
+    def destroy(self):
+        if self.h:
+            check(lib().fsx_amg_destroy(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -221,6 +221,52 @@ int fsx_gs_relax(fsx_ctx * ctx, fsx_gs * G, int sweep, int nrelax, const fsx_vec
 int fsx_gs_levels(const fsx_gs * G, int64_t * forward, int64_t * backward);
 int fsx_gs_destroy(fsx_gs * G);
 
+/* ---- unsmoothed-aggregation AMG (mg::ua::bound_solver, one colour) -------
+ * Setup (solvers/mg/ua.hh:110-139; coarsen.hh:26-655): double (or triple)
+ * pairwise aggregation with the reference's dominance check and priority
+ * order, Galerkin coarse operators, levels until one has fewer than
+ * min_coarse rows or max_levels exist, dense LU of the coarsest (at most
+ * 4096 rows).  Apply (ua.hh:141-172): x = 0, then up to maxiter cycles
+ * (cycle.hh: 0 v, 1 relaxed-w, 2 relaxed-kappa, 3 kappa-k) with nrelax
+ * symmetric hybrid Gauss-Seidel sweeps before and after each coarse
+ * correction, stopping once |b - A x| <= atol.  V, relaxed W and relaxed
+ * kappa cycles are bit-identical to the sequential algorithm.  A must
+ * outlive the solver.  Reference option defaults: min_coarse 5, maxiter
+ * 10, atol 1e-6, beta 0.25, pairwise_passes 2, nrelax 2, cycle v,
+ * kappa 1, ktol 0.25 (ua.cc). */
+typedef struct {
+	int max_levels;
+	int64_t min_coarse;
+	int maxiter;
+	float atol;
+	float beta;
+	int pairwise_passes;
+	int nrelax;
+	int cycle;
+	int kappa;
+	float ktol;
+} fsx_amg_settings;
+typedef struct fsx_amg fsx_amg;
+int fsx_amg_create(fsx_ctx * ctx, const fsx_csr * A, const fsx_amg_settings * s,
+                   fsx_amg ** out);
+/* Number of levels; rows and stored entries of the first `cap` levels. */
+int fsx_amg_levels(const fsx_amg * M, int * nlevels, int64_t * rows, int64_t * nnz, int cap);
+/* Aggregate of each row of `level` on level + 1 (-1: in no aggregate). */
+int fsx_amg_aggregates(const fsx_amg * M, int level, int64_t * aggt);
+/* The matrix of `level` (owned by the solver; level 0 is A). */
+int fsx_amg_level_matrix(const fsx_amg * M, int level, fsx_csr ** A);
+/* x = the solver applied to b; cycles run and the final |b - A x|. */
+int fsx_amg_apply(fsx_ctx * ctx, fsx_amg * M, const fsx_vec * b, fsx_vec * x, int * cycles,
+                  double * res);
+int fsx_amg_destroy(fsx_amg * M);
+/* CG (flexible = 0) or FCG (flexible = 1) preconditioned by M: the generic
+ * op::cg / op::fcg (solvers/cg.hh) with mg::ua::bound_solver as P, as a
+ * caller of the reference writes cg::solver(s, work)(op::ref(A),
+ * op::ref(amg)).  M's fine matrix must be A. */
+int fsx_krylov_cg_amg(fsx_ctx * ctx, const fsx_csr * A, fsx_amg * M, int flexible,
+                      const fsx_vec * b, fsx_vec * x, const fsx_settings * s, fsx_info * info,
+                      double * res_hist, int hist_cap);
+
 /* ---- solve drivers -------------------------------------------------------
  * fsx_cg_solve: op::cg::apply (solvers/cg.hh:43-139) with identity (dinv ==
  * NULL) or Jacobi preconditioning, run as fused kernels with device-resident

@@ -64,7 +64,9 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <map>
 #include <numeric>
+#include <set>
 #include <random>
 #include <sstream>
 #include <string>
@@ -1121,6 +1123,471 @@ int or_bicgstab(const or_csr * A, const or_prec * P, const double * b,
 	op_t a; a.A = A; a.kind = 1;
 	return bicgstab(a, make_prec(P), b, x, A->nrows, *s, diag_t{diag, user},
 	                info, res_hist, hist_cap);
+}
+
+} // extern "C"
+
+// ---- mg::ua::bound_solver at one colour (solvers/mg/ua.hh:94-200) -----------
+//
+// Setup: coarsen.hh:26-655 with the reference's own containers (std::set
+// strength sets, the prospect's std::map / std::multimap priority queue,
+// std::map accumulation of coarse entries), so the aggregation visits
+// candidates in exactly the reference's order; the quirks kept are the int
+// accumulator of the dominance check (std::reduce with an int init, then
+// `+=`, coarsen.hh:181-195) and the std::map operator[] that gives column
+// id 0 to rows outside every aggregate in create_aux (coarsen.hh:289-292).
+// Apply: ua.hh:141-172 with the cycles of cycle.hh:43-272, hybrid_gs
+// symmetric smoothing (hybrid_gs_dir above) and a dense LU coarse solve.
+// Deliberate deviations: Eigen is not vendored, so the coarse solve is a
+// textbook partial-pivoting LU (parity with Eigen's PartialPivLU to
+// rounding only) on a zero-initialised dense matrix (Eigen leaves it
+// uninitialised); the prolongation's correction vector starts at zero
+// (rows outside every aggregate are never written by inject).
+
+namespace {
+
+struct ua_csr {
+	int64_t n = 0;
+	std::vector<int64_t> rp{0}, ci;
+	vec v;
+};
+using ua_agg = std::vector<std::vector<std::size_t>>;
+using ua_soc = std::vector<std::set<std::size_t>>;
+
+struct ua_prospect { // coarsen.hh:37-131
+	std::size_t pop() {
+		auto selected = prio_node.cbegin();
+		const std::size_t node = selected->second;
+		node_prio.erase(node);
+		prio_node.erase(selected);
+		return node;
+	}
+	void remove(std::size_t e) {
+		auto eprio = node_prio.at(e);
+		node_prio.erase(e);
+		auto rng = prio_node.equal_range(eprio);
+		for (auto i = rng.first; i != rng.second; ++i)
+			if (i->second == e) {
+				prio_node.erase(i);
+				break;
+			}
+	}
+	void decrement_priority(std::size_t e) {
+		auto npnode = node_prio.find(e);
+		if (npnode != node_prio.end()) {
+			auto prio = npnode->second;
+			auto rng = prio_node.equal_range(prio);
+			for (auto i = rng.first; i != rng.second; ++i)
+				if (i->second == e) {
+					prio_node.erase(i);
+					break;
+				}
+			prio_node.insert({prio - 1, e});
+			--node_prio[e];
+		}
+	}
+	void insert(std::size_t node) { node_prio[node]; }
+	bool contains(std::size_t node) const { return node_prio.find(node) != node_prio.end(); }
+	bool empty() const { return node_prio.empty(); }
+	void prioritize(const ua_soc & soc) {
+		for (const auto & strength : soc)
+			for (auto node : strength) ++node_prio[node];
+		for (const auto & nd : node_prio) prio_node.insert({nd.second, nd.first});
+	}
+	std::multimap<std::size_t, std::size_t> prio_node;
+	std::map<std::size_t, std::size_t> node_prio;
+};
+
+ua_agg ua_pairwise(float beta, const ua_csr & A, bool checkdd) {
+	const int64_t n = A.n;
+	ua_prospect unmarked;
+	for (int64_t r = 0; r < n; ++r) { // get_unmarked
+		if (checkdd) {
+			int offd_sum = 0; // std::reduce over the empty offd block, init 0
+			double diag_val = 0.;
+			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) {
+				if (r != A.ci[off]) offd_sum += std::abs(A.v[off]);
+				else diag_val = A.v[off];
+			}
+			if (!(diag_val > 5 * offd_sum)) unmarked.insert((std::size_t)r);
+		}
+		else
+			unmarked.insert((std::size_t)r);
+	}
+	ua_soc soc((std::size_t)n); // soc_graph
+	for (int64_t r = 0; r < n; ++r) {
+		auto first = A.v.begin() + A.rp[r], last = A.v.begin() + A.rp[r + 1];
+		auto min_el = std::min_element(first, last);
+		const double max_couple =
+			min_el == A.v.end() ? std::numeric_limits<double>::max() : *min_el;
+		for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) {
+			const std::size_t c = (std::size_t)A.ci[off];
+			if ((std::size_t)r != c && (!checkdd || unmarked.contains(c)) &&
+			    c < (std::size_t)n && A.v[off] < (beta * max_couple))
+				soc[(std::size_t)r].insert(c);
+		}
+	}
+	unmarked.prioritize(soc);
+	ua_agg agg;
+	while (!unmarked.empty()) {
+		const std::size_t selected = unmarked.pop();
+		double best = std::numeric_limits<double>::max();
+		std::size_t pair = (std::size_t)-1; // find_pair
+		for (int64_t off = A.rp[selected]; off < A.rp[selected + 1]; ++off) {
+			const std::size_t c = (std::size_t)A.ci[off];
+			if (unmarked.contains(c) && A.v[off] < best) {
+				best = A.v[off];
+				pair = c;
+			}
+		}
+		auto update_priorities = [&](std::size_t k) {
+			for (auto l : soc[k]) unmarked.decrement_priority(l);
+		};
+		if (soc[selected].find(pair) != soc[selected].end()) {
+			unmarked.remove(pair);
+			update_priorities(pair);
+			agg.push_back({selected, pair});
+		}
+		else
+			agg.push_back({selected});
+		update_priorities(selected);
+	}
+	return agg;
+}
+
+ua_csr ua_create_aux(const ua_csr & A, const ua_agg & agg) {
+	std::map<std::size_t, std::size_t> aggt; // local_aggregate_transpose
+	for (std::size_t i = 0; i < agg.size(); ++i)
+		for (auto fi : agg[i]) aggt[fi] = i;
+	ua_csr C;
+	C.n = (int64_t)agg.size();
+	for (std::size_t rc = 0; rc < agg.size(); ++rc) {
+		std::map<std::size_t, double> agg_values;
+		for (auto r : agg[rc])
+			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off)
+				if (aggt[(std::size_t)A.ci[off]] != std::numeric_limits<std::size_t>::max())
+					agg_values[aggt[(std::size_t)A.ci[off]]] += A.v[off];
+		for (const auto & ce : agg_values) {
+			C.ci.push_back((int64_t)ce.first);
+			C.v.push_back(ce.second);
+		}
+		C.rp.push_back((int64_t)C.ci.size());
+	}
+	return C;
+}
+
+ua_agg ua_union(const ua_agg & a1, const ua_agg & a2) {
+	ua_agg a(a2.size());
+	for (std::size_t i = 0; i < a2.size(); ++i)
+		for (auto e2 : a2[i])
+			for (auto e1 : a1[e2]) a[i].push_back(e1);
+	return a;
+}
+
+ua_csr ua_galerkin(const ua_csr & A, const ua_agg & agg, const std::vector<int64_t> & aggt) {
+	ua_csr C;
+	C.n = (int64_t)agg.size();
+	for (std::size_t rc = 0; rc < agg.size(); ++rc) {
+		std::map<int64_t, double> agg_values;
+		for (auto r : agg[rc])
+			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off)
+				if (aggt[(std::size_t)A.ci[off]] != -1)
+					agg_values[aggt[(std::size_t)A.ci[off]]] += A.v[off];
+		for (const auto & ce : agg_values) {
+			C.ci.push_back(ce.first);
+			C.v.push_back(ce.second);
+		}
+		C.rp.push_back((int64_t)C.ci.size());
+	}
+	return C;
+}
+
+void ua_spmv(const ua_csr & A, const double * x, double * y) {
+	for (int64_t i = 0; i < A.n; ++i) {
+		y[i] = 0.;
+		for (int64_t off = A.rp[i]; off < A.rp[i + 1]; ++off) y[i] += A.v[off] * x[A.ci[off]];
+	}
+}
+
+void ua_residual(const ua_csr & A, const double * b, const double * x, double * r) {
+	ua_spmv(A, x, r); // apply(x, r); r.subtract(b, r)
+	for (int64_t i = 0; i < A.n; ++i) r[i] = b[i] - r[i];
+}
+
+double ua_dot(const vec & x, const vec & y) {
+	double s = 0;
+	for (std::size_t i = 0; i < x.size(); ++i) s += x[i] * y[i];
+	return s;
+}
+
+// symmetric hybrid Gauss-Seidel, one colour (gs.hh:56-125)
+void ua_gs(const ua_csr & A, const double * b, double * x, int nrelax) {
+	auto dir = [&](bool fwd) {
+		auto update = [&](int64_t r) {
+			double diag = 0.;
+			double rsum = 0.;
+			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) {
+				if (A.ci[off] == r) diag = A.v[off];
+				else rsum += A.v[off] * x[A.ci[off]];
+			}
+			const double dinv = 1. / diag;
+			x[r] = dinv * (b[r] - rsum);
+		};
+		if (fwd)
+			for (int64_t r = 0; r < A.n; ++r) update(r);
+		else
+			for (int64_t r = A.n - 1; r >= 0; --r) update(r);
+	};
+	for (int i = 0; i < nrelax; ++i) {
+		dir(true);
+		dir(false);
+	}
+}
+
+struct ua_lu { // partial pivoting, row of the largest |a_ik| (first on ties)
+	int64_t n = 0;
+	vec a;
+	std::vector<int64_t> piv;
+	void factor(const ua_csr & A) {
+		n = A.n;
+		a.assign((std::size_t)(n * n), 0.0);
+		for (int64_t r = 0; r < n; ++r)
+			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) a[r * n + A.ci[off]] = A.v[off];
+		piv.assign((std::size_t)n, 0);
+		for (int64_t k = 0; k < n; ++k) {
+			int64_t p = k;
+			double m = std::abs(a[k * n + k]);
+			for (int64_t i = k + 1; i < n; ++i)
+				if (std::abs(a[i * n + k]) > m) {
+					m = std::abs(a[i * n + k]);
+					p = i;
+				}
+			piv[k] = p;
+			if (p != k)
+				for (int64_t j = 0; j < n; ++j) std::swap(a[k * n + j], a[p * n + j]);
+			const double d = a[k * n + k];
+			if (d == 0.0) continue;
+			for (int64_t i = k + 1; i < n; ++i) {
+				const double l = a[i * n + k] / d;
+				a[i * n + k] = l;
+				for (int64_t j = k + 1; j < n; ++j) a[i * n + j] = a[i * n + j] - l * a[k * n + j];
+			}
+		}
+	}
+	void solve(const double * b, double * x) const {
+		vec y(b, b + n);
+		for (int64_t k = 0; k < n; ++k) std::swap(y[k], y[piv[k]]);
+		for (int64_t i = 0; i < n; ++i)
+			for (int64_t j = 0; j < i; ++j) y[i] = y[i] - a[i * n + j] * y[j];
+		for (int64_t i = n - 1; i >= 0; --i) {
+			for (int64_t j = i + 1; j < n; ++j) y[i] = y[i] - a[i * n + j] * y[j];
+			y[i] = y[i] / a[i * n + i];
+		}
+		std::copy(y.begin(), y.end(), x);
+	}
+};
+
+} // namespace
+
+extern "C" {
+
+typedef struct {
+	int max_levels;
+	int64_t min_coarse;
+	int maxiter;
+	float atol;
+	float beta;
+	int pairwise_passes;
+	int nrelax;
+	int cycle;
+	int kappa;
+	float ktol;
+} or_amg_settings;
+
+struct or_amg {
+	or_amg_settings s;
+	std::vector<ua_csr> A;                 // level matrices
+	std::vector<std::vector<int64_t>> aggt; // level l -> l + 1 (-1: none)
+	std::vector<vec> res, rhs, sol, corr, c, v, bt, d, w;
+	ua_lu lu;
+};
+
+// ua::bound_solver::setup (ua.hh:110-139)
+or_amg * or_amg_create(const or_csr * Ain, const or_amg_settings * s) {
+	if (!Ain || !s || s->max_levels < 1 || (s->pairwise_passes != 2 && s->pairwise_passes != 3) ||
+	    Ain->nrows != Ain->ncols || Ain->nrows < 1)
+		return nullptr;
+	auto M = new or_amg();
+	M->s = *s;
+	ua_csr H;
+	H.n = Ain->nrows;
+	H.rp.resize(H.n + 1);
+	H.ci.resize(Ain->nnz);
+	H.v.assign(Ain->vals, Ain->vals + Ain->nnz);
+	for (int64_t i = 0; i <= H.n; ++i)
+		H.rp[i] = Ain->itype == 0 ? ((const int32_t *)Ain->rowptr)[i] : ((const int64_t *)Ain->rowptr)[i];
+	for (int64_t e = 0; e < Ain->nnz; ++e)
+		H.ci[e] = Ain->itype == 0 ? ((const int32_t *)Ain->colind)[e] : ((const int64_t *)Ain->colind)[e];
+	M->A.push_back(H);
+	for (int i = 0; i + 1 < s->max_levels; ++i) {
+		const ua_csr & F = M->A.back();
+		ua_agg a1 = ua_pairwise(s->beta, F, true);
+		ua_csr aux = ua_create_aux(F, a1);
+		ua_agg a2 = ua_pairwise(s->beta, aux, false);
+		ua_agg agg;
+		if (s->pairwise_passes == 2) agg = ua_union(a1, a2);
+		else {
+			ua_csr aux1 = ua_create_aux(aux, a2);
+			ua_agg a3 = ua_pairwise(s->beta, aux1, false);
+			agg = ua_union(ua_union(a1, a2), a3);
+		}
+		std::vector<int64_t> at((std::size_t)F.n, -1); // aggt field filled with max
+		for (std::size_t k = 0; k < agg.size(); ++k)
+			for (auto fi : agg[k]) at[fi] = (int64_t)k;
+		ua_csr C = ua_galerkin(F, agg, at);
+		M->aggt.push_back(std::move(at));
+		M->A.push_back(std::move(C));
+		if (M->A.back().n < s->min_coarse) break;
+	}
+	for (const auto & L : M->A) {
+		const std::size_t n = (std::size_t)L.n;
+		for (auto * vv : {&M->res, &M->rhs, &M->sol, &M->corr, &M->c, &M->v, &M->bt, &M->d, &M->w})
+			vv->emplace_back(n, 0.0);
+	}
+	M->lu.factor(M->A.back());
+	return M;
+}
+
+int or_amg_nlevels(const or_amg * M) { return (int)M->A.size(); }
+int64_t or_amg_level_rows(const or_amg * M, int l) { return M->A[l].n; }
+int64_t or_amg_level_nnz(const or_amg * M, int l) { return (int64_t)M->A[l].ci.size(); }
+void or_amg_level_csr(const or_amg * M, int l, int64_t * rp, int64_t * ci, double * v) {
+	const auto & L = M->A[l];
+	std::copy(L.rp.begin(), L.rp.end(), rp);
+	std::copy(L.ci.begin(), L.ci.end(), ci);
+	std::copy(L.v.begin(), L.v.end(), v);
+}
+void or_amg_aggregates(const or_amg * M, int l, int64_t * aggt) {
+	std::copy(M->aggt[l].begin(), M->aggt[l].end(), aggt);
+}
+void or_amg_destroy(or_amg * M) { delete M; }
+
+// One pairwise pass with or without the dominance check: aggregate id per
+// row (-1 outside every aggregate); returns the number of aggregates.
+int64_t or_pairwise_agg(const or_csr * Ain, float beta, int checkdd, int64_t * aggt) {
+	ua_csr H;
+	H.n = Ain->nrows;
+	H.rp.resize(H.n + 1);
+	H.ci.resize(Ain->nnz);
+	H.v.assign(Ain->vals, Ain->vals + Ain->nnz);
+	for (int64_t i = 0; i <= H.n; ++i)
+		H.rp[i] = Ain->itype == 0 ? ((const int32_t *)Ain->rowptr)[i] : ((const int64_t *)Ain->rowptr)[i];
+	for (int64_t e = 0; e < Ain->nnz; ++e)
+		H.ci[e] = Ain->itype == 0 ? ((const int32_t *)Ain->colind)[e] : ((const int64_t *)Ain->colind)[e];
+	ua_agg agg = ua_pairwise(beta, H, checkdd != 0);
+	std::fill(aggt, aggt + H.n, -1);
+	for (std::size_t k = 0; k < agg.size(); ++k)
+		for (auto fi : agg[k]) aggt[fi] = (int64_t)k;
+	return (int64_t)agg.size();
+}
+
+} // extern "C"
+
+namespace {
+
+// ncycle / relaxed_wcycle / relaxed_kappa_cycle / kappa_kcycle
+// (cycle.hh:43-272); kind 0 v, 1 relaxed-w, 2 relaxed-kappa, 3 kappa-k
+void ua_cycle(or_amg & M, std::size_t l, const vec & b, vec & x, int kappa) {
+	const double tau = 1.75;
+	const int kind = M.s.cycle;
+	const std::size_t coarse_lvl = M.A.size() - 1;
+	const ua_csr & A = M.A[l];
+	const ua_csr & Ac = M.A[l + 1];
+	const std::size_t nc = (std::size_t)Ac.n;
+	ua_gs(A, b.data(), x.data(), M.s.nrelax);
+	vec & r = M.res[l];
+	ua_residual(A, b.data(), x.data(), r.data());
+	vec & cb = M.rhs[l + 1];
+	vec & cx = M.sol[l + 1];
+	std::fill(cb.begin(), cb.end(), 0.0); // restrict::ave
+	for (std::size_t i = 0; i < (std::size_t)A.n; ++i)
+		if (M.aggt[l][i] != -1) cb[M.aggt[l][i]] += r[i];
+	std::fill(cx.begin(), cx.end(), 0.0);
+	vec &c = M.c[l + 1], &v = M.v[l + 1], &bt = M.bt[l + 1], &d = M.d[l + 1], &w = M.w[l + 1];
+	if (l + 1 == coarse_lvl)
+		M.lu.solve(cb.data(), cx.data());
+	else if (kind == 0)
+		ua_cycle(M, l + 1, cb, cx, kappa);
+	else if (kind == 1 || (kind == 2 && kappa > 1)) {
+		std::fill(c.begin(), c.end(), 0.0);
+		ua_cycle(M, l + 1, cb, c, kappa);
+		ua_spmv(Ac, c.data(), v.data());
+		for (std::size_t i = 0; i < nc; ++i) bt[i] = (-tau) * v[i] + cb[i];
+		std::fill(d.begin(), d.end(), 0.0);
+		ua_cycle(M, l + 1, bt, d, kind == 2 ? kappa - 1 : kappa);
+		for (std::size_t i = 0; i < nc; ++i) cx[i] = tau * c[i] + tau * d[i];
+	}
+	else if (kind == 2)
+		ua_cycle(M, l + 1, cb, cx, kappa);
+	else if (kappa > 1) {
+		std::fill(c.begin(), c.end(), 0.0);
+		ua_cycle(M, l + 1, cb, c, kappa);
+		ua_spmv(Ac, c.data(), v.data());
+		const double rho1 = ua_dot(c, v);
+		const double alpha1 = ua_dot(c, cb);
+		const double tau1 = alpha1 / rho1;
+		for (std::size_t i = 0; i < nc; ++i) bt[i] = (-tau1) * v[i] + cb[i];
+		if (std::sqrt(ua_dot(bt, bt)) < M.s.ktol * std::sqrt(ua_dot(cb, cb))) {
+			for (std::size_t i = 0; i < nc; ++i) cx[i] = tau1 * c[i] + cx[i];
+		}
+		else {
+			std::fill(d.begin(), d.end(), 0.0);
+			ua_cycle(M, l + 1, bt, d, kappa - 1);
+			ua_spmv(Ac, d.data(), w.data());
+			const double gamma = ua_dot(d, v);
+			const double beta = ua_dot(d, w);
+			const double alpha2 = ua_dot(d, bt);
+			const double rho2 = beta - ((gamma * gamma) / rho1);
+			const double tau2 = tau1 - (gamma * alpha2) / (rho1 * rho2);
+			const double tau3 = alpha2 / rho2;
+			for (std::size_t i = 0; i < nc; ++i) cx[i] = tau2 * c[i] + tau3 * d[i];
+		}
+	}
+	else {
+		std::fill(cx.begin(), cx.end(), 0.0);
+		ua_cycle(M, l + 1, cb, cx, kappa);
+	}
+	vec & corr = M.corr[l]; // prolong::inject, then x.add(x, correction)
+	for (std::size_t i = 0; i < (std::size_t)A.n; ++i)
+		if (M.aggt[l][i] != -1) corr[i] = cx[M.aggt[l][i]];
+	for (std::size_t i = 0; i < (std::size_t)A.n; ++i) x[i] = x[i] + corr[i];
+	ua_gs(A, b.data(), x.data(), M.s.nrelax);
+}
+
+} // namespace
+
+extern "C" {
+
+// bound_solver::apply (ua.hh:141-172)
+int or_amg_apply(or_amg * M, const double * bp, double * xp, int * cycles, double * res) {
+	const std::size_t n = (std::size_t)M->A[0].n;
+	vec b(bp, bp + n), x(n, 0.0);
+	vec & r = M->res[0];
+	ua_residual(M->A[0], b.data(), x.data(), r.data());
+	double rnorm = std::sqrt(ua_dot(r, r));
+	int done = 0;
+	for (int i = 0; i < M->s.maxiter; ++i) {
+		if (M->A.size() == 1) M->lu.solve(b.data(), x.data());
+		else ua_cycle(*M, 0, b, x, M->s.kappa);
+		++done;
+		ua_residual(M->A[0], b.data(), x.data(), r.data());
+		rnorm = std::sqrt(ua_dot(r, r));
+		if (rnorm <= (double)M->s.atol) break;
+	}
+	std::copy(x.begin(), x.end(), xp);
+	if (cycles) *cycles = done;
+	if (res) *res = rnorm;
+	return 0;
 }
 
 } // extern "C"

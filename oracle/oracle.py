@@ -97,6 +97,22 @@ def lib():
         L.or_jacobi_relax.argtypes = [C.POINTER(_Csr), C.c_float, C.c_int, P, P]
         L.or_hybrid_gs.argtypes = [C.POINTER(_Csr), C.c_int, C.c_int, C.c_int, P, P]
         L.or_hybrid_gs.restype = C.c_int
+        L.or_amg_create.argtypes = [C.POINTER(_Csr), C.POINTER(AmgSettings)]
+        L.or_amg_create.restype = C.c_void_p
+        L.or_amg_nlevels.argtypes = [C.c_void_p]
+        L.or_amg_nlevels.restype = C.c_int
+        L.or_amg_level_rows.argtypes = [C.c_void_p, C.c_int]
+        L.or_amg_level_rows.restype = C.c_int64
+        L.or_amg_level_nnz.argtypes = [C.c_void_p, C.c_int]
+        L.or_amg_level_nnz.restype = C.c_int64
+        L.or_amg_level_csr.argtypes = [C.c_void_p, C.c_int, P, P, P]
+        L.or_amg_aggregates.argtypes = [C.c_void_p, C.c_int, P]
+        L.or_amg_apply.argtypes = [C.c_void_p, P, P, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double)]
+        L.or_amg_apply.restype = C.c_int
+        L.or_amg_destroy.argtypes = [C.c_void_p]
+        L.or_pairwise_agg.argtypes = [C.POINTER(_Csr), C.c_float, C.c_int, P]
+        L.or_pairwise_agg.restype = C.c_int64
         solver_args = [C.POINTER(_Csr), C.POINTER(_Prec), P, P, C.POINTER(Settings),
                        DIAG_FN, P, C.POINTER(Info)]
         L.or_cg.argtypes = solver_args + [_D, C.c_int]
@@ -168,6 +184,64 @@ class Csr:
                               _p(b), _p(x)) != 0:
             raise ValueError("hybrid_gs: bad colours or sweep")
         return x
+
+
+class AmgSettings(C.Structure):
+    """or_amg_settings (mg::ua::solver_settings)."""
+    _fields_ = [("max_levels", C.c_int), ("min_coarse", C.c_int64), ("maxiter", C.c_int),
+                ("atol", C.c_float), ("beta", C.c_float), ("pairwise_passes", C.c_int),
+                ("nrelax", C.c_int), ("cycle", C.c_int), ("kappa", C.c_int),
+                ("ktol", C.c_float)]
+
+    def __init__(self, max_levels=20, min_coarse=5, maxiter=10, atol=1e-6, beta=0.25,
+                 pairwise_passes=2, nrelax=2, cycle=0, kappa=1, ktol=0.25):
+        super().__init__(max_levels, min_coarse, maxiter, atol, beta, pairwise_passes,
+                         nrelax, cycle, kappa, ktol)
+
+
+class Amg:
+    """The oracle's mg::ua::bound_solver (setup and apply, one colour)."""
+
+    def __init__(self, A: "Csr", settings: AmgSettings | None = None):
+        self.A = A
+        self.settings = settings or AmgSettings()
+        self.h = lib().or_amg_create(C.byref(A.s), C.byref(self.settings))
+        if not self.h:
+            raise ValueError("or_amg_create: bad settings or matrix")
+
+    def nlevels(self):
+        return lib().or_amg_nlevels(self.h)
+
+    def level(self, l):
+        n = lib().or_amg_level_rows(self.h, l)
+        nz = lib().or_amg_level_nnz(self.h, l)
+        rp, ci, v = np.empty(n + 1, np.int64), np.empty(nz, np.int64), np.empty(nz)
+        lib().or_amg_level_csr(self.h, l, _p(rp), _p(ci), _p(v))
+        return rp, ci, v
+
+    def aggregates(self, l):
+        out = np.empty(lib().or_amg_level_rows(self.h, l), np.int64)
+        lib().or_amg_aggregates(self.h, l, _p(out))
+        return out
+
+    def apply(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(b)
+        cy, res = C.c_int(), C.c_double()
+        lib().or_amg_apply(self.h, _p(b), _p(x), C.byref(cy), C.byref(res))
+        return x, cy.value, res.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_amg_destroy(self.h)
+            self.h = None
+
+
+def pairwise_agg(A: "Csr", beta=0.25, checkdd=True):
+    """One pairwise aggregation pass: aggregate id per row (-1: none), count."""
+    out = np.empty(A.s.nrows, np.int64)
+    k = lib().or_pairwise_agg(C.byref(A.s), float(beta), int(checkdd), _p(out))
+    return out, k
 
 
 def prec(kind: int = 0, M: Csr | None = None, omega=np.float32(2 / 3.),

@@ -1,0 +1,194 @@
+"""Oracle UA-AMG (mg::ua::bound_solver, flecsolve/solvers/mg/{ua,coarsen,
+intergrid,cycle,cg_solve}.hh) against what the mathematics fixes.
+
+Pins:
+  * one pairwise pass on the 1D Laplacian, traced by hand through the
+    reference's priority queue: aggregates {0,1}, {7,6}, {2,3}, {5,4}
+    (the tie between rows 7 and 2 goes to the earlier insertion);
+  * the dominance check's int accumulator: off-diagonals summing to less
+    than 1 in magnitude count as 0, so such rows stay out of every
+    aggregate; a row whose truncated sum makes a_rr <= 5 sum stays in;
+  * aggregation invariants: every candidate in exactly one aggregate, pairs
+    only along strong couplings, sizes <= 4 after two passes;
+  * the Galerkin identity of the reference's galerkin_prod.cc test:
+    A_c = P^T A P with P the aggregate injection;
+  * the coarse solve: one level is a dense LU solve (numpy.linalg.solve);
+  * the V-cycle against a dense numpy restatement built from the oracle's
+    own hierarchy (Gauss-Seidel in matrix form, P and P^T as matrices);
+  * every cycle type converges on the 2D Laplacian, at the rates known for
+    unsmoothed aggregation.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+
+
+def tridiag(n, d=2.0, o=-1.0):
+    M = sp.diags([np.full(n - 1, o), np.full(n, d), np.full(n - 1, o)], [-1, 0, 1],
+                 format="csr")
+    return M.indptr.astype(np.int64), M.indices.astype(np.int64), M.data.copy()
+
+
+def test_one_pass_on_1d_laplacian_by_hand():
+    rp, ci, v = tridiag(8)
+    aggt, k = O.pairwise_agg(O.Csr(rp, ci, v), 0.25, True)
+    assert k == 4
+    np.testing.assert_array_equal(aggt, [0, 0, 2, 2, 3, 3, 1, 1])
+
+
+def int_dominance(rp, ci, v):
+    """Rows the dominance check keeps out (a_rr > 5 * int-truncated sum)."""
+    out = []
+    for r in range(rp.size - 1):
+        s, d = 0, 0.0
+        for e in range(rp[r], rp[r + 1]):
+            if ci[e] != r:
+                s = int(s + abs(v[e]))
+            else:
+                d = v[e]
+        out.append(d > 5 * s)
+    return np.array(out)
+
+
+def test_dominance_check_truncates():
+    # row 0: off-diagonals -0.6, -0.3 -> sum 0 -> dominant, left out
+    # row 1: -0.6, -2.5 -> int(0.6) = 0, int(0 + 2.5) = 2 -> 10 > 10 false
+    # row 2: -2.5, -0.3 -> 2, then int(2.3) = 2 -> 3 > 10 false
+    # row 3: -0.3 -> 0 -> 4 > 0: left out
+    M = sp.csr_matrix(np.array([[10, -0.6, -0.3, 0],
+                                [-0.6, 10, -2.5, 0],
+                                [0, -2.5, 3, -0.3],
+                                [0, 0, -0.3, 4.0]]))
+    rp, ci, v = M.indptr.astype(np.int64), M.indices.astype(np.int64), M.data
+    dd = int_dominance(rp, ci, v)
+    np.testing.assert_array_equal(dd, [True, False, False, True])
+    aggt, _ = O.pairwise_agg(O.Csr(rp, ci, v), 0.25, True)
+    assert (aggt[dd] == -1).all() and (aggt[~dd] >= 0).all()
+    aggt2, _ = O.pairwise_agg(O.Csr(rp, ci, v), 0.25, False)
+    assert (aggt2 >= 0).all()
+
+
+def lognormal3d(n, seed):
+    return P.diffusion3d(n, P.lognormal_field(n, 1.0, 3.0, seed))
+
+
+@pytest.mark.parametrize("case", ["p2d", "d3d"])
+def test_pass_invariants(case):
+    rp, ci, v = P.poisson2d(20) if case == "p2d" else lognormal3d(9, 3)
+    rp, ci = rp.astype(np.int64), ci.astype(np.int64)
+    A = O.Csr(rp, ci, v)
+    S = sp.csr_matrix((v, ci, rp))
+    dd = int_dominance(rp, ci, v)
+    aggt, k = O.pairwise_agg(A, 0.25, True)
+    assert (aggt[dd] == -1).all() and (aggt[~dd] >= 0).all()
+    sizes = np.bincount(aggt[aggt >= 0], minlength=k)
+    assert sizes.min() >= 1 and sizes.max() <= 2
+    rowmin = np.array([S.data[S.indptr[r]:S.indptr[r + 1]].min() for r in range(S.shape[0])])
+    for a in np.flatnonzero(sizes == 2):
+        i, j = np.flatnonzero(aggt == a)
+        # the pair is a strong coupling in at least the selected row's sense
+        assert S[i, j] < 0.25 * rowmin[i] or S[j, i] < 0.25 * rowmin[j]
+
+
+def dense_hierarchy(M):
+    mats, Ps = [], []
+    for l in range(M.nlevels()):
+        rp, ci, v = M.level(l)
+        mats.append(sp.csr_matrix((v, ci, rp)).toarray())
+        if l + 1 < M.nlevels():
+            at = M.aggregates(l)
+            nc = M.level(l + 1)[0].size - 1
+            Pm = np.zeros((at.size, nc))
+            for i, a in enumerate(at):
+                if a >= 0:
+                    Pm[i, a] = 1.0
+            Ps.append(Pm)
+    return mats, Ps
+
+
+@pytest.mark.parametrize("passes", [2, 3])
+def test_galerkin_identity_and_sizes(passes):
+    rp, ci, v = lognormal3d(10, 5)
+    M = O.Amg(O.Csr(rp, ci, v), O.AmgSettings(max_levels=4, pairwise_passes=passes))
+    mats, Ps = dense_hierarchy(M)
+    for l, Pm in enumerate(Ps):
+        G = Pm.T @ mats[l] @ Pm
+        np.testing.assert_allclose(mats[l + 1], G, rtol=1e-12, atol=1e-12)
+        at = M.aggregates(l)
+        assert np.bincount(at[at >= 0]).max() <= 2 ** passes
+    assert mats[0].shape[0] > mats[1].shape[0] > mats[2].shape[0]
+
+
+def test_single_level_is_a_dense_solve():
+    rp, ci, v = P.poisson2d(12)
+    M = O.Amg(O.Csr(rp, ci, v), O.AmgSettings(max_levels=1, maxiter=1, atol=0.0))
+    b = P.uniform(144, 2)
+    x, cycles, _ = M.apply(b)
+    assert cycles == 1
+    np.testing.assert_allclose(x, np.linalg.solve(sp.csr_matrix((v, ci, rp)).toarray(), b),
+                               rtol=1e-12, atol=1e-12)
+
+
+def gs_sym(A, b, x, nrelax):
+    for _ in range(nrelax):
+        L = np.tril(A)
+        x = sla.solve_triangular(L, b - (A - L) @ x, lower=True)
+        U = np.triu(A)
+        x = sla.solve_triangular(U, b - (A - U) @ x, lower=False)
+    return x
+
+
+def vcycle_dense(mats, Ps, l, b, x, nrelax):
+    A = mats[l]
+    x = gs_sym(A, b, x, nrelax)
+    r = b - A @ x
+    bc = Ps[l].T @ r
+    if l + 1 == len(mats) - 1:
+        xc = np.linalg.solve(mats[l + 1], bc)
+    else:
+        xc = vcycle_dense(mats, Ps, l + 1, bc, np.zeros_like(bc), nrelax)
+    x = x + Ps[l] @ xc
+    return gs_sym(A, b, x, nrelax)
+
+
+@pytest.mark.parametrize("nrelax", [1, 2])
+def test_vcycle_matches_dense_restatement(nrelax):
+    rp, ci, v = lognormal3d(8, 11)
+    M = O.Amg(O.Csr(rp, ci, v), O.AmgSettings(max_levels=5, maxiter=2, atol=0.0,
+                                               nrelax=nrelax))
+    mats, Ps = dense_hierarchy(M)
+    b = P.uniform(512, 12)
+    x, cycles, res = M.apply(b)
+    assert cycles == 2
+    ref = np.zeros(512)
+    for _ in range(2):
+        ref = vcycle_dense(mats, Ps, 0, b, ref, nrelax)
+    np.testing.assert_allclose(x, ref, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(res, np.linalg.norm(b - mats[0] @ x), rtol=1e-12)
+
+
+def test_cycles_converge_on_laplacian():
+    """Unsmoothed aggregation: V-cycles contract slowly (~0.75 per cycle on
+    the 2D Laplacian), the recursive relaxed-W / kappa / K-cycles ~0.5."""
+    rp, ci, v = P.poisson2d(64)
+    S = sp.csr_matrix((v, ci, rp))
+    b = P.uniform(64 * 64, 4)
+    for cyc, bound in ((0, 5e-3), (1, 1e-4), (2, 1e-4), (3, 1e-4)):
+        prev = 1.0
+        for k in (2, 6, 12):
+            M = O.Amg(O.Csr(rp, ci, v), O.AmgSettings(max_levels=10, maxiter=k, atol=0.0,
+                                                       cycle=cyc, kappa=2))
+            x, cycles, res = M.apply(b)
+            assert cycles == k
+            rel = np.linalg.norm(b - S @ x) / np.linalg.norm(b)
+            assert rel < prev
+            prev = rel
+        assert rel < bound, (cyc, rel)
+    # atol stops the cycles early
+    M = O.Amg(O.Csr(rp, ci, v), O.AmgSettings(max_levels=10, maxiter=50, atol=1e-3))
+    x, cycles, res = M.apply(b)
+    assert res <= 1e-3 and cycles < 50
