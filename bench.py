@@ -116,7 +116,7 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the SpMV and copy kernels only (profiling runs)")
     ap.add_argument("--workload", choices=["diffusion", "batched", "adjoint", "convdiff",
-                                           "smoother"],
+                                           "smoother", "amg"],
                     default="diffusion")
     ap.add_argument("--heat-steps", type=int, default=50)
     ap.add_argument("--ckpt", type=int, default=10)
@@ -145,6 +145,8 @@ def main():
         return bench_convdiff(a, fsx, P)
     if a.workload == "smoother":
         return bench_smoother(a, fsx, P)
+    if a.workload == "amg":
+        return bench_amg(a, fsx, P)
 
     seeds = P.seeds()
     n = a.n
@@ -727,6 +729,82 @@ def bench_smoother(a, fsx, P):
                                "sample": f"one symmetric sweep of the same {n}^3 problem "
                                          "from x = 0 (the sequential sweep of the serial "
                                          "backend)"}
+    print(json.dumps(out), flush=True)
+
+
+def bench_amg(a, fsx, P):
+    """CG preconditioned by one UA-AMG V-cycle (mg::ua::bound_solver,
+    maxiter 1) on the 256^3 diffusion problem to 1e-10, with the reference's
+    hybrid Gauss-Seidel smoothing and with weighted Jacobi smoothing, beside
+    the fused Jacobi-preconditioned CG."""
+    import scipy.sparse as sps
+    n = a.n
+    seeds = P.seeds()
+    k = P.lognormal_field(n, a.sigma, 8.0, seeds["diffusion_k"])
+    rp, ci, v = P.diffusion3d(n, k)
+    del k
+    N, nnz = n ** 3, int(ci.size)
+    b = P.uniform(N, seeds["diffusion_rhs"])
+    S = sps.csr_matrix((v, ci, rp), shape=(N, N))
+    ctx = fsx.Context(0)
+    A = ctx.csr(rp, ci, v)
+    bd = ctx.vec(b)
+    s = fsx.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True)
+    results = {}
+    x = ctx.zeros(N)
+    ts = []
+    dinv = A.diag_inv()
+    for i in range(a.warmup + a.steps):
+        ctx.sync()
+        t = time.perf_counter()
+        info = fsx.cg_solve(A, bd, x, s, dinv=dinv)
+        ctx.sync()
+        if i >= a.warmup:
+            ts.append(time.perf_counter() - t)
+    results["jacobi_cg_fused"] = {"seconds": round(sum(ts) / len(ts), 4), "iters": info.iters}
+    log(f"fused Jacobi CG: {results['jacobi_cg_fused']['seconds']:.3f} s, {info.iters} iters")
+    for name, smoother, nrelax in (("amg_jacobi_smoother", 1, 2), ("amg_hybrid_gs", 0, 1)):
+        t0 = time.perf_counter()
+        M = fsx.AMG(A, fsx.AmgSettings(max_levels=20, maxiter=1, atol=0.0, smoother=smoother,
+                                       nrelax=nrelax))
+        t_setup = time.perf_counter() - t0
+        lv = M.levels()
+        log(f"{name}: setup {t_setup:.2f} s, {len(lv)} levels, rows {[r for r, _ in lv]}")
+        ts = []
+        for i in range(a.warmup + a.steps):
+            ctx.sync()
+            t = time.perf_counter()
+            info = M.cg(bd, x, s)
+            ctx.sync()
+            dt = time.perf_counter() - t
+            log(f"{name} {'warmup' if i < a.warmup else 'step'} {i}: {dt:.3f} s, "
+                f"{info.iters} iters, {info.reason}")
+            if i >= a.warmup:
+                ts.append(dt)
+        rel = float(np.linalg.norm(b - S @ x.get()) / np.linalg.norm(b))
+        results[name] = {"seconds": round(sum(ts) / len(ts), 4), "iters": info.iters,
+                         "status": info.reason, "setup_seconds": round(t_setup, 2),
+                         "levels": len(lv), "nrelax": nrelax, "true_rel_residual": rel}
+        M.destroy()
+    best = min(results["amg_jacobi_smoother"]["seconds"], results["amg_hybrid_gs"]["seconds"])
+    out = {
+        "metric": f"time-to-1e-10 residual, {n}^3 diffusion: UA-AMG-preconditioned CG",
+        "value": results["amg_jacobi_smoother"]["seconds"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(results["amg_jacobi_smoother"]["seconds"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal GRF coefficient, uniform rhs)",
+        "config": {"workload": f"amg_cg_{n}^3_lognormal_sigma{a.sigma:g}_rtol1e-10",
+                   "rows": N, "nnz": nnz, "solvers": results,
+                   "best_amg_speedup_vs_jacobi_cg":
+                       round(results["jacobi_cg_fused"]["seconds"] / best, 3)},
+    }
     print(json.dumps(out), flush=True)
 
 

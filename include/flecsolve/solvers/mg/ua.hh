@@ -63,6 +63,8 @@ inline fsx_amg_settings to_fsx(const solver_settings & s) {
 	o.cycle = (int)s.cycle;
 	o.kappa = s.kappa;
 	o.ktol = s.ktol;
+	o.smoother = 0; // hybrid Gauss-Seidel, as create_smoother (ua.hh:184-197)
+	o.jacobi_weight = s.jacobi_weight;
 	return o;
 }
 

@@ -245,6 +245,10 @@ typedef struct {
 	int cycle;
 	int kappa;
 	float ktol;
+	int smoother;        /* 0 hybrid Gauss-Seidel (create_smoother, ua.hh:184-197),
+	                        1 weighted Jacobi with jacobi_weight (the alternative
+	                        create_smoother keeps commented out, ua.hh:188-190) */
+	float jacobi_weight; /* 0.6666666 */
 } fsx_amg_settings;
 typedef struct fsx_amg fsx_amg;
 int fsx_amg_create(fsx_ctx * ctx, const fsx_csr * A, const fsx_amg_settings * s,

@@ -1402,13 +1402,15 @@ typedef struct {
 	int cycle;
 	int kappa;
 	float ktol;
+	int smoother; // 0 hybrid Gauss-Seidel, 1 weighted Jacobi (jacobi_weight)
+	float jacobi_weight;
 } or_amg_settings;
 
 struct or_amg {
 	or_amg_settings s;
 	std::vector<ua_csr> A;                 // level matrices
 	std::vector<std::vector<int64_t>> aggt; // level l -> l + 1 (-1: none)
-	std::vector<vec> res, rhs, sol, corr, c, v, bt, d, w;
+	std::vector<vec> res, rhs, sol, corr, c, v, bt, d, w, tmp;
 	ua_lu lu;
 };
 
@@ -1451,7 +1453,8 @@ or_amg * or_amg_create(const or_csr * Ain, const or_amg_settings * s) {
 	}
 	for (const auto & L : M->A) {
 		const std::size_t n = (std::size_t)L.n;
-		for (auto * vv : {&M->res, &M->rhs, &M->sol, &M->corr, &M->c, &M->v, &M->bt, &M->d, &M->w})
+		for (auto * vv : {&M->res, &M->rhs, &M->sol, &M->corr, &M->c, &M->v, &M->bt, &M->d, &M->w,
+		                  &M->tmp})
 			vv->emplace_back(n, 0.0);
 	}
 	M->lu.factor(M->A.back());
@@ -1497,6 +1500,31 @@ namespace {
 
 // ncycle / relaxed_wcycle / relaxed_kappa_cycle / kappa_kcycle
 // (cycle.hh:43-272); kind 0 v, 1 relaxed-w, 2 relaxed-kappa, 3 kappa-k
+// the level smoother: symmetric hybrid Gauss-Seidel, or weighted Jacobi
+// (mg::bound_jacobi, jacobi.hh:58-93) for settings.smoother == 1
+void ua_smooth(or_amg & M, std::size_t l, const vec & b, vec & x) {
+	const ua_csr & A = M.A[l];
+	if (M.s.smoother == 0) {
+		ua_gs(A, b.data(), x.data(), M.s.nrelax);
+		return;
+	}
+	vec & tmp = M.tmp[l];
+	const double omega = (double)M.s.jacobi_weight;
+	for (int s = 0; s < M.s.nrelax; ++s) {
+		std::copy(x.begin(), x.end(), tmp.begin());
+		for (int64_t r = 0; r < A.n; ++r) {
+			double diag = 0;
+			double lpu_x = 0;
+			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) {
+				if (A.ci[off] == r) diag = A.v[off];
+				else lpu_x += A.v[off] * tmp[A.ci[off]];
+			}
+			const double dinv = 1. / diag;
+			x[r] = omega * dinv * (b[r] - lpu_x) + (1 - omega) * tmp[r];
+		}
+	}
+}
+
 void ua_cycle(or_amg & M, std::size_t l, const vec & b, vec & x, int kappa) {
 	const double tau = 1.75;
 	const int kind = M.s.cycle;
@@ -1504,7 +1532,7 @@ void ua_cycle(or_amg & M, std::size_t l, const vec & b, vec & x, int kappa) {
 	const ua_csr & A = M.A[l];
 	const ua_csr & Ac = M.A[l + 1];
 	const std::size_t nc = (std::size_t)Ac.n;
-	ua_gs(A, b.data(), x.data(), M.s.nrelax);
+	ua_smooth(M, l, b, x);
 	vec & r = M.res[l];
 	ua_residual(A, b.data(), x.data(), r.data());
 	vec & cb = M.rhs[l + 1];
@@ -1561,7 +1589,7 @@ void ua_cycle(or_amg & M, std::size_t l, const vec & b, vec & x, int kappa) {
 	for (std::size_t i = 0; i < (std::size_t)A.n; ++i)
 		if (M.aggt[l][i] != -1) corr[i] = cx[M.aggt[l][i]];
 	for (std::size_t i = 0; i < (std::size_t)A.n; ++i) x[i] = x[i] + corr[i];
-	ua_gs(A, b.data(), x.data(), M.s.nrelax);
+	ua_smooth(M, l, b, x);
 }
 
 } // namespace

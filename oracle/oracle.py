@@ -191,12 +191,13 @@ class AmgSettings(C.Structure):
     _fields_ = [("max_levels", C.c_int), ("min_coarse", C.c_int64), ("maxiter", C.c_int),
                 ("atol", C.c_float), ("beta", C.c_float), ("pairwise_passes", C.c_int),
                 ("nrelax", C.c_int), ("cycle", C.c_int), ("kappa", C.c_int),
-                ("ktol", C.c_float)]
+                ("ktol", C.c_float), ("smoother", C.c_int), ("jacobi_weight", C.c_float)]
 
     def __init__(self, max_levels=20, min_coarse=5, maxiter=10, atol=1e-6, beta=0.25,
-                 pairwise_passes=2, nrelax=2, cycle=0, kappa=1, ktol=0.25):
+                 pairwise_passes=2, nrelax=2, cycle=0, kappa=1, ktol=0.25, smoother=0,
+                 jacobi_weight=0.6666666):
         super().__init__(max_levels, min_coarse, maxiter, atol, beta, pairwise_passes,
-                         nrelax, cycle, kappa, ktol)
+                         nrelax, cycle, kappa, ktol, smoother, jacobi_weight)
 
 
 class Amg:

@@ -14,7 +14,8 @@ Pins:
     A_c = P^T A P with P the aggregate injection;
   * the coarse solve: one level is a dense LU solve (numpy.linalg.solve);
   * the V-cycle against a dense numpy restatement built from the oracle's
-    own hierarchy (Gauss-Seidel in matrix form, P and P^T as matrices);
+    own hierarchy (Gauss-Seidel in matrix form or weighted Jacobi, P and
+    P^T as matrices);
   * every cycle type converges on the 2D Laplacian, at the rates known for
     unsmoothed aggregation.
 """
@@ -133,40 +134,45 @@ def test_single_level_is_a_dense_solve():
                                rtol=1e-12, atol=1e-12)
 
 
-def gs_sym(A, b, x, nrelax):
+def smooth_dense(A, b, x, nrelax, smoother):
     for _ in range(nrelax):
-        L = np.tril(A)
-        x = sla.solve_triangular(L, b - (A - L) @ x, lower=True)
-        U = np.triu(A)
-        x = sla.solve_triangular(U, b - (A - U) @ x, lower=False)
+        if smoother == 0:  # symmetric Gauss-Seidel in matrix form
+            L = np.tril(A)
+            x = sla.solve_triangular(L, b - (A - L) @ x, lower=True)
+            U = np.triu(A)
+            x = sla.solve_triangular(U, b - (A - U) @ x, lower=False)
+        else:  # weighted Jacobi: x + w D^-1 (b - A x)
+            w = float(np.float32(0.6666666))
+            x = x + w * (b - A @ x) / np.diag(A)
     return x
 
 
-def vcycle_dense(mats, Ps, l, b, x, nrelax):
+def vcycle_dense(mats, Ps, l, b, x, nrelax, smoother):
     A = mats[l]
-    x = gs_sym(A, b, x, nrelax)
+    x = smooth_dense(A, b, x, nrelax, smoother)
     r = b - A @ x
     bc = Ps[l].T @ r
     if l + 1 == len(mats) - 1:
         xc = np.linalg.solve(mats[l + 1], bc)
     else:
-        xc = vcycle_dense(mats, Ps, l + 1, bc, np.zeros_like(bc), nrelax)
+        xc = vcycle_dense(mats, Ps, l + 1, bc, np.zeros_like(bc), nrelax, smoother)
     x = x + Ps[l] @ xc
-    return gs_sym(A, b, x, nrelax)
+    return smooth_dense(A, b, x, nrelax, smoother)
 
 
+@pytest.mark.parametrize("smoother", [0, 1])
 @pytest.mark.parametrize("nrelax", [1, 2])
-def test_vcycle_matches_dense_restatement(nrelax):
+def test_vcycle_matches_dense_restatement(nrelax, smoother):
     rp, ci, v = lognormal3d(8, 11)
     M = O.Amg(O.Csr(rp, ci, v), O.AmgSettings(max_levels=5, maxiter=2, atol=0.0,
-                                               nrelax=nrelax))
+                                               nrelax=nrelax, smoother=smoother))
     mats, Ps = dense_hierarchy(M)
     b = P.uniform(512, 12)
     x, cycles, res = M.apply(b)
     assert cycles == 2
     ref = np.zeros(512)
     for _ in range(2):
-        ref = vcycle_dense(mats, Ps, 0, b, ref, nrelax)
+        ref = vcycle_dense(mats, Ps, 0, b, ref, nrelax, smoother)
     np.testing.assert_allclose(x, ref, rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(res, np.linalg.norm(b - mats[0] @ x), rtol=1e-12)
 
