@@ -355,45 +355,6 @@ __global__ __launch_bounds__(256) void amg_prolong_add_kernel(int64_t n, const I
 	x[i] = x[i] + (a >= 0 ? xc[a] : 0.0);
 }
 
-// Weighted Jacobi sweep (mg::bound_jacobi::relax, jacobi.hh:58-93): tmp is
-// x before the sweep; row r sums its off-diagonal entries in CSR order, in
-// tiles of 8 loads issued before use.
-template<class I>
-__global__ __launch_bounds__(256) void amg_jacobi_kernel(int64_t n, const I * __restrict__ rp,
-                                                         const I * __restrict__ ci,
-                                                         const double * __restrict__ va,
-                                                         double omega,
-                                                         const double * __restrict__ b,
-                                                         const double * __restrict__ tmp,
-                                                         double * __restrict__ x) {
-	const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if (r >= n) return;
-	const I lo = rp[r], hi = rp[r + 1];
-	const double br = b[r], tr = tmp[r];
-	double diag = 0;
-	double lpu_x = 0;
-	for (I e0 = lo; e0 < hi; e0 += 8) {
-		int64_t c[8];
-		double v[8], xv[8];
-#pragma unroll
-		for (int k = 0; k < 8; ++k) {
-			const I e = min((I)(e0 + k), (I)(hi - 1));
-			c[k] = ci[e];
-			v[k] = va[e];
-		}
-#pragma unroll
-		for (int k = 0; k < 8; ++k) xv[k] = tmp[c[k]];
-#pragma unroll
-		for (int k = 0; k < 8; ++k) {
-			if (e0 + k >= hi) continue;
-			if (c[k] == r) diag = v[k];
-			else lpu_x += v[k] * xv[k];
-		}
-	}
-	const double dinv = 1. / diag;
-	x[r] = omega * dinv * (br - lpu_x) + (1 - omega) * tr;
-}
-
 // z = a x + b y (linear_sum) and z = a x + y (axpy), elementwise
 __global__ __launch_bounds__(256) void amg_lsum_kernel(int64_t n, double a, const double * __restrict__ x,
                                                        double b, const double * __restrict__ y,
@@ -418,7 +379,7 @@ struct fsx_amg {
 		std::unique_ptr<dbuf> aggt, mptr, midx;
 		std::vector<int64_t> aggt_host;
 		fsx_vec *res = nullptr, *rhs = nullptr, *sol = nullptr, *c = nullptr, *v = nullptr,
-		        *bt = nullptr, *d = nullptr, *w = nullptr, *tmp = nullptr;
+		        *bt = nullptr, *d = nullptr, *w = nullptr;
 	};
 	fsx_ctx * ctx = nullptr;
 	fsx_amg_settings s{};
@@ -429,7 +390,7 @@ struct fsx_amg {
 		for (auto & L : lv) {
 			if (L.G) fsx_gs_destroy(L.G);
 			if (L.owned && L.A) fsx_csr_destroy(L.A);
-			for (fsx_vec * p : {L.res, L.rhs, L.sol, L.c, L.v, L.bt, L.d, L.w, L.tmp})
+			for (fsx_vec * p : {L.res, L.rhs, L.sol, L.c, L.v, L.bt, L.d, L.w})
 				if (p) fsx_vec_release(p);
 		}
 	}
@@ -492,23 +453,7 @@ void smooth(fsx_amg & M, size_t l, const fsx_vec * b, fsx_vec * x) {
 		rc(fsx_gs_relax(M.ctx, L.G, 2, M.s.nrelax, b, x));
 		return;
 	}
-	const double omega = (double)M.s.jacobi_weight;
-	for (int s = 0; s < M.s.nrelax; ++s) {
-		launch_copy(M.ctx, x, L.tmp);
-		if (L.A->it == FSX_I32)
-			hipLaunchKernelGGL(amg_jacobi_kernel<int32_t>, dim3(blocks(L.n)), dim3(256), 0,
-			                   M.ctx->stream, L.n, (const int32_t *)L.A->rowptr(),
-			                   (const int32_t *)L.A->colind(), (const double *)L.A->vals->p,
-			                   omega, (const double *)b->ptr(), (const double *)L.tmp->ptr(),
-			                   (double *)x->ptr());
-		else
-			hipLaunchKernelGGL(amg_jacobi_kernel<int64_t>, dim3(blocks(L.n)), dim3(256), 0,
-			                   M.ctx->stream, L.n, (const int64_t *)L.A->rowptr(),
-			                   (const int64_t *)L.A->colind(), (const double *)L.A->vals->p,
-			                   omega, (const double *)b->ptr(), (const double *)L.tmp->ptr(),
-			                   (double *)x->ptr());
-		FSX_HIP(hipGetLastError());
-	}
+	rc(fsx_jacobi_relax(M.ctx, L.A, M.s.jacobi_weight, M.s.nrelax, b, x));
 }
 
 void restrict_to(fsx_amg & M, size_t l, const fsx_vec * r, fsx_vec * y) {
@@ -690,7 +635,6 @@ int fsx_amg_create(fsx_ctx * ctx, const fsx_csr * A, const fsx_amg_settings * s,
 			auto & L = M->lv[l];
 			if (l + 1 < M->lv.size() && s->smoother == 0) rc(fsx_gs_create(ctx, L.A, 1, &L.G));
 			L.res = newvec(ctx, L.n);
-			if (l + 1 < M->lv.size() && s->smoother == 1) L.tmp = newvec(ctx, L.n);
 			if (l > 0) {
 				L.rhs = newvec(ctx, L.n);
 				L.sol = newvec(ctx, L.n);

@@ -276,6 +276,14 @@ void ctx_release(fsx_ctx * c) {
 	}
 }
 
+double * ensure_relax_buffer(fsx_ctx * ctx, size_t n) {
+	if (n <= ctx->relax_cap) return (double *)ctx->relax_buf->p;
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+	ctx->relax_buf = std::make_unique<dbuf>(std::max<size_t>(n, 1) * sizeof(double));
+	ctx->relax_cap = std::max<size_t>(n, 1);
+	return (double *)ctx->relax_buf->p;
+}
+
 double * ensure_partials(fsx_ctx * ctx, size_t count) {
 	if (count <= ctx->part_cap) return ctx->ws.partials;
 	size_t cap = std::max<size_t>(count, 2 * ctx->part_cap);

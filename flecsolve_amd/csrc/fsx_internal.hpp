@@ -103,6 +103,8 @@ struct fsx_ctx {
 	std::unique_ptr<fsx::dbuf> ws_buf;
 	std::unique_ptr<fsx::dbuf> part_buf; // per-block partials
 	size_t part_cap = 0;                 // in doubles
+	std::unique_ptr<fsx::dbuf> relax_buf; // relaxation sweep buffer
+	size_t relax_cap = 0;                 // in doubles
 	fsx::reduce_ws ws;
 	double * host_scalars = nullptr; // pinned, kScalarSlots
 	uint64_t host_syncs = 0;
@@ -206,6 +208,9 @@ void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A);
 // Make ctx->ws.partials hold at least `count` doubles.  Reallocation drops
 // captured graphs (they hold the old address).
 double * ensure_partials(fsx_ctx * ctx, size_t count);
+// A context-owned vector of at least n doubles for relaxation sweeps (not
+// used inside captured graphs; reallocation synchronises the stream).
+double * ensure_relax_buffer(fsx_ctx * ctx, size_t n);
 // Drop captured solver graphs of ctx (defined with the CG driver).
 void invalidate_graphs(fsx_ctx * ctx);
 

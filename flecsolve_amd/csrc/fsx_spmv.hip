@@ -42,6 +42,29 @@ struct epi_residual {
 	__device__ void block_end(const double (&)[NV], unsigned, unsigned) const {}
 };
 
+// One weighted Jacobi sweep from xin (the x of the sweep's start) into
+// xout: x_i = omega (1/a_ii)(b_i - sum_{j != i} a_ij xin_j) + (1 - omega)
+// xin_i, the off-diagonal sum in CSR order (mg::bound_jacobi::relax,
+// solvers/mg/jacobi.hh:58-93).  Served by the sliced and slice-offset
+// kernels only.
+struct epi_relax {
+	static constexpr int NV = 1;
+	static constexpr bool reduces = false;
+	static constexpr bool offdiag = true;
+	double omega;
+	const double * b;
+	double * xout;
+	__device__ bool skip() const { return false; }
+	template<class XI>
+	__device__ void row_offdiag(int64_t i, double s, double diag, XI && xi) const {
+		const double dinv = 1. / diag;
+		xout[i] = omega * dinv * (b[i] - s) + (1 - omega) * xi();
+	}
+	template<class XI>
+	__device__ void row(int64_t, double, double (&)[NV], XI &&) const {}
+	__device__ void block_end(const double (&)[NV], unsigned, unsigned) const {}
+};
+
 // w = A p and the per-block partials of w . p  (cg.hh:95-98: A.apply(p, w);
 // w.dot(p)); a finalize kernel sums the partials.
 template<class ACC, class Y>
@@ -922,34 +945,58 @@ int fsx_jacobi_relax(fsx_ctx * ctx, const fsx_csr * A, float omega, int nrelax,
 		                (int64_t)b->n == A->nrows,
 		            "jacobi_relax: shape mismatch");
 		FSX_REQUIRE(nrelax >= 0, "nrelax must be >= 0");
+		FSX_REQUIRE(b->ptr() != x->ptr() || A->nrows == 0, "jacobi_relax: b and x must differ");
 		if (A->nrows == 0 || nrelax == 0) return;
-		dbuf tmp(A->nrows * 8);
-		unsigned g = (unsigned)((A->nrows + 255) / 256);
+		const size_t n = (size_t)A->nrows;
+		double * tmp = ensure_relax_buffer(ctx, n);
+		const unsigned g = (unsigned)((n + 255) / 256);
+		double * xp = (double *)x->ptr();
+		if (A->plan_kind == 2 || A->plan_kind == 3) {
+			// sweeps alternate between x and the buffer and end in x: an odd
+			// count starts from a copy of x (jacobi.hh:63's tmp = x)
+			double * src = xp;
+			double * dst = tmp;
+			if (nrelax % 2) {
+				hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+				                   (const double *)xp, tmp);
+				std::swap(src, dst);
+			}
+			for (int s = 0; s < nrelax; ++s) {
+				epi_relax e{(double)omega, (const double *)b->ptr(), dst};
+				by_matrix_type(A, [&](auto * vtag, auto * itag) {
+					using V = std::remove_pointer_t<decltype(vtag)>;
+					using I = std::remove_pointer_t<decltype(itag)>;
+					if constexpr (std::is_same_v<V, double>)
+						spmv_launch_typed<epi_relax, double, V, I, double>(ctx->stream, A,
+						                                                    (const double *)src, e);
+				});
+				std::swap(src, dst);
+			}
+			FSX_HIP(hipGetLastError());
+			ctx->launches += nrelax + (nrelax % 2);
+			return;
+		}
 		for (int s = 0; s < nrelax; ++s) {
 			// tmp = x (jacobi.hh:63), then one sweep
-			hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, ctx->stream,
-			                   (size_t)A->nrows, (const double *)x->ptr(),
-			                   (double *)tmp.p);
+			hipLaunchKernelGGL(copy_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+			                   (const double *)xp, tmp);
 			if (A->it == FSX_I32)
 				hipLaunchKernelGGL((jacobi_relax_kernel<int32_t>), dim3(g), dim3(256),
 				                   0, ctx->stream, A->nrows,
 				                   (const int32_t *)A->rowptr(),
 				                   (const int32_t *)A->colind(),
 				                   (const double *)A->vals->p, (double)omega,
-				                   (const double *)b->ptr(), (const double *)tmp.p,
-				                   (double *)x->ptr());
+				                   (const double *)b->ptr(), (const double *)tmp, xp);
 			else
 				hipLaunchKernelGGL((jacobi_relax_kernel<int64_t>), dim3(g), dim3(256),
 				                   0, ctx->stream, A->nrows,
 				                   (const int64_t *)A->rowptr(),
 				                   (const int64_t *)A->colind(),
 				                   (const double *)A->vals->p, (double)omega,
-				                   (const double *)b->ptr(), (const double *)tmp.p,
-				                   (double *)x->ptr());
+				                   (const double *)b->ptr(), (const double *)tmp, xp);
 			FSX_HIP(hipGetLastError());
 			ctx->launches += 2;
 		}
-		FSX_HIP(hipStreamSynchronize(ctx->stream)); // tmp is freed on return
 	});
 }
 

@@ -50,8 +50,18 @@
 #include "fsx_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fsx {
+
+// Epilogues with `offdiag = true` receive the row sum without the diagonal
+// entry (the remaining entries still in CSR order) and the diagonal value:
+// row_offdiag(i, sum, diag, x_i).  Only the sliced and slice-offset kernels
+// serve them.
+template<class E, class = void>
+struct is_offdiag : std::false_type {};
+template<class E>
+struct is_offdiag<E, std::void_t<decltype(E::offdiag)>> : std::bool_constant<E::offdiag> {};
 
 constexpr int kSpmvThreads = 256;
 constexpr int kStreamRows = 256;   // rows per workgroup (LDS-staged mapping)
@@ -219,6 +229,7 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sliced_kernel(
 		// (epilogues that need it then do not read x again)
 		typename G::value_type xself{};
 		bool has_self = false;
+		double dgv = 0.; // diagonal value (offdiag epilogues)
 		// Branch-free body as in the slice-offset kernel: all loads of a tile
 		// first, padding (column -1) and slots past the width excluded by a
 		// select.
@@ -238,14 +249,23 @@ __global__ __launch_bounds__(kSpmvThreads) void spmv_sliced_kernel(
 #pragma unroll
 			for (int k = 0; k < kSliceUnroll; ++k) {
 				const ACC t = sum + (ACC)v[k] * (ACC)xv[k];
-				sum = c[k] >= 0 ? t : sum;
-				if ((int64_t)c[k] == r) {
+				const bool isd = (int64_t)c[k] == r;
+				if constexpr (is_offdiag<Epi>::value) {
+					sum = c[k] >= 0 && !isd ? t : sum;
+					dgv = isd ? (double)v[k] : dgv;
+				}
+				else
+					sum = c[k] >= 0 ? t : sum;
+				if (isd) {
 					xself = xv[k];
 					has_self = true;
 				}
 			}
 		}
-		epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
+		if constexpr (is_offdiag<Epi>::value)
+			epi.row_offdiag(r, sum, dgv, [&] { return has_self ? xself : x(r); });
+		else
+			epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
 	}
 	epi.block_end(d, blockIdx.x, gridDim.x);
 }
@@ -279,6 +299,7 @@ __global__ __launch_bounds__(BT) void spmv_sdia_kernel(
 		ACC sum = ACC(0);
 		typename G::value_type xself{};
 		bool has_self = false;
+		double dgv = 0.; // diagonal value (offdiag epilogues)
 		// Branch-free body: every load of a tile is issued before the first
 		// use (slots past the width re-read the last column, absent lanes
 		// gather x[0]); a select keeps `sum` unchanged for absent entries,
@@ -305,14 +326,25 @@ __global__ __launch_bounds__(BT) void spmv_sdia_kernel(
 #pragma unroll
 			for (int k = 0; k < kSliceUnroll; ++k) {
 				const ACC t = sum + (ACC)v[k] * (ACC)xv[k];
-				sum = pr[k] ? t : sum;
-				if (pr[k] && off[k] == 0) {
+				const bool isd = pr[k] && off[k] == 0;
+				if constexpr (is_offdiag<Epi>::value) {
+					sum = pr[k] && !isd ? t : sum;
+					dgv = isd ? (double)v[k] : dgv;
+				}
+				else
+					sum = pr[k] ? t : sum;
+				if (isd) {
 					xself = xv[k];
 					has_self = true;
 				}
 			}
 		}
-		if (r < nrows) epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
+		if (r < nrows) {
+			if constexpr (is_offdiag<Epi>::value)
+				epi.row_offdiag(r, sum, dgv, [&] { return has_self ? xself : x(r); });
+			else
+				epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
+		}
 	}
 	epi.block_end(d, blockIdx.x, gridDim.x);
 }

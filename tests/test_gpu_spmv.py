@@ -235,6 +235,29 @@ def test_diag_inv_and_jacobi(ctx):
     np.testing.assert_array_equal(xd.get(), Ao.jacobi_relax(b, x0, np.float32(2 / 3.), 3))
 
 
+@pytest.mark.parametrize("plan", [3, 2, 0])
+@pytest.mark.parametrize("nrelax", [1, 2, 3])
+def test_jacobi_relax_layouts_bit_exact(ctx, plan, nrelax):
+    """Weighted Jacobi on the slice-offset (3) and sliced (2) layouts (the
+    off-diagonal row sum in CSR order, sweeps alternating between x and a
+    context buffer) and on the CSR kernel (0): each equals the oracle's
+    sequential sweeps bit for bit, for odd and even sweep counts."""
+    n = 14
+    rp, ci, v = P.diffusion3d(n, P.lognormal_field(n, 1.0, 3.0, 61))
+    A = ctx.csr(rp, ci, v)
+    A.set_plan(plan, 0)
+    assert A.plan()[0] == plan
+    Ao = O.Csr(rp, ci, v)
+    b = P.uniform(n ** 3, 62)
+    x0 = P.uniform(n ** 3, 63)
+    xd = ctx.vec(x0)
+    bd = ctx.vec(b)  # keep the handle alive across the call
+    fsx.check(fsx.lib().fsx_jacobi_relax(ctx.h, A.h, np.float32(0.6666666), nrelax,
+                                         bd.h, xd.h))
+    np.testing.assert_array_equal(xd.get(),
+                                  Ao.jacobi_relax(b, x0, np.float32(0.6666666), nrelax))
+
+
 def test_transpose_product(ctx):
     rp, ci, v = ragged(3000, 4)
     A = ctx.csr(rp, ci, v)
