@@ -88,6 +88,18 @@ def build_problems() -> Path:
     return out
 
 
+def build_config_check() -> Path:
+    """Config-file reading check (tests/cpp/config_check.cpp; no device use)."""
+    src = ROOT / "tests" / "cpp" / "config_check.cpp"
+    out = ROOT / "tests" / "cpp" / "config_check"
+    tmp = ROOT / "tests" / "cpp" / "config_check.tmp"
+    _run(["g++", "-O1", "-std=c++17", "-Wall", "-I", str(INC), str(src), "-o", str(tmp),
+          "-L", str(PKG), "-lfsx", f"-Wl,-rpath,{PKG}",
+          "-Wl,-rpath,$ORIGIN/../../flecsolve_amd"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_cpp_tests() -> Path:
     """C++ drop-in check against the header API (tests/cpp)."""
     src = ROOT / "tests" / "cpp" / "cg_dropin.cpp"
@@ -111,6 +123,7 @@ def build_all(oracle: bool = True) -> None:
         build_oracle()
     build_fsx()
     build_cpp_tests()
+    build_config_check()
 
 
 def main() -> None:

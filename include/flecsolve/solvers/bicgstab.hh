@@ -135,6 +135,11 @@ struct bicgstab : base<Params, typename Params::input_var_t,
 namespace flecsolve::bicgstab {
 inline constexpr std::size_t nwork = 8; // res, r~, p, v, p^, s, s^, t
 struct settings : solver_settings {};
+struct options : solver_options {
+	using settings_type = settings;
+	options(const char * pre) : solver_options(pre) {}
+	optdesc operator()(settings & s) { return solver_options::operator()(s); }
+};
 
 template<class Work>
 struct solver : krylov_solver<op::bicgstab, settings, Work> {

@@ -180,6 +180,7 @@ struct fcg : base<Params, typename Params::input_var_t,
 namespace flecsolve::cg {
 inline constexpr std::size_t nwork = 4; // r, z, p, w
 using settings = solver_settings;
+using options = solver_options;
 
 template<class Work>
 struct solver : krylov_solver<op::cg, settings, Work> {

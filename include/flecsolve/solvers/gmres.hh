@@ -41,6 +41,30 @@ struct settings : solver_settings {
 			                            "to max_krylov_dim when not using restart");
 	}
 };
+
+// gmres.hh:368-380
+inline std::istream & operator>>(std::istream & in, precond_side & s) {
+	std::string tok;
+	in >> tok;
+	if (tok == "left") s = precond_side::left;
+	else if (tok == "right") s = precond_side::right;
+	else in.setstate(std::ios_base::failbit);
+	return in;
+}
+
+// gmres.hh:406-421: the common keys plus max-krylov-dim (-1), pre-side
+// (right) and restart (false)
+struct options : solver_options {
+	using settings_type = settings;
+	options(const char * pre) : solver_options(pre) {}
+	optdesc operator()(settings & s) {
+		optdesc d = solver_options::operator()(s);
+		d.with_default(label("max-krylov-dim"), &s.max_krylov_dim, -1)
+			.with_default(label("pre-side"), &s.pre_side, precond_side::right)
+			.with_default(label("restart"), &s.restart, false);
+		return d;
+	}
+};
 }
 
 namespace flecsolve::op {

@@ -3,13 +3,15 @@
 // Same fields and types as the reference's
 // flecsolve/solvers/solver_settings.hh:31-85: rtol and atol are float (so a
 // configured 1e-10 is used as (double)1.00000001e-10f), solve_info norms are
-// float, and success() accepts the three converged reasons.  Configuration
-// parsing (boost::program_options, solver_options) is not part of this path;
-// settings are filled directly.
+// float, and success() accepts the three converged reasons.  solver_options
+// (solver_settings.hh:36-56) declares the same config-file keys for
+// read_config (flecsolve/util/config.hh).
 #ifndef FLECSOLVE_AMD_SOLVERS_SOLVER_SETTINGS_HH
 #define FLECSOLVE_AMD_SOLVERS_SOLVER_SETTINGS_HH
 
 #include <ostream>
+
+#include "flecsolve/util/config.hh"
 
 namespace flecsolve {
 
@@ -18,6 +20,21 @@ struct solver_settings {
 	float rtol = 0;
 	float atol = 0;
 	bool use_zero_guess = false;
+};
+
+// maxiter and use-zero-guess required, rtol and atol default 0
+struct solver_options : with_label {
+	using settings_type = solver_settings;
+	solver_options(const char * pre) : with_label(pre) {}
+	optdesc operator()(solver_settings & s) {
+		optdesc d;
+		d.required(label("maxiter"), &s.maxiter)
+			.with_default(label("rtol"), &s.rtol, 0.0f)
+			.with_default(label("atol"), &s.atol, 0.0f)
+			.required(label("use-zero-guess"), &s.use_zero_guess);
+		return d;
+	}
+	const std::string & get_prefix() const { return prefix; }
 };
 
 struct solve_info {

@@ -4,7 +4,8 @@
 // diagnostic that tracks the A-norm of the error (the solution is 0):
 // monotone decrease and the Trefethen-Bau bound 2((sqrt k - 1)/(sqrt k + 1))^i.
 // Also runs GMRES and BiCGSTAB through the same API, and CG on a Matrix
-// Market file read with matrix_market<>::read(...).tocsr() (argv[1]).
+// Market file read with matrix_market<>::read(...).tocsr() (argv[1]), and
+// solvers chosen at run time from config files (argv[2]: tests/golden).
 // Exit status 0 = pass.
 #include <cmath>
 #include <cstdio>
@@ -15,6 +16,7 @@
 #include "flecsolve/matrices/io/matrix_market.hh"
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
+#include "flecsolve/solvers/factory.hh"
 #include "flecsolve/solvers/gmres.hh"
 #include "flecsolve/solvers/mg/gs.hh"
 #include "fsx_problems.h"
@@ -149,6 +151,26 @@ int main(int argc, char ** argv) {
 		std::printf("cg+symmetric hybrid_gs: %d iterations (no preconditioner: %d), "
 		            "true rel res %.3e\n", fi.iters, ci2.iters, frel);
 		fails += !fi.success() || frel > 1e-9 || fi.iters >= ci2.iters;
+	}
+	if (argc > 2) {
+		// run-time solver choice from config files (krylov_factory), as the
+		// reference's applications build their linear solver
+		op::core<mat::device_csr> A(ctx, Ah);
+		vec::device<double> rhs(ctx, n * n), y(ctx, n * n), r(ctx, n * n);
+		rhs.set_random(3);
+		for (const char * name : {"krylov_cg.cfg", "krylov_gmres.cfg", "krylov_bicgstab.cfg"}) {
+			const std::string path = std::string(argv[2]) + "/" + name;
+			auto fs = read_config(path.c_str(), krylov_factory::options("linear-solver"));
+			auto slv = krylov_factory::make(fs, rhs, op::ref(A));
+			y.set_scalar(0.0);
+			solve_info fi = slv(rhs, y);
+			A.residual(rhs, y, r);
+			const double rel = r.l2norm().get() / rhs.l2norm().get();
+			const double want = name == std::string("krylov_cg.cfg") ? 1e-6 : 1e-10;
+			std::printf("factory %s: %d iterations, %d restarts, true rel res %.3e\n", name,
+			            fi.iters, fi.restarts, rel);
+			fails += !fi.success() || rel > 1.01 * want;
+		}
 	}
 	if (argc > 1) {
 		// the reference's test input path: read, tocsr, device operator, CG
