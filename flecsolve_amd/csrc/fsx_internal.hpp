@@ -162,6 +162,9 @@ struct fsx_csr {
 	int plan_lanes = 0;
 	int64_t max_row = 0;
 	double avg_row = 0;
+	// bumped whenever the values change in place (copies built from them,
+	// e.g. a smoother's, check it)
+	uint64_t vals_version = 0;
 	const void * rowptr() const { return idx->rowptr->p; }
 	const void * colind() const { return idx->colind->p; }
 };

@@ -24,6 +24,17 @@
 // the length of the longest dependency chain (3n - 2 for the natural-order
 // 7-point stencil on n^3): a sweep is a chain of that many short launches,
 // captured once per (b, x, sweep kind, nrelax) in a HIP graph.
+//
+// Each level costs its chain of dependent loads, so each sweep direction
+// keeps a level-ordered copy of what its rows read: per position (a row of
+// a level, levels one after another) the row id, the diagonal value and the
+// entry count, and per level an ELL block (entry j of the level's t-th row
+// at base + j * count + t) of the off-diagonal entries in the order the
+// reference adds them, each column tagged with where its x comes from
+// (this sweep's x, or the snapshot: stored as -c - 1).  A thread loads all
+// of that at once, then b and the x gathers, then stores: two dependent
+// steps instead of four through row ids and row pointers.  The copy's
+// values are refreshed from the matrix when its values change in place.
 #include "fsx_internal.hpp"
 
 #include <algorithm>
@@ -40,69 +51,56 @@ struct colour_map {
 	}
 };
 
-// One pass over row r's entries in CSR order, restricted to own-colour
-// columns (OWN) or to the other colours' columns (!OWN; COLOURS only).
-// Entries go in tiles of kGsTile: every load of a tile is issued before the
-// first use, then the tile is consumed in order (a loop that loaded one
-// entry, gathered its x and added it before the next paid the memory
-// latency once per entry).
 constexpr int kGsTile = 8;
 
-template<class I, bool FWD, bool COLOURS, bool OWN>
-__device__ __forceinline__ void gs_row_pass(int64_t r, I lo, I hi, int64_t cr,
-                                            const I * __restrict__ ci,
-                                            const double * __restrict__ va, const double * x,
-                                            const double * __restrict__ xold,
-                                            const colour_map & cm, double & diag,
-                                            double & rsum) {
-	for (I e0 = lo; e0 < hi; e0 += kGsTile) {
-		int64_t c[kGsTile];
-		double v[kGsTile], xv[kGsTile];
-		bool pr[kGsTile];
-#pragma unroll
-		for (int k = 0; k < kGsTile; ++k) {
-			const I e = min((I)(e0 + k), (I)(hi - 1));
-			c[k] = ci[e];
-			v[k] = va[e];
-			pr[k] = e0 + k < hi;
-			if (COLOURS) pr[k] = pr[k] && ((cm.of(c[k]) == cr) == OWN);
-		}
-#pragma unroll
-		for (int k = 0; k < kGsTile; ++k) {
-			// x only for present entries of rows finished in lower levels;
-			// everything else (absent slots included) reads the snapshot
-			const bool visited = OWN && pr[k] && (FWD ? c[k] < r : c[k] > r);
-			xv[k] = (visited ? x : xold)[c[k]];
-		}
-#pragma unroll
-		for (int k = 0; k < kGsTile; ++k) {
-			if (!pr[k]) continue;
-			if (OWN && c[k] == r) diag = v[k];
-			else rsum += v[k] * xv[k];
-		}
-	}
-}
-
-template<class I, bool FWD, bool COLOURS>
+// One level: thread t relaxes the level's t-th row from the level-ordered
+// copy (entries loaded in tiles of kGsTile before use).
+template<class I>
 __global__ __launch_bounds__(256) void gs_level_kernel(
-	const I * __restrict__ rp, const I * __restrict__ ci, const double * __restrict__ va,
-	const I * __restrict__ rows, int64_t count, const double * __restrict__ b, double * x,
-	const double * __restrict__ xold, colour_map cm) {
+	const I * __restrict__ srow, const int32_t * __restrict__ slen,
+	const double * __restrict__ sdiag, const I * __restrict__ scol,
+	const double * __restrict__ sval, int64_t start, int64_t cnt, int64_t base,
+	const double * __restrict__ b, double * x, const double * __restrict__ xold) {
 	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if (t >= count) return;
-	const int64_t r = rows[t];
-	const I lo = rp[r], hi = rp[r + 1];
+	if (t >= cnt) return;
+	const int64_t p = start + t;
+	const int64_t r = srow[p];
+	const int len = slen[p];
+	const double diag = sdiag[p];
+	const I * cp = scol + base + t;
+	const double * vp = sval + base + t;
 	const double br = b[r];
-	const int64_t cr = COLOURS ? cm.of(r) : 0;
-	double diag = 0.;
 	double rsum = 0.;
-	// the colour's own block (gs.hh:94-104), CSR order
-	gs_row_pass<I, FWD, COLOURS, true>(r, lo, hi, cr, ci, va, x, xold, cm, diag, rsum);
-	// the other colours' columns (gs.hh:105-107), values before the sweep
-	if (COLOURS)
-		gs_row_pass<I, FWD, COLOURS, false>(r, lo, hi, cr, ci, va, x, xold, cm, diag, rsum);
+	for (int j0 = 0; j0 < len; j0 += kGsTile) {
+		I c[kGsTile];
+		double v[kGsTile], xv[kGsTile];
+#pragma unroll
+		for (int k = 0; k < kGsTile; ++k) {
+			const int64_t j = min(j0 + k, len - 1);
+			c[k] = cp[j * cnt];
+			v[k] = vp[j * cnt];
+		}
+#pragma unroll
+		for (int k = 0; k < kGsTile; ++k)
+			xv[k] = (c[k] >= 0 ? x : xold)[c[k] >= 0 ? (int64_t)c[k] : -(int64_t)c[k] - 1];
+#pragma unroll
+		for (int k = 0; k < kGsTile; ++k)
+			if (j0 + k < len) rsum += v[k] * xv[k];
+	}
 	const double dinv = 1. / diag;
 	x[r] = dinv * (br - rsum);
+}
+
+// Refresh the copy's values from the matrix (entry index per slot, -1 for
+// padding / a missing diagonal).
+__global__ __launch_bounds__(256) void gs_gather_kernel(size_t n, const int64_t * __restrict__ from,
+                                                        const double * __restrict__ vals,
+                                                        double * __restrict__ out) {
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) {
+		const int64_t e = from[i];
+		out[i] = e >= 0 ? vals[e] : 0.0;
+	}
 }
 
 __global__ __launch_bounds__(256) void gs_snapshot_kernel(size_t n, const double * __restrict__ x,
@@ -120,9 +118,16 @@ struct fsx_gs {
 	const fsx_csr * A = nullptr;
 	int ncolours = 1;
 	colour_map cm;
-	// rows sorted by (level, row) and level offsets, forward and backward
-	std::unique_ptr<dbuf> rows_f, rows_b, xold;
-	std::vector<int64_t> lev_f, lev_b;
+	// one sweep direction: level offsets, each level's ELL base, and the
+	// level-ordered copy (row ids, entry counts, diagonal, tagged columns,
+	// values) with the matrix entry each value came from
+	struct dir {
+		std::vector<int64_t> lev, base;
+		std::unique_ptr<dbuf> srow, slen, sdiag, scol, sval, dfrom, vfrom;
+		size_t nslots = 0;
+	} fw, bw;
+	std::unique_ptr<dbuf> xold;
+	uint64_t version = 0; // of the matrix values the copies hold
 	struct graph {
 		hipGraph_t g = nullptr;
 		hipGraphExec_t ge = nullptr;
@@ -166,15 +171,95 @@ void schedule(const std::vector<int64_t> & rp, const std::vector<int64_t> & ci,
 	for (int64_t r = 0; r < n; ++r) order[fill[lev[r]]++] = r;
 }
 
-template<class I>
-std::unique_ptr<dbuf> upload_rows(fsx_ctx * ctx, const std::vector<int64_t> & order) {
-	std::vector<I> h(order.begin(), order.end());
-	auto d = std::make_unique<dbuf>(std::max<size_t>(1, h.size()) * sizeof(I));
+template<class T>
+std::unique_ptr<dbuf> upload(fsx_ctx * ctx, const std::vector<T> & h) {
+	auto d = std::make_unique<dbuf>(std::max<size_t>(1, h.size()) * sizeof(T));
 	if (!h.empty())
-		FSX_HIP(hipMemcpyAsync(d->p, h.data(), h.size() * sizeof(I), hipMemcpyHostToDevice,
+		FSX_HIP(hipMemcpyAsync(d->p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice,
 		                       ctx->stream));
-	FSX_HIP(hipStreamSynchronize(ctx->stream)); // h goes out of scope
+	FSX_HIP(hipStreamSynchronize(ctx->stream)); // h may go out of scope
 	return d;
+}
+
+// The level-ordered copy of one sweep direction.  Entry order per row: the
+// own colour's off-diagonal entries in CSR order, then the other colours'
+// (gs.hh:94-107); the diagonal is the last entry whose column is the row.
+template<class I>
+void build_dir(fsx_ctx * ctx, const std::vector<int64_t> & rp, const std::vector<int64_t> & ci,
+               const colour_map & cm, bool colours, bool fwd, fsx_gs::dir & D) {
+	std::vector<int64_t> order;
+	schedule(rp, ci, cm, colours, fwd, D.lev, order);
+	const int64_t n = (int64_t)order.size();
+	const size_t nl = D.lev.size() - 1;
+	std::vector<I> srow(n);
+	std::vector<int32_t> slen(n);
+	std::vector<int64_t> dfrom(n, -1);
+	D.base.assign(nl, 0);
+	size_t nslots = 0;
+	for (size_t l = 0; l < nl; ++l) {
+		int64_t w = 0;
+		for (int64_t p = D.lev[l]; p < D.lev[l + 1]; ++p) {
+			const int64_t r = order[p];
+			int64_t len = 0;
+			for (int64_t e = rp[r]; e < rp[r + 1]; ++e) len += ci[e] != r;
+			w = std::max(w, len);
+		}
+		D.base[l] = (int64_t)nslots;
+		nslots += (size_t)(w * (D.lev[l + 1] - D.lev[l]));
+	}
+	std::vector<I> scol(nslots, (I)-1);
+	std::vector<int64_t> vfrom(nslots, -1);
+	for (size_t l = 0; l < nl; ++l) {
+		const int64_t cnt = D.lev[l + 1] - D.lev[l];
+		for (int64_t p = D.lev[l]; p < D.lev[l + 1]; ++p) {
+			const int64_t r = order[p], t = p - D.lev[l];
+			const int64_t cr = colours ? cm.of(r) : 0;
+			srow[p] = (I)r;
+			int64_t j = 0;
+			auto put = [&](int64_t e, bool own) {
+				const int64_t c = ci[e];
+				const bool visited = own && (fwd ? c < r : c > r);
+				const int64_t slot = D.base[l] + j * cnt + t;
+				scol[slot] = (I)(visited ? c : -c - 1);
+				vfrom[slot] = e;
+				++j;
+			};
+			for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+				const int64_t c = ci[e];
+				if (colours && cm.of(c) != cr) continue;
+				if (c == r) dfrom[p] = e;
+				else put(e, true);
+			}
+			if (colours)
+				for (int64_t e = rp[r]; e < rp[r + 1]; ++e)
+					if (cm.of(ci[e]) != cr) put(e, false);
+			slen[p] = (int32_t)j;
+		}
+	}
+	D.nslots = nslots;
+	D.srow = upload(ctx, srow);
+	D.slen = upload(ctx, slen);
+	D.scol = upload(ctx, scol);
+	D.dfrom = upload(ctx, dfrom);
+	D.vfrom = upload(ctx, vfrom);
+	D.sdiag = std::make_unique<dbuf>(std::max<int64_t>(1, n) * sizeof(double));
+	D.sval = std::make_unique<dbuf>(std::max<size_t>(1, nslots) * sizeof(double));
+}
+
+void refresh_values(const fsx_gs & G) {
+	hipStream_t s = G.ctx->stream;
+	const double * vals = (const double *)G.A->vals->p;
+	for (const fsx_gs::dir * D : {&G.fw, &G.bw}) {
+		const size_t n = (size_t)G.A->nrows;
+		hipLaunchKernelGGL(gs_gather_kernel, dim3((unsigned)std::max<size_t>(1, (n + 255) / 256)),
+		                   dim3(256), 0, s, n, (const int64_t *)D->dfrom->p, vals,
+		                   (double *)D->sdiag->p);
+		if (D->nslots)
+			hipLaunchKernelGGL(gs_gather_kernel, dim3((unsigned)((D->nslots + 255) / 256)),
+			                   dim3(256), 0, s, D->nslots, (const int64_t *)D->vfrom->p, vals,
+			                   (double *)D->sval->p);
+	}
+	FSX_HIP(hipGetLastError());
 }
 
 template<class I>
@@ -184,32 +269,15 @@ void enqueue_sweep(const fsx_gs & G, bool fwd, const double * b, double * x) {
 	const size_t n = (size_t)A->nrows;
 	hipLaunchKernelGGL(gs_snapshot_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
 	                   n, (const double *)x, (double *)G.xold->p);
-	const std::vector<int64_t> & ptr = fwd ? G.lev_f : G.lev_b;
-	const I * rows = (const I *)(fwd ? G.rows_f : G.rows_b)->p;
-	const I * rp = (const I *)A->rowptr();
-	const I * ci = (const I *)A->colind();
-	const double * va = (const double *)A->vals->p;
+	const fsx_gs::dir & D = fwd ? G.fw : G.bw;
 	const double * xo = (const double *)G.xold->p;
-	for (size_t l = 0; l + 1 < ptr.size(); ++l) {
-		const int64_t cnt = ptr[l + 1] - ptr[l];
+	for (size_t l = 0; l + 1 < D.lev.size(); ++l) {
+		const int64_t cnt = D.lev[l + 1] - D.lev[l];
 		const unsigned g = (unsigned)((cnt + 255) / 256);
-		const I * rl = rows + ptr[l];
-		if (G.ncolours > 1) {
-			if (fwd)
-				hipLaunchKernelGGL((gs_level_kernel<I, true, true>), dim3(g), dim3(256), 0, s, rp,
-				                   ci, va, rl, cnt, b, x, xo, G.cm);
-			else
-				hipLaunchKernelGGL((gs_level_kernel<I, false, true>), dim3(g), dim3(256), 0, s, rp,
-				                   ci, va, rl, cnt, b, x, xo, G.cm);
-		}
-		else {
-			if (fwd)
-				hipLaunchKernelGGL((gs_level_kernel<I, true, false>), dim3(g), dim3(256), 0, s,
-				                   rp, ci, va, rl, cnt, b, x, xo, G.cm);
-			else
-				hipLaunchKernelGGL((gs_level_kernel<I, false, false>), dim3(g), dim3(256), 0, s,
-				                   rp, ci, va, rl, cnt, b, x, xo, G.cm);
-		}
+		hipLaunchKernelGGL((gs_level_kernel<I>), dim3(g), dim3(256), 0, s,
+		                   (const I *)D.srow->p, (const int32_t *)D.slen->p,
+		                   (const double *)D.sdiag->p, (const I *)D.scol->p,
+		                   (const double *)D.sval->p, D.lev[l], cnt, D.base[l], b, x, xo);
 	}
 	FSX_HIP(hipGetLastError());
 }
@@ -276,12 +344,14 @@ int fsx_gs_create(fsx_ctx * ctx, const fsx_csr * A, int ncolours, fsx_gs ** out)
 			G->cm.m = n % ncolours;
 			G->cm.big = G->cm.m * (G->cm.q + 1);
 		}
-		std::vector<int64_t> order;
-		schedule(rp, ci, G->cm, ncolours > 1, true, G->lev_f, order);
-		G->rows_f = isz == 4 ? upload_rows<int32_t>(ctx, order) : upload_rows<int64_t>(ctx, order);
-		schedule(rp, ci, G->cm, ncolours > 1, false, G->lev_b, order);
-		G->rows_b = isz == 4 ? upload_rows<int32_t>(ctx, order) : upload_rows<int64_t>(ctx, order);
+		for (bool fwd : {true, false}) {
+			fsx_gs::dir & D = fwd ? G->fw : G->bw;
+			if (isz == 4) build_dir<int32_t>(ctx, rp, ci, G->cm, ncolours > 1, fwd, D);
+			else build_dir<int64_t>(ctx, rp, ci, G->cm, ncolours > 1, fwd, D);
+		}
 		G->xold = std::make_unique<dbuf>(std::max<int64_t>(1, n) * sizeof(double));
+		refresh_values(*G);
+		G->version = A->vals_version;
 		ctx_retain(ctx);
 		*out = G.release();
 	});
@@ -290,8 +360,8 @@ int fsx_gs_create(fsx_ctx * ctx, const fsx_csr * A, int ncolours, fsx_gs ** out)
 int fsx_gs_levels(const fsx_gs * G, int64_t * forward, int64_t * backward) {
 	return guard([&] {
 		FSX_REQUIRE(G, "null smoother");
-		if (forward) *forward = (int64_t)G->lev_f.size() - 1;
-		if (backward) *backward = (int64_t)G->lev_b.size() - 1;
+		if (forward) *forward = (int64_t)G->fw.lev.size() - 1;
+		if (backward) *backward = (int64_t)G->bw.lev.size() - 1;
 	});
 }
 
@@ -308,6 +378,10 @@ int fsx_gs_relax(fsx_ctx * ctx, fsx_gs * G, int sweep, int nrelax, const fsx_vec
 		FSX_REQUIRE(b->dt == FSX_F64 && x->dt == FSX_F64, "hybrid_gs: FP64 vectors");
 		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "hybrid_gs: b and x must differ");
 		if (nrelax == 0 || G->A->nrows == 0) return;
+		if (G->version != G->A->vals_version) { // values changed in place
+			refresh_values(*G);
+			G->version = G->A->vals_version;
+		}
 		const double * bp = (const double *)b->ptr();
 		double * xp = (double *)x->ptr();
 		if (!gs_graphs()) {
@@ -315,7 +389,7 @@ int fsx_gs_relax(fsx_ctx * ctx, fsx_gs * G, int sweep, int nrelax, const fsx_vec
 			return;
 		}
 		std::vector<const void *> key{bp, xp, (const void *)(intptr_t)sweep,
-		                              (const void *)(intptr_t)nrelax, G->A->vals->p};
+		                              (const void *)(intptr_t)nrelax};
 		auto it = G->graphs.find(key);
 		if (it == G->graphs.end()) {
 			if (G->graphs.size() >= 16) {

@@ -360,6 +360,7 @@ bool ensure_sdia(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
 }
 
 void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A) {
+	++A->vals_version;
 	if (A->idx->dg_ptr && A->dg_vals) sdia_fill(ctx, A);
 	if (!(A->idx->sl_ptr && A->sl_vals)) return;
 	if (A->it == FSX_I32) {
