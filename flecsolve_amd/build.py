@@ -112,6 +112,26 @@ def build_cpp_tests() -> Path:
     return out
 
 
+def build_bdf_checks() -> None:
+    """Time-integrator check (tests/cpp/bdf_check.cpp) over host vectors (no
+    device use) and over device vectors, and the heat example
+    (flecsolve_amd/examples/heat_implicit.cpp)."""
+    cpp = ROOT / "tests" / "cpp"
+    common = ["g++", "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", "-I", str(INC),
+              "-I", str(cpp)]
+    rpath = [f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../flecsolve_amd"]
+    for out, extra, libs in (("bdf_check_host", ["-DBDF_HOST"], ["-lfsx_problems"]),
+                             ("bdf_check", [], ["-lfsx", "-lfsx_problems"])):
+        tmp = cpp / (out + ".tmp")
+        _run(common + extra + [str(cpp / "bdf_check.cpp"), "-o", str(tmp), "-L", str(PKG)]
+             + libs + rpath)
+        os.replace(tmp, cpp / out)
+    tmp = PKG / "heat_implicit.tmp"
+    _run(common + [str(PKG / "examples" / "heat_implicit.cpp"), "-o", str(tmp), "-L", str(PKG),
+                   "-lfsx", "-lfsx_problems", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, PKG / "heat_implicit")
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
     return ROOT / "oracle" / "liboracle.so"
@@ -124,6 +144,7 @@ def build_all(oracle: bool = True) -> None:
     build_fsx()
     build_cpp_tests()
     build_config_check()
+    build_bdf_checks()
 
 
 def main() -> None:

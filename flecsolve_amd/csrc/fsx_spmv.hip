@@ -552,6 +552,32 @@ __global__ void diag_inv_kernel(int64_t nrows, const I * __restrict__ rp,
 	d[r] = found ? v : 0.0;
 }
 
+// a = alpha I + beta f on one CSR pattern: every entry beta * f, the
+// diagonal entry alpha + beta * f (one row per thread).
+template<class I>
+__global__ void shift_scale_kernel(int64_t nrows, const I * __restrict__ rp,
+                                   const I * __restrict__ ci,
+                                   const double * __restrict__ f, double alpha,
+                                   double beta, double * __restrict__ a) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	for (I e = rp[r]; e < rp[r + 1]; ++e) {
+		const double v = beta * f[e];
+		a[e] = ((int64_t)ci[e] == r) ? alpha + v : v;
+	}
+}
+
+// missing[0] = 1 when some row has no stored diagonal entry
+template<class I>
+__global__ void diag_missing_kernel(int64_t nrows, const I * __restrict__ rp,
+                                    const I * __restrict__ ci, int * __restrict__ missing) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	bool found = false;
+	for (I e = rp[r]; e < rp[r + 1]; ++e) found |= ((int64_t)ci[e] == r);
+	if (!found) missing[0] = 1;
+}
+
 template<class S, class D>
 __global__ void cast_kernel(size_t n, const S * __restrict__ s, D * __restrict__ d) {
 	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -868,6 +894,56 @@ int fsx_csr_set_values(fsx_ctx * ctx, fsx_csr * A, const void * vals) {
 			                       hipMemcpyHostToDevice, ctx->stream));
 		refresh_sliced_values(ctx, A);
 		FSX_HIP(hipStreamSynchronize(ctx->stream)); // the host buffer may go away
+	});
+}
+
+int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double beta,
+                        fsx_csr * A) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && F && A, "null argument");
+		FSX_REQUIRE(A != F, "shift_scale: A must not be F");
+		FSX_REQUIRE(A->idx == F->idx,
+		            "shift_scale: A must share F's pattern (fsx_csr_convert(F, FSX_F64))");
+		FSX_REQUIRE(A->vt == FSX_F64 && F->vt == FSX_F64, "shift_scale: FP64 matrices");
+		FSX_REQUIRE(F->nrows == F->ncols, "shift_scale: square matrix");
+		const int64_t n = F->nrows;
+		if (n == 0) return;
+		const unsigned g = (unsigned)((n + 255) / 256);
+		csr_index & ix = *F->idx;
+		if (ix.diag_full < 0) {
+			dbuf flag(sizeof(int));
+			FSX_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+			if (F->it == FSX_I32)
+				hipLaunchKernelGGL((diag_missing_kernel<int32_t>), dim3(g), dim3(256), 0,
+				                   ctx->stream, n, (const int32_t *)F->rowptr(),
+				                   (const int32_t *)F->colind(), (int *)flag.p);
+			else
+				hipLaunchKernelGGL((diag_missing_kernel<int64_t>), dim3(g), dim3(256), 0,
+				                   ctx->stream, n, (const int64_t *)F->rowptr(),
+				                   (const int64_t *)F->colind(), (int *)flag.p);
+			FSX_HIP(hipGetLastError());
+			int missing = 0;
+			FSX_HIP(hipMemcpyAsync(&missing, flag.p, sizeof(int), hipMemcpyDeviceToHost,
+			                       ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			ctx->host_syncs++;
+			ix.diag_full = missing ? 0 : 1;
+		}
+		FSX_REQUIRE(ix.diag_full == 1 || alpha == 0.0,
+		            "shift_scale: a row of F has no stored diagonal entry");
+		if (F->it == FSX_I32)
+			hipLaunchKernelGGL((shift_scale_kernel<int32_t>), dim3(g), dim3(256), 0,
+			                   ctx->stream, n, (const int32_t *)F->rowptr(),
+			                   (const int32_t *)F->colind(), (const double *)F->vals->p,
+			                   alpha, beta, (double *)A->vals->p);
+		else
+			hipLaunchKernelGGL((shift_scale_kernel<int64_t>), dim3(g), dim3(256), 0,
+			                   ctx->stream, n, (const int64_t *)F->rowptr(),
+			                   (const int64_t *)F->colind(), (const double *)F->vals->p,
+			                   alpha, beta, (double *)A->vals->p);
+		FSX_HIP(hipGetLastError());
+		ctx->launches++;
+		refresh_sliced_values(ctx, A);
 	});
 }
 

@@ -1,0 +1,195 @@
+// Implicit 2D heat equation: the reference's heat example
+// (examples/heat_equation/implicit.cc, heat.hh, implicit.cfg) on one GPU.
+//
+//   u_t = alpha Laplacian(u) on (0,10)^2, u = 0 on the boundary,
+//   u(0) = 50 on [4,6]^2 and 0 elsewhere (examples/heat_equation/README.md),
+//
+// on an n x n grid of interior vertices (spacing 10/(n+1); the reference's
+// mesh extents count the boundary vertices too, so its "N N" run is n = N-2
+// here).  The time integrator and the stage solver come from the config file
+// exactly as in the reference: read_config(file,
+// bdf::options("time-integrator"), krylov_factory::options("linear-solver")).
+//
+// Two ways to run the stage solves:
+//   generic (default): operator_adapter over the device CSR of alpha
+//     Laplacian -- each product an SpMV plus an axpy -- and the solver
+//     krylov_factory builds, the reference's structure with one host round
+//     trip per reduction;
+//   --fused: csr_adapter, which keeps I - gamma F as a device CSR (rewritten
+//     by one kernel when gamma changes), solved by the fused graph-captured
+//     solver of the configured type (cg, gmres or bicgstab).
+//
+// usage: heat_implicit <cfg> <n> [--alpha A] [--fused] [--out file] [--quiet]
+// Prints one line per accepted step (unless --quiet) and a summary line
+//   "heat n N steps S rejects R solves K iters I time T wall W"
+// with W the seconds spent in the time loop.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "flecsolve/matrices/device_csr.hh"
+#include "flecsolve/solvers/factory.hh"
+#include "flecsolve/solvers/fused.hh"
+#include "flecsolve/time-integrators/bdf.hh"
+#include "flecsolve/time-integrators/operator_adapter.hh"
+#include "flecsolve/vectors/device.hh"
+#include "fsx_problems.h"
+
+using namespace flecsolve;
+namespace ti = flecsolve::time_integrator;
+using dvec = vec::device<double>;
+using dcore = dvec::base; // the type operators are applied to
+
+namespace {
+
+struct heat_problem {
+	std::vector<int32_t> rp, ci;
+	std::vector<double> f, u0;
+};
+
+// f(u) = alpha Laplacian(u) as a CSR on the interior vertices
+heat_problem make_heat(int n, double alpha) {
+	heat_problem p;
+	p.rp.resize((std::size_t)n * n + 1);
+	p.ci.resize(fsx_gen_poisson2d_nnz(n));
+	p.f.resize(p.ci.size());
+	fsx_gen_poisson2d(n, p.rp.data(), p.ci.data(), p.f.data());
+	const double h = 10.0 / (n + 1);
+	const double c = alpha / (h * h);
+	for (auto & e : p.f) e = -c * e;
+	p.u0.resize((std::size_t)n * n);
+	for (int j = 0; j < n; ++j)
+		for (int i = 0; i < n; ++i) {
+			const double x = (i + 1) * h, y = (j + 1) * h;
+			p.u0[(std::size_t)j * n + i] = (x >= 4 && x <= 6 && y >= 4 && y <= 6) ? 50.0 : 0.0;
+		}
+	return p;
+}
+
+// stage solver with its solve and iteration counts
+struct counted_solver : op::base<> {
+	template<class B, class X>
+	solve_info apply(const B & b, X & x) const {
+		auto info = fn(b, x);
+		iters += info.iters;
+		++solves;
+		return info;
+	}
+	std::function<solve_info(const dcore &, dcore &)> fn;
+	mutable long iters = 0, solves = 0;
+};
+
+template<class F, class S>
+int run(const ti::bdf::settings & ts, F op, S solver, dvec & u, dvec & unew, int n,
+        bool quiet, const char * out, fsx_ctx * ctx) {
+	ti::bdf::integrator t(ti::bdf::parameters(ts, op, ti::bdf::make_work(u), solver));
+	// the loop of examples/heat_equation/implicit.cc:41-55
+	const auto t0 = std::chrono::steady_clock::now();
+	auto dt = t.get_current_dt();
+	bool first_step = true;
+	while (t.get_current_time() < t.get_final_time()) {
+		t.advance(dt, first_step, u, unew);
+		const bool good = t.check_solution();
+		if (good) {
+			if (!quiet)
+				std::printf("Step %d advanced %gs to time %gs\n", t.get_current_step(), dt,
+				            t.get_current_time());
+			t.update();
+			std::swap(u, unew);
+			first_step = false;
+		}
+		dt = t.get_next_dt(good);
+	}
+	fsx_check(fsx_sync(ctx), "sync");
+	const double wall =
+		std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	if (out) {
+		std::vector<double> h(u.data.size());
+		fsx_check(fsx_vec_download(ctx, u.handle(), h.data(), h.size()), "download");
+		std::ofstream(out, std::ios::binary)
+			.write(reinterpret_cast<const char *>(h.data()), (std::streamsize)(h.size() * 8));
+	}
+	std::printf("heat n %d steps %d rejects %d solves %ld iters %ld time %.17g wall %.6f\n", n,
+	            t.get_current_step(), t.num_step_rejects(), solver.get().solves,
+	            solver.get().iters, t.get_current_time(), wall);
+	return 0;
+}
+
+}
+
+int main(int argc, char ** argv) {
+	if (argc < 3) {
+		std::fprintf(stderr,
+		             "usage: heat_implicit <cfg> <n> [--alpha A] [--fused] [--out file] [--quiet]\n");
+		return 2;
+	}
+	const char * cfg = argv[1];
+	const int n = std::atoi(argv[2]);
+	double alpha = 1.0;
+	bool fused = false, quiet = false;
+	const char * out = nullptr;
+	for (int i = 3; i < argc; ++i) {
+		if (!std::strcmp(argv[i], "--alpha") && i + 1 < argc) alpha = std::atof(argv[++i]);
+		else if (!std::strcmp(argv[i], "--fused")) fused = true;
+		else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
+		else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out = argv[++i];
+		else {
+			std::fprintf(stderr, "unknown argument %s\n", argv[i]);
+			return 2;
+		}
+	}
+	try {
+		if (n < 1) throw std::invalid_argument("n must be positive");
+		auto [ts, ss] = read_config(cfg, ti::bdf::options("time-integrator"),
+		                            krylov_factory::options("linear-solver"));
+		const auto p = make_heat(n, alpha);
+		fsx_ctx * ctx;
+		fsx_check(fsx_ctx_create(0, &ctx), "ctx");
+		fsx_csr * Fh;
+		fsx_check(fsx_csr_create(ctx, (int64_t)n * n, (int64_t)n * n, (int64_t)p.ci.size(),
+		                         FSX_I32, p.rp.data(), p.ci.data(), FSX_F64, p.f.data(), &Fh),
+		          "csr");
+		const auto Fm = mat::device_csr::adopt(ctx, Fh);
+		dvec u(ctx, p.u0.size()), unew(ctx, p.u0.size());
+		fsx_check(fsx_vec_upload(ctx, u.handle(), p.u0.data(), p.u0.size()), "upload");
+		auto solver = op::make_shared<counted_solver>();
+		int rc;
+		if (fused) {
+			auto F = op::make_shared<ti::csr_adapter>(Fm);
+			const auto A = F.get().matrix();
+			switch (ss.target_id.value()) {
+			case krylov_target::gmres: {
+				gmres::fused g(std::get<gmres::settings>(ss.target_settings), A);
+				solver.get().fn = [g](const dcore & b, dcore & x) { return g.apply(b, x); };
+				break;
+			}
+			case krylov_target::bicgstab: {
+				bicgstab::fused g(std::get<bicgstab::settings>(ss.target_settings), A);
+				solver.get().fn = [g](const dcore & b, dcore & x) { return g.apply(b, x); };
+				break;
+			}
+			default: {
+				cg::fused g(std::get<cg::settings>(ss.target_settings), A);
+				solver.get().fn = [g](const dcore & b, dcore & x) { return g.apply(b, x); };
+			}
+			}
+			rc = run(ts, F, solver, u, unew, n, quiet, out, ctx);
+		}
+		else {
+			auto F = op::make_shared<ti::operator_adapter<mat::device_csr>>(Fm);
+			auto inner = krylov_factory::make_shared(ss, u, F);
+			solver.get().fn = [inner](const dcore & b, dcore & x) { return inner.get().apply(b, x); };
+			rc = run(ts, F, solver, u, unew, n, quiet, out, ctx);
+		}
+		return rc;
+	}
+	catch (const std::exception & e) {
+		std::fprintf(stderr, "error: %s\n", e.what());
+		return 1;
+	}
+}

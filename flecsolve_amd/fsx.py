@@ -176,6 +176,7 @@ _SIGS = {
     "fsx_csr_transpose": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "fsx_csr_set_values": (C.c_int, [_P, _P, _P]),
     "fsx_csr_set_values_f32_from_f64": (C.c_int, [_P, _P, _P]),
+    "fsx_csr_shift_scale": (C.c_int, [_P, _P, C.c_double, C.c_double, _P]),
     "fsx_reduce_async": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "fsx_cg_update": (C.c_int, [_P, C.c_double, _P, _P, _P, _P, _P, _P,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -384,6 +385,11 @@ class CSR:
         """FP32 copy takes A64's current values (same structure)."""
         check(lib().fsx_csr_set_values_f32_from_f64(self.ctx.h, self.h, A64.h),
               "fsx_csr_set_values_f32_from_f64")
+
+    def shift_scale(self, F: "CSR", alpha: float, beta: float):
+        """self = alpha I + beta F on F's pattern (self = F.convert(F64))."""
+        check(lib().fsx_csr_shift_scale(self.ctx.h, F.h, alpha, beta, self.h),
+              "fsx_csr_shift_scale")
 
     def transpose(self) -> "CSR":
         h = C.c_void_p()

@@ -6,9 +6,10 @@
 // `<label>.options` (the keys of cg::options, gmres::options,
 // bicgstab::options); read_config(file, krylov_factory::options(label))
 // returns krylov_factory::settings, and krylov_factory::make(settings,
-// exemplar, A[, P[, diagnostic]]) builds that solver with device work
-// vectors shaped like the exemplar and returns an operator whose
-// apply(b, x) runs it and returns its solve_info.  The reference's fourth
+// exemplar, A[, P[, diagnostic]]) builds that solver with work vectors
+// shaped like the exemplar and returns an operator whose apply(b, x) runs it
+// and returns its solve_info; make_shared(...) returns it behind an
+// op::handle, as the time integrators take their solver.  The reference's fourth
 // target, nka (a nonlinear Krylov accelerator), is not part of the linear
 // solve path built here.
 #ifndef FLECSOLVE_AMD_SOLVERS_FACTORY_HH
@@ -22,8 +23,9 @@
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
 #include "flecsolve/solvers/gmres.hh"
+#include "flecsolve/operators/handle.hh"
 #include "flecsolve/util/config.hh"
-#include "flecsolve/vectors/device.hh"
+#include "flecsolve/vectors/util.hh"
 
 namespace flecsolve {
 
@@ -48,7 +50,7 @@ struct krylov_registry<krylov_target::cg> {
 	using options = cg::options;
 	template<class V, class... Args>
 	static auto make(const settings & s, const V & v, Args &&... args) {
-		return cg::solver(s, vec::make_device_work<cg::nwork>(v))(std::forward<Args>(args)...);
+		return cg::solver(s, vec::make_work<cg::nwork>(v))(std::forward<Args>(args)...);
 	}
 };
 
@@ -58,7 +60,7 @@ struct krylov_registry<krylov_target::gmres> {
 	using options = gmres::options;
 	template<class V, class... Args>
 	static auto make(const settings & s, const V & v, Args &&... args) {
-		return gmres::solver(s, vec::make_device_work<gmres::nwork>(v))(
+		return gmres::solver(s, vec::make_work<gmres::nwork>(v))(
 			std::forward<Args>(args)...);
 	}
 };
@@ -69,7 +71,7 @@ struct krylov_registry<krylov_target::bicgstab> {
 	using options = bicgstab::options;
 	template<class V, class... Args>
 	static auto make(const settings & s, const V & v, Args &&... args) {
-		return bicgstab::solver(s, vec::make_device_work<bicgstab::nwork>(v))(
+		return bicgstab::solver(s, vec::make_work<bicgstab::nwork>(v))(
 			std::forward<Args>(args)...);
 	}
 };
@@ -77,7 +79,8 @@ struct krylov_registry<krylov_target::bicgstab> {
 // The made operator: apply visits the solver chosen at run time
 // (operators/factory.hh:28-53).
 template<class Variant>
-struct factory_prod {
+struct factory_prod : op::base<> {
+	explicit factory_prod(Variant v) : var(std::move(v)) {}
 	template<class D, class R>
 	decltype(auto) apply(const D & x, R & y) const {
 		return std::visit([&](const auto & p) { return p.apply(x, y); }, var);
@@ -138,6 +141,14 @@ struct krylov_factory {
 		}
 		target parsed = target::cg;
 	};
+
+	// the made operator behind a shared handle (operators/factory.hh:148-153)
+	template<class V, class... Args>
+	static auto make_shared(const settings & s, const V & exemplar, Args &&... args) {
+		auto p = make(s, exemplar, std::forward<Args>(args)...);
+		using prod = decltype(p);
+		return op::handle<op::core<prod>>{std::make_shared<op::core<prod>>(std::move(p))};
+	}
 
 	// operators/factory.hh:141-177
 	template<class V, class... Args>

@@ -1,0 +1,99 @@
+// flecsolve/solvers/fused.hh -- CG, BiCGSTAB and GMRES on a device CSR as
+// fused, graph-captured device solves.
+//
+// The generic solvers of cg.hh, bicgstab.hh and gmres.hh run the reference's
+// iteration over any vec::core vector with one host round trip per
+// reduction, as the reference does.  When the operator is a device CSR (and
+// the preconditioner the identity or its Jacobi diagonal), the same
+// iterations -- same operation order, tolerance tests and solve_info, no user
+// diagnostic -- run as fused kernels with device-resident scalars captured in
+// HIP graphs (fsx_cg_solve, fsx_bicgstab_solve, fsx_gmres_solve).  These
+// policies give that path the operator interface of a bound solver:
+//
+//   auto S = op::make_shared<cg::fused>(settings, A);        // or A, dinv
+//   solve_info info = S.get().apply(b, x);
+//
+// The matrix is held by handle: values changed in place (fsx_csr_set_values,
+// fsx_csr_shift_scale) are seen by the next solve.
+#ifndef FLECSOLVE_AMD_SOLVERS_FUSED_HH
+#define FLECSOLVE_AMD_SOLVERS_FUSED_HH
+
+#include <optional>
+#include <type_traits>
+#include <utility>
+
+#include "fsx.h"
+#include "flecsolve/matrices/device_csr.hh"
+#include "flecsolve/solvers/bicgstab.hh"
+#include "flecsolve/solvers/cg.hh"
+#include "flecsolve/solvers/gmres.hh"
+
+namespace flecsolve::detail {
+
+inline solve_info from_fsx(const fsx_info & fi) {
+	solve_info s;
+	s.status = static_cast<solve_info::stop_reason>(fi.status);
+	s.iters = fi.iters;
+	s.restarts = fi.restarts;
+	s.res_norm_initial = fi.res_norm_initial;
+	s.res_norm_final = fi.res_norm_final;
+	s.sol_norm_initial = fi.sol_norm_initial;
+	s.sol_norm_final = fi.sol_norm_final;
+	s.rhs_norm = fi.rhs_norm;
+	return s;
+}
+
+inline fsx_settings to_fsx(const solver_settings & s) {
+	fsx_settings f{};
+	f.maxiter = s.maxiter;
+	f.rtol = s.rtol;
+	f.atol = s.atol;
+	f.use_zero_guess = s.use_zero_guess;
+	f.max_krylov_dim = -1;
+	f.pre_side = 1;
+	f.restart = 0;
+	return f;
+}
+
+using fused_fn = int (*)(fsx_ctx *, const fsx_csr *, const fsx_vec *, const fsx_vec *,
+                         fsx_vec *, const fsx_settings *, fsx_info *);
+
+template<class Settings, fused_fn Fn>
+struct fused_solver : op::base<> {
+	fused_solver(const Settings & s, mat::device_csr A) : settings(s), A(std::move(A)) {}
+	fused_solver(const Settings & s, mat::device_csr A, const mat::device_dinv & D)
+		: settings(s), A(std::move(A)), dinv(D.dinv) {}
+
+	template<class B, class X>
+	solve_info apply(const B & b, X & x) const {
+		fsx_settings f = to_fsx(settings);
+		if constexpr (std::is_same_v<Settings, gmres::settings>) {
+			f.max_krylov_dim = settings.max_krylov_dim;
+			f.pre_side = settings.pre_side == gmres::precond_side::left ? 0 : 1;
+			f.restart = settings.restart;
+		}
+		fsx_info fi{};
+		fsx_check(Fn(A.context(), A.handle(), dinv ? dinv->handle() : nullptr, b.data.get(),
+		             x.data.get(), &f, &fi),
+		          "fused solve");
+		return from_fsx(fi);
+	}
+
+	Settings settings;
+	mat::device_csr A;
+	std::optional<vec::device<double>> dinv;
+};
+
+}
+
+namespace flecsolve::cg {
+using fused = detail::fused_solver<settings, fsx_cg_solve>;
+}
+namespace flecsolve::bicgstab {
+using fused = detail::fused_solver<settings, fsx_bicgstab_solve>;
+}
+namespace flecsolve::gmres {
+using fused = detail::fused_solver<settings, fsx_gmres_solve>;
+}
+
+#endif

@@ -9,8 +9,10 @@
 //     "section.key"; `#` starts a comment; blank lines are skipped;
 //   * options are typed: integers and floats must parse completely, bools
 //     accept true/false, yes/no, on/off, 1/0 (any case), enums read with
-//     their operator>>; a key given twice is an error;
-//   * required options must be present, others take their default;
+//     their operator>>, std::vector<double> takes blank-separated numbers
+//     (a multitoken option); a key given twice is an error;
+//   * required options must be present, others take their default or, when
+//     declared without one, keep the value they had;
 //   * options may depend on earlier values (a factory adds the chosen
 //     solver's options once its `type` is read), so the file is applied
 //     repeatedly until the set of unrecognised keys stops changing (at most
@@ -48,6 +50,16 @@ template<class T>
 T parse_value(const std::string & key, const std::string & text) {
 	if constexpr (std::is_same_v<T, std::string>)
 		return text;
+	else if constexpr (std::is_same_v<T, std::vector<double>>) {
+		// a multi-token value: blank-separated numbers
+		std::istringstream is(text);
+		std::vector<double> v;
+		std::string tok;
+		while (is >> tok) v.push_back(parse_value<double>(key, tok));
+		if (v.empty())
+			throw std::invalid_argument("option '" + key + "': empty list");
+		return v;
+	}
 	else if constexpr (std::is_same_v<T, bool>) {
 		std::string t(text);
 		std::transform(t.begin(), t.end(), t.begin(),
@@ -96,6 +108,15 @@ public:
 		opts_.push_back({name,
 		                 [=](const std::string & v) { *dst = config::parse_value<T>(name, v); },
 		                 [=] { *dst = (T)def; }});
+		return *this;
+	}
+	// no default: left as it is when the file does not give it (an option
+	// declared without default_value in program_options)
+	template<class T>
+	optdesc & optional(const std::string & name, T * dst) {
+		opts_.push_back({name,
+		                 [=](const std::string & v) { *dst = config::parse_value<T>(name, v); },
+		                 [] {}});
 		return *this;
 	}
 	optdesc & add(const optdesc & o) {

@@ -86,6 +86,8 @@ struct device_data {
 
 	fsx_ctx * ctx() const { return s->ctx; }
 	std::size_t size() const { return s->n; }
+	// a new (lazily allocated) buffer of the same length on the same context
+	device_data make_like() const { return device_data(s->ctx, s->n); }
 	fsx_vec * get() const {
 		if (!s->v)
 			fsx_check(fsx_vec_create(s->ctx, s->n, Config::dtype, &s->v),

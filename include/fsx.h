@@ -178,6 +178,15 @@ int fsx_csr_set_values(fsx_ctx * ctx, fsx_csr * A, const void * vals);
 /* A32 (= fsx_csr_convert(A64, FSX_F32), shared structure) takes A64's
  * current values rounded to FP32, on the device. */
 int fsx_csr_set_values_f32_from_f64(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr * A64);
+/* A = alpha I + beta F on F's pattern (A = fsx_csr_convert(F, FSX_F64), or
+ * any matrix sharing F's index arrays; FP64, square, every row storing its
+ * diagonal entry unless alpha == 0): each entry beta f_ij, each diagonal
+ * entry alpha + beta f_ii, computed on the device; A's SpMV layouts are
+ * refreshed in place.  Forms the implicit stage matrix I - gamma F of
+ * time_integrator::operator_adapter (time-integrators/operator_adapter.hh:
+ * 29-35, y = x - gamma f(x)) as one matrix. */
+int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double beta,
+                        fsx_csr * A);
 int fsx_csr_download(fsx_ctx * ctx, const fsx_csr * A, void * rowptr,
                      void * colind, void * vals);
 /* y = A x   (sparse::apply -> mult -> Ops::spmv, matrices/seq.hh:46-60) */

@@ -1,0 +1,748 @@
+"""CPU restatement of the reference's BDF / Crank-Nicolson time integrator.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ and nowhere else.
+
+Follows flecsolve/time-integrators/bdf.hh (integrator, history),
+bdf_parameters.hh (settings, validate, option defaults), bdf.cc
+(memory_size, order), base.hh (time, step and dt bookkeeping) and
+operator_adapter.hh (y = x - gamma f(x)), step for step, on numpy float64
+vectors.  Every vector update is one elementwise numpy expression with the
+reference's operand order (a*x + y is two roundings, as the device kernels
+built without contraction compute it), so on inputs whose norms are exact
+(inf norm, constant vectors) the device integrator matches bit for bit.
+
+Where the reference reads a member it never initialised, this restatement
+takes: current_steprejects 0, prev_successive_rejects False and
+time_trunc_err_est 1.0 (the value the reference assigns when it has no
+estimate, bdf.hh:799-805).  With these the reference's own known-answer test
+(time-integrators/test/implicit.cc:121-134: BDF5 takes 48 steps with 14
+rejected) is reproduced exactly.
+
+Pins: that known-answer test (tests/test_oracle_bdf.py), polynomial
+exactness of the variable-step BDF2-5 coefficients, the BDF6 coefficient
+a6 as the reference computes it (its denominator uses h1 - h5 where the
+textbook formula has h1 - h6, bdf.hh:433), first-, second- and k-th order
+convergence of BE, CN and BDFk at constant steps, and the reduction of the
+heat equation on a Laplacian eigenvector to the scalar rate problem.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+EPS = np.finfo(np.float64).eps
+
+# method / predictor / strategy / controller / error_scaling
+# (bdf_parameters.hh:29-39), in the enum order of the reference
+CN, BE, BDF2, BDF3, BDF4, BDF5, BDF6 = range(7)
+AB2, LEAPFROG = range(2)
+TRUNCATION_ERROR, CONSTANT, FINAL_CONSTANT, LIMIT_RELATIVE_CHANGE = range(4)
+H211B, PC4_7, PC11, DEADBEAT = range(4)
+FIXED_RESOLUTION, FIXED_SCALING = range(2)
+NORM_INF, NORM_L2, NORM_L1 = range(3)
+
+METHODS = {"BE": BE, "CN": CN, "BDF2": BDF2, "BDF3": BDF3, "BDF4": BDF4,
+           "BDF5": BDF5, "BDF6": BDF6}
+PREDICTORS = {"ab2": AB2, "leapfrog": LEAPFROG}
+STRATEGIES = {"truncation-error": TRUNCATION_ERROR, "constant": CONSTANT,
+              "final-constant": FINAL_CONSTANT,
+              "limit-relative-change": LIMIT_RELATIVE_CHANGE}
+CONTROLLERS = {"PC.4.7": PC4_7, "H211b": H211B, "PC11": PC11, "Deadbeat": DEADBEAT}
+SCALINGS = {"fixed-scaling": FIXED_SCALING, "fixed-resolution": FIXED_RESOLUTION}
+NORMS = {"inf": NORM_INF, "l2": NORM_L2, "l1": NORM_L1}
+
+
+def memory_size(m: int) -> int:
+    """bdf.cc:5-8: CN (0) -> 1, otherwise the enum value."""
+    return 1 if m == 0 else m
+
+
+def order(m: int) -> int:
+    """bdf.cc:10-22"""
+    return {CN: 2, BE: 1}.get(m, m)
+
+
+@dataclass
+class Settings:
+    """bdf::settings with the option defaults of bdf_parameters.hh:117-145
+    (base_settings: parameters.hh:30-57)."""
+    initial_time: float = 0.0
+    final_time: float = 1.0
+    max_steps: int = 1000
+    max_dt: float = np.finfo(np.float64).max
+    min_dt: float = np.finfo(np.float64).tiny
+    initial_dt: float = 0.0
+    use_predictor: bool = True
+    use_initial_predictor: bool = True
+    has_source_term: bool = False
+    combine_timestep_estimators: bool = False
+    predictor: int = LEAPFROG
+    timestep_strategy: int = TRUNCATION_ERROR
+    dt_cut_lower_bound: float = 0.58754407
+    dt_growth_upper_bound: float = 1.702
+    number_of_time_intervals: int = 100
+    integrator: int = BDF2
+    starting_integrator: int = CN
+    calculate_time_trunc_error: bool = False
+    time_trunc_err_norm: int = NORM_INF
+    target_relative_change: float = 0.0
+    use_pi_controller: bool = True
+    pi_controller_type: int = PC4_7
+    control_timestep_variation: bool = False
+    time_error_scaling: int = FIXED_SCALING
+    time_rtol: float = 1e-9
+    time_atol: float = 1e-15
+    problem_scales: list = field(default_factory=list)
+
+    def validate(self):
+        """bdf_parameters.hh:62-107"""
+        assert memory_size(self.starting_integrator) == 1, \
+            "Starting integrator must be CN or BE"
+        if self.use_predictor:
+            self.calculate_time_trunc_error = True
+        if self.timestep_strategy == TRUNCATION_ERROR:
+            self.calculate_time_trunc_error = True
+            self.use_predictor = True
+        else:
+            self.combine_timestep_estimators = False
+            self.control_timestep_variation = False
+            self.use_pi_controller = False
+        if self.calculate_time_trunc_error:
+            self.use_predictor = True
+            if self.time_error_scaling == FIXED_SCALING:
+                assert len(self.problem_scales) > 0
+        if self.use_predictor:
+            if memory_size(self.integrator) > 1:
+                assert self.predictor == LEAPFROG
+            elif self.integrator == CN:
+                assert self.predictor == AB2
+
+
+_KEYS = {  # config key -> (field, parser)
+    "initial-time": ("initial_time", float), "final-time": ("final_time", float),
+    "max-steps": ("max_steps", int), "max-dt": ("max_dt", float),
+    "min-dt": ("min_dt", float), "initial-dt": ("initial_dt", float),
+    "use-predictor": ("use_predictor", None),
+    "has-source-term": ("has_source_term", None),
+    "use-initial-predictor": ("use_initial_predictor", None),
+    "predictor": ("predictor", PREDICTORS),
+    "timestep-selection-strategy": ("timestep_strategy", STRATEGIES),
+    "dt-cut-lower-bound": ("dt_cut_lower_bound", float),
+    "dt-growth-upper-bound": ("dt_growth_upper_bound", float),
+    "number-of-time-intervals": ("number_of_time_intervals", int),
+    "integrator": ("integrator", METHODS),
+    "starting-integrator": ("starting_integrator", METHODS),
+    "calculate-time-trunc-error": ("calculate_time_trunc_error", None),
+    "time-trunc-error-norm": ("time_trunc_err_norm", NORMS),
+    "target-relative-change": ("target_relative_change", float),
+    "use-pi-controller": ("use_pi_controller", None),
+    "pi-controller-type": ("pi_controller_type", CONTROLLERS),
+    "control-timestep-variation": ("control_timestep_variation", None),
+    "time-error-scaling": ("time_error_scaling", SCALINGS),
+    "trunc-error-rtol": ("time_rtol", float), "trunc-error-atol": ("time_atol", float),
+    "combine-timestep-estimators": ("combine_timestep_estimators", None),
+    "problem-scales": ("problem_scales", "list"),
+}
+
+
+def settings_from_section(kv: dict) -> Settings:
+    """Settings from one config section's key -> text map."""
+    s = Settings()
+    for k, text in kv.items():
+        name, parse = _KEYS[k]
+        if parse is None:
+            val = text.strip().lower() in ("true", "1", "yes", "on")
+        elif parse == "list":
+            val = [float(t) for t in text.split()]
+        elif isinstance(parse, dict):
+            tok = text.strip()
+            val = parse[tok.lower()] if tok.lower() in parse else parse[tok]
+        else:
+            val = parse(text)
+        setattr(s, name, val)
+    return s
+
+
+def read_sections(path) -> dict:
+    """section -> {key: text} of an INI-style config file."""
+    out, sec = {}, ""
+    for line in open(path):
+        line = line.split("#", 1)[0].strip()
+        if not line:
+            continue
+        if line.startswith("["):
+            sec = line[1:-1].strip()
+            out.setdefault(sec, {})
+            continue
+        k, v = line.split("=", 1)
+        out.setdefault(sec, {})[k.strip()] = v.strip()
+    return out
+
+
+def coefficients(m: int, h: list) -> tuple:
+    """compute_source_term's (a1..ak) and gamma = a0 for BDF2-6
+    (bdf.hh:290-437); h[i] = t_{n+1} - t_{n-i}."""
+    if m == BDF2:
+        h1, h2 = h[0], h[1]
+        alpha = h1 + h2
+        a0 = (h1 * h2) / alpha
+        a1 = -(h2 * h2) / (alpha * (h2 - h1))
+        a2 = -(h1 * h1) / (alpha * (h1 - h2))
+        return [a1, a2], a0, alpha
+    if m == BDF3:
+        h1, h2, h3 = h[0], h[1], h[2]
+        n1, n2, n3 = h2 * h3, h3 * h1, h1 * h2
+        alpha = n1 + n2 + n3
+        a0 = (h1 * h2 * h3) / alpha
+        a1 = -(n1 * n1) / (alpha * (h2 - h1) * (h3 - h1))
+        a2 = -(n2 * n2) / (alpha * (h1 - h2) * (h3 - h2))
+        a3 = -(n3 * n3) / (alpha * (h1 - h3) * (h2 - h3))
+        return [a1, a2, a3], a0, alpha
+    if m == BDF4:
+        h1, h2, h3, h4 = h[0], h[1], h[2], h[3]
+        n1, n2, n3, n4 = h2 * h3 * h4, h3 * h4 * h1, h4 * h1 * h2, h1 * h2 * h3
+        alpha = n1 + n2 + n3 + n4
+        a0 = (h1 * h2 * h3 * h4) / alpha
+        a1 = -(n1 * n1) / (alpha * (h2 - h1) * (h3 - h1) * (h4 - h1))
+        a2 = -(n2 * n2) / (alpha * (h1 - h2) * (h3 - h2) * (h4 - h2))
+        a3 = -(n3 * n3) / (alpha * (h1 - h3) * (h2 - h3) * (h4 - h3))
+        a4 = -(n4 * n4) / (alpha * (h1 - h4) * (h2 - h4) * (h3 - h4))
+        return [a1, a2, a3, a4], a0, alpha
+    if m == BDF5:
+        h1, h2, h3, h4, h5 = h[0], h[1], h[2], h[3], h[4]
+        n1 = h2 * h3 * h4 * h5
+        n2 = h3 * h4 * h5 * h1
+        n3 = h4 * h5 * h1 * h2
+        n4 = h5 * h1 * h2 * h3
+        n5 = h1 * h2 * h3 * h4
+        alpha = n1 + n2 + n3 + n4 + n5
+        a0 = (h1 * h2 * h3 * h4 * h5) / alpha
+        a1 = -(n1 * n1) / (alpha * (h2 - h1) * (h3 - h1) * (h4 - h1) * (h5 - h1))
+        a2 = -(n2 * n2) / (alpha * (h1 - h2) * (h3 - h2) * (h4 - h2) * (h5 - h2))
+        a3 = -(n3 * n3) / (alpha * (h1 - h3) * (h2 - h3) * (h4 - h3) * (h5 - h3))
+        a4 = -(n4 * n4) / (alpha * (h1 - h4) * (h2 - h4) * (h3 - h4) * (h5 - h4))
+        a5 = -(n5 * n5) / (alpha * (h1 - h5) * (h2 - h5) * (h3 - h5) * (h4 - h5))
+        return [a1, a2, a3, a4, a5], a0, alpha
+    if m == BDF6:
+        h1, h2, h3, h4, h5, h6 = h[0], h[1], h[2], h[3], h[4], h[5]
+        n1 = h2 * h3 * h4 * h5 * h6
+        n2 = h3 * h4 * h5 * h6 * h1
+        n3 = h4 * h5 * h6 * h1 * h2
+        n4 = h5 * h6 * h1 * h2 * h3
+        n5 = h6 * h1 * h2 * h3 * h4
+        n6 = h1 * h2 * h3 * h4 * h5
+        alpha = n1 + n2 + n3 + n4 + n5 + n6
+        a0 = (h1 * h2 * h3 * h4 * h5 * h6) / alpha
+        a1 = -(n1 * n1) / (alpha * (h2 - h1) * (h3 - h1) * (h4 - h1) * (h5 - h1) * (h6 - h1))
+        a2 = -(n2 * n2) / (alpha * (h1 - h2) * (h3 - h2) * (h4 - h2) * (h5 - h2) * (h6 - h2))
+        a3 = -(n3 * n3) / (alpha * (h1 - h3) * (h2 - h3) * (h4 - h3) * (h5 - h3) * (h6 - h3))
+        a4 = -(n4 * n4) / (alpha * (h1 - h4) * (h2 - h4) * (h3 - h4) * (h5 - h4) * (h6 - h4))
+        a5 = -(n5 * n5) / (alpha * (h1 - h5) * (h2 - h5) * (h3 - h5) * (h4 - h5) * (h6 - h5))
+        # the reference's a6 denominator starts with (h1 - h5), bdf.hh:433
+        a6 = -(n6 * n6) / (alpha * (h1 - h5) * (h2 - h6) * (h3 - h6) * (h4 - h6) * (h5 - h6))
+        return [a1, a2, a3, a4, a5, a6], a0, alpha
+    raise ValueError(m)
+
+
+class RateOp:
+    """The reference test's operator (time-integrators/test/implicit.cc:
+    27-63): f(x) = lambda x, apply(x) = x - gamma f(x)."""
+
+    def __init__(self, lam):
+        self.lam, self.gamma = lam, 1.0
+
+    def apply_rhs(self, x):
+        return self.lam * x
+
+    def is_valid(self, v):
+        return True
+
+    def set_scaling(self, g):
+        self.gamma = g
+
+
+def rate_solver(op):
+    """implicit.cc:69-85: x = min(b) / (1 - lambda gamma), always converged."""
+    def solve(b, x):
+        sol = b.min() / (1.0 - op.lam * op.gamma)
+        x[:] = sol
+        return True
+    return solve
+
+
+class Integrator:
+    """bdf::integrator (bdf.hh:51-872) over numpy vectors of length n.
+
+    op: apply_rhs(x) -> f(x), set_scaling(gamma), is_valid(v);
+    solver(b, x) -> success, solving x - gamma f(x) = b in place from the
+    initial guess held in x."""
+
+    def __init__(self, s: Settings, op, solver, n: int):
+        self.p = s
+        s.validate()
+        self.op, self.solver, self.n = op, solver, n
+        # base.hh:26-29
+        self.current_time = s.initial_time
+        self.current_dt = s.initial_dt
+        self.old_dt = s.initial_dt
+        self.integrator_step = 0
+        self.max_integrator_steps = s.max_steps
+        z = lambda: np.zeros(n)
+        self.w = {k: z() for k in ("rhs", "source", "current_sol", "scratch",
+                                   "previous_function", "current_function",
+                                   "scratch_function", "old_td", "predict",
+                                   "time_deriv")}
+        # history (bdf.hh:880-947): ring of memory_size(integrator) vectors
+        self.hlen = memory_size(s.integrator)
+        self.hvec = [z() for _ in range(self.hlen)]
+        self.hdt = [0.0] * self.hlen
+        self.hpos = 0
+        self.solver_success = False
+        self.total_steprejects = 0
+        self.current_steprejects = 0
+        self.prev_successive_rejects = False
+        self.time_trunc_err_est = 1.0
+        self.prev_time_trunc_err_est = 1.0
+        self.time_err_est_ratio = 1.0
+        self.alpha = 1.0
+        self.gamma = 0.0
+        self.first_step = True
+        self.new_time = self.current_time
+
+    # ---- history -------------------------------------------------------
+    def _hidx(self, index):
+        L = self.hlen
+        beg = L - 1 - self.hpos
+        ind = (beg + index + 1) % L
+        return L - 1 - ind
+
+    def prev(self, i):
+        return self.hvec[self._hidx(i)]
+
+    def prev_dt(self, i):
+        return self.hdt[self._hidx(i)]
+
+    def _push(self, v, dt):
+        self.hvec[self.hpos][:] = v
+        self.hdt[self.hpos] = dt
+        self.hpos = (self.hpos + 1) % self.hlen
+
+    # ---- base.hh --------------------------------------------------------
+    def get_current_time(self):
+        return self.current_time
+
+    def get_final_time(self):
+        return self.p.final_time
+
+    def get_current_dt(self):
+        return self.current_dt
+
+    def get_current_step(self):
+        return self.integrator_step
+
+    def num_step_rejects(self):
+        return self.total_steprejects
+
+    # ---- bdf.hh:74-95 ----------------------------------------------------
+    def advance(self, dt, first_step, curr, out):
+        assert curr is not out
+        assert self.integrator_step < self.max_integrator_steps and \
+            self.current_time < self.p.final_time, "already finished integrating"
+        self.prev(0)[:] = curr
+        self.current_dt = dt
+        self.set_initial_guess(first_step)
+        w = self.w
+        w["rhs"][:] = -1.0 * w["source"]
+        self.solver_success = bool(self.solver(w["rhs"], w["current_sol"]))
+        out[:] = w["current_sol"]
+
+    def check_solution(self):
+        """bdf.hh:99-130"""
+        check_passed = False
+        if self.solver_success:
+            if self.p.calculate_time_trunc_error:
+                self.calculate_temporal_truncation_error()
+            if self.p.timestep_strategy == TRUNCATION_ERROR:
+                check_passed = self.time_trunc_err_est <= 1.0
+                if check_passed:
+                    self.prev_successive_rejects = self.current_steprejects > 1
+                    self.current_steprejects = 0
+                else:
+                    self.current_steprejects += 1
+            else:
+                check_passed = True
+        else:
+            self.current_steprejects += 1
+        if not check_passed:
+            self.total_steprejects += 1
+        return check_passed
+
+    def update(self):
+        """bdf.hh:132-149"""
+        self.current_time += self.current_dt
+        self.new_time = self.current_time
+        if self.p.use_predictor:
+            if self.p.predictor == AB2:
+                w = self.w
+                w["old_td"], w["current_function"] = w["current_function"], w["old_td"]
+            self.estimate_time_derivative()
+        self._push(self.w["current_sol"], self.current_dt)
+        self.integrator_step += 1
+
+    _final_constant_i = 1  # the reference's function-local static (bdf.hh:187)
+
+    def get_next_dt(self, good_solution):
+        """bdf.hh:151-219"""
+        p = self.p
+        tmp_dt = self.current_dt
+        if p.timestep_strategy == TRUNCATION_ERROR:
+            if good_solution:
+                self.current_dt = self.estimate_dt_with_truncation_error_estimates(
+                    self.current_dt, good_solution)
+                if p.max_dt < self.current_dt:
+                    self.current_dt = p.max_dt
+            else:
+                if self.solver_success:
+                    self.current_dt = self.estimate_dt_with_truncation_error_estimates(
+                        self.current_dt, good_solution)
+                else:
+                    self.current_dt = p.dt_cut_lower_bound * self.current_dt
+            self.evaluate_predictor()
+            valid = self.op.is_valid(self.w["predict"])
+            if not valid:
+                for _ in range(10):
+                    self.current_dt = 0.5 * self.current_dt
+                    self.evaluate_predictor()
+                    valid = self.op.is_valid(self.w["predict"])
+        else:
+            if good_solution:
+                if p.timestep_strategy == CONSTANT:
+                    self.current_dt = p.initial_dt
+                elif p.timestep_strategy == FINAL_CONSTANT:
+                    i = Integrator._final_constant_i
+                    if i < p.number_of_time_intervals:
+                        self.current_dt = p.initial_dt + float(i) / float(
+                            p.number_of_time_intervals) * (p.max_dt - p.initial_dt)
+                        Integrator._final_constant_i = i + 1
+                    else:
+                        self.current_dt = p.max_dt
+                elif p.timestep_strategy == LIMIT_RELATIVE_CHANGE:
+                    self.current_dt = self.estimate_dynamical_time_scale(self.current_dt)
+            else:
+                self.current_dt = 0.0
+        if good_solution:
+            self.old_dt = tmp_dt
+        self.current_dt = min(min(self.current_dt, p.max_dt),
+                              p.final_time - self.current_time)
+        return self.current_dt
+
+    # ---- protected -------------------------------------------------------
+    def get_current_integrator(self):
+        """bdf.hh:224-234"""
+        if self.integrator_step == 0:
+            return self.p.starting_integrator
+        if self.integrator_step + 1 < memory_size(self.p.integrator):
+            return self.integrator_step + 1
+        return self.p.integrator
+
+    def set_initial_guess(self, first_step):
+        """bdf.hh:236-277"""
+        self.first_step = first_step
+        w = self.w
+        if self.get_current_integrator() == CN:
+            w["scratch"][:] = self.prev(0)
+            w["previous_function"][:] = self.op.apply_rhs(w["scratch"])
+        if self.p.use_predictor:
+            if first_step and not self.p.use_initial_predictor:
+                w["predict"][:] = self.prev(0)
+            else:
+                self.evaluate_predictor()
+            w["current_sol"][:] = w["predict"]
+        else:
+            w["current_sol"][:] = self.prev(0)
+        self.compute_source_term()
+
+    def compute_source_term(self):
+        """bdf.hh:279-444"""
+        f = self.w["source"]
+        m = self.get_current_integrator()
+        h = [0.0] * memory_size(m)
+        h[0] = self.current_dt
+        for i in range(1, memory_size(m)):
+            h[i] = h[i - 1] + self.prev_dt(i - 1)
+        if m == BE:
+            f[:] = -1.0 * self.prev(0)
+            self.gamma = self.current_dt
+        elif m == CN:
+            eps = 0.0
+            alpha = 0.5 * self.current_dt + eps
+            beta = 0.5 * self.current_dt - eps
+            f[:] = beta * self.w["previous_function"] + self.prev(0)
+            f[:] = -1.0 * f
+            self.gamma = alpha
+        else:
+            a, a0, self.alpha = coefficients(m, h)
+            f[:] = a[0] * self.prev(0)
+            for k in range(1, len(a)):
+                f[:] = a[k] * self.prev(k) + f
+            self.gamma = a0
+        self.op.set_scaling(self.gamma)
+
+    def estimate_time_derivative(self):
+        """bdf.hh:450-505"""
+        w = self.w
+        m = self.get_current_integrator()
+        td, cs = w["time_deriv"], w["current_sol"]
+        if m == CN:
+            td[:] = -1.0 * self.prev(0) + cs
+            td[:] = (2.0 / self.current_dt) * td + (-1.0) * w["previous_function"]
+        elif m == BE:
+            td[:] = -1.0 * self.prev(0) + cs
+            td[:] = (1.0 / self.current_dt) * td
+        else:
+            dtt = self.current_dt / self.old_dt
+            alpha = (2.0 * dtt + 1.0) / ((dtt + 1.0) * self.current_dt)
+            beta = -(dtt + 1.0) / self.current_dt
+            gamma = (dtt * dtt) / ((dtt + 1.0) * self.current_dt)
+            td[:] = alpha * cs + beta * self.prev(0)
+            td[:] = gamma * self.prev(1) + td
+
+    def evaluate_predictor(self):
+        """bdf.hh:507-546"""
+        p = self.p
+        if p.integrator == CN:
+            if p.predictor != AB2:
+                raise RuntimeError("Crank-Nicolson currently only supports ab2 predictor")
+            self.evaluate_ab2_predictor()
+        elif p.integrator == BE:
+            self.evaluate_forward_euler_predictor()
+        else:
+            if self.first_step:
+                if p.starting_integrator not in (CN, BE):
+                    raise RuntimeError("starting integrator must be CN or BE")
+                self.evaluate_forward_euler_predictor()
+            else:
+                if p.predictor != LEAPFROG:
+                    raise RuntimeError("BDF2-6 only supports the leapfrog predictor")
+                self.evaluate_leapfrog_predictor()
+        if not self.op.is_valid(self.w["predict"]):
+            self.evaluate_forward_euler_predictor()
+
+    def evaluate_ab2_predictor(self):
+        """bdf.hh:548-560"""
+        w = self.w
+        r = self.current_dt / self.old_dt
+        alpha = self.current_dt * (2.0 + r) / 2.0
+        beta = -self.current_dt * r / 2.0
+        w["predict"][:] = alpha * w["current_function"] + beta * w["old_td"]
+        w["predict"][:] = w["predict"] + self.prev(0)
+
+    def evaluate_leapfrog_predictor(self):
+        """bdf.hh:562-573"""
+        w = self.w
+        r = self.current_dt / self.old_dt
+        alpha = r * r
+        beta = 1.0 - alpha
+        gamma = (1.0 + r) * self.current_dt
+        w["predict"][:] = alpha * self.prev(1) + beta * self.prev(0)
+        w["predict"][:] = gamma * w["time_deriv"] + w["predict"]
+
+    def evaluate_forward_euler_predictor(self):
+        """bdf.hh:575-595"""
+        w = self.w
+        gamma = self.current_dt
+        if self.first_step:
+            w["scratch"][:] = self.prev(0)
+            w["current_function"][:] = self.op.apply_rhs(w["scratch"])
+            w["time_deriv"][:] = w["current_function"]
+        w["predict"][:] = self.prev(0)
+        w["predict"][:] = gamma * w["time_deriv"] + w["predict"]
+
+    def _norm(self, v):
+        if self.p.time_trunc_err_norm == NORM_INF:
+            return float(np.max(np.abs(v))) if v.size else 0.0
+        return float(np.sqrt(np.dot(v, v)))
+
+    def estimate_dynamical_time_scale(self, curr_dt):
+        """bdf.hh:601-644; the norm is taken of scratch_function, as the
+        reference does, not of the relative change it computes in scratch."""
+        if self.p.integrator == CN:
+            raise RuntimeError("Not implemented")
+        w = self.w
+        w["scratch"][:] = self.prev(0) + EPS
+        w["scratch"][:] = self.prev(1) / w["scratch"]
+        w["scratch"][:] = w["scratch"] + (-1.0)
+        w["scratch"][:] = np.abs(w["scratch"])
+        rc = self._norm(w["scratch_function"])
+        factor = 1.05 if curr_dt > 0.5 else 1.1
+        factor = 1.03 if curr_dt > 10.0 else factor
+        cfl_new_dt = math.sqrt(self.p.target_relative_change / rc) * curr_dt
+        return min(cfl_new_dt, factor * curr_dt)
+
+    def estimate_dt_with_truncation_error_estimates(self, current_dt, good_solution):
+        """bdf.hh:650-727"""
+        p = self.p
+        pe = float(order(self.get_current_integrator()))
+        est = max(self.time_trunc_err_est, EPS)
+        if p.use_pi_controller and good_solution and not self.prev_successive_rejects:
+            sf = 0.8
+            c = p.pi_controller_type
+            if c == H211B:
+                b = 4.0
+                f = (sf / est) ** (1.0 / (b * (pe + 1.0)))
+                f *= (sf / max(self.prev_time_trunc_err_est, EPS)) ** (1.0 / (b * (pe + 1.0)))
+                f *= self.alpha ** -0.25
+            elif c == PC4_7:
+                f = (sf / est) ** (0.4 / (pe + 1.0))
+                f *= (self.time_err_est_ratio ** (0.7 / (pe + 1.0))) * self.alpha
+            elif c == PC11:
+                f = (sf / est) ** (1.0 / (pe + 1.0))
+                f *= (self.time_err_est_ratio ** (1.0 / (pe + 1.0))) * self.alpha
+            else:
+                f = (sf / est) ** (1.0 / (pe + 1.0))
+        else:
+            f = (0.8 / est) ** (1.0 / (pe + 1.0))
+        if p.control_timestep_variation and not p.use_pi_controller:
+            if abs(1.0 - f) < 0.1:
+                f = 1.0
+        self.current_dt = current_dt * f
+        return self.current_dt
+
+    def calculate_LTE_scaling_factor(self):
+        """bdf.hh:729-763"""
+        ef = 0.0
+        if self.first_step:
+            if self.p.starting_integrator == BE:
+                ef = 0.5
+            elif self.p.starting_integrator == CN:
+                ef = 1.5
+                ef *= self.current_dt
+        else:
+            a = self.alpha
+            if self.p.predictor == LEAPFROG:
+                ef = (1.0 + a) / (2.0 + 3.0 * a)
+            else:
+                ef = 2.0 / (6.0 - 1.0 / ((a + 1) * (a + 1)))
+        return ef
+
+    def calculate_scaled_LTE_norm(self, x, y):
+        """bdf.hh:765-802"""
+        w, p = self.w, self.p
+        sc = w["scratch"]
+        if p.time_error_scaling == FIXED_RESOLUTION:
+            sc[:] = p.time_rtol * x
+            sc[:] = sc + p.time_atol
+        else:
+            sc[:] = 0.0
+            sc[:] = x + p.problem_scales[0]
+            sc[:] = p.time_rtol * sc
+        sf = w["scratch_function"]
+        sf[:] = x - y
+        sf[:] = sf / sc
+        return self._norm(sf)
+
+    def calculate_temporal_truncation_error(self):
+        """bdf.hh:804-855"""
+        p = self.p
+        sol = self.w["current_sol"]
+        if self.integrator_step > 0 or (self.first_step and p.use_initial_predictor):
+            self.alpha = self.current_dt / self.old_dt
+            ef = self.calculate_LTE_scaling_factor()
+            y = self.prev(0) if (self.first_step and p.starting_integrator == CN) \
+                else self.w["predict"]
+            est = self.calculate_scaled_LTE_norm(sol, y) * ef
+            self.prev_time_trunc_err_est = self.time_trunc_err_est
+            self.time_trunc_err_est = max(EPS, est)
+            self.time_err_est_ratio = self.prev_time_trunc_err_est / self.time_trunc_err_est
+        else:
+            self.prev_time_trunc_err_est = 1.0
+            self.time_trunc_err_est = 1.0
+            self.time_err_est_ratio = 1.0
+
+
+def run(ti: Integrator, u0: np.ndarray, on_step=None):
+    """The driver loop of the reference's test and heat example
+    (time-integrators/test/implicit.cc:101-114; examples/heat_equation/
+    implicit.cc:41-55).  Returns the final solution."""
+    x = np.array(u0, dtype=np.float64, copy=True)
+    xnew = np.zeros_like(x)
+    dt = ti.get_current_dt()
+    first_step = True
+    while ti.get_current_time() < ti.get_final_time():
+        ti.advance(dt, first_step, x, xnew)
+        good = ti.check_solution()
+        if good:
+            ti.update()
+            x, xnew = xnew, x
+            first_step = False
+            if on_step is not None:
+                on_step(ti, x)
+        dt = ti.get_next_dt(good)
+    return x
+
+
+class CsrRhsOp:
+    """f(x) = F x for a CSR matrix F (rows summed in order, the oracle's
+    spmv), with the stage matrix I - gamma F formed on F's pattern as
+    fsx_csr_shift_scale forms it (each entry (-gamma) f, the diagonal
+    1 + (-gamma) f)."""
+
+    def __init__(self, rp, ci, fv):
+        from oracle import oracle as O
+        self.O = O
+        self.rp = np.asarray(rp)
+        self.ci = np.asarray(ci)
+        self.fv = np.asarray(fv, np.float64)
+        self.F = O.Csr(self.rp, self.ci, self.fv)
+        rows = np.repeat(np.arange(self.rp.size - 1), np.diff(self.rp))
+        self.diag = self.ci == rows
+        self.gamma = 1.0
+
+    def apply_rhs(self, x):
+        return self.F.spmv(x)
+
+    def is_valid(self, v):
+        return True
+
+    def set_scaling(self, g):
+        self.gamma = g
+
+    def stage(self):
+        v = (-self.gamma) * self.fv
+        v[self.diag] = 1.0 + v[self.diag]
+        return self.O.Csr(self.rp, self.ci, v)
+
+
+def cg_stage_solver(op: CsrRhsOp, settings, kind="cg"):
+    """Stage solve x - gamma F x = b by the oracle's CG (op::cg::apply), or
+    BiCGSTAB with kind="bicgstab", from the guess in x; records the solves
+    and iterations in .solves and .iters."""
+    method = getattr(op.O, kind)
+
+    def solve(b, x):
+        xs, info, _ = method(op.stage(), b, x, settings)
+        x[:] = xs
+        solve.iters += info.iters
+        solve.solves += 1
+        return info.reason in ("converged_atol", "converged_rtol", "converged_user")
+    solve.iters = 0
+    solve.solves = 0
+    return solve
+
+
+def heat2d(n: int, alpha: float):
+    """f(u) = alpha Laplacian(u) on an n x n interior grid of (0,10)^2 with
+    u = 0 on the boundary (the reference's heat example), and its initial
+    state u0 = 50 on [4,6]^2, 0 elsewhere (examples/heat_equation/README.md)."""
+    from flecsolve_amd import problems as P
+    rp, ci, v = P.poisson2d(n)
+    h = 10.0 / (n + 1)
+    c = alpha / (h * h)
+    fv = -c * v
+    ii = np.arange(n)
+    xx = (ii + 1) * h
+    inside = (xx >= 4) & (xx <= 6)
+    u0 = np.where(inside[None, :] & inside[:, None], 50.0, 0.0).ravel()
+    return rp, ci, fv, u0

@@ -1,0 +1,89 @@
+"""The time integrator over device vectors against the oracle (gpu).
+
+  * the reference's rate test (time-integrators/test/implicit.cc) through
+    tests/cpp/bdf_check on device vectors: bit-identical to the oracle (same
+    steps, rejections and final value; every norm here is exact);
+  * the heat example (flecsolve_amd/heat_implicit and bdf_check heat):
+    generic path (operator_adapter + krylov_factory CG, a host round trip
+    per reduction) and fused path (csr_adapter stage matrix + graph-captured
+    CG or BiCGSTAB): the oracle's steps, rejections, stage solves and Krylov
+    iterations, final solution within 1e-10 (device reductions sum in
+    another order than the oracle's);
+  * fsx_csr_shift_scale: I - gamma F bit for bit on every SpMV layout, and
+    the missing-diagonal error.
+"""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import bdf_common as BC
+from flecsolve_amd import fsx
+from flecsolve_amd import problems as P
+from oracle import bdf as B
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+CHECK = ROOT / "tests" / "cpp" / "bdf_check"
+HEAT = ROOT / "flecsolve_amd" / "heat_implicit"
+
+
+def run(args):
+    r = subprocess.run([str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_rate_bit_identical():
+    out = run([CHECK, "rate", BC.GOLDEN / "bdf_rate.cfg", -1.0, 3.0, 1000])
+    assert BC.parse_rate(out) == BC.oracle_rate(-1.0, 3.0, 1000)
+
+
+def test_heat_generic_check(tmp_path):
+    n = 48
+    f = tmp_path / "u.bin"
+    out = run([CHECK, "heat", BC.GOLDEN / "heat_implicit.cfg", n, 1.0, f])
+    ref, u = BC.oracle_heat(n, 1.0)
+    assert BC.parse_heat(out) == ref
+    assert np.abs(np.fromfile(f) - u).max() <= 1e-10 * np.abs(u).max()
+
+
+@pytest.mark.parametrize("kind", ["cg", "bicgstab"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_heat_example(tmp_path, kind, fused):
+    n, alpha = 64, 1.5
+    cfg = tmp_path / "heat.cfg"
+    cfg.write_text((BC.GOLDEN / "heat_implicit.cfg").read_text().replace(
+        "type = cg", f"type = {kind}"))
+    f = tmp_path / "u.bin"
+    args = [HEAT, cfg, n, "--alpha", alpha, "--out", f, "--quiet"] + (["--fused"] if fused else [])
+    out = run(args)
+    ref, u = BC.oracle_heat(n, alpha, cfg)
+    print(out.strip(), "oracle", ref)
+    assert BC.parse_heat(out) == ref
+    assert np.abs(np.fromfile(f) - u).max() <= 1e-10 * np.abs(u).max()
+
+
+def test_shift_scale(ctx):
+    rp, ci, fv, _ = B.heat2d(70, 2.0)
+    op = B.CsrRhsOp(rp, ci, fv)
+    F = ctx.csr(rp, ci, fv)
+    A = F.convert(fsx.F64)
+    x = P.uniform(70 * 70, 3)
+    xd, y = ctx.vec(x), ctx.zeros(x.size)
+    for g in (0.25, 3.0e-3):
+        A.shift_scale(F, 1.0, -g)
+        op.set_scaling(g)
+        S = op.stage()
+        np.testing.assert_array_equal(A.download()[2], S.vals)
+        A.spmv(xd, y)
+        np.testing.assert_array_equal(y.get(), S.spmv(x))
+    # a row without its diagonal entry: alpha must be 0
+    M = ctx.csr(np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.array([1.0, 2.0]))
+    Mc = M.convert(fsx.F64)
+    with pytest.raises(fsx.FsxError):
+        Mc.shift_scale(M, 1.0, -0.5)
+    Mc.shift_scale(M, 0.0, -0.5)
+    np.testing.assert_array_equal(Mc.download()[2], [-0.5, -1.0])
