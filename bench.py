@@ -26,6 +26,12 @@ along 20 seeded directions.
 
 `--workload convdiff`: 128^3 upwind convection-diffusion (cell Peclet 10)
 solved with BiCGSTAB and with GMRES(30) (right Jacobi) to 1e-10.
+
+`--workload heat`: the reference's implicit heat example (BDF2 after a CN
+start, truncation-error step control, CG stage solves to 1e-6, integrated to
+t = 0.1) on a 2046^2 interior grid; a step is the whole integration, run by
+flecsolve_amd/heat_implicit with the fused stage path, the reference-shaped
+generic path timed beside it.
 """
 from __future__ import annotations
 
@@ -116,7 +122,7 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the SpMV and copy kernels only (profiling runs)")
     ap.add_argument("--workload", choices=["diffusion", "batched", "adjoint", "convdiff",
-                                           "smoother", "amg"],
+                                           "smoother", "amg", "heat"],
                     default="diffusion")
     ap.add_argument("--heat-steps", type=int, default=50)
     ap.add_argument("--ckpt", type=int, default=10)
@@ -126,6 +132,8 @@ def main():
     ap.add_argument("--emin", type=int, default=8)
     ap.add_argument("--emax", type=int, default=48)
     ap.add_argument("--cpu-sample-systems", type=int, default=16)
+    ap.add_argument("--heat-n", type=int, default=2046)
+    ap.add_argument("--heat-cpu-n", type=int, default=638)
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -147,6 +155,8 @@ def main():
         return bench_smoother(a, fsx, P)
     if a.workload == "amg":
         return bench_amg(a, fsx, P)
+    if a.workload == "heat":
+        return bench_heat(a, fsx, P)
 
     seeds = P.seeds()
     n = a.n
@@ -805,6 +815,119 @@ def bench_amg(a, fsx, P):
                    "best_amg_speedup_vs_jacobi_cg":
                        round(results["jacobi_cg_fused"]["seconds"] / best, 3)},
     }
+    print(json.dumps(out), flush=True)
+
+
+def heat_runs(n, fused, repeat, cfg):
+    """Run flecsolve_amd/heat_implicit; one (steps, rejects, solves, iters,
+    wall seconds) per repeat."""
+    import subprocess
+    exe = ROOT / "flecsolve_amd" / "heat_implicit"
+    if not exe.exists():
+        sys.exit("flecsolve_amd/heat_implicit is not built (python -m flecsolve_amd.build)")
+    cmd = [str(exe), str(cfg), str(n), "--quiet", "--repeat", str(repeat)]
+    if fused:
+        cmd.append("--fused")
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        sys.exit("heat_implicit failed:\n" + r.stdout + r.stderr)
+    runs = []
+    for line in r.stdout.splitlines():
+        t = line.split()
+        if t and t[0] == "heat":
+            kv = dict(zip(t[1::2], t[2::2]))
+            runs.append((int(kv["steps"]), int(kv["rejects"]), int(kv["solves"]),
+                         int(kv["iters"]), float(kv["wall"])))
+    return runs
+
+
+def bench_heat(a, fsx, P):
+    """The reference's implicit heat example (examples/heat_equation/
+    implicit.cc with its implicit.cfg) integrated to t = 0.1: the BDF
+    integrator over device vectors with the fused stage path (csr_adapter +
+    graph-captured CG), the generic path beside it, the stage SpMV against
+    the HBM roofline, and the oracle (numpy integrator + C CG, one thread)
+    on a smaller grid as the CPU baseline."""
+    cfg = ROOT / "tests" / "golden" / "heat_implicit.cfg"
+    n = a.heat_n
+    N = n * n
+    reps = a.warmup + a.steps
+    fused = heat_runs(n, True, reps, cfg)
+    generic = heat_runs(n, False, reps, cfg)
+    for name, runs in (("fused", fused), ("generic", generic)):
+        for i, r in enumerate(runs):
+            log(f"heat {name} {'warmup' if i < a.warmup else 'step'} {i}: {r[4]:.3f} s, "
+                f"{r[0]} steps, {r[1]} rejects, {r[2]} solves, {r[3]} CG iterations")
+    tf = [r[4] for r in fused[a.warmup:]]
+    tg = [r[4] for r in generic[a.warmup:]]
+    fs, gs = sum(tf) / len(tf), sum(tg) / len(tg)
+    steps, rejects, solves, iters = fused[-1][:4]
+
+    # stage matrix I - gamma F (gamma of a mid-run BDF2 step) and its SpMV
+    rp, ci, v = P.poisson2d(n)
+    h = 10.0 / (n + 1)
+    fv = -(1.0 / (h * h)) * v
+    nnz = int(ci.size)
+    ctx = fsx.Context(0)
+    F = ctx.csr(rp, ci, fv)
+    A = F.convert(fsx.F64)
+    A.shift_scale(F, 1.0, -2e-3)
+    xd = ctx.vec(P.uniform(N, 7))
+    y = ctx.zeros(N)
+    ms = fsx_time(fsx, "fsx_time_spmv", ctx, A, xd, y, a.spmv_reps)
+    B = spmv_bytes(N, nnz, 8)
+    gbs = B / (ms * 1e-3) / 1e9
+    plan = A.plan()
+    log(f"stage spmv {n}^2: {ms:.4f} ms, {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.1%}), plan {plan}")
+    out = {
+        "metric": "heat example (BDF2/CN, CG stages) time to t = 0.1",
+        "value": round(fs, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(fs * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (the reference example's initial state: 50 on [4,6]^2)",
+        "config": {"workload": f"heat2d_{n}^2_bdf2_cn_start_t0.1", "rows": N, "nnz": nnz,
+                   "time_steps": steps, "rejected": rejects, "stage_solves": solves,
+                   "cg_iterations": iters,
+                   "ms_per_cg_iteration": round(fs * 1e3 / iters, 4),
+                   "generic_path_seconds": round(gs, 4),
+                   "fused_speedup_vs_generic": round(gs / fs, 3)},
+        "roofline": {"bound": "hbm", "kernel": "stage-matrix SpMV (I - gamma F), plan "
+                     f"{plan[0]}", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": B, "avg_ms": round(ms, 5)},
+    }
+    if not a.no_cpu_baseline:
+        import oracle.bdf as OB
+        from oracle import oracle as O
+        nc = a.heat_cpu_n
+        secs = OB.read_sections(cfg)
+        lso = secs["linear-solver.options"]
+        st = O.Settings(maxiter=int(lso["maxiter"]), rtol=float(lso["rtol"]),
+                        use_zero_guess=lso["use-zero-guess"] == "true")
+        rpc, cic, fvc, u0 = OB.heat2d(nc, 1.0)
+        op = OB.CsrRhsOp(rpc, cic, fvc)
+        solve = OB.cg_stage_solver(op, st)
+        ti = OB.Integrator(OB.settings_from_section(secs["time-integrator"]), op, solve,
+                           u0.size)
+        t = time.perf_counter()
+        OB.run(ti, u0)
+        tc = time.perf_counter() - t
+        log(f"cpu baseline (oracle) {nc}^2: {tc:.2f} s, {ti.get_current_step()} steps, "
+            f"{solve.iters} CG iterations")
+        out["cpu_baseline"] = {
+            "value": round(tc, 3), "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"the same integration on a {nc}^2 grid ({solve.iters} CG iterations, "
+                      f"{ti.get_current_step()} steps): oracle integrator (numpy) + oracle "
+                      "CG (C, serial)",
+            "ns_per_row_iteration": round(tc * 1e9 / (nc * nc * solve.iters), 3),
+            "gpu_ns_per_row_iteration": round(fs * 1e9 / (N * iters), 4)}
     print(json.dumps(out), flush=True)
 
 

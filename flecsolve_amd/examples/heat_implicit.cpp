@@ -20,9 +20,12 @@
 //     solver of the configured type (cg, gmres or bicgstab).
 //
 // usage: heat_implicit <cfg> <n> [--alpha A] [--fused] [--out file] [--quiet]
+//                      [--repeat R]
 // Prints one line per accepted step (unless --quiet) and a summary line
 //   "heat n N steps S rejects R solves K iters I time T wall W"
-// with W the seconds spent in the time loop.
+// with W the seconds spent in the time loop; --repeat runs the whole
+// integration R times from u(0) (one summary line each; the first includes
+// the one-time graph captures).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -84,10 +87,14 @@ struct counted_solver : op::base<> {
 	mutable long iters = 0, solves = 0;
 };
 
-template<class F, class S>
-int run(const ti::bdf::settings & ts, F op, S solver, dvec & u, dvec & unew, int n,
-        bool quiet, const char * out, fsx_ctx * ctx) {
-	ti::bdf::integrator t(ti::bdf::parameters(ts, op, ti::bdf::make_work(u), solver));
+// work vectors copy as aliases: every repeat reuses the same device buffers
+// (and so the solver graphs captured for them)
+template<class F, class S, class W>
+int run(const ti::bdf::settings & ts, F op, S solver, const W & work, dvec & u, dvec & unew,
+        int n, bool quiet, const char * out, fsx_ctx * ctx) {
+	solver.get().iters = 0;
+	solver.get().solves = 0;
+	ti::bdf::integrator t(ti::bdf::parameters(ts, op, W(work), solver));
 	// the loop of examples/heat_equation/implicit.cc:41-55
 	const auto t0 = std::chrono::steady_clock::now();
 	auto dt = t.get_current_dt();
@@ -133,11 +140,13 @@ int main(int argc, char ** argv) {
 	double alpha = 1.0;
 	bool fused = false, quiet = false;
 	const char * out = nullptr;
+	int repeat = 1;
 	for (int i = 3; i < argc; ++i) {
 		if (!std::strcmp(argv[i], "--alpha") && i + 1 < argc) alpha = std::atof(argv[++i]);
 		else if (!std::strcmp(argv[i], "--fused")) fused = true;
 		else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
 		else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out = argv[++i];
+		else if (!std::strcmp(argv[i], "--repeat") && i + 1 < argc) repeat = std::atoi(argv[++i]);
 		else {
 			std::fprintf(stderr, "unknown argument %s\n", argv[i]);
 			return 2;
@@ -156,9 +165,10 @@ int main(int argc, char ** argv) {
 		          "csr");
 		const auto Fm = mat::device_csr::adopt(ctx, Fh);
 		dvec u(ctx, p.u0.size()), unew(ctx, p.u0.size());
-		fsx_check(fsx_vec_upload(ctx, u.handle(), p.u0.data(), p.u0.size()), "upload");
+		const auto work = ti::bdf::make_work(u);
 		auto solver = op::make_shared<counted_solver>();
-		int rc;
+		int rc = 0;
+		if (repeat < 1) throw std::invalid_argument("--repeat must be positive");
 		if (fused) {
 			auto F = op::make_shared<ti::csr_adapter>(Fm);
 			const auto A = F.get().matrix();
@@ -178,13 +188,19 @@ int main(int argc, char ** argv) {
 				solver.get().fn = [g](const dcore & b, dcore & x) { return g.apply(b, x); };
 			}
 			}
-			rc = run(ts, F, solver, u, unew, n, quiet, out, ctx);
+			for (int r = 0; r < repeat; ++r) {
+				fsx_check(fsx_vec_upload(ctx, u.handle(), p.u0.data(), p.u0.size()), "upload");
+				rc = run(ts, F, solver, work, u, unew, n, quiet, out, ctx);
+			}
 		}
 		else {
 			auto F = op::make_shared<ti::operator_adapter<mat::device_csr>>(Fm);
 			auto inner = krylov_factory::make_shared(ss, u, F);
 			solver.get().fn = [inner](const dcore & b, dcore & x) { return inner.get().apply(b, x); };
-			rc = run(ts, F, solver, u, unew, n, quiet, out, ctx);
+			for (int r = 0; r < repeat; ++r) {
+				fsx_check(fsx_vec_upload(ctx, u.handle(), p.u0.data(), p.u0.size()), "upload");
+				rc = run(ts, F, solver, work, u, unew, n, quiet, out, ctx);
+			}
 		}
 		return rc;
 	}
