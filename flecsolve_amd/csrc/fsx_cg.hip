@@ -239,6 +239,23 @@ __global__ void ir_update_kernel(size_t n, double * __restrict__ x,
 	if (i < n) x[i] = x[i] + s * (double)d[i];
 }
 
+// symmetric Jacobi scaling: r32 = (float)(s (r / |r|)), x += |r| (s d)
+__global__ void ir_scale_sym_kernel(size_t n, const double * __restrict__ r,
+                                    const double * __restrict__ sc, float * __restrict__ r32,
+                                    const double * rnorm2) {
+	const double inv = 1.0 / sqrt(*rnorm2);
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) r32[i] = (float)(sc[i] * (r[i] * inv));
+}
+
+__global__ void ir_update_sym_kernel(size_t n, double * __restrict__ x,
+                                     const float * __restrict__ d,
+                                     const double * __restrict__ sc, const double * rnorm2) {
+	const double s = sqrt(*rnorm2);
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) x[i] = x[i] + s * (sc[i] * (double)d[i]);
+}
+
 // r32 = (float)(r / |r|)
 __global__ void ir_scale_kernel(size_t n, const double * __restrict__ r,
                                 float * __restrict__ r32,
@@ -1275,21 +1292,37 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 		                A32->ncols == A64->ncols && (int64_t)b->n == A64->nrows,
 		            "IR: shape mismatch");
 		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "IR: b and x must differ");
+		FSX_REQUIRE(jacobi >= 0 && jacobi <= 2, "IR: jacobi must be 0, 1 or 2");
+		FSX_REQUIRE(jacobi != 2 || A32->idx == A64->idx,
+		            "IR: jacobi 2 needs A32 = fsx_csr_convert(A64, FSX_F32) holding "
+		            "fsx_csr_set_values_f32_sym_scaled values");
 		std::memset(info, 0, sizeof(*info));
 		info->status = FSX_UNKNOWN;
 		const size_t n = b->n;
 		fsx_vec * r = pooled(ctx, n, FSX_F64, 10);
 		fsx_vec * r32 = pooled(ctx, n, FSX_F32, 11);
 		fsx_vec * d32 = pooled(ctx, n, FSX_F32, 12);
-		fsx_vec * dinv32 = jacobi ? pooled(ctx, n, FSX_F32, 13) : nullptr;
+		fsx_vec * dinv32 = jacobi == 1 ? pooled(ctx, n, FSX_F32, 13) : nullptr;
 		fsx_vec * ir = pooled(ctx, n, FSX_F32, 14);
-		fsx_vec * iz = jacobi ? pooled(ctx, n, FSX_F32, 15) : nullptr;
+		fsx_vec * iz = jacobi == 1 ? pooled(ctx, n, FSX_F32, 15) : nullptr;
 		fsx_vec * ip = pooled(ctx, n, FSX_F32, 16);
 		fsx_vec * ip2 = pooled(ctx, n, FSX_F32, 18);
 		fsx_vec * iw = pooled(ctx, n, FSX_F32, 17);
-		if (jacobi) {
+		fsx_vec * sc = jacobi == 2 ? pooled(ctx, n, FSX_F64, 19) : nullptr;
+		if (jacobi == 1) {
 			int rc = fsx_csr_diag_inv(ctx, A64, dinv32);
 			if (rc) throw error(rc, get_error());
+		}
+		if (jacobi == 2 && n) {
+			// s = D^{-1/2} of A64 (the scaling A32's values were written with)
+			int * flag = (int *)(ctx->ws.scalars + 9);
+			FSX_HIP(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+			sym_scale_vector(ctx, A64, (double *)sc->ptr(), flag);
+			int bad = 0;
+			FSX_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			ctx->host_syncs++;
+			FSX_REQUIRE(!bad, "IR: a diagonal entry of A64 is missing or not positive");
 		}
 		const double b_norm2 = reduce_host(ctx, RED_L2SQ, b, nullptr);
 		double b_norm = std::sqrt(b_norm2);
@@ -1327,20 +1360,31 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 				info->status = rnorm == 0.0 ? FSX_CONVERGED_RTOL : FSX_DIVERGED_ITERS;
 				break;
 			}
-			// normalised residual in FP32
-			hipLaunchKernelGGL(ir_scale_kernel, dim3(g), dim3(256), 0, ctx->stream,
-			                   n, (const double *)r->ptr(), (float *)r32->ptr(),
-			                   (const double *)rn2);
+			// normalised residual in FP32 (scaled by s for the symmetrically
+			// scaled inner matrix)
+			if (sc)
+				hipLaunchKernelGGL(ir_scale_sym_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+				                   (const double *)r->ptr(), (const double *)sc->ptr(),
+				                   (float *)r32->ptr(), (const double *)rn2);
+			else
+				hipLaunchKernelGGL(ir_scale_kernel, dim3(g), dim3(256), 0, ctx->stream,
+				                   n, (const double *)r->ptr(), (float *)r32->ptr(),
+				                   (const double *)rn2);
 			FSX_HIP(hipGetLastError());
 			fsx_info ii;
 			int cnt = 0;
 			cg_solve_impl<float>(ctx, A32, dinv32, r32, d32, is, &ii, ir, iz, ip, ip2,
 			                     iw, &cnt);
 			info->inner_iters_total += cnt;
-			// x += |r| d  (FP64)
-			hipLaunchKernelGGL(ir_update_kernel, dim3(g), dim3(256), 0, ctx->stream,
-			                   n, (double *)x->ptr(), (const float *)d32->ptr(),
-			                   (const double *)rn2);
+			// x += |r| d  (FP64; d = s y for the scaled inner system)
+			if (sc)
+				hipLaunchKernelGGL(ir_update_sym_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+				                   (double *)x->ptr(), (const float *)d32->ptr(),
+				                   (const double *)sc->ptr(), (const double *)rn2);
+			else
+				hipLaunchKernelGGL(ir_update_kernel, dim3(g), dim3(256), 0, ctx->stream,
+				                   n, (double *)x->ptr(), (const float *)d32->ptr(),
+				                   (const double *)rn2);
 			FSX_HIP(hipGetLastError());
 			ctx->launches += 2;
 			++outer;

@@ -189,6 +189,10 @@ void reduce_async(fsx_ctx * ctx, red_kind k, const fsx_vec * x,
 double reduce_host(fsx_ctx * ctx, red_kind k, const fsx_vec * x,
                    const fsx_vec * y);
 
+// s_r = 1 / sqrt(a_rr) of the FP64 matrix A on ctx's stream; *bad (device,
+// optional) is set to 1 when a diagonal entry is missing or not positive.
+void sym_scale_vector(fsx_ctx * ctx, const fsx_csr * A, double * s, int * bad);
+
 // Elementwise kernels used by the solve drivers.
 void launch_axpy(fsx_ctx * ctx, double a, const fsx_vec * x, const fsx_vec * y,
                  fsx_vec * z);

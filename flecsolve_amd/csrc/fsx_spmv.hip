@@ -567,6 +567,34 @@ __global__ void shift_scale_kernel(int64_t nrows, const I * __restrict__ rp,
 	}
 }
 
+// s_r = 1 / sqrt(a_rr) (FP64); bad[0] = 1 when a row has no stored
+// diagonal entry or a non-positive one (that row's s is 0)
+template<class I>
+__global__ void sym_scale_kernel(int64_t nrows, const I * __restrict__ rp,
+                                 const I * __restrict__ ci, const double * __restrict__ v,
+                                 double * __restrict__ s, int * __restrict__ bad) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	double d = 0.0;
+	for (I e = rp[r]; e < rp[r + 1]; ++e)
+		if ((int64_t)ci[e] == r) d = v[e];
+	const bool ok = d > 0.0;
+	s[r] = ok ? 1.0 / sqrt(d) : 0.0;
+	if (!ok && bad) bad[0] = 1;
+}
+
+// a32_rc = (float)((s_r a_rc) s_c)
+template<class I>
+__global__ void sym_scaled_f32_kernel(int64_t nrows, const I * __restrict__ rp,
+                                      const I * __restrict__ ci,
+                                      const double * __restrict__ v,
+                                      const double * __restrict__ s, float * __restrict__ a32) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	const double sr = s[r];
+	for (I e = rp[r]; e < rp[r + 1]; ++e) a32[e] = (float)((sr * v[e]) * s[ci[e]]);
+}
+
 // missing[0] = 1 when some row has no stored diagonal entry
 template<class I>
 __global__ void diag_missing_kernel(int64_t nrows, const I * __restrict__ rp,
@@ -588,6 +616,22 @@ __global__ void copy_kernel(size_t n, const double * __restrict__ s,
                             double * __restrict__ d) {
 	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) d[i] = s[i];
+}
+
+void sym_scale_vector(fsx_ctx * ctx, const fsx_csr * A, double * s, int * bad) {
+	const int64_t n = A->nrows;
+	if (n == 0) return;
+	const unsigned g = (unsigned)((n + 255) / 256);
+	if (A->it == FSX_I32)
+		hipLaunchKernelGGL((sym_scale_kernel<int32_t>), dim3(g), dim3(256), 0, ctx->stream, n,
+		                   (const int32_t *)A->rowptr(), (const int32_t *)A->colind(),
+		                   (const double *)A->vals->p, s, bad);
+	else
+		hipLaunchKernelGGL((sym_scale_kernel<int64_t>), dim3(g), dim3(256), 0, ctx->stream, n,
+		                   (const int64_t *)A->rowptr(), (const int64_t *)A->colind(),
+		                   (const double *)A->vals->p, s, bad);
+	FSX_HIP(hipGetLastError());
+	ctx->launches++;
 }
 
 } // namespace fsx
@@ -894,6 +938,43 @@ int fsx_csr_set_values(fsx_ctx * ctx, fsx_csr * A, const void * vals) {
 			                       hipMemcpyHostToDevice, ctx->stream));
 		refresh_sliced_values(ctx, A);
 		FSX_HIP(hipStreamSynchronize(ctx->stream)); // the host buffer may go away
+	});
+}
+
+int fsx_csr_set_values_f32_sym_scaled(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr * A64) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A32 && A64, "null argument");
+		FSX_REQUIRE(A32->vt == FSX_F32 && A64->vt == FSX_F64,
+		            "set_values_f32_sym_scaled: FP32 destination, FP64 source");
+		FSX_REQUIRE(A32->idx == A64->idx,
+		            "set_values_f32_sym_scaled: A32 must be fsx_csr_convert(A64, FSX_F32)");
+		FSX_REQUIRE(A64->nrows == A64->ncols, "set_values_f32_sym_scaled: square matrix");
+		const int64_t n = A64->nrows;
+		if (n == 0) return;
+		dbuf s((size_t)n * sizeof(double));
+		dbuf flag(sizeof(int));
+		FSX_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+		sym_scale_vector(ctx, A64, (double *)s.p, (int *)flag.p);
+		int bad = 0;
+		FSX_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->host_syncs++;
+		FSX_REQUIRE(!bad, "set_values_f32_sym_scaled: a diagonal entry is missing or not positive");
+		const unsigned g = (unsigned)((n + 255) / 256);
+		if (A64->it == FSX_I32)
+			hipLaunchKernelGGL((sym_scaled_f32_kernel<int32_t>), dim3(g), dim3(256), 0,
+			                   ctx->stream, n, (const int32_t *)A64->rowptr(),
+			                   (const int32_t *)A64->colind(), (const double *)A64->vals->p,
+			                   (const double *)s.p, (float *)A32->vals->p);
+		else
+			hipLaunchKernelGGL((sym_scaled_f32_kernel<int64_t>), dim3(g), dim3(256), 0,
+			                   ctx->stream, n, (const int64_t *)A64->rowptr(),
+			                   (const int64_t *)A64->colind(), (const double *)A64->vals->p,
+			                   (const double *)s.p, (float *)A32->vals->p);
+		FSX_HIP(hipGetLastError());
+		ctx->launches++;
+		refresh_sliced_values(ctx, A32);
+		FSX_HIP(hipStreamSynchronize(ctx->stream)); // s and flag are freed on return
 	});
 }
 

@@ -177,6 +177,7 @@ _SIGS = {
     "fsx_csr_set_values": (C.c_int, [_P, _P, _P]),
     "fsx_csr_set_values_f32_from_f64": (C.c_int, [_P, _P, _P]),
     "fsx_csr_shift_scale": (C.c_int, [_P, _P, C.c_double, C.c_double, _P]),
+    "fsx_csr_set_values_f32_sym_scaled": (C.c_int, [_P, _P, _P]),
     "fsx_reduce_async": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "fsx_cg_update": (C.c_int, [_P, C.c_double, _P, _P, _P, _P, _P, _P,
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -386,6 +387,12 @@ class CSR:
         check(lib().fsx_csr_set_values_f32_from_f64(self.ctx.h, self.h, A64.h),
               "fsx_csr_set_values_f32_from_f64")
 
+    def set_values_sym_scaled(self, A64: "CSR"):
+        """FP32 copy takes S A64 S, S = D^{-1/2} (inner matrix of
+        ir_cg_solve(..., jacobi=2))."""
+        check(lib().fsx_csr_set_values_f32_sym_scaled(self.ctx.h, self.h, A64.h),
+              "fsx_csr_set_values_f32_sym_scaled")
+
     def shift_scale(self, F: "CSR", alpha: float, beta: float):
         """self = alpha I + beta F on F's pattern (self = F.convert(F64))."""
         check(lib().fsx_csr_shift_scale(self.ctx.h, F.h, alpha, beta, self.h),
@@ -504,7 +511,9 @@ def cg_history(ctx: Context) -> np.ndarray:
 
 
 def ir_cg_solve(A64: CSR, A32: CSR, b: Vector, x: Vector, rtol_outer: float,
-                inner: Settings, jacobi: bool = True, max_outer: int = 50) -> IRInfo:
+                inner: Settings, jacobi: int = 1, max_outer: int = 50) -> IRInfo:
+    """jacobi: 0 none, 1 Jacobi-preconditioned inner CG, 2 symmetrically
+    scaled inner matrix (A32.set_values_sym_scaled(A64) first)."""
     info = IRInfo()
     check(lib().fsx_ir_cg_solve(A64.ctx.h, A64.h, A32.h, int(jacobi), b.h, x.h,
                                 rtol_outer, max_outer, C.byref(inner),

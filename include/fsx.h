@@ -178,6 +178,11 @@ int fsx_csr_set_values(fsx_ctx * ctx, fsx_csr * A, const void * vals);
 /* A32 (= fsx_csr_convert(A64, FSX_F32), shared structure) takes A64's
  * current values rounded to FP32, on the device. */
 int fsx_csr_set_values_f32_from_f64(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr * A64);
+/* A32 (= fsx_csr_convert(A64, FSX_F32)) takes S A64 S rounded to FP32,
+ * S = diag(1 / sqrt(a_ii)) in FP64: a32_ij = (float)((s_i a_ij) s_j).  Every
+ * diagonal entry of A64 must be stored and positive.  The inner matrix of
+ * fsx_ir_cg_solve with jacobi = 2.  New, no reference counterpart. */
+int fsx_csr_set_values_f32_sym_scaled(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr * A64);
 /* A = alpha I + beta F on F's pattern (A = fsx_csr_convert(F, FSX_F64), or
  * any matrix sharing F's index arrays; FP64, square, every row storing its
  * diagonal entry unless alpha == 0): each entry beta f_ij, each diagonal
@@ -318,8 +323,13 @@ int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
                     fsx_info * info);
 
 /* Mixed-precision iterative refinement (new, no reference counterpart):
- * outer FP64 r = b - A64 x, |r|/|b| check, inner FP32 CG(+Jacobi) on A32 to
- * `inner->rtol` with FP64 reductions, FP64 update x += |r| d. */
+ * outer FP64 r = b - A64 x, |r|/|b| check, inner FP32 CG on A32 to
+ * `inner->rtol` with FP64 reductions, FP64 update x += |r| d.
+ * jacobi: 0 inner CG unpreconditioned; 1 inner CG + Jacobi (D^{-1} of A64);
+ * 2 symmetric Jacobi scaling: A32 holds S A64 S, S = D^{-1/2} (written by
+ * fsx_csr_set_values_f32_sym_scaled), the inner CG runs unpreconditioned on
+ * S r and d = S y -- Jacobi-preconditioned CG in exact arithmetic, without
+ * the preconditioner's vector passes. */
 typedef struct {
 	int outer_iters;
 	int inner_iters_total;

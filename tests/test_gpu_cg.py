@@ -138,17 +138,20 @@ def test_fp32_cg(ctx):
     assert np.linalg.norm(b - S @ x) / np.linalg.norm(b) < 2e-4
 
 
+@pytest.mark.parametrize("jacobi", [1, 2])
 @pytest.mark.parametrize("sigma", [1.0, 2.5])
-def test_ir_reaches_1e10_and_matches_fp64(ctx, sigma):
+def test_ir_reaches_1e10_and_matches_fp64(ctx, sigma, jacobi):
     n = 32
     rp, ci, v, Ao = diffusion(n, sigma, seed=3001)
     N = n ** 3
     b = P.uniform(N, 3002)
     A64 = ctx.csr(rp, ci, v)
     A32 = A64.convert(fsx.F32)
+    if jacobi == 2:
+        A32.set_values_sym_scaled(A64)
     xd = ctx.zeros(N)
     info = fsx.ir_cg_solve(A64, A32, ctx.vec(b), xd, 1e-10,
-                           fsx.Settings(maxiter=2000, rtol=1e-4))
+                           fsx.Settings(maxiter=2000, rtol=1e-4), jacobi=jacobi)
     assert info.status == 1 and info.rel_res_final <= 1e-10
     S = sp.csr_matrix((v, ci, rp))
     x = xd.get()
@@ -240,3 +243,34 @@ def test_graphs_after_matrix_rebuild(ctx):
         assert np.linalg.norm(b - S @ x2.get()) / np.linalg.norm(b) <= 1e-9
         del A, A32, x, x2
         gc.collect()
+
+
+@pytest.mark.parametrize("plan", [3, 2, 0])
+def test_sym_scaled_f32_values(ctx, plan):
+    """S A S in FP32, S = 1/sqrt(diag): values bit for bit against numpy
+    ((s_i a_ij) s_j rounded once), on each SpMV layout; errors for a missing
+    or non-positive diagonal."""
+    n = 20
+    rp, ci, v, _ = diffusion(n, 2.0, seed=5001)
+    A64 = ctx.csr(rp, ci, v)
+    A64.set_plan(plan)
+    A32 = A64.convert(fsx.F32)
+    A32.set_values_sym_scaled(A64)
+    rows = np.repeat(np.arange(n ** 3), np.diff(rp))
+    d = v[ci == rows]
+    sv = 1.0 / np.sqrt(d)
+    want = ((sv[rows] * v) * sv[ci]).astype(np.float32)
+    np.testing.assert_array_equal(A32.download()[2], want)
+    x = P.uniform(n ** 3, 5002).astype(np.float32)
+    y = ctx.zeros(n ** 3, fsx.F32)
+    A32.spmv(ctx.vec(x, fsx.F32), y)
+    ref = sp.csr_matrix((want.astype(np.float64), ci, rp)) @ x.astype(np.float64)
+    assert np.abs(y.get() - ref).max() <= 1e-5 * np.abs(ref).max()
+    bad = ctx.csr(np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32),
+                  np.array([1.0, 1.0]))
+    with pytest.raises(fsx.FsxError):
+        bad.convert(fsx.F32).set_values_sym_scaled(bad)
+    neg = ctx.csr(np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32),
+                  np.array([1.0, -1.0]))
+    with pytest.raises(fsx.FsxError):
+        neg.convert(fsx.F32).set_values_sym_scaled(neg)
