@@ -373,6 +373,9 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	FSX_HIP(hipGetLastError());
 }
 
+// chunk length used while a solve's length is unknown or nearly reached
+constexpr int kShortChunk = 4;
+
 static int chunk_size() {
 	static int c = [] {
 		const char * e = getenv("FSX_CG_CHUNK");
@@ -447,67 +450,103 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 		                                     2 * ((B.n + per - 1) / per));
 		ensure_partials(ctx, need);
 	}
-	hipGraphExec_t ge = nullptr;
+	// Two chunk lengths: short chunks while the solve's length is unknown or
+	// nearly reached, long ones in between.  Iterations past convergence are
+	// launches that return at once, so the schedule changes only how many of
+	// those run, never the iterates.
+	const int Kbig = K;
+	const int Ksmall = std::min(Kbig, kShortChunk);
+	hipGraphExec_t gexec[2] = {nullptr, nullptr}; // [0] Ksmall, [1] Kbig
 	if (use_graphs()) {
-		std::vector<const void *> key{(const void *)&enqueue_iteration<T, V, I>,
-		                              B.x, B.r, B.z, B.p, B.p2, B.w, B.dinv,
-		                              (const void *)ctx->ws.partials, (const void *)st,
-		                              (const void *)hist, (const void *)(intptr_t)B.n,
-		                              (const void *)(intptr_t)K};
-		append_layout(key, B.A);
-		auto it = cache.mv.find(key);
-		if (it == cache.mv.end()) {
-			if (cache.mv.size() >= 32) cache.clear_graphs();
-			cg_graph_cache::entry en;
-			FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-			try {
-				for (int k = 0; k < K; ++k)
-					enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
-			}
-			catch (...) {
-				hipGraph_t junk = nullptr;
-				(void)hipStreamEndCapture(ctx->stream, &junk);
-				if (junk) (void)hipGraphDestroy(junk);
-				throw;
-			}
-			FSX_HIP(hipStreamEndCapture(ctx->stream, &en.g));
-			FSX_HIP(hipGraphInstantiate(&en.ge, en.g, nullptr, nullptr, 0));
-			cache.mv[key] = en;
-			ge = en.ge;
+		auto key_for = [&](int k) {
+			std::vector<const void *> key{(const void *)&enqueue_iteration<T, V, I>,
+			                              B.x, B.r, B.z, B.p, B.p2, B.w, B.dinv,
+			                              (const void *)ctx->ws.partials, (const void *)st,
+			                              (const void *)hist, (const void *)(intptr_t)B.n,
+			                              (const void *)(intptr_t)k};
+			append_layout(key, B.A);
+			return key;
+		};
+		const int lens[2] = {Ksmall, Kbig};
+		const int nlen = Ksmall == Kbig ? 1 : 2;
+		std::vector<const void *> keys[2];
+		int missing = 0;
+		for (int j = 0; j < nlen; ++j) {
+			keys[j] = key_for(lens[j]);
+			missing += cache.mv.count(keys[j]) ? 0 : 1;
 		}
-		else
-			ge = it->second.ge;
+		// make room before capturing, so no executable obtained here is freed
+		// (the previous solve ended with a stream synchronisation)
+		if (missing && cache.mv.size() + missing > 32) cache.clear_graphs();
+		for (int j = 0; j < nlen; ++j) {
+			auto it = cache.mv.find(keys[j]);
+			if (it == cache.mv.end()) {
+				cg_graph_cache::entry en;
+				FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+				try {
+					for (int k = 0; k < lens[j]; ++k)
+						enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
+				}
+				catch (...) {
+					hipGraph_t junk = nullptr;
+					(void)hipStreamEndCapture(ctx->stream, &junk);
+					if (junk) (void)hipGraphDestroy(junk);
+					throw;
+				}
+				FSX_HIP(hipStreamEndCapture(ctx->stream, &en.g));
+				FSX_HIP(hipGraphInstantiate(&en.ge, en.g, nullptr, nullptr, 0));
+				cache.mv[keys[j]] = en;
+				gexec[j] = en.ge;
+			}
+			else
+				gexec[j] = it->second.ge;
+		}
+		if (nlen == 1) gexec[1] = gexec[0];
 	}
-	// pipelined loop: keep one chunk in flight while checking the previous
+	// pipelined loop: up to two chunks in flight, the host checking the older
 	cg_state * flags = pinned + 1;
 	hipEvent_t ev[2];
 	FSX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
 	FSX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-	auto launch_chunk = [&](int slot) {
+	auto launch_chunk = [&](int slot, int len) {
+		hipGraphExec_t ge = gexec[len == Ksmall ? 0 : 1];
 		if (ge) FSX_HIP(hipGraphLaunch(ge, ctx->stream));
 		else
-			for (int k = 0; k < K; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
-		ctx->launches += 5 * K;
+			for (int k = 0; k < len; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
+		ctx->launches += 5 * len;
 		FSX_HIP(hipMemcpyAsync(&flags[slot], st, sizeof(cg_state),
 		                       hipMemcpyDeviceToHost, ctx->stream));
 		FSX_HIP(hipEventRecord(ev[slot], ctx->stream));
 	};
-	const int max_chunks = (st_host.maxiter + K - 1) / K;
+	const long maxit = st_host.maxiter;
+	const double res0 = st_host.res, tol = st_host.tol;
+	long launched_it = 0; // iterations launched
+	long want = -1;       // iterations predicted beyond those launched (-1: unknown)
 	int launched = 0, checked = 0;
-	if (max_chunks > 0) {
-		launch_chunk(0);
-		launched = 1;
-	}
-	while (checked < launched) {
-		if (launched < max_chunks) {
-			launch_chunk(launched & 1);
+	for (;;) {
+		const bool in_flight = launched > checked;
+		if (launched - checked < 2 && launched_it < maxit && (!in_flight || want != 0)) {
+			const int len = (launched < 2 || (want >= 0 && want <= Ksmall + 2)) ? Ksmall : Kbig;
+			launch_chunk(launched & 1, len);
 			++launched;
+			launched_it += len;
+			if (want > 0) want = std::max<long>(0, want - len);
 		}
+		if (checked == launched) break; // nothing in flight and nothing to launch
 		FSX_HIP(hipEventSynchronize(ev[checked & 1]));
 		ctx->host_syncs++;
 		const cg_state & f = flags[checked & 1];
 		++checked;
 		if (f.done) break;
+		// remaining iterations at the mean contraction per iteration so far
+		want = -1;
+		if (f.iter > 0 && f.res > 0 && res0 > 0 && tol > 0) {
+			const double rate = std::pow(f.res / res0, 1.0 / f.iter);
+			if (rate > 0 && rate < 1) {
+				const double m = std::ceil(std::log(tol / f.res) / std::log(rate));
+				if (m < 1e9) want = std::max<long>(0, (long)m - (launched_it - f.iter));
+			}
+		}
 	}
 	if (launched > 0 && B.n > 0) {
 		// the iteration that set `done` skipped K3: apply its x update

@@ -145,8 +145,10 @@ struct csr_index {
 	int64_t dg_nnz = 0;
 	// Lazily built transpose structure for A^T x (adjoint).
 	std::unique_ptr<fsx::dbuf> t_rowptr, t_colind, t_perm;
-	// every row stores its diagonal entry: -1 not yet checked, 0 no, 1 yes
+	// every row stores its diagonal entry: -1 not yet checked, 0 no, 1 yes;
+	// diag_pos (int64 per row, -1 when absent) once checked
 	int diag_full = -1;
+	std::unique_ptr<fsx::dbuf> diag_pos;
 };
 
 struct fsx_csr {

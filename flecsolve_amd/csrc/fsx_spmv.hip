@@ -552,19 +552,52 @@ __global__ void diag_inv_kernel(int64_t nrows, const I * __restrict__ rp,
 	d[r] = found ? v : 0.0;
 }
 
-// a = alpha I + beta f on one CSR pattern: every entry beta * f, the
-// diagonal entry alpha + beta * f (one row per thread).
-template<class I>
-__global__ void shift_scale_kernel(int64_t nrows, const I * __restrict__ rp,
-                                   const I * __restrict__ ci,
-                                   const double * __restrict__ f, double alpha,
-                                   double beta, double * __restrict__ a) {
+// a = beta f on every stored entry (coalesced; the diagonal is fixed next)
+__global__ void scale_values_kernel(size_t nnz, const double * __restrict__ f, double beta,
+                                    double * __restrict__ a) {
+	const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < nnz) a[e] = beta * f[e];
+}
+
+// a at each row's diagonal position = alpha + beta f (rows without one: -1)
+__global__ void diag_shift_kernel(int64_t nrows, const int64_t * __restrict__ dpos,
+                                  const double * __restrict__ f, double alpha, double beta,
+                                  double * __restrict__ a) {
 	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= nrows) return;
-	for (I e = rp[r]; e < rp[r + 1]; ++e) {
-		const double v = beta * f[e];
-		a[e] = ((int64_t)ci[e] == r) ? alpha + v : v;
-	}
+	const int64_t e = dpos[r];
+	if (e >= 0) a[e] = alpha + beta * f[e];
+}
+
+// The same on the slice-offset layout shared by f and a: slot k = 64 q +
+// lane holds row (slice, lane)'s entry at offset off[q] when the lane's mask
+// bit is set (absent slots stay zero); the diagonal is offset 0.
+__global__ void shift_scale_sdia_kernel(int64_t nslots_total, const uint64_t * __restrict__ mask,
+                                        const int32_t * __restrict__ off,
+                                        const double * __restrict__ f, double alpha,
+                                        double beta, double * __restrict__ a) {
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nslots_total) return;
+	const int64_t q = k / kSliceRows;
+	const int lane = (int)(k % kSliceRows);
+	if (!((mask[q] >> lane) & 1ull)) return;
+	const double v = beta * f[k];
+	a[k] = off[q] == 0 ? alpha + v : v;
+}
+
+// dpos[r] = position of row r's diagonal entry (the last one if the row
+// stores several), -1 if none; missing[0] = 1 when some row has none
+template<class I>
+__global__ void diag_pos_kernel(int64_t nrows, const I * __restrict__ rp,
+                                const I * __restrict__ ci, int64_t * __restrict__ dpos,
+                                int * __restrict__ missing) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	int64_t pos = -1;
+	for (I e = rp[r]; e < rp[r + 1]; ++e)
+		if ((int64_t)ci[e] == r) pos = (int64_t)e;
+	dpos[r] = pos;
+	if (pos < 0) missing[0] = 1;
 }
 
 // s_r = 1 / sqrt(a_rr) (FP64); bad[0] = 1 when a row has no stored
@@ -989,19 +1022,21 @@ int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double b
 		FSX_REQUIRE(F->nrows == F->ncols, "shift_scale: square matrix");
 		const int64_t n = F->nrows;
 		if (n == 0) return;
-		const unsigned g = (unsigned)((n + 255) / 256);
 		csr_index & ix = *F->idx;
-		if (ix.diag_full < 0) {
+		const unsigned g = (unsigned)((n + 255) / 256);
+		if (!ix.diag_pos) {
+			// each row's diagonal position, found once per pattern
+			auto pos = std::make_unique<dbuf>((size_t)n * sizeof(int64_t));
 			dbuf flag(sizeof(int));
 			FSX_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
 			if (F->it == FSX_I32)
-				hipLaunchKernelGGL((diag_missing_kernel<int32_t>), dim3(g), dim3(256), 0,
+				hipLaunchKernelGGL((diag_pos_kernel<int32_t>), dim3(g), dim3(256), 0,
 				                   ctx->stream, n, (const int32_t *)F->rowptr(),
-				                   (const int32_t *)F->colind(), (int *)flag.p);
+				                   (const int32_t *)F->colind(), (int64_t *)pos->p, (int *)flag.p);
 			else
-				hipLaunchKernelGGL((diag_missing_kernel<int64_t>), dim3(g), dim3(256), 0,
+				hipLaunchKernelGGL((diag_pos_kernel<int64_t>), dim3(g), dim3(256), 0,
 				                   ctx->stream, n, (const int64_t *)F->rowptr(),
-				                   (const int64_t *)F->colind(), (int *)flag.p);
+				                   (const int64_t *)F->colind(), (int64_t *)pos->p, (int *)flag.p);
 			FSX_HIP(hipGetLastError());
 			int missing = 0;
 			FSX_HIP(hipMemcpyAsync(&missing, flag.p, sizeof(int), hipMemcpyDeviceToHost,
@@ -1009,22 +1044,41 @@ int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double b
 			FSX_HIP(hipStreamSynchronize(ctx->stream));
 			ctx->host_syncs++;
 			ix.diag_full = missing ? 0 : 1;
+			ix.diag_pos = std::move(pos);
 		}
 		FSX_REQUIRE(ix.diag_full == 1 || alpha == 0.0,
 		            "shift_scale: a row of F has no stored diagonal entry");
-		if (F->it == FSX_I32)
-			hipLaunchKernelGGL((shift_scale_kernel<int32_t>), dim3(g), dim3(256), 0,
-			                   ctx->stream, n, (const int32_t *)F->rowptr(),
-			                   (const int32_t *)F->colind(), (const double *)F->vals->p,
+		const size_t nnz = (size_t)F->nnz;
+		if (nnz) {
+			hipLaunchKernelGGL(scale_values_kernel, dim3((unsigned)((nnz + 255) / 256)), dim3(256),
+			                   0, ctx->stream, nnz, (const double *)F->vals->p, beta,
+			                   (double *)A->vals->p);
+			hipLaunchKernelGGL(diag_shift_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+			                   (const int64_t *)ix.diag_pos->p, (const double *)F->vals->p,
 			                   alpha, beta, (double *)A->vals->p);
-		else
-			hipLaunchKernelGGL((shift_scale_kernel<int64_t>), dim3(g), dim3(256), 0,
-			                   ctx->stream, n, (const int64_t *)F->rowptr(),
-			                   (const int64_t *)F->colind(), (const double *)F->vals->p,
-			                   alpha, beta, (double *)A->vals->p);
-		FSX_HIP(hipGetLastError());
-		ctx->launches++;
-		refresh_sliced_values(ctx, A);
+			FSX_HIP(hipGetLastError());
+			ctx->launches += 2;
+		}
+		// the SpMV layouts: the slice-offset one rewritten in place from F's,
+		// the sliced one re-copied from the CSR values
+		++A->vals_version;
+		if (ix.dg_ptr && A->dg_vals) {
+			if (F->dg_vals && ix.dg_nnz > 0) {
+				const int64_t t = ix.dg_nnz;
+				hipLaunchKernelGGL(shift_scale_sdia_kernel, dim3((unsigned)((t + 255) / 256)),
+				                   dim3(256), 0, ctx->stream, t, (const uint64_t *)ix.dg_mask->p,
+				                   (const int32_t *)ix.dg_off->p, (const double *)F->dg_vals->p,
+				                   alpha, beta, (double *)A->dg_vals->p);
+				FSX_HIP(hipGetLastError());
+				ctx->launches++;
+			}
+			else
+				sdia_fill(ctx, A);
+		}
+		if (ix.sl_ptr && A->sl_vals) {
+			if (A->it == FSX_I32) refresh_sliced_t<int32_t, double>(ctx, A);
+			else refresh_sliced_t<int64_t, double>(ctx, A);
+		}
 	});
 }
 

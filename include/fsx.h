@@ -185,9 +185,9 @@ int fsx_csr_set_values_f32_from_f64(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr 
 int fsx_csr_set_values_f32_sym_scaled(fsx_ctx * ctx, fsx_csr * A32, const fsx_csr * A64);
 /* A = alpha I + beta F on F's pattern (A = fsx_csr_convert(F, FSX_F64), or
  * any matrix sharing F's index arrays; FP64, square, every row storing its
- * diagonal entry unless alpha == 0): each entry beta f_ij, each diagonal
- * entry alpha + beta f_ii, computed on the device; A's SpMV layouts are
- * refreshed in place.  Forms the implicit stage matrix I - gamma F of
+ * diagonal entry unless alpha == 0): each entry beta f_ij, the row's
+ * diagonal entry (the last one if the row stores several) alpha + beta f_ii,
+ * computed on the device; A's SpMV layouts are rewritten in place.  Forms the implicit stage matrix I - gamma F of
  * time_integrator::operator_adapter (time-integrators/operator_adapter.hh:
  * 29-35, y = x - gamma f(x)) as one matrix. */
 int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double beta,

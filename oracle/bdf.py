@@ -686,8 +686,8 @@ def run(ti: Integrator, u0: np.ndarray, on_step=None):
 class CsrRhsOp:
     """f(x) = F x for a CSR matrix F (rows summed in order, the oracle's
     spmv), with the stage matrix I - gamma F formed on F's pattern as
-    fsx_csr_shift_scale forms it (each entry (-gamma) f, the diagonal
-    1 + (-gamma) f)."""
+    fsx_csr_shift_scale forms it (each entry (-gamma) f, the row's last
+    stored diagonal entry 1 + (-gamma) f)."""
 
     def __init__(self, rp, ci, fv):
         from oracle import oracle as O
@@ -697,7 +697,11 @@ class CsrRhsOp:
         self.fv = np.asarray(fv, np.float64)
         self.F = O.Csr(self.rp, self.ci, self.fv)
         rows = np.repeat(np.arange(self.rp.size - 1), np.diff(self.rp))
-        self.diag = self.ci == rows
+        # the last stored diagonal entry of each row takes the identity
+        dpos = np.full(self.rp.size - 1, -1, np.int64)
+        hit = np.flatnonzero(self.ci == rows)
+        dpos[rows[hit]] = hit
+        self.diag = dpos[dpos >= 0]
         self.gamma = 1.0
 
     def apply_rhs(self, x):

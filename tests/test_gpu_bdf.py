@@ -87,3 +87,9 @@ def test_shift_scale(ctx):
         Mc.shift_scale(M, 1.0, -0.5)
     Mc.shift_scale(M, 0.0, -0.5)
     np.testing.assert_array_equal(Mc.download()[2], [-0.5, -1.0])
+    # a row storing its diagonal twice: the identity goes to the last one
+    D = ctx.csr(np.array([0, 2, 3], np.int32), np.array([0, 0, 1], np.int32),
+                np.array([2.0, 4.0, 8.0]))
+    Dc = D.convert(fsx.F64)
+    Dc.shift_scale(D, 1.0, -0.5)
+    np.testing.assert_array_equal(Dc.download()[2], [-1.0, -1.0, -3.0])
