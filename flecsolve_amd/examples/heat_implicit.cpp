@@ -19,8 +19,12 @@
 //     by one kernel when gamma changes), solved by the fused graph-captured
 //     solver of the configured type (cg, gmres or bicgstab).
 //
-// usage: heat_implicit <cfg> <n> [--alpha A] [--fused] [--out file] [--quiet]
-//                      [--repeat R]
+// --explicit instead runs the reference's explicit example
+// (examples/heat_equation/explicit.cc with its explicit.cfg): RK23 with the
+// SpMV as f, no linear solves.
+//
+// usage: heat_implicit <cfg> <n> [--alpha A] [--fused | --explicit] [--out file]
+//                      [--quiet] [--repeat R]
 // Prints one line per accepted step (unless --quiet) and a summary line
 //   "heat n N steps S rejects R solves K iters I time T wall W"
 // with W the seconds spent in the time loop; --repeat runs the whole
@@ -40,6 +44,7 @@
 #include "flecsolve/solvers/fused.hh"
 #include "flecsolve/time-integrators/bdf.hh"
 #include "flecsolve/time-integrators/operator_adapter.hh"
+#include "flecsolve/time-integrators/rk23.hh"
 #include "flecsolve/vectors/device.hh"
 #include "fsx_problems.h"
 
@@ -87,6 +92,15 @@ struct counted_solver : op::base<> {
 	mutable long iters = 0, solves = 0;
 };
 
+// download u to `out` (n*n doubles) when a file is named
+void write_solution(const dvec & u, const char * out, fsx_ctx * ctx) {
+	if (!out) return;
+	std::vector<double> h(u.data.size());
+	fsx_check(fsx_vec_download(ctx, u.handle(), h.data(), h.size()), "download");
+	std::ofstream(out, std::ios::binary)
+		.write(reinterpret_cast<const char *>(h.data()), (std::streamsize)(h.size() * 8));
+}
+
 // work vectors copy as aliases: every repeat reuses the same device buffers
 // (and so the solver graphs captured for them)
 template<class F, class S, class W>
@@ -115,15 +129,43 @@ int run(const ti::bdf::settings & ts, F op, S solver, const W & work, dvec & u, 
 	fsx_check(fsx_sync(ctx), "sync");
 	const double wall =
 		std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-	if (out) {
-		std::vector<double> h(u.data.size());
-		fsx_check(fsx_vec_download(ctx, u.handle(), h.data(), h.size()), "download");
-		std::ofstream(out, std::ios::binary)
-			.write(reinterpret_cast<const char *>(h.data()), (std::streamsize)(h.size() * 8));
-	}
+	write_solution(u, out, ctx);
 	std::printf("heat n %d steps %d rejects %d solves %ld iters %ld time %.17g wall %.6f\n", n,
 	            t.get_current_step(), t.num_step_rejects(), solver.get().solves,
 	            solver.get().iters, t.get_current_time(), wall);
+	return 0;
+}
+
+// the explicit example (examples/heat_equation/explicit.cc:18-48): RK23
+// with the [time-integrator] section of its explicit.cfg, f(u) = F u
+int run_explicit(const char * cfg, const heat_problem & p, int n, bool quiet,
+                 const char * out, fsx_ctx * ctx, const mat::device_csr & Fm) {
+	op::core<mat::device_csr> F(Fm);
+	dvec u(ctx, p.u0.size()), unew(ctx, p.u0.size());
+	fsx_check(fsx_vec_upload(ctx, u.handle(), p.u0.data(), p.u0.size()), "upload");
+	ti::rk23::integrator t(ti::rk23::parameters(
+		read_config(cfg, ti::rk23::options("time-integrator")), op::ref(F),
+		ti::rk23::make_work(u)));
+	const auto t0 = std::chrono::steady_clock::now();
+	auto dt = t.get_current_dt();
+	while (t.get_current_time() < t.get_final_time()) {
+		t.advance(dt, u, unew);
+		const bool good = t.check_solution();
+		if (good) {
+			if (!quiet)
+				std::printf("Step %d advanced %gs to time %gs\n", t.get_current_step(), dt,
+				            t.get_current_time());
+			t.update();
+			std::swap(u, unew);
+		}
+		dt = t.get_next_dt(good);
+	}
+	fsx_check(fsx_sync(ctx), "sync");
+	const double wall =
+		std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	write_solution(u, out, ctx);
+	std::printf("heat n %d steps %d rejects %d solves 0 iters 0 time %.17g wall %.6f\n", n,
+	            t.get_current_step(), t.num_step_rejects(), t.get_current_time(), wall);
 	return 0;
 }
 
@@ -132,18 +174,20 @@ int run(const ti::bdf::settings & ts, F op, S solver, const W & work, dvec & u, 
 int main(int argc, char ** argv) {
 	if (argc < 3) {
 		std::fprintf(stderr,
-		             "usage: heat_implicit <cfg> <n> [--alpha A] [--fused] [--out file] [--quiet]\n");
+		             "usage: heat_implicit <cfg> <n> [--alpha A] [--fused | --explicit] "
+		             "[--out file] [--quiet] [--repeat R]\n");
 		return 2;
 	}
 	const char * cfg = argv[1];
 	const int n = std::atoi(argv[2]);
 	double alpha = 1.0;
-	bool fused = false, quiet = false;
+	bool fused = false, quiet = false, expl = false;
 	const char * out = nullptr;
 	int repeat = 1;
 	for (int i = 3; i < argc; ++i) {
 		if (!std::strcmp(argv[i], "--alpha") && i + 1 < argc) alpha = std::atof(argv[++i]);
 		else if (!std::strcmp(argv[i], "--fused")) fused = true;
+		else if (!std::strcmp(argv[i], "--explicit")) expl = true;
 		else if (!std::strcmp(argv[i], "--quiet")) quiet = true;
 		else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out = argv[++i];
 		else if (!std::strcmp(argv[i], "--repeat") && i + 1 < argc) repeat = std::atoi(argv[++i]);
@@ -154,8 +198,6 @@ int main(int argc, char ** argv) {
 	}
 	try {
 		if (n < 1) throw std::invalid_argument("n must be positive");
-		auto [ts, ss] = read_config(cfg, ti::bdf::options("time-integrator"),
-		                            krylov_factory::options("linear-solver"));
 		const auto p = make_heat(n, alpha);
 		fsx_ctx * ctx;
 		fsx_check(fsx_ctx_create(0, &ctx), "ctx");
@@ -164,6 +206,9 @@ int main(int argc, char ** argv) {
 		                         FSX_I32, p.rp.data(), p.ci.data(), FSX_F64, p.f.data(), &Fh),
 		          "csr");
 		const auto Fm = mat::device_csr::adopt(ctx, Fh);
+		if (expl) return run_explicit(cfg, p, n, quiet, out, ctx, Fm);
+		auto [ts, ss] = read_config(cfg, ti::bdf::options("time-integrator"),
+		                            krylov_factory::options("linear-solver"));
 		dvec u(ctx, p.u0.size()), unew(ctx, p.u0.size());
 		const auto work = ti::bdf::make_work(u);
 		auto solver = op::make_shared<counted_solver>();

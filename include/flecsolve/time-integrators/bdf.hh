@@ -302,7 +302,7 @@ protected:
 			break;
 		}
 		default: {
-			double a[6], a0;
+			double a[6] = {}, a0 = 0;
 			bdf_coefficients(m, h, a, a0);
 			f.scale(a[0], hist(0));
 			for (int i = 1; i < memory_size(m); ++i) f.axpy(a[i], hist(i), f);

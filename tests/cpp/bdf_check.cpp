@@ -1,6 +1,7 @@
 // C++ check of the time-integrator headers (include/flecsolve/time-integrators)
-// written the way the reference's own test uses its integrator
-// (flecsolve/time-integrators/test/implicit.cc:27-135): the rate problem
+// written the way the reference's own tests use its integrators
+// (flecsolve/time-integrators/test/implicit.cc:27-135, explicit.cc:16-101):
+// RK23 / RK45 on u' = lambda u, and the rate problem
 // u' = lambda u with a direct stage solver, configured from the reference's
 // implicit.cfg (bdf-2, bdf-5 sections), and the 2D heat equation of the
 // reference's heat example (examples/heat_equation/implicit.cc) with its
@@ -12,6 +13,10 @@
 //   bdf_check rate <cfg> <lambda> <ic> <n>
 //       runs the bdf-2 and bdf-5 sections; prints per section
 //       "<section> steps S rejects R time T value V" (V = max of u, as %a)
+//   bdf_check explicit <cfg> <lambda> <ic> <n>
+//       the reference's explicit test (time-integrators/test/explicit.cc):
+//       RK23 and RK45 from the fixed and variable sections; prints
+//       "<rk23|rk45>-<fixed|variable> steps S rejects R time T value V"
 //   bdf_check heat <cfg> <n> <alpha> <out.bin>
 //       heat on an n x n interior grid of (0,10)^2, u0 = 50 on [4,6]^2;
 //       prints "heat steps S rejects R time T solves K iters I" and writes
@@ -29,6 +34,7 @@
 #include "flecsolve/solvers/factory.hh"
 #include "flecsolve/time-integrators/bdf.hh"
 #include "flecsolve/time-integrators/operator_adapter.hh"
+#include "flecsolve/time-integrators/rk45.hh"
 #include "fsx_problems.h"
 
 #ifdef BDF_HOST
@@ -174,6 +180,55 @@ static int run_rate(backend & be, const char * cfg, double lambda, double ic, st
 	return 0;
 }
 
+// time-integrators/test/explicit.cc:16-23: f(x) = lambda x
+struct explicit_rate_parameters {
+	double lambda;
+};
+struct explicit_rate : op::base<explicit_rate_parameters> {
+	explicit explicit_rate(double l) : op::base<explicit_rate_parameters>(explicit_rate_parameters{l}) {}
+	template<class D, class R>
+	void apply(const D & x, R & y) const { y.scale(params.lambda, x); }
+};
+
+// explicit.cc:45-101: the variable and fixed sections for RK23 and RK45
+static int run_explicit(backend & be, const char * cfg, double lambda, double ic,
+                        std::size_t n) {
+	op::core<explicit_rate> F(lambda);
+	auto x = be.make(n), xnew = be.make(n);
+	auto [t23v, t23f] = std::apply(
+		[&](auto &&... s) {
+			return std::make_tuple(
+				ti::rk23::integrator(ti::rk23::parameters(s, op::ref(F), ti::rk23::make_work(x)))...);
+		},
+		read_config(cfg, ti::rk23::options("variable"), ti::rk23::options("fixed")));
+	auto [t45v, t45f] = std::apply(
+		[&](auto &&... s) {
+			return std::make_tuple(
+				ti::rk45::integrator(ti::rk45::parameters(s, op::ref(F), ti::rk45::make_work(x)))...);
+		},
+		read_config(cfg, ti::rk45::options("variable"), ti::rk45::options("fixed")));
+	auto run = [&](auto & t, const char * name) {
+		x.set_scalar(ic);
+		auto dt = t.get_current_dt();
+		while (t.get_current_time() < t.get_final_time()) {
+			t.advance(dt, x, xnew);
+			const bool good = t.check_solution();
+			if (good || t.fixed_dt()) {
+				t.update();
+				std::swap(x, xnew);
+			}
+			dt = t.get_next_dt(good);
+		}
+		std::printf("%s steps %d rejects %d time %.17g value %a\n", name, t.get_current_step(),
+		            t.num_step_rejects(), t.get_current_time(), (double)x.max().get());
+	};
+	run(t23f, "rk23-fixed");
+	run(t45f, "rk45-fixed");
+	run(t23v, "rk23-variable");
+	run(t45v, "rk45-variable");
+	return 0;
+}
+
 static int run_heat(backend & be, const char * cfg, int n, double alpha, const char * out) {
 	std::vector<int32_t> rp((std::size_t)n * n + 1), ci(fsx_gen_poisson2d_nnz(n));
 	std::vector<double> v(ci.size());
@@ -211,9 +266,13 @@ int main(int argc, char ** argv) {
 		if (argc == 6 && !std::strcmp(argv[1], "rate"))
 			return run_rate(be, argv[2], std::atof(argv[3]), std::atof(argv[4]),
 			                (std::size_t)std::atol(argv[5]));
+		if (argc == 6 && !std::strcmp(argv[1], "explicit"))
+			return run_explicit(be, argv[2], std::atof(argv[3]), std::atof(argv[4]),
+			                    (std::size_t)std::atol(argv[5]));
 		if (argc == 6 && !std::strcmp(argv[1], "heat"))
 			return run_heat(be, argv[2], std::atoi(argv[3]), std::atof(argv[4]), argv[5]);
 		std::fprintf(stderr, "usage: bdf_check rate <cfg> <lambda> <ic> <n>\n"
+		                     "       bdf_check explicit <cfg> <lambda> <ic> <n>\n"
 		                     "       bdf_check heat <cfg> <n> <alpha> <out.bin>\n");
 		return 2;
 	}

@@ -1,7 +1,7 @@
 """The C++ time-integrator headers (include/flecsolve/time-integrators)
 over host vectors against the oracle (no device use): tests/cpp/bdf_check_host
-runs the reference's rate test (bit-identical to the oracle: same steps,
-rejections and final value) and the heat example with krylov_factory CG
+runs the reference's rate test and its explicit RK23 / RK45 test
+(bit-identical to the oracle: same steps, rejections and final value) and the heat example with krylov_factory CG
 stage solves (same steps, rejections, solves and CG iterations; solution
 within 1e-12), and rejects malformed integrator configs."""
 import subprocess
@@ -27,6 +27,11 @@ def run(args, ok=True):
 def test_rate_bit_identical():
     out = run(["rate", BC.GOLDEN / "bdf_rate.cfg", -1.0, 3.0, 5]).stdout
     assert BC.parse_rate(out) == BC.oracle_rate(-1.0, 3.0, 5)
+
+
+def test_explicit_rk_bit_identical():
+    out = run(["explicit", BC.GOLDEN / "rk_rate.cfg", -1.0, 3.0, 1]).stdout
+    assert BC.parse_explicit(out) == BC.oracle_explicit(-1.0, 3.0, 1)
 
 
 def test_heat_generic_cg(tmp_path):

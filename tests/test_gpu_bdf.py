@@ -1,14 +1,17 @@
 """The time integrator over device vectors against the oracle (gpu).
 
-  * the reference's rate test (time-integrators/test/implicit.cc) through
-    tests/cpp/bdf_check on device vectors: bit-identical to the oracle (same
-    steps, rejections and final value; every norm here is exact);
+  * the reference's rate test (time-integrators/test/implicit.cc) and its
+    explicit RK23 / RK45 test (explicit.cc) through tests/cpp/bdf_check on
+    device vectors: bit-identical to the oracle (same steps, rejections and
+    final value; every norm here is exact);
   * the heat example (flecsolve_amd/heat_implicit and bdf_check heat):
     generic path (operator_adapter + krylov_factory CG, a host round trip
     per reduction) and fused path (csr_adapter stage matrix + graph-captured
     CG or BiCGSTAB): the oracle's steps, rejections, stage solves and Krylov
     iterations, final solution within 1e-10 (device reductions sum in
     another order than the oracle's);
+  * the explicit heat example (RK23, explicit.cfg): the oracle's steps and
+    rejections, solution within 1e-10;
   * fsx_csr_shift_scale: I - gamma F bit for bit on every SpMV layout, and
     the missing-diagonal error.
 """
@@ -41,6 +44,12 @@ def test_rate_bit_identical():
     assert BC.parse_rate(out) == BC.oracle_rate(-1.0, 3.0, 1000)
 
 
+def test_explicit_rk_bit_identical():
+    # one-cell vectors, as the reference's explicit test uses: every norm exact
+    out = run([CHECK, "explicit", BC.GOLDEN / "rk_rate.cfg", -1.0, 3.0, 1])
+    assert BC.parse_explicit(out) == BC.oracle_explicit(-1.0, 3.0, 1)
+
+
 def test_heat_generic_check(tmp_path):
     n = 48
     f = tmp_path / "u.bin"
@@ -63,6 +72,19 @@ def test_heat_example(tmp_path, kind, fused):
     ref, u = BC.oracle_heat(n, alpha, cfg)
     print(out.strip(), "oracle", ref)
     assert BC.parse_heat(out) == ref
+    assert np.abs(np.fromfile(f) - u).max() <= 1e-10 * np.abs(u).max()
+
+
+@pytest.mark.parametrize("n", [48, 98])
+def test_heat_explicit_example(tmp_path, n):
+    """The reference's explicit example (RK23, explicit.cfg) on device
+    vectors: the oracle's steps and rejections; solution within 1e-10 (the
+    error norm is a device reduction, so dt can differ in the last bits)."""
+    f = tmp_path / "u.bin"
+    out = run([HEAT, BC.GOLDEN / "heat_explicit.cfg", n, "--explicit", "--out", f, "--quiet"])
+    ref, u = BC.oracle_heat_explicit(n, 1.0)
+    print(out.strip(), "oracle", ref)
+    assert BC.parse_heat(out)[:2] == ref
     assert np.abs(np.fromfile(f) - u).max() <= 1e-10 * np.abs(u).max()
 
 
