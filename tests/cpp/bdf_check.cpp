@@ -13,6 +13,8 @@
 //   bdf_check rate <cfg> <lambda> <ic> <n>
 //       runs the bdf-2 and bdf-5 sections; prints per section
 //       "<section> steps S rejects R time T value V" (V = max of u, as %a)
+//   bdf_check rateone <cfg> <section> <lambda> <ic> <n>
+//       the rate problem with the file's only section; prints one such line
 //   bdf_check explicit <cfg> <lambda> <ic> <n>
 //       the reference's explicit test (time-integrators/test/explicit.cc):
 //       RK23 and RK45 from the fixed and variable sections; prints
@@ -180,6 +182,22 @@ static int run_rate(backend & be, const char * cfg, double lambda, double ic, st
 	return 0;
 }
 
+// one configuration file holding one section: that integrator on the rate
+// problem (the other branches of the integrator, against the oracle)
+static int run_rate_one(backend & be, const char * cfg, const char * section, double lambda,
+                        double ic, std::size_t n) {
+	auto F = op::make_shared<rate>(lambda);
+	auto x = be.make(n), xnew = be.make(n);
+	auto solver = op::make_shared<rate_solver>(F);
+	auto s = read_config(cfg, ti::bdf::options(section));
+	ti::bdf::integrator t(ti::bdf::parameters(s, F, ti::bdf::make_work(x), solver));
+	x.set_scalar(ic);
+	integrate(t, x, xnew);
+	std::printf("%s steps %d rejects %d time %.17g value %a\n", section, t.get_current_step(),
+	            t.num_step_rejects(), t.get_current_time(), (double)x.max().get());
+	return 0;
+}
+
 // time-integrators/test/explicit.cc:16-23: f(x) = lambda x
 struct explicit_rate_parameters {
 	double lambda;
@@ -266,12 +284,16 @@ int main(int argc, char ** argv) {
 		if (argc == 6 && !std::strcmp(argv[1], "rate"))
 			return run_rate(be, argv[2], std::atof(argv[3]), std::atof(argv[4]),
 			                (std::size_t)std::atol(argv[5]));
+		if (argc == 7 && !std::strcmp(argv[1], "rateone"))
+			return run_rate_one(be, argv[2], argv[3], std::atof(argv[4]), std::atof(argv[5]),
+			                    (std::size_t)std::atol(argv[6]));
 		if (argc == 6 && !std::strcmp(argv[1], "explicit"))
 			return run_explicit(be, argv[2], std::atof(argv[3]), std::atof(argv[4]),
 			                    (std::size_t)std::atol(argv[5]));
 		if (argc == 6 && !std::strcmp(argv[1], "heat"))
 			return run_heat(be, argv[2], std::atoi(argv[3]), std::atof(argv[4]), argv[5]);
 		std::fprintf(stderr, "usage: bdf_check rate <cfg> <lambda> <ic> <n>\n"
+		                     "       bdf_check rateone <cfg> <section> <lambda> <ic> <n>\n"
 		                     "       bdf_check explicit <cfg> <lambda> <ic> <n>\n"
 		                     "       bdf_check heat <cfg> <n> <alpha> <out.bin>\n");
 		return 2;

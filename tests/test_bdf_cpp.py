@@ -1,7 +1,9 @@
 """The C++ time-integrator headers (include/flecsolve/time-integrators)
 over host vectors against the oracle (no device use): tests/cpp/bdf_check_host
-runs the reference's rate test and its explicit RK23 / RK45 test
-(bit-identical to the oracle: same steps, rejections and final value) and the heat example with krylov_factory CG
+runs the reference's rate test and its explicit RK23 / RK45 test, and the
+rate problem under every method, predictor, controller, step strategy and
+error scaling (bit-identical to the oracle: same steps, rejections and final
+value) and the heat example with krylov_factory CG
 stage solves (same steps, rejections, solves and CG iterations; solution
 within 1e-12), and rejects malformed integrator configs."""
 import subprocess
@@ -11,6 +13,8 @@ import numpy as np
 import pytest
 
 import bdf_common as BC
+
+MATRIX = BC.integrator_matrix()
 
 HOST = Path(__file__).resolve().parent / "cpp" / "bdf_check_host"
 
@@ -67,3 +71,11 @@ def test_config_bad_values(tmp_path):
         cfg.write_text(text.replace(old, new, 1))
         r = run(["rate", cfg, -1.0, 3.0, 2], ok=False)
         assert r.returncode == 1 and msg in r.stderr, (new, r.stderr)
+
+
+@pytest.mark.parametrize("name", sorted(MATRIX))
+def test_integrator_branches_bit_identical(tmp_path, name):
+    cfg = tmp_path / "one.cfg"
+    BC.write_section(cfg, name, MATRIX[name])
+    out = run(["rateone", cfg, name, -1.0, 3.0, 4]).stdout
+    assert BC.parse_rate_one(out, name) == BC.oracle_rate_one(name, MATRIX[name], -1.0, 3.0, 4)

@@ -3,7 +3,8 @@
   * the reference's rate test (time-integrators/test/implicit.cc) and its
     explicit RK23 / RK45 test (explicit.cc) through tests/cpp/bdf_check on
     device vectors: bit-identical to the oracle (same steps, rejections and
-    final value; every norm here is exact);
+    final value; every norm here is exact); likewise the rate problem under
+    every method, predictor, controller, step strategy and error scaling;
   * the heat example (flecsolve_amd/heat_implicit and bdf_check heat):
     generic path (operator_adapter + krylov_factory CG, a host round trip
     per reduction) and fused path (csr_adapter stage matrix + graph-captured
@@ -42,6 +43,19 @@ def run(args):
 def test_rate_bit_identical():
     out = run([CHECK, "rate", BC.GOLDEN / "bdf_rate.cfg", -1.0, 3.0, 1000])
     assert BC.parse_rate(out) == BC.oracle_rate(-1.0, 3.0, 1000)
+
+
+MATRIX = BC.integrator_matrix()
+
+
+@pytest.mark.parametrize("name", sorted(MATRIX))
+def test_integrator_branches_bit_identical(tmp_path, name):
+    """Every method, predictor, controller, step strategy and error scaling
+    on device vectors of length 4 (inf and l2 norms exact)."""
+    cfg = tmp_path / "one.cfg"
+    BC.write_section(cfg, name, MATRIX[name])
+    out = run([CHECK, "rateone", cfg, name, -1.0, 3.0, 4])
+    assert BC.parse_rate_one(out, name) == BC.oracle_rate_one(name, MATRIX[name], -1.0, 3.0, 4)
 
 
 def test_explicit_rk_bit_identical():
