@@ -432,9 +432,18 @@ static fsx_vec * pooled(fsx_ctx * ctx, size_t n, fsx_dtype dt, int slot) {
 
 // Runs the iteration loop on the device.  `st_host` holds the initial state
 // and receives the final one.
+// Device scalar slots of a CG solve's setup reductions (|b|^2, |x|^2, |r|^2,
+// rho, final |x|^2) and where the host reads them (pinned, past the state
+// copies of the iteration loop).
+constexpr int kCgSetupSlot = 16;
+constexpr int kCgHostSlot = 40;
+static_assert(kCgSetupSlot + 5 <= kScalarSlots && kCgHostSlot + 5 <= kScalarSlots, "");
+static_assert(sizeof(cg_state) * 3 <= sizeof(double) * kCgHostSlot,
+              "setup values must not overlap the loop's pinned state copies");
+
 template<class T, class V, class I>
 static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
-                        double * hist) {
+                        double * hist, const fsx_vec * xv, double * x2_out) {
 	auto & cache = cache_of(ctx);
 	cg_state * st = (cg_state *)cache.statebuf->p;
 	cg_state * pinned = (cg_state *)ctx->host_scalars; // pinned host space
@@ -558,10 +567,18 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 	}
 	FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(cg_state), hipMemcpyDeviceToHost,
 	                       ctx->stream));
+	// |x|^2 of the final x, read with the same synchronisation
+	double * x2slot = ctx->ws.scalars + kCgSetupSlot + 4;
+	if (xv && B.n > 0) {
+		reduce_async(ctx, RED_L2SQ, xv, nullptr, x2slot);
+		FSX_HIP(hipMemcpyAsync(ctx->host_scalars + kCgHostSlot + 4, x2slot, sizeof(double),
+		                       hipMemcpyDeviceToHost, ctx->stream));
+	}
 	FSX_HIP(hipStreamSynchronize(ctx->stream));
 	(void)hipEventDestroy(ev[0]);
 	(void)hipEventDestroy(ev[1]);
 	std::memcpy(&st_host, pinned, sizeof(cg_state));
+	if (x2_out) *x2_out = (xv && B.n > 0) ? ctx->host_scalars[kCgHostSlot + 4] : 0.0;
 }
 
 // Shared driver for FP64 (T = double) and FP32 inner (T = float) solves.
@@ -577,27 +594,20 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 	info->status = FSX_UNKNOWN;
 	if (hist_count) *hist_count = 0;
 	const size_t n = b->n;
-	double b_norm = std::sqrt(reduce_host(ctx, RED_L2SQ, b, nullptr));
-	if (b_norm == 0.0) b_norm = 1.0;
-	const double terminate_tol = s.rtol * b_norm;
-	info->rhs_norm = (float)b_norm;
+	// The setup reductions |b|^2, |x|^2, |r|^2 and rho = z.r are enqueued
+	// into device slots and read with one synchronisation (the same kernels
+	// as one-by-one reads, so the same values); z = D^{-1} r is formed even
+	// when the initial residual already meets the tolerance.
+	double * sl = ctx->ws.scalars + kCgSetupSlot;
+	double * hs = ctx->host_scalars + kCgHostSlot;
 	if (s.use_zero_guess) {
-		info->sol_norm_initial = 0;
 		launch_set(ctx, 0., x);
 		launch_copy(ctx, b, r);
 	}
 	else {
-		info->sol_norm_initial =
-			(float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
+		if (n) reduce_async(ctx, RED_L2SQ, x, nullptr, sl + 1);
 		int rc = fsx_csr_residual(ctx, A, b, x, r);
 		if (rc) throw error(rc, get_error());
-	}
-	double current_res = std::sqrt(reduce_host(ctx, RED_L2SQ, r, nullptr));
-	if (current_res < terminate_tol) {
-		info->res_norm_initial = (float)current_res;
-		info->res_norm_final = (float)current_res;
-		info->status = FSX_CONVERGED_RTOL;
-		return;
 	}
 	fsx_vec * zz = r; // op::I: z is a copy of r, so read r directly
 	if (dinv) {
@@ -605,13 +615,39 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 		if (rc) throw error(rc, get_error());
 		zz = z;
 	}
+	double b2 = 0, x2 = 0, r2 = 0, rho = 0;
+	if (n) {
+		reduce_async(ctx, RED_L2SQ, b, nullptr, sl + 0);
+		reduce_async(ctx, RED_L2SQ, r, nullptr, sl + 2);
+		reduce_async(ctx, RED_DOT, zz, r, sl + 3);
+		FSX_HIP(hipMemcpyAsync(hs, sl, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->host_syncs++;
+		b2 = hs[0];
+		x2 = hs[1];
+		r2 = hs[2];
+		rho = hs[3];
+	}
+	double b_norm = std::sqrt(b2);
+	if (b_norm == 0.0) b_norm = 1.0;
+	const double terminate_tol = s.rtol * b_norm;
+	info->rhs_norm = (float)b_norm;
+	info->sol_norm_initial = s.use_zero_guess ? 0.f : (float)std::sqrt(x2);
+	double current_res = std::sqrt(r2);
+	if (current_res < terminate_tol) {
+		info->res_norm_initial = (float)current_res;
+		info->res_norm_final = (float)current_res;
+		info->status = FSX_CONVERGED_RTOL;
+		return;
+	}
 	cg_state st{};
-	st.rho1 = reduce_host(ctx, RED_DOT, zz, r);
+	st.rho1 = rho;
 	st.rho0 = st.rho1;
 	st.tol = terminate_tol;
 	st.res = current_res;
 	st.maxiter = s.maxiter;
 	launch_copy(ctx, zz, p);
+	double xf2 = 0;
 	if (s.maxiter > 0) {
 		double * hist = history_buffer(ctx, s.maxiter);
 		cg_bufs B{A, x->ptr(), r->ptr(), zz->ptr(), p->ptr(), p2->ptr(), w->ptr(),
@@ -619,15 +655,17 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 		by_matrix_type(A, [&](auto * vtag, auto * itag) {
 			using V = std::remove_pointer_t<decltype(vtag)>;
 			using I = std::remove_pointer_t<decltype(itag)>;
-			run_cg_loop<T, V, I>(ctx, B, st, hist);
+			run_cg_loop<T, V, I>(ctx, B, st, hist, x, &xf2);
 		});
 		current_res = st.res;
 		info->iters = st.iters;
 		if (st.iters) info->status = st.status;
 		if (hist_count) *hist_count = st.iter;
 	}
+	else
+		xf2 = reduce_host(ctx, RED_L2SQ, x, nullptr);
 	info->res_norm_final = (float)current_res;
-	info->sol_norm_final = (float)std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr));
+	info->sol_norm_final = (float)std::sqrt(xf2);
 	if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
 }
 
