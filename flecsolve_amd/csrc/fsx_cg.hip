@@ -275,7 +275,7 @@ struct cg_graph_cache {
 		hipGraph_t g = nullptr;
 		hipGraphExec_t ge = nullptr;
 	};
-	// key: the driver, every device address a captured launch uses and the
+	// key: the iteration function, every device address a captured launch uses and the
 	// sizes and plan its launch shapes follow from
 	std::map<std::vector<const void *>, entry> mv;
 	std::map<std::tuple<size_t, int, int>, fsx_vec *> pool;

@@ -663,7 +663,7 @@ class Integrator:
 
 
 def run(ti: Integrator, u0: np.ndarray, on_step=None):
-    """The driver loop of the reference's test and heat example
+    """The time loop of the reference's test and heat example
     (time-integrators/test/implicit.cc:101-114; examples/heat_equation/
     implicit.cc:41-55).  Returns the final solution."""
     x = np.array(u0, dtype=np.float64, copy=True)

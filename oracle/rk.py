@@ -196,7 +196,7 @@ class RK45(_Base):
 
 
 def run(ti, u0):
-    """The driver loop of time-integrators/test/explicit.cc:71-80."""
+    """The time loop of time-integrators/test/explicit.cc:71-80."""
     x = np.array(u0, dtype=np.float64, copy=True)
     xnew = np.zeros_like(x)
     dt = ti.get_current_dt()

@@ -143,7 +143,7 @@ struct counting : op::base<> {
 	mutable long solves = 0;
 };
 
-// the driver loop of implicit.cc:101-114 and examples/heat_equation/implicit.cc:41-55
+// the time loop of implicit.cc:101-114 and examples/heat_equation/implicit.cc:41-55
 template<class TI, class V>
 void integrate(TI & t, V & x, V & xnew) {
 	auto dt = t.get_current_dt();
