@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--cpu-sample-systems", type=int, default=16)
     ap.add_argument("--heat-n", type=int, default=2046)
     ap.add_argument("--heat-cpu-n", type=int, default=638)
+    ap.add_argument("--heat-spmv-only", action="store_true",
+                    help="heat workload: time the stage-matrix SpMV only (profiling runs)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -851,29 +853,11 @@ def heat_runs(n, fused, repeat, cfg):
     return runs
 
 
-def bench_heat(a, fsx, P):
-    """The reference's implicit heat example (examples/heat_equation/
-    implicit.cc with its implicit.cfg) integrated to t = 0.1: the BDF
-    integrator over device vectors with the fused stage path (csr_adapter +
-    graph-captured CG), the generic path beside it, the stage SpMV against
-    the HBM roofline, and the oracle (numpy integrator + C CG, one thread)
-    on a smaller grid as the CPU baseline."""
-    cfg = ROOT / "tests" / "golden" / "heat_implicit.cfg"
-    n = a.heat_n
+def heat_stage_spmv(a, fsx, P, n, quiet=False):
+    """The stage matrix I - gamma F of the heat example on n^2 (gamma of a
+    mid-run BDF2 step) and its SpMV timed with HIP events on the library
+    stream; returns (ms, GB/s, algorithmic bytes, plan, nnz)."""
     N = n * n
-    reps = a.warmup + a.steps
-    fused = heat_runs(n, True, reps, cfg)
-    generic = heat_runs(n, False, reps, cfg)
-    for name, runs in (("fused", fused), ("generic", generic)):
-        for i, r in enumerate(runs):
-            log(f"heat {name} {'warmup' if i < a.warmup else 'step'} {i}: {r[4]:.3f} s, "
-                f"{r[0]} steps, {r[1]} rejects, {r[2]} solves, {r[3]} CG iterations")
-    tf = [r[4] for r in fused[a.warmup:]]
-    tg = [r[4] for r in generic[a.warmup:]]
-    fs, gs = sum(tf) / len(tf), sum(tg) / len(tg)
-    steps, rejects, solves, iters = fused[-1][:4]
-
-    # stage matrix I - gamma F (gamma of a mid-run BDF2 step) and its SpMV
     rp, ci, v = P.poisson2d(n)
     h = 10.0 / (n + 1)
     fv = -(1.0 / (h * h)) * v
@@ -889,6 +873,47 @@ def bench_heat(a, fsx, P):
     gbs = B / (ms * 1e-3) / 1e9
     plan = A.plan()
     log(f"stage spmv {n}^2: {ms:.4f} ms, {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.1%}), plan {plan}")
+    if not quiet:
+        # the copy kernel beside it calibrates the PMC byte counters
+        copy_ms = fsx_time(fsx, "fsx_time_copy", ctx, None, xd, y, a.spmv_reps)
+        log(f"copy kernel {N} doubles: {copy_ms:.4f} ms")
+        print(json.dumps({"metric": "stage SpMV % HBM peak", "config": {
+            "workload": f"heat2d stage matrix {n}^2", "rows": N, "nnz": nnz},
+            "avg_ms": round(ms, 5), "achieved": round(gbs, 1), "algorithmic_bytes": B}),
+            flush=True)
+    return ms, gbs, B, plan, nnz
+
+
+def bench_heat(a, fsx, P):
+    """The reference's implicit heat example (examples/heat_equation/
+    implicit.cc with its implicit.cfg) integrated to t = 0.1: the BDF
+    integrator over device vectors with the fused stage path (csr_adapter +
+    graph-captured CG), the generic path beside it, the stage SpMV against
+    the HBM roofline, and the oracle (numpy integrator + C CG, one thread)
+    on a smaller grid as the CPU baseline."""
+    cfg = ROOT / "tests" / "golden" / "heat_implicit.cfg"
+    n = a.heat_n
+    N = n * n
+    if a.heat_spmv_only:
+        return heat_stage_spmv(a, fsx, P, n)
+    reps = a.warmup + a.steps
+    fused = heat_runs(n, True, reps, cfg)
+    generic = heat_runs(n, False, reps, cfg)
+    for name, runs in (("fused", fused), ("generic", generic)):
+        for i, r in enumerate(runs):
+            log(f"heat {name} {'warmup' if i < a.warmup else 'step'} {i}: {r[4]:.3f} s, "
+                f"{r[0]} steps, {r[1]} rejects, {r[2]} solves, {r[3]} CG iterations")
+    tf = [r[4] for r in fused[a.warmup:]]
+    tg = [r[4] for r in generic[a.warmup:]]
+    fs, gs = sum(tf) / len(tf), sum(tg) / len(tg)
+    steps, rejects, solves, iters = fused[-1][:4]
+
+    ms, gbs, B, plan, nnz = heat_stage_spmv(a, fsx, P, n, quiet=True)
+    traffic, traffic_src = None, None
+    pmc = ROOT / "profiles" / "spmv_stage_heat_2046sq_pmc.json"
+    if n == 2046 and pmc.exists():
+        traffic = round(json.loads(pmc.read_text())["traffic_bytes"])
+        traffic_src = "profiles/spmv_stage_heat_2046sq_pmc.md"
     out = {
         "metric": "heat example (BDF2/CN, CG stages) time to t = 0.1",
         "value": round(fs, 4),
@@ -910,8 +935,9 @@ def bench_heat(a, fsx, P):
                    "fused_speedup_vs_generic": round(gs / fs, 3)},
         "roofline": {"bound": "hbm", "kernel": "stage-matrix SpMV (I - gamma F), plan "
                      f"{plan[0]}", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes": B, "avg_ms": round(ms, 5)},
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src, "algorithmic_bytes": B,
+                     "avg_ms": round(ms, 5)},
     }
     if not a.no_cpu_baseline:
         import oracle.bdf as OB
