@@ -395,8 +395,11 @@ def bench_batched(a, fsx, P):
         sys.exit(f"{int((st64 != 1).sum())} systems did not converge (FP64)")
     rel64 = true_rel()
     x64 = x.get()
+    if a.ir_jacobi == 2:
+        A32.set_values_sym_scaled(A64)  # setup, like the FP32 conversion
     (oit, iit, stir, _), tir = timed(
-        lambda: B.ir_cg(A32, bd, x, 1e-10, inner, max_outer=a.max_outer), "ir batched cg")
+        lambda: B.ir_cg(A32, bd, x, 1e-10, inner, jacobi=a.ir_jacobi, max_outer=a.max_outer),
+        "ir batched cg")
     if not (stir == 1).all():
         sys.exit(f"{int((stir != 1).sum())} systems did not reach 1e-10 (IR)")
     relir = true_rel()
@@ -429,7 +432,10 @@ def bench_batched(a, fsx, P):
             "fp64_cg": {"seconds": round(t64, 4), "iters_min": int(it64.min()),
                         "iters_max": int(it64.max()), "iters_sum": int(it64.sum()),
                         "max_true_rel_residual": float(rel64.max())},
-            "ir": {"seconds": round(tir, 4), "outer_max": int(oit.max()),
+            "ir": {"seconds": round(tir, 4),
+                   "inner_preconditioning": ("symmetric Jacobi scaling of the FP32 matrix"
+                                             if a.ir_jacobi == 2 else "Jacobi"),
+                   "outer_max": int(oit.max()),
                    "inner_sum": int(iit.sum()),
                    "max_true_rel_residual": float(relir.max()),
                    "max_rel_err_vs_fp64": err_ir},

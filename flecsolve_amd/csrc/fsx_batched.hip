@@ -76,6 +76,7 @@ struct batch_args {
 	double * x;
 	const double * dinv64;
 	const float * dinv32;
+	const double * sym; // refinement on S A S, S = D^{-1/2} (null: unscaled A32)
 	double * r, * z, * p, * w;          // FP64 work
 	float * fr, * fz, * fp, * fw, * fd; // FP32 inner work
 	float * fb;                         // FP32 inner right-hand side
@@ -448,8 +449,12 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batched_cg_kernel(batch_args
 					break;
 				}
 				const double inv = 1.0 / res;
-				for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-					a.fb[r0 + i] = (float)(r[i] * inv);
+				if (a.sym)
+					for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+						a.fb[r0 + i] = (float)(a.sym[r0 + i] * (r[i] * inv));
+				else
+					for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+						a.fb[r0 + i] = (float)(r[i] * inv);
 				__syncthreads();
 				int it_in = 0, st_in = 0;
 				double res_in = 0.0;
@@ -459,8 +464,12 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batched_cg_kernel(batch_args
 				                         a.dinv32 + r0, a.maxiter, a.rtol, true, lds,
 				                         it_in, st_in, res_in);
 				inner += it_in ? it_in : a.maxiter;
-				for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
-					x[i] = x[i] + res * (double)a.fd[r0 + i];
+				if (a.sym)
+					for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+						x[i] = x[i] + res * (a.sym[r0 + i] * (double)a.fd[r0 + i]);
+				else
+					for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+						x[i] = x[i] + res * (double)a.fd[r0 + i];
 				__syncthreads();
 				++outer;
 			}
@@ -561,7 +570,7 @@ batch_args base_args(fsx_batch * B, const fsx_csr * A32) {
 
 // Work vectors pooled on the batch (stable addresses; sized to the total rows).
 struct batch_work {
-	std::unique_ptr<dbuf> r, z, p, w, fr, fz, fp, fw, fd, fb, dinv32;
+	std::unique_ptr<dbuf> r, z, p, w, fr, fz, fp, fw, fd, fb, dinv32, sym;
 };
 std::map<const fsx_batch *, batch_work> g_work;
 
@@ -766,8 +775,22 @@ int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
 		            "A32 must be fsx_csr_convert(A, FSX_F32) of the batch matrix");
 		FSX_REQUIRE(B->mapping != 0 || A32->sl_vals, "A32 lacks the sliced layout");
 		FSX_REQUIRE(B->mapping != 2 || A32->dg_vals, "A32 lacks the slice-offset layout");
+		FSX_REQUIRE(jacobi >= 0 && jacobi <= 2, "jacobi must be 0, 1 or 2");
 		auto & W = work_for(B, true);
-		if (jacobi && !W.dinv32) {
+		if (jacobi == 2) {
+			// S = D^{-1/2} of the FP64 batch matrix (A32 holds S A S, written by
+			// fsx_csr_set_values_f32_sym_scaled)
+			if (!W.sym) W.sym = std::make_unique<dbuf>(std::max<int64_t>(1, B->A->nrows) * 8);
+			dbuf flag(sizeof(int));
+			FSX_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+			sym_scale_vector(ctx, B->A, (double *)W.sym->p, (int *)flag.p);
+			int bad = 0;
+			FSX_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			ctx->host_syncs++;
+			FSX_REQUIRE(!bad, "a diagonal entry of the batch matrix is missing or not positive");
+		}
+		if (jacobi == 1 && !W.dinv32) {
 			W.dinv32 = std::make_unique<dbuf>(std::max<int64_t>(1, B->A->nrows) * 4);
 			fsx_vec * tmp = nullptr;
 			int rc = fsx_vec_wrap(ctx, W.dinv32->p, B->A->nrows, FSX_F32, &tmp);
@@ -780,7 +803,8 @@ int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
 		batch_args a = base_args(B, A32);
 		a.b = (const double *)b->ptr();
 		a.x = (double *)x->ptr();
-		a.dinv32 = jacobi ? (const float *)W.dinv32->p : nullptr;
+		a.dinv32 = jacobi == 1 ? (const float *)W.dinv32->p : nullptr;
+		a.sym = jacobi == 2 ? (const double *)W.sym->p : nullptr;
 		a.r = (double *)W.r->p;
 		a.fr = (float *)W.fr->p;
 		a.fz = (float *)W.fz->p;
@@ -798,7 +822,7 @@ int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
 		a.res = (double *)o.res->p;
 		a.inner_iters = (int *)o.inner->p;
 		if (B->nsys > 0) {
-			if (jacobi) launch<true, true>(ctx, B, a);
+			if (jacobi == 1) launch<true, true>(ctx, B, a);
 			else launch<false, true>(ctx, B, a);
 		}
 		copy_out(ctx, o, B->nsys, outer_iters, status, res, inner_iters);

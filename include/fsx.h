@@ -457,7 +457,10 @@ int fsx_batched_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_vec * dinv,
                    const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
                    int * iters, int * status, double * res);
 /* FP32-inner / FP64-outer refinement per system; A32 =
- * fsx_csr_convert(A, FSX_F32).  outer_iters/inner_iters/status/res per system. */
+ * fsx_csr_convert(A, FSX_F32).  jacobi as for fsx_ir_cg_solve (0 none, 1
+ * Jacobi-preconditioned inner CG, 2 A32 holding S A S from
+ * fsx_csr_set_values_f32_sym_scaled).  outer_iters/inner_iters/status/res per
+ * system. */
 int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
                       int jacobi, const fsx_vec * b, fsx_vec * x,
                       double rtol_outer, int max_outer, const fsx_settings * inner,

@@ -102,14 +102,18 @@ def test_batched_autotune(ctx):
     check_against_oracle(rows, rp, ci, v, b, x.get(), it, st)
 
 
+@pytest.mark.parametrize("jacobi", [1, 2])
 @pytest.mark.parametrize("sigma", [1.0, 2.5])
-def test_batched_ir_reaches_1e10(ctx, sigma):
+def test_batched_ir_reaches_1e10(ctx, sigma, jacobi):
     e, rows, rp, ci, v, b = make(32, 4, 14, sigma=sigma)
     A = ctx.csr(rp, ci, v)
     A32 = A.convert(fsx.F32)
+    if jacobi == 2:
+        A32.set_values_sym_scaled(A)
     B = fsx.Batch(A, rows)
     bd, x = ctx.vec(b), ctx.zeros(b.size)
-    outer, inner, st, rs = B.ir_cg(A32, bd, x, 1e-10, fsx.Settings(maxiter=2000, rtol=1e-4))
+    outer, inner, st, rs = B.ir_cg(A32, bd, x, 1e-10, fsx.Settings(maxiter=2000, rtol=1e-4),
+                                   jacobi=jacobi)
     assert (st == 1).all()
     S = sp.csr_matrix((v, ci, rp))
     xg = x.get()
