@@ -537,6 +537,28 @@ def bench_adjoint(a, fsx, P):
         worst = max(worst, err)
         log(f"direction {j}: fd {fd:.10e} adjoint {g @ d:.10e} rel err {err:.2e}")
     log(f"finite differences: {a.fd_directions} directions in {time.perf_counter()-t:.1f} s")
+    # the same evaluation with every solve a mixed-precision refinement on the
+    # symmetrically scaled FP32 copy of A (fsx_be_adjoint_ir), beside it
+    prob_ir = HeatProblem(ctx, n=n, dt=1e-3, steps=a.heat_steps, ckpt=a.ckpt,
+                          rtol=a.heat_rtol, refine=True)
+    ti_, res_ir = [], None
+    for i in range(a.warmup + a.steps):
+        ctx.sync()
+        t = time.perf_counter()
+        res_ir = prob_ir.loss_and_grad(theta)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        log(f"refined loss+gradient {'warmup' if i < a.warmup else 'step'} {i}: {dt:.3f} s")
+        if i >= a.warmup:
+            ti_.append(dt)
+    loss_ir, g_ir, _, st_ir = res_ir
+    t_ir = sum(ti_) / len(ti_)
+    refined = {"seconds": round(t_ir, 4), "speedup_vs_cg": round(t_grad / t_ir, 3),
+               "loss_rel_diff": abs(loss_ir - loss) / abs(loss),
+               "grad_rel_diff": float(np.linalg.norm(g_ir - g) / np.linalg.norm(g)),
+               "stats": st_ir.as_dict()}
+    log(f"refined solves: {t_ir:.3f} s, loss rel diff {refined['loss_rel_diff']:.2e}, "
+        f"gradient rel diff {refined['grad_rel_diff']:.2e}")
     iters = st["forward_iters"] + st["recompute_iters"] + st["adjoint_iters"]
     N, nnz = prob.N, prob.A.nnz
     gbs = iters * cg_iter_bytes(N, nnz) / t_grad / 1e9
@@ -561,6 +583,7 @@ def bench_adjoint(a, fsx, P):
             "fd_check": {"directions": a.fd_directions, "eps": eps,
                          "max_rel_err": worst},
             "loss": loss,
+            "refined_solves": refined,
         },
         "roofline": {"bound": "hbm",
                      "kernel": f"fused CG iteration (SpMV plan {prob.A.plan()[0]} + BLAS-1)",

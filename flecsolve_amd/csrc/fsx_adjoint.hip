@@ -4,7 +4,9 @@
 // backward-Euler operator (time-integrators/operator_adapter.hh:29-35:
 // y = x - gamma F(x); bdf.hh:292-296: gamma = dt, right-hand side the
 // previous solution; bdf.hh:269-272: the previous solution is the initial
-// guess).  Each step is one CG (+ Jacobi) solve through fsx_cg_solve.
+// guess).  Each step is one CG (+ Jacobi) solve through fsx_cg_solve, or
+// (the _ir entry points) one mixed-precision refinement through
+// fsx_ir_cg_solve on the symmetrically scaled FP32 copy of A.
 //
 // Loss L = |u^N - u_target|^2.  Adjoint: lambda^N = 2 (u^N - u_target);
 // backwards, mu = A^{-T} lambda^{s+1}, dL/dA += -mu (u^{s+1})^T sampled on
@@ -185,16 +187,37 @@ void check_diffusion_pattern(const fsx_csr * A, int n) {
 	            "A does not have the 7-point pattern of an n^3 grid");
 }
 
-// One backward-Euler step: x <- solution of A x = b from the guess x.
+// Refined solves: FP32 inner CG on A32 = S A S (S = D^{-1/2}), FP64 outer
+// residual and update, to the outer tolerance s->rtol.
+struct refine {
+	const fsx_csr * A32;
+	fsx_settings inner;
+};
+constexpr int kRefineMaxOuter = 20;
+
+// One backward-Euler step: x <- solution of A x = b from the guess x, by CG
+// (+ Jacobi) or, with R, by mixed-precision refinement (inner iterations
+// counted).
 void be_step(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
              const fsx_vec * b, fsx_vec * x, const fsx_settings * s, int step,
-             const char * what, int64_t & iters) {
-	fsx_info info{};
-	call(fsx_cg_solve(ctx, A, dinv, b, x, s, &info));
-	if (!(info.status == FSX_CONVERGED_RTOL || info.status == FSX_CONVERGED_ATOL))
+             const char * what, int64_t & iters, const refine * R) {
+	bool ok;
+	if (R) {
+		fsx_ir_info ii{};
+		call(fsx_ir_cg_solve(ctx, A, R->A32, 2, b, x, (double)s->rtol, kRefineMaxOuter,
+		                     &R->inner, &ii));
+		ok = ii.status == FSX_CONVERGED_RTOL;
+		iters += ii.inner_iters_total;
+	}
+	else {
+		fsx_info info{};
+		call(fsx_cg_solve(ctx, A, dinv, b, x, s, &info));
+		ok = info.status == FSX_CONVERGED_RTOL || info.status == FSX_CONVERGED_ATOL;
+		iters += info.iters;
+	}
+	if (!ok)
 		throw error(FSX_ERR_STATE, std::string(what) + " solve of step " +
 		                               std::to_string(step) + " did not converge");
-	iters += info.iters;
 }
 
 } // namespace
@@ -377,9 +400,10 @@ int fsx_diffusion3d_coeff_grad(fsx_ctx * ctx, int n, const fsx_csr * A, const fs
 	});
 }
 
-int fsx_be_forward(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv, const fsx_vec * u0,
-                   int steps, const fsx_settings * s, fsx_vec * uT, fsx_be_stats * st) {
-	return guard([&] {
+static void be_forward_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                            const fsx_vec * u0, int steps, const fsx_settings * s, fsx_vec * uT,
+                            fsx_be_stats * st, const refine * R) {
+	{
 		FSX_REQUIRE(ctx && A && s, "null argument");
 		check_vec(u0, "u0");
 		check_vec(uT, "uT");
@@ -395,7 +419,7 @@ int fsx_be_forward(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv, const
 		for (int k = 0; k < steps; ++k) {
 			launch_copy(ctx, cur, nxt); // initial guess: the previous solution
 			try {
-				be_step(ctx, A, dinv, cur, nxt, s, k, "forward", S.forward_iters);
+				be_step(ctx, A, dinv, cur, nxt, s, k, "forward", S.forward_iters, R);
 			}
 			catch (...) {
 				S.failed_step = k;
@@ -408,14 +432,15 @@ int fsx_be_forward(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv, const
 		launch_copy(ctx, cur, uT);
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
 		if (st) *st = S;
-	});
+	}
 }
 
-int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT, const fsx_vec * dinv,
-                   const fsx_vec * u0, const fsx_vec * u_target, int steps, int ckpt,
-                   const fsx_settings * s, double * loss, fsx_vec * G, fsx_vec * uT,
-                   fsx_vec * grad_u0, fsx_be_stats * st) {
-	return guard([&] {
+static void be_adjoint_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT,
+                            const fsx_vec * dinv, const fsx_vec * u0, const fsx_vec * u_target,
+                            int steps, int ckpt, const fsx_settings * s, double * loss,
+                            fsx_vec * G, fsx_vec * uT, fsx_vec * grad_u0, fsx_be_stats * st,
+                            const refine * R) {
+	{
 		FSX_REQUIRE(ctx && A && s && loss, "null argument");
 		check_vec(u0, "u0");
 		check_vec(u_target, "u_target");
@@ -458,7 +483,7 @@ int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT, const f
 			if (k == last0) launch_copy(ctx, cur, seg[0].v);
 			launch_copy(ctx, cur, nxt);
 			try {
-				be_step(ctx, A, dinv, cur, nxt, s, k, "forward", S.forward_iters);
+				be_step(ctx, A, dinv, cur, nxt, s, k, "forward", S.forward_iters, R);
 			}
 			catch (...) {
 				fail_at(k);
@@ -486,7 +511,7 @@ int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT, const f
 				launch_copy(ctx, seg[k - s0].v, x);
 				try {
 					be_step(ctx, A, dinv, seg[k - s0].v, x, s, k, "recompute",
-					        S.recompute_iters);
+					        S.recompute_iters, R);
 				}
 				catch (...) {
 					fail_at(k);
@@ -498,7 +523,7 @@ int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT, const f
 				// mu = A^{-T} lambda^{k+1}, guess lambda^{k+1}
 				launch_copy(ctx, lam, mu);
 				try {
-					be_step(ctx, AA, dinv, lam, mu, s, k, "adjoint", S.adjoint_iters);
+					be_step(ctx, AA, dinv, lam, mu, s, k, "adjoint", S.adjoint_iters, R);
 				}
 				catch (...) {
 					fail_at(k);
@@ -513,6 +538,51 @@ int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT, const f
 		if (grad_u0) launch_copy(ctx, lam, grad_u0);
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
 		if (st) *st = S;
+	}
+}
+
+
+int fsx_be_forward(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv, const fsx_vec * u0,
+                   int steps, const fsx_settings * s, fsx_vec * uT, fsx_be_stats * st) {
+	return guard([&] { be_forward_impl(ctx, A, dinv, u0, steps, s, uT, st, nullptr); });
+}
+
+int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT, const fsx_vec * dinv,
+                   const fsx_vec * u0, const fsx_vec * u_target, int steps, int ckpt,
+                   const fsx_settings * s, double * loss, fsx_vec * G, fsx_vec * uT,
+                   fsx_vec * grad_u0, fsx_be_stats * st) {
+	return guard([&] {
+		be_adjoint_impl(ctx, A, AT, dinv, u0, u_target, steps, ckpt, s, loss, G, uT, grad_u0, st,
+		                nullptr);
+	});
+}
+
+static refine refine_args(const fsx_csr * A, const fsx_csr * A32, const fsx_settings * inner) {
+	FSX_REQUIRE(A && A32 && inner, "null argument");
+	FSX_REQUIRE(A32->vt == FSX_F32 && A32->idx == A->idx,
+	            "A32 must be fsx_csr_convert(A, FSX_F32) holding "
+	            "fsx_csr_set_values_f32_sym_scaled values");
+	return refine{A32, *inner};
+}
+
+int fsx_be_forward_ir(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
+                      const fsx_vec * u0, int steps, const fsx_settings * s,
+                      const fsx_settings * inner, fsx_vec * uT, fsx_be_stats * st) {
+	return guard([&] {
+		const refine R = refine_args(A, A32, inner);
+		be_forward_impl(ctx, A, nullptr, u0, steps, s, uT, st, &R);
+	});
+}
+
+int fsx_be_adjoint_ir(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
+                      const fsx_vec * u0, const fsx_vec * u_target, int steps, int ckpt,
+                      const fsx_settings * s, const fsx_settings * inner, double * loss,
+                      fsx_vec * G, fsx_vec * uT, fsx_vec * grad_u0, fsx_be_stats * st) {
+	return guard([&] {
+		const refine R = refine_args(A, A32, inner);
+		// A symmetric: the adjoint solves use A (and A32) themselves
+		be_adjoint_impl(ctx, A, nullptr, nullptr, u0, u_target, steps, ckpt, s, loss, G, uT,
+		                grad_u0, st, &R);
 	});
 }
 

@@ -189,6 +189,11 @@ _SIGS = {
     "fsx_be_adjoint": (C.c_int, [_P, _P, _P, _P, _P, _P, C.c_int, C.c_int,
                                  C.POINTER(Settings), C.POINTER(C.c_double), _P, _P, _P,
                                  C.POINTER(BEStats)]),
+    "fsx_be_forward_ir": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(Settings),
+                                    C.POINTER(Settings), _P, C.POINTER(BEStats)]),
+    "fsx_be_adjoint_ir": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.c_int,
+                                    C.POINTER(Settings), C.POINTER(Settings),
+                                    C.POINTER(C.c_double), _P, _P, _P, C.POINTER(BEStats)]),
     "fsx_time_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
     "fsx_time_copy": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
 }
@@ -581,6 +586,32 @@ def be_adjoint(A: CSR, u0: Vector, u_target: Vector, steps: int, ckpt: int,
                                dinv.h if dinv is not None else None, u0.h, u_target.h,
                                steps, ckpt, C.byref(settings), C.byref(loss), G.h, uT.h,
                                gu0.h, C.byref(st)), "fsx_be_adjoint")
+    return loss.value, G, uT, gu0, st
+
+
+def be_forward_ir(A: CSR, A32: CSR, u0: Vector, steps: int, settings: Settings,
+                  inner: Settings):
+    """be_forward with each step solved by mixed-precision refinement on A32
+    (S A S in FP32: A32.set_values_sym_scaled(A) first)."""
+    uT = Vector(A.ctx, A.nrows)
+    st = BEStats()
+    check(lib().fsx_be_forward_ir(A.ctx.h, A.h, A32.h, u0.h, steps, C.byref(settings),
+                                  C.byref(inner), uT.h, C.byref(st)), "fsx_be_forward_ir")
+    return uT, st
+
+
+def be_adjoint_ir(A: CSR, A32: CSR, u0: Vector, u_target: Vector, steps: int, ckpt: int,
+                  settings: Settings, inner: Settings):
+    """be_adjoint with every solve by mixed-precision refinement on A32
+    (symmetric A).  Returns (loss, G, u^steps, dL/du^0, stats)."""
+    G = Vector(A.ctx, A.nnz)
+    uT = Vector(A.ctx, A.nrows)
+    gu0 = Vector(A.ctx, A.nrows)
+    loss = C.c_double()
+    st = BEStats()
+    check(lib().fsx_be_adjoint_ir(A.ctx.h, A.h, A32.h, u0.h, u_target.h, steps, ckpt,
+                                  C.byref(settings), C.byref(inner), C.byref(loss), G.h, uT.h,
+                                  gu0.h, C.byref(st)), "fsx_be_adjoint_ir")
     return loss.value, G, uT, gu0, st
 
 

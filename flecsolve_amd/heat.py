@@ -8,7 +8,10 @@ zero boundary, the 7-point operator with arithmetic-mean face coefficients.
 Each step is a CG + Jacobi solve from the previous state.  The loss is
 |u_T - u_target|^2, u_target the trajectory end of a second coefficient
 field; its gradient comes from the library's discrete adjoint
-(fsx_be_adjoint) chained to log k (fsx_diffusion3d_coeff_grad).  Every
+(fsx_be_adjoint) chained to log k (fsx_diffusion3d_coeff_grad).  With
+refine=True every solve is a mixed-precision refinement on the
+symmetrically scaled FP32 copy of A (fsx_be_forward_ir / fsx_be_adjoint_ir)
+to the same tolerance.  Every
 solve, assembly and gradient kernel runs on the GPU through the C ABI.
 """
 from __future__ import annotations
@@ -24,8 +27,11 @@ from . import problems as P
 class HeatProblem:
     def __init__(self, ctx: fsx.Context, n: int = 128, dt: float = 1e-3, steps: int = 50,
                  ckpt: int = 10, sigma: float = 1.0, ell: float = 8.0, bumps: int = 8,
-                 rtol: float = 1e-12, maxiter: int = 5000, seeds: dict | None = None):
+                 rtol: float = 1e-12, maxiter: int = 5000, seeds: dict | None = None,
+                 refine: bool = False, inner_rtol: float = 1e-4):
         s = seeds or P.seeds()
+        self.refine = refine
+        self.inner = fsx.Settings(maxiter=maxiter, rtol=inner_rtol)
         self.ctx, self.n, self.steps, self.ckpt = ctx, n, steps, ckpt
         self.gamma = dt * (n + 1) ** 2          # dt / h^2
         self.N = n ** 3
@@ -38,20 +44,28 @@ class HeatProblem:
         self.dinv = ctx.zeros(self.N)
         self.u0 = ctx.vec(P.bumps(n, bumps, s["heat_u0"]))
         self.settings = fsx.Settings(maxiter=maxiter, rtol=rtol, use_zero_guess=False)
+        self.A32 = self.A.convert(fsx.F32) if refine else None
         self._set_k(k_target)
-        self.u_target, self.target_stats = fsx.be_forward(self.A, self.u0, steps,
-                                                          self.settings, self.dinv)
+        self.u_target, self.target_stats = self._forward()
         self._d = ctx.zeros(self.N)
 
     def _set_k(self, k: np.ndarray):
         self.kvec.upload(k)
         self.A.set_diffusion3d_values(self.n, self.kvec, 1.0, self.gamma)
         fsx.check(fsx.lib().fsx_csr_diag_inv(self.ctx.h, self.A.h, self.dinv.h), "diag_inv")
+        if self.refine:
+            self.A32.set_values_sym_scaled(self.A)
+
+    def _forward(self):
+        if self.refine:
+            return fsx.be_forward_ir(self.A, self.A32, self.u0, self.steps, self.settings,
+                                     self.inner)
+        return fsx.be_forward(self.A, self.u0, self.steps, self.settings, self.dinv)
 
     def loss(self, theta: np.ndarray) -> float:
         """|u_T(theta) - u_target|^2 with k = exp(theta)."""
         self._set_k(np.exp(theta))
-        uT, _ = fsx.be_forward(self.A, self.u0, self.steps, self.settings, self.dinv)
+        uT, _ = self._forward()
         fsx.check(fsx.lib().fsx_axpy(self.ctx.h, -1.0, self.u_target.h, uT.h, self._d.h),
                   "axpy")
         return fsx.dot(self._d, self._d)
@@ -59,7 +73,12 @@ class HeatProblem:
     def loss_and_grad(self, theta: np.ndarray):
         """(loss, dL/dtheta on the (n+2)^3 lattice, dL/du0, adjoint stats)."""
         self._set_k(np.exp(theta))
-        loss, G, uT, gu0, st = fsx.be_adjoint(self.A, self.u0, self.u_target, self.steps,
-                                              self.ckpt, self.settings, self.dinv)
+        if self.refine:
+            loss, G, uT, gu0, st = fsx.be_adjoint_ir(self.A, self.A32, self.u0, self.u_target,
+                                                     self.steps, self.ckpt, self.settings,
+                                                     self.inner)
+        else:
+            loss, G, uT, gu0, st = fsx.be_adjoint(self.A, self.u0, self.u_target, self.steps,
+                                                  self.ckpt, self.settings, self.dinv)
         g = self.A.diffusion3d_coeff_grad(self.n, G, self.kvec, self.gamma)
         return loss, g.get(), gu0.get(), st

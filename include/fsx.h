@@ -433,6 +433,20 @@ int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT,
                    const fsx_settings * s, double * loss, fsx_vec * G,
                    fsx_vec * uT, fsx_vec * grad_u0, fsx_be_stats * stats);
 
+/* The same with every solve (forward, recomputed and adjoint) run by
+ * fsx_ir_cg_solve with jacobi = 2 from the same guess: FP32 inner CG on A32
+ * (= fsx_csr_convert(A, FSX_F32) holding fsx_csr_set_values_f32_sym_scaled
+ * values) to inner->rtol, FP64 outer residual and update until
+ * |b - A x| / |b| <= s->rtol.  A must be symmetric (the adjoint solves use A).
+ * Stats count inner iterations.  New, no reference counterpart. */
+int fsx_be_forward_ir(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
+                      const fsx_vec * u0, int steps, const fsx_settings * s,
+                      const fsx_settings * inner, fsx_vec * uT, fsx_be_stats * stats);
+int fsx_be_adjoint_ir(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
+                      const fsx_vec * u0, const fsx_vec * u_target, int steps, int ckpt,
+                      const fsx_settings * s, const fsx_settings * inner, double * loss,
+                      fsx_vec * G, fsx_vec * uT, fsx_vec * grad_u0, fsx_be_stats * stats);
+
 /* ---- batched solves (new: many independent systems, one persistent launch)
  * A is the block-diagonal concatenation (int32 indices, FP64 values, global
  * column ids) of nsys systems; system s owns rows [sys_rows[s],
