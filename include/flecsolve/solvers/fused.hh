@@ -13,6 +13,9 @@
 //   auto S = op::make_shared<cg::fused>(settings, A);        // or A, dinv
 //   solve_info info = S.get().apply(b, x);
 //
+// ir::fused is the mixed-precision refinement (FP32 inner CG on the
+// symmetrically Jacobi-scaled matrix) with the same interface.
+//
 // The matrix is held by handle: values changed in place (fsx_csr_set_values,
 // fsx_csr_shift_scale) are seen by the next solve.
 #ifndef FLECSOLVE_AMD_SOLVERS_FUSED_HH
@@ -88,6 +91,51 @@ struct fused_solver : op::base<> {
 
 namespace flecsolve::cg {
 using fused = detail::fused_solver<settings, fsx_cg_solve>;
+}
+
+namespace flecsolve::ir {
+
+// FP32-inner / FP64-outer refinement (fsx_ir_cg_solve; no reference
+// counterpart) as an operator: apply(b, x) refines x from its current value
+// until |b - A x| / |b| <= settings.rtol (outer, FP64), each correction an
+// FP32 CG on S A S (S = D^{-1/2}) to inner.rtol.  Holds an FP32 copy of A's
+// values, written at construction; call refresh() after A's values change.
+// solve_info: status, iters = inner iterations, restarts = outer
+// iterations, res_norm_final = final relative residual.
+struct fused : op::base<> {
+	fused(const solver_settings & s, const solver_settings & inner, mat::device_csr A,
+	      int max_outer = 20)
+		: settings(s), inner(inner), A(std::move(A)), max_outer(max_outer) {
+		fsx_csr * a32 = nullptr;
+		fsx_check(fsx_csr_convert(this->A.context(), this->A.handle(), FSX_F32, &a32),
+		          "ir: FP32 copy");
+		A32 = mat::device_csr::adopt(this->A.context(), a32);
+		refresh();
+	}
+	void refresh() {
+		fsx_check(fsx_csr_set_values_f32_sym_scaled(A.context(), A32.handle(), A.handle()),
+		          "ir: scaled FP32 values");
+	}
+	template<class B, class X>
+	solve_info apply(const B & b, X & x) const {
+		fsx_settings in = detail::to_fsx(inner);
+		fsx_ir_info ii{};
+		fsx_check(fsx_ir_cg_solve(A.context(), A.handle(), A32.handle(), 2, b.data.get(),
+		                          x.data.get(), (double)settings.rtol, max_outer, &in, &ii),
+		          "ir solve");
+		solve_info info;
+		info.status = static_cast<solve_info::stop_reason>(ii.status);
+		info.iters = ii.inner_iters_total;
+		info.restarts = ii.outer_iters;
+		info.rhs_norm = (float)ii.rhs_norm;
+		info.res_norm_final = (float)ii.rel_res_final;
+		return info;
+	}
+	solver_settings settings, inner;
+	mat::device_csr A, A32;
+	int max_outer;
+};
+
 }
 namespace flecsolve::bicgstab {
 using fused = detail::fused_solver<settings, fsx_bicgstab_solve>;

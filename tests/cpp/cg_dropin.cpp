@@ -5,7 +5,8 @@
 // monotone decrease and the Trefethen-Bau bound 2((sqrt k - 1)/(sqrt k + 1))^i.
 // Also runs GMRES and BiCGSTAB through the same API, and CG on a Matrix
 // Market file read with matrix_market<>::read(...).tocsr() (argv[1]), and
-// solvers chosen at run time from config files (argv[2]: tests/golden).
+// solvers chosen at run time from config files (argv[2]: tests/golden), and
+// the fused CG and mixed-precision refinement policies (solvers/fused.hh).
 // Exit status 0 = pass.
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,7 @@
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
 #include "flecsolve/solvers/factory.hh"
+#include "flecsolve/solvers/fused.hh"
 #include "flecsolve/solvers/gmres.hh"
 #include "flecsolve/solvers/mg/gs.hh"
 #include "fsx_problems.h"
@@ -151,6 +153,35 @@ int main(int argc, char ** argv) {
 		std::printf("cg+symmetric hybrid_gs: %d iterations (no preconditioner: %d), "
 		            "true rel res %.3e\n", fi.iters, ci2.iters, frel);
 		fails += !fi.success() || frel > 1e-9 || fi.iters >= ci2.iters;
+	}
+	{
+		// the fused solves behind the operator interface: CG + Dinv and the
+		// mixed-precision refinement, against each other
+		op::core<mat::device_csr> A(ctx, Ah);
+		mat::device_csr Am(ctx, Ah);
+		vec::device<double> rhs(ctx, n * n), y1(ctx, n * n), y2(ctx, n * n), r(ctx, n * n);
+		rhs.set_random(5);
+		cg::settings cs;
+		cs.maxiter = 3000;
+		cs.rtol = 1e-10;
+		cs.use_zero_guess = true;
+		mat::device_dinv D(Am);
+		auto C = op::make_shared<cg::fused>(cs, Am, D);
+		solve_info c1 = C.get().apply(rhs, y1);
+		solver_settings inner;
+		inner.maxiter = 3000;
+		inner.rtol = 1e-4f;
+		auto R = op::make_shared<ir::fused>(cs, inner, Am);
+		y2.set_scalar(0.0);
+		solve_info c2 = R.get().apply(rhs, y2);
+		A.residual(rhs, y2, r);
+		const double rel2 = r.l2norm().get() / rhs.l2norm().get();
+		y2.axpy(-1.0, y1, y2);
+		const double dx = y2.l2norm().get() / y1.l2norm().get();
+		std::printf("fused cg+dinv: %d iterations; refinement: %d outer, %d inner, "
+		            "true rel res %.3e, |x_ir - x_cg| / |x_cg| %.2e\n",
+		            c1.iters, c2.restarts, c2.iters, rel2, dx);
+		fails += !c1.success() || !c2.success() || rel2 > 1.01e-10 || dx > 1e-6;
 	}
 	if (argc > 2) {
 		// run-time solver choice from config files (krylov_factory), as the
