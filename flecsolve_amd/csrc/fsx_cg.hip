@@ -25,8 +25,10 @@
 // overshoots convergence costs only empty launches and the reported
 // iteration count is exact.  The iteration that sets `done` skips its K3;
 // its x update is applied by one kernel after the loop (`xpend`).  Chunks of
-// iterations are captured into a HIP graph and replayed; the host reads the
-// flag once per chunk with one chunk in flight.
+// iterations (two lengths, 4 and 16) are captured into HIP graphs and
+// replayed; the host reads the flag once per chunk with a chunk in flight,
+// choosing the next length from the iterations the residual contraction so
+// far predicts are left.
 #include "fsx_spmv_kernels.hpp"
 
 #include <cfloat>
