@@ -32,8 +32,10 @@
 //     one-colour result bit for bit.  The equal_map partition is FleCSI's
 //     published formula (FleCSI is not vendored): parity unpinned for P > 1
 //     beyond these checks.
-//   * fcg: equals cg's solution on SPD systems (parity unpinned for iteration
-//     trajectories beyond that; fcg is not named by the hot path).
+//   * fcg: with a fixed SPD preconditioner (identity, Jacobi) its iterates
+//     equal preconditioned cg's iterate by iterate to rounding (exact
+//     arithmetic makes its extra orthogonalisation term vanish), and it
+//     reaches cg's solution.
 //
 // Reference quirks reproduced on purpose (they change iteration counts or
 // stored values):

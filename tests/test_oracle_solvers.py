@@ -218,3 +218,31 @@ def test_cg_colours_baseline():
     x4, i4 = O.cg_colours(A, dinv, b, np.zeros(n ** 3), s, 4)
     assert abs(i4.iters - io.iters) <= 2
     assert np.linalg.norm(x4 - xs) <= 1e-10 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_fcg_trajectory_equals_pcg(jacobi):
+    """Flexible CG with a fixed SPD preconditioner produces the iterates of
+    preconditioned CG in exact arithmetic (its extra orthogonalisation term
+    vanishes): the oracle's FCG and CG agree iterate by iterate, to rounding,
+    for the identity and the Jacobi preconditioner."""
+    A, S = lap2d(16)
+    b = P.uniform(256, 7)
+    s = O.Settings(maxiter=40, rtol=0.0, use_zero_guess=True)
+    Pc = O.prec(1, A.dinv()) if jacobi else None
+    xs_cg, res_cg, xs_fcg, res_fcg = [], [], [], []
+
+    def rec(xs, rs):
+        def f(x, res):
+            xs.append(np.array(x))
+            rs.append(res)
+            return False
+        return f
+
+    O.cg(A, b, np.zeros(256), s, P=Pc, diag=rec(xs_cg, res_cg))
+    O.fcg(A, b, np.zeros(256), s, P=Pc, diag=rec(xs_fcg, res_fcg))
+    assert len(xs_cg) == len(xs_fcg) == 40
+    for k in range(40):
+        scale = np.linalg.norm(xs_cg[k])
+        assert np.linalg.norm(xs_fcg[k] - xs_cg[k]) <= 1e-8 * scale, k
+        assert abs(res_fcg[k] - res_cg[k]) <= 1e-6 * res_cg[0], k
