@@ -50,6 +50,8 @@ protected:
 };
 
 // f(x) = F x for a device CSR F; the stage operator is the CSR I - gamma F.
+// F must be square, FP64, and store every row's diagonal entry (the identity
+// is added there; construction throws otherwise).
 struct csr_adapter : op::base<> {
 	explicit csr_adapter(mat::device_csr F) : F(std::move(F)) {
 		fsx_csr * s = nullptr;
