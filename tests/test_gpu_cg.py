@@ -274,3 +274,30 @@ def test_sym_scaled_f32_values(ctx, plan):
                   np.array([1.0, -1.0]))
     with pytest.raises(fsx.FsxError):
         neg.convert(fsx.F32).set_values_sym_scaled(neg)
+
+
+def test_new_entry_points_reject_bad_arguments(ctx):
+    """Errors, not faults: a refinement (jacobi = 2) or refined adjoint whose
+    FP32 matrix is not a copy of A's pattern, an unknown mode, a stage matrix
+    that is F itself or does not share F's pattern."""
+    rp, ci, v = P.poisson2d(12)
+    A = ctx.csr(rp, ci, v)
+    other = ctx.csr(rp, ci, v)          # same values, its own index arrays
+    A32_other = other.convert(fsx.F32)
+    b, x = ctx.vec(P.uniform(144, 1)), ctx.zeros(144)
+    inner = fsx.Settings(maxiter=100, rtol=1e-4)
+    with pytest.raises(fsx.FsxError):
+        fsx.ir_cg_solve(A, A32_other, b, x, 1e-10, inner, jacobi=2)
+    with pytest.raises(fsx.FsxError):
+        fsx.ir_cg_solve(A, A.convert(fsx.F32), b, x, 1e-10, inner, jacobi=3)
+    with pytest.raises(fsx.FsxError):
+        fsx.be_forward_ir(A, A32_other, b, 2, fsx.Settings(maxiter=100, rtol=1e-10), inner)
+    with pytest.raises(fsx.FsxError):
+        A.shift_scale(A, 1.0, -0.5)
+    with pytest.raises(fsx.FsxError):
+        other.convert(fsx.F64).shift_scale(A, 1.0, -0.5)
+    # the good path still works afterwards
+    A32 = A.convert(fsx.F32)
+    A32.set_values_sym_scaled(A)
+    info = fsx.ir_cg_solve(A, A32, b, x, 1e-10, inner, jacobi=2)
+    assert info.status == 1
