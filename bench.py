@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--emin", type=int, default=8)
     ap.add_argument("--emax", type=int, default=48)
     ap.add_argument("--cpu-sample-systems", type=int, default=16)
+    ap.add_argument("--amg-vcycles", type=int, default=0,
+                    help="amg workload: only time this many V-cycle applications "
+                         "(Jacobi smoothing; profiling runs)")
     ap.add_argument("--heat-n", type=int, default=2046)
     ap.add_argument("--heat-cpu-n", type=int, default=638)
     ap.add_argument("--heat-spmv-only", action="store_true",
@@ -800,6 +803,24 @@ def bench_amg(a, fsx, P):
     ctx = fsx.Context(0)
     A = ctx.csr(rp, ci, v)
     bd = ctx.vec(b)
+    if a.amg_vcycles > 0:
+        # one V-cycle per application (the preconditioner), Jacobi smoothing
+        M = fsx.AMG(A, fsx.AmgSettings(max_levels=20, maxiter=1, atol=0.0, smoother=1,
+                                       nrelax=2))
+        x = ctx.zeros(N)
+        M.apply(bd, x)
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(a.amg_vcycles):
+            M.apply(bd, x)
+        ctx.sync()
+        ms = (time.perf_counter() - t) / a.amg_vcycles * 1e3
+        log(f"V-cycle (Jacobi smoothing): {ms:.3f} ms per application, levels "
+            f"{[r for r, _ in M.levels()]}")
+        print(json.dumps({"metric": "UA-AMG V-cycle time", "value": round(ms, 4), "unit": "ms",
+                          "config": {"workload": f"amg_vcycle_{n}^3", "rows": N,
+                                     "applications": a.amg_vcycles}}), flush=True)
+        return
     s = fsx.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True)
     results = {}
     x = ctx.zeros(N)
