@@ -32,6 +32,11 @@ start, truncation-error step control, CG stage solves to 1e-6, integrated to
 t = 0.1) on a 2046^2 interior grid; a step is the whole integration, run by
 flecsolve_amd/heat_implicit with the fused stage path, the reference-shaped
 generic path timed beside it.
+
+`--workload parcsr [--colours 8]`: the parallel CSR product (the matrix
+split over colours as the reference's parcsr splits it over ranks: ghost
+copy, diag SpMV, offd rows) on the 256^3 diffusion matrix, all colours on
+one device, beside the one-colour SpMV.
 """
 from __future__ import annotations
 
@@ -125,8 +130,10 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the SpMV and copy kernels only (profiling runs)")
     ap.add_argument("--workload", choices=["diffusion", "batched", "adjoint", "convdiff",
-                                           "smoother", "amg", "heat"],
+                                           "smoother", "amg", "heat", "parcsr"],
                     default="diffusion")
+    ap.add_argument("--colours", type=int, default=8,
+                    help="parcsr workload: number of colours (row blocks)")
     ap.add_argument("--heat-steps", type=int, default=50)
     ap.add_argument("--ckpt", type=int, default=10)
     ap.add_argument("--fd-directions", type=int, default=20)
@@ -165,6 +172,8 @@ def main():
         return bench_amg(a, fsx, P)
     if a.workload == "heat":
         return bench_heat(a, fsx, P)
+    if a.workload == "parcsr":
+        return bench_parcsr(a, fsx, P)
 
     seeds = P.seeds()
     n = a.n
@@ -1014,6 +1023,116 @@ def bench_heat(a, fsx, P):
                       "CG (C, serial)",
             "ns_per_row_iteration": round(tc * 1e9 / (nc * nc * solve.iters), 3),
             "gpu_ns_per_row_iteration": round(fs * 1e9 / (N * iters), 4)}
+    print(json.dumps(out), flush=True)
+
+
+def bench_parcsr(a, fsx, P):
+    """The parallel CSR product (parcsr_ops::spmv over `--colours` colours,
+    all on one device: ghost copy, diag SpMV, offd rows) on the 256^3
+    diffusion matrix, beside the one-colour SpMV.  The colours are FleCSI
+    equal_map row blocks (z-slabs of the grid when they divide n).  Checks
+    on the full problem: rows without offd entries equal the one-colour
+    product bit for bit, the others agree to rounding."""
+    n, ncol = a.n, a.colours
+    seeds = P.seeds()
+    k = P.lognormal_field(n, a.sigma, 8.0, seeds["diffusion_k"])
+    rp, ci, v = P.diffusion3d(n, k)
+    del k
+    N, nnz = n ** 3, int(ci.size)
+    b = P.uniform(N, seeds["diffusion_rhs"])
+    ctx = fsx.Context(0)
+    A = ctx.csr(rp, ci, v)
+    t0 = time.perf_counter()
+    Mp = fsx.ParCSR(A, ncol)
+    t_split = time.perf_counter() - t0
+    infos = [Mp.colour_info(c) for c in range(ncol)]
+    ng = sum(i["ghosts"] for i in infos)
+    nnz_d = sum(i["nnz_diag"] for i in infos)
+    nnz_o = sum(i["nnz_offd"] for i in infos)
+    log(f"parcsr {n}^3 over {ncol} colours: split in {t_split:.1f} s; {ng} ghosts, "
+        f"{nnz_o} offd entries; diag plan {Mp.diag_plan()}")
+    bd, y1, y2 = ctx.vec(b), ctx.zeros(N), ctx.zeros(N)
+    A.spmv(bd, y1)
+    Mp.spmv(bd, y2)
+    ctx.sync()
+    g1, g2 = y1.get(), y2.get()
+    # rows with offd entries: those whose row has a column outside its block
+    rb = np.array([c * (N // ncol) + min(c, N % ncol) for c in range(ncol + 1)])
+    row_col = np.repeat(np.searchsorted(rb, np.arange(N), side="right") - 1, np.diff(rp))
+    col_col = np.searchsorted(rb, ci, side="right") - 1
+    boundary = np.zeros(N, bool)
+    boundary[np.repeat(np.arange(N), np.diff(rp))[row_col != col_col]] = True
+    interior_exact = bool(np.array_equal(g1[~boundary], g2[~boundary]))
+    rel = float(np.max(np.abs(g1[boundary] - g2[boundary]) /
+                       np.maximum(np.abs(g1[boundary]), 1e-300))) if boundary.any() else 0.0
+    log(f"check: interior rows bit-identical {interior_exact}; boundary rows "
+        f"({int(boundary.sum())}) max rel diff {rel:.2e}")
+    del row_col, col_col
+    ms_plain = fsx_time(fsx, "fsx_time_spmv", ctx, A, bd, y1, a.spmv_reps)
+    ts = []
+    for i in range(a.warmup + a.steps):
+        ms = fsx_time(fsx, "fsx_time_parcsr_spmv", ctx, Mp, bd, y2, a.spmv_reps)
+        log(f"parcsr spmv {'warmup' if i < a.warmup else 'step'} {i}: {ms:.4f} ms")
+        if i >= a.warmup:
+            ts.append(ms)
+    ms = sum(ts) / len(ts)
+    # bytes the three launches must move: the diag blocks' SpMV (int32
+    # indices), the ghost gather (colmap, x, ghost), the offd rows (row ids,
+    # row pointers, columns, values, ghost values, y read and written)
+    B_diag = spmv_bytes(N, nnz_d, 8)
+    B_gather = ng * (4 + 8 + 8)
+    nb = int(boundary.sum())
+    B_offd = (2 * nb + 1) * 4 + nnz_o * (4 + 8 + 8) + nb * 16
+    B = B_diag + B_gather + B_offd
+    gbs = B / (ms * 1e-3) / 1e9
+    B_plain = spmv_bytes(N, nnz, 8)
+    out = {
+        "metric": f"parallel CSR SpMV over {ncol} colours on one device, {n}^3 diffusion",
+        "value": round(ms, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal GRF coefficient, uniform x)",
+        "config": {"workload": f"parcsr_spmv_{n}^3_{ncol}colours", "rows": N, "nnz": nnz,
+                   "colours": ncol, "ghosts": ng, "offd_entries": nnz_o, "offd_rows": nb,
+                   "split_seconds": round(t_split, 2),
+                   "one_colour_spmv_ms": round(ms_plain, 4),
+                   "one_colour_gbs": round(B_plain / (ms_plain * 1e-3) / 1e9, 1),
+                   "overhead_vs_one_colour": round(ms / ms_plain - 1, 4),
+                   "interior_rows_bit_identical": interior_exact,
+                   "boundary_rows_max_rel_diff": rel},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": B,
+                     "bytes": {"diag_spmv": B_diag, "ghost_gather": B_gather,
+                               "offd_rows": B_offd},
+                     "avg_ms": round(ms, 5)},
+    }
+    if not a.no_cpu_baseline:
+        # the oracle's colour-by-colour product on the 128^3 problem of the
+        # same kind, scaled to this one by stored entries
+        from oracle import parcsr as PC
+        m = min(n, 128)
+        km = P.lognormal_field(m, a.sigma, 8.0, seeds["diffusion_k"])
+        rps, cis, vs = P.diffusion3d(m, km)
+        blocks = PC.split(rps, cis, vs, m ** 3, m ** 3, ncol)
+        xs = P.uniform(m ** 3, 3)
+        reps, t = 3, time.perf_counter()
+        for _ in range(reps):
+            PC.spmv(blocks, xs, m ** 3)
+        t_cpu = (time.perf_counter() - t) / reps * (nnz / cis.size)
+        out["cpu_baseline"] = {"value": round(t_cpu * 1e3, 1), "unit": "ms", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle parcsr product on {m}^3 over {ncol} colours "
+                                         f"(numpy, 1 thread), {reps} runs, scaled by stored "
+                                         "entries"}
+        log(f"oracle parcsr product: {t_cpu*1e3:.1f} ms scaled to {n}^3")
     print(json.dumps(out), flush=True)
 
 

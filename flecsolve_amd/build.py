@@ -28,7 +28,7 @@ ARCH = os.environ.get("FSX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["fsx_core.hip", "fsx_spmv.hip", "fsx_cg.hip", "fsx_batched.hip",
-               "fsx_adjoint.hip", "fsx_smooth.hip", "fsx_amg.hip"]
+               "fsx_adjoint.hip", "fsx_smooth.hip", "fsx_amg.hip", "fsx_parcsr.hip"]
 CPP_SOURCES = ["fsx_krylov.cpp", "fsx_io.cpp"]
 
 # -ffp-contract=off: no fused multiply-add contraction, so elementwise results

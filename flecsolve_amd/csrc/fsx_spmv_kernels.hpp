@@ -15,8 +15,9 @@
 //    the CSR-Stream scheme of Greathouse & Daga (SC'14).  Because every row is
 //    summed serially from 0 in CSR order with separate multiply and add
 //    roundings, y is bit-identical to the reference's serial loop.  A block
-//    whose nonzeros overflow the LDS budget falls back to one wavefront per
-//    row for that block only.
+//    whose nonzeros overflow the LDS budget falls back to one thread per row
+//    (serial, uncoalesced) for that block only, so the order is kept there
+//    too.
 //
 //  * Sub-wavefront vector per row (long rows): LANES lanes (2..64) share a
 //    row, stride through it, and combine with cross-lane shuffles.  The
@@ -121,19 +122,13 @@ __device__ __forceinline__ ACC stream_rows(int64_t r0, int nr,
 			sum = s;
 		}
 	}
-	else {
-		// overflow: one wavefront per row within this block
-		const int lane = tid & 63, wv = tid >> 6;
-		for (int rr = wv; rr < nr; rr += kSpmvThreads / 64) {
-			const I lo = srow[rr], hi = srow[rr + 1];
-			ACC s = ACC(0);
-			for (I e = lo + lane; e < hi; e += 64)
-				s += (ACC)ldnt(vals + e) * (ACC)x(ldnt(colind + e));
-			s = wave_sum(s);
-			if (lane == 0) prod[rr] = s;
-		}
-		__syncthreads();
-		if (tid < nr) sum = prod[tid];
+	else if (tid < nr) {
+		// overflow (the block's entries do not fit the stage): each thread
+		// sums its own row serially, keeping the CSR order
+		const I lo = srow[tid], hi = srow[tid + 1];
+		ACC s = ACC(0);
+		for (I e = lo; e < hi; ++e) s += (ACC)ldnt(vals + e) * (ACC)x(ldnt(colind + e));
+		sum = s;
 	}
 	return sum;
 }

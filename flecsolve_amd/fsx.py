@@ -121,6 +121,14 @@ _SIGS = {
     "fsx_csr_residual": (C.c_int, [_P, _P, _P, _P, _P]),
     "fsx_csr_diag_inv": (C.c_int, [_P, _P, _P]),
     "fsx_csr_spmv_t": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_parcsr_create": (C.c_int, [_P, _P, C.c_int, C.POINTER(_P)]),
+    "fsx_parcsr_destroy": (C.c_int, [_P]),
+    "fsx_parcsr_colour_info": (C.c_int, [_P, C.c_int, _P]),
+    "fsx_parcsr_download": (C.c_int, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
+    "fsx_parcsr_spmv": (C.c_int, [_P, _P, _P, _P]),
+    "fsx_parcsr_residual": (C.c_int, [_P, _P, _P, _P, _P]),
+    "fsx_parcsr_diag": (C.c_int, [_P, C.POINTER(_P)]),
+    "fsx_time_parcsr_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
     "fsx_spmv_dot": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_double)]),
     "fsx_jacobi_apply": (C.c_int, [_P, _P, _P, _P]),
     "fsx_jacobi_relax": (C.c_int, [_P, _P, C.c_float, C.c_int, _P, _P]),
@@ -632,6 +640,69 @@ def krylov(kind: str, A: CSR, b: Vector, x: Vector, settings: Settings,
                 C.byref(info), hp, hist_cap)
     check(rc, f"fsx_krylov_{kind}")
     return info, hist[:min(hist_cap, max(info.iters, 0) or hist_cap)]
+
+
+class ParCSR:
+    """Parallel CSR over `ncolours` colours on one device (fsx_parcsr_*;
+    topo::csr colouring and mat::parcsr of flecsolve/topo/csr.hh,
+    flecsolve/matrices/parcsr.hh).  A (FP64) is copied."""
+
+    def __init__(self, A: CSR, ncolours: int):
+        self.ctx = A.ctx
+        self.nrows, self.ncols, self.ncolours = A.nrows, A.ncols, int(ncolours)
+        self.h = C.c_void_p()
+        check(lib().fsx_parcsr_create(A.ctx.h, A.h, int(ncolours), C.byref(self.h)),
+              "fsx_parcsr_create")
+
+    def colour_info(self, c: int) -> dict:
+        s = np.zeros(5, np.int64)
+        check(lib().fsx_parcsr_colour_info(self.h, int(c), _ptr(s)), "fsx_parcsr_colour_info")
+        return dict(rows=int(s[0]), cols=int(s[1]), ghosts=int(s[2]), nnz_diag=int(s[3]),
+                    nnz_offd=int(s[4]))
+
+    def download(self, c: int) -> dict:
+        """Colour c's blocks in the reference's storage (keys as
+        oracle.parcsr.split)."""
+        i = self.colour_info(c)
+        d = dict(rows=i["rows"], cols=i["cols"],
+                 diag_rowptr=np.empty(i["rows"] + 1, np.int64),
+                 diag_colind=np.empty(i["nnz_diag"], np.int64),
+                 diag_vals=np.empty(i["nnz_diag"]),
+                 offd_rowptr=np.empty(i["rows"] + 1, np.int64),
+                 offd_colind=np.empty(i["nnz_offd"], np.int64),
+                 offd_vals=np.empty(i["nnz_offd"]),
+                 colmap=np.empty(i["ghosts"], np.int64))
+        check(lib().fsx_parcsr_download(
+            self.ctx.h, self.h, int(c), _ptr(d["diag_rowptr"]), _ptr(d["diag_colind"]),
+            _ptr(d["diag_vals"]), _ptr(d["offd_rowptr"]), _ptr(d["offd_colind"]),
+            _ptr(d["offd_vals"]), _ptr(d["colmap"])), "fsx_parcsr_download")
+        return d
+
+    def spmv(self, x: Vector, y: Vector):
+        check(lib().fsx_parcsr_spmv(self.ctx.h, self.h, x.h, y.h), "fsx_parcsr_spmv")
+
+    def residual(self, b: Vector, x: Vector, r: Vector):
+        check(lib().fsx_parcsr_residual(self.ctx.h, self.h, b.h, x.h, r.h),
+              "fsx_parcsr_residual")
+
+    def diag_plan(self) -> tuple[int, int]:
+        """SpMV row mapping of the diag blocks' matrix (owned by self)."""
+        d = C.c_void_p()
+        check(lib().fsx_parcsr_diag(self.h, C.byref(d)), "fsx_parcsr_diag")
+        k, l = C.c_int(), C.c_int()
+        check(lib().fsx_csr_plan(d, C.byref(k), C.byref(l)))
+        return k.value, l.value
+
+    def destroy(self):
+        if self.h:
+            check(lib().fsx_parcsr_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class HybridGS:

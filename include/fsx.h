@@ -206,6 +206,41 @@ int fsx_csr_diag_inv(fsx_ctx * ctx, const fsx_csr * A, fsx_vec * dinv);
 int fsx_csr_spmv_t(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
                    fsx_vec * y);
 
+/* ---- parallel CSR: topo::csr colouring + mat::parcsr apply, all colours on
+ *      one device --------------------------------------------------------
+ * fsx_parcsr_create splits the FP64 matrix A over `ncolours` colours as
+ * topo::csr::color / init_mats do (topo/csr.hh:482-618): FleCSI equal_map
+ * row and column blocks; each owned row's entries, in CSR order, go to the
+ * colour's diag block (column in the colour's column block, local id) or
+ * its offd block (column ncols_local + rank in the colour's sorted unique
+ * ghost list, colmap).  A is copied (it may be destroyed afterwards).
+ * ncolours in [1, min(nrows, ncols)]. */
+typedef struct fsx_parcsr fsx_parcsr;
+int fsx_parcsr_create(fsx_ctx * ctx, const fsx_csr * A, int ncolours, fsx_parcsr ** out);
+int fsx_parcsr_destroy(fsx_parcsr * M);
+/* Colour c's index-space sizes (color(), csr.hh:524-528): sizes[0] rows,
+ * [1] local columns, [2] ghost columns, [3] diag entries, [4] offd entries. */
+int fsx_parcsr_colour_info(const fsx_parcsr * M, int colour, int64_t * sizes);
+/* Colour c's blocks as the reference stores them (int64 indices): diag
+ * rowptr (rows + 1) / colind (local ids) / vals, offd rowptr / colind
+ * (ncols_local + ghost rank) / vals, colmap (ghost global ids).  Any
+ * output may be NULL. */
+int fsx_parcsr_download(fsx_ctx * ctx, const fsx_parcsr * M, int colour,
+                        int64_t * diag_rowptr, int64_t * diag_colind, double * diag_vals,
+                        int64_t * offd_rowptr, int64_t * offd_colind, double * offd_vals,
+                        int64_t * colmap);
+/* y = A x as parcsr_ops::spmv computes it over the colours
+ * (matrices/parcsr.hh:60-91): ghost copy of x, tmp = offd x_ghost,
+ * y = diag x_local, y = y + tmp, each block summed from 0 in CSR order.
+ * Bit-identical to the reference's ncolours-colour product. */
+int fsx_parcsr_spmv(fsx_ctx * ctx, const fsx_parcsr * M, const fsx_vec * x, fsx_vec * y);
+/* r = b - A x (op::core::residual, operators/core.hh:129-142). */
+int fsx_parcsr_residual(fsx_ctx * ctx, const fsx_parcsr * M, const fsx_vec * b,
+                        const fsx_vec * x, fsx_vec * r);
+/* The diag blocks of all colours as one CSR with global column ids (owned
+ * by M). */
+int fsx_parcsr_diag(const fsx_parcsr * M, fsx_csr ** D);
+
 /* ---- fused kernels (CG iteration pieces, cg.hh:95-130) ----------------- */
 /* w = A p and out = w . p in one pass (cg.hh:95-98) */
 int fsx_spmv_dot(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p,
@@ -489,6 +524,9 @@ int fsx_time_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
 /* Same for a plain device copy of n doubles (bandwidth reference). */
 int fsx_time_copy(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst, int reps,
                   double * avg_ms);
+/* Same for fsx_parcsr_spmv (ghost copy, diag SpMV and offd rows together). */
+int fsx_time_parcsr_spmv(fsx_ctx * ctx, const fsx_parcsr * M, const fsx_vec * x,
+                         fsx_vec * y, int reps, double * avg_ms);
 
 #ifdef __cplusplus
 }

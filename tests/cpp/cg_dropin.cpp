@@ -6,7 +6,8 @@
 // Also runs GMRES and BiCGSTAB through the same API, and CG on a Matrix
 // Market file read with matrix_market<>::read(...).tocsr() (argv[1]), and
 // solvers chosen at run time from config files (argv[2]: tests/golden), and
-// the fused CG and mixed-precision refinement policies (solvers/fused.hh).
+// the fused CG and mixed-precision refinement policies (solvers/fused.hh),
+// and CG over the parallel CSR operator (matrices/device_parcsr.hh).
 // Exit status 0 = pass.
 #include <cmath>
 #include <cstdio>
@@ -14,6 +15,7 @@
 #include <vector>
 
 #include "flecsolve/matrices/device_csr.hh"
+#include "flecsolve/matrices/device_parcsr.hh"
 #include "flecsolve/matrices/io/matrix_market.hh"
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
@@ -182,6 +184,43 @@ int main(int argc, char ** argv) {
 		            "true rel res %.3e, |x_ir - x_cg| / |x_cg| %.2e\n",
 		            c1.iters, c2.restarts, c2.iters, rel2, dx);
 		fails += !c1.success() || !c2.success() || rel2 > 1.01e-10 || dx > 1e-6;
+	}
+	{
+		// the reference's parcsr test pattern (matrices/test/par_csr.cc:21-35):
+		// b = 0, x = 2, CG through op::core of the parallel CSR; here over 1
+		// and 4 colours against the plain CSR operator.  One colour is the
+		// plain product, so its solve is the same bit for bit; four colours
+		// round some rows differently and must converge alike.
+		op::core<mat::device_csr> A(ctx, Ah);
+		cg::settings ps;
+		ps.maxiter = 3000;
+		ps.rtol = 1e-8;
+		ps.use_zero_guess = false;
+		vec::device<double> pb(ctx, n * n), px(ctx, n * n), r(ctx, n * n);
+		pb.set_scalar(0.0);
+		px.set_scalar(2.0);
+		auto plain = cg::solver(ps, vec::make_device_work<cg::nwork>(pb))(op::ref(A));
+		solve_info p0 = plain(pb, px);
+		int iters[2];
+		double dx[2];
+		for (int k = 0; k < 2; ++k) {
+			op::core<mat::device_parcsr> M(ctx, Ah, k == 0 ? 1 : 4);
+			vec::device<double> y(ctx, n * n);
+			y.set_scalar(2.0);
+			auto slv = cg::solver(ps, vec::make_device_work<cg::nwork>(pb))(op::ref(M));
+			solve_info pi = slv(pb, y);
+			iters[k] = pi.iters;
+			y.axpy(-1.0, px, y);
+			// the 2-norm: inf_norm starts from the smallest normal double
+			// (vectors/seq.hh:251), so it never reads exactly 0
+			dx[k] = y.l2norm().get();
+			fails += !pi.success();
+		}
+		std::printf("parcsr cg (b = 0, x0 = 2): plain %d iterations; 1 colour %d (|dx| "
+		            "%.1e); 4 colours %d (|dx| %.1e)\n",
+		            p0.iters, iters[0], dx[0], iters[1], dx[1]);
+		fails += !p0.success() || iters[0] != p0.iters || dx[0] != 0.0 ||
+		         std::abs(iters[1] - p0.iters) > 2 || dx[1] > 1e-6;
 	}
 	if (argc > 2) {
 		// run-time solver choice from config files (krylov_factory), as the

@@ -1,0 +1,164 @@
+"""Oracle parallel CSR (topo::csr colouring + parcsr_ops::spmv of
+flecsolve/topo/csr.hh, flecsolve/matrices/parcsr.hh) against what the
+mathematics and the reference's structure fix.
+
+Pins:
+  * one colour: diag = A entry for entry, empty offd and ghost list;
+  * the 2D five-point Laplacian split over whole grid lines: closed-form
+    ghost lists (the neighbouring colours' edge lines), offd counts, and
+    diag = the principal sub-matrix of the colour;
+  * a rectangular [I | I] matrix: hand-derived per-colour blocks;
+  * reassembly: on random ragged matrices with unsorted and repeated
+    columns the blocks give back every row's entries, in CSR order, with
+    ghost columns mapped back through colmap;
+  * the product: exact on integer data, bit-identical to the serial product
+    on colour-block-diagonal matrices, and the reference's order on a row
+    whose serial order rounds differently ([1e16 offd, 1 diag, -1e16 offd]).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from flecsolve_amd import problems as P
+from oracle import parcsr as PC
+
+
+def random_ragged(nrows, ncols, seed, maxlen=9, repeat=True):
+    """Rows of 0..maxlen entries, columns in random order, some repeated."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, maxlen + 1, nrows)
+    lens[rng.integers(0, nrows)] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = rng.integers(0, ncols, rp[-1]).astype(np.int64)
+    if repeat and rp[-1] > 4:
+        ci[3] = ci[2]
+    v = rng.standard_normal(rp[-1])
+    return rp, ci, v
+
+
+def serial(rp, ci, v, x):
+    return PC.block_spmv(rp, ci, v, x)
+
+
+def test_equal_map():
+    assert list(PC.equal_map(10, 3)) == [0, 4, 7, 10]
+    assert list(PC.equal_map(9, 3)) == [0, 3, 6, 9]
+    assert list(PC.equal_map(5, 5)) == [0, 1, 2, 3, 4, 5]
+
+
+def test_one_colour_is_identity():
+    rp, ci, v = random_ragged(40, 40, 1)
+    (b,) = PC.split(rp, ci, v, 40, 40, 1)
+    assert b["rows"] == 40 and b["cols"] == 40
+    np.testing.assert_array_equal(b["diag_rowptr"], rp)
+    np.testing.assert_array_equal(b["diag_colind"], ci)
+    np.testing.assert_array_equal(b["diag_vals"], v)
+    assert b["offd_vals"].size == 0 and b["colmap"].size == 0
+    np.testing.assert_array_equal(b["offd_rowptr"], np.zeros(41, np.int64))
+
+
+@pytest.mark.parametrize("n,ncol", [(12, 3), (12, 4), (16, 2)])
+def test_laplacian_grid_lines(n, ncol):
+    rp, ci, v = P.poisson2d(n)
+    N = n * n
+    A = sp.csr_matrix((v, ci, rp), shape=(N, N))
+    blocks = PC.split(rp, ci, v, N, N, ncol)
+    lines = n // ncol
+    for c, b in enumerate(blocks):
+        r0, r1 = c * lines * n, (c + 1) * lines * n
+        assert b["rows"] == b["cols"] == r1 - r0
+        want = []
+        if c > 0:
+            want += list(range(r0 - n, r0))
+        if c < ncol - 1:
+            want += list(range(r1, r1 + n))
+        np.testing.assert_array_equal(b["colmap"], want)
+        assert b["offd_vals"].size == n * ((c > 0) + (c < ncol - 1))
+        assert np.all(b["offd_vals"] == -1.0)
+        D = sp.csr_matrix((b["diag_vals"], b["diag_colind"], b["diag_rowptr"]),
+                          shape=(r1 - r0, r1 - r0))
+        np.testing.assert_array_equal(D.toarray(), A[r0:r1, r0:r1].toarray())
+
+
+def test_rectangular_hand_derived():
+    # A = [I | I], 6 x 12, three colours: rows [0,2,4,6], columns [0,4,8,12]
+    n = 6
+    rp = np.arange(0, 2 * n + 1, 2, dtype=np.int64)
+    ci = np.array([[i, i + n] for i in range(n)], np.int64).ravel()
+    v = np.arange(1.0, 2 * n + 1)
+    b0, b1, b2 = PC.split(rp, ci, v, n, 2 * n, 3)
+    assert (b0["rows"], b0["cols"]) == (2, 4)
+    np.testing.assert_array_equal(b0["diag_colind"], [0, 1])
+    np.testing.assert_array_equal(b0["colmap"], [6, 7])
+    np.testing.assert_array_equal(b0["offd_colind"], [4, 5])
+    np.testing.assert_array_equal(b1["diag_rowptr"], [0, 0, 0])
+    np.testing.assert_array_equal(b1["colmap"], [2, 3, 8, 9])
+    np.testing.assert_array_equal(b1["offd_colind"], [4, 6, 5, 7])
+    np.testing.assert_array_equal(b1["offd_vals"], [5.0, 6.0, 7.0, 8.0])
+    np.testing.assert_array_equal(b2["colmap"], [4, 5])
+    np.testing.assert_array_equal(b2["diag_colind"], [2, 3])
+    np.testing.assert_array_equal(b2["offd_colind"], [4, 5])
+
+
+@pytest.mark.parametrize("ncol", [2, 3, 7])
+@pytest.mark.parametrize("shape", [(50, 50), (37, 61), (64, 29)])
+def test_reassembly(ncol, shape):
+    nr, nc = shape
+    rp, ci, v = random_ragged(nr, nc, 100 + ncol)
+    blocks = PC.split(rp, ci, v, nr, nc, ncol)
+    rb, cb = PC.equal_map(nr, ncol), PC.equal_map(nc, ncol)
+    for c, b in enumerate(blocks):
+        assert np.all(np.diff(b["colmap"]) > 0)
+        for lr in range(b["rows"]):
+            r = rb[c] + lr
+            row_c, row_v = ci[rp[r]:rp[r + 1]], v[rp[r]:rp[r + 1]]
+            own = (row_c >= cb[c]) & (row_c < cb[c + 1])
+            d0, d1 = b["diag_rowptr"][lr:lr + 2]
+            o0, o1 = b["offd_rowptr"][lr:lr + 2]
+            np.testing.assert_array_equal(b["diag_colind"][d0:d1] + cb[c], row_c[own])
+            np.testing.assert_array_equal(b["diag_vals"][d0:d1], row_v[own])
+            g = b["colmap"][b["offd_colind"][o0:o1] - b["cols"]]
+            np.testing.assert_array_equal(g, row_c[~own])
+            np.testing.assert_array_equal(b["offd_vals"][o0:o1], row_v[~own])
+
+
+@pytest.mark.parametrize("ncol", [1, 2, 5])
+def test_product_exact_on_integers(ncol):
+    rng = np.random.default_rng(ncol)
+    nr, nc = 80, 90
+    rp, ci, _ = random_ragged(nr, nc, 7)
+    v = rng.integers(-9, 10, rp[-1]).astype(np.float64)
+    x = rng.integers(-9, 10, nc).astype(np.float64)
+    y = PC.spmv(PC.split(rp, ci, v, nr, nc, ncol), x, nr)
+    A = sp.csr_matrix((v, ci, rp), shape=(nr, nc))
+    np.testing.assert_array_equal(y, A @ x)
+
+
+def test_product_block_diagonal_matches_serial():
+    rng = np.random.default_rng(3)
+    n, ncol = 60, 4
+    bounds = PC.equal_map(n, ncol)
+    rows = []
+    for c in range(ncol):
+        for r in range(bounds[c], bounds[c + 1]):
+            k = rng.integers(1, 6)
+            rows.append(rng.integers(bounds[c], bounds[c + 1], k))
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+    ci = np.concatenate(rows).astype(np.int64)
+    v = rng.standard_normal(rp[-1])
+    x = rng.standard_normal(n)
+    blocks = PC.split(rp, ci, v, n, n, ncol)
+    assert all(b["offd_vals"].size == 0 for b in blocks)
+    np.testing.assert_array_equal(PC.spmv(blocks, x, n), serial(rp, ci, v, x))
+
+
+def test_product_follows_reference_order():
+    # row 0 of a 2-colour 4x4 matrix: columns 3 (offd), 0 (diag), 2 (offd)
+    rp = np.array([0, 3, 4, 5, 6], np.int64)
+    ci = np.array([3, 0, 2, 1, 2, 3], np.int64)
+    v = np.array([1e16, 1.0, -1e16, 1.0, 1.0, 1.0])
+    x = np.ones(4)
+    assert serial(rp, ci, v, x)[0] == 0.0          # (1e16 + 1) - 1e16
+    y = PC.spmv(PC.split(rp, ci, v, 4, 4, 2), x, 4)
+    assert y[0] == 1.0                              # 1 + (1e16 - 1e16)
+    np.testing.assert_array_equal(y[1:], [1.0, 1.0, 1.0])
