@@ -355,6 +355,8 @@ int fsx_parcsr_spmv(fsx_ctx * ctx, const fsx_parcsr * M, const fsx_vec * x, fsx_
 	return guard([&] {
 		FSX_REQUIRE(ctx, "null context");
 		FSX_REQUIRE(M, "null matrix");
+		// the ghost buffer is M's own, ordered on the stream of M's context
+		FSX_REQUIRE(ctx == M->ctx, "parcsr spmv: use the context the matrix was created on");
 		check_vec(x, "x");
 		check_vec(y, "y");
 		FSX_REQUIRE(x->dt == FSX_F64 && y->dt == FSX_F64, "parcsr spmv: FP64 vectors");

@@ -232,7 +232,9 @@ int fsx_parcsr_download(fsx_ctx * ctx, const fsx_parcsr * M, int colour,
 /* y = A x as parcsr_ops::spmv computes it over the colours
  * (matrices/parcsr.hh:60-91): ghost copy of x, tmp = offd x_ghost,
  * y = diag x_local, y = y + tmp, each block summed from 0 in CSR order.
- * Bit-identical to the reference's ncolours-colour product. */
+ * Bit-identical to the reference's ncolours-colour product.  ctx must be
+ * the context M was created on (M's ghost buffer is ordered on its
+ * stream). */
 int fsx_parcsr_spmv(fsx_ctx * ctx, const fsx_parcsr * M, const fsx_vec * x, fsx_vec * y);
 /* r = b - A x (op::core::residual, operators/core.hh:129-142). */
 int fsx_parcsr_residual(fsx_ctx * ctx, const fsx_parcsr * M, const fsx_vec * b,

@@ -133,3 +133,10 @@ def test_errors(ctx):
         Mp.download(2)
     with pytest.raises(fsx.FsxError):
         Mp.residual(x, ctx.vec(np.ones(64)), x)
+    # the ghost buffer is ordered on the creating context's stream only
+    ctx2 = fsx.Context(0)
+    try:
+        assert fsx.lib().fsx_parcsr_spmv(ctx2.h, Mp.h, x.h, ctx.vec(np.ones(64)).h) != 0
+        assert "context" in fsx.lib().fsx_last_error().decode()
+    finally:
+        ctx2.close()
