@@ -134,6 +134,9 @@ def main():
                     default="diffusion")
     ap.add_argument("--colours", type=int, default=8,
                     help="parcsr workload: number of colours (row blocks)")
+    ap.add_argument("--parcsr-spmv-only", action="store_true",
+                    help="parcsr workload: time the parcsr product and the copy kernel "
+                         "only (profiling runs)")
     ap.add_argument("--heat-steps", type=int, default=50)
     ap.add_argument("--ckpt", type=int, default=10)
     ap.add_argument("--fd-directions", type=int, default=20)
@@ -1052,6 +1055,16 @@ def bench_parcsr(a, fsx, P):
     log(f"parcsr {n}^3 over {ncol} colours: split in {t_split:.1f} s; {ng} ghosts, "
         f"{nnz_o} offd entries; diag plan {Mp.diag_plan()}")
     bd, y1, y2 = ctx.vec(b), ctx.zeros(N), ctx.zeros(N)
+    if a.parcsr_spmv_only:
+        # the parcsr product alone, then the copy kernel that calibrates the
+        # PMC byte counters (profiling runs)
+        ms = fsx_time(fsx, "fsx_time_parcsr_spmv", ctx, Mp, bd, y2, a.spmv_reps)
+        copy_ms = fsx_time(fsx, "fsx_time_copy", ctx, None, bd, y1, a.spmv_reps)
+        log(f"parcsr spmv {ms:.4f} ms; copy kernel {N} doubles {copy_ms:.4f} ms")
+        print(json.dumps({"metric": "parcsr SpMV", "config": {
+            "workload": f"parcsr_spmv_{n}^3_{ncol}colours"}, "avg_ms": round(ms, 5)}),
+            flush=True)
+        return
     A.spmv(bd, y1)
     Mp.spmv(bd, y2)
     ctx.sync()
@@ -1086,6 +1099,11 @@ def bench_parcsr(a, fsx, P):
     B = B_diag + B_gather + B_offd
     gbs = B / (ms * 1e-3) / 1e9
     B_plain = spmv_bytes(N, nnz, 8)
+    traffic, traffic_src = None, None
+    pmc = ROOT / "profiles" / "parcsr_spmv_256cube_8colours_pmc.json"
+    if n == 256 and ncol == 8 and abs(a.sigma - 1.0) < 1e-12 and pmc.exists():
+        traffic = round(json.loads(pmc.read_text())["traffic_bytes"])
+        traffic_src = "profiles/parcsr_spmv_256cube_8colours_pmc.md"
     out = {
         "metric": f"parallel CSR SpMV over {ncol} colours on one device, {n}^3 diffusion",
         "value": round(ms, 4),
@@ -1108,7 +1126,8 @@ def bench_parcsr(a, fsx, P):
                    "interior_rows_bit_identical": interior_exact,
                    "boundary_rows_max_rel_diff": rel},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "algorithmic_bytes": B,
                      "bytes": {"diag_spmv": B_diag, "ghost_gather": B_gather,
                                "offd_rows": B_offd},
