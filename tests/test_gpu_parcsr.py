@@ -74,6 +74,14 @@ def test_random_ragged(ctx, itype, ncol, shape):
     run(ctx, rp.astype(itype), ci.astype(itype), v, nr, nc, ncol, seed=ncol)
 
 
+@pytest.mark.parametrize("shape", [(16, 40), (40, 16)])
+def test_colours_at_the_size_limit(ctx, shape):
+    # ncolours = min(nrows, ncols): single-row (or single-column) blocks
+    nr, nc = shape
+    rp, ci, v = random_ragged(nr, nc, 12)
+    run(ctx, rp.astype(np.int32), ci.astype(np.int32), v, nr, nc, min(nr, nc), seed=4)
+
+
 def test_long_rows_keep_csr_order(ctx):
     # about 24 entries a row: the plain CSR takes the vector-per-row mapping,
     # the diag blocks must not

@@ -122,6 +122,25 @@ def test_reassembly(ncol, shape):
             np.testing.assert_array_equal(b["offd_vals"][o0:o1], row_v[~own])
 
 
+def test_one_row_per_colour():
+    # P = nrows: colour c owns row c and column block c (of 40 / 16 columns);
+    # every entry outside that block is a ghost of the row's own colour
+    nr, nc = 16, 40
+    rp, ci, v = random_ragged(nr, nc, 5)
+    blocks = PC.split(rp, ci, v, nr, nc, nr)
+    cb = PC.equal_map(nc, nr)
+    for c, b in enumerate(blocks):
+        row = ci[rp[c]:rp[c + 1]]
+        own = (row >= cb[c]) & (row < cb[c + 1])
+        assert b["rows"] == 1 and b["cols"] == cb[c + 1] - cb[c]
+        np.testing.assert_array_equal(b["colmap"], np.unique(row[~own]))
+        assert b["diag_vals"].size == own.sum()
+    x = np.random.default_rng(1).integers(-5, 6, nc).astype(np.float64)
+    vi = np.round(v * 4)
+    A = sp.csr_matrix((vi, ci, rp), shape=(nr, nc))
+    np.testing.assert_array_equal(PC.spmv(PC.split(rp, ci, vi, nr, nc, nr), x, nr), A @ x)
+
+
 @pytest.mark.parametrize("ncol", [1, 2, 5])
 def test_product_exact_on_integers(ncol):
     rng = np.random.default_rng(ncol)
