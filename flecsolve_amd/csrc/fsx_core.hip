@@ -234,6 +234,24 @@ __global__ __launch_bounds__(kRedThreads) void reduce_finalize_kernel(
 		*out = red_comb<K>(red_comb<K>(red_comb<K>(sh[0], sh[1]), sh[2]), sh[3]);
 }
 
+// sum of pow(x_i, p): the general-p lp_norm of the FleCSI vector path
+// (vectors/operations/topo_tasks.hh:216-226 takes std::pow(u, p) with no
+// absolute value)
+template<class T>
+__global__ __launch_bounds__(kRedThreads) void powsum_kernel(size_t n, const T * __restrict__ x,
+                                                             double p,
+                                                             double * __restrict__ partials) {
+	double acc = 0.0;
+	const size_t stride = (size_t)gridDim.x * kRedThreads;
+	for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += stride)
+		acc += pow((double)x[i], p);
+	__shared__ double sh[4];
+	acc = wave_red<RED_DOT>(acc);
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+	__syncthreads();
+	if (threadIdx.x == 0) partials[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
 template<red_kind K>
 static void reduce_launch_k(fsx_ctx * ctx, const fsx_vec * x,
                             const fsx_vec * y, double * dslot) {
@@ -735,6 +753,33 @@ int fsx_nrminf(fsx_ctx * ctx, const fsx_vec * x, double * out) {
 	return guard([&] {
 		check_ctx(ctx); check_vec(x, "x");
 		*out = reduce_host(ctx, RED_INF, x, nullptr);
+	});
+}
+int fsx_nrmp(fsx_ctx * ctx, const fsx_vec * x, int p, double * out) {
+	return guard([&] {
+		check_ctx(ctx); check_vec(x, "x");
+		FSX_REQUIRE(out, "null output");
+		FSX_REQUIRE(p >= 1, "lp norm: p must be >= 1");
+		if (p == 1) { *out = reduce_host(ctx, RED_L1, x, nullptr); return; }
+		if (p == 2) { *out = std::sqrt(reduce_host(ctx, RED_L2SQ, x, nullptr)); return; }
+		double sum = 0.0;
+		if (x->n > 0) {
+			const int g = red_grid(x->n);
+			double * part = ensure_partials(ctx, (size_t)g);
+			if (x->dt == FSX_F64)
+				hipLaunchKernelGGL((powsum_kernel<double>), dim3(g), dim3(kRedThreads), 0,
+				                   ctx->stream, x->n, (const double *)x->ptr(), (double)p, part);
+			else
+				hipLaunchKernelGGL((powsum_kernel<float>), dim3(g), dim3(kRedThreads), 0,
+				                   ctx->stream, x->n, (const float *)x->ptr(), (double)p, part);
+			double * slot = ctx->ws.scalars;
+			hipLaunchKernelGGL((reduce_finalize_kernel<RED_DOT>), dim3(1), dim3(kRedThreads), 0,
+			                   ctx->stream, (const double *)part, (unsigned)g, slot);
+			FSX_HIP(hipGetLastError());
+			ctx->launches += 2;
+			sum = fetch_scalar(ctx, slot);
+		}
+		*out = std::pow(sum, 1. / p);
 	});
 }
 int fsx_min(fsx_ctx * ctx, const fsx_vec * x, double * out) {

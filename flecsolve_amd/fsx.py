@@ -105,6 +105,7 @@ _SIGS = {
     "fsx_nrm1": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "fsx_nrm2": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "fsx_nrminf": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    "fsx_nrmp": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_double)]),
     "fsx_min": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "fsx_max": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "fsx_csr_create": (C.c_int, [_P, C.c_int64, C.c_int64, C.c_int64, C.c_int,
@@ -471,6 +472,13 @@ def nrm1(x: Vector) -> float:
 
 def nrminf(x: Vector) -> float:
     return _red("fsx_nrminf", x.ctx, x)
+
+
+def nrmp(x: Vector, p: int) -> float:
+    """lp_norm<p> of the FleCSI vector path: (sum x_i^p)^(1/p), no abs."""
+    out = C.c_double()
+    check(lib().fsx_nrmp(x.ctx.h, x.h, int(p), C.byref(out)), "fsx_nrmp")
+    return out.value
 
 
 def vmin(x: Vector) -> float:

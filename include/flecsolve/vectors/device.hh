@@ -18,11 +18,13 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <string_view>
 #include <utility>
+#include <vector>
 
 #include "fsx.h"
 #include "flecsolve/vectors/core.hh"
@@ -172,7 +174,17 @@ struct device_ops {
 	static void set_random(Data & x, unsigned seed) {
 		fsx_check(fsx_set_random(x.ctx(), x.get(), seed), "set_random");
 	}
-	static void dump(std::string_view, const Data &) {}
+	// one value per line to "<pre>-<colour>" (the dump task of
+	// vectors/operations/topo_tasks.hh:316-323; one colour here: "-0")
+	static void dump(std::string_view pre, const Data & x) {
+		std::vector<scalar> h(x.size());
+		if (!h.empty())
+			fsx_check(fsx_vec_download(x.ctx(), x.get(), h.data(), h.size()), "dump");
+		std::string fname{pre};
+		fname += "-0";
+		std::ofstream ofile(fname);
+		for (const scalar & v : h) ofile << v << '\n';
+	}
 
 	static ready<real> min(const Data & x) {
 		double v;
@@ -184,12 +196,15 @@ struct device_ops {
 		fsx_check(fsx_max(x.ctx(), x.get(), &v), "max");
 		return {(real)v};
 	}
+	// p >= 3: (sum pow(x_i, p))^(1/p) as the FleCSI vector path computes it
+	// (vectors/operations/topo_view.hh:243-257; no absolute value)
 	template<unsigned short p>
 	static ready<real> lp_norm(const Data & x) {
-		static_assert(p == 1 || p == 2, "lp_norm: p must be 1 or 2");
+		static_assert(p >= 1, "lp_norm: p must be >= 1");
 		double v;
 		if constexpr (p == 1) fsx_check(fsx_nrm1(x.ctx(), x.get(), &v), "l1norm");
-		else fsx_check(fsx_nrm2(x.ctx(), x.get(), &v), "l2norm");
+		else if constexpr (p == 2) fsx_check(fsx_nrm2(x.ctx(), x.get(), &v), "l2norm");
+		else fsx_check(fsx_nrmp(x.ctx(), x.get(), p, &v), "lp_norm");
 		return {(real)v};
 	}
 	static ready<real> inf_norm(const Data & x) {

@@ -148,6 +148,11 @@ int fsx_reduce_async(fsx_ctx * ctx, int kind, const fsx_vec * x, const fsx_vec *
 int fsx_nrm1(fsx_ctx * ctx, const fsx_vec * x, double * out);
 int fsx_nrm2(fsx_ctx * ctx, const fsx_vec * x, double * out);
 int fsx_nrminf(fsx_ctx * ctx, const fsx_vec * x, double * out);
+/* lp_norm<p> of the FleCSI vector path (vectors/operations/topo_view.hh:
+ * 243-257, topo_tasks.hh:216-226): p = 1 and 2 as fsx_nrm1 / fsx_nrm2;
+ * p >= 3 gives (sum_i pow(x_i, p))^(1/p) -- with no absolute value, as
+ * there (NaN when an odd p makes the sum negative). */
+int fsx_nrmp(fsx_ctx * ctx, const fsx_vec * x, int p, double * out);
 int fsx_min(fsx_ctx * ctx, const fsx_vec * x, double * out);
 int fsx_max(fsx_ctx * ctx, const fsx_vec * x, double * out);
 

@@ -11,7 +11,10 @@
 // Exit status 0 = pass.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <fstream>
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "flecsolve/matrices/device_csr.hh"
@@ -83,6 +86,24 @@ int main(int argc, char ** argv) {
 	                         ci.data(), FSX_F64, v.data(), &Ah),
 	          "csr");
 	int fails = 0;
+	{
+		// the rest of the vector surface: lp_norm<3> of the FleCSI path
+		// ((sum x^3)^(1/3), no abs: 7^(1/3) for [-1, 2]) and dump to
+		// "<pre>-0", one value per line (topo_tasks.hh:216-226, 316-323)
+		vec::device<double> w(ctx, 2);
+		const double hw[2] = {-1.0, 2.0};
+		fsx_check(fsx_vec_upload(ctx, w.data.get(), hw, 2), "upload");
+		const double l3 = w.lp_norm<3>().get();
+		const std::string pre = std::string(std::getenv("TMPDIR") ? std::getenv("TMPDIR") : "/tmp") +
+		                        "/fsx_dump_check";
+		w.dump(pre);
+		std::ifstream in(pre + "-0");
+		double a = 0, b = 0;
+		in >> a >> b;
+		std::printf("vector surface: lp_norm<3> %.17g, dump read back %g %g\n", l3, a, b);
+		fails += std::abs(l3 - std::cbrt(7.0)) > 1e-15 || a != -1.0 || b != 2.0;
+		std::remove((pre + "-0").c_str());
+	}
 	{
 		op::core<mat::device_csr> A(ctx, Ah);
 		vec::device<double> b(ctx, n * n), x(ctx, n * n);

@@ -121,6 +121,33 @@ def test_reduction_edge_cases(ctx):
     assert fsx.dot(e, e) == 0.0
 
 
+@pytest.mark.parametrize("n", [1, 1000, 300_001])
+def test_lp_norm_general_p(ctx, n):
+    """lp_norm<p> for p >= 3 as the FleCSI vector path computes it
+    (vectors/operations/topo_tasks.hh:216-226): (sum x^p)^(1/p) with no
+    absolute value; p = 1, 2 are the l1 / l2 norms.  Closed forms: x = c
+    (constant) gives c n^(1/p); x = [-1, 2] at p = 3 gives 7^(1/3) where the
+    absolute-value form would give 9^(1/3)."""
+    rng = np.random.default_rng(n)
+    x = rng.random(n) + 0.5
+    xd = ctx.vec(x)
+    assert fsx.nrmp(xd, 1) == fsx.nrm1(xd)
+    assert fsx.nrmp(xd, 2) == fsx.nrm2(xd)
+    for p in (3, 4, 7):
+        want = np.sum(x ** p) ** (1.0 / p)
+        assert abs(fsx.nrmp(xd, p) - want) <= 1e-13 * want, p
+        assert abs(fsx.nrmp(ctx.vec(x.astype(np.float32)), p)
+                   - np.sum(x.astype(np.float32).astype(np.float64) ** p) ** (1.0 / p)) \
+            <= 1e-13 * want
+    c = ctx.vec(np.full(n, 0.75))
+    assert abs(fsx.nrmp(c, 5) - 0.75 * n ** 0.2) <= 1e-13
+    assert abs(fsx.nrmp(ctx.vec(np.array([-1.0, 2.0])), 3) - 7 ** (1 / 3)) <= 1e-15
+    assert np.isnan(fsx.nrmp(ctx.vec(np.array([-2.0, 1.0])), 3))
+    assert fsx.nrmp(ctx.zeros(0), 3) == 0.0
+    with pytest.raises(fsx.FsxError):
+        fsx.nrmp(xd, 0)
+
+
 def test_set_random_matches_reference_stream(ctx):
     v = ctx.zeros(1001)
     fsx.check(fsx.lib().fsx_set_random(ctx.h, v.h, 7))
