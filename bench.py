@@ -67,49 +67,6 @@ def spmv_bytes(nrows, nnz, vbytes, ibytes=4, xbytes=8):
     return nnz * (vbytes + ibytes) + (nrows + 1) * ibytes + nrows * xbytes * 2
 
 
-def cpu_threads():
-    """Host threads for the threaded CPU baseline: the CPUs this process may
-    use, at most 16 (the host share of one GPU)."""
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
-
-
-def cpu_baseline(rp, ci, v, b, iters_gpu, sample_iters):
-    """The oracle (restatement of the reference's CG + Dinv path) on a
-    bounded sample of the same problem: `sample_iters` iterations, scaled to
-    the GPU solve's iteration count.  Timed twice: serial (the reference's
-    seq path, one thread) and with P colours on P threads (the reference's
-    MPI backend with one colour per rank: row blocks, block partials summed
-    in block order); `value` is the P-thread figure."""
-    from oracle import oracle as O
-    A = O.Csr(rp, ci, v)
-    D = A.dinv()
-    s = O.Settings(maxiter=sample_iters, rtol=1e-10, use_zero_guess=True)
-    t0 = time.perf_counter()
-    O.cg(A, b, np.zeros_like(b), s, P=O.prec(1, D))
-    dt1 = time.perf_counter() - t0
-    P = cpu_threads()
-    dinv = D.vals
-    t0 = time.perf_counter()
-    O.cg_colours(A, dinv, b, np.zeros_like(b), s, P)
-    dtp = time.perf_counter() - t0
-    return {
-        "value": dtp / sample_iters * iters_gpu,
-        "unit": "s",
-        "cores": P,
-        "kind": "port",
-        "single_thread_value": dt1 / sample_iters * iters_gpu,
-        "sample": (f"{sample_iters} CG+Jacobi FP64 iterations of the same {len(b)}-row "
-                   f"problem: {P} colours on {P} threads {dtp:.1f} s "
-                   f"({dtp / sample_iters * 1e3:.0f} ms/iter), 1 thread {dt1:.1f} s "
-                   f"({dt1 / sample_iters * 1e3:.0f} ms/iter); scaled to the GPU's "
-                   f"{iters_gpu} iterations"),
-    }
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,8 +76,14 @@ def main():
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--cpu-sample-iters", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-iters", type=int, default=10,
+                    help="iterations of the one-thread oracle sample")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads/colours (0: the usable CPUs, at most 16)")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the oracle's full-size solve (CPU baseline and parity)")
+    ap.add_argument("--no-sigma25", action="store_true",
+                    help="skip the sigma = 2.5 comparison")
     ap.add_argument("--maxiter", type=int, default=5000,
                     help="CG iteration cap (FP64 solve and each inner solve)")
     ap.add_argument("--max-outer", type=int, default=10)
@@ -128,7 +91,7 @@ def main():
                     help="inner FP32 solve: 1 Jacobi-preconditioned CG on A32, "
                          "2 CG on the symmetrically Jacobi-scaled A32")
     ap.add_argument("--spmv-only", action="store_true",
-                    help="time the SpMV and copy kernels only (profiling runs)")
+                    help="time the CG K1, SpMV and copy kernels only (profiling runs)")
     ap.add_argument("--workload", choices=["diffusion", "batched", "adjoint", "convdiff",
                                            "smoother", "amg", "heat", "parcsr"],
                     default="diffusion")
@@ -178,13 +141,68 @@ def main():
     if a.workload == "parcsr":
         return bench_parcsr(a, fsx, P)
 
+    return bench_diffusion(a, fsx, P)
+
+
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def diffusion_problem(P, n, sigma):
     seeds = P.seeds()
-    n = a.n
-    t0 = time.perf_counter()
-    k = P.lognormal_field(n, a.sigma, 8.0, seeds["diffusion_k"])
+    k = P.lognormal_field(n, sigma, 8.0, seeds["diffusion_k"])
     rp, ci, v = P.diffusion3d(n, k)
     b = P.uniform(n ** 3, seeds["diffusion_rhs"])
-    del k
+    return rp, ci, v, b
+
+
+def solve_fp64(fsx, ctx, A64, dinv, bd, x, maxiter, rtol=1e-10):
+    s = fsx.Settings(maxiter=maxiter, rtol=rtol, use_zero_guess=True)
+    ctx.sync()
+    t = time.perf_counter()
+    info = fsx.cg_solve(A64, bd, x, s, dinv=dinv)
+    ctx.sync()
+    return info, time.perf_counter() - t
+
+
+def solve_ir(fsx, ctx, A64, A32, bd, x, a):
+    inner = fsx.Settings(maxiter=a.maxiter, rtol=a.inner_rtol)
+    fsx.check(fsx.lib().fsx_zero(ctx.h, x.h))  # x0 = 0
+    ctx.sync()
+    t = time.perf_counter()
+    ii = fsx.ir_cg_solve(A64, A32, bd, x, 1e-10, inner, jacobi=a.ir_jacobi,
+                         max_outer=a.max_outer)
+    ctx.sync()
+    return ii, time.perf_counter() - t
+
+
+def true_rel_res(fsx, ctx, A64, bd, x):
+    r = ctx.zeros(bd.n)
+    A64.residual(bd, x, r)
+    return fsx.nrm2(r) / fsx.nrm2(bd)
+
+
+def rel_err(x, y):
+    return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+
+def bench_diffusion(a, fsx, P):
+    n = a.n
+    t0 = time.perf_counter()
+    rp, ci, v, b = diffusion_problem(P, n, a.sigma)
     N, nnz = n ** 3, ci.size
     log(f"problem {n}^3: {N} rows, {nnz} nnz, generated in {time.perf_counter()-t0:.1f} s")
 
@@ -192,122 +210,118 @@ def main():
     t0 = time.perf_counter()
     A64 = ctx.csr(rp, ci, v)
     A32 = A64.convert(fsx.F32)
+    if a.ir_jacobi == 2:
+        A32.set_values_sym_scaled(A64)  # the refinement's inner matrix (setup)
     dinv = A64.diag_inv()
     bd = ctx.vec(b)
     x = ctx.zeros(N)
     log(f"uploaded and layouts built in {time.perf_counter()-t0:.1f} s; "
         f"SpMV plan {A64.plan()}")
 
-    # ---- SpMV roofline (HIP events on the library stream) ----------------
+    # ---- kernel roofline (HIP events on the library stream) --------------
+    # The timed region's dominant kernel is the inner FP32 CG's K1 (w = A p
+    # and the partials of w.p on the refinement's FP32 matrix); the FP64
+    # product and the FP64 CG's K1 are reported beside it.
+    reps = a.spmv_reps
+    p32 = ctx.vec(b.astype(np.float32))
+    w32 = ctx.zeros(N, fsx.F32)
+    k1_ms = fsx_time(fsx, "fsx_time_cg_spmv", ctx, A32, p32, w32, reps)
+    k1_bytes = A32.layout_bytes(4)
+    k1_csr = spmv_bytes(N, nnz, 4, xbytes=4)
+    del p32, w32
     y = ctx.zeros(N)
-    fsx.check(fsx.lib().fsx_csr_spmv(ctx.h, A64.h, bd.h, y.h))
-    ms = fsx_time(fsx, "fsx_time_spmv", ctx, A64, bd, y, a.spmv_reps)
-    B64 = spmv_bytes(N, nnz, 8)
-    gbs64 = B64 / (ms * 1e-3) / 1e9
-    copy_ms = fsx_time(fsx, "fsx_time_copy", ctx, None, bd, y, a.spmv_reps)
+    spmv_ms = fsx_time(fsx, "fsx_time_spmv", ctx, A64, bd, y, reps)
+    k164_ms = fsx_time(fsx, "fsx_time_cg_spmv", ctx, A64, bd, y, reps)
+    b64 = A64.layout_bytes(8)
+    b64_csr = spmv_bytes(N, nnz, 8)
+    copy_ms = fsx_time(fsx, "fsx_time_copy", ctx, None, bd, y, reps)
     copy_gbs = 16.0 * N / (copy_ms * 1e-3) / 1e9
-    log(f"spmv fp64: {ms:.4f} ms, {gbs64:.0f} GB/s ({gbs64/HBM_PEAK_GBS:.1%} of 8 TB/s); "
-        f"copy kernel {copy_gbs:.0f} GB/s")
-    x32 = ctx.zeros(N, fsx.F32)
-    y32 = ctx.zeros(N, fsx.F32)
-    ms32 = fsx_time(fsx, "fsx_time_spmv", ctx, A32, x32, y32, a.spmv_reps)
-    B32 = spmv_bytes(N, nnz, 4, xbytes=4)
-    gbs32 = B32 / (ms32 * 1e-3) / 1e9
-    log(f"spmv fp32: {ms32:.4f} ms, {gbs32:.0f} GB/s")
-    del x32, y32, y
-    # HBM bytes per launch from the committed rocprofv3 PMC passes (FETCH_SIZE,
-    # WRITE_SIZE scaled by the copy-kernel calibration) for this workload, and
-    # the bytes the chosen layout must move (below the CSR-based algorithmic
-    # figure for the slice-offset layout, which stores no column per entry)
+    del y
+
+    def gbs(bytes_, ms):
+        return bytes_ / (ms * 1e-3) / 1e9
+
+    log(f"inner fp32 K1: {k1_ms:.4f} ms, {gbs(k1_bytes, k1_ms):.0f} GB/s on layout bytes "
+        f"({gbs(k1_bytes, k1_ms) / HBM_PEAK_GBS:.1%} of 8 TB/s)")
+    log(f"fp64 spmv: {spmv_ms:.4f} ms, {gbs(b64, spmv_ms):.0f} GB/s layout; fp64 K1 "
+        f"{k164_ms:.4f} ms; copy kernel {copy_gbs:.0f} GB/s")
+    # HBM bytes per launch measured by rocprofv3 PMC passes (FETCH_SIZE and
+    # WRITE_SIZE, scaled by the copy-kernel calibration), when profiled for
+    # this workload
+    traffic = {}
+    pmc = ROOT / "profiles" / "cg_k1_256cube_pmc.json"
+    if n == 256 and abs(a.sigma - 1.0) < 1e-12 and a.ir_jacobi == 2 and pmc.exists():
+        traffic = json.loads(pmc.read_text())
     plan = A64.plan()
-    kernels = {0: "spmv_stream_kernel (LDS-staged CSR)", 1: "spmv_vector_kernel",
-               2: "spmv_sliced_kernel (sliced rows)",
-               3: "spmv_sdia_kernel (slice-offset layout)"}
-    pmc_files = {2: "spmv_sliced_256cube_pmc", 3: "spmv_sdia_256cube_pmc"}
-    traffic, traffic_src, layout64, layout32 = None, None, None, None
-    if n == 256 and abs(a.sigma - 1.0) < 1e-12 and plan[0] in pmc_files:
-        pmc = ROOT / "profiles" / (pmc_files[plan[0]] + ".json")
-        if pmc.exists():
-            d = json.loads(pmc.read_text())
-            traffic = round(d["spmv_fp64"]["traffic_bytes"])
-            traffic_src = f"profiles/{pmc_files[plan[0]]}.md"
-            layout64 = d["spmv_fp64"].get("layout_bytes")
-            layout32 = d["spmv_fp32"].get("layout_bytes")
+    kname = {0: "spmv_stream_kernel (LDS-staged CSR)", 1: "spmv_vector_kernel",
+             2: "spmv_sliced_kernel (sliced rows)",
+             3: "spmv_sdia_kernel (slice-offset layout)"}[plan[0]]
     roofline = {
         "bound": "hbm",
-        "kernel": kernels[plan[0]] + ", FP64 values, int32 CSR input",
-        "achieved": round(gbs64, 1),
+        "kernel": f"{kname} with the fused CG epilogue (K1: w = A p, partials of w.p), "
+                  "FP32 values and vectors: the inner CG of the refinement",
+        "achieved": round(gbs(k1_bytes, k1_ms), 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "traffic_source": traffic_src,
-        "algorithmic_bytes": B64,
-        "avg_ms": round(ms, 5),
-        "layout_bytes": layout64,
-        "layout_frac": (round(layout64 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                        if layout64 else None),
-        "fp32_values": {"achieved": round(gbs32, 1), "avg_ms": round(ms32, 5),
-                        "algorithmic_bytes": B32, "layout_bytes": layout32},
+        "frac": round(gbs(k1_bytes, k1_ms) / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get("k1_fp32", {}).get("traffic_bytes"),
+        "algorithmic_bytes": k1_bytes,
+        "bytes_basis": "compulsory bytes of the slice-offset layout (values, offsets, "
+                       "lane masks, slice pointers; p read once, w written once)",
+        "avg_ms": round(k1_ms, 5),
+        # the CSR-with-int32-columns figure per product over the same time:
+        # a rate for comparison with CSR kernels, not a roofline fraction
+        # (this layout stores no column per entry)
+        "csr_equiv_bytes": k1_csr,
+        "csr_equiv_gbs": round(gbs(k1_csr, k1_ms), 1),
+        "fp64_spmv": {"avg_ms": round(spmv_ms, 5), "algorithmic_bytes": b64,
+                      "achieved": round(gbs(b64, spmv_ms), 1),
+                      "frac": round(gbs(b64, spmv_ms) / HBM_PEAK_GBS, 4),
+                      "csr_equiv_bytes": b64_csr,
+                      "csr_equiv_gbs": round(gbs(b64_csr, spmv_ms), 1),
+                      "traffic": traffic.get("spmv_fp64", {}).get("traffic_bytes")},
+        "fp64_k1": {"avg_ms": round(k164_ms, 5),
+                    "achieved": round(gbs(b64, k164_ms), 1),
+                    "frac": round(gbs(b64, k164_ms) / HBM_PEAK_GBS, 4),
+                    "traffic": traffic.get("k1_fp64", {}).get("traffic_bytes")},
         "copy_kernel_gbs": round(copy_gbs, 1),
+        "traffic_source": "profiles/cg_k1_256cube_pmc.md" if traffic else None,
     }
     if a.spmv_only:
-        print(json.dumps({"metric": "SpMV % HBM peak", "config": {
-            "workload": f"spmv diffusion3d_{n}^3", "rows": N, "nnz": int(nnz)},
+        print(json.dumps({"metric": "CG K1 / SpMV % HBM peak", "config": {
+            "workload": f"diffusion3d_{n}^3", "rows": N, "nnz": int(nnz)},
             "roofline": roofline}), flush=True)
         return
 
     # ---- FP64 CG + Jacobi -------------------------------------------------
-    s64 = fsx.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True)
     cg_times = []
     for i in range(max(1, min(a.steps, 2)) + 1):
-        ctx.sync()
-        t = time.perf_counter()
-        info64 = fsx.cg_solve(A64, bd, x, s64, dinv=dinv)
-        ctx.sync()
-        cg_times.append(time.perf_counter() - t)
-        log(f"fp64 cg run {i}: {cg_times[-1]:.3f} s, {info64.iters} iters, {info64.reason}")
+        info64, dt = solve_fp64(fsx, ctx, A64, dinv, bd, x, a.maxiter)
+        cg_times.append(dt)
+        log(f"fp64 cg run {i}: {dt:.3f} s, {info64.iters} iters, {info64.reason}")
         if info64.reason != "converged_rtol":
             sys.exit(f"fp64 CG did not converge within {a.maxiter} iterations")
     cg_s = min(cg_times[1:])
     x64 = x.get()
-    r = ctx.zeros(N)
-    A64.residual(bd, x, r)
-    rel64 = fsx.nrm2(r) / fsx.nrm2(bd)
+    rel64 = true_rel_res(fsx, ctx, A64, bd, x)
 
     # ---- FP32-inner / FP64-outer refinement (timed steps) -----------------
-    inner = fsx.Settings(maxiter=a.maxiter, rtol=a.inner_rtol)
-    L = fsx.lib()
-    if a.ir_jacobi == 2:
-        A32.set_values_sym_scaled(A64)  # setup, like the FP32 conversion
     for i in range(a.warmup):
-        fsx.check(L.fsx_zero(ctx.h, x.h))
-        ctx.sync()
-        t = time.perf_counter()
-        ii = fsx.ir_cg_solve(A64, A32, bd, x, 1e-10, inner, jacobi=a.ir_jacobi,
-                             max_outer=a.max_outer)
-        log(f"ir warmup {i}: {time.perf_counter()-t:.3f} s outer {ii.outer_iters} "
-            f"inner {ii.inner_iters_total}")
+        ii, dt = solve_ir(fsx, ctx, A64, A32, bd, x, a)
+        log(f"ir warmup {i}: {dt:.3f} s outer {ii.outer_iters} inner {ii.inner_iters_total}")
     ctx.sync()
     times = []
     for i in range(a.steps):
-        fsx.check(L.fsx_zero(ctx.h, x.h))   # x0 = 0 for every step
-        ctx.sync()
-        t = time.perf_counter()
-        ii = fsx.ir_cg_solve(A64, A32, bd, x, 1e-10, inner, jacobi=a.ir_jacobi,
-                             max_outer=a.max_outer)
-        ctx.sync()
-        times.append(time.perf_counter() - t)
-        log(f"ir step {i}: {times[-1]:.3f} s outer {ii.outer_iters} "
+        ii, dt = solve_ir(fsx, ctx, A64, A32, bd, x, a)
+        times.append(dt)
+        log(f"ir step {i}: {dt:.3f} s outer {ii.outer_iters} "
             f"inner {ii.inner_iters_total} rel res {ii.rel_res_final:.3e}")
         if ii.status != 1:
             sys.exit("refinement did not reach 1e-10")
-    total = sum(times)
-    ir_s = total / len(times)
+    ir_s = sum(times) / len(times)
     xir = x.get()
-    err_vs_fp64 = float(np.linalg.norm(xir - x64) / np.linalg.norm(x64))
-    A64.residual(bd, x, r)
-    rel_ir = fsx.nrm2(r) / fsx.nrm2(bd)
+    rel_ir = true_rel_res(fsx, ctx, A64, bd, x)
+    roofline["share_of_step"] = round(k1_ms * 1e-3 * ii.inner_iters_total / ir_s, 3)
 
     out = {
         "metric": METRIC,
@@ -332,18 +346,127 @@ def main():
                    "outer_iters": ii.outer_iters,
                    "inner_iters_total": ii.inner_iters_total,
                    "true_rel_residual": rel_ir,
-                   "rel_err_vs_fp64_cg": err_vs_fp64},
+                   "rel_err_vs_fp64_cg": rel_err(xir, x64)},
             "fp64_cg": {"seconds": round(cg_s, 4), "iters": info64.iters,
                         "true_rel_residual": rel64},
             "ir_speedup_vs_fp64_cg": round(cg_s / ir_s, 3),
         },
         "roofline": roofline,
     }
+
+    # ---- a tight FP64 solve: the reference point for forward errors --------
+    tight = None
     if not a.no_cpu_baseline:
-        log(f"cpu baseline (oracle: 1 thread, then {cpu_threads()} colours/threads) ...")
-        out["cpu_baseline"] = cpu_baseline(rp, ci, v, b, info64.iters,
-                                           a.cpu_sample_iters)
+        it_, dt = solve_fp64(fsx, ctx, A64, dinv, bd, x, 4 * a.maxiter, rtol=1e-13)
+        tight = x.get()
+        log(f"tight fp64 cg (rtol 1e-13): {it_.iters} iters, {dt:.2f} s, "
+            f"true rel res {true_rel_res(fsx, ctx, A64, bd, x):.2e}")
+    del A32
+
+    # ---- the same comparison at sigma = 2.5 (one solve each) ---------------
+    if not a.no_sigma25 and abs(a.sigma - 2.5) > 1e-12:
+        out["config"]["sigma2_5"] = bench_sigma25(a, fsx, P, ctx, n)
+
+    # ---- the oracle to convergence on the same problem ---------------------
+    if not a.no_cpu_baseline:
+        del A64, dinv, bd, x
+        out["cpu_baseline"], out["config"]["oracle_parity"] = oracle_run(
+            a, rp, ci, v, b, info64.iters, x64, xir, tight)
     print(json.dumps(out), flush=True)
+
+
+def bench_sigma25(a, fsx, P, ctx, n):
+    """FP64 CG + Jacobi and the refinement once each (after a warm run) on
+    the sigma = 2.5 field: the mixed-precision configuration's contrast."""
+    t0 = time.perf_counter()
+    rp, ci, v, b = diffusion_problem(P, n, 2.5)
+    A64 = ctx.csr(rp, ci, v)
+    A32 = A64.convert(fsx.F32)
+    if a.ir_jacobi == 2:
+        A32.set_values_sym_scaled(A64)
+    dinv = A64.diag_inv()
+    bd = ctx.vec(b)
+    x = ctx.zeros(n ** 3)
+    del rp, ci, v
+    log(f"sigma 2.5 problem set up in {time.perf_counter() - t0:.1f} s")
+    cg_s, ir_s = [], []
+    for i in range(2):
+        info, dt = solve_fp64(fsx, ctx, A64, dinv, bd, x, 4 * a.maxiter)
+        cg_s.append(dt)
+        log(f"sigma 2.5 fp64 cg run {i}: {dt:.3f} s, {info.iters} iters, {info.reason}")
+    x64 = x.get()
+    rel64 = true_rel_res(fsx, ctx, A64, bd, x)
+    for i in range(2):
+        ii, dt = solve_ir(fsx, ctx, A64, A32, bd, x, a)
+        ir_s.append(dt)
+        log(f"sigma 2.5 ir run {i}: {dt:.3f} s outer {ii.outer_iters} "
+            f"inner {ii.inner_iters_total}")
+    relir = true_rel_res(fsx, ctx, A64, bd, x)
+    xir = x.get()
+    info_t, _ = solve_fp64(fsx, ctx, A64, dinv, bd, x, 8 * a.maxiter, rtol=1e-13)
+    tight = x.get()
+    return {
+        "fp64_cg": {"seconds": round(cg_s[-1], 4), "iters": info.iters,
+                    "status": info.reason, "true_rel_residual": rel64},
+        "ir": {"seconds": round(ir_s[-1], 4), "outer_iters": ii.outer_iters,
+               "inner_iters_total": ii.inner_iters_total,
+               "status": "converged_rtol" if ii.status == 1 else "not converged",
+               "true_rel_residual": relir,
+               "rel_err_vs_fp64_cg": rel_err(xir, x64)},
+        "ir_speedup_vs_fp64_cg": round(cg_s[-1] / ir_s[-1], 3),
+        "forward_error_vs_tight_solve": {"fp64_cg": rel_err(x64, tight),
+                                         "ir": rel_err(xir, tight),
+                                         "tight_iters": info_t.iters},
+    }
+
+
+def oracle_run(a, rp, ci, v, b, iters_gpu, x64, xir, tight):
+    """The oracle (the reference's CG + Dinv path restated on the CPU) run to
+    convergence on the same problem, with P colours on P threads (the
+    reference's MPI backend with one colour per rank: row blocks, block
+    partials summed in block order).  Its measured time is the CPU baseline;
+    its solution checks the GPU solutions at full size."""
+    from oracle import oracle as O
+    cpu = host_cpu()
+    nthr = a.cpu_threads if a.cpu_threads > 0 else max(1, min(16, cpu["usable"]))
+    Ao = O.Csr(rp, ci, v)
+    dv = Ao.dinv().vals
+    s = O.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True)
+    log(f"oracle cg+jacobi to 1e-10: {nthr} colours on {nthr} threads "
+        f"({cpu['model']}, {cpu['nproc']} CPUs, {cpu['usable']} usable) ...")
+    t0 = time.perf_counter()
+    xo, io = O.cg_colours(Ao, dv, b, np.zeros_like(b), s, nthr)
+    dt = time.perf_counter() - t0
+    log(f"oracle: {io.iters} iters ({io.reason}) in {dt:.1f} s")
+    # one thread (the reference's seq path): a short sample, per iteration
+    s1 = O.Settings(maxiter=a.cpu_sample_iters, rtol=1e-10, use_zero_guess=True)
+    t0 = time.perf_counter()
+    O.cg_colours(Ao, dv, b, np.zeros_like(b), s1, 1)
+    dt1 = (time.perf_counter() - t0) / a.cpu_sample_iters
+    e64, eir = rel_err(x64, xo), rel_err(xir, xo)
+    fe = {"oracle": rel_err(xo, tight), "fp64_cg": rel_err(x64, tight),
+          "ir": rel_err(xir, tight)}
+    parity = {
+        "oracle_iters": io.iters, "oracle_status": io.reason,
+        "fp64_cg_iters": iters_gpu, "iters_diff": abs(iters_gpu - io.iters),
+        "iters_bar": 2,
+        "rel_err_fp64_cg_vs_oracle": e64, "rel_err_ir_vs_oracle": eir, "x_bar": 1e-10,
+        # forward errors against an FP64 solve to rtol 1e-13: the difference
+        # of two solutions is at most the sum of theirs
+        "forward_error_vs_tight_solve": fe,
+        "fp64_cg_within_forward_errors": e64 <= fe["oracle"] + fe["fp64_cg"],
+        "ir_within_forward_errors": eir <= fe["oracle"] + fe["ir"],
+    }
+    base = {
+        "value": round(dt, 2), "unit": "s", "cores": nthr, "kind": "port",
+        "sample": (f"the whole solve: oracle CG + Jacobi FP64 on the same {len(b)}-row "
+                   f"problem from x0 = 0 to rtol 1e-10, {io.iters} iterations, {nthr} "
+                   f"colours on {nthr} threads"),
+        "cpu_model": cpu["model"], "nproc": cpu["nproc"], "usable_cpus": cpu["usable"],
+        "single_thread_ms_per_iter": round(dt1 * 1e3, 1),
+        "single_thread_sample": f"{a.cpu_sample_iters} iterations on 1 thread",
+    }
+    return base, parity
 
 
 def batch_cg_bytes(rows_s, nnz_s, vbytes, xbytes):

@@ -33,8 +33,11 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <string>
 #include <tuple>
 #include <vector>
 
@@ -143,7 +146,9 @@ __global__ __launch_bounds__(kUpdThreads) void cg_update_kernel(
 
 // |r| = sqrt(r.r); diagnostic-free tolerance test; rho and beta
 // (cg.hh:111-129)
-template<bool JAC>
+// XCUR: x was already updated in this iteration (the diagnostic loop), so
+// no x update is left pending for after the loop.
+template<bool JAC, bool XCUR = false>
 struct fin_beta {
 	cg_state * st;
 	double * hist;
@@ -158,12 +163,12 @@ struct fin_beta {
 			st->iters = it + 1;
 			st->status = FSX_CONVERGED_RTOL;
 			st->done = 1;
-			st->xpend = 1;
+			st->xpend = XCUR ? 0 : 1;
 			return;
 		}
 		if (it + 1 >= st->maxiter) { // loop exhausted
 			st->done = 1;
-			st->xpend = 1;
+			st->xpend = XCUR ? 0 : 1;
 			return;
 		}
 		// P.apply(r, z) happened in K2; rho and beta (cg.hh:124-129)
@@ -193,6 +198,35 @@ __global__ __launch_bounds__(256) void cg_xp_update_kernel(
 			x[i] = (T)(alpha * pi + (double)x[i]);
 			p[i] = (T)(beta * pi + (double)z[i]);
 		}
+	}
+}
+
+// Diagnostic loop: x += alpha p (cg.hh:108) ahead of F2, since the user
+// diagnostic reads x after the update; p = beta p + z (cg.hh:130) after the
+// host's check.  The expressions are K3's.
+template<class T>
+__global__ __launch_bounds__(256) void cg_x_update_kernel(size_t n, T * __restrict__ x,
+                                                          const T * __restrict__ p,
+                                                          const cg_state * st) {
+	if (st->done) return;
+	const double alpha = st->alpha;
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) {
+		if constexpr (std::is_same_v<T, double>) x[i] = alpha * p[i] + x[i];
+		else x[i] = (T)(alpha * (double)p[i] + (double)x[i]);
+	}
+}
+
+template<class T>
+__global__ __launch_bounds__(256) void cg_p_update_kernel(size_t n, T * __restrict__ p,
+                                                          const T * __restrict__ z,
+                                                          const cg_state * st) {
+	if (st->done) return;
+	const double beta = st->beta;
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) {
+		if constexpr (std::is_same_v<T, double>) p[i] = beta * p[i] + z[i];
+		else p[i] = (T)(beta * (double)p[i] + (double)z[i]);
 	}
 }
 
@@ -267,6 +301,51 @@ __global__ void ir_scale_kernel(size_t n, const double * __restrict__ r,
 	if (i < n) r32[i] = (float)(r[i] * inv);
 }
 
+// What a cached graph's launches touch, for the check before each replay:
+// every device range it captured, with its size at capture, and the residual
+// history it writes (one entry per iteration up to the solve's maxiter).
+namespace graph_ledger {
+struct entry {
+	std::vector<ledger::span> spans;
+	const double * hist = nullptr;
+};
+
+// Throws (FSX_ERR_STATE) when a captured range is no longer inside a live
+// buffer of at least its captured size.
+static void check(const entry & e, int maxiter) {
+	auto fail = [](const void * p, size_t b) {
+		char msg[160];
+		snprintf(msg, sizeof msg,
+		         "captured graph refers to %zu bytes at %p that are not inside a live "
+		         "buffer; the graph is not launched",
+		         b, p);
+		throw error(FSX_ERR_STATE, msg);
+	};
+	for (const auto & s : e.spans)
+		if (s.p && s.bytes && !ledger::live(s.p, s.bytes)) fail(s.p, s.bytes);
+	if (e.hist && maxiter > 0 && !ledger::live(e.hist, (size_t)maxiter * sizeof(double)))
+		fail(e.hist, (size_t)maxiter * sizeof(double));
+}
+
+// every array of A's layouts, at its present size
+static void add_layout(entry & e, const fsx_csr * A) {
+	auto add = [&](const std::unique_ptr<dbuf> & b) {
+		if (b && b->p) e.spans.push_back({b->p, b->bytes});
+	};
+	const csr_index & X = *A->idx;
+	add(X.rowptr);
+	add(X.colind);
+	add(A->vals);
+	add(X.sl_ptr);
+	add(X.sl_col);
+	add(A->sl_vals);
+	add(X.dg_ptr);
+	add(X.dg_off);
+	add(X.dg_mask);
+	add(A->dg_vals);
+}
+} // namespace graph_ledger
+
 } // namespace fsx
 
 // Per-context solver cache: pooled work vectors (so their addresses are stable
@@ -276,6 +355,7 @@ struct cg_graph_cache {
 	struct entry {
 		hipGraph_t g = nullptr;
 		hipGraphExec_t ge = nullptr;
+		fsx::graph_ledger::entry led;
 	};
 	// key: the iteration function, every device address a captured launch uses and the
 	// sizes and plan its launch shapes follow from
@@ -329,20 +409,75 @@ struct cg_bufs {
 	void * x;
 	void * r;
 	void * z; // == r for identity
-	void * p;  // search direction, buffer 0
-	void * p2; // search direction, buffer 1
+	void * p;
 	void * w;
 	const void * dinv;
+	double * part; // per-block partials of the reducing kernels
 	size_t n;
 };
 
-// One CG iteration (K1, F1, K2, F2, K3).
-template<class T, class V, class I>
+// Device scalar slots of a CG solve's setup reductions (|b|^2, |x|^2, |r|^2,
+// rho, final |x|^2) and where the host reads them (pinned, past the state
+// copies of the iteration loop).
+constexpr int kCgSetupSlot = 16;
+constexpr int kCgHostSlot = 40;
+static_assert(kCgSetupSlot + 5 <= kScalarSlots && kCgHostSlot + 5 <= kScalarSlots, "");
+static_assert(sizeof(cg_state) * 3 <= sizeof(double) * kCgHostSlot,
+              "setup values must not overlap the loop's pinned state copies");
+
+// Where a CG solve keeps its work.  The allocating entry points point these
+// at context-owned buffers (pooled vectors, the context's partials, the
+// cache's state buffer); a caller that may not allocate points them at
+// memory of its own.
+struct cg_work {
+	void *r, *z, *p, *w; // z: null without a preconditioner
+	double * part;       // partials: at least cg_partials_need(A, n) doubles
+	double * slots;      // 5 device scalars (setup reductions, final |x|^2)
+	cg_state * st;       // device iteration state
+	double * hist;       // device residual history (maxiter entries) or null
+	// false: kernels launched directly with one chunk of iterations in
+	// flight -- no graphs, events or allocations
+	bool graphs;
+};
+
+// Partials a CG solve on A over n rows needs: the SpMV+dot (1 per block),
+// the update (2 per block) and the setup reductions.
+static size_t cg_partials_need(const fsx_csr * A, size_t n) {
+	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+	return std::max<size_t>(std::max<size_t>(spmv_partials_size(A, 1),
+	                                         2 * ((n + per - 1) / per)),
+	                        (size_t)red_grid(n));
+}
+
+// The ranges a captured CG chunk touches (vectors, partials, state, every
+// layout array of A) and its history buffer.
+template<class T>
+static graph_ledger::entry cg_ledger_entry(const cg_bufs & B, const cg_state * st,
+                                           const double * hist) {
+	graph_ledger::entry e;
+	const size_t vb = B.n * sizeof(T);
+	e.spans = {{B.x, vb},
+	           {B.r, vb},
+	           {B.z, vb},
+	           {B.p, vb},
+	           {B.w, vb},
+	           {B.dinv, B.dinv ? vb : 0},
+	           {B.part, cg_partials_need(B.A, B.n) * sizeof(double)},
+	           {st, sizeof(cg_state)}};
+	graph_ledger::add_layout(e, B.A);
+	e.hist = hist;
+	return e;
+}
+
+// One CG iteration (K1, F1, K2, F2, K3).  DIAG: the diagnostic loop's
+// iteration -- K1, F1, K2, the x update, F2; the p update is launched by the
+// loop after the host has called the diagnostic.
+template<class T, class V, class I, bool DIAG = false>
 static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
                               double * hist, int) {
 	hipStream_t s = ctx->stream;
 	const fsx_csr * A = B.A;
-	double * part = ctx->ws.partials;
+	double * part = B.part;
 	T * p = (T *)B.p;
 	// K1 + F1
 	using ACC = cg_acc_t<T, V>;
@@ -352,26 +487,34 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	// K2 + F2
 	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
 	const unsigned g2 = (unsigned)((B.n + per - 1) / per);
+	const unsigned g3 = (unsigned)((B.n + 255) / 256);
+	auto xupd = [&] {
+		if constexpr (DIAG)
+			hipLaunchKernelGGL((cg_x_update_kernel<T>), dim3(g3), dim3(256), 0, s, B.n,
+			                   (T *)B.x, (const T *)p, (const cg_state *)st);
+	};
 	if (B.dinv) {
 		hipLaunchKernelGGL((cg_update_kernel<T, true>), dim3(g2), dim3(kUpdThreads),
 		                   0, s, B.n, (T *)B.r, (const T *)B.w, (const T *)B.dinv,
 		                   (T *)B.z, part, (const cg_state *)st);
-		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<true>>), dim3(1),
+		xupd();
+		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<true, DIAG>>), dim3(1),
 		                   dim3(kFinThreads), 0, s, (const double *)part, g2,
-		                   fin_beta<true>{st, hist});
+		                   fin_beta<true, DIAG>{st, hist});
 	}
 	else {
 		hipLaunchKernelGGL((cg_update_kernel<T, false>), dim3(g2), dim3(kUpdThreads),
 		                   0, s, B.n, (T *)B.r, (const T *)B.w, (const T *)nullptr,
 		                   (T *)nullptr, part, (const cg_state *)st);
-		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<false>>), dim3(1),
+		xupd();
+		hipLaunchKernelGGL((finalize_kernel<2, fin_beta<false, DIAG>>), dim3(1),
 		                   dim3(kFinThreads), 0, s, (const double *)part, g2,
-		                   fin_beta<false>{st, hist});
+		                   fin_beta<false, DIAG>{st, hist});
 	}
 	// K3
-	const unsigned g3 = (unsigned)((B.n + 255) / 256);
-	hipLaunchKernelGGL((cg_xp_update_kernel<T>), dim3(g3), dim3(256), 0, s, B.n,
-	                   (T *)B.x, p, (const T *)B.z, (const cg_state *)st);
+	if constexpr (!DIAG)
+		hipLaunchKernelGGL((cg_xp_update_kernel<T>), dim3(g3), dim3(256), 0, s, B.n,
+		                   (T *)B.x, p, (const T *)B.z, (const cg_state *)st);
 	FSX_HIP(hipGetLastError());
 }
 
@@ -432,35 +575,56 @@ static fsx_vec * pooled(fsx_ctx * ctx, size_t n, fsx_dtype dt, int slot) {
 	return v;
 }
 
-// Runs the iteration loop on the device.  `st_host` holds the initial state
-// and receives the final one.
-// Device scalar slots of a CG solve's setup reductions (|b|^2, |x|^2, |r|^2,
-// rho, final |x|^2) and where the host reads them (pinned, past the state
-// copies of the iteration loop).
-constexpr int kCgSetupSlot = 16;
-constexpr int kCgHostSlot = 40;
-static_assert(kCgSetupSlot + 5 <= kScalarSlots && kCgHostSlot + 5 <= kScalarSlots, "");
-static_assert(sizeof(cg_state) * 3 <= sizeof(double) * kCgHostSlot,
-              "setup values must not overlap the loop's pinned state copies");
+// Work of an allocating CG solve: pooled context vectors in slots slot0 ..
+// slot0 + 3 (r, z, p, w), the context's partials grown to what the solve
+// needs, the cache's state buffer and a history buffer for maxiter entries.
+static cg_work pooled_work(fsx_ctx * ctx, const fsx_csr * A, size_t n, fsx_dtype dt,
+                           bool jac, int slot0, int maxiter) {
+	cg_work wk{};
+	wk.r = pooled(ctx, n, dt, slot0 + 0)->ptr();
+	wk.z = jac ? pooled(ctx, n, dt, slot0 + 1)->ptr() : nullptr;
+	wk.p = pooled(ctx, n, dt, slot0 + 2)->ptr();
+	wk.w = pooled(ctx, n, dt, slot0 + 3)->ptr();
+	wk.part = ensure_partials(ctx, cg_partials_need(A, n));
+	wk.slots = ctx->ws.scalars + kCgSetupSlot;
+	wk.st = (cg_state *)cache_of(ctx).statebuf->p;
+	wk.hist = maxiter > 0 ? history_buffer(ctx, maxiter) : nullptr;
+	wk.graphs = true;
+	return wk;
+}
 
+// Iteration loop without graphs, events or allocations:
+// chunks of iterations launched directly, the host reading the state after
+// each.  Same kernels in the same order as the captured loop, so the same
+// iterates.
 template<class T, class V, class I>
-static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
-                        double * hist, const fsx_vec * xv, double * x2_out) {
-	auto & cache = cache_of(ctx);
-	cg_state * st = (cg_state *)cache.statebuf->p;
-	cg_state * pinned = (cg_state *)ctx->host_scalars; // pinned host space
-	static_assert(sizeof(cg_state) * 3 <= sizeof(double) * kScalarSlots, "");
-	std::memcpy(pinned, &st_host, sizeof(cg_state));
-	FSX_HIP(hipMemcpyAsync(st, pinned, sizeof(cg_state), hipMemcpyHostToDevice,
-	                       ctx->stream));
+static void run_cg_loop_direct(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
+                               cg_state * pinned, double * hist) {
 	const int K = chunk_size();
-	{
-		// partials for the SpMV+dot (1 per block) and the update (2 per block)
-		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
-		const size_t need = std::max<size_t>(spmv_partials_size(B.A, 1),
-		                                     2 * ((B.n + per - 1) / per));
-		ensure_partials(ctx, need);
+	const long maxit = pinned->maxiter;
+	cg_state * flag = pinned + 1;
+	long launched_it = 0;
+	while (launched_it < maxit) {
+		const int len = (int)std::min<long>(K, maxit - launched_it);
+		for (int k = 0; k < len; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
+		ctx->launches += 5 * len;
+		launched_it += len;
+		FSX_HIP(hipMemcpyAsync(flag, st, sizeof(cg_state), hipMemcpyDeviceToHost,
+		                       ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->host_syncs++;
+		if (flag->done) break;
 	}
+}
+
+// Captured, pipelined loop: chunks of iterations replayed from cached HIP
+// graphs (or launched directly with FSX_CG_GRAPH=0), up to two in flight,
+// the host checking the older one's state.
+template<class T, class V, class I>
+static void run_cg_loop_pipelined(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
+                                  cg_state * pinned, double * hist,
+                                  const cg_state & st0) {
+	const int K = chunk_size();
 	// Two chunk lengths: short chunks while the solve's length is unknown or
 	// nearly reached, long ones in between.  Iterations past convergence are
 	// launches that return at once, so the schedule changes only how many of
@@ -468,11 +632,13 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 	const int Kbig = K;
 	const int Ksmall = std::min(Kbig, kShortChunk);
 	hipGraphExec_t gexec[2] = {nullptr, nullptr}; // [0] Ksmall, [1] Kbig
+	const graph_ledger::entry * gled[2] = {nullptr, nullptr};
 	if (use_graphs()) {
+		auto & cache = cache_of(ctx);
 		auto key_for = [&](int k) {
 			std::vector<const void *> key{(const void *)&enqueue_iteration<T, V, I>,
-			                              B.x, B.r, B.z, B.p, B.p2, B.w, B.dinv,
-			                              (const void *)ctx->ws.partials, (const void *)st,
+			                              B.x, B.r, B.z, B.p, B.w, B.dinv,
+			                              (const void *)B.part, (const void *)st,
 			                              (const void *)hist, (const void *)(intptr_t)B.n,
 			                              (const void *)(intptr_t)k};
 			append_layout(key, B.A);
@@ -506,22 +672,29 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 				}
 				FSX_HIP(hipStreamEndCapture(ctx->stream, &en.g));
 				FSX_HIP(hipGraphInstantiate(&en.ge, en.g, nullptr, nullptr, 0));
-				cache.mv[keys[j]] = en;
-				gexec[j] = en.ge;
+				en.led = cg_ledger_entry<T>(B, st, hist);
+				it = cache.mv.emplace(keys[j], std::move(en)).first;
 			}
-			else
-				gexec[j] = it->second.ge;
+			gexec[j] = it->second.ge;
+			gled[j] = &it->second.led;
 		}
-		if (nlen == 1) gexec[1] = gexec[0];
+		if (nlen == 1) {
+			gexec[1] = gexec[0];
+			gled[1] = gled[0];
+		}
 	}
-	// pipelined loop: up to two chunks in flight, the host checking the older
 	cg_state * flags = pinned + 1;
 	hipEvent_t ev[2];
 	FSX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
 	FSX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
 	auto launch_chunk = [&](int slot, int len) {
-		hipGraphExec_t ge = gexec[len == Ksmall ? 0 : 1];
-		if (ge) FSX_HIP(hipGraphLaunch(ge, ctx->stream));
+		const int j = len == Ksmall ? 0 : 1;
+		if (gexec[j]) {
+			// every address the graph captured must still be a live buffer
+			// at least as large as at capture
+			graph_ledger::check(*gled[j], st0.maxiter);
+			FSX_HIP(hipGraphLaunch(gexec[j], ctx->stream));
+		}
 		else
 			for (int k = 0; k < len; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
 		ctx->launches += 5 * len;
@@ -529,37 +702,104 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 		                       hipMemcpyDeviceToHost, ctx->stream));
 		FSX_HIP(hipEventRecord(ev[slot], ctx->stream));
 	};
-	const long maxit = st_host.maxiter;
-	const double res0 = st_host.res, tol = st_host.tol;
+	const long maxit = st0.maxiter;
+	const double res0 = st0.res, tol = st0.tol;
 	long launched_it = 0; // iterations launched
 	long want = -1;       // iterations predicted beyond those launched (-1: unknown)
 	int launched = 0, checked = 0;
-	for (;;) {
-		const bool in_flight = launched > checked;
-		if (launched - checked < 2 && launched_it < maxit && (!in_flight || want != 0)) {
-			const int len = (launched < 2 || (want >= 0 && want <= Ksmall + 2)) ? Ksmall : Kbig;
-			launch_chunk(launched & 1, len);
-			++launched;
-			launched_it += len;
-			if (want > 0) want = std::max<long>(0, want - len);
-		}
-		if (checked == launched) break; // nothing in flight and nothing to launch
-		FSX_HIP(hipEventSynchronize(ev[checked & 1]));
-		ctx->host_syncs++;
-		const cg_state & f = flags[checked & 1];
-		++checked;
-		if (f.done) break;
-		// remaining iterations at the mean contraction per iteration so far
-		want = -1;
-		if (f.iter > 0 && f.res > 0 && res0 > 0 && tol > 0) {
-			const double rate = std::pow(f.res / res0, 1.0 / f.iter);
-			if (rate > 0 && rate < 1) {
-				const double m = std::ceil(std::log(tol / f.res) / std::log(rate));
-				if (m < 1e9) want = std::max<long>(0, (long)m - (launched_it - f.iter));
+	try {
+		for (;;) {
+			const bool in_flight = launched > checked;
+			if (launched - checked < 2 && launched_it < maxit && (!in_flight || want != 0)) {
+				const int len =
+					(launched < 2 || (want >= 0 && want <= Ksmall + 2)) ? Ksmall : Kbig;
+				launch_chunk(launched & 1, len);
+				++launched;
+				launched_it += len;
+				if (want > 0) want = std::max<long>(0, want - len);
+			}
+			if (checked == launched) break; // nothing in flight and nothing to launch
+			FSX_HIP(hipEventSynchronize(ev[checked & 1]));
+			ctx->host_syncs++;
+			const cg_state & f = flags[checked & 1];
+			++checked;
+			if (f.done) break;
+			// remaining iterations at the mean contraction per iteration so far
+			want = -1;
+			if (f.iter > 0 && f.res > 0 && res0 > 0 && tol > 0) {
+				const double rate = std::pow(f.res / res0, 1.0 / f.iter);
+				if (rate > 0 && rate < 1) {
+					const double m = std::ceil(std::log(tol / f.res) / std::log(rate));
+					if (m < 1e9) want = std::max<long>(0, (long)m - (launched_it - f.iter));
+				}
 			}
 		}
 	}
-	if (launched > 0 && B.n > 0) {
+	catch (...) {
+		(void)hipStreamSynchronize(ctx->stream);
+		(void)hipEventDestroy(ev[0]);
+		(void)hipEventDestroy(ev[1]);
+		throw;
+	}
+	(void)hipEventDestroy(ev[0]);
+	(void)hipEventDestroy(ev[1]);
+}
+
+// Iteration loop with a user diagnostic (cg.hh:112-116): one iteration at a
+// time, the host reading |r| after each and calling diag(x, |r|) before the
+// tolerance test takes effect -- a true return stops the solve with
+// converged_user and iters = it + 1 (also when the tolerance was met in the
+// same iteration, as the reference checks the diagnostic first).
+template<class T, class V, class I>
+static void run_cg_loop_diag(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
+                             cg_state * pinned, double * hist, fsx_vec * xv,
+                             fsx_diag_fn fn, void * user) {
+	cg_state * flag = pinned + 1;
+	const int maxit = pinned->maxiter;
+	const unsigned g3 = (unsigned)((B.n + 255) / 256);
+	*flag = *pinned;
+	for (int it = 0; it < maxit; ++it) {
+		enqueue_iteration<T, V, I, true>(ctx, B, st, hist, it);
+		ctx->launches += 5;
+		FSX_HIP(hipMemcpyAsync(flag, st, sizeof(cg_state), hipMemcpyDeviceToHost,
+		                       ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->host_syncs++;
+		const int d = fn(user, xv, flag->res);
+		if (d < 0) throw error(FSX_ERR_STATE, "cg: the user diagnostic reported an error");
+		if (d > 0) {
+			flag->iters = it + 1;
+			flag->status = FSX_CONVERGED_USER;
+			flag->done = 1;
+			break;
+		}
+		if (flag->done) break;
+		if (B.n) {
+			hipLaunchKernelGGL((cg_p_update_kernel<T>), dim3(g3), dim3(256), 0, ctx->stream,
+			                   B.n, (T *)B.p, (const T *)B.z, (const cg_state *)st);
+			FSX_HIP(hipGetLastError());
+			ctx->launches++;
+		}
+	}
+	*pinned = *flag;
+}
+
+// Runs the iteration loop on the device.  `st_host` holds the initial state
+// and receives the final one; *x2_out receives |x|^2 of the final x.
+template<class T, class V, class I>
+static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, const cg_work & wk,
+                        cg_state & st_host, fsx_vec * xv, double * x2_out,
+                        fsx_diag_fn diag, void * user) {
+	cg_state * st = wk.st;
+	cg_state * pinned = (cg_state *)ctx->host_scalars; // pinned host space
+	static_assert(sizeof(cg_state) * 3 <= sizeof(double) * kScalarSlots, "");
+	std::memcpy(pinned, &st_host, sizeof(cg_state));
+	FSX_HIP(hipMemcpyAsync(st, pinned, sizeof(cg_state), hipMemcpyHostToDevice,
+	                       ctx->stream));
+	if (diag) run_cg_loop_diag<T, V, I>(ctx, B, st, pinned, wk.hist, xv, diag, user);
+	else if (wk.graphs) run_cg_loop_pipelined<T, V, I>(ctx, B, st, pinned, wk.hist, st_host);
+	else run_cg_loop_direct<T, V, I>(ctx, B, st, pinned, wk.hist);
+	if (B.n > 0 && !diag) {
 		// the iteration that set `done` skipped K3: apply its x update
 		const unsigned g3 = (unsigned)((B.n + 255) / 256);
 		hipLaunchKernelGGL((cg_xfinal_kernel<T>), dim3(g3), dim3(256), 0, ctx->stream,
@@ -567,61 +807,105 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, cg_state & st_host,
 		FSX_HIP(hipGetLastError());
 		ctx->launches++;
 	}
-	FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(cg_state), hipMemcpyDeviceToHost,
-	                       ctx->stream));
+	// the final state (the diagnostic loop has it on the host already, with
+	// its own stop recorded)
+	if (!diag)
+		FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(cg_state), hipMemcpyDeviceToHost,
+		                       ctx->stream));
 	// |x|^2 of the final x, read with the same synchronisation
-	double * x2slot = ctx->ws.scalars + kCgSetupSlot + 4;
+	double * x2slot = wk.slots + 4;
 	if (xv && B.n > 0) {
-		reduce_async(ctx, RED_L2SQ, xv, nullptr, x2slot);
+		reduce_raw(ctx, RED_L2SQ, xv->dt, B.n, xv->ptr(), nullptr, x2slot, wk.part);
 		FSX_HIP(hipMemcpyAsync(ctx->host_scalars + kCgHostSlot + 4, x2slot, sizeof(double),
 		                       hipMemcpyDeviceToHost, ctx->stream));
 	}
 	FSX_HIP(hipStreamSynchronize(ctx->stream));
-	(void)hipEventDestroy(ev[0]);
-	(void)hipEventDestroy(ev[1]);
 	std::memcpy(&st_host, pinned, sizeof(cg_state));
 	if (x2_out) *x2_out = (xv && B.n > 0) ? ctx->host_scalars[kCgHostSlot + 4] : 0.0;
 }
 
+// r = b - A x (op::core::residual, operators/core.hh:129-142: apply, then
+// r = b - r), the same expression as fsx_csr_residual's epilogue
+template<class ACC, class T>
+struct epi_cg_residual {
+	static constexpr int NV = 1;
+	static constexpr bool reduces = false;
+	T * r;
+	const T * b;
+	__device__ bool skip() const { return false; }
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&)[NV], XI &&) const {
+		r[i] = b[i] - (T)s;
+	}
+	__device__ void block_end(const double (&)[NV], unsigned, unsigned) const {}
+};
+
+// z = D^{-1} r as the Dinv CSR row computes it (y_i = 0; y_i += dinv_i r_i),
+// the expression of fsx_jacobi_apply
+template<class T>
+__global__ __launch_bounds__(256) void cg_jacobi_kernel(size_t n, const T * __restrict__ dinv,
+                                                        const T * __restrict__ r,
+                                                        T * __restrict__ z) {
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) z[i] = T(0) + dinv[i] * r[i];
+}
+
 // Shared driver for FP64 (T = double) and FP32 inner (T = float) solves.
-// Follows cg.hh:43-139 step by step; see the file comment.  `hist_count`
-// receives the number of residual-history entries written on the device.
+// Follows cg.hh:43-139 step by step; see the file comment.  The work vectors,
+// partials, scalar slots and state come from `wk` (allocates nothing itself).
+// `hist_count` receives the number of residual-history entries written on
+// the device.
 template<class T>
 static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
                           const fsx_vec * b, fsx_vec * x, const fsx_settings & s,
-                          fsx_info * info, fsx_vec * r, fsx_vec * z,
-                          fsx_vec * p, fsx_vec * p2, fsx_vec * w,
-                          int * hist_count) {
+                          fsx_info * info, const cg_work & wk, int * hist_count,
+                          fsx_diag_fn diag = nullptr, void * user = nullptr) {
 	std::memset(info, 0, sizeof(*info));
 	info->status = FSX_UNKNOWN;
 	if (hist_count) *hist_count = 0;
 	const size_t n = b->n;
+	const fsx_dtype dt = b->dt;
+	T * r = (T *)wk.r;
 	// The setup reductions |b|^2, |x|^2, |r|^2 and rho = z.r are enqueued
 	// into device slots and read with one synchronisation (the same kernels
 	// as one-by-one reads, so the same values); z = D^{-1} r is formed even
 	// when the initial residual already meets the tolerance.
-	double * sl = ctx->ws.scalars + kCgSetupSlot;
+	double * sl = wk.slots;
 	double * hs = ctx->host_scalars + kCgHostSlot;
 	if (s.use_zero_guess) {
 		launch_set(ctx, 0., x);
-		launch_copy(ctx, b, r);
+		if (n)
+			FSX_HIP(hipMemcpyAsync(r, b->ptr(), n * sizeof(T), hipMemcpyDeviceToDevice,
+			                       ctx->stream));
 	}
-	else {
-		if (n) reduce_async(ctx, RED_L2SQ, x, nullptr, sl + 1);
-		int rc = fsx_csr_residual(ctx, A, b, x, r);
-		if (rc) throw error(rc, get_error());
+	else if (n) {
+		reduce_raw(ctx, RED_L2SQ, dt, n, x->ptr(), nullptr, sl + 1, wk.part);
+		by_matrix_type(A, [&](auto * vtag, auto * itag) {
+			using V = std::remove_pointer_t<decltype(vtag)>;
+			using I = std::remove_pointer_t<decltype(itag)>;
+			using ACC = cg_acc_t<T, V>;
+			epi_cg_residual<ACC, T> e{r, (const T *)b->ptr()};
+			spmv_launch_typed<decltype(e), ACC, V, I, T>(ctx->stream, A, (const T *)x->ptr(),
+			                                             e);
+		});
+		ctx->launches++;
 	}
-	fsx_vec * zz = r; // op::I: z is a copy of r, so read r directly
+	T * zz = r; // op::I: z is a copy of r, so read r directly
 	if (dinv) {
-		int rc = fsx_jacobi_apply(ctx, dinv, r, z);
-		if (rc) throw error(rc, get_error());
-		zz = z;
+		zz = (T *)wk.z;
+		if (n) {
+			hipLaunchKernelGGL((cg_jacobi_kernel<T>), dim3((unsigned)((n + 255) / 256)),
+			                   dim3(256), 0, ctx->stream, n, (const T *)dinv->ptr(),
+			                   (const T *)r, zz);
+			FSX_HIP(hipGetLastError());
+			ctx->launches++;
+		}
 	}
 	double b2 = 0, x2 = 0, r2 = 0, rho = 0;
 	if (n) {
-		reduce_async(ctx, RED_L2SQ, b, nullptr, sl + 0);
-		reduce_async(ctx, RED_L2SQ, r, nullptr, sl + 2);
-		reduce_async(ctx, RED_DOT, zz, r, sl + 3);
+		reduce_raw(ctx, RED_L2SQ, dt, n, b->ptr(), nullptr, sl + 0, wk.part);
+		reduce_raw(ctx, RED_L2SQ, dt, n, r, nullptr, sl + 2, wk.part);
+		reduce_raw(ctx, RED_DOT, dt, n, zz, r, sl + 3, wk.part);
 		FSX_HIP(hipMemcpyAsync(hs, sl, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
 		ctx->host_syncs++;
@@ -648,24 +932,29 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 	st.tol = terminate_tol;
 	st.res = current_res;
 	st.maxiter = s.maxiter;
-	launch_copy(ctx, zz, p);
+	if (n)
+		FSX_HIP(hipMemcpyAsync(wk.p, zz, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
 	double xf2 = 0;
 	if (s.maxiter > 0) {
-		double * hist = history_buffer(ctx, s.maxiter);
-		cg_bufs B{A, x->ptr(), r->ptr(), zz->ptr(), p->ptr(), p2->ptr(), w->ptr(),
-		          dinv ? dinv->ptr() : nullptr, n};
+		cg_bufs B{A, x->ptr(), r, zz, wk.p, wk.w, dinv ? dinv->ptr() : nullptr, wk.part, n};
 		by_matrix_type(A, [&](auto * vtag, auto * itag) {
 			using V = std::remove_pointer_t<decltype(vtag)>;
 			using I = std::remove_pointer_t<decltype(itag)>;
-			run_cg_loop<T, V, I>(ctx, B, st, hist, x, &xf2);
+			run_cg_loop<T, V, I>(ctx, B, wk, st, x, &xf2, diag, user);
 		});
 		current_res = st.res;
 		info->iters = st.iters;
 		if (st.iters) info->status = st.status;
 		if (hist_count) *hist_count = st.iter;
 	}
-	else
-		xf2 = reduce_host(ctx, RED_L2SQ, x, nullptr);
+	else if (n) {
+		reduce_raw(ctx, RED_L2SQ, dt, n, x->ptr(), nullptr, sl + 4, wk.part);
+		FSX_HIP(hipMemcpyAsync(hs + 4, sl + 4, sizeof(double), hipMemcpyDeviceToHost,
+		                       ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->host_syncs++;
+		xf2 = hs[4];
+	}
 	info->res_norm_final = (float)current_res;
 	info->sol_norm_final = (float)std::sqrt(xf2);
 	if (info->iters == 0) info->status = FSX_DIVERGED_ITERS;
@@ -1328,20 +1617,88 @@ int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 		            "cg: FP64 vectors need an FP64 matrix");
 		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "cg: b and x must differ");
 		const size_t n = b->n;
-		fsx_vec * r = pooled(ctx, n, b->dt, 0);
-		fsx_vec * z = dinv ? pooled(ctx, n, b->dt, 1) : nullptr;
-		fsx_vec * p = pooled(ctx, n, b->dt, 2);
-		fsx_vec * p2 = pooled(ctx, n, b->dt, 4);
-		fsx_vec * w = pooled(ctx, n, b->dt, 3);
+		const cg_work wk = pooled_work(ctx, A, n, b->dt, dinv != nullptr, 0, s->maxiter);
 		int cnt = 0;
-		if (b->dt == FSX_F64)
-			cg_solve_impl<double>(ctx, A, dinv, b, x, *s, info, r, z, p, p2, w, &cnt);
-		else
-			cg_solve_impl<float>(ctx, A, dinv, b, x, *s, info, r, z, p, p2, w, &cnt);
+		if (b->dt == FSX_F64) cg_solve_impl<double>(ctx, A, dinv, b, x, *s, info, wk, &cnt);
+		else cg_solve_impl<float>(ctx, A, dinv, b, x, *s, info, wk, &cnt);
 		ctx->last_cg_hist.assign(cnt, 0.0);
 		if (cnt > 0)
 			FSX_HIP(hipMemcpy(ctx->last_cg_hist.data(), ctx->cg_cache->histbuf->p,
 			                  sizeof(double) * cnt, hipMemcpyDeviceToHost));
+	});
+}
+
+int fsx_cg_solve_diag(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                      const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                      fsx_diag_fn diag, void * user, fsx_info * info) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A && s && info && diag, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)b->n == A->nrows,
+		            "cg: shape mismatch");
+		if (dinv) check_same(dinv, b);
+		FSX_REQUIRE(b->dt == FSX_F32 || A->vt == FSX_F64,
+		            "cg: FP64 vectors need an FP64 matrix");
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "cg: b and x must differ");
+		const size_t n = b->n;
+		const cg_work wk = pooled_work(ctx, A, n, b->dt, dinv != nullptr, 0, s->maxiter);
+		int cnt = 0;
+		if (b->dt == FSX_F64)
+			cg_solve_impl<double>(ctx, A, dinv, b, x, *s, info, wk, &cnt, diag, user);
+		else cg_solve_impl<float>(ctx, A, dinv, b, x, *s, info, wk, &cnt, diag, user);
+		ctx->last_cg_hist.assign(cnt, 0.0);
+		if (cnt > 0)
+			FSX_HIP(hipMemcpy(ctx->last_cg_hist.data(), ctx->cg_cache->histbuf->p,
+			                  sizeof(double) * cnt, hipMemcpyDeviceToHost));
+	});
+}
+
+int fsx_time_cg_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_vec * w,
+                     int reps, double * avg_ms) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A && avg_ms, "null argument");
+		check_vec(p, "p");
+		check_vec(w, "w");
+		check_same(p, w);
+		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)p->n == A->nrows, "shape mismatch");
+		FSX_REQUIRE(p->dt == FSX_F32 || A->vt == FSX_F64,
+		            "FP64 vectors need an FP64 matrix");
+		FSX_REQUIRE(reps > 0, "reps must be > 0");
+		// a state whose `done` flag stays clear (alpha etc. are not read by K1)
+		dbuf stb(sizeof(cg_state));
+		FSX_HIP(hipMemsetAsync(stb.p, 0, sizeof(cg_state), ctx->stream));
+		double * part = ensure_partials(ctx, spmv_partials_size(A, 1));
+		const cg_state * st = (const cg_state *)stb.p;
+		auto launch = [&] {
+			by_matrix_type(A, [&](auto * vtag, auto * itag) {
+				using V = std::remove_pointer_t<decltype(vtag)>;
+				using I = std::remove_pointer_t<decltype(itag)>;
+				auto go = [&](auto * ttag) {
+					using T = std::remove_pointer_t<decltype(ttag)>;
+					using ACC = cg_acc_t<T, V>;
+					epi_cg_spmv<ACC, T> e{(T *)w->ptr(), part, st};
+					spmv_launch_typed<decltype(e), ACC, V, I, T>(ctx->stream, A,
+					                                             (const T *)p->ptr(), e);
+				};
+				if (p->dt == FSX_F64) go((double *)nullptr);
+				else go((float *)nullptr);
+			});
+		};
+		launch(); // warm
+		hipEvent_t e0, e1;
+		FSX_HIP(hipEventCreate(&e0));
+		FSX_HIP(hipEventCreate(&e1));
+		FSX_HIP(hipEventRecord(e0, ctx->stream));
+		for (int i = 0; i < reps; ++i) launch();
+		FSX_HIP(hipEventRecord(e1, ctx->stream));
+		FSX_HIP(hipEventSynchronize(e1));
+		float ms = 0;
+		FSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		*avg_ms = (double)ms / reps;
 	});
 }
 
@@ -1382,11 +1739,6 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 		fsx_vec * r32 = pooled(ctx, n, FSX_F32, 11);
 		fsx_vec * d32 = pooled(ctx, n, FSX_F32, 12);
 		fsx_vec * dinv32 = jacobi == 1 ? pooled(ctx, n, FSX_F32, 13) : nullptr;
-		fsx_vec * ir = pooled(ctx, n, FSX_F32, 14);
-		fsx_vec * iz = jacobi == 1 ? pooled(ctx, n, FSX_F32, 15) : nullptr;
-		fsx_vec * ip = pooled(ctx, n, FSX_F32, 16);
-		fsx_vec * ip2 = pooled(ctx, n, FSX_F32, 18);
-		fsx_vec * iw = pooled(ctx, n, FSX_F32, 17);
 		fsx_vec * sc = jacobi == 2 ? pooled(ctx, n, FSX_F64, 19) : nullptr;
 		if (jacobi == 1) {
 			int rc = fsx_csr_diag_inv(ctx, A64, dinv32);
@@ -1424,11 +1776,15 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 			ctx->launches += 2;
 			return std::sqrt(fetch_scalar(ctx, rn2));
 		};
-		ensure_partials(ctx, spmv_partials_size(A64, 1));
-		double rnorm = residual();
-		const unsigned g = (unsigned)((n + 255) / 256);
+		// partials for the outer residual and the inner solves, grown once
+		// before any buffer address is taken
+		ensure_partials(ctx, std::max(spmv_partials_size(A64, 1), cg_partials_need(A32, n)));
 		fsx_settings is = *inner;
 		is.use_zero_guess = 1;
+		// inner work: pooled slots 14-17
+		const cg_work iwk = pooled_work(ctx, A32, n, FSX_F32, jacobi == 1, 14, is.maxiter);
+		double rnorm = residual();
+		const unsigned g = (unsigned)((n + 255) / 256);
 		int outer = 0;
 		for (;;) {
 			if (rnorm / b_norm <= rtol_outer) {
@@ -1452,8 +1808,7 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 			FSX_HIP(hipGetLastError());
 			fsx_info ii;
 			int cnt = 0;
-			cg_solve_impl<float>(ctx, A32, dinv32, r32, d32, is, &ii, ir, iz, ip, ip2,
-			                     iw, &cnt);
+			cg_solve_impl<float>(ctx, A32, dinv32, r32, d32, is, &ii, iwk, &cnt);
 			info->inner_iters_total += cnt;
 			// x += |r| d  (FP64; d = s y for the scaled inner system)
 			if (sc)

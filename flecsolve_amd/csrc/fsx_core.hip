@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <random>
 
 namespace fsx {
@@ -24,18 +26,68 @@ static thread_local std::string g_last_error;
 void set_error(const std::string & msg) { g_last_error = msg; }
 const char * get_error() { return g_last_error.c_str(); }
 
+namespace ledger {
+// base address -> sizes of the live buffers starting there (wrapped caller
+// memory may be registered more than once)
+static std::mutex g_mu;
+static std::multimap<uintptr_t, size_t> g_live;
+
+void add(const void * p, size_t bytes) {
+	if (!p || !bytes) return;
+	std::lock_guard<std::mutex> lk(g_mu);
+	g_live.emplace((uintptr_t)p, bytes);
+}
+void remove(const void * p, size_t bytes) {
+	if (!p || !bytes) return;
+	std::lock_guard<std::mutex> lk(g_mu);
+	auto rg = g_live.equal_range((uintptr_t)p);
+	for (auto it = rg.first; it != rg.second; ++it)
+		if (it->second == bytes) {
+			g_live.erase(it);
+			return;
+		}
+}
+bool live(const void * p, size_t bytes) {
+	const uintptr_t a = (uintptr_t)p;
+	std::lock_guard<std::mutex> lk(g_mu);
+	// buffers starting at or below a, nearest first
+	for (auto it = g_live.upper_bound(a); it != g_live.begin();) {
+		--it;
+		if (a + bytes <= it->first + it->second) return true;
+	}
+	return false;
+}
+} // namespace ledger
+
+// hipMalloc calls made and bytes held by live owned allocations
+static std::atomic<uint64_t> g_alloc_count{0}, g_alloc_live{0};
+
 dbuf::dbuf(size_t b) : bytes(b) {
 	if (b == 0) return;
 	hipError_t e = hipMalloc(&p, b);
+	g_alloc_count.fetch_add(1);
 	if (e != hipSuccess) {
 		p = nullptr;
 		throw error(FSX_ERR_ALLOC, std::string("hipMalloc(") +
 		                               std::to_string(b) + " B): " +
 		                               hipGetErrorString(e));
 	}
+	g_alloc_live.fetch_add(b);
+	ledger::add(p, b);
+}
+void dbuf::adopt(void * q, size_t b) {
+	ledger::remove(p, bytes);
+	p = q;
+	bytes = b;
+	owns = false;
+	ledger::add(p, bytes);
 }
 dbuf::~dbuf() {
-	if (p && owns) (void)hipFree(p);
+	ledger::remove(p, bytes);
+	if (p && owns) {
+		(void)hipFree(p);
+		g_alloc_live.fetch_sub(bytes);
+	}
 }
 
 void check_vec(const fsx_vec * v, const char * name) {
@@ -253,36 +305,39 @@ __global__ __launch_bounds__(kRedThreads) void powsum_kernel(size_t n, const T *
 }
 
 template<red_kind K>
-static void reduce_launch_k(fsx_ctx * ctx, const fsx_vec * x,
-                            const fsx_vec * y, double * dslot) {
-	int g = red_grid(x->n);
-	double * part = ensure_partials(ctx, (size_t)g);
-	if (x->dt == FSX_F64)
+static void reduce_launch_k(fsx_ctx * ctx, fsx_dtype dt, size_t n, const void * x,
+                            const void * y, double * dslot, double * part) {
+	int g = red_grid(n);
+	if (dt == FSX_F64)
 		hipLaunchKernelGGL((reduce_kernel<K, double>), dim3(g), dim3(kRedThreads),
-		                   0, ctx->stream, x->n, (const double *)x->ptr(),
-		                   y ? (const double *)y->ptr() : nullptr, part);
+		                   0, ctx->stream, n, (const double *)x, (const double *)y, part);
 	else
 		hipLaunchKernelGGL((reduce_kernel<K, float>), dim3(g), dim3(kRedThreads),
-		                   0, ctx->stream, x->n, (const float *)x->ptr(),
-		                   y ? (const float *)y->ptr() : nullptr, part);
+		                   0, ctx->stream, n, (const float *)x, (const float *)y, part);
 	hipLaunchKernelGGL((reduce_finalize_kernel<K>), dim3(1), dim3(kRedThreads), 0,
 	                   ctx->stream, (const double *)part, (unsigned)g, dslot);
 	FSX_HIP(hipGetLastError());
 	ctx->launches += 2;
 }
 
+void reduce_raw(fsx_ctx * ctx, red_kind k, fsx_dtype dt, size_t n, const void * x,
+                const void * y, double * dslot, double * part) {
+	switch (k) {
+	case RED_DOT: reduce_launch_k<RED_DOT>(ctx, dt, n, x, y, dslot, part); break;
+	case RED_L1: reduce_launch_k<RED_L1>(ctx, dt, n, x, y, dslot, part); break;
+	case RED_L2SQ: reduce_launch_k<RED_L2SQ>(ctx, dt, n, x, y, dslot, part); break;
+	case RED_INF: reduce_launch_k<RED_INF>(ctx, dt, n, x, y, dslot, part); break;
+	case RED_MIN: reduce_launch_k<RED_MIN>(ctx, dt, n, x, y, dslot, part); break;
+	case RED_MAX: reduce_launch_k<RED_MAX>(ctx, dt, n, x, y, dslot, part); break;
+	}
+}
+
 void reduce_async(fsx_ctx * ctx, red_kind k, const fsx_vec * x,
                   const fsx_vec * y, double * dslot) {
 	check_vec(x, "x");
 	if (y) check_same(x, y);
-	switch (k) {
-	case RED_DOT: reduce_launch_k<RED_DOT>(ctx, x, y, dslot); break;
-	case RED_L1: reduce_launch_k<RED_L1>(ctx, x, y, dslot); break;
-	case RED_L2SQ: reduce_launch_k<RED_L2SQ>(ctx, x, y, dslot); break;
-	case RED_INF: reduce_launch_k<RED_INF>(ctx, x, y, dslot); break;
-	case RED_MIN: reduce_launch_k<RED_MIN>(ctx, x, y, dslot); break;
-	case RED_MAX: reduce_launch_k<RED_MAX>(ctx, x, y, dslot); break;
-	}
+	double * part = ensure_partials(ctx, (size_t)red_grid(x->n));
+	reduce_raw(ctx, k, x->dt, x->n, x->ptr(), y ? y->ptr() : nullptr, dslot, part);
 }
 
 void ctx_retain(fsx_ctx * c) { c->refs.fetch_add(1); }
@@ -407,6 +462,13 @@ int fsx_sync(fsx_ctx * ctx) {
 	});
 }
 
+int fsx_alloc_stats(uint64_t * allocations, uint64_t * live_bytes) {
+	return guard([&] {
+		if (allocations) *allocations = g_alloc_count.load();
+		if (live_bytes) *live_bytes = g_alloc_live.load();
+	});
+}
+
 int fsx_ctx_stream(fsx_ctx * ctx, void ** stream) {
 	return guard([&] {
 		check_ctx(ctx);
@@ -453,9 +515,7 @@ int fsx_vec_wrap(fsx_ctx * ctx, void * dptr, size_t n, fsx_dtype dt,
 		v->n = n;
 		v->dt = dt;
 		v->buf = std::make_shared<dbuf>();
-		v->buf->p = dptr;
-		v->buf->bytes = n * v->elem();
-		v->buf->owns = false;
+		v->buf->adopt(dptr, n * v->elem());
 		v->id = ctx->next_id++;
 		ctx_retain(ctx);
 		*out = v.release();

@@ -65,6 +65,23 @@ int guard(F && body) {
 
 // ---- device buffers ------------------------------------------------------
 
+// Live device-buffer ledger: every dbuf records its address range while it
+// lives (allocations and wrapped caller memory alike).  Cached HIP graphs
+// keep the ranges their launches touch and are checked against it before
+// every replay, so a graph whose captured addresses are no longer backed by
+// a live buffer of at least the captured size fails with an error instead of
+// launching.
+namespace ledger {
+struct span {
+	const void * p;
+	size_t bytes;
+};
+void add(const void * p, size_t bytes);
+void remove(const void * p, size_t bytes);
+// true when [p, p + bytes) lies inside one live buffer
+bool live(const void * p, size_t bytes);
+} // namespace ledger
+
 struct dbuf {
 	void * p = nullptr;
 	size_t bytes = 0;
@@ -72,6 +89,8 @@ struct dbuf {
 	dbuf() = default;
 	explicit dbuf(size_t b);
 	~dbuf();
+	// refer to caller-owned device memory (not freed here)
+	void adopt(void * q, size_t b);
 	dbuf(const dbuf &) = delete;
 	dbuf & operator=(const dbuf &) = delete;
 };
@@ -190,6 +209,11 @@ void reduce_async(fsx_ctx * ctx, red_kind k, const fsx_vec * x,
                   const fsx_vec * y, double * dslot);
 double reduce_host(fsx_ctx * ctx, red_kind k, const fsx_vec * x,
                    const fsx_vec * y);
+// The same reductions over n elements of dtype dt at x (and y for a dot),
+// with the caller's partials (at least red_grid(n) doubles).  Allocates
+// nothing.
+void reduce_raw(fsx_ctx * ctx, red_kind k, fsx_dtype dt, size_t n, const void * x,
+                const void * y, double * dslot, double * part);
 
 // s_r = 1 / sqrt(a_rr) of the FP64 matrix A on ctx's stream; *bad (device,
 // optional) is set to 1 when a diagonal entry is missing or not positive.

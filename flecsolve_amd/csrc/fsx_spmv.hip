@@ -1251,6 +1251,28 @@ int fsx_csr_spmv_t(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
 
 // ---- timing --------------------------------------------------------------
 
+int fsx_csr_layout_bytes(const fsx_csr * A, size_t x_bytes, uint64_t * bytes) {
+	return guard([&] {
+		FSX_REQUIRE(A && bytes, "null argument");
+		const uint64_t vb = A->vt == FSX_F64 ? 8 : 4, ib = A->it == FSX_I32 ? 4 : 8;
+		const uint64_t n = (uint64_t)A->nrows, nsl = (n + kSliceRows - 1) / kSliceRows;
+		const uint64_t vec = n * x_bytes * 2; // x read once, y written once
+		const csr_index & X = *A->idx;
+		if (A->plan_kind == 3) {
+			// value slots; per slice one 8-byte pointer; per offset entry (64
+			// slots) a 4-byte offset and an 8-byte lane mask
+			const uint64_t slots = (uint64_t)X.dg_nnz;
+			*bytes = slots * vb + (nsl + 1) * 8 + slots / kSliceRows * 12 + vec;
+		}
+		else if (A->plan_kind == 2) {
+			const uint64_t slots = (uint64_t)X.sl_nnz;
+			*bytes = slots * (vb + ib) + (nsl + 1) * ib + vec;
+		}
+		else
+			*bytes = (uint64_t)A->nnz * (vb + ib) + (n + 1) * ib + vec;
+	});
+}
+
 int fsx_time_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
                   fsx_vec * y, int reps, double * avg_ms) {
 	return guard([&] {

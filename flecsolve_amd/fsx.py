@@ -67,6 +67,8 @@ class BEStats(C.Structure):
 _lib = None
 
 _P = C.c_void_p
+# fsx_diag_fn: int (*)(void * user, fsx_vec * x, double res_norm)
+DIAG_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_double)
 _SIGS = {
     "fsx_last_error": (C.c_char_p, []),
     "fsx_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
@@ -74,6 +76,9 @@ _SIGS = {
     "fsx_sync": (C.c_int, [_P]),
     "fsx_ctx_stream": (C.c_int, [_P, C.POINTER(_P)]),
     "fsx_ctx_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "fsx_alloc_stats": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "fsx_time_cg_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
+    "fsx_csr_layout_bytes": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint64)]),
     "fsx_vec_create": (C.c_int, [_P, C.c_size_t, C.c_int, C.POINTER(_P)]),
     "fsx_vec_wrap": (C.c_int, [_P, _P, C.c_size_t, C.c_int, C.POINTER(_P)]),
     "fsx_vec_retain": (C.c_int, [_P]),
@@ -150,6 +155,8 @@ _SIGS = {
                                      C.POINTER(Info)]),
     "fsx_gmres_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
                                   C.POINTER(Info)]),
+    "fsx_cg_solve_diag": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings), DIAG_FN, _P,
+                                    C.POINTER(Info)]),
     "fsx_cg_history": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int,
                                  C.POINTER(C.c_int)]),
     "fsx_ir_cg_solve": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double,
@@ -365,6 +372,12 @@ class CSR:
               "fsx_csr_convert")
         return CSR(self.ctx, None, None, None, _handle=h)
 
+    def layout_bytes(self, x_bytes: int) -> int:
+        """Compulsory bytes of one product on the layout this matrix's plan reads."""
+        out = C.c_uint64()
+        check(lib().fsx_csr_layout_bytes(self.h, x_bytes, C.byref(out)), "layout_bytes")
+        return out.value
+
     def plan(self) -> tuple[int, int]:
         k, l = C.c_int(), C.c_int()
         check(lib().fsx_csr_plan(self.h, C.byref(k), C.byref(l)))
@@ -502,6 +515,38 @@ def cg_solve(A: CSR, b: Vector, x: Vector, settings: Settings,
                              b.h, x.h, C.byref(settings), C.byref(info)),
           "fsx_cg_solve")
     return info
+
+
+def cg_solve_diag(A: CSR, b: Vector, x: Vector, settings: Settings, diag,
+                  dinv: Vector | None = None) -> Info:
+    """fsx_cg_solve_diag: the fused CG calling diag(x, res) on the host after
+    every iteration (before the tolerance test); a true return stops the
+    solve with converged_user.  An exception in diag aborts the solve and is
+    raised here."""
+    err = []
+
+    def _cb(_user, _xh, res):
+        try:
+            return 1 if diag(x, res) else 0
+        except Exception as e:  # carried back to the caller
+            err.append(e)
+            return -1
+
+    fn = DIAG_FN(_cb)
+    info = Info()
+    rc = lib().fsx_cg_solve_diag(A.ctx.h, A.h, dinv.h if dinv is not None else None,
+                                 b.h, x.h, C.byref(settings), fn, None, C.byref(info))
+    if err:
+        raise err[0]
+    check(rc, "fsx_cg_solve_diag")
+    return info
+
+
+def alloc_stats() -> tuple[int, int]:
+    """(hipMalloc calls the library has made, bytes its live allocations hold)."""
+    n, b = C.c_uint64(), C.c_uint64()
+    check(lib().fsx_alloc_stats(C.byref(n), C.byref(b)), "fsx_alloc_stats")
+    return n.value, b.value
 
 
 def bicgstab_solve(A: CSR, b: Vector, x: Vector, settings: Settings,

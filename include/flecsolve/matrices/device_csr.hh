@@ -92,3 +92,6 @@ struct bound_jacobi : op::base<> {
 }
 
 #endif
+
+// the bound Krylov solvers' hand-off to the fused drivers for this operator
+#include "flecsolve/solvers/device_dispatch.hh"

@@ -28,8 +28,18 @@ struct bicgstab : base<Params, typename Params::input_var_t,
 	explicit bicgstab(Params p) : base_t(std::move(p)) {}
 	const auto & get_operator() const { return params.A(); }
 
+	// On a device CSR with the identity or its Jacobi diagonal and device
+	// vectors: the fused device driver (flecsolve/solvers/device_dispatch.hh);
+	// otherwise the reference's sequence, generic().
 	template<class DomainVec, class RangeVec>
 	solve_info apply(const RangeVec & b, DomainVec & x) const {
+		using drv = detail::fused_driver<detail::bicgstab_kind, Params, DomainVec, RangeVec>;
+		if constexpr (drv::value) return drv::apply(params, b, x);
+		else return generic(b, x);
+	}
+
+	template<class DomainVec, class RangeVec>
+	solve_info generic(const RangeVec & b, DomainVec & x) const {
 		solve_info info;
 		const auto & A = params.A();
 		const auto & P = params.P();

@@ -29,8 +29,18 @@ struct cg : base<Params, typename Params::input_var_t,
 	const auto & get_operator() const { return params.A(); }
 	long negative_curvature = 0; // the reference logs these (cg.hh:101-104)
 
+	// On a device CSR with the identity or its Jacobi diagonal and device
+	// vectors: the fused device driver (flecsolve/solvers/device_dispatch.hh);
+	// otherwise the reference's sequence, generic().
 	template<class DomainVec, class RangeVec>
 	solve_info apply(const RangeVec & b, DomainVec & x) const {
+		using drv = detail::fused_driver<detail::cg_kind, Params, DomainVec, RangeVec>;
+		if constexpr (drv::value) return drv::apply(params, b, x);
+		else return generic(b, x);
+	}
+
+	template<class DomainVec, class RangeVec>
+	solve_info generic(const RangeVec & b, DomainVec & x) const {
 		using scalar = typename DomainVec::scalar;
 		using real = typename DomainVec::real;
 		solve_info info;

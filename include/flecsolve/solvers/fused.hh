@@ -5,10 +5,12 @@
 // iteration over any vec::core vector with one host round trip per
 // reduction, as the reference does.  When the operator is a device CSR (and
 // the preconditioner the identity or its Jacobi diagonal), the same
-// iterations -- same operation order, tolerance tests and solve_info, no user
-// diagnostic -- run as fused kernels with device-resident scalars captured in
-// HIP graphs (fsx_cg_solve, fsx_bicgstab_solve, fsx_gmres_solve).  These
-// policies give that path the operator interface of a bound solver:
+// iterations -- same operation order, tolerance tests and solve_info -- run
+// as fused kernels with device-resident scalars captured in HIP graphs
+// (fsx_cg_solve, fsx_bicgstab_solve, fsx_gmres_solve).  A bound solver
+// (cg::solver(s, work)(op::ref(A), P)) hands its solves to them by itself
+// (device_dispatch.hh); these policies are the same path held as an
+// operator without a work array:
 //
 //   auto S = op::make_shared<cg::fused>(settings, A);        // or A, dinv
 //   solve_info info = S.get().apply(b, x);
@@ -29,34 +31,10 @@
 #include "flecsolve/matrices/device_csr.hh"
 #include "flecsolve/solvers/bicgstab.hh"
 #include "flecsolve/solvers/cg.hh"
+#include "flecsolve/solvers/device_dispatch.hh"
 #include "flecsolve/solvers/gmres.hh"
 
 namespace flecsolve::detail {
-
-inline solve_info from_fsx(const fsx_info & fi) {
-	solve_info s;
-	s.status = static_cast<solve_info::stop_reason>(fi.status);
-	s.iters = fi.iters;
-	s.restarts = fi.restarts;
-	s.res_norm_initial = fi.res_norm_initial;
-	s.res_norm_final = fi.res_norm_final;
-	s.sol_norm_initial = fi.sol_norm_initial;
-	s.sol_norm_final = fi.sol_norm_final;
-	s.rhs_norm = fi.rhs_norm;
-	return s;
-}
-
-inline fsx_settings to_fsx(const solver_settings & s) {
-	fsx_settings f{};
-	f.maxiter = s.maxiter;
-	f.rtol = s.rtol;
-	f.atol = s.atol;
-	f.use_zero_guess = s.use_zero_guess;
-	f.max_krylov_dim = -1;
-	f.pre_side = 1;
-	f.restart = 0;
-	return f;
-}
 
 using fused_fn = int (*)(fsx_ctx *, const fsx_csr *, const fsx_vec *, const fsx_vec *,
                          fsx_vec *, const fsx_settings *, fsx_info *);

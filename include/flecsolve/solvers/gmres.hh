@@ -91,8 +91,18 @@ struct gmres : base<Params, typename Params::input_var_t,
 		dy.assign(dim + 1, 0.0);
 	}
 
+	// On a device CSR with the identity or its Jacobi diagonal and device
+	// vectors: the fused device driver (flecsolve/solvers/device_dispatch.hh);
+	// otherwise the reference's sequence, generic().
 	template<class DomainVec, class RangeVec>
 	solve_info apply(const RangeVec & b, DomainVec & x) const {
+		using drv = detail::fused_driver<detail::gmres_kind, Params, DomainVec, RangeVec>;
+		if constexpr (drv::value) return drv::apply(params, b, x);
+		else return generic(b, x);
+	}
+
+	template<class DomainVec, class RangeVec>
+	solve_info generic(const RangeVec & b, DomainVec & x) const {
 		using flecsolve::gmres::precond_side;
 		const auto & A = params.A();
 		const auto & P = params.P();

@@ -76,6 +76,19 @@ auto make_params(S && s, W && w, op::handle<A> a, op::handle<P> p, D && d) {
 }
 }
 
+namespace detail {
+// A fused device driver a bound solver's apply() may hand the solve to.
+// Specialised by flecsolve/solvers/device_dispatch.hh (included with the
+// device CSR operator) for a device CSR, the identity or Jacobi
+// preconditioner and device vectors; value == false keeps the generic
+// sequence.
+struct cg_kind {};
+struct bicgstab_kind {};
+struct gmres_kind {};
+template<class Kind, class Params, class DomainVec, class RangeVec, class = void>
+struct fused_driver : std::false_type {};
+}
+
 template<template<class> class OpType, class Settings, class Workspace>
 struct krylov_solver {
 	using settings_type = Settings;

@@ -93,6 +93,11 @@ int fsx_ctx_stream(fsx_ctx * ctx, void ** stream);
 /* Per-context counters of device reductions forced to the host (each is one
  * blocking .get() of the reference, topo_view.hh:265-269). */
 int fsx_ctx_counters(fsx_ctx * ctx, uint64_t * host_syncs, uint64_t * launches);
+/* Device allocations the library has made in this process (hipMalloc calls,
+ * all contexts) and the bytes its live allocations hold now.  No counterpart
+ * in the reference: lets a caller confirm that an entry point allocated
+ * nothing. */
+int fsx_alloc_stats(uint64_t * allocations, uint64_t * live_bytes);
 
 /* ---- vectors: vec::core data (vectors/core.hh:26-353) ------------------ */
 int fsx_vec_create(fsx_ctx * ctx, size_t n, fsx_dtype dt, fsx_vec ** out);
@@ -336,6 +341,18 @@ int fsx_krylov_cg_amg(fsx_ctx * ctx, const fsx_csr * A, fsx_amg * M, int flexibl
 int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
                  const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
                  fsx_info * info);
+/* fsx_cg_solve with the reference's user diagnostic (cg.hh:112-116,
+ * krylov_parameters.hh:29-31): after each iteration's x and r updates the
+ * library calls diag(user, x, |r|) on the host thread, before the tolerance
+ * test; a positive return stops the solve with FSX_CONVERGED_USER and
+ * iters = iteration + 1, zero continues, a negative one aborts the solve
+ * with FSX_ERR_STATE.  The callback may run library operations on x (same
+ * context).  Same kernels as fsx_cg_solve, one iteration at a time (one host
+ * synchronisation per iteration instead of the reference's three). */
+typedef int (*fsx_diag_fn)(void * user, fsx_vec * x, double res_norm);
+int fsx_cg_solve_diag(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                      const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                      fsx_diag_fn diag, void * user, fsx_info * info);
 /* Residual history of the last fsx_cg_solve on ctx (|r| per iteration). */
 int fsx_cg_history(fsx_ctx * ctx, double * hist, int cap, int * count);
 /* One pass of cg.hh:108-127: x += alpha p; r -= alpha w; z = D^{-1} r (when
@@ -528,6 +545,17 @@ int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
  * for A, timed with HIP events on the context stream. */
 int fsx_time_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
                   fsx_vec * y, int reps, double * avg_ms);
+/* Same for the fused CG's K1 kernel (w = A p with the per-block partials of
+ * w.p, the kernel and launch shape fsx_cg_solve and the refinement's inner
+ * solves use for A and p's dtype). */
+int fsx_time_cg_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_vec * w,
+                     int reps, double * avg_ms);
+/* Compulsory HBM bytes of one y = A x on the layout A's plan reads: every
+ * stored value slot and index of that layout once, x read once and y written
+ * once at x_bytes per element.  (For the slice-offset layout: value slots,
+ * one 8-byte pointer per 64-row slice, a 4-byte offset and an 8-byte lane
+ * mask per 64 slots; no column per entry.) */
+int fsx_csr_layout_bytes(const fsx_csr * A, size_t x_bytes, uint64_t * bytes);
 /* Same for a plain device copy of n doubles (bandwidth reference). */
 int fsx_time_copy(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst, int reps,
                   double * avg_ms);

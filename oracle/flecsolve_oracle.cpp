@@ -601,6 +601,12 @@ struct gmres_t {
 
 // ---- op::bicgstab::apply (flecsolve/solvers/bicgstab.hh:39-187) -----------
 
+// Planted mistake for the tests only: when set, the early exit compares |s|
+// with rtol * |b| (a relative tolerance) instead of the reference's absolute
+// rtol (bicgstab.hh:136-142), so a test can show that its inputs tell the two
+// apart.  Off unless a test switches it on.
+static int g_mutant_relative_early_exit = 0;
+
 int bicgstab(const op_t & A, const op_t & P, const double * b, double * x,
              int64_t n, const or_settings & settings,
              const diag_t & user_diagnostic, or_info * info,
@@ -667,7 +673,9 @@ int bicgstab(const op_t & A, const op_t & P, const double * b, double * x,
 		alpha = rho[1] / alpha;
 		axpy(-alpha, v.data(), res.data(), s.data(), n);
 		const double s_norm = l2norm(s.data(), n);
-		if (s_norm < settings.rtol) {
+		const double early_tol =
+			g_mutant_relative_early_exit ? terminate_tol : (double)settings.rtol;
+		if (s_norm < early_tol) {
 			axpy(alpha, p_hat.data(), x, x, n);
 			info->iters = iter;
 			info->status = OR_CONVERGED_RTOL;
@@ -1117,6 +1125,8 @@ int or_gmres(const or_csr * A, const or_prec * P, const double * b, double * x,
 // solver object to be re-applied with changed settings; `or_gmres` builds a
 // fresh object per call, which matches that test since reset() only sizes
 // workspace and each apply re-initialises everything it reads.
+
+void or_mutant_bicgstab_relative_early_exit(int on) { g_mutant_relative_early_exit = on; }
 
 int or_bicgstab(const or_csr * A, const or_prec * P, const double * b,
                 double * x, const or_settings * s, or_diag_fn diag,

@@ -124,6 +124,8 @@ def lib():
         L.or_cg_colours.argtypes = [C.POINTER(_Csr), P, P, P, C.POINTER(Settings), C.c_int,
                                     C.POINTER(Info)]
         L.or_cg_colours.restype = C.c_int
+        L.or_mutant_bicgstab_relative_early_exit.argtypes = [C.c_int]
+        L.or_mutant_bicgstab_relative_early_exit.restype = None
         _lib = L
     return _lib
 
@@ -309,6 +311,20 @@ def gmres(A, b, x0, settings, P=None, diag=None, hist_cap=0):
 def bicgstab(A, b, x0, settings, P=None, diag=None, hist_cap=0):
     x, info, rc, h = _solve("or_bicgstab", A, b, x0, settings, P, diag, hist_cap)
     return x, info, h
+
+
+class mutant_relative_early_exit:
+    """Context manager planting a mistake in the oracle's BiCGSTAB (tests
+    only): inside it, the early exit compares |s| with rtol * |b| instead of
+    the absolute rtol of bicgstab.hh:136-142."""
+
+    def __enter__(self):
+        lib().or_mutant_bicgstab_relative_early_exit(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_mutant_bicgstab_relative_early_exit(0)
+        return False
 
 
 # ---- BLAS-1 ----------------------------------------------------------------

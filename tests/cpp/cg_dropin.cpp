@@ -178,6 +178,102 @@ int main(int argc, char ** argv) {
 		fails += !fi.success() || frel > 1e-9 || fi.iters >= ci2.iters;
 	}
 	{
+		// the reference-shaped call on the device operator hands the solve to
+		// the fused driver: the same solve_info and x bits as calling
+		// fsx_cg_solve / fsx_bicgstab_solve / fsx_gmres_solve directly, with a
+		// host synchronisation per chunk of iterations instead of three per
+		// iteration; generic() still runs the reference's sequence
+		op::core<mat::device_csr> A(ctx, Ah);
+		op::core<mat::device_dinv> D(A);
+		const std::size_t m = (std::size_t)n * n;
+		vec::device<double> rhs(ctx, m), y1(ctx, m), y2(ctx, m), y3(ctx, m);
+		rhs.set_random(11);
+		cg::settings cs;
+		cs.maxiter = 3000;
+		cs.rtol = 1e-10;
+		cs.use_zero_guess = true;
+		auto same = [&](const solve_info & a, const fsx_info & f, vec::device<double> & u,
+		                vec::device<double> & w) {
+			std::vector<double> hu(m), hw(m);
+			fsx_check(fsx_vec_download(ctx, u.handle(), hu.data(), m), "download");
+			fsx_check(fsx_vec_download(ctx, w.handle(), hw.data(), m), "download");
+			return (int)a.status == f.status && a.iters == f.iters &&
+			       a.restarts == f.restarts && a.res_norm_final == f.res_norm_final &&
+			       a.sol_norm_final == f.sol_norm_final && a.rhs_norm == f.rhs_norm &&
+			       hu == hw;
+		};
+		uint64_t s0, s1, s2, l;
+		auto slv = cg::solver(cs, vec::make_device_work<cg::nwork>(rhs))(op::ref(A), op::ref(D));
+		fsx_ctx_counters(ctx, &s0, &l);
+		solve_info a = slv(rhs, y1);
+		fsx_ctx_counters(ctx, &s1, &l);
+		fsx_settings f = detail::to_fsx(cs);
+		fsx_info fi{};
+		fsx_check(fsx_cg_solve(ctx, Ah, D.dinv.handle(), rhs.handle(), y2.handle(), &f, &fi),
+		          "cg");
+		const bool cg_same = same(a, fi, y1, y2);
+		fsx_ctx_counters(ctx, &s1, &l);
+		s1 = s1 - s0;
+		fsx_ctx_counters(ctx, &s0, &l);
+		solve_info g = slv.generic(rhs, y3);
+		fsx_ctx_counters(ctx, &s2, &l);
+		s2 = s2 - s0;
+		std::printf("dispatch cg+dinv: %d iterations = fsx_cg_solve: %s; host syncs: "
+		            "reference-shaped call %llu, generic() %llu (%d iterations)\n",
+		            a.iters, cg_same ? "yes" : "NO", (unsigned long long)s1,
+		            (unsigned long long)s2, g.iters);
+		fails += !a.success() || !cg_same || s1 * 8 > (uint64_t)a.iters ||
+		         s2 < 3 * (uint64_t)g.iters || std::abs(g.iters - a.iters) > 2;
+		bicgstab::settings bs;
+		bs.maxiter = 400;
+		bs.rtol = 1e-10;
+		bs.use_zero_guess = true;
+		auto bi = bicgstab::solver(bs, vec::make_device_work<bicgstab::nwork>(rhs))(op::ref(A));
+		solve_info ba = bi(rhs, y1);
+		f = detail::to_fsx(bs);
+		fsx_check(fsx_bicgstab_solve(ctx, Ah, nullptr, rhs.handle(), y2.handle(), &f, &fi),
+		          "bicgstab");
+		const bool bi_same = same(ba, fi, y1, y2);
+		gmres::settings gs;
+		gs.maxiter = 3000;
+		gs.rtol = 1e-10;
+		gs.use_zero_guess = true;
+		gs.max_krylov_dim = 30;
+		gs.restart = true;
+		auto gm = gmres::solver(gs, vec::make_device_work<gmres::nwork>(rhs))(op::ref(A),
+		                                                                      op::ref(D));
+		solve_info ga = gm(rhs, y1);
+		f = detail::to_fsx(gs);
+		f.max_krylov_dim = 30;
+		f.pre_side = 1;
+		f.restart = 1;
+		fsx_check(fsx_gmres_solve(ctx, Ah, D.dinv.handle(), rhs.handle(), y2.handle(), &f, &fi),
+		          "gmres");
+		const bool gm_same = same(ga, fi, y1, y2);
+		std::printf("dispatch bicgstab: %d iterations = fsx_bicgstab_solve: %s; gmres(30)+dinv: "
+		            "%d iterations = fsx_gmres_solve: %s\n",
+		            ba.iters, bi_same ? "yes" : "NO", ga.iters, gm_same ? "yes" : "NO");
+		fails += !ba.success() || !bi_same || !ga.success() || !gm_same;
+		// a user diagnostic: the fused per-iteration path, stopping the solve
+		// at the iteration it asks for (converged_user, iters = it + 1)
+		int calls = 0;
+		auto stop7 = [&](const auto &, double) { return ++calls == 7; };
+		auto ds = cg::solver(cs, vec::make_device_work<cg::nwork>(rhs))(op::ref(A), op::ref(D),
+		                                                                 stop7);
+		solve_info di = ds(rhs, y1);
+		calls = 0;
+		solve_info dg = ds.generic(rhs, y2);
+		y1.axpy(-1.0, y2, y1);
+		const double ddx = y1.l2norm().get() / y2.l2norm().get();
+		std::printf("dispatch cg with a stopping diagnostic: %d iterations, %s; generic %d, "
+		            "|dx| %.1e\n",
+		            di.iters, di.status == solve_info::stop_reason::converged_user ? "user" : "?",
+		            dg.iters, ddx);
+		fails += di.iters != 7 || dg.iters != 7 ||
+		         di.status != solve_info::stop_reason::converged_user ||
+		         dg.status != solve_info::stop_reason::converged_user || ddx > 1e-12;
+	}
+	{
 		// the fused solves behind the operator interface: CG + Dinv and the
 		// mixed-precision refinement, against each other
 		op::core<mat::device_csr> A(ctx, Ah);
@@ -221,7 +317,10 @@ int main(int argc, char ** argv) {
 		pb.set_scalar(0.0);
 		px.set_scalar(2.0);
 		auto plain = cg::solver(ps, vec::make_device_work<cg::nwork>(pb))(op::ref(A));
-		solve_info p0 = plain(pb, px);
+		// the parallel CSR runs the generic sequence: compare with the plain
+		// operator's generic() (the reference-shaped call on the plain operator
+		// takes the fused path and sums its dot products in another order)
+		solve_info p0 = plain.generic(pb, px);
 		int iters[2];
 		double dx[2];
 		for (int k = 0; k < 2; ++k) {

@@ -143,3 +143,49 @@ def test_batched_edge_cases(ctx):
     A2 = ctx.csr(rp2, ci2, v)
     with pytest.raises(fsx.FsxError):
         fsx.Batch(A2, rows)
+
+
+def test_batched_full_edge_range_matches_oracle(ctx):
+    """Edges spanning 8..48 as the 2048-system workload draws them, with an
+    8^3 (512 rows) and a 48^3 system (110,592 rows in one workgroup): FP64
+    CG + Jacobi under the autotuned mapping and under each mapping family,
+    system by system against the oracle; then the FP32-inner refinement,
+    every system at true residual <= 1e-10 and within 1e-8 of the oracle's
+    1e-10 CG solution (each 1e-10-residual solution of these systems carries
+    a forward error of up to 5e-10: the oracle's against its own rtol 1e-14
+    solve)."""
+    e, rows, rp, ci, v = P.batch(12, 8, 48, 4007, 4100, 1.0)
+    assert 8 in e and 48 in e
+    b = P.uniform(int(rows[-1]), 4200)
+    A = ctx.csr(rp, ci, v)
+    assert A.plan() == (3, 0)
+    A.set_plan(2)   # the sliced layout too, for mapping 0
+    A.set_plan(3)
+    B = fsx.Batch(A, rows)
+    d = A.diag_inv()
+    bd = ctx.vec(b)
+    s64 = fsx.Settings(maxiter=1000, rtol=1e-10, use_zero_guess=True)
+    x = ctx.zeros(b.size)
+    B.tune(bd, x, d, sample=12, iters=10)
+    for cfg in (None, (1024, 2, 1), (256, 0, 1), (512, 1, 4)):
+        if cfg is not None:
+            B.set_config(*cfg)
+        x = ctx.zeros(b.size)
+        it, st, rs = B.cg(bd, x, s64, dinv=d)
+        check_against_oracle(rows, rp, ci, v, b, x.get(), it, st)
+    A32 = A.convert(fsx.F32)
+    A32.set_values_sym_scaled(A)
+    x = ctx.zeros(b.size)
+    outer, inner, st, rs = B.ir_cg(A32, bd, x, 1e-10, fsx.Settings(maxiter=2000, rtol=1e-4),
+                                   jacobi=2)
+    assert (st == 1).all()
+    S = sp.csr_matrix((v, ci, rp))
+    xg = x.get()
+    for k in range(rows.size - 1):
+        sl = slice(rows[k], rows[k + 1])
+        assert np.linalg.norm(b[sl] - S[sl] @ xg) / np.linalg.norm(b[sl]) <= 1.01e-10, k
+        Ao = O.Csr(*local(rows, rp, ci, v, k))
+        xo, io, _ = O.cg(Ao, b[sl], np.zeros(rows[k + 1] - rows[k]),
+                         O.Settings(maxiter=1000, rtol=1e-10, use_zero_guess=True),
+                         P=O.prec(1, Ao.dinv()))
+        assert rel(xg[sl], xo) <= 1e-8, (k, e[k])

@@ -112,3 +112,32 @@ def test_breakdown_restart(ctx, M):
     info = fsx.bicgstab_solve(ctx.csr(rp, ci, v), ctx.vec(b), x, fsx.Settings(**s))
     assert (info.restarts, info.iters, info.reason) == (io.restarts, io.iters, io.reason)
     np.testing.assert_allclose(x.get(), np.linalg.solve(M, b), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("scale", [1e-5, 1e5])
+def test_early_exit_absolute_tolerance(ctx, scale):
+    """Where rtol and rtol |b| disagree (b scaled down: the absolute early
+    exit on |s| < rtol fires, iters = iter; b scaled up: rtol <= |s| <
+    rtol |b| at the converging iteration, no early exit, iters = iter + 1),
+    the fused and the generic device solvers stop at the oracle's iteration
+    with its status (bicgstab.hh:136-142).  The oracle's own pin shows these
+    inputs tell the absolute from a relative tolerance."""
+    n = 12
+    rp, ci, v = P.convdiff3d(n, (10.0, 5.0, 2.5))
+    b = P.uniform(n ** 3, 4) * scale
+    Ao = O.Csr(rp, ci, v)
+    s = dict(maxiter=500, rtol=1e-8, use_zero_guess=True)
+    xo, io, _ = O.bicgstab(Ao, b, np.zeros(n ** 3), O.Settings(**s))
+    with O.mutant_relative_early_exit():
+        _, im, _ = O.bicgstab(Ao, b, np.zeros(n ** 3), O.Settings(**s))
+    assert im.iters != io.iters
+    A = ctx.csr(rp, ci, v)
+    for label in ("fused", "generic"):
+        x = ctx.zeros(n ** 3)
+        if label == "fused":
+            info = fsx.bicgstab_solve(A, ctx.vec(b), x, fsx.Settings(**s))
+        else:
+            info, _ = fsx.krylov("bicgstab", A, ctx.vec(b), x, fsx.Settings(**s))
+        assert info.reason == io.reason == "converged_rtol", label
+        assert info.iters == io.iters, (label, info.iters, io.iters)
+        assert rel(x.get(), xo) <= 1e-10, label

@@ -193,6 +193,34 @@ def test_grid_stride_sizes_iteration_matched(ctx, plan):
     assert rel(xd.get(), xo) <= 1e-12
 
 
+def test_wide_block_shape_iteration_matched(ctx):
+    """From 2^23 rows the FP64 reducing product runs in 512-thread blocks
+    (another block partial tree than below that size): 204^3 = 8,489,664
+    rows, the SpMV+dot bit-compared and 25 CG + Jacobi iterations against
+    the oracle, iteration-matched (both stop at maxiter)."""
+    n = 204
+    rp, ci, v, Ao = diffusion(n)
+    N = n ** 3
+    assert N >= 1 << 23
+    b = P.uniform(N, 2002)
+    A = ctx.csr(rp, ci, v)
+    assert A.plan() == (3, 0)
+    bd = ctx.vec(b)
+    w = ctx.zeros(N)
+    pw = fsx.spmv_dot(A, bd, w)
+    wref = Ao.spmv(b)
+    np.testing.assert_array_equal(w.get(), wref)
+    assert abs(pw - O.dot(wref, b)) <= 1e-12 * np.abs(wref * b).sum()
+    s = dict(maxiter=25, rtol=1e-12, use_zero_guess=True)
+    xo, io, ho = O.cg(Ao, b, np.zeros(N), O.Settings(**s), P=O.prec(1, Ao.dinv()),
+                      hist_cap=25)
+    xd = ctx.zeros(N)
+    info = fsx.cg_solve(A, bd, xd, fsx.Settings(**s), dinv=A.diag_inv())
+    assert info.reason == io.reason == "diverged_iters"
+    np.testing.assert_allclose(fsx.cg_history(ctx), ho, rtol=1e-10)
+    assert rel(xd.get(), xo) <= 1e-12
+
+
 def test_layouts_give_identical_trajectories(ctx):
     """Every serial-row layout gives the same SpMV bits; the fused dot
     products are reduced over each layout's own block grid, so iterates
@@ -301,3 +329,78 @@ def test_new_entry_points_reject_bad_arguments(ctx):
     A32.set_values_sym_scaled(A)
     info = fsx.ir_cg_solve(A, A32, b, x, 1e-10, inner, jacobi=2)
     assert info.status == 1
+
+
+def test_cg_user_diagnostic(ctx):
+    """The fused CG with the reference's user diagnostic (cg.hh:112-116):
+    called after every iteration with the updated x and |r|, before the
+    tolerance test.  A diagnostic that never stops gives fsx_cg_solve's
+    iterates bit for bit, its |r| values are the residual history; one that
+    stops at iteration 9 ends the solve there with converged_user and the
+    oracle's x (its diagnostic stopping at the same call); the reference
+    test's A-norm diagnostic (solvers/test/cg.cc:16-54: monotone decrease
+    and the Trefethen-Bau bound for b = 0, x0 = set_random) holds; an
+    exception in the diagnostic aborts the solve."""
+    n = 40
+    rp, ci, v = P.poisson2d(n)
+    Ao = O.Csr(rp, ci, v)
+    A = ctx.csr(rp, ci, v)
+    d = A.diag_inv()
+    b = P.uniform(n * n, 31)
+    s = dict(maxiter=2000, rtol=1e-10, use_zero_guess=True)
+    x1 = ctx.zeros(n * n)
+    i1 = fsx.cg_solve(A, ctx.vec(b), x1, fsx.Settings(**s), dinv=d)
+    h1 = fsx.cg_history(ctx)
+    seen = []
+    x2 = ctx.zeros(n * n)
+    i2 = fsx.cg_solve_diag(A, ctx.vec(b), x2, fsx.Settings(**s),
+                           lambda x, res: seen.append(res) and False, dinv=d)
+    assert i2.as_dict() == i1.as_dict()
+    assert np.array_equal(x2.get(), x1.get())
+    np.testing.assert_array_equal(np.array(seen), h1)
+
+    # stop at the 9th call: converged_user, iters = 9
+    calls = []
+    x3 = ctx.zeros(n * n)
+    i3 = fsx.cg_solve_diag(A, ctx.vec(b), x3, fsx.Settings(**s),
+                           lambda x, res: calls.append(res) or len(calls) == 9, dinv=d)
+    oc = []
+    xo, io, _ = O.cg(Ao, b, np.zeros(n * n), O.Settings(**s), P=O.prec(1, Ao.dinv()),
+                     diag=lambda x, res: oc.append(res) or len(oc) == 9)
+    assert i3.reason == io.reason == "converged_user"
+    assert i3.iters == io.iters == 9
+    assert rel(x3.get(), xo) <= 1e-13
+    np.testing.assert_allclose(calls, oc, rtol=1e-12)
+
+    # the reference test: b = 0, x0 = random, the A-norm of the error (= x)
+    # falls monotonically and below 2 ((sqrt k - 1)/(sqrt k + 1))^i |e0|_A
+    lam = np.array([4 - 2 * np.cos(np.pi * i / (n + 1)) - 2 * np.cos(np.pi * j / (n + 1))
+                    for i in range(1, n + 1) for j in range(1, n + 1)])
+    kappa = lam.max() / lam.min()
+    S = sp.csr_matrix((v, ci, rp))
+    x0 = np.random.default_rng(7).random(n * n)
+    anorm = lambda y: np.sqrt(y @ (S @ y))
+    e0 = anorm(x0)
+    errs = []
+    xd = ctx.vec(x0)
+    i4 = fsx.cg_solve_diag(A, ctx.zeros(n * n), xd,
+                           fsx.Settings(maxiter=2000, rtol=1e-9, use_zero_guess=False),
+                           lambda x, res: errs.append(anorm(x.get())) and False)
+    assert i4.reason == "converged_rtol" and i4.iters == len(errs)
+    frac = (np.sqrt(kappa) - 1) / (np.sqrt(kappa) + 1)
+    bound = 2 * frac ** np.arange(1, len(errs) + 1) * e0
+    assert np.all(np.diff(np.concatenate([[e0], errs])) < 0)
+    assert np.all(np.array(errs) < bound)
+
+    class Stop(RuntimeError):
+        pass
+
+    def bad(x, res):
+        raise Stop("from the diagnostic")
+
+    with pytest.raises(Stop):
+        fsx.cg_solve_diag(A, ctx.vec(b), ctx.zeros(n * n), fsx.Settings(**s), bad, dinv=d)
+    # the context still solves afterwards
+    x5 = ctx.zeros(n * n)
+    i5 = fsx.cg_solve(A, ctx.vec(b), x5, fsx.Settings(**s), dinv=d)
+    assert i5.as_dict() == i1.as_dict()

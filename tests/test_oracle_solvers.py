@@ -246,3 +246,46 @@ def test_fcg_trajectory_equals_pcg(jacobi):
         scale = np.linalg.norm(xs_cg[k])
         assert np.linalg.norm(xs_fcg[k] - xs_cg[k]) <= 1e-8 * scale, k
         assert abs(res_fcg[k] - res_cg[k]) <= 1e-6 * res_cg[0], k
+
+
+def test_bicgstab_early_exit_tolerance_is_absolute():
+    """bicgstab.hh:136-142 exits when |s| < rtol with rtol absolute (not
+    rtol |b|) and reports iters = iter.  Pinned where the two tolerances
+    disagree, on both sides: b scaled down (the absolute test fires first:
+    early exit with a true residual below rtol but far above rtol |b|) and
+    b scaled up (rtol <= |s| < rtol |b| at the converging iteration: the
+    absolute test does not fire, the full step runs and iters = iter + 1).
+    The planted relative-tolerance mistake changes the outcome in both."""
+    n = 12
+    rp, ci, v = P.convdiff3d(n, (10.0, 5.0, 2.5))
+    A = O.Csr(rp, ci, v)
+    S = sp.csr_matrix((v, ci, rp))
+    b0 = P.uniform(n ** 3, 4)
+    s = O.Settings(maxiter=500, rtol=1e-8, use_zero_guess=True)
+    rtol = float(np.float32(1e-8))
+    # b scaled down: |b| = 2.4e-4, so rtol |b| = 2.4e-12
+    b = b0 * 1e-5
+    x, info, h = O.bicgstab(A, b, np.zeros(n ** 3), s, hist_cap=500)
+    tr = np.linalg.norm(b - S @ x)
+    assert info.reason == "converged_rtol"
+    assert tr < rtol and tr > 100 * rtol * np.linalg.norm(b)
+    # the early exit skips the residual update: the last recorded |res| is
+    # that of the previous iteration, still above rtol
+    assert info.res_norm_final > rtol
+    assert np.count_nonzero(h) == info.iters
+    with O.mutant_relative_early_exit():
+        xm, im, _ = O.bicgstab(A, b, np.zeros(n ** 3), s)
+    assert im.iters > info.iters + 2
+    # b scaled up: |b| = 2.4e6; the converging iteration's |s| lies between
+    # rtol and rtol |b|, so only the planted mistake takes the early exit
+    b = b0 * 1e5
+    x, info, h = O.bicgstab(A, b, np.zeros(n ** 3), s, hist_cap=500)
+    tol = rtol * np.linalg.norm(b)
+    tr = np.linalg.norm(b - S @ x)
+    assert info.reason == "converged_rtol"
+    assert tr < 1.001 * tol
+    assert h[info.iters - 1] < tol <= h[info.iters - 2]
+    with O.mutant_relative_early_exit():
+        xm, im, _ = O.bicgstab(A, b, np.zeros(n ** 3), s)
+    assert im.iters == info.iters - 1
+    assert not np.array_equal(xm, x)
