@@ -427,8 +427,8 @@ static_assert(sizeof(cg_state) * 3 <= sizeof(double) * kCgHostSlot,
 
 // Where a CG solve keeps its work.  The allocating entry points point these
 // at context-owned buffers (pooled vectors, the context's partials, the
-// cache's state buffer); a caller that may not allocate points them at
-// memory of its own.
+// cache's state buffer); fsx_cg_solve_noalloc points every device field
+// into its static scratch array.
 struct cg_work {
 	void *r, *z, *p, *w; // z: null without a preconditioner
 	double * part;       // partials: at least cg_partials_need(A, n) doubles
@@ -593,7 +593,7 @@ static cg_work pooled_work(fsx_ctx * ctx, const fsx_csr * A, size_t n, fsx_dtype
 	return wk;
 }
 
-// Iteration loop without graphs, events or allocations:
+// Iteration loop without graphs, events or allocations (fsx_cg_solve_noalloc):
 // chunks of iterations launched directly, the host reading the state after
 // each.  Same kernels in the same order as the captured loop, so the same
 // iterates.
@@ -1598,6 +1598,65 @@ static void gm_enqueue_cycle(fsx_ctx * ctx, const gm_bufs & B, gm_state * st) {
 
 } // namespace fsx
 
+// ---- scratch for the no-allocation CG entry point ---------------------------
+//
+// fsx_cg_solve_noalloc keeps everything a CG solve needs beyond its inputs and
+// outputs -- the state, setup scalars, reduction partials and the work
+// vectors r, z, p, w -- in one file-scope device array.  Each GPU holds its
+// own copy (34 MiB) from the library's load until the process ends.  Host
+// code never passes the array's name to a kernel: it is a host-side stand-in,
+// not a device address.  fsx_ctx_create looks the address up on its device
+// (hipGetSymbolAddress, once per device) and the entry point uses that.
+
+namespace fsx {
+constexpr size_t kNoallocMaxRows = FSX_NOALLOC_MAX_ROWS;
+constexpr size_t kNoallocMaxPartials = (size_t)1 << 18; // doubles
+constexpr size_t kNoallocVecBytes = kNoallocMaxRows * sizeof(double);
+constexpr size_t kNoallocStateOff = 0;
+constexpr size_t kNoallocSlotOff = 256;
+constexpr size_t kNoallocPartOff = 4096;
+constexpr size_t kNoallocVecOff = kNoallocPartOff + kNoallocMaxPartials * sizeof(double);
+constexpr size_t kNoallocBytes = kNoallocVecOff + 4 * kNoallocVecBytes;
+static_assert(sizeof(cg_state) <= kNoallocSlotOff &&
+                  kNoallocSlotOff + 5 * sizeof(double) <= kNoallocPartOff,
+              "scratch header layout");
+static_assert(kNoallocVecOff % 256 == 0 && kNoallocVecBytes % 256 == 0, "alignment");
+} // namespace fsx
+
+static __device__ __align__(256) unsigned char scratch[fsx::kNoallocBytes];
+
+namespace fsx {
+constexpr int kNoallocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void * g_scratch_ptr[kNoallocMaxDevices];
+static hipError_t g_scratch_err[kNoallocMaxDevices];
+static bool g_scratch_done[kNoallocMaxDevices];
+
+void noalloc_init(int device) {
+	if (device < 0 || device >= kNoallocMaxDevices) return;
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (g_scratch_done[device]) return;
+	void * p = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	g_scratch_err[device] = e;
+	g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+	g_scratch_done[device] = true;
+}
+
+// the scratch array's device address on `device` (set up by fsx_ctx_create)
+static char * noalloc_base(int device) {
+	FSX_REQUIRE(device >= 0 && device < kNoallocMaxDevices, "cg_solve_noalloc: device index");
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_done[device])
+		throw error(FSX_ERR_STATE, "cg_solve_noalloc: scratch address not looked up on this "
+		                           "device (no context was created on it)");
+	if (g_scratch_err[device] != hipSuccess)
+		throw error(FSX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+		                             hipGetErrorString(g_scratch_err[device]));
+	return (char *)g_scratch_ptr[device];
+}
+} // namespace fsx
+
 using namespace fsx;
 
 extern "C" {
@@ -1652,6 +1711,46 @@ int fsx_cg_solve_diag(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 		if (cnt > 0)
 			FSX_HIP(hipMemcpy(ctx->last_cg_hist.data(), ctx->cg_cache->histbuf->p,
 			                  sizeof(double) * cnt, hipMemcpyDeviceToHost));
+	});
+}
+
+int fsx_cg_solve_noalloc(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                         const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                         fsx_info * info) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A && s && info, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)b->n == A->nrows,
+		            "cg_noalloc: shape mismatch");
+		if (dinv) check_same(dinv, b);
+		FSX_REQUIRE(b->dt == FSX_F32 || A->vt == FSX_F64,
+		            "cg_noalloc: FP64 vectors need an FP64 matrix");
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "cg_noalloc: b and x must differ");
+		const size_t n = b->n;
+		FSX_REQUIRE(n <= kNoallocMaxRows,
+		            "cg_noalloc: " + std::to_string(n) + " rows exceed the scratch limit of " +
+		                std::to_string(kNoallocMaxRows) + " (FSX_NOALLOC_MAX_ROWS)");
+		FSX_REQUIRE(cg_partials_need(A, n) <= kNoallocMaxPartials,
+		            "cg_noalloc: the matrix's reducing SpMV leaves " +
+		                std::to_string(cg_partials_need(A, n)) +
+		                " partials, more than the scratch array's " +
+		                std::to_string(kNoallocMaxPartials));
+		char * base = noalloc_base(ctx->device);
+		char * vec = base + kNoallocVecOff;
+		cg_work wk{};
+		wk.st = (cg_state *)(base + kNoallocStateOff);
+		wk.slots = (double *)(base + kNoallocSlotOff);
+		wk.part = (double *)(base + kNoallocPartOff);
+		wk.r = vec;
+		wk.z = dinv ? vec + kNoallocVecBytes : nullptr;
+		wk.p = vec + 2 * kNoallocVecBytes;
+		wk.w = vec + 3 * kNoallocVecBytes;
+		wk.hist = nullptr;
+		wk.graphs = false;
+		if (b->dt == FSX_F64) cg_solve_impl<double>(ctx, A, dinv, b, x, *s, info, wk, nullptr);
+		else cg_solve_impl<float>(ctx, A, dinv, b, x, *s, info, wk, nullptr);
 	});
 }
 

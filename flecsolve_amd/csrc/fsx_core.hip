@@ -432,6 +432,8 @@ int fsx_ctx_create(int device, fsx_ctx ** out) {
 		c->num_cus = prop.multiProcessorCount;
 		// default-priority, non-blocking stream
 		FSX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+		// the no-allocation CG's scratch address on this device
+		noalloc_init(device);
 		size_t ws_bytes = sizeof(double) * kScalarSlots;
 		c->ws_buf = std::make_unique<dbuf>(ws_bytes);
 		char * base = (char *)c->ws_buf->p;

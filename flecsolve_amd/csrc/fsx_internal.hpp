@@ -248,6 +248,9 @@ double * ensure_partials(fsx_ctx * ctx, size_t count);
 double * ensure_relax_buffer(fsx_ctx * ctx, size_t n);
 // Drop captured solver graphs of ctx (defined with the CG driver).
 void invalidate_graphs(fsx_ctx * ctx);
+// Look up the no-allocation entry point's scratch array on `device` (the
+// current device), once per device; called by fsx_ctx_create.
+void noalloc_init(int device);
 
 // host-side scalar fetch of a device slot (sync)
 double fetch_scalar(fsx_ctx * ctx, const double * dslot);

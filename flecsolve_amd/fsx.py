@@ -155,6 +155,8 @@ _SIGS = {
                                      C.POINTER(Info)]),
     "fsx_gmres_solve": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
                                   C.POINTER(Info)]),
+    "fsx_cg_solve_noalloc": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings),
+                                       C.POINTER(Info)]),
     "fsx_cg_solve_diag": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings), DIAG_FN, _P,
                                     C.POINTER(Info)]),
     "fsx_cg_history": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int,
@@ -540,6 +542,20 @@ def cg_solve_diag(A: CSR, b: Vector, x: Vector, settings: Settings, diag,
         raise err[0]
     check(rc, "fsx_cg_solve_diag")
     return info
+
+
+def cg_solve_noalloc(A: CSR, b: Vector, x: Vector, settings: Settings,
+                     dinv: Vector | None = None) -> Info:
+    """fsx_cg_solve_noalloc: the fused CG with every work buffer in the
+    library's static per-device scratch array (n <= NOALLOC_MAX_ROWS)."""
+    info = Info()
+    check(lib().fsx_cg_solve_noalloc(A.ctx.h, A.h, dinv.h if dinv is not None else None,
+                                     b.h, x.h, C.byref(settings), C.byref(info)),
+          "fsx_cg_solve_noalloc")
+    return info
+
+
+NOALLOC_MAX_ROWS = 1 << 20
 
 
 def alloc_stats() -> tuple[int, int]:

@@ -353,6 +353,25 @@ typedef int (*fsx_diag_fn)(void * user, fsx_vec * x, double res_norm);
 int fsx_cg_solve_diag(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
                       const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
                       fsx_diag_fn diag, void * user, fsx_info * info);
+/* fsx_cg_solve_noalloc: no counterpart in the reference.  fsx_cg_solve for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels,
+ * iterates, iteration count and solve_info as fsx_cg_solve; it allocates
+ * nothing: the iteration state, setup scalars, reduction partials and the
+ * work vectors r, z, p, w live in a static device array of each GPU (34 MiB,
+ * held from the library's load until the process ends), whose address
+ * fsx_ctx_create looks up on its device.  Iterations are launched directly
+ * (no graphs), the host reading the state after each chunk; no residual
+ * history is recorded (fsx_cg_history keeps the last fsx_cg_solve's).
+ * Limit: n <= FSX_NOALLOC_MAX_ROWS (the reduction partials of every SpMV
+ * plan then fit too); larger systems return FSX_ERR_ARG.  All calls on one
+ * GPU share the array and it keeps the last call's contents (every solve
+ * writes what it reads first): calls that overlap on one GPU (several host
+ * threads or contexts) race. */
+#define FSX_NOALLOC_MAX_ROWS ((size_t)1 << 20)
+int fsx_cg_solve_noalloc(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
+                         const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                         fsx_info * info);
 /* Residual history of the last fsx_cg_solve on ctx (|r| per iteration). */
 int fsx_cg_history(fsx_ctx * ctx, double * hist, int cap, int * count);
 /* One pass of cg.hh:108-127: x += alpha p; r -= alpha w; z = D^{-1} r (when
