@@ -212,7 +212,6 @@ int main(int argc, char ** argv) {
 		fsx_check(fsx_cg_solve(ctx, Ah, D.dinv.handle(), rhs.handle(), y2.handle(), &f, &fi),
 		          "cg");
 		const bool cg_same = same(a, fi, y1, y2);
-		fsx_ctx_counters(ctx, &s1, &l);
 		s1 = s1 - s0;
 		fsx_ctx_counters(ctx, &s0, &l);
 		solve_info g = slv.generic(rhs, y3);
@@ -222,7 +221,7 @@ int main(int argc, char ** argv) {
 		            "reference-shaped call %llu, generic() %llu (%d iterations)\n",
 		            a.iters, cg_same ? "yes" : "NO", (unsigned long long)s1,
 		            (unsigned long long)s2, g.iters);
-		fails += !a.success() || !cg_same || s1 * 8 > (uint64_t)a.iters ||
+		fails += !a.success() || !cg_same || s1 * 4 > (uint64_t)a.iters ||
 		         s2 < 3 * (uint64_t)g.iters || std::abs(g.iters - a.iters) > 2;
 		bicgstab::settings bs;
 		bs.maxiter = 400;
