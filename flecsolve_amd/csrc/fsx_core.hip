@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_finalize_kernel(
 
 // sum of pow(x_i, p): the general-p lp_norm of the FleCSI vector path
 // (vectors/operations/topo_tasks.hh:216-226 takes std::pow(u, p) with no
-// absolute value)
+// absolute value), folded in `real` = double for FP64 vectors
 template<class T>
 __global__ __launch_bounds__(kRedThreads) void powsum_kernel(size_t n, const T * __restrict__ x,
                                                              double p,
@@ -302,6 +302,37 @@ __global__ __launch_bounds__(kRedThreads) void powsum_kernel(size_t n, const T *
 	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
 	__syncthreads();
 	if (threadIdx.x == 0) partials[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// FP32 vectors: lp_norm_local folds into `real`, float here -- each term
+// std::pow(float, int) (evaluated in double) is rounded to float and summed
+// in float, within a block and then over the blocks in block order
+__global__ __launch_bounds__(kRedThreads) void powsum_f32_kernel(size_t n,
+                                                                 const float * __restrict__ x,
+                                                                 double p,
+                                                                 double * __restrict__ partials) {
+	float acc = 0.f;
+	const size_t stride = (size_t)gridDim.x * kRedThreads;
+	for (size_t i = (size_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += stride)
+		acc += (float)pow((double)x[i], p);
+	__shared__ float sh[4];
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+	__syncthreads();
+	if (threadIdx.x == 0) partials[blockIdx.x] = (double)(((sh[0] + sh[1]) + sh[2]) + sh[3]);
+}
+
+__global__ __launch_bounds__(kRedThreads) void powsum_f32_finalize_kernel(
+	const double * __restrict__ partials, unsigned nblk, double * out) {
+	float v = 0.f;
+	for (unsigned b = threadIdx.x; b < nblk; b += kRedThreads) v += (float)partials[b];
+	__shared__ float sh[4];
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) *out = (double)(((sh[0] + sh[1]) + sh[2]) + sh[3]);
 }
 
 template<red_kind K>
@@ -581,6 +612,39 @@ int fsx_vec_download(fsx_ctx * ctx, const fsx_vec * v, void * host, size_t n) {
 	});
 }
 
+int fsx_vec_get(fsx_ctx * ctx, const fsx_vec * v, size_t i, double * out) {
+	return guard([&] {
+		check_ctx(ctx);
+		check_vec(v, "v");
+		FSX_REQUIRE(out, "null output");
+		FSX_REQUIRE(i < v->n, "element " + std::to_string(i) + " of a vector of " +
+		                          std::to_string(v->n));
+		double * h = ctx->host_scalars;
+		FSX_HIP(hipMemcpyAsync(h, (const char *)v->ptr() + i * v->elem(), v->elem(),
+		                       hipMemcpyDeviceToHost, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->host_syncs++;
+		*out = v->dt == FSX_F64 ? h[0] : (double)*(const float *)h;
+	});
+}
+
+int fsx_vec_set(fsx_ctx * ctx, fsx_vec * v, size_t i, double value) {
+	return guard([&] {
+		check_ctx(ctx);
+		check_vec(v, "v");
+		FSX_REQUIRE(i < v->n, "element " + std::to_string(i) + " of a vector of " +
+		                          std::to_string(v->n));
+		// staged in the context's pinned scalars: the copy has completed
+		// when the call returns
+		double * h = ctx->host_scalars;
+		if (v->dt == FSX_F64) h[0] = value;
+		else *(float *)h = (float)value;
+		FSX_HIP(hipMemcpyAsync((char *)v->ptr() + i * v->elem(), h, v->elem(),
+		                       hipMemcpyHostToDevice, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+	});
+}
+
 // ---- BLAS-1 --------------------------------------------------------------
 
 #define FSX_EW3(NAME, EXPR)                                                 \
@@ -828,15 +892,19 @@ int fsx_nrmp(fsx_ctx * ctx, const fsx_vec * x, int p, double * out) {
 		if (x->n > 0) {
 			const int g = red_grid(x->n);
 			double * part = ensure_partials(ctx, (size_t)g);
-			if (x->dt == FSX_F64)
+			double * slot = ctx->ws.scalars;
+			if (x->dt == FSX_F64) {
 				hipLaunchKernelGGL((powsum_kernel<double>), dim3(g), dim3(kRedThreads), 0,
 				                   ctx->stream, x->n, (const double *)x->ptr(), (double)p, part);
-			else
-				hipLaunchKernelGGL((powsum_kernel<float>), dim3(g), dim3(kRedThreads), 0,
-				                   ctx->stream, x->n, (const float *)x->ptr(), (double)p, part);
-			double * slot = ctx->ws.scalars;
-			hipLaunchKernelGGL((reduce_finalize_kernel<RED_DOT>), dim3(1), dim3(kRedThreads), 0,
-			                   ctx->stream, (const double *)part, (unsigned)g, slot);
+				hipLaunchKernelGGL((reduce_finalize_kernel<RED_DOT>), dim3(1), dim3(kRedThreads),
+				                   0, ctx->stream, (const double *)part, (unsigned)g, slot);
+			}
+			else {
+				hipLaunchKernelGGL(powsum_f32_kernel, dim3(g), dim3(kRedThreads), 0, ctx->stream,
+				                   x->n, (const float *)x->ptr(), (double)p, part);
+				hipLaunchKernelGGL(powsum_f32_finalize_kernel, dim3(1), dim3(kRedThreads), 0,
+				                   ctx->stream, (const double *)part, (unsigned)g, slot);
+			}
 			FSX_HIP(hipGetLastError());
 			ctx->launches += 2;
 			sum = fetch_scalar(ctx, slot);

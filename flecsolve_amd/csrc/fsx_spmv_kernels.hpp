@@ -38,7 +38,16 @@
 // that default order is faster than remapping blocks to give each XCD a
 // contiguous row range (all XCDs then stream neighbouring rows at once), and
 // no faster for the slice-offset kernel than grouping runs of 4-32
-// consecutive blocks on one XCD (so slices +-n rows apart share an L2).
+// consecutive blocks on one XCD (so slices +-n rows apart share an L2).  For
+// the slice-offset kernel at 256^3, numbering blocks XCD-major with one
+// contiguous run of slices each (every XCD sweeping one eighth of the rows)
+// gave the FP32 CG product 0.122 -> 0.119 ms but the FP64 product 0.228 ->
+// 0.245 ms and the FP64 CG product 0.232 -> 0.252 ms; not used.  Skipping
+// the value loads of a unit diagonal (the scaled FP32 matrix) through a
+// wave-uniform address select cost more issue time than the 8% of bytes it
+// saved (FP32 CG product 0.120 -> 0.127 ms, and the FP64 products slower
+// too); a branch per load split the merged scalar loads of offsets and
+// masks; not used.
 //
 // The kernels take an epilogue policy `Epi` that consumes each row's sum
 // (row() *adds* its dot-product terms to d: a thread may own several rows):

@@ -89,6 +89,8 @@ _SIGS = {
     "fsx_vec_device_ptr": (C.c_int, [_P, C.POINTER(_P)]),
     "fsx_vec_upload": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "fsx_vec_download": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "fsx_vec_get": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_double)]),
+    "fsx_vec_set": (C.c_int, [_P, _P, C.c_size_t, C.c_double]),
     "fsx_copy": (C.c_int, [_P, _P, _P]),
     "fsx_zero": (C.c_int, [_P, _P]),
     "fsx_set_scalar": (C.c_int, [_P, C.c_double, _P]),
@@ -314,6 +316,16 @@ class Vector:
         a = np.ascontiguousarray(a, dtype=self.np_dtype).ravel()
         check(lib().fsx_vec_upload(self.ctx.h, self.h, _ptr(a), a.size), "upload")
         return self
+
+    def __getitem__(self, i: int) -> float:
+        """One element (fsx_vec_get: vec::core::operator[] read)."""
+        out = C.c_double()
+        check(lib().fsx_vec_get(self.ctx.h, self.h, int(i), C.byref(out)), "fsx_vec_get")
+        return out.value
+
+    def __setitem__(self, i: int, value: float) -> None:
+        """One element (fsx_vec_set: vec::core::operator[] write)."""
+        check(lib().fsx_vec_set(self.ctx.h, self.h, int(i), float(value)), "fsx_vec_set")
 
     def get(self) -> np.ndarray:
         out = np.empty(self.n, dtype=self.np_dtype)

@@ -95,6 +95,12 @@ struct core : Config {
 	auto global_size() const { return ops::global_size(data); }
 	std::size_t local_size() const { return ops::local_size(data); }
 
+	// element access (reference core.hh:338-340): what Ops::retreive returns,
+	// a reference into host storage or, for device storage, a proxy that
+	// reads or writes the one element
+	decltype(auto) operator[](len_t i) { return ops::retreive(data, i); }
+	decltype(auto) operator[](len_t i) const { return ops::retreive(data, i); }
+
 	void set_random() { ops::set_random(data, std::random_device{}()); }
 	void set_random(unsigned seed) { ops::set_random(data, seed); }
 	void dump(std::string_view name) { ops::dump(name, data); }

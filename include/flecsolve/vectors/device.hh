@@ -219,6 +219,30 @@ struct device_ops {
 	}
 	static ready<len_t> global_size(const Data & x) { return {x.size()}; }
 	static len_t local_size(const Data & x) { return x.size(); }
+
+	// One device element (vectors/seq.hh:271-280 returns a reference into
+	// host storage): reading converts to scalar, assigning writes it; each
+	// access is one small synchronous copy, ordered after queued work.
+	struct element {
+		const Data * d;
+		len_t i;
+		operator scalar() const {
+			double v;
+			fsx_check(fsx_vec_get(d->ctx(), d->get(), i, &v), "element read");
+			return (scalar)v;
+		}
+		element & operator=(scalar a) {
+			fsx_check(fsx_vec_set(d->ctx(), d->get(), i, (double)a), "element write");
+			return *this;
+		}
+		element & operator=(const element & o) { return *this = (scalar)o; }
+		element & operator+=(scalar a) { return *this = (scalar)*this + a; }
+		element & operator-=(scalar a) { return *this = (scalar)*this - a; }
+		element & operator*=(scalar a) { return *this = (scalar)*this * a; }
+		element & operator/=(scalar a) { return *this = (scalar)*this / a; }
+	};
+	static element retreive(Data & x, len_t i) { return {&x, i}; }
+	static scalar retreive(const Data & x, len_t i) { return (scalar)element{&x, i}; }
 };
 
 template<class T = double>

@@ -112,6 +112,11 @@ int fsx_vec_id(const fsx_vec * v, uint64_t * id);
 int fsx_vec_device_ptr(const fsx_vec * v, void ** dptr);
 int fsx_vec_upload(fsx_ctx * ctx, fsx_vec * v, const void * host, size_t n);
 int fsx_vec_download(fsx_ctx * ctx, const fsx_vec * v, void * host, size_t n);
+/* One element, for vec::core::operator[] / Ops::retreive (vectors/core.hh:
+ * 338-340, vectors/seq.hh:271-280): read (converted to double) or write
+ * element i, ordered after the context's queued work; both synchronise. */
+int fsx_vec_get(fsx_ctx * ctx, const fsx_vec * v, size_t i, double * out);
+int fsx_vec_set(fsx_ctx * ctx, fsx_vec * v, size_t i, double value);
 
 /* ---- BLAS-1, mirroring the Ops policy (vectors/core.hh:59-310;
  *      semantics vectors/seq.hh:93-304; tasks operations/topo_tasks.hh) ---- */
@@ -156,7 +161,9 @@ int fsx_nrminf(fsx_ctx * ctx, const fsx_vec * x, double * out);
 /* lp_norm<p> of the FleCSI vector path (vectors/operations/topo_view.hh:
  * 243-257, topo_tasks.hh:216-226): p = 1 and 2 as fsx_nrm1 / fsx_nrm2;
  * p >= 3 gives (sum_i pow(x_i, p))^(1/p) -- with no absolute value, as
- * there (NaN when an odd p makes the sum negative). */
+ * there (NaN when an odd p makes the sum negative).  The sum is folded in
+ * the vector's real type, as lp_norm_local folds into `real`: double for
+ * FP64 vectors, float for FP32 ones (each term rounded to float). */
 int fsx_nrmp(fsx_ctx * ctx, const fsx_vec * x, int p, double * out);
 int fsx_min(fsx_ctx * ctx, const fsx_vec * x, double * out);
 int fsx_max(fsx_ctx * ctx, const fsx_vec * x, double * out);

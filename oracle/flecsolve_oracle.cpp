@@ -815,6 +815,15 @@ double or_dot(const double * x, const double * y, int64_t n) {
 	return scalar_prod(x, y, n);
 }
 // seq_ops::set_random (seq.hh:103-115): mt19937(seed), U[0,1) doubles
+// lp_norm_local for FP32 vectors (vectors/operations/topo_tasks.hh:216-226):
+// each term std::pow(u[dof], p) -- a float and an int, evaluated in double --
+// folded into a float accumulator in index order (the serial fold).
+float or_lp_powsum_f32(const float * x, int64_t n, int p) {
+	float acc = 0.f;
+	for (int64_t i = 0; i < n; ++i) acc += (float)std::pow((double)x[i], (double)p);
+	return acc;
+}
+
 void or_set_random(double * x, int64_t n, unsigned seed) {
 	std::mt19937 gen(seed);
 	std::uniform_real_distribution<double> dis(0., 1.);

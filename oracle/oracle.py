@@ -124,6 +124,8 @@ def lib():
         L.or_cg_colours.argtypes = [C.POINTER(_Csr), P, P, P, C.POINTER(Settings), C.c_int,
                                     C.POINTER(Info)]
         L.or_cg_colours.restype = C.c_int
+        L.or_lp_powsum_f32.argtypes = [P, i64, C.c_int]
+        L.or_lp_powsum_f32.restype = C.c_float
         L.or_mutant_bicgstab_relative_early_exit.argtypes = [C.c_int]
         L.or_mutant_bicgstab_relative_early_exit.restype = None
         _lib = L
@@ -361,6 +363,12 @@ def vmin(x):
 def vmax(x):
     x = _a(x)
     return lib().or_max(_p(x), x.size)
+
+
+def lp_powsum_f32(x, p):
+    """FP32 lp_norm_local: float terms folded in float, index order."""
+    x = np.ascontiguousarray(x, np.float32)
+    return float(lib().or_lp_powsum_f32(x.ctypes.data_as(C.c_void_p), x.size, int(p)))
 
 
 def set_random(n, seed):

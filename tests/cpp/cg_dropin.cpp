@@ -103,6 +103,13 @@ int main(int argc, char ** argv) {
 		std::printf("vector surface: lp_norm<3> %.17g, dump read back %g %g\n", l3, a, b);
 		fails += std::abs(l3 - std::cbrt(7.0)) > 1e-15 || a != -1.0 || b != 2.0;
 		std::remove((pre + "-0").c_str());
+		// element access, operator[] / Ops::retreive (core.hh:338-340)
+		w[0] = 5.0;
+		w[1] += 0.5;
+		const auto & cw = w;
+		const double e0 = w[0], e1 = cw[1];
+		std::printf("vector surface: w[0] = %g, w[1] = %g after w[0] = 5, w[1] += 0.5\n", e0, e1);
+		fails += e0 != 5.0 || e1 != 2.5;
 	}
 	{
 		op::core<mat::device_csr> A(ctx, Ah);

@@ -133,12 +133,23 @@ def test_lp_norm_general_p(ctx, n):
     xd = ctx.vec(x)
     assert fsx.nrmp(xd, 1) == fsx.nrm1(xd)
     assert fsx.nrmp(xd, 2) == fsx.nrm2(xd)
+    x32 = x.astype(np.float32)
     for p in (3, 4, 7):
         want = np.sum(x ** p) ** (1.0 / p)
         assert abs(fsx.nrmp(xd, p) - want) <= 1e-13 * want, p
-        assert abs(fsx.nrmp(ctx.vec(x.astype(np.float32)), p)
-                   - np.sum(x.astype(np.float32).astype(np.float64) ** p) ** (1.0 / p)) \
-            <= 1e-13 * want
+        # FP32: float terms folded in float (the oracle's serial fold); two
+        # float folds of positive terms in different orders differ by at most
+        # 2 (n - 1) 2^-24 of the sum, 1/p of that after the root
+        s32 = O.lp_powsum_f32(x32, p)
+        got = fsx.nrmp(ctx.vec(x32), p)
+        assert abs(got - s32 ** (1.0 / p)) <= (2 * max(n - 1, 1) * 2.0 ** -24 / p + 1e-15) \
+            * s32 ** (1.0 / p), p
+    # float fold pinned exactly: 16 terms (1 + 2^-10)^3 rounded to float sum
+    # without rounding in any order; the double fold gives other bits
+    e = np.full(16, 1 + 2.0 ** -10, np.float32)
+    t = 1 + 3 * 2.0 ** -10 + 3 * 2.0 ** -20
+    assert fsx.nrmp(ctx.vec(e), 3) == (16 * t) ** (1.0 / 3)
+    assert (16 * t) ** (1.0 / 3) != (16 * (1 + 2.0 ** -10) ** 3) ** (1.0 / 3)
     c = ctx.vec(np.full(n, 0.75))
     assert abs(fsx.nrmp(c, 5) - 0.75 * n ** 0.2) <= 1e-13
     assert abs(fsx.nrmp(ctx.vec(np.array([-1.0, 2.0])), 3) - 7 ** (1 / 3)) <= 1e-15
@@ -222,3 +233,26 @@ def test_cg_update(ctx, jac):
         assert abs(rz.value - O.dot(re, ze)) <= 1e-12 * np.abs(re * ze).sum()
     else:
         assert rz.value == rr.value
+
+
+@pytest.mark.parametrize("dtype", [fsx.F64, fsx.F32])
+def test_element_access(ctx, dtype):
+    """vec::core::operator[] / Ops::retreive (vectors/core.hh:338-340,
+    vectors/seq.hh:271-280) on device vectors: reads see queued work,
+    writes land in place, out-of-range indices are errors."""
+    n = 1000
+    x = np.arange(n, dtype=np.float64) * 0.5
+    v = ctx.vec(x if dtype == fsx.F64 else x.astype(np.float32))
+    fsx.check(fsx.lib().fsx_scale(ctx.h, 2.0, v.h))   # queued, then read
+    assert v[0] == 0.0 and v[7] == 7.0 and v[n - 1] == n - 1.0
+    v[3] = -1.25
+    v[n - 1] = 0.1
+    want = x * 2
+    want[3] = -1.25
+    want[n - 1] = 0.1 if dtype == fsx.F64 else float(np.float32(0.1))
+    np.testing.assert_array_equal(v.get(), want.astype(v.np_dtype))
+    assert v[n - 1] == want[n - 1]
+    with pytest.raises(fsx.FsxError):
+        v[n]
+    with pytest.raises(fsx.FsxError):
+        v[n] = 1.0

@@ -75,3 +75,20 @@ def test_axpy_rounding_matches_expression(alpha):
     x, y = rng.standard_normal(1000), rng.standard_normal(1000)
     # no fused multiply-add: alpha*x rounded, then + y rounded
     np.testing.assert_array_equal(O.axpy(alpha, x, y), alpha * x + y)
+
+
+def test_lp_powsum_f32_folds_in_float():
+    """FP32 lp_norm_local (topo_tasks.hh:216-226) folds std::pow(float, int)
+    terms -- evaluated in double -- into a float.  x = 1 + 2^-10: x^3 =
+    1 + 3 2^-10 + 3 2^-20 + 2^-30 in double, 1 + 3 2^-10 + 3 2^-20 once
+    rounded to float; 16 of them sum exactly in float (21 significant bits),
+    so the fold is 16 (1 + 3 2^-10 + 3 2^-20), not the double sum's
+    16 (1 + 3 2^-10 + 3 2^-20 + 2^-30).  Random input: the fold is the
+    serial float sum of float-rounded terms, in index order."""
+    x = np.full(16, 1 + 2.0 ** -10, np.float32)
+    t = 1 + 3 * 2.0 ** -10 + 3 * 2.0 ** -20
+    assert O.lp_powsum_f32(x, 3) == 16 * t
+    assert 16 * t != 16 * (1 + 2.0 ** -10) ** 3
+    r = np.random.default_rng(5).random(1000).astype(np.float32) + np.float32(0.5)
+    terms = np.power(r.astype(np.float64), 3).astype(np.float32)
+    assert O.lp_powsum_f32(r, 3) == float(np.cumsum(terms, dtype=np.float32)[-1])
