@@ -244,6 +244,119 @@ __global__ __launch_bounds__(256) void cg_xfinal_kernel(size_t n, T * __restrict
 	}
 }
 
+// ---- single-reduction iteration (unpreconditioned inner solves) -----------
+//
+// The refinement's inner CG on the symmetrically scaled FP32 matrix has no
+// preconditioner (z = r) and no reference counterpart, so it may use the
+// single-reduction form of the same recurrences: with r_k . A p_k = p_k . A p_k
+// (A-conjugacy), |r_{k+1}|^2 = alpha_k^2 |A p_k|^2 - |r_k|^2.  K1 also sums
+// w.w, so after its one finalize alpha_k, that estimate and beta_k are known
+// and the r, x and p updates fuse into one pass (28 instead of 32 bytes of
+// vectors per row in FP32, one finalize launch fewer).  The pass also sums
+// the recursive |r_{k+1}|^2 itself, which the next finalize uses for
+// alpha_{k+1} and for the tolerance test of r_{k+1}, so the stopping test is
+// the standard one on the recursive residual, taken one kernel later (one
+// product after convergence is discarded).
+
+// K1s: w = A p and the partials of w.p and w.w
+template<class ACC, class T>
+struct epi_cg_spmv2 {
+	static constexpr int NV = 2;
+	static constexpr bool reduces = true;
+	T * w;
+	double * partials;
+	const cg_state * st;
+	__device__ bool skip() const { return st->done != 0; }
+	template<class XI>
+	__device__ void row(int64_t i, ACC s, double (&d)[NV], XI && pi) const {
+		const double pv = (double)pi();
+		const T wi = (T)s;
+		w[i] = wi;
+		d[0] += (double)wi * pv;          // w.dot(p)
+		d[1] += (double)wi * (double)wi;  // w.dot(w)
+	}
+	__device__ void block_end(const double (&d)[NV], unsigned blk, unsigned) const {
+		store_block_partials<NV>(d, partials, blk);
+	}
+};
+
+// F1s: rho_k = |r_k|^2 (the previous fused pass's partials; the setup's rho
+// at k = 0) with r_k's tolerance test and maxiter, then alpha_k and beta_k.
+__global__ __launch_bounds__(kFinThreads) void sr_finalize_kernel(
+	const double * part1, unsigned nblk1, const double * part2, unsigned nblk2,
+	cg_state * st, double * hist) {
+	if (st->done) return;
+	const int it = st->iter; // completed updates
+	double t[2];
+	sum_partials<2>(part1, nblk1, t);
+	double u[1] = {0.0};
+	if (it > 0) sum_partials<1>(part2, nblk2, u);
+	if (threadIdx.x != 0) return;
+	const double rho = it > 0 ? u[0] : st->rho1;
+	if (it > 0) {
+		const double res = sqrt(rho);
+		st->res = res;
+		if (hist) hist[it - 1] = res;
+		if (res < st->tol) {
+			st->iters = it;
+			st->status = FSX_CONVERGED_RTOL;
+			st->done = 1;
+			return;
+		}
+		if (it >= st->maxiter) { // loop exhausted
+			st->done = 1;
+			return;
+		}
+	}
+	const double pw = t[0];
+	st->pw = pw;
+	if (pw <= 0.0) st->ncurv++;
+	const double alpha = rho / pw;
+	const double rho_next = alpha * alpha * t[1] - rho;
+	st->alpha = alpha;
+	st->rho0 = rho;
+	st->rho1 = rho_next;
+	st->beta = rho_next / rho;
+	st->iter = it + 1;
+}
+
+// K23: r -= alpha w; x += alpha p; p = beta p + r; partials of |r|^2 (the
+// expressions of cg_update_kernel and cg_xp_update_kernel)
+template<class T>
+__global__ __launch_bounds__(kUpdThreads) void cg_sr_update_kernel(
+	size_t n, T * __restrict__ x, T * __restrict__ r, T * __restrict__ p,
+	const T * __restrict__ w, double * partials, const cg_state * st) {
+	if (st->done) return;
+	const double alpha = st->alpha, beta = st->beta;
+	const unsigned blk = blockIdx.x;
+	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
+	double rr = 0.0;
+#pragma unroll
+	for (int j = 0; j < kUpdPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			if constexpr (std::is_same_v<T, double>) {
+				const double pi = p[i];
+				const double ri = (-alpha) * w[i] + r[i];
+				r[i] = ri;
+				rr += ri * ri;
+				x[i] = alpha * pi + x[i];
+				p[i] = beta * pi + ri;
+			}
+			else {
+				const double pi = (double)p[i];
+				const T ri = (T)((-alpha) * (double)w[i] + (double)r[i]);
+				r[i] = ri;
+				rr += (double)ri * (double)ri;
+				x[i] = (T)(alpha * pi + (double)x[i]);
+				p[i] = (T)(beta * pi + (double)ri);
+			}
+		}
+	}
+	double d[1] = {rr};
+	store_block_partials<1>(d, partials, blk);
+}
+
 // ---- residual + norm (FP64 outer refinement) --------------------------------
 
 template<class ACC, class T>
@@ -412,7 +525,8 @@ struct cg_bufs {
 	void * p;
 	void * w;
 	const void * dinv;
-	double * part; // per-block partials of the reducing kernels
+	double * part;  // per-block partials of the reducing kernels
+	double * part2; // single-reduction iteration: the fused update's partials
 	size_t n;
 };
 
@@ -438,15 +552,22 @@ struct cg_work {
 	// false: kernels launched directly with one chunk of iterations in
 	// flight -- no graphs, events or allocations
 	bool graphs;
+	// the single-reduction iteration (unpreconditioned solves of the
+	// refinement only): K1 with w.p and w.w, one finalize, one fused update
+	bool single_reduction = false;
 };
 
 // Partials a CG solve on A over n rows needs: the SpMV+dot (1 per block),
 // the update (2 per block) and the setup reductions.
-static size_t cg_partials_need(const fsx_csr * A, size_t n) {
+static size_t cg_partials_need(const fsx_csr * A, size_t n, bool sr = false) {
 	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
-	return std::max<size_t>(std::max<size_t>(spmv_partials_size(A, 1),
-	                                         2 * ((n + per - 1) / per)),
-	                        (size_t)red_grid(n));
+	const size_t upd = (n + per - 1) / per;
+	size_t need = std::max<size_t>(std::max<size_t>(spmv_partials_size(A, 1), 2 * upd),
+	                               (size_t)red_grid(n));
+	// single reduction: the product's two sums per block, then the fused
+	// update's one per block in a region of its own
+	if (sr) need = std::max(need, spmv_partials_size(A, 2) + upd);
+	return need;
 }
 
 // The ranges a captured CG chunk touches (vectors, partials, state, every
@@ -462,7 +583,7 @@ static graph_ledger::entry cg_ledger_entry(const cg_bufs & B, const cg_state * s
 	           {B.p, vb},
 	           {B.w, vb},
 	           {B.dinv, B.dinv ? vb : 0},
-	           {B.part, cg_partials_need(B.A, B.n) * sizeof(double)},
+	           {B.part, cg_partials_need(B.A, B.n, B.part2 != nullptr) * sizeof(double)},
 	           {st, sizeof(cg_state)}};
 	graph_ledger::add_layout(e, B.A);
 	e.hist = hist;
@@ -515,6 +636,26 @@ static void enqueue_iteration(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 	if constexpr (!DIAG)
 		hipLaunchKernelGGL((cg_xp_update_kernel<T>), dim3(g3), dim3(256), 0, s, B.n,
 		                   (T *)B.x, p, (const T *)B.z, (const cg_state *)st);
+	FSX_HIP(hipGetLastError());
+}
+
+// One single-reduction iteration (K1s, F1s, K23); unpreconditioned only.
+template<class T, class V, class I>
+static void enqueue_iteration_sr(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
+                                 double * hist, int) {
+	hipStream_t s = ctx->stream;
+	const fsx_csr * A = B.A;
+	using ACC = cg_acc_t<T, V>;
+	epi_cg_spmv2<ACC, T> e1{(T *)B.w, B.part, st};
+	spmv_launch_typed<decltype(e1), ACC, V, I, T>(s, A, (const T *)B.p, e1);
+	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+	const unsigned g2 = (unsigned)((B.n + per - 1) / per);
+	hipLaunchKernelGGL(sr_finalize_kernel, dim3(1), dim3(kFinThreads), 0, s,
+	                   (const double *)B.part, spmv_blocks(A), (const double *)B.part2, g2, st,
+	                   hist);
+	hipLaunchKernelGGL((cg_sr_update_kernel<T>), dim3(g2), dim3(kUpdThreads), 0, s, B.n,
+	                   (T *)B.x, (T *)B.r, (T *)B.p, (const T *)B.w, B.part2,
+	                   (const cg_state *)st);
 	FSX_HIP(hipGetLastError());
 }
 
@@ -579,13 +720,14 @@ static fsx_vec * pooled(fsx_ctx * ctx, size_t n, fsx_dtype dt, int slot) {
 // slot0 + 3 (r, z, p, w), the context's partials grown to what the solve
 // needs, the cache's state buffer and a history buffer for maxiter entries.
 static cg_work pooled_work(fsx_ctx * ctx, const fsx_csr * A, size_t n, fsx_dtype dt,
-                           bool jac, int slot0, int maxiter) {
+                           bool jac, int slot0, int maxiter, bool sr = false) {
 	cg_work wk{};
 	wk.r = pooled(ctx, n, dt, slot0 + 0)->ptr();
 	wk.z = jac ? pooled(ctx, n, dt, slot0 + 1)->ptr() : nullptr;
 	wk.p = pooled(ctx, n, dt, slot0 + 2)->ptr();
 	wk.w = pooled(ctx, n, dt, slot0 + 3)->ptr();
-	wk.part = ensure_partials(ctx, cg_partials_need(A, n));
+	wk.single_reduction = sr && !jac;
+	wk.part = ensure_partials(ctx, cg_partials_need(A, n, wk.single_reduction));
 	wk.slots = ctx->ws.scalars + kCgSetupSlot;
 	wk.st = (cg_state *)cache_of(ctx).statebuf->p;
 	wk.hist = maxiter > 0 ? history_buffer(ctx, maxiter) : nullptr;
@@ -620,10 +762,13 @@ static void run_cg_loop_direct(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
 // Captured, pipelined loop: chunks of iterations replayed from cached HIP
 // graphs (or launched directly with FSX_CG_GRAPH=0), up to two in flight,
 // the host checking the older one's state.
-template<class T, class V, class I>
+template<class T, class V, class I, bool SR = false>
 static void run_cg_loop_pipelined(fsx_ctx * ctx, const cg_bufs & B, cg_state * st,
                                   cg_state * pinned, double * hist,
                                   const cg_state & st0) {
+	// the iteration: the reference's sequence, or the single-reduction form
+	constexpr auto iterate = SR ? &enqueue_iteration_sr<T, V, I> : &enqueue_iteration<T, V, I>;
+	constexpr int kLaunches = SR ? 3 : 5;
 	const int K = chunk_size();
 	// Two chunk lengths: short chunks while the solve's length is unknown or
 	// nearly reached, long ones in between.  Iterations past convergence are
@@ -636,7 +781,7 @@ static void run_cg_loop_pipelined(fsx_ctx * ctx, const cg_bufs & B, cg_state * s
 	if (use_graphs()) {
 		auto & cache = cache_of(ctx);
 		auto key_for = [&](int k) {
-			std::vector<const void *> key{(const void *)&enqueue_iteration<T, V, I>,
+			std::vector<const void *> key{(const void *)iterate,
 			                              B.x, B.r, B.z, B.p, B.w, B.dinv,
 			                              (const void *)B.part, (const void *)st,
 			                              (const void *)hist, (const void *)(intptr_t)B.n,
@@ -662,7 +807,7 @@ static void run_cg_loop_pipelined(fsx_ctx * ctx, const cg_bufs & B, cg_state * s
 				FSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
 				try {
 					for (int k = 0; k < lens[j]; ++k)
-						enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
+						iterate(ctx, B, st, hist, k);
 				}
 				catch (...) {
 					hipGraph_t junk = nullptr;
@@ -696,8 +841,8 @@ static void run_cg_loop_pipelined(fsx_ctx * ctx, const cg_bufs & B, cg_state * s
 			FSX_HIP(hipGraphLaunch(gexec[j], ctx->stream));
 		}
 		else
-			for (int k = 0; k < len; ++k) enqueue_iteration<T, V, I>(ctx, B, st, hist, k);
-		ctx->launches += 5 * len;
+			for (int k = 0; k < len; ++k) iterate(ctx, B, st, hist, k);
+		ctx->launches += kLaunches * len;
 		FSX_HIP(hipMemcpyAsync(&flags[slot], st, sizeof(cg_state),
 		                       hipMemcpyDeviceToHost, ctx->stream));
 		FSX_HIP(hipEventRecord(ev[slot], ctx->stream));
@@ -797,6 +942,8 @@ static void run_cg_loop(fsx_ctx * ctx, const cg_bufs & B, const cg_work & wk,
 	FSX_HIP(hipMemcpyAsync(st, pinned, sizeof(cg_state), hipMemcpyHostToDevice,
 	                       ctx->stream));
 	if (diag) run_cg_loop_diag<T, V, I>(ctx, B, st, pinned, wk.hist, xv, diag, user);
+	else if (wk.graphs && B.part2)
+		run_cg_loop_pipelined<T, V, I, true>(ctx, B, st, pinned, wk.hist, st_host);
 	else if (wk.graphs) run_cg_loop_pipelined<T, V, I>(ctx, B, st, pinned, wk.hist, st_host);
 	else run_cg_loop_direct<T, V, I>(ctx, B, st, pinned, wk.hist);
 	if (B.n > 0 && !diag) {
@@ -936,7 +1083,9 @@ static void cg_solve_impl(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv
 		FSX_HIP(hipMemcpyAsync(wk.p, zz, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
 	double xf2 = 0;
 	if (s.maxiter > 0) {
-		cg_bufs B{A, x->ptr(), r, zz, wk.p, wk.w, dinv ? dinv->ptr() : nullptr, wk.part, n};
+		const bool sr = wk.single_reduction && !dinv && !diag;
+		cg_bufs B{A, x->ptr(), r, zz, wk.p, wk.w, dinv ? dinv->ptr() : nullptr, wk.part,
+		          sr ? wk.part + spmv_partials_size(A, 2) : nullptr, n};
 		by_matrix_type(A, [&](auto * vtag, auto * itag) {
 			using V = std::remove_pointer_t<decltype(vtag)>;
 			using I = std::remove_pointer_t<decltype(itag)>;
@@ -1877,11 +2026,15 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 		};
 		// partials for the outer residual and the inner solves, grown once
 		// before any buffer address is taken
-		ensure_partials(ctx, std::max(spmv_partials_size(A64, 1), cg_partials_need(A32, n)));
+		ensure_partials(ctx, std::max(spmv_partials_size(A64, 1),
+		                              cg_partials_need(A32, n, jacobi != 1)));
 		fsx_settings is = *inner;
 		is.use_zero_guess = 1;
 		// inner work: pooled slots 14-17
-		const cg_work iwk = pooled_work(ctx, A32, n, FSX_F32, jacobi == 1, 14, is.maxiter);
+		// unpreconditioned inner solves (jacobi 0, 2) take the single-reduction
+		// iteration
+		const cg_work iwk =
+			pooled_work(ctx, A32, n, FSX_F32, jacobi == 1, 14, is.maxiter, jacobi != 1);
 		double rnorm = residual();
 		const unsigned g = (unsigned)((n + 255) / 256);
 		int outer = 0;
