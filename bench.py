@@ -216,7 +216,7 @@ def bench_diffusion(a, fsx, P):
     bd = ctx.vec(b)
     x = ctx.zeros(N)
     log(f"uploaded and layouts built in {time.perf_counter()-t0:.1f} s; "
-        f"SpMV plan {A64.plan()}")
+        f"SpMV plan {A64.plan()}, FP32 copy {A32.plan()}")
 
     # ---- kernel roofline (HIP events on the library stream) --------------
     # The timed region's dominant kernel is the inner FP32 CG's K1 (w = A p
@@ -252,10 +252,18 @@ def bench_diffusion(a, fsx, P):
     pmc = ROOT / "profiles" / "cg_k1_256cube_pmc.json"
     if n == 256 and abs(a.sigma - 1.0) < 1e-12 and a.ir_jacobi == 2 and pmc.exists():
         traffic = json.loads(pmc.read_text())
-    plan = A64.plan()
-    kname = {0: "spmv_stream_kernel (LDS-staged CSR)", 1: "spmv_vector_kernel",
-             2: "spmv_sliced_kernel (sliced rows)",
-             3: "spmv_sdia_kernel (slice-offset layout)"}[plan[0]]
+    knames = {0: "spmv_stream_kernel (LDS-staged CSR)", 1: "spmv_vector_kernel",
+              2: "spmv_sliced_kernel (sliced rows)",
+              3: "spmv_sdia_kernel (slice-offset layout)",
+              4: "spmv_stencil_kernel (symmetric stencil layout)"}
+    bases = {3: "compulsory bytes of the slice-offset layout (values, offsets, lane masks, "
+                "slice pointers; input read once, output written once)",
+             4: "compulsory bytes of the symmetric stencil layout (values at offsets >= 0 and "
+                "lane masks; entries below the diagonal re-read from stored values, counted "
+                "once; input read once, output written once)"}
+    csr_basis = "CSR bytes (values, int32 columns, row pointers; input read once, output written once)"
+    plan32, plan64 = A32.plan()[0], A64.plan()[0]
+    kname, basis = knames[plan32], bases.get(plan32, csr_basis)
     roofline = {
         "bound": "hbm",
         "kernel": f"{kname} with the fused CG epilogue (K1: w = A p, partials of w.p), "
@@ -266,14 +274,15 @@ def bench_diffusion(a, fsx, P):
         "frac": round(gbs(k1_bytes, k1_ms) / HBM_PEAK_GBS, 4),
         "traffic": traffic.get("k1_fp32", {}).get("traffic_bytes"),
         "algorithmic_bytes": k1_bytes,
-        "bytes_basis": "compulsory bytes of the slice-offset layout (values, offsets, "
-                       "lane masks, slice pointers; p read once, w written once)",
+        "bytes_basis": basis,
         "avg_ms": round(k1_ms, 5),
         # the CSR-with-int32-columns figure per product over the same time:
         # a rate for comparison with CSR kernels, not a roofline fraction
         # (this layout stores no column per entry)
         "csr_equiv_bytes": k1_csr,
         "csr_equiv_gbs": round(gbs(k1_csr, k1_ms), 1),
+        "fp64_kernel": knames[plan64],
+        "fp64_bytes_basis": bases.get(plan64, csr_basis),
         "fp64_spmv": {"avg_ms": round(spmv_ms, 5), "algorithmic_bytes": b64,
                       "achieved": round(gbs(b64, spmv_ms), 1),
                       "frac": round(gbs(b64, spmv_ms) / HBM_PEAK_GBS, 4),

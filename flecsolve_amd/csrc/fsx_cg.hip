@@ -456,6 +456,8 @@ static void add_layout(entry & e, const fsx_csr * A) {
 	add(X.dg_off);
 	add(X.dg_mask);
 	add(A->dg_vals);
+	add(X.st_mask);
+	add(A->st_vals);
 }
 } // namespace graph_ledger
 
@@ -504,7 +506,8 @@ static void append_layout(std::vector<const void *> & key, const fsx_csr * A) {
 	const csr_index & X = *A->idx;
 	key.insert(key.end(), {p(X.rowptr), p(X.colind), p(A->vals), p(X.sl_ptr), p(X.sl_col),
 	                       p(A->sl_vals), p(X.dg_ptr), p(X.dg_off), p(X.dg_mask),
-	                       p(A->dg_vals), (const void *)(intptr_t)A->nrows,
+	                       p(A->dg_vals), p(X.st_mask), p(A->st_vals),
+	                       (const void *)(intptr_t)A->nrows,
 	                       (const void *)(intptr_t)A->nnz,
 	                       (const void *)(intptr_t)A->plan_kind,
 	                       (const void *)(intptr_t)A->plan_lanes});
@@ -1531,6 +1534,9 @@ struct gather_dinv {
 	const double * x;
 	__device__ __forceinline__ void init() {}
 	__device__ __forceinline__ double operator()(int64_t c) const { return 0.0 + dinv[c] * x[c]; }
+	__device__ __forceinline__ double at32(uint32_t c) const {
+		return 0.0 + ld32(dinv, c) * ld32(x, c);
+	}
 };
 
 template<class ACC, class T, bool LEFTJ>

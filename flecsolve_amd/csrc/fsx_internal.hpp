@@ -99,6 +99,7 @@ struct dbuf {
 // scalars.
 constexpr int kRedBlocks = 2048;   // maximum blocks of a reduction grid
 constexpr int kScalarSlots = 64;   // device scalar slots per context
+constexpr int kStencilMax = 8;     // offsets of the symmetric stencil layout
 
 struct reduce_ws {
 	double * partials = nullptr;   // grown on demand (ensure_partials)
@@ -162,6 +163,20 @@ struct csr_index {
 	// Index bytes per entry: 12/64 instead of 4; rows keep CSR order.
 	std::unique_ptr<fsx::dbuf> dg_ptr, dg_off, dg_mask;
 	int64_t dg_nnz = 0;
+	// Symmetric stencil layout (built next to the slice-offset one when every
+	// row's offsets come from one ascending list of at most kStencilMax
+	// offsets and the structure is symmetric): st_off lists the offsets; a
+	// 64-bit lane mask per (slice, offset) at st_mask[s * st_w + j]; values
+	// only for the st_wu offsets >= 0, entry-major per slice at
+	// (s * st_wu + st_ju[j]) * 64 + lane.  The entry of row r at offset
+	// o < 0 is read from row r + o's stored entry at -o (st_ju[j] names that
+	// column), which holds the same value when the matrix is symmetric.
+	// st_state: -1 not analysed, 0 not applicable, 1 built.
+	int st_state = -1;
+	int st_w = 0, st_wu = 0;
+	int32_t st_off[fsx::kStencilMax] = {};
+	int32_t st_ju[fsx::kStencilMax] = {};
+	std::unique_ptr<fsx::dbuf> st_mask;
 	// Lazily built transpose structure for A^T x (adjoint).
 	std::unique_ptr<fsx::dbuf> t_rowptr, t_colind, t_perm;
 	// every row stores its diagonal entry: -1 not yet checked, 0 no, 1 yes;
@@ -179,8 +194,12 @@ struct fsx_csr {
 	std::unique_ptr<fsx::dbuf> vals;
 	std::unique_ptr<fsx::dbuf> sl_vals; // values in the sliced-row layout
 	std::unique_ptr<fsx::dbuf> dg_vals; // values in the slice-offset layout
+	// values in the symmetric stencil layout; st_sym: the CSR values are
+	// bitwise symmetric (1), not (0), or not yet checked (-1)
+	std::unique_ptr<fsx::dbuf> st_vals;
+	int st_sym = -1;
 	// SpMV row mapping: 0 LDS-staged row blocks, 1 vector per row, 2 sliced
-	// rows, 3 slice offsets
+	// rows, 3 slice offsets, 4 symmetric stencil
 	int plan_kind = 0;
 	int plan_lanes = 0;
 	int64_t max_row = 0;

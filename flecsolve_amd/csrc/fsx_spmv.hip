@@ -359,9 +359,223 @@ bool ensure_sdia(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
 	return true;
 }
 
+// ---- symmetric stencil layout ---------------------------------------------
+
+// Host analysis (once per pattern): the ascending union of all rows' column
+// offsets, at most kStencilMax and closed under negation; every entry's
+// mirror present (row r has column r + o exactly when row r + o has column
+// r); the stored slots (offsets >= 0, whole slices) within 25% of the
+// entries they hold (+ one slice).  Builds the lane masks per (slice, offset).
+template<class I>
+static bool stencil_analyze(fsx_ctx * ctx, int64_t nrows, const I * rp, const I * ci,
+                            csr_index & X) {
+	std::vector<int64_t> offs;
+	for (int64_t r = 0; r < nrows; ++r)
+		for (I e = rp[r]; e < rp[r + 1]; ++e) {
+			// columns strictly ascending: the layout sums in offset order
+			if (e > rp[r] && ci[e] <= ci[e - 1]) return false;
+			const int64_t o = (int64_t)ci[e] - r;
+			if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
+				if ((int)offs.size() == kStencilMax) return false;
+				offs.push_back(o);
+			}
+		}
+	if (offs.empty()) return false;
+	std::sort(offs.begin(), offs.end());
+	const int w = (int)offs.size();
+	auto find = [&](int64_t o) {
+		return (int)(std::lower_bound(offs.begin(), offs.end(), o) - offs.begin());
+	};
+	int jm[kStencilMax];
+	for (int j = 0; j < w; ++j) {
+		jm[j] = find(-offs[j]);
+		if (jm[j] == w || offs[jm[j]] != -offs[j]) return false;
+	}
+	const int nneg = find(0);
+	const int wu = w - nneg;
+	std::vector<uint8_t> bits((size_t)nrows, 0);
+	int64_t upper = 0;
+	for (int64_t r = 0; r < nrows; ++r)
+		for (I e = rp[r]; e < rp[r + 1]; ++e) {
+			const int64_t o = (int64_t)ci[e] - r;
+			bits[(size_t)r] |= (uint8_t)(1u << find(o));
+			upper += o >= 0;
+		}
+	for (int64_t r = 0; r < nrows; ++r)
+		for (int j = 0; j < w; ++j) {
+			if (!((bits[(size_t)r] >> j) & 1)) continue;
+			const int64_t rr = r + offs[j];
+			if (rr < 0 || rr >= nrows || !((bits[(size_t)rr] >> jm[j]) & 1)) return false;
+		}
+	const int64_t ns = (nrows + kSliceRows - 1) / kSliceRows;
+	if (ns * kSliceRows * wu > upper + upper / 4 + (int64_t)kSliceRows * wu) return false;
+	std::vector<uint64_t> mask((size_t)(ns * w + kStencilMax), 0);
+	for (int64_t r = 0; r < nrows; ++r)
+		for (int j = 0; j < w; ++j)
+			if ((bits[(size_t)r] >> j) & 1)
+				mask[(size_t)((r / kSliceRows) * w + j)] |= 1ull << (r % kSliceRows);
+	X.st_w = w;
+	X.st_wu = wu;
+	for (int j = 0; j < kStencilMax; ++j) {
+		X.st_off[j] = j < w ? (int32_t)offs[j] : 0;
+		X.st_ju[j] = j >= w ? 0 : offs[j] >= 0 ? j - nneg : jm[j] - nneg;
+	}
+	X.st_mask = std::make_unique<dbuf>(mask.size() * sizeof(uint64_t));
+	FSX_HIP(hipMemcpyAsync(X.st_mask->p, mask.data(), mask.size() * sizeof(uint64_t),
+	                       hipMemcpyHostToDevice, ctx->stream));
+	FSX_HIP(hipStreamSynchronize(ctx->stream)); // the host mask goes out of scope
+	return true;
+}
+
+// Values of the offsets >= 0 of row r (thread r of the padded slice range)
+// into the layout: zero the row's slots, then place each entry.  Rows and
+// offsets are both ascending, so one forward scan finds them.
+template<class I, class V>
+__global__ void stencil_fill_kernel(int64_t nrows, int64_t nslots, const I * __restrict__ rp,
+                                    const I * __restrict__ ci, const V * __restrict__ v,
+                                    stencil_desc sd, V * __restrict__ st_val) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nslots) return;
+	const int64_t s = r / kSliceRows;
+	const int lane = (int)(r % kSliceRows);
+	for (int u = 0; u < sd.wu; ++u) st_val[(s * sd.wu + u) * kSliceRows + lane] = V(0);
+	if (r >= nrows) return;
+	int j = 0;
+	for (I e = rp[r]; e < rp[r + 1]; ++e) {
+		const int64_t o = (int64_t)ci[e] - r;
+		while (j < sd.w && sd.off[j] != o) ++j;
+		if (j == sd.w) return; // not this layout's structure (cannot happen)
+		if (o >= 0) st_val[(s * sd.wu + sd.ju[j]) * kSliceRows + lane] = v[e];
+		++j;
+	}
+}
+
+__device__ __forceinline__ uint64_t value_bits(double a) {
+	return (uint64_t)__double_as_longlong(a);
+}
+__device__ __forceinline__ uint64_t value_bits(float a) {
+	return (uint64_t)(uint32_t)__float_as_int(a);
+}
+
+// bad[0] = 1 when an entry at an offset < 0 differs in any bit from its
+// mirror's stored value
+template<class I, class V>
+__global__ void stencil_sym_check_kernel(int64_t nrows, const I * __restrict__ rp,
+                                         const I * __restrict__ ci, const V * __restrict__ v,
+                                         stencil_desc sd, const V * __restrict__ st_val,
+                                         int * __restrict__ bad) {
+	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (r >= nrows) return;
+	int j = 0;
+	for (I e = rp[r]; e < rp[r + 1]; ++e) {
+		const int64_t o = (int64_t)ci[e] - r;
+		if (o >= 0) break;
+		while (j < sd.w && sd.off[j] != o) ++j;
+		if (j == sd.w) {
+			bad[0] = 1;
+			return;
+		}
+		const int64_t rr = r + o;
+		const V m = st_val[((rr / kSliceRows) * sd.wu + sd.ju[j]) * kSliceRows + rr % kSliceRows];
+		if (value_bits(m) != value_bits(v[e])) bad[0] = 1;
+		++j;
+	}
+}
+
+// Fill A's stencil values from its CSR values and check their symmetry
+// (one host synchronisation); sets A->st_sym.
+template<class I, class V>
+static void stencil_fill_t(fsx_ctx * ctx, fsx_csr * A) {
+	const int64_t nslots = ((A->nrows + kSliceRows - 1) / kSliceRows) * kSliceRows;
+	const stencil_desc sd = make_stencil_desc(*A->idx);
+	dbuf flag(sizeof(int));
+	FSX_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+	const unsigned g = (unsigned)((nslots + 255) / 256);
+	hipLaunchKernelGGL((stencil_fill_kernel<I, V>), dim3(g), dim3(256), 0, ctx->stream,
+	                   A->nrows, nslots, (const I *)A->rowptr(), (const I *)A->colind(),
+	                   (const V *)A->vals->p, sd, (V *)A->st_vals->p);
+	hipLaunchKernelGGL((stencil_sym_check_kernel<I, V>), dim3(g), dim3(256), 0, ctx->stream,
+	                   A->nrows, (const I *)A->rowptr(), (const I *)A->colind(),
+	                   (const V *)A->vals->p, sd, (const V *)A->st_vals->p, (int *)flag.p);
+	FSX_HIP(hipGetLastError());
+	ctx->launches += 2;
+	int bad = 0;
+	FSX_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	FSX_HIP(hipStreamSynchronize(ctx->stream));
+	ctx->host_syncs++;
+	A->st_sym = bad ? 0 : 1;
+}
+
+static void stencil_fill(fsx_ctx * ctx, fsx_csr * A) {
+	if (A->it == FSX_I32) {
+		if (A->vt == FSX_F64) stencil_fill_t<int32_t, double>(ctx, A);
+		else stencil_fill_t<int32_t, float>(ctx, A);
+	}
+	else {
+		if (A->vt == FSX_F64) stencil_fill_t<int64_t, double>(ctx, A);
+		else stencil_fill_t<int64_t, float>(ctx, A);
+	}
+}
+
+// The stencil index (analysed once per pattern, from host copies of the
+// structure when given, else downloaded), then A's values in it; true when
+// A's values are symmetric and the layout can serve its products.  A's
+// stencil values are dropped when they are not.
+bool ensure_stencil(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
+                    const void * host_colind) {
+	csr_index & X = *A->idx;
+	if (X.st_state < 0) {
+		if (A->nrows == 0 || A->nrows != A->ncols) X.st_state = 0;
+		else {
+			std::vector<char> hrp, hci;
+			if (!host_rowptr || !host_colind) {
+				const size_t isz = A->it == FSX_I32 ? 4 : 8;
+				hrp.resize((A->nrows + 1) * isz);
+				hci.resize(std::max<int64_t>(1, A->nnz) * isz);
+				FSX_HIP(hipMemcpyAsync(hrp.data(), A->rowptr(), hrp.size(),
+				                       hipMemcpyDeviceToHost, ctx->stream));
+				if (A->nnz)
+					FSX_HIP(hipMemcpyAsync(hci.data(), A->colind(), A->nnz * isz,
+					                       hipMemcpyDeviceToHost, ctx->stream));
+				FSX_HIP(hipStreamSynchronize(ctx->stream));
+				host_rowptr = hrp.data();
+				host_colind = hci.data();
+			}
+			const bool ok =
+				A->it == FSX_I32
+					? stencil_analyze<int32_t>(ctx, A->nrows, (const int32_t *)host_rowptr,
+					                           (const int32_t *)host_colind, X)
+					: stencil_analyze<int64_t>(ctx, A->nrows, (const int64_t *)host_rowptr,
+					                           (const int64_t *)host_colind, X);
+			X.st_state = ok ? 1 : 0;
+		}
+	}
+	if (X.st_state != 1) return false;
+	if (!A->st_vals) {
+		const int64_t ns = (A->nrows + kSliceRows - 1) / kSliceRows;
+		const size_t vsz = A->vt == FSX_F64 ? 8 : 4;
+		A->st_vals = std::make_unique<dbuf>((size_t)(ns * kSliceRows * X.st_wu) * vsz);
+		stencil_fill(ctx, A);
+	}
+	if (A->st_sym != 1) {
+		A->st_vals.reset();
+		return false;
+	}
+	return true;
+}
+
 void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A) {
 	++A->vals_version;
 	if (A->idx->dg_ptr && A->dg_vals) sdia_fill(ctx, A);
+	if (A->idx->st_state == 1 && A->st_vals) {
+		// refilled in place; values that are no longer symmetric send the
+		// products to the slice-offset layout
+		stencil_fill(ctx, A);
+		if (A->st_sym != 1) {
+			A->st_vals.reset();
+			if (A->plan_kind == 4) A->plan_kind = 3;
+		}
+	}
 	if (!(A->idx->sl_ptr && A->sl_vals)) return;
 	if (A->it == FSX_I32) {
 		if (A->vt == FSX_F64) refresh_sliced_t<int32_t, double>(ctx, A);
@@ -374,7 +588,8 @@ void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A) {
 }
 
 // Short rows whose slices share few column offsets (stencils) -> slice
-// offsets; other short rows of similar length -> sliced rows (padding <= 10%,
+// offsets, or the symmetric stencil layout when all rows draw on one short
+// offset list and the values are symmetric; other short rows of similar length -> sliced rows (padding <= 10%,
 // addressable with the index type); other short-row matrices -> LDS-staged
 // row blocks; long rows -> LANES lanes per row with LANES the next power of
 // two of the mean row length (2..64).
@@ -398,7 +613,12 @@ void choose_plan(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
 	const double per_row_budget = (double)kStreamCap / kStreamRows;
 	if (A->avg_row <= per_row_budget) {
 		if (host_colind && ensure_sdia(ctx, A, host_rowptr, host_colind, true)) {
-			A->plan_kind = 3;
+			// symmetric FP64 stencils: the stencil layout, the slice-offset
+			// one kept for a later change of values.  FP32 matrices stay on
+			// slice offsets, measured faster there (256^3 FP32 CG K1: 0.119
+			// against 0.123-0.130 ms for the stencil variants)
+			A->plan_kind =
+				A->vt == FSX_F64 && ensure_stencil(ctx, A, host_rowptr, host_colind) ? 4 : 3;
 			A->plan_lanes = 0;
 			return;
 		}
@@ -585,6 +805,23 @@ __global__ void shift_scale_sdia_kernel(int64_t nslots_total, const uint64_t * _
 	a[k] = off[q] == 0 ? alpha + v : v;
 }
 
+// The same on the stencil layout shared by f and a: slot k holds column
+// u = (k / 64) % wu, offset off[w - wu + u], of row (slice k / 64 / wu,
+// lane k % 64) when that lane's mask bit is set (absent slots stay zero).
+__global__ void shift_scale_stencil_kernel(int64_t nslots_total, const uint64_t * __restrict__ mask,
+                                           stencil_desc sd, const double * __restrict__ f,
+                                           double alpha, double beta, double * __restrict__ a) {
+	const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nslots_total) return;
+	const int64_t c = k / kSliceRows;
+	const int64_t s = c / sd.wu;
+	const int j = sd.w - sd.wu + (int)(c % sd.wu);
+	const int lane = (int)(k % kSliceRows);
+	if (!((mask[s * sd.w + j] >> lane) & 1ull)) return;
+	const double v = beta * f[k];
+	a[k] = sd.off[j] == 0 ? alpha + v : v;
+}
+
 // dpos[r] = position of row r's diagonal entry (the last one if the row
 // stores several), -1 if none; missing[0] = 1 when some row has none
 template<class I>
@@ -616,7 +853,8 @@ __global__ void sym_scale_kernel(int64_t nrows, const I * __restrict__ rp,
 	if (!ok && bad) bad[0] = 1;
 }
 
-// a32_rc = (float)((s_r a_rc) s_c)
+// a32_rc = (float)(a_rc (s_r s_c)): the product of the two scales is the
+// same for a_rc and a_cr, so symmetric values give bitwise symmetric ones
 template<class I>
 __global__ void sym_scaled_f32_kernel(int64_t nrows, const I * __restrict__ rp,
                                       const I * __restrict__ ci,
@@ -625,7 +863,7 @@ __global__ void sym_scaled_f32_kernel(int64_t nrows, const I * __restrict__ rp,
 	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= nrows) return;
 	const double sr = s[r];
-	for (I e = rp[r]; e < rp[r + 1]; ++e) a32[e] = (float)((sr * v[e]) * s[ci[e]]);
+	for (I e = rp[r]; e < rp[r + 1]; ++e) a32[e] = (float)(v[e] * (sr * s[ci[e]]));
 }
 
 // missing[0] = 1 when some row has no stored diagonal entry
@@ -747,63 +985,41 @@ int fsx_csr_convert(fsx_ctx * ctx, const fsx_csr * A, fsx_dtype vt,
 		B->max_row = A->max_row;
 		B->avg_row = A->avg_row;
 		const size_t vsz = vt == FSX_F64 ? 8 : 4;
-		B->vals = std::make_unique<dbuf>(std::max<int64_t>(A->nnz, 1) * vsz);
-		const size_t n = (size_t)A->nnz;
-		if (n) {
-			unsigned g = (unsigned)((n + 255) / 256);
+		// m values of src (A's type) into a new buffer of B's type
+		auto cast_copy = [&](const dbuf & src, size_t m) {
+			auto dst = std::make_unique<dbuf>(std::max<size_t>(m, 1) * vsz);
+			if (m == 0) return dst;
+			const unsigned g = (unsigned)((m + 255) / 256);
 			if (A->vt == FSX_F64 && vt == FSX_F32)
 				hipLaunchKernelGGL((cast_kernel<double, float>), dim3(g), dim3(256), 0,
-				                   ctx->stream, n, (const double *)A->vals->p,
-				                   (float *)B->vals->p);
+				                   ctx->stream, m, (const double *)src.p, (float *)dst->p);
 			else if (A->vt == FSX_F32 && vt == FSX_F64)
 				hipLaunchKernelGGL((cast_kernel<float, double>), dim3(g), dim3(256), 0,
-				                   ctx->stream, n, (const float *)A->vals->p,
-				                   (double *)B->vals->p);
+				                   ctx->stream, m, (const float *)src.p, (double *)dst->p);
 			else
-				FSX_HIP(hipMemcpyAsync(B->vals->p, A->vals->p, n * vsz,
-				                       hipMemcpyDeviceToDevice, ctx->stream));
+				FSX_HIP(hipMemcpyAsync(dst->p, src.p, m * vsz, hipMemcpyDeviceToDevice,
+				                       ctx->stream));
 			FSX_HIP(hipGetLastError());
-		}
-		if (A->sl_vals) {
-			// same sliced layout, values converted (padding stays zero)
-			const size_t m = (size_t)std::max<int64_t>(1, A->idx->sl_nnz);
-			B->sl_vals = std::make_unique<dbuf>(m * vsz);
-			const unsigned gs = (unsigned)((m + 255) / 256);
-			if (A->vt == FSX_F64 && vt == FSX_F32)
-				hipLaunchKernelGGL((cast_kernel<double, float>), dim3(gs), dim3(256), 0,
-				                   ctx->stream, m, (const double *)A->sl_vals->p,
-				                   (float *)B->sl_vals->p);
-			else if (A->vt == FSX_F32 && vt == FSX_F64)
-				hipLaunchKernelGGL((cast_kernel<float, double>), dim3(gs), dim3(256), 0,
-				                   ctx->stream, m, (const float *)A->sl_vals->p,
-				                   (double *)B->sl_vals->p);
-			else
-				FSX_HIP(hipMemcpyAsync(B->sl_vals->p, A->sl_vals->p, m * vsz,
-				                       hipMemcpyDeviceToDevice, ctx->stream));
-			FSX_HIP(hipGetLastError());
-		}
+			return dst;
+		};
+		const size_t avsz = A->vt == FSX_F64 ? 8 : 4;
+		B->vals = cast_copy(*A->vals, (size_t)A->nnz);
+		// the same sliced, slice-offset and stencil layouts, values converted
+		// (padding stays zero); a cast of symmetric values is symmetric
+		if (A->sl_vals) B->sl_vals = cast_copy(*A->sl_vals, A->sl_vals->bytes / avsz);
 		else if (B->plan_kind == 2)
 			B->plan_kind = 0;
-		if (A->dg_vals) {
-			// same slice-offset layout, values converted (padding stays zero)
-			const size_t m = (size_t)std::max<int64_t>(1, A->idx->dg_nnz);
-			B->dg_vals = std::make_unique<dbuf>(m * vsz);
-			const unsigned gs = (unsigned)((m + 255) / 256);
-			if (A->vt == FSX_F64 && vt == FSX_F32)
-				hipLaunchKernelGGL((cast_kernel<double, float>), dim3(gs), dim3(256), 0,
-				                   ctx->stream, m, (const double *)A->dg_vals->p,
-				                   (float *)B->dg_vals->p);
-			else if (A->vt == FSX_F32 && vt == FSX_F64)
-				hipLaunchKernelGGL((cast_kernel<float, double>), dim3(gs), dim3(256), 0,
-				                   ctx->stream, m, (const float *)A->dg_vals->p,
-				                   (double *)B->dg_vals->p);
-			else
-				FSX_HIP(hipMemcpyAsync(B->dg_vals->p, A->dg_vals->p, m * vsz,
-				                       hipMemcpyDeviceToDevice, ctx->stream));
-			FSX_HIP(hipGetLastError());
-		}
+		if (A->dg_vals) B->dg_vals = cast_copy(*A->dg_vals, A->dg_vals->bytes / avsz);
 		else if (B->plan_kind == 3)
 			B->plan_kind = 0;
+		// FP32 copies default to slice offsets (as at creation); the stencil
+		// values are built on request (fsx_csr_set_plan)
+		if (A->st_vals && vt == FSX_F64) {
+			B->st_vals = cast_copy(*A->st_vals, A->st_vals->bytes / avsz);
+			B->st_sym = A->st_sym;
+		}
+		else if (B->plan_kind == 4)
+			B->plan_kind = B->dg_vals ? 3 : 0;
 		FSX_HIP(hipStreamSynchronize(ctx->stream));
 		ctx_retain(ctx);
 		*out = B.release();
@@ -845,7 +1061,7 @@ int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes) {
 int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes) {
 	return guard([&] {
 		FSX_REQUIRE(A, "null matrix");
-		FSX_REQUIRE(kind >= 0 && kind <= 3, "plan kind must be 0, 1, 2 or 3");
+		FSX_REQUIRE(kind >= 0 && kind <= 4, "plan kind must be 0, 1, 2, 3 or 4");
 		if (kind == 1)
 			FSX_REQUIRE(lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 ||
 			                lanes == 32 || lanes == 64,
@@ -867,6 +1083,13 @@ int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes) {
 			FSX_HIP(hipStreamSynchronize(ctx->stream));
 			FSX_REQUIRE(ensure_sdia(ctx, A, hrp.data(), hci.data(), false),
 			            "matrix structure does not fit the slice-offset layout");
+		}
+		if (kind == 4) {
+			FSX_REQUIRE(A->nrows > 0, "the stencil layout needs a non-empty matrix");
+			FSX_REQUIRE(ensure_stencil(A->ctx, A, nullptr, nullptr),
+			            A->idx->st_state != 1
+			                ? "matrix structure does not fit the symmetric stencil layout"
+			                : "matrix values are not symmetric (stencil layout)");
 		}
 		A->plan_kind = kind;
 		A->plan_lanes = kind == 1 ? lanes : 0;
@@ -1075,6 +1298,26 @@ int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double b
 			else
 				sdia_fill(ctx, A);
 		}
+		if (ix.st_state == 1 && A->st_vals) {
+			if (F->st_vals) {
+				// F's stencil values exist only when they are symmetric, and
+				// alpha I + beta F keeps that
+				const int64_t t = (int64_t)(A->st_vals->bytes / sizeof(double));
+				hipLaunchKernelGGL(shift_scale_stencil_kernel, dim3((unsigned)((t + 255) / 256)),
+				                   dim3(256), 0, ctx->stream, t, (const uint64_t *)ix.st_mask->p,
+				                   make_stencil_desc(ix), (const double *)F->st_vals->p, alpha,
+				                   beta, (double *)A->st_vals->p);
+				FSX_HIP(hipGetLastError());
+				ctx->launches++;
+				A->st_sym = 1;
+			}
+			else
+				stencil_fill(ctx, A);
+			if (A->st_sym != 1) {
+				A->st_vals.reset();
+				if (A->plan_kind == 4) A->plan_kind = 3;
+			}
+		}
 		if (ix.sl_ptr && A->sl_vals) {
 			if (A->it == FSX_I32) refresh_sliced_t<int32_t, double>(ctx, A);
 			else refresh_sliced_t<int64_t, double>(ctx, A);
@@ -1163,7 +1406,7 @@ int fsx_jacobi_relax(fsx_ctx * ctx, const fsx_csr * A, float omega, int nrelax,
 		double * tmp = ensure_relax_buffer(ctx, n);
 		const unsigned g = (unsigned)((n + 255) / 256);
 		double * xp = (double *)x->ptr();
-		if (A->plan_kind == 2 || A->plan_kind == 3) {
+		if (A->plan_kind == 2 || A->plan_kind == 3 || A->plan_kind == 4) {
 			// sweeps alternate between x and the buffer and end in x: an odd
 			// count starts from a copy of x (jacobi.hh:63's tmp = x)
 			double * src = xp;
@@ -1258,7 +1501,12 @@ int fsx_csr_layout_bytes(const fsx_csr * A, size_t x_bytes, uint64_t * bytes) {
 		const uint64_t n = (uint64_t)A->nrows, nsl = (n + kSliceRows - 1) / kSliceRows;
 		const uint64_t vec = n * x_bytes * 2; // x read once, y written once
 		const csr_index & X = *A->idx;
-		if (A->plan_kind == 3) {
+		if (A->plan_kind == 4) {
+			// value slots of the offsets >= 0; per slice one 8-byte lane mask
+			// per offset (entries at offsets < 0 re-read stored slots)
+			*bytes = nsl * kSliceRows * (uint64_t)X.st_wu * vb + nsl * (uint64_t)X.st_w * 8 + vec;
+		}
+		else if (A->plan_kind == 3) {
 			// value slots; per slice one 8-byte pointer; per offset entry (64
 			// slots) a 4-byte offset and an 8-byte lane mask
 			const uint64_t slots = (uint64_t)X.dg_nnz;

@@ -34,6 +34,17 @@
 //    reference's loop.  Measured on the 256^3 7-point matrix this moves the
 //    kernel from ~64% (best LDS-staged CSR variant) to ~70% of HBM peak.
 //
+//  * Symmetric stencil (structurally and bitwise symmetric matrices whose
+//    rows all draw on one list of at most 8 column offsets: the 3-D 7-point
+//    and 2-D 5-point stencils): the offsets travel as kernel arguments, each
+//    64-row slice keeps one lane mask per offset, and values are stored only
+//    for the offsets >= 0 (the diagonal and the entries above it); the entry
+//    of row r at a negative offset o is read from row r + o's stored entry at
+//    -o.  FP64 at 256^3: y = A x 0.227 -> 0.198 ms and the CG K1 0.232 ->
+//    0.200 ms against the slice-offset layout; the FP32 CG K1 measured
+//    0.123-0.130 ms against 0.119 there, so FP32 matrices stay on slice
+//    offsets.  Rows keep their CSR order: bit-identical to the serial loop.
+//
 // Workgroups are dealt to XCDs round-robin by block id.  Measured: keeping
 // that default order is faster than remapping blocks to give each XCD a
 // contiguous row range (all XCDs then stream neighbouring rows at once), and
@@ -47,7 +58,10 @@
 // wave-uniform address select cost more issue time than the 8% of bytes it
 // saved (FP32 CG product 0.120 -> 0.127 ms, and the FP64 products slower
 // too); a branch per load split the merged scalar loads of offsets and
-// masks; not used.
+// masks; not used.  Taking a wave's slices two at a time with every load of
+// both issued before the first gather (more loads in flight per wave)
+// raised the FP32 CG product's registers to 83 (0.141 ms) or, held to 64,
+// gave 0.125 ms against 0.118; not used.
 //
 // The kernels take an epilogue policy `Epi` that consumes each row's sum
 // (row() *adds* its dot-product terms to d: a thread may own several rows):
@@ -82,13 +96,23 @@ __device__ __forceinline__ T ldnt(const T * p) {
 	return __builtin_nontemporal_load(p);
 }
 
-// Gather policies: the value of the multiplied vector at column c.
+// Element c of p through a 32-bit byte offset from the (uniform) base: one
+// address register per lane instead of two.  Valid while c * sizeof(T)
+// stays below 2^32.
+template<class T>
+__device__ __forceinline__ T ld32(const T * p, uint32_t c) {
+	return *(const T *)((const char *)p + c * (uint32_t)sizeof(T));
+}
+
+// Gather policies: the value of the multiplied vector at column c (at32:
+// the same through a 32-bit offset).
 template<class X>
 struct gather_x {
 	using value_type = X;
 	const X * x;
 	__device__ __forceinline__ void init() {}
 	__device__ __forceinline__ X operator()(int64_t c) const { return x[c]; }
+	__device__ __forceinline__ X at32(uint32_t c) const { return ld32(x, c); }
 };
 
 // Sum of the thread's row for the LDS-staged mapping.  Valid when
@@ -353,6 +377,133 @@ __global__ __launch_bounds__(BT) void spmv_sdia_kernel(
 	epi.block_end(d, blockIdx.x, gridDim.x);
 }
 
+// Symmetric stencil layout: one ascending offset list for the whole matrix,
+// passed by value, so a wave loads no per-slice pointer or offsets -- only
+// the slice's lane masks (w consecutive words, one scalar load).  Values are
+// stored for the offsets >= 0 only; the entry of row r at offset o < 0 is
+// the stored entry of row r + o at offset -o, read from that row's slot (a
+// recently streamed slice: an L2 hit, not HBM traffic).  The layout is used
+// only when the CSR values are bitwise symmetric, so each row is still
+// summed over its own entries' values in CSR order: bit-identical to the
+// serial loop.
+struct stencil_desc {
+	int w;  // offsets in the list (<= kStencilMax)
+	int wu; // offsets >= 0: stored values per row
+	int32_t off[kStencilMax];
+	int32_t ju[kStencilMax]; // value column of entry j (of its mirror for off < 0)
+};
+
+inline stencil_desc make_stencil_desc(const csr_index & X) {
+	stencil_desc sd{};
+	sd.w = X.st_w;
+	sd.wu = X.st_wu;
+	for (int k = 0; k < kStencilMax; ++k) {
+		sd.off[k] = X.st_off[k];
+		sd.ju[k] = X.st_ju[k];
+	}
+	return sd;
+}
+
+// The slices are dealt to waves exactly as in the slice-offset kernel (same
+// grid, same grid stride), so each block's partials cover the same rows and
+// a whole solve runs bit for bit as on that layout.  Per slice, every value
+// load is issued before the gathers and every gather before the first sum
+// (scheduling barriers keep that order).  O32: value and gather addresses
+// are the array base (uniform) plus a 32-bit byte offset per lane.
+// MW: the minimum waves per SIMD the register allocation must allow (1: the
+// compiler's own choice).  With 32-bit offsets every instance fits 8
+// without spills.
+template<class Epi, class ACC, class V, class G, int BT, bool O32, int MW>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(MW))) void
+spmv_stencil_kernel(
+	int64_t nrows, stencil_desc sd, const uint64_t * __restrict__ st_mask,
+	const V * __restrict__ st_val, G x, Epi epi) {
+	if (epi.skip()) return;
+	x.init();
+	using XV = typename G::value_type;
+	const int lane = threadIdx.x & (kSliceRows - 1);
+	double d[Epi::NV];
+#pragma unroll
+	for (int i = 0; i < Epi::NV; ++i) d[i] = 0.0;
+	constexpr int kWaves = BT / kSliceRows;
+	const int64_t nslices = (nrows + kSliceRows - 1) / kSliceRows;
+	const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kSliceRows));
+	const int64_t sstride = (int64_t)gridDim.x * kWaves;
+	const int sw = sd.wu * kSliceRows; // values per slice
+	for (int64_t s = (int64_t)blockIdx.x * kWaves + wave; s < nslices; s += sstride) {
+		const int64_t r = s * kSliceRows + lane;
+		const uint64_t * mp = st_mask + s * sd.w;
+		V v[kStencilMax];
+		XV xv[kStencilMax];
+		// No load waits for the lane masks: a slot or vector element whose
+		// row lies in [0, nrows) is read whether or not the entry exists
+		// (the sum below skips absent entries), so every load of the slice
+		// issues at once.  Entry k of row r sits in the slot of row q at
+		// column ju[k]: q = r for offsets >= 0, the mirror row r + off[k]
+		// for offsets < 0 (rows before the first read their own slot).
+		// Row indices are 32-bit under O32 (nrows < 2^29 there).
+		using RI = std::conditional_t<O32, int32_t, int64_t>;
+		using RU = std::conditional_t<O32, uint32_t, uint64_t>;
+		const RI ri = (RI)r;
+#pragma unroll
+		for (int k = 0; k < kStencilMax; ++k) {
+			const RI rr = ri + (RI)sd.off[k];
+			const bool inb = (RU)rr < (RU)nrows;
+			const RI q = (sd.off[k] < 0 && inb) ? rr : ri;
+			const RU e = (RU)(q >> 6) * (RU)sw + (RU)(sd.ju[k] * kSliceRows) + (RU)(q & 63);
+			if constexpr (O32)
+				v[k] = ld32(st_val, e);
+			else
+				v[k] = st_val[e];
+		}
+		__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+		for (int k = 0; k < kStencilMax; ++k) {
+			const RI rr = ri + (RI)sd.off[k];
+			const bool inb = (RU)rr < (RU)nrows;
+			if constexpr (O32)
+				xv[k] = x.at32(inb ? (uint32_t)rr : 0u);
+			else
+				xv[k] = x(inb ? (int64_t)rr : 0);
+		}
+		__builtin_amdgcn_sched_barrier(0);
+		bool pr[kStencilMax];
+#pragma unroll
+		for (int k = 0; k < kStencilMax; ++k) {
+			// the mask array carries kStencilMax words of padding; entries
+			// past w have offset 0 and column 0 and are never present
+			const uint64_t m = mp[k];
+			pr[k] = (k < sd.w) & (((m >> lane) & 1) != 0);
+		}
+		ACC sum = ACC(0);
+		XV xself{};
+		bool has_self = false;
+		double dgv = 0.; // diagonal value (offdiag epilogues)
+#pragma unroll
+		for (int k = 0; k < kStencilMax; ++k) {
+			const ACC t = sum + (ACC)v[k] * (ACC)xv[k];
+			const bool isd = pr[k] && sd.off[k] == 0;
+			if constexpr (is_offdiag<Epi>::value) {
+				sum = pr[k] && !isd ? t : sum;
+				dgv = isd ? (double)v[k] : dgv;
+			}
+			else
+				sum = pr[k] ? t : sum;
+			if (isd) {
+				xself = xv[k];
+				has_self = true;
+			}
+		}
+		if (r < nrows) {
+			if constexpr (is_offdiag<Epi>::value)
+				epi.row_offdiag(r, sum, dgv, [&] { return has_self ? xself : x(r); });
+			else
+				epi.row(r, sum, d, [&] { return has_self ? xself : x(r); });
+		}
+	}
+	epi.block_end(d, blockIdx.x, gridDim.x);
+}
+
 // Epilogues that leave per-block partials cap the grid (fewer partials for
 // the finalize kernel); plain products use one row per thread.
 inline unsigned sliced_blocks(int64_t nrows, bool reduces = true) {
@@ -401,7 +552,40 @@ void spmv_launch_gather(hipStream_t st, const fsx_csr * A, const G & x,
 	const I * ci = (const I *)A->colind();
 	const V * vv = (const V *)A->vals->p;
 	if (A->nrows == 0) return;
-	if (A->plan_kind == 3) {
+	if (A->plan_kind == 4) {
+		const csr_index & X = *A->idx;
+		const stencil_desc sd = make_stencil_desc(X);
+		// 32-bit offsets when the value array and the gathered vector (at
+		// most 8-byte elements) span less than 4 GiB
+		const bool o32 = A->st_vals->bytes < ((size_t)1 << 32) &&
+		                 (uint64_t)A->ncols * 8 < ((uint64_t)1 << 32) &&
+		                 (uint64_t)A->nrows * 8 < ((uint64_t)1 << 32);
+		// Reducing products at 8 waves per SIMD (FP64 CG K1 at 256^3: 0.198
+		// against 0.215 ms), plain products at the compiler's choice (FP64
+		// y = A x: 0.196 against 0.229 ms)
+		constexpr int MW = Epi::reduces ? 8 : 1;
+		auto go = [&](auto bt_c, unsigned g) {
+			constexpr int BT = decltype(bt_c)::value;
+			if (o32)
+				hipLaunchKernelGGL((spmv_stencil_kernel<Epi, ACC, V, G, BT, true, MW>), dim3(g),
+				                   dim3(BT), 0, st, A->nrows, sd, (const uint64_t *)X.st_mask->p,
+				                   (const V *)A->st_vals->p, x, epi);
+			else
+				hipLaunchKernelGGL((spmv_stencil_kernel<Epi, ACC, V, G, BT, false, 1>), dim3(g),
+				                   dim3(BT), 0, st, A->nrows, sd, (const uint64_t *)X.st_mask->p,
+				                   (const V *)A->st_vals->p, x, epi);
+		};
+		if constexpr (!Epi::reduces)
+			go(std::integral_constant<int, kSpmvThreads>{},
+			   sdia_blocks(A->nrows, false, 0, 0));
+		else {
+			const sdia_reduce_shape sh = sdia_reduce(A->vt, A->nrows);
+			const unsigned g = sdia_blocks(A->nrows, true, sh.threads, sh.max_blocks);
+			if (sh.threads == 512) go(std::integral_constant<int, 512>{}, g);
+			else go(std::integral_constant<int, 256>{}, g);
+		}
+	}
+	else if (A->plan_kind == 3) {
 		auto go = [&](auto bt_c, unsigned g) {
 			constexpr int BT = decltype(bt_c)::value;
 			hipLaunchKernelGGL((spmv_sdia_kernel<Epi, ACC, V, G, BT>), dim3(g), dim3(BT), 0,
@@ -462,7 +646,7 @@ void spmv_launch_typed(hipStream_t st, const fsx_csr * A, const X * x,
 
 // Number of partials (each NV doubles) a reducing product on A leaves.
 inline unsigned spmv_blocks(const fsx_csr * A) {
-	if (A->plan_kind == 3) {
+	if (A->plan_kind == 3 || A->plan_kind == 4) {
 		const sdia_reduce_shape sh = sdia_reduce(A->vt, A->nrows);
 		return sdia_blocks(A->nrows, true, sh.threads, sh.max_blocks);
 	}

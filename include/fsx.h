@@ -184,8 +184,15 @@ int fsx_csr_info(const fsx_csr * A, int64_t * nrows, int64_t * ncols,
  * vector per row (`lanes` is the vector width), 2 = sliced rows (64-row
  * slices, entry-major, column per entry), 3 = slice offsets (64-row slices
  * storing the distinct column offsets of their rows and lane masks instead
- * of a column per entry; chosen for stencil-like structure).  Every mapping
- * sums each row serially in CSR order except 1. */
+ * of a column per entry; chosen for stencil-like structure), 4 = symmetric
+ * stencil (one offset list of at most 8 offsets for the whole matrix, lane
+ * masks, values stored only for the offsets >= 0; the entries below the
+ * diagonal are read from their mirrors above it, so it serves only
+ * structurally and bitwise symmetric values; chosen for such FP64 matrices,
+ * with slice offsets kept beside it).  A change of values that breaks the
+ * symmetry moves a plan-4 matrix to plan 3; fsx_csr_set_plan(A, 4, 0)
+ * fails for a structure or values that do not fit.  Every mapping sums each
+ * row serially in CSR order except 1. */
 int fsx_csr_plan(const fsx_csr * A, int * kind, int * lanes);
 int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes);
 /* Replace A's values (nnz entries of A's value type, CSR order) from host
@@ -580,7 +587,10 @@ int fsx_time_cg_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_ve
  * stored value slot and index of that layout once, x read once and y written
  * once at x_bytes per element.  (For the slice-offset layout: value slots,
  * one 8-byte pointer per 64-row slice, a 4-byte offset and an 8-byte lane
- * mask per 64 slots; no column per entry.) */
+ * mask per 64 slots; no column per entry.  For the symmetric stencil layout:
+ * the value slots of the offsets >= 0 and one 8-byte lane mask per offset
+ * and 64-row slice; the entries below the diagonal re-read stored slots and
+ * are not counted again.) */
 int fsx_csr_layout_bytes(const fsx_csr * A, size_t x_bytes, uint64_t * bytes);
 /* Same for a plain device copy of n doubles (bandwidth reference). */
 int fsx_time_copy(fsx_ctx * ctx, const fsx_vec * src, fsx_vec * dst, int reps,

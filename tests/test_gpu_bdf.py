@@ -106,16 +106,21 @@ def test_shift_scale(ctx):
     rp, ci, fv, _ = B.heat2d(70, 2.0)
     op = B.CsrRhsOp(rp, ci, fv)
     F = ctx.csr(rp, ci, fv)
+    assert F.plan() == (4, 0)
     A = F.convert(fsx.F64)
     x = P.uniform(70 * 70, 3)
     xd, y = ctx.vec(x), ctx.zeros(x.size)
-    for g in (0.25, 3.0e-3):
-        A.shift_scale(F, 1.0, -g)
-        op.set_scaling(g)
-        S = op.stage()
-        np.testing.assert_array_equal(A.download()[2], S.vals)
-        A.spmv(xd, y)
-        np.testing.assert_array_equal(y.get(), S.spmv(x))
+    # the symmetric stencil and slice-offset layouts are rewritten in place
+    for plan in (4, 3):
+        A.set_plan(plan)
+        for g in (0.25, 3.0e-3):
+            A.shift_scale(F, 1.0, -g)
+            assert A.plan() == (plan, 0)
+            op.set_scaling(g)
+            S = op.stage()
+            np.testing.assert_array_equal(A.download()[2], S.vals)
+            A.spmv(xd, y)
+            np.testing.assert_array_equal(y.get(), S.spmv(x))
     # a row without its diagonal entry: alpha must be 0
     M = ctx.csr(np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.array([1.0, 2.0]))
     Mc = M.convert(fsx.F64)

@@ -163,19 +163,19 @@ def test_ir_reaches_1e10_and_matches_fp64(ctx, sigma, jacobi):
     assert rel(x, xo) < 1e-6
 
 
-@pytest.mark.parametrize("plan", [3, 2])
+@pytest.mark.parametrize("plan", [4, 3, 2])
 def test_grid_stride_sizes_iteration_matched(ctx, plan):
     """More rows than one capped SpMV grid covers (> 8192 x 256), so each
     thread owns several rows: SpMV+dot and 25 CG iterations against the
-    oracle, iteration-matched (both stop at maxiter), on the slice-offset
-    and the sliced layouts."""
+    oracle, iteration-matched (both stop at maxiter), on the symmetric
+    stencil, slice-offset and sliced layouts."""
     n = 132
     rp, ci, v, Ao = diffusion(n)
     N = n ** 3
     assert N > 8192 * 256
     b = P.uniform(N, 2002)
     A = ctx.csr(rp, ci, v)
-    assert A.plan() == (3, 0)
+    assert A.plan() == (4, 0)
     A.set_plan(plan)
     bd = ctx.vec(b)
     w = ctx.zeros(N)
@@ -204,7 +204,7 @@ def test_wide_block_shape_iteration_matched(ctx):
     assert N >= 1 << 23
     b = P.uniform(N, 2002)
     A = ctx.csr(rp, ci, v)
-    assert A.plan() == (3, 0)
+    assert A.plan() == (4, 0)
     bd = ctx.vec(b)
     w = ctx.zeros(N)
     pw = fsx.spmv_dot(A, bd, w)
@@ -232,7 +232,7 @@ def test_layouts_give_identical_trajectories(ctx):
     d = A.diag_inv()
     bd = ctx.vec(b)
     out = []
-    for plan in (3, 2, 0):
+    for plan in (4, 3, 2, 0):
         A.set_plan(plan)
         x = ctx.zeros(n ** 3)
         info = fsx.cg_solve(A, bd, x, fsx.Settings(maxiter=2000, rtol=1e-10,
@@ -273,22 +273,27 @@ def test_graphs_after_matrix_rebuild(ctx):
         gc.collect()
 
 
-@pytest.mark.parametrize("plan", [3, 2, 0])
+@pytest.mark.parametrize("plan", [4, 3, 2, 0])
 def test_sym_scaled_f32_values(ctx, plan):
     """S A S in FP32, S = 1/sqrt(diag): values bit for bit against numpy
-    ((s_i a_ij) s_j rounded once), on each SpMV layout; errors for a missing
-    or non-positive diagonal."""
+    (a_ij (s_i s_j) rounded once, so symmetric A gives bitwise symmetric
+    values and keeps the symmetric stencil layout), on each SpMV layout;
+    errors for a missing or non-positive diagonal."""
     n = 20
     rp, ci, v, _ = diffusion(n, 2.0, seed=5001)
     A64 = ctx.csr(rp, ci, v)
     A64.set_plan(plan)
     A32 = A64.convert(fsx.F32)
+    A32.set_plan(plan)
     A32.set_values_sym_scaled(A64)
+    assert A32.plan() == (plan, 0)
     rows = np.repeat(np.arange(n ** 3), np.diff(rp))
     d = v[ci == rows]
     sv = 1.0 / np.sqrt(d)
-    want = ((sv[rows] * v) * sv[ci]).astype(np.float32)
+    want = (v * (sv[rows] * sv[ci])).astype(np.float32)
     np.testing.assert_array_equal(A32.download()[2], want)
+    W = sp.csr_matrix((want, ci, rp))
+    assert (W != W.T).nnz == 0
     x = P.uniform(n ** 3, 5002).astype(np.float32)
     y = ctx.zeros(n ** 3, fsx.F32)
     A32.spmv(ctx.vec(x, fsx.F32), y)

@@ -1,9 +1,9 @@
 """Device CSR SpMV and fused kernels against the oracle (gpu).
 
-Bar: structure round trips bit-exact; SpMV on the LDS-staged row-block
-mapping bit-identical to the reference's serial CSR loop (it sums each row
-serially in CSR order); the vector-per-row mapping within 1e-13 relative
-(tree order inside a row).
+Bar: structure round trips bit-exact; SpMV on the LDS-staged row-block,
+sliced, slice-offset and symmetric stencil layouts bit-identical to the
+reference's serial CSR loop (each sums every row serially in CSR order); the
+vector-per-row mapping within 1e-13 relative (tree order inside a row).
 """
 from pathlib import Path
 
@@ -57,7 +57,7 @@ def test_structure_round_trip(ctx):
     np.testing.assert_array_equal(r2, rp)
     np.testing.assert_array_equal(c2, ci)
     np.testing.assert_array_equal(v2, v)
-    assert A.plan() == (3, 0)  # slice offsets for the stencil structure
+    assert A.plan() == (4, 0)  # symmetric stencil: one offset list, symmetric values
 
 
 @pytest.mark.parametrize("n", [1, 7, 64])
@@ -80,42 +80,122 @@ def test_convdiff_bit_exact(ctx):
 
 @pytest.mark.parametrize("n", [3, 33, 70])
 def test_every_plan_on_stencil(ctx, n):
-    # plans 0 (LDS-staged), 2 (sliced) and 3 (slice offsets) sum rows
-    # serially: bit-exact; plan 1 (lanes per row) sums in a tree: rounding
+    # plans 0 (LDS-staged), 2 (sliced), 3 (slice offsets) and 4 (symmetric
+    # stencil) sum rows serially: bit-exact; plan 1 (lanes per row) sums in a
+    # tree: rounding
     k = P.lognormal_field(n, 1.0, 8.0, 11)
     rp, ci, v = P.diffusion3d(n, k)
     A = ctx.csr(rp, ci, v)
-    assert A.plan() == ((3, 0) if n > 3 else (0, 0))  # 27 rows: too much padding
+    assert A.plan() == ((4, 0) if n > 3 else (0, 0))  # 27 rows: too much padding
     x = np.random.default_rng(3).standard_normal(n ** 3)
     ref = O.Csr(rp, ci, v).spmv(x)
     xd, yd = ctx.vec(x), ctx.zeros(n ** 3)
-    for plan in [(0, 0), (2, 0), (3, 0), (1, 2), (1, 8), (1, 64)]:
+    for plan in [(0, 0), (2, 0), (3, 0), (4, 0), (1, 2), (1, 8), (1, 64)]:
         A.set_plan(*plan)
         A.spmv(xd, yd)
         if plan[0] != 1:
             np.testing.assert_array_equal(yd.get(), ref, err_msg=str(plan))
         else:
             assert rel(yd.get(), ref) <= 1e-13, plan
-    # fused residual and SpMV+dot on both slice layouts
+    # fused residual and SpMV+dot on the slice layouts
     b = np.random.default_rng(4).standard_normal(n ** 3)
     r = ctx.zeros(n ** 3)
     bd = ctx.vec(b)
-    for plan in (2, 3):
+    for plan in (2, 3, 4):
         A.set_plan(plan)
         A.residual(bd, xd, r)
         np.testing.assert_array_equal(r.get(), b - ref)
         pw = fsx.spmv_dot(A, xd, yd)
         np.testing.assert_array_equal(yd.get(), ref)
         assert abs(pw - O.dot(ref, x)) <= 1e-12 * np.abs(ref * x).sum()
-    # FP32 copy shares the layout
-    A.set_plan(3)
-    A32 = A.convert(fsx.F32)
-    assert A32.plan() == (3, 0)
-    x32, y32 = ctx.vec(x.astype(np.float32)), ctx.zeros(n ** 3, fsx.F32)
-    A32.spmv(x32, y32)
+    # FP32 copies share the index arrays and start on slice offsets; both
+    # layouts sum each row serially in FP32, so their results agree bit for
+    # bit
+    x32 = ctx.vec(x.astype(np.float32))
     ref32 = O.Csr(rp, ci, v.astype(np.float32).astype(np.float64)).spmv(
         x.astype(np.float32).astype(np.float64))
-    assert rel(y32.get().astype(np.float64), ref32) <= 1e-6
+    ys = []
+    for plan in (3, 4):
+        A.set_plan(plan)
+        A32 = A.convert(fsx.F32)
+        assert A32.plan() == (3, 0)
+        A32.set_plan(plan)
+        y32 = ctx.zeros(n ** 3, fsx.F32)
+        A32.spmv(x32, y32)
+        assert rel(y32.get().astype(np.float64), ref32) <= 1e-6
+        ys.append(y32.get())
+    np.testing.assert_array_equal(ys[0], ys[1])
+
+
+def test_stencil_layout_needs_symmetry(ctx):
+    """The symmetric stencil layout reads each entry below the diagonal
+    from its mirror above, so it serves only bitwise symmetric values and
+    structures: asymmetric values at creation, a signed zero mirrored by +0,
+    a missing mirror entry, values set asymmetric later (the products fall
+    back to the slice-offset layout), and shift-scaled copies; every product
+    bit-exact against the oracle."""
+    n = 18
+    k = P.lognormal_field(n, 1.0, 8.0, 21)
+    rp, ci, v = P.diffusion3d(n, k)
+    N = n ** 3
+    x = np.random.default_rng(5).standard_normal(N)
+    xd, yd = ctx.vec(x), ctx.zeros(N)
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    lower = np.flatnonzero(ci < rows)
+
+    def exact(A, vals):
+        A.spmv(xd, yd)
+        np.testing.assert_array_equal(yd.get(), O.Csr(rp, ci, vals).spmv(x))
+
+    A = ctx.csr(rp, ci, v)
+    assert A.plan() == (4, 0)
+    exact(A, v)
+    # one entry below the diagonal off by one ulp
+    va = v.copy()
+    va[lower[100]] = np.nextafter(va[lower[100]], 0.0)
+    B = ctx.csr(rp, ci, va)
+    assert B.plan() == (3, 0)
+    exact(B, va)
+    with pytest.raises(fsx.FsxError, match="not symmetric"):
+        B.set_plan(4)
+    # -0.0 below the diagonal, +0.0 above: equal values, different bits
+    vz = v.copy()
+    e = lower[7]
+    i, j = rows[e], ci[e]
+    m = rp[j] + np.flatnonzero(ci[rp[j]:rp[j + 1]] == i)[0]
+    vz[e], vz[m] = -0.0, 0.0
+    Z = ctx.csr(rp, ci, vz)
+    assert Z.plan() == (3, 0)
+    exact(Z, vz)
+    # values set asymmetric later: back to the slice-offset layout, exact;
+    # symmetric again: the stencil layout on request
+    A.set_values(va)
+    assert A.plan() == (3, 0)
+    exact(A, va)
+    A.set_values(v)
+    A.set_plan(4)
+    exact(A, v)
+    # a structure without an entry's mirror
+    keep = np.ones(rp[-1], bool)
+    keep[lower[50]] = False
+    rp2 = np.concatenate([[0], np.cumsum(np.add.reduceat(keep, rp[:-1]))]).astype(np.int32)
+    ci2, v2 = ci[keep], v[keep]
+    C = ctx.csr(rp2, ci2, v2)
+    assert C.plan() == (3, 0)
+    with pytest.raises(fsx.FsxError, match="structure"):
+        C.set_plan(4)
+    C.spmv(xd, yd)
+    np.testing.assert_array_equal(yd.get(), O.Csr(rp2, ci2, v2).spmv(x))
+    # alpha I + beta F on the stencil layout, bit for bit
+    F = ctx.csr(rp, ci, v)
+    S = F.convert(fsx.F64)
+    assert S.plan() == (4, 0)
+    S.shift_scale(F, 1.0, -0.37)
+    vs = -0.37 * v
+    vs[ci == rows] = 1.0 + vs[ci == rows]
+    assert S.plan() == (4, 0)
+    exact(S, vs)
+    np.testing.assert_array_equal(S.download()[2], vs)
 
 
 def test_slice_offsets_fallback_and_block_diagonal(ctx):
