@@ -84,6 +84,8 @@ def main():
                     help="skip the oracle's full-size solve (CPU baseline and parity)")
     ap.add_argument("--no-sigma25", action="store_true",
                     help="skip the sigma = 2.5 comparison")
+    ap.add_argument("--no-generic", action="store_true",
+                    help="skip the generic-sequence CG beside the fused driver")
     ap.add_argument("--maxiter", type=int, default=5000,
                     help="CG iteration cap (FP64 solve and each inner solve)")
     ap.add_argument("--max-outer", type=int, default=10)
@@ -332,6 +334,36 @@ def bench_diffusion(a, fsx, P):
     rel_ir = true_rel_res(fsx, ctx, A64, bd, x)
     roofline["share_of_step"] = round(k1_ms * 1e-3 * ii.inner_iters_total / ir_s, 3)
 
+    # ---- the same FP64 CG + Jacobi through the generic sequence -----------
+    # (the reference's loop over per-operation kernels, three blocking
+    # reductions per iteration) beside the fused driver, with the host
+    # synchronisations each makes
+    generic = None
+    if not a.no_generic:
+        s0 = ctx.counters()[0]
+        info_f, dt_f = solve_fp64(fsx, ctx, A64, dinv, bd, x, a.maxiter)
+        syncs_f = ctx.counters()[0] - s0
+        st = fsx.Settings(maxiter=a.maxiter, rtol=1e-10, use_zero_guess=True)
+        fsx.check(fsx.lib().fsx_zero(ctx.h, x.h))
+        ctx.sync()
+        s0 = ctx.counters()[0]
+        t = time.perf_counter()
+        info_g, _ = fsx.krylov("cg_generic", A64, bd, x, st, prec=1)
+        ctx.sync()
+        dt_g = time.perf_counter() - t
+        syncs_g = ctx.counters()[0] - s0
+        log(f"generic cg+jacobi: {dt_g:.3f} s, {info_g.iters} iters, {syncs_g} host syncs; "
+            f"fused {dt_f:.3f} s, {info_f.iters} iters, {syncs_f} host syncs")
+        generic = {"seconds": round(dt_g, 4), "iters": info_g.iters,
+                   "status": info_g.reason, "host_syncs": int(syncs_g),
+                   "true_rel_residual": true_rel_res(fsx, ctx, A64, bd, x),
+                   "fused_seconds": round(dt_f, 4), "fused_iters": info_f.iters,
+                   "fused_host_syncs": int(syncs_f),
+                   "fused_speedup": round(dt_g / dt_f, 3),
+                   "path": "fsx_krylov_cg_generic: op::cg generic() over vec::device "
+                           "and op::core<mat::device_csr> (one run, work vectors "
+                           "allocated inside)"}
+
     out = {
         "metric": METRIC,
         "value": round(ir_s, 4),
@@ -359,6 +391,7 @@ def bench_diffusion(a, fsx, P):
             "fp64_cg": {"seconds": round(cg_s, 4), "iters": info64.iters,
                         "true_rel_residual": rel64},
             "ir_speedup_vs_fp64_cg": round(cg_s / ir_s, 3),
+            "generic_cg": generic,
         },
         "roofline": roofline,
     }

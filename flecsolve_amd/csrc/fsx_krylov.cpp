@@ -130,8 +130,26 @@ int fsx_krylov_cg(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
 		with_prec(ctx, A, prec, nrelax, [&](auto Ah, auto Ph) {
 			cg::solver slv(to_settings<cg::settings>(s),
 			               vec::make_device_work<cg::nwork>(bv));
-			auto op = slv(Ah, Ph, std::ref(h));
-			to_info(op(bv, xv), info);
+			// the history is a diagnostic only when one is asked for (a
+			// bound diagnostic is called after every iteration)
+			if (res_hist && hist_cap > 0) to_info(slv(Ah, Ph, std::ref(h))(bv, xv), info);
+			else to_info(slv(Ah, Ph)(bv, xv), info);
+		});
+	});
+}
+
+int fsx_krylov_cg_generic(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
+                          const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                          fsx_info * info, double * res_hist, int hist_cap) {
+	return guarded([&] {
+		check_args(ctx, A, b, x, s, info);
+		dvec bv = dvec::borrow(ctx, const_cast<fsx_vec *>(b));
+		dvec xv = dvec::borrow(ctx, x);
+		history h{res_hist, hist_cap};
+		with_prec(ctx, A, prec, nrelax, [&](auto Ah, auto Ph) {
+			cg::solver slv(to_settings<cg::settings>(s),
+			               vec::make_device_work<cg::nwork>(bv));
+			to_info(slv(Ah, Ph, std::ref(h)).generic(bv, xv), info);
 		});
 	});
 }

@@ -169,6 +169,9 @@ _SIGS = {
     "fsx_krylov_cg": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
                                 C.POINTER(Settings), C.POINTER(Info),
                                 C.POINTER(C.c_double), C.c_int]),
+    "fsx_krylov_cg_generic": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
+                                        C.POINTER(Settings), C.POINTER(Info),
+                                        C.POINTER(C.c_double), C.c_int]),
     "fsx_krylov_fcg": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
                                  C.POINTER(Settings), C.POINTER(Info)]),
     "fsx_krylov_gmres": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
@@ -706,8 +709,8 @@ def be_adjoint_ir(A: CSR, A32: CSR, u0: Vector, u_target: Vector, steps: int, ck
 
 def krylov(kind: str, A: CSR, b: Vector, x: Vector, settings: Settings,
            prec: int = 0, nrelax: int = 1, hist_cap: int = 0):
-    """Generic solver (cg, fcg, gmres, bicgstab) over device vectors.
-    Returns (Info, residual history array)."""
+    """Reference-shaped solver (cg, cg_generic, fcg, gmres, bicgstab) over
+    device vectors.  Returns (Info, residual history array)."""
     info = Info()
     hist = np.zeros(max(hist_cap, 1))
     hp = hist.ctypes.data_as(C.POINTER(C.c_double))

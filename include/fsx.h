@@ -434,15 +434,27 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
                     double rtol_outer, int max_outer,
                     const fsx_settings * inner, fsx_ir_info * info);
 
-/* Generic drop-in solvers: the C++ op::cg / op::gmres / op::bicgstab
- * templates (include/flecsolve/solvers/) instantiated over the device vector
- * type, one blocking reduction per .get() exactly as the reference.
- * prec: 0 identity (op::I), 1 Jacobi z = D^{-1} r (Dinv CSR), 2 weighted
- * Jacobi relaxation (omega 2/3 float, `nrelax` sweeps).
+/* Reference-shaped solves: the C++ op::cg / op::fcg / op::gmres /
+ * op::bicgstab templates (include/flecsolve/solvers/) instantiated over the
+ * device vector type and called as the reference's callers call them.
+ * With the identity or Jacobi, CG, BiCGSTAB and GMRES hand the solve to the
+ * fused drivers (fsx_cg_solve, fsx_cg_solve_diag when res_hist is given,
+ * fsx_bicgstab_solve, fsx_gmres_solve); weighted Jacobi relaxation and FCG
+ * run the generic sequence, one blocking reduction per .get() exactly as
+ * the reference.  prec: 0 identity (op::I), 1 Jacobi z = D^{-1} r (Dinv
+ * CSR), 2 weighted Jacobi relaxation (omega 2/3 float, `nrelax` sweeps).
  * res_hist (optional) receives the per-iteration residual estimate. */
 int fsx_krylov_cg(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
                   const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
                   fsx_info * info, double * res_hist, int hist_cap);
+/* The same CG always through the generic sequence (op::cg's generic(): the
+ * reference's loop, flecsolve/solvers/cg.hh:44-140, with every vector
+ * operation a libfsx kernel and three blocking reductions per iteration),
+ * for comparison with the fused driver and for callers that want the
+ * reference's operation order. */
+int fsx_krylov_cg_generic(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
+                          const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
+                          fsx_info * info, double * res_hist, int hist_cap);
 int fsx_krylov_fcg(fsx_ctx * ctx, const fsx_csr * A, int prec, int nrelax,
                    const fsx_vec * b, fsx_vec * x, const fsx_settings * s,
                    fsx_info * info);

@@ -33,6 +33,7 @@ def _problem(name):
     return rp, ci, v, P.uniform(8000, 2002), 1e-10
 
 
+@pytest.mark.parametrize("kind", ["cg", "cg_generic"])
 @pytest.mark.parametrize("name,prec,slack", [
     ("poisson64", 0, 2),      # 2D Poisson 64^2, unpreconditioned, rtol 1e-8
     ("diffusion20", 1, 2),    # lognormal-k diffusion, CG + Jacobi, rtol 1e-10
@@ -41,7 +42,10 @@ def _problem(name):
     # the dot products moves the count by up to ~1% there.
     ("diffusion20", 0, 4),
 ])
-def test_generic_cg(ctx, name, prec, slack):
+def test_generic_cg(ctx, kind, name, prec, slack):
+    """fsx_krylov_cg (the reference-shaped call: with a history it runs the
+    fused driver with a diagnostic) and fsx_krylov_cg_generic (the
+    reference's loop over per-operation kernels) against the oracle."""
     rp, ci, v, b, rtol = _problem(name)
     n = b.size
     Ao = O.Csr(rp, ci, v)
@@ -50,7 +54,7 @@ def test_generic_cg(ctx, name, prec, slack):
                       P=O.prec(1, Ao.dinv()) if prec else None, hist_cap=2000)
     A = ctx.csr(rp, ci, v)
     xd = ctx.zeros(n)
-    info, h = fsx.krylov("cg", A, ctx.vec(b), xd, fsx.Settings(**s), prec=prec,
+    info, h = fsx.krylov(kind, A, ctx.vec(b), xd, fsx.Settings(**s), prec=prec,
                          hist_cap=2000)
     assert info.reason == io.reason == "converged_rtol"
     assert abs(info.iters - io.iters) <= slack
@@ -61,6 +65,12 @@ def test_generic_cg(ctx, name, prec, slack):
         # although the final iterates agree to 1e-10
         m = min(info.iters, io.iters) - 5
         np.testing.assert_allclose(h[:m], ho[:m], rtol=1e-4)
+    if kind == "cg":
+        # without a history: the fused driver, the same solve bit for bit
+        x2 = ctx.zeros(n)
+        i2, _ = fsx.krylov(kind, A, ctx.vec(b), x2, fsx.Settings(**s), prec=prec)
+        assert i2.iters == info.iters and i2.reason == info.reason
+        np.testing.assert_array_equal(x2.get(), xd.get())
 
 
 def test_generic_fcg(ctx):
