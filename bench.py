@@ -221,13 +221,17 @@ def bench_diffusion(a, fsx, P):
         f"SpMV plan {A64.plan()}, FP32 copy {A32.plan()}")
 
     # ---- kernel roofline (HIP events on the library stream) --------------
-    # The timed region's dominant kernel is the inner FP32 CG's K1 (w = A p
-    # and the partials of w.p on the refinement's FP32 matrix); the FP64
-    # product and the FP64 CG's K1 are reported beside it.
+    # The timed region's dominant kernel is the inner FP32 CG's K1 on the
+    # refinement's FP32 matrix: w = A p with the partials of w.p and w.w (the
+    # single-reduction inner CG of the scaled matrix) or of w.p (Jacobi-
+    # preconditioned inner CG); the FP64 product and the FP64 CG's K1 are
+    # reported beside it.
     reps = a.spmv_reps
     p32 = ctx.vec(b.astype(np.float32))
     w32 = ctx.zeros(N, fsx.F32)
-    k1_ms = fsx_time(fsx, "fsx_time_cg_spmv", ctx, A32, p32, w32, reps)
+    k1_sr = a.ir_jacobi == 2
+    k1_ms = fsx_time(fsx, "fsx_time_cg_spmv_sr" if k1_sr else "fsx_time_cg_spmv",
+                     ctx, A32, p32, w32, reps)
     k1_bytes = A32.layout_bytes(4)
     k1_csr = spmv_bytes(N, nnz, 4, xbytes=4)
     del p32, w32
@@ -268,8 +272,9 @@ def bench_diffusion(a, fsx, P):
     kname, basis = knames[plan32], bases.get(plan32, csr_basis)
     roofline = {
         "bound": "hbm",
-        "kernel": f"{kname} with the fused CG epilogue (K1: w = A p, partials of w.p), "
-                  "FP32 values and vectors: the inner CG of the refinement",
+        "kernel": f"{kname} with the fused CG epilogue (K1: w = A p, partials of "
+                  f"{'w.p and w.w' if k1_sr else 'w.p'}), FP32 values and vectors: the "
+                  "inner CG of the refinement",
         "achieved": round(gbs(k1_bytes, k1_ms), 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -511,12 +516,16 @@ def oracle_run(a, rp, ci, v, b, iters_gpu, x64, xir, tight):
     return base, parity
 
 
-def batch_cg_bytes(rows_s, nnz_s, vbytes, xbytes):
+def batch_cg_bytes(rows_s, nnz_s, vbytes, xbytes, matrix_bytes=None):
     """Algorithmic bytes of one CG + Jacobi iteration of one system: the SpMV
-    (values + column ids + row pointers, p read once, w written) and the
-    vector passes (x, r read+written; p, w, dinv read; z written; then p
-    read+written and z read)."""
-    spmv = nnz_s * (vbytes + 4) + (rows_s + 1) * 4 + rows_s * xbytes * 2
+    (the matrix bytes, p read once, w written) and the vector passes (x, r
+    read+written; p, w, dinv read; z written; then p read+written and z
+    read).  matrix_bytes: per system (the layout's bytes apportioned by
+    stored entries); default the CSR arrays (values + column ids + row
+    pointers)."""
+    if matrix_bytes is None:
+        matrix_bytes = nnz_s * (vbytes + 4) + (rows_s + 1) * 4
+    spmv = matrix_bytes + rows_s * xbytes * 2
     vec = rows_s * xbytes * (4 + 3 + 1 + 3)
     return spmv + vec
 
@@ -586,7 +595,13 @@ def bench_batched(a, fsx, P):
     xir = x.get()
     err_ir = float(np.max(np.sqrt(np.add.reduceat((xir - x64) ** 2, rows[:-1])) /
                           np.sqrt(np.add.reduceat(x64 * x64, rows[:-1]))))
-    bytes64 = float(np.sum(it64.astype(np.float64) * batch_cg_bytes(sizes, nnz_s, 8, 8)))
+    # the batch matrix's layout bytes (values and index of the layout the
+    # kernel reads, without the vectors) apportioned to the systems by their
+    # stored entries; the CSR figure beside it
+    mat_bytes = A64.layout_bytes(8) - 16 * int(rows[-1])
+    mat_s = mat_bytes * nnz_s.astype(np.float64) / float(nnz_s.sum())
+    bytes64 = float(np.sum(it64.astype(np.float64) * batch_cg_bytes(sizes, nnz_s, 8, 8, mat_s)))
+    bytes64_csr = float(np.sum(it64.astype(np.float64) * batch_cg_bytes(sizes, nnz_s, 8, 8)))
     gbs = bytes64 / t64 / 1e9
     log(f"fp64: {t64:.3f} s, iters {it64.min()}..{it64.max()} (sum {int(it64.sum())}), "
         f"{gbs:.0f} GB/s algorithmic; ir: {tir:.3f} s, outer {oit.max()}, "
@@ -625,8 +640,11 @@ def bench_batched(a, fsx, P):
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": bytes64,
-                     "note": "algorithmic bytes of every system's CG iterations over "
-                             "the FP64 batch's wall time; small systems run from L2"},
+                     "csr_equiv_gbs": round(bytes64_csr / t64 / 1e9, 1),
+                     "note": "algorithmic bytes of every system's CG iterations (the "
+                             "layout's matrix bytes apportioned by stored entries, plus "
+                             "the vector passes) over the FP64 batch's wall time; small "
+                             "systems run from L2"},
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_batched(a, rows, rp, ci, v, b, it64, x64)
@@ -668,10 +686,11 @@ def cpu_baseline_batched(a, rows, rp, ci, v, b, it64, x64):
     }
 
 
-def cg_iter_bytes(N, nnz):
-    """Algorithmic bytes of one fused CG + Jacobi iteration (FP64, int32):
-    the SpMV plus 88 N of vector traffic (update 64 N, p-update 24 N)."""
-    return spmv_bytes(N, nnz, 8) + 88 * N
+def cg_iter_bytes(A):
+    """Algorithmic bytes of one fused CG + Jacobi iteration on FP64 A: the
+    compulsory bytes of A's product on its layout plus 88 N of vector traffic
+    (update 64 N, p-update 24 N)."""
+    return A.layout_bytes(8) + 88 * A.nrows
 
 
 def bench_adjoint(a, fsx, P):
@@ -741,7 +760,7 @@ def bench_adjoint(a, fsx, P):
         f"gradient rel diff {refined['grad_rel_diff']:.2e}")
     iters = st["forward_iters"] + st["recompute_iters"] + st["adjoint_iters"]
     N, nnz = prob.N, prob.A.nnz
-    gbs = iters * cg_iter_bytes(N, nnz) / t_grad / 1e9
+    gbs = iters * cg_iter_bytes(prob.A) / t_grad / 1e9
     out = {
         "metric": f"BE heat {n}^3 x {a.heat_steps} steps: loss + adjoint gradient wrt log k",
         "value": round(t_grad, 4),
@@ -770,7 +789,8 @@ def bench_adjoint(a, fsx, P):
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "note": "every CG iteration of the forward, recompute and adjoint "
-                             "solves at the fused floor B_spmv + 88 N bytes, over the "
+                             "solves at the fused floor B_spmv + 88 N bytes (B_spmv: the "
+                             "compulsory bytes of the product's layout), over the "
                              "loss+gradient wall time"},
     }
     if not a.no_cpu_baseline:
@@ -1096,19 +1116,26 @@ def heat_stage_spmv(a, fsx, P, n, quiet=False):
     xd = ctx.vec(P.uniform(N, 7))
     y = ctx.zeros(N)
     ms = fsx_time(fsx, "fsx_time_spmv", ctx, A, xd, y, a.spmv_reps)
-    B = spmv_bytes(N, nnz, 8)
+    # compulsory bytes of the layout the product reads; the CSR figure (a
+    # column per nonzero, which the stencil and slice layouts never read)
+    # as a rate beside it
+    B = A.layout_bytes(8)
+    B_csr = spmv_bytes(N, nnz, 8)
     gbs = B / (ms * 1e-3) / 1e9
     plan = A.plan()
-    log(f"stage spmv {n}^2: {ms:.4f} ms, {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.1%}), plan {plan}")
+    log(f"stage spmv {n}^2: {ms:.4f} ms, {gbs:.0f} GB/s on layout bytes "
+        f"({gbs / HBM_PEAK_GBS:.1%}), CSR-equivalent {B_csr / (ms * 1e-3) / 1e9:.0f} GB/s, "
+        f"plan {plan}")
     if not quiet:
         # the copy kernel beside it calibrates the PMC byte counters
         copy_ms = fsx_time(fsx, "fsx_time_copy", ctx, None, xd, y, a.spmv_reps)
         log(f"copy kernel {N} doubles: {copy_ms:.4f} ms")
         print(json.dumps({"metric": "stage SpMV % HBM peak", "config": {
             "workload": f"heat2d stage matrix {n}^2", "rows": N, "nnz": nnz},
-            "avg_ms": round(ms, 5), "achieved": round(gbs, 1), "algorithmic_bytes": B}),
+            "avg_ms": round(ms, 5), "achieved": round(gbs, 1), "algorithmic_bytes": B,
+            "csr_equiv_bytes": B_csr}),
             flush=True)
-    return ms, gbs, B, plan, nnz
+    return ms, gbs, B, plan, nnz, B_csr
 
 
 def bench_heat(a, fsx, P):
@@ -1135,7 +1162,7 @@ def bench_heat(a, fsx, P):
     fs, gs = sum(tf) / len(tf), sum(tg) / len(tg)
     steps, rejects, solves, iters = fused[-1][:4]
 
-    ms, gbs, B, plan, nnz = heat_stage_spmv(a, fsx, P, n, quiet=True)
+    ms, gbs, B, plan, nnz, B_csr = heat_stage_spmv(a, fsx, P, n, quiet=True)
     traffic, traffic_src = None, None
     pmc = ROOT / "profiles" / "spmv_stage_heat_2046sq_pmc.json"
     if n == 2046 and pmc.exists():
@@ -1164,6 +1191,10 @@ def bench_heat(a, fsx, P):
                      f"{plan[0]}", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src, "algorithmic_bytes": B,
+                     "bytes_basis": "compulsory bytes of the product's layout "
+                                    "(fsx_csr_layout_bytes)",
+                     "csr_equiv_bytes": B_csr,
+                     "csr_equiv_gbs": round(B_csr / (ms * 1e-3) / 1e9, 1),
                      "avg_ms": round(ms, 5)},
     }
     if not a.no_cpu_baseline:
@@ -1257,13 +1288,14 @@ def bench_parcsr(a, fsx, P):
     # bytes the three launches must move: the diag blocks' SpMV (int32
     # indices), the ghost gather (colmap, x, ghost), the offd rows (row ids,
     # row pointers, columns, values, ghost values, y read and written)
-    B_diag = spmv_bytes(N, nnz_d, 8)
+    B_diag = Mp.diag_layout_bytes(8)  # compulsory bytes of its layout
+    B_diag_csr = spmv_bytes(N, nnz_d, 8)
     B_gather = ng * (4 + 8 + 8)
     nb = int(boundary.sum())
     B_offd = (2 * nb + 1) * 4 + nnz_o * (4 + 8 + 8) + nb * 16
     B = B_diag + B_gather + B_offd
     gbs = B / (ms * 1e-3) / 1e9
-    B_plain = spmv_bytes(N, nnz, 8)
+    B_plain = A.layout_bytes(8)  # the one-colour product on its layout
     traffic, traffic_src = None, None
     pmc = ROOT / "profiles" / "parcsr_spmv_256cube_8colours_pmc.json"
     if n == 256 and ncol == 8 and abs(a.sigma - 1.0) < 1e-12 and pmc.exists():
@@ -1296,6 +1328,11 @@ def bench_parcsr(a, fsx, P):
                      "algorithmic_bytes": B,
                      "bytes": {"diag_spmv": B_diag, "ghost_gather": B_gather,
                                "offd_rows": B_offd},
+                     "bytes_basis": "diag blocks: compulsory bytes of their layout "
+                                    "(fsx_csr_layout_bytes); ghost gather and offd rows: "
+                                    "CSR arrays they read and write",
+                     "csr_equiv_gbs": round((B_diag_csr + B_gather + B_offd) /
+                                            (ms * 1e-3) / 1e9, 1),
                      "avg_ms": round(ms, 5)},
     }
     if not a.no_cpu_baseline:

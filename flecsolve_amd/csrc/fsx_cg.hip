@@ -1814,6 +1814,63 @@ static char * noalloc_base(int device) {
 
 using namespace fsx;
 
+namespace fsx {
+// Average duration of `reps` back-to-back launches of the fused CG's K1 on A
+// (NV = 1: partials of w.p; NV = 2: of w.p and w.w, the single-reduction
+// inner CG), on the context stream.
+template<int NV>
+static void time_cg_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_vec * w,
+                         int reps, double * avg_ms) {
+	FSX_REQUIRE(ctx && A && avg_ms, "null argument");
+	check_vec(p, "p");
+	check_vec(w, "w");
+	check_same(p, w);
+	FSX_REQUIRE(A->nrows == A->ncols && (int64_t)p->n == A->nrows, "shape mismatch");
+	FSX_REQUIRE(p->dt == FSX_F32 || A->vt == FSX_F64, "FP64 vectors need an FP64 matrix");
+	FSX_REQUIRE(reps > 0, "reps must be > 0");
+	// a state whose `done` flag stays clear (alpha etc. are not read by K1)
+	dbuf stb(sizeof(cg_state));
+	FSX_HIP(hipMemsetAsync(stb.p, 0, sizeof(cg_state), ctx->stream));
+	double * part = ensure_partials(ctx, spmv_partials_size(A, NV));
+	const cg_state * st = (const cg_state *)stb.p;
+	auto launch = [&] {
+		by_matrix_type(A, [&](auto * vtag, auto * itag) {
+			using V = std::remove_pointer_t<decltype(vtag)>;
+			using I = std::remove_pointer_t<decltype(itag)>;
+			auto go = [&](auto * ttag) {
+				using T = std::remove_pointer_t<decltype(ttag)>;
+				using ACC = cg_acc_t<T, V>;
+				if constexpr (NV == 1) {
+					epi_cg_spmv<ACC, T> e{(T *)w->ptr(), part, st};
+					spmv_launch_typed<decltype(e), ACC, V, I, T>(ctx->stream, A,
+					                                             (const T *)p->ptr(), e);
+				}
+				else {
+					epi_cg_spmv2<ACC, T> e{(T *)w->ptr(), part, st};
+					spmv_launch_typed<decltype(e), ACC, V, I, T>(ctx->stream, A,
+					                                             (const T *)p->ptr(), e);
+				}
+			};
+			if (p->dt == FSX_F64) go((double *)nullptr);
+			else go((float *)nullptr);
+		});
+	};
+	launch(); // warm
+	hipEvent_t e0, e1;
+	FSX_HIP(hipEventCreate(&e0));
+	FSX_HIP(hipEventCreate(&e1));
+	FSX_HIP(hipEventRecord(e0, ctx->stream));
+	for (int i = 0; i < reps; ++i) launch();
+	FSX_HIP(hipEventRecord(e1, ctx->stream));
+	FSX_HIP(hipEventSynchronize(e1));
+	float ms = 0;
+	FSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	*avg_ms = (double)ms / reps;
+}
+} // namespace fsx
+
 extern "C" {
 
 int fsx_cg_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
@@ -1911,49 +1968,12 @@ int fsx_cg_solve_noalloc(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 
 int fsx_time_cg_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_vec * w,
                      int reps, double * avg_ms) {
-	return guard([&] {
-		FSX_REQUIRE(ctx && A && avg_ms, "null argument");
-		check_vec(p, "p");
-		check_vec(w, "w");
-		check_same(p, w);
-		FSX_REQUIRE(A->nrows == A->ncols && (int64_t)p->n == A->nrows, "shape mismatch");
-		FSX_REQUIRE(p->dt == FSX_F32 || A->vt == FSX_F64,
-		            "FP64 vectors need an FP64 matrix");
-		FSX_REQUIRE(reps > 0, "reps must be > 0");
-		// a state whose `done` flag stays clear (alpha etc. are not read by K1)
-		dbuf stb(sizeof(cg_state));
-		FSX_HIP(hipMemsetAsync(stb.p, 0, sizeof(cg_state), ctx->stream));
-		double * part = ensure_partials(ctx, spmv_partials_size(A, 1));
-		const cg_state * st = (const cg_state *)stb.p;
-		auto launch = [&] {
-			by_matrix_type(A, [&](auto * vtag, auto * itag) {
-				using V = std::remove_pointer_t<decltype(vtag)>;
-				using I = std::remove_pointer_t<decltype(itag)>;
-				auto go = [&](auto * ttag) {
-					using T = std::remove_pointer_t<decltype(ttag)>;
-					using ACC = cg_acc_t<T, V>;
-					epi_cg_spmv<ACC, T> e{(T *)w->ptr(), part, st};
-					spmv_launch_typed<decltype(e), ACC, V, I, T>(ctx->stream, A,
-					                                             (const T *)p->ptr(), e);
-				};
-				if (p->dt == FSX_F64) go((double *)nullptr);
-				else go((float *)nullptr);
-			});
-		};
-		launch(); // warm
-		hipEvent_t e0, e1;
-		FSX_HIP(hipEventCreate(&e0));
-		FSX_HIP(hipEventCreate(&e1));
-		FSX_HIP(hipEventRecord(e0, ctx->stream));
-		for (int i = 0; i < reps; ++i) launch();
-		FSX_HIP(hipEventRecord(e1, ctx->stream));
-		FSX_HIP(hipEventSynchronize(e1));
-		float ms = 0;
-		FSX_HIP(hipEventElapsedTime(&ms, e0, e1));
-		(void)hipEventDestroy(e0);
-		(void)hipEventDestroy(e1);
-		*avg_ms = (double)ms / reps;
-	});
+	return guard([&] { time_cg_spmv<1>(ctx, A, p, w, reps, avg_ms); });
+}
+
+int fsx_time_cg_spmv_sr(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_vec * w,
+                        int reps, double * avg_ms) {
+	return guard([&] { time_cg_spmv<2>(ctx, A, p, w, reps, avg_ms); });
 }
 
 int fsx_cg_history(fsx_ctx * ctx, double * hist, int cap, int * count) {

@@ -78,6 +78,7 @@ _SIGS = {
     "fsx_ctx_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "fsx_alloc_stats": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "fsx_time_cg_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
+    "fsx_time_cg_spmv_sr": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
     "fsx_csr_layout_bytes": (C.c_int, [_P, C.c_size_t, C.POINTER(C.c_uint64)]),
     "fsx_vec_create": (C.c_int, [_P, C.c_size_t, C.c_int, C.POINTER(_P)]),
     "fsx_vec_wrap": (C.c_int, [_P, _P, C.c_size_t, C.c_int, C.POINTER(_P)]),
@@ -776,6 +777,15 @@ class ParCSR:
         k, l = C.c_int(), C.c_int()
         check(lib().fsx_csr_plan(d, C.byref(k), C.byref(l)))
         return k.value, l.value
+
+    def diag_layout_bytes(self, x_bytes: int = 8) -> int:
+        """Compulsory bytes of one product of the diag blocks' matrix on its
+        layout (fsx_csr_layout_bytes)."""
+        d = C.c_void_p()
+        check(lib().fsx_parcsr_diag(self.h, C.byref(d)), "fsx_parcsr_diag")
+        b = C.c_uint64()
+        check(lib().fsx_csr_layout_bytes(d, x_bytes, C.byref(b)), "fsx_csr_layout_bytes")
+        return b.value
 
     def destroy(self):
         if self.h:

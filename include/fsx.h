@@ -595,6 +595,10 @@ int fsx_time_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * x,
  * solves use for A and p's dtype). */
 int fsx_time_cg_spmv(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_vec * w,
                      int reps, double * avg_ms);
+/* Same for the single-reduction CG's K1 (partials of w.p and w.w), which
+ * the refinement's unpreconditioned inner solves (jacobi = 2) launch. */
+int fsx_time_cg_spmv_sr(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * p, fsx_vec * w,
+                        int reps, double * avg_ms);
 /* Compulsory HBM bytes of one y = A x on the layout A's plan reads: every
  * stored value slot and index of that layout once, x read once and y written
  * once at x_bytes per element.  (For the slice-offset layout: value slots,

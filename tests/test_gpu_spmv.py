@@ -150,6 +150,21 @@ def test_stencil_layout_needs_symmetry(ctx):
     A = ctx.csr(rp, ci, v)
     assert A.plan() == (4, 0)
     exact(A, v)
+    # int64 indices: the same layout
+    A64i = ctx.csr(rp.astype(np.int64), ci.astype(np.int64), v)
+    assert A64i.plan() == (4, 0)
+    exact(A64i, v)
+    # an FP32 matrix starts on slice offsets; on request the stencil layout,
+    # the same bits
+    v32 = v.astype(np.float32)
+    F = ctx.csr(rp, ci, v32)
+    assert F.plan() == (3, 0)
+    x32, y32 = ctx.vec(x.astype(np.float32)), ctx.zeros(N, fsx.F32)
+    F.spmv(x32, y32)
+    y3 = y32.get()
+    F.set_plan(4)
+    F.spmv(x32, y32)
+    np.testing.assert_array_equal(y32.get(), y3)
     # one entry below the diagonal off by one ulp
     va = v.copy()
     va[lower[100]] = np.nextafter(va[lower[100]], 0.0)
