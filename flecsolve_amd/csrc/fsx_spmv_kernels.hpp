@@ -42,8 +42,14 @@
 //    of row r at a negative offset o is read from row r + o's stored entry at
 //    -o.  FP64 at 256^3: y = A x 0.227 -> 0.198 ms and the CG K1 0.232 ->
 //    0.200 ms against the slice-offset layout; the FP32 CG K1 measured
-//    0.123-0.130 ms against 0.119 there, so FP32 matrices stay on slice
-//    offsets.  Rows keep their CSR order: bit-identical to the serial loop.
+//    0.123-0.184 ms against 0.119-0.124 there across launch shapes, so FP32
+//    matrices stay on slice offsets.  Measured and not kept: mirroring only
+//    the offsets that are whole slices (-256, -65536; the -1 entry stored,
+//    so no mirror load straddles lanes): FP32 0.159 ms, FP64 0.206 ms;
+//    non-temporal loads for the last use of a value (mirror reads, the
+//    diagonal): FP32 0.157 ms, FP64 K1 0.214 ms; XCD-major block order:
+//    FP64 0.202 / 0.209 ms.  Rows keep their CSR order: bit-identical to the
+//    serial loop.
 //
 // Workgroups are dealt to XCDs round-robin by block id.  Measured: keeping
 // that default order is faster than remapping blocks to give each XCD a
