@@ -375,6 +375,7 @@ static bool stencil_analyze(fsx_ctx * ctx, int64_t nrows, const I * rp, const I 
 			// columns strictly ascending: the layout sums in offset order
 			if (e > rp[r] && ci[e] <= ci[e - 1]) return false;
 			const int64_t o = (int64_t)ci[e] - r;
+			if (o < INT32_MIN || o > INT32_MAX) return false; // offsets are stored as int32
 			if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
 				if ((int)offs.size() == kStencilMax) return false;
 				offs.push_back(o);
