@@ -303,6 +303,13 @@ def bench_diffusion(a, fsx, P):
         "copy_kernel_gbs": round(copy_gbs, 1),
         "traffic_source": "profiles/cg_k1_256cube_pmc.md" if traffic else None,
     }
+    # the rate of the measured HBM-side traffic over the live time: what the
+    # memory system delivered (beside `achieved`, the compulsory-byte rate)
+    for sub, ms in ((roofline, k1_ms), (roofline["fp64_spmv"], spmv_ms),
+                    (roofline["fp64_k1"], k164_ms)):
+        if sub.get("traffic"):
+            sub["traffic_gbs"] = round(gbs(sub["traffic"], ms), 1)
+            sub["traffic_frac"] = round(gbs(sub["traffic"], ms) / HBM_PEAK_GBS, 4)
     if a.spmv_only:
         print(json.dumps({"metric": "CG K1 / SpMV % HBM peak", "config": {
             "workload": f"diffusion3d_{n}^3", "rows": N, "nnz": int(nnz)},

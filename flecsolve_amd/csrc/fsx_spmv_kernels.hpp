@@ -48,7 +48,10 @@
 //    so no mirror load straddles lanes): FP32 0.159 ms, FP64 0.206 ms;
 //    non-temporal loads for the last use of a value (mirror reads, the
 //    diagonal): FP32 0.157 ms, FP64 K1 0.214 ms; XCD-major block order:
-//    FP64 0.202 / 0.209 ms.  Rows keep their CSR order: bit-identical to the
+//    FP64 0.202 / 0.209 ms; other shapes for the FP64 K1 (256- or
+//    1024-thread blocks one slice per wave, 256 or 512 threads striding over
+//    2-4 slices): 0.196-0.207 ms against 0.196 ms, steady state, for a plain
+//    product of 0.192 ms.  Rows keep their CSR order: bit-identical to the
 //    serial loop.
 //
 // Workgroups are dealt to XCDs round-robin by block id.  Measured: keeping
