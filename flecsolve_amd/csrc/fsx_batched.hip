@@ -543,6 +543,8 @@ struct batch_outputs {
 
 batch_args base_args(fsx_batch * B, const fsx_csr * A32) {
 	const fsx_csr * A = B->A;
+	ensure_dg_fresh(B->ctx, A);
+	if (A32) ensure_dg_fresh(B->ctx, A32);
 	batch_args a{};
 	a.nsys = B->nsys;
 	a.rows = (const int64_t *)B->rows->p;

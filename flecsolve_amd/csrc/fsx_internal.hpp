@@ -180,9 +180,10 @@ struct csr_index {
 	// Lazily built transpose structure for A^T x (adjoint).
 	std::unique_ptr<fsx::dbuf> t_rowptr, t_colind, t_perm;
 	// every row stores its diagonal entry: -1 not yet checked, 0 no, 1 yes;
-	// diag_pos (int64 per row, -1 when absent) once checked
+	// diag_pos (int64 per row, -1 when absent) once checked; diag_bits: one
+	// bit per entry, set at each diag_pos entry
 	int diag_full = -1;
-	std::unique_ptr<fsx::dbuf> diag_pos;
+	std::unique_ptr<fsx::dbuf> diag_pos, diag_bits;
 };
 
 struct fsx_csr {
@@ -198,6 +199,9 @@ struct fsx_csr {
 	// bitwise symmetric (1), not (0), or not yet checked (-1)
 	std::unique_ptr<fsx::dbuf> st_vals;
 	int st_sym = -1;
+	// dg_vals lag the CSR values (a rewrite served by the stencil layout
+	// skipped them); rebuilt by ensure_dg_fresh before any use
+	bool dg_stale = false;
 	// SpMV row mapping: 0 LDS-staged row blocks, 1 vector per row, 2 sliced
 	// rows, 3 slice offsets, 4 symmetric stencil
 	int plan_kind = 0;
@@ -217,6 +221,9 @@ namespace fsx {
 
 void ctx_retain(fsx_ctx * c);
 void ctx_release(fsx_ctx * c);
+// Rebuild A's slice-offset values from its CSR values if they lag
+// (stream-ordered; A's layouts are caches of its CSR values).
+void ensure_dg_fresh(fsx_ctx * ctx, const fsx_csr * A);
 
 inline fsx_ctx * ctx_of(const fsx_vec * v) { return v->ctx; }
 void check_vec(const fsx_vec * v, const char * name);

@@ -565,9 +565,17 @@ bool ensure_stencil(fsx_ctx * ctx, fsx_csr * A, const void * host_rowptr,
 	return true;
 }
 
+void ensure_dg_fresh(fsx_ctx * ctx, const fsx_csr * A) {
+	if (!A->dg_stale) return;
+	fsx_csr * M = const_cast<fsx_csr *>(A);
+	if (M->idx->dg_ptr && M->dg_vals) sdia_fill(ctx, M);
+	M->dg_stale = false;
+}
+
 void refresh_sliced_values(fsx_ctx * ctx, fsx_csr * A) {
 	++A->vals_version;
 	if (A->idx->dg_ptr && A->dg_vals) sdia_fill(ctx, A);
+	A->dg_stale = false;
 	if (A->idx->st_state == 1 && A->st_vals) {
 		// refilled in place; values that are no longer symmetric send the
 		// products to the slice-offset layout
@@ -774,21 +782,32 @@ __global__ void diag_inv_kernel(int64_t nrows, const I * __restrict__ rp,
 }
 
 // a = beta f on every stored entry (coalesced; the diagonal is fixed next)
-__global__ void scale_values_kernel(size_t nnz, const double * __restrict__ f, double beta,
-                                    double * __restrict__ a) {
+// a = alpha I + beta f over the CSR values: alpha + beta f at the entries
+// whose bit is set in `diag` (each row's diagonal position), beta f
+// elsewhere; one coalesced pass
+__global__ void shift_scale_csr_kernel(size_t nnz, const uint32_t * __restrict__ diag,
+                                       const double * __restrict__ f, double alpha, double beta,
+                                       double * __restrict__ a) {
 	const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (e < nnz) a[e] = beta * f[e];
+	if (e >= nnz) return;
+	const double v = beta * f[e];
+	a[e] = ((diag[e >> 5] >> (e & 31)) & 1u) ? alpha + v : v;
 }
 
-// a at each row's diagonal position = alpha + beta f (rows without one: -1)
-__global__ void diag_shift_kernel(int64_t nrows, const int64_t * __restrict__ dpos,
-                                  const double * __restrict__ f, double alpha, double beta,
-                                  double * __restrict__ a) {
+__global__ void zero_u32_kernel(size_t n, uint32_t * __restrict__ w) {
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) w[i] = 0u;
+}
+
+// the bit of each row's diagonal position
+__global__ void diag_bits_kernel(int64_t nrows, const int64_t * __restrict__ dpos,
+                                 uint32_t * __restrict__ bits) {
 	const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (r >= nrows) return;
 	const int64_t e = dpos[r];
-	if (e >= 0) a[e] = alpha + beta * f[e];
+	if (e >= 0) atomicOr(&bits[e >> 5], 1u << (e & 31));
 }
+
 
 // The same on the slice-offset layout shared by f and a: slot k = 64 q +
 // lane holds row (slice, lane)'s entry at offset off[q] when the lane's mask
@@ -973,6 +992,7 @@ int fsx_csr_convert(fsx_ctx * ctx, const fsx_csr * A, fsx_dtype vt,
                     fsx_csr ** out) {
 	return guard([&] {
 		FSX_REQUIRE(ctx && A && out, "null argument");
+		ensure_dg_fresh(ctx, A); // B copies A's layouts
 		auto B = std::make_unique<fsx_csr>();
 		B->ctx = ctx;
 		B->nrows = A->nrows;
@@ -1071,6 +1091,7 @@ int fsx_csr_set_plan(fsx_csr * A, int kind, int lanes) {
 			FSX_REQUIRE(A->nrows > 0, "sliced rows need a non-empty matrix");
 			ensure_sliced(A->ctx, A, nullptr);
 		}
+		if (kind == 3) ensure_dg_fresh(A->ctx, A);
 		if (kind == 3 && !(A->idx->dg_ptr && A->dg_vals)) {
 			FSX_REQUIRE(A->nrows > 0, "slice offsets need a non-empty matrix");
 			fsx_ctx * ctx = A->ctx;
@@ -1269,25 +1290,39 @@ int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double b
 			ctx->host_syncs++;
 			ix.diag_full = missing ? 0 : 1;
 			ix.diag_pos = std::move(pos);
+			// the diagonal positions as one bit per entry
+			const size_t words = (size_t)F->nnz / 32 + 1;
+			auto bits = std::make_unique<dbuf>(words * sizeof(uint32_t));
+			hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
+			                   ctx->stream, words, (uint32_t *)bits->p);
+			hipLaunchKernelGGL(diag_bits_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+			                   (const int64_t *)ix.diag_pos->p, (uint32_t *)bits->p);
+			FSX_HIP(hipGetLastError());
+			ix.diag_bits = std::move(bits);
 		}
 		FSX_REQUIRE(ix.diag_full == 1 || alpha == 0.0,
 		            "shift_scale: a row of F has no stored diagonal entry");
 		const size_t nnz = (size_t)F->nnz;
 		if (nnz) {
-			hipLaunchKernelGGL(scale_values_kernel, dim3((unsigned)((nnz + 255) / 256)), dim3(256),
-			                   0, ctx->stream, nnz, (const double *)F->vals->p, beta,
-			                   (double *)A->vals->p);
-			hipLaunchKernelGGL(diag_shift_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
-			                   (const int64_t *)ix.diag_pos->p, (const double *)F->vals->p,
-			                   alpha, beta, (double *)A->vals->p);
+			hipLaunchKernelGGL(shift_scale_csr_kernel, dim3((unsigned)((nnz + 255) / 256)),
+			                   dim3(256), 0, ctx->stream, nnz, (const uint32_t *)ix.diag_bits->p,
+			                   (const double *)F->vals->p, alpha, beta, (double *)A->vals->p);
 			FSX_HIP(hipGetLastError());
-			ctx->launches += 2;
+			ctx->launches += 1;
 		}
-		// the SpMV layouts: the slice-offset one rewritten in place from F's,
-		// the sliced one re-copied from the CSR values
+		// the SpMV layouts: the stencil one and the slice-offset one rewritten
+		// in place from F's, the sliced one re-copied from the CSR values.  A
+		// matrix whose products run on the stencil layout (and stay there:
+		// F's stencil values exist, so they are symmetric) leaves its
+		// slice-offset values behind, rebuilt when next needed.
 		++A->vals_version;
+		const bool st_serves = A->plan_kind == 4 && ix.st_state == 1 && A->st_vals && F->st_vals;
 		if (ix.dg_ptr && A->dg_vals) {
-			if (F->dg_vals && ix.dg_nnz > 0) {
+			if (st_serves)
+				A->dg_stale = true;
+			else if (F->dg_vals && ix.dg_nnz > 0) {
+				ensure_dg_fresh(ctx, F);
+				A->dg_stale = false;
 				const int64_t t = ix.dg_nnz;
 				hipLaunchKernelGGL(shift_scale_sdia_kernel, dim3((unsigned)((t + 255) / 256)),
 				                   dim3(256), 0, ctx->stream, t, (const uint64_t *)ix.dg_mask->p,
@@ -1296,8 +1331,10 @@ int fsx_csr_shift_scale(fsx_ctx * ctx, const fsx_csr * F, double alpha, double b
 				FSX_HIP(hipGetLastError());
 				ctx->launches++;
 			}
-			else
+			else {
 				sdia_fill(ctx, A);
+				A->dg_stale = false;
+			}
 		}
 		if (ix.st_state == 1 && A->st_vals) {
 			if (F->st_vals) {

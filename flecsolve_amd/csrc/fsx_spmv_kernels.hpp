@@ -595,6 +595,7 @@ void spmv_launch_gather(hipStream_t st, const fsx_csr * A, const G & x,
 		}
 	}
 	else if (A->plan_kind == 3) {
+		FSX_REQUIRE(!A->dg_stale, "slice-offset values lag the CSR values");
 		auto go = [&](auto bt_c, unsigned g) {
 			constexpr int BT = decltype(bt_c)::value;
 			hipLaunchKernelGGL((spmv_sdia_kernel<Epi, ACC, V, G, BT>), dim3(g), dim3(BT), 0,

@@ -13,8 +13,10 @@
     another order than the oracle's);
   * the explicit heat example (RK23, explicit.cfg): the oracle's steps and
     rejections, solution within 1e-10;
-  * fsx_csr_shift_scale: I - gamma F bit for bit on every SpMV layout, and
-    the missing-diagonal error.
+  * fsx_csr_shift_scale: I - gamma F bit for bit on every SpMV layout, the
+    slice-offset copy left stale by stencil-layout rewrites rebuilt before a
+    plan switch, a copy or a rewrite that reads it, and the missing-diagonal
+    error.
 """
 import subprocess
 from pathlib import Path
@@ -121,6 +123,24 @@ def test_shift_scale(ctx):
             np.testing.assert_array_equal(A.download()[2], S.vals)
             A.spmv(xd, y)
             np.testing.assert_array_equal(y.get(), S.spmv(x))
+    # rewrites served by the stencil layout leave the slice-offset values
+    # behind: a product, a rewrite or a copy that reads them rebuilds them
+    A.set_plan(4)
+    A.shift_scale(F, 1.0, -0.5)
+    A.shift_scale(F, 1.0, -0.125)
+    op.set_scaling(0.125)
+    S = op.stage()
+    T = F.convert(fsx.F64)
+    T.set_plan(3)
+    T.shift_scale(A, 0.0, 1.0)
+    C = A.convert(fsx.F64)
+    C.set_plan(3)
+    A.set_plan(3)
+    for M in (T, C, A):
+        assert M.plan() == (3, 0)
+        np.testing.assert_array_equal(M.download()[2], S.vals)
+        M.spmv(xd, y)
+        np.testing.assert_array_equal(y.get(), S.spmv(x))
     # a row without its diagonal entry: alpha must be 0
     M = ctx.csr(np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.array([1.0, 2.0]))
     Mc = M.convert(fsx.F64)
