@@ -74,7 +74,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--sigma", type=float, default=1.0)
-    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--inner-rtol", type=float, default=1e-4,
+                    help="restarted refinement: each inner solve's tolerance")
+    ap.add_argument("--segment-rtol", type=float, default=1e-2,
+                    help="residual replacement: reduction of the FP32 recursive "
+                         "residual between two FP64 replacements")
+    ap.add_argument("--ir-method", choices=["rr", "restart"], default="rr",
+                    help="the timed mixed-precision solve: rr one FP32 CG with FP64 "
+                         "residual replacement (fsx_ir_cg_rr_solve), restart the "
+                         "restarted refinement (fsx_ir_cg_solve); the other is run once "
+                         "beside it")
+    ap.add_argument("--max-segments", type=int, default=100)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cpu-sample-iters", type=int, default=10,
                     help="iterations of the one-thread oracle sample")
@@ -118,6 +128,8 @@ def main():
     ap.add_argument("--heat-spmv-only", action="store_true",
                     help="heat workload: time the stage-matrix SpMV only (profiling runs)")
     a = ap.parse_args()
+    if a.ir_method == "rr" and a.ir_jacobi != 2:
+        ap.error("--ir-method rr runs on the symmetrically scaled FP32 matrix (--ir-jacobi 2)")
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the Krylov path is built and "
@@ -180,15 +192,32 @@ def solve_fp64(fsx, ctx, A64, dinv, bd, x, maxiter, rtol=1e-10):
     return info, time.perf_counter() - t
 
 
-def solve_ir(fsx, ctx, A64, A32, bd, x, a):
-    inner = fsx.Settings(maxiter=a.maxiter, rtol=a.inner_rtol)
+def solve_ir(fsx, ctx, A64, A32, bd, x, a, method=None):
+    """The mixed-precision solve from x0 = 0: residual replacement (rr) or
+    the restarted refinement (restart)."""
+    method = method or a.ir_method
     fsx.check(fsx.lib().fsx_zero(ctx.h, x.h))  # x0 = 0
     ctx.sync()
     t = time.perf_counter()
-    ii = fsx.ir_cg_solve(A64, A32, bd, x, 1e-10, inner, jacobi=a.ir_jacobi,
-                         max_outer=a.max_outer)
+    if method == "rr":
+        inner = fsx.Settings(maxiter=a.maxiter, rtol=a.segment_rtol)
+        ii = fsx.ir_cg_rr_solve(A64, A32, bd, x, 1e-10, inner, max_segments=a.max_segments)
+    else:
+        inner = fsx.Settings(maxiter=a.maxiter, rtol=a.inner_rtol)
+        ii = fsx.ir_cg_solve(A64, A32, bd, x, 1e-10, inner, jacobi=a.ir_jacobi,
+                             max_outer=a.max_outer)
     ctx.sync()
     return ii, time.perf_counter() - t
+
+
+IR_METHODS = {
+    "rr": "one FP32 CG on the symmetrically Jacobi-scaled FP32 matrix, its recursive "
+          "residual replaced by the true FP64 residual (x updated in FP64) after each "
+          "segment of --segment-rtol reduction, the search direction kept "
+          "(fsx_ir_cg_rr_solve)",
+    "restart": "restarted refinement: FP64 residual, FP32 CG correction solve to "
+               "--inner-rtol from zero, FP64 update (fsx_ir_cg_solve)",
+}
 
 
 def true_rel_res(fsx, ctx, A64, bd, x):
@@ -345,6 +374,20 @@ def bench_diffusion(a, fsx, P):
     xir = x.get()
     rel_ir = true_rel_res(fsx, ctx, A64, bd, x)
     roofline["share_of_step"] = round(k1_ms * 1e-3 * ii.inner_iters_total / ir_s, 3)
+    # the other mixed-precision method once (after a warm run), beside it
+    other = "restart" if a.ir_method == "rr" else "rr"
+    if a.ir_jacobi == 2:
+        solve_ir(fsx, ctx, A64, A32, bd, x, a, other)
+        io_, dto = solve_ir(fsx, ctx, A64, A32, bd, x, a, other)
+        log(f"{other}: {dto:.3f} s outer {io_.outer_iters} inner {io_.inner_iters_total} "
+            f"rel res {io_.rel_res_final:.3e}")
+        other_block = {"method": IR_METHODS[other], "seconds": round(dto, 4),
+                       "outer_iters": io_.outer_iters,
+                       "inner_iters_total": io_.inner_iters_total,
+                       "true_rel_residual": true_rel_res(fsx, ctx, A64, bd, x),
+                       "speedup_vs_fp64_cg": round(cg_s / dto, 3)}
+    else:
+        other_block = None
 
     # ---- the same FP64 CG + Jacobi through the generic sequence -----------
     # (the reference's loop over per-operation kernels, three blocking
@@ -393,9 +436,12 @@ def bench_diffusion(a, fsx, P):
             "workload": f"diffusion3d_{n}^3_lognormal_sigma{a.sigma:g}_rtol1e-10",
             "rows": N, "nnz": int(nnz), "index": "int32",
             "preconditioner": "jacobi",
-            "ir": {"inner_rtol": a.inner_rtol,
+            "ir": {"method": IR_METHODS[a.ir_method],
+                   "segment_rtol" if a.ir_method == "rr" else "inner_rtol":
+                       a.segment_rtol if a.ir_method == "rr" else a.inner_rtol,
                    "inner_preconditioning": ("symmetric Jacobi scaling of the FP32 matrix"
-                                             if a.ir_jacobi == 2 else "Jacobi"),
+                                             if a.ir_jacobi == 2 or a.ir_method == "rr"
+                                             else "Jacobi"),
                    "outer_iters": ii.outer_iters,
                    "inner_iters_total": ii.inner_iters_total,
                    "true_rel_residual": rel_ir,
@@ -403,6 +449,7 @@ def bench_diffusion(a, fsx, P):
             "fp64_cg": {"seconds": round(cg_s, 4), "iters": info64.iters,
                         "true_rel_residual": rel64},
             "ir_speedup_vs_fp64_cg": round(cg_s / ir_s, 3),
+            "ir_" + other: other_block,
             "generic_cg": generic,
         },
         "roofline": roofline,
@@ -457,6 +504,17 @@ def bench_sigma25(a, fsx, P, ctx, n):
             f"inner {ii.inner_iters_total}")
     relir = true_rel_res(fsx, ctx, A64, bd, x)
     xir = x.get()
+    other = "restart" if a.ir_method == "rr" else "rr"
+    other_block = None
+    if a.ir_jacobi == 2:
+        io_, dto = solve_ir(fsx, ctx, A64, A32, bd, x, a, other)
+        log(f"sigma 2.5 {other}: {dto:.3f} s outer {io_.outer_iters} "
+            f"inner {io_.inner_iters_total}")
+        other_block = {"method": IR_METHODS[other], "seconds": round(dto, 4),
+                       "outer_iters": io_.outer_iters,
+                       "inner_iters_total": io_.inner_iters_total,
+                       "true_rel_residual": true_rel_res(fsx, ctx, A64, bd, x),
+                       "speedup_vs_fp64_cg": round(cg_s[-1] / dto, 3)}
     info_t, _ = solve_fp64(fsx, ctx, A64, dinv, bd, x, 8 * a.maxiter, rtol=1e-13)
     tight = x.get()
     return {
@@ -468,6 +526,7 @@ def bench_sigma25(a, fsx, P, ctx, n):
                "true_rel_residual": relir,
                "rel_err_vs_fp64_cg": rel_err(xir, x64)},
         "ir_speedup_vs_fp64_cg": round(cg_s[-1] / ir_s[-1], 3),
+        "ir_" + other: other_block,
         "forward_error_vs_tight_solve": {"fp64_cg": rel_err(x64, tight),
                                          "ir": rel_err(xir, tight),
                                          "tight_iters": info_t.iters},

@@ -414,6 +414,50 @@ __global__ void ir_scale_kernel(size_t n, const double * __restrict__ r,
 	if (i < n) r32[i] = (float)(r[i] * inv);
 }
 
+// ---- FP32 CG with residual replacement (fsx_ir_cg_rr_solve) ---------------
+//
+// One FP32 CG on the scaled system S A S y = S r0 / c (c = |r0|) whose
+// recursive residual rh is replaced, at the end of each segment, by the true
+// one: x += c S y in FP64, r = b - A x in FP64, rh <- (float)(S r / c), and
+// p += rh_new - rh_old, so that p = rh_new + beta p_prev as if the new
+// residual had been the recursive one (residual replacement, van der Vorst
+// and Ye, SIAM J. Sci. Comput. 22 (2000) 835-852).  Unlike a restarted
+// refinement the search direction survives, so the Krylov space built so
+// far is not thrown away.
+
+// FIRST: rh = (float)(S r / c), p = rh, y = 0; otherwise the replacement.
+// Partials of |rh|^2 per block, in the fused update's block layout.
+template<bool FIRST>
+__global__ __launch_bounds__(kUpdThreads) void rr_replace_kernel(
+	size_t n, const double * __restrict__ r, const double * __restrict__ sc, double inv_c,
+	float * __restrict__ rh, float * __restrict__ p, float * __restrict__ y,
+	double * partials) {
+	const unsigned blk = blockIdx.x;
+	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
+	double rr = 0.0;
+#pragma unroll
+	for (int j = 0; j < kUpdPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			const float v = (float)(sc[i] * (r[i] * inv_c));
+			if constexpr (FIRST) p[i] = v;
+			else p[i] = (float)((double)p[i] + ((double)v - (double)rh[i]));
+			rh[i] = v;
+			y[i] = 0.0f;
+			rr += (double)v * (double)v;
+		}
+	}
+	double d[1] = {rr};
+	store_block_partials<1>(d, partials, blk);
+}
+
+// x += c (S y) in FP64
+__global__ void rr_update_kernel(size_t n, double * __restrict__ x, const float * __restrict__ y,
+                                 const double * __restrict__ sc, double c) {
+	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n) x[i] = x[i] + c * (sc[i] * (double)y[i]);
+}
+
 // What a cached graph's launches touch, for the check before each replay:
 // every device range it captured, with its size at capture, and the residual
 // history it writes (one entry per iteration up to the solve's maxiter).
@@ -2103,6 +2147,146 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
 			rnorm = residual();
 		}
 		info->outer_iters = outer;
+		info->rel_res_final = rnorm / b_norm;
+	});
+}
+
+int fsx_ir_cg_rr_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
+                       const fsx_vec * b, fsx_vec * x, double rtol, int max_segments,
+                       const fsx_settings * inner, fsx_ir_info * info) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A64 && A32 && inner && info, "null argument");
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE(b->dt == FSX_F64, "RR: b and x are FP64");
+		FSX_REQUIRE(A64->vt == FSX_F64 && A32->vt == FSX_F32,
+		            "RR: needs FP64 and FP32 matrices");
+		FSX_REQUIRE(A64->nrows == A64->ncols && A32->nrows == A64->nrows &&
+		                A32->ncols == A64->ncols && (int64_t)b->n == A64->nrows,
+		            "RR: shape mismatch");
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "RR: b and x must differ");
+		FSX_REQUIRE(A32->idx == A64->idx,
+		            "RR: needs A32 = fsx_csr_convert(A64, FSX_F32) holding "
+		            "fsx_csr_set_values_f32_sym_scaled values");
+		FSX_REQUIRE(inner->rtol > 0.0 && inner->rtol < 1.0,
+		            "RR: the segment reduction inner->rtol must lie in (0, 1)");
+		FSX_REQUIRE(inner->maxiter >= 1, "RR: inner->maxiter must be at least 1");
+		FSX_REQUIRE(max_segments >= 0, "RR: max_segments must not be negative");
+		std::memset(info, 0, sizeof(*info));
+		info->status = FSX_UNKNOWN;
+		const size_t n = b->n;
+		fsx_vec * r = pooled(ctx, n, FSX_F64, 10);
+		fsx_vec * y = pooled(ctx, n, FSX_F32, 12);
+		fsx_vec * sc = pooled(ctx, n, FSX_F64, 19);
+		if (n) {
+			// s = D^{-1/2} of A64 (the scaling A32's values were written with)
+			int * flag = (int *)(ctx->ws.scalars + 9);
+			FSX_HIP(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+			sym_scale_vector(ctx, A64, (double *)sc->ptr(), flag);
+			int bad = 0;
+			FSX_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+			FSX_HIP(hipStreamSynchronize(ctx->stream));
+			ctx->host_syncs++;
+			FSX_REQUIRE(!bad, "RR: a diagonal entry of A64 is missing or not positive");
+		}
+		const double b_norm2 = reduce_host(ctx, RED_L2SQ, b, nullptr);
+		double b_norm = std::sqrt(b_norm2);
+		info->rhs_norm = b_norm;
+		if (b_norm == 0.0) b_norm = 1.0;
+		double * rn2 = ctx->ws.scalars + 8;
+		// partials for the FP64 residual and the segments, grown once before
+		// any buffer address is taken
+		ensure_partials(ctx, std::max(spmv_partials_size(A64, 1), cg_partials_need(A32, n, true)));
+		const cg_work iwk = pooled_work(ctx, A32, n, FSX_F32, false, 14, inner->maxiter, true);
+		auto residual = [&]() {
+			by_matrix_type(A64, [&](auto * vtag, auto * itag) {
+				using V = std::remove_pointer_t<decltype(vtag)>;
+				using I = std::remove_pointer_t<decltype(itag)>;
+				epi_res_norm<double, double> e{(double *)r->ptr(), (const double *)b->ptr(),
+				                               ctx->ws.partials};
+				spmv_launch_typed<decltype(e), double, V, I, double>(
+					ctx->stream, A64, (const double *)x->ptr(), e);
+			});
+			spmv_finalize<1>(ctx->stream, A64, ctx->ws.partials, fin_store<1>{rn2});
+			FSX_HIP(hipGetLastError());
+			ctx->launches += 2;
+			return std::sqrt(fetch_scalar(ctx, rn2));
+		};
+		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+		const unsigned g2 = (unsigned)((n + per - 1) / per);
+		const unsigned g = (unsigned)((n + 255) / 256);
+		double * rho_slot = iwk.slots;
+		// rh <- (float)(S r / c) (FIRST: p = rh) and |rh|^2
+		auto replace = [&](bool first, double c) {
+			if (first)
+				hipLaunchKernelGGL((rr_replace_kernel<true>), dim3(g2), dim3(kUpdThreads), 0,
+				                   ctx->stream, n, (const double *)r->ptr(),
+				                   (const double *)sc->ptr(), 1.0 / c, (float *)iwk.r,
+				                   (float *)iwk.p, (float *)y->ptr(), iwk.part);
+			else
+				hipLaunchKernelGGL((rr_replace_kernel<false>), dim3(g2), dim3(kUpdThreads), 0,
+				                   ctx->stream, n, (const double *)r->ptr(),
+				                   (const double *)sc->ptr(), 1.0 / c, (float *)iwk.r,
+				                   (float *)iwk.p, (float *)y->ptr(), iwk.part);
+			hipLaunchKernelGGL((finalize_kernel<1, fin_store<1>>), dim3(1), dim3(kFinThreads), 0,
+			                   ctx->stream, (const double *)iwk.part, g2,
+			                   fin_store<1>{rho_slot});
+			FSX_HIP(hipGetLastError());
+			ctx->launches += 2;
+			return fetch_scalar(ctx, rho_slot);
+		};
+		double rnorm = n ? residual() : 0.0;
+		int seg = 0;
+		if (rnorm / b_norm <= rtol || rnorm == 0.0)
+			info->status = FSX_CONVERGED_RTOL;
+		else {
+			const double c = rnorm;
+			double rho = replace(true, c);
+			cg_bufs B{A32, y->ptr(), iwk.r, iwk.r, iwk.p, iwk.w, nullptr, iwk.part,
+			          iwk.part + spmv_partials_size(A32, 2), n};
+			for (;;) {
+				if (seg >= max_segments) {
+					info->status = FSX_DIVERGED_ITERS;
+					break;
+				}
+				if (!(rho > 0.0)) {
+					// the scaled residual vanished in FP32 while the FP64 one
+					// did not: no further progress is possible
+					info->status = FSX_DIVERGED_BREAKDOWN;
+					break;
+				}
+				// the segment ends after a reduction of inner->rtol, or once
+				// the true residual is estimated below half the target (the
+				// ratio |r| / |rh| of the last replacement)
+				const double rh_norm = std::sqrt(rho);
+				cg_state st{};
+				st.rho1 = rho;
+				st.rho0 = rho;
+				st.res = rh_norm;
+				st.tol = std::max(inner->rtol * rh_norm, 0.5 * rtol * b_norm * rh_norm / rnorm);
+				st.maxiter = inner->maxiter;
+				by_matrix_type(A32, [&](auto * vtag, auto * itag) {
+					using V = std::remove_pointer_t<decltype(vtag)>;
+					using I = std::remove_pointer_t<decltype(itag)>;
+					run_cg_loop<float, V, I>(ctx, B, iwk, st, nullptr, nullptr, nullptr, nullptr);
+				});
+				info->inner_iters_total += st.iter;
+				++seg;
+				hipLaunchKernelGGL(rr_update_kernel, dim3(g), dim3(256), 0, ctx->stream, n,
+				                   (double *)x->ptr(), (const float *)y->ptr(),
+				                   (const double *)sc->ptr(), c);
+				FSX_HIP(hipGetLastError());
+				ctx->launches++;
+				rnorm = residual();
+				if (rnorm / b_norm <= rtol || rnorm == 0.0) {
+					info->status = FSX_CONVERGED_RTOL;
+					break;
+				}
+				rho = replace(false, c);
+			}
+		}
+		info->outer_iters = seg;
 		info->rel_res_final = rnorm / b_norm;
 	});
 }

@@ -167,6 +167,8 @@ _SIGS = {
     "fsx_ir_cg_solve": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double,
                                   C.c_int, C.POINTER(Settings),
                                   C.POINTER(IRInfo)]),
+    "fsx_ir_cg_rr_solve": (C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_int,
+                                     C.POINTER(Settings), C.POINTER(IRInfo)]),
     "fsx_krylov_cg": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P,
                                 C.POINTER(Settings), C.POINTER(Info),
                                 C.POINTER(C.c_double), C.c_int]),
@@ -616,6 +618,18 @@ def ir_cg_solve(A64: CSR, A32: CSR, b: Vector, x: Vector, rtol_outer: float,
     check(lib().fsx_ir_cg_solve(A64.ctx.h, A64.h, A32.h, int(jacobi), b.h, x.h,
                                 rtol_outer, max_outer, C.byref(inner),
                                 C.byref(info)), "fsx_ir_cg_solve")
+    return info
+
+
+def ir_cg_rr_solve(A64: CSR, A32: CSR, b: Vector, x: Vector, rtol: float,
+                   inner: Settings, max_segments: int = 100) -> IRInfo:
+    """FP32 CG on the symmetrically scaled A32 (A32.set_values_sym_scaled(A64)
+    first) with FP64 residual replacement after each segment of inner.rtol
+    reduction; x is the initial guess."""
+    info = IRInfo()
+    check(lib().fsx_ir_cg_rr_solve(A64.ctx.h, A64.h, A32.h, b.h, x.h, rtol,
+                                   int(max_segments), C.byref(inner), C.byref(info)),
+          "fsx_ir_cg_rr_solve")
     return info
 
 

@@ -434,6 +434,27 @@ int fsx_ir_cg_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
                     double rtol_outer, int max_outer,
                     const fsx_settings * inner, fsx_ir_info * info);
 
+/* Mixed precision by residual replacement (new, no reference counterpart):
+ * one FP32 CG on the symmetrically scaled system (A32 as for
+ * fsx_ir_cg_solve with jacobi = 2: S A64 S in FP32 on A64's pattern) whose
+ * iteration runs in segments.  A segment ends once the recursive residual
+ * has fallen by `inner->rtol` (or is estimated below half the target, or
+ * after inner->maxiter iterations); then x += |r0| S y in FP64, the true
+ * FP64 residual r = b - A64 x is formed and checked against rtol |b|, and
+ * it replaces the FP32 recursive residual, the search direction corrected
+ * by the difference (p += rh_new - rh_old) and kept, so the CG continues
+ * instead of restarting (residual replacement: H. A. van der Vorst and
+ * Q. Ye, SIAM J. Sci. Comput. 22 (2000) 835-852).  x is the initial
+ * guess.  info: outer_iters = segments, inner_iters_total = FP32
+ * iterations, status converged_rtol / diverged_iters (max_segments
+ * reached) / diverged_breakdown (the FP32 residual vanished while the
+ * FP64 one did not), rel_res_final = true |b - A64 x| / |b|.  The same
+ * pooled work vectors as fsx_ir_cg_solve, so calls on one context do not
+ * overlap. */
+int fsx_ir_cg_rr_solve(fsx_ctx * ctx, const fsx_csr * A64, const fsx_csr * A32,
+                       const fsx_vec * b, fsx_vec * x, double rtol, int max_segments,
+                       const fsx_settings * inner, fsx_ir_info * info);
+
 /* Reference-shaped solves: the C++ op::cg / op::fcg / op::gmres /
  * op::bicgstab templates (include/flecsolve/solvers/) instantiated over the
  * device vector type and called as the reference's callers call them.

@@ -1,0 +1,68 @@
+"""The CPU model of the FP32 CG with FP64 residual replacement
+(tests/rr_model.py, the checker of fsx_ir_cg_rr_solve), pinned on the CPU:
+
+  * run in FP64 its recurrences are CG with Jacobi preconditioning: after 30
+    iterations of one segment its x equals the oracle's CG + Dinv after 30
+    iterations (cg.hh:43-139) to 1e-9 -- a wrong alpha, beta, scaling S or
+    update changes x far beyond that;
+  * run in FP32 it reaches a true FP64 residual <= 1e-10 within a few
+    iterations of the oracle's FP64 CG + Dinv, and restarting the FP32 CG at
+    each replacement (keep_p=False) takes at least 20% more iterations: the
+    retained search direction is what the device test pins through the
+    iteration count;
+  * the segment bookkeeping: one segment with max_segments = 1 stops with
+    status 4, and a start at the solution takes no segment.
+"""
+import numpy as np
+import pytest
+
+import rr_model as M
+from flecsolve_amd import problems as P
+from oracle import oracle as O
+
+
+def problem(n, sigma, seed=3001):
+    k = P.lognormal_field(n, sigma, 8.0, seed)
+    rp, ci, v = P.diffusion3d(n, k)
+    return rp, ci, v, P.uniform(n ** 3, seed + 1)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_fp64_model_is_jacobi_cg():
+    rp, ci, v, b = problem(16, 2.0)
+    Ao = O.Csr(rp, ci, v)
+    xo, io = O.cg(Ao, b, np.zeros_like(b), O.Settings(maxiter=30, rtol=1e-30, use_zero_guess=True),
+                  P=O.prec(1, Ao.dinv()))[:2]
+    assert io.reason == "diverged_iters"  # stopped at maxiter = 30
+    x, segs, its, st, _ = M.rr_solve(rp, ci, v, b, 1e-30, 1e-30, 30, 1, low=np.float64)
+    assert (segs, its, st) == (1, 30, 4)
+    assert rel(x, xo) < 1e-9
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.5])
+def test_fp32_model_converges_near_fp64_cg(sigma):
+    rp, ci, v, b = problem(24, sigma)
+    Ao = O.Csr(rp, ci, v)
+    _, io = O.cg(Ao, b, np.zeros_like(b), O.Settings(maxiter=5000, rtol=1e-10, use_zero_guess=True),
+                 P=O.prec(1, Ao.dinv()))[:2]
+    x, segs, its, st, rr = M.rr_solve(rp, ci, v, b, 1e-10, 1e-2, 2000, 100)
+    assert st == 1 and rr <= 1e-10
+    import scipy.sparse as sp
+    A = sp.csr_matrix((v, ci, rp))
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1e-10
+    assert io.iters <= its <= io.iters + 8
+    assert 4 <= segs <= 8
+    _, _, its_r, st_r, _ = M.rr_solve(rp, ci, v, b, 1e-10, 1e-2, 2000, 100, keep_p=False)
+    assert st_r == 1 and its_r >= 1.2 * its
+
+
+def test_segment_bookkeeping():
+    rp, ci, v, b = problem(12, 1.0)
+    x, segs, its, st, rr = M.rr_solve(rp, ci, v, b, 1e-10, 1e-2, 2000, 1)
+    assert (segs, st) == (1, 4) and rr > 1e-10 and its > 0
+    xs, *_ = M.rr_solve(rp, ci, v, b, 1e-13, 1e-2, 2000, 100)
+    _, segs0, its0, st0, _ = M.rr_solve(rp, ci, v, b, 1e-10, 1e-2, 2000, 100, x0=xs)
+    assert (segs0, its0, st0) == (0, 0, 1)
