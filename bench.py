@@ -85,6 +85,8 @@ def main():
                          "restarted refinement (fsx_ir_cg_solve); the other is run once "
                          "beside it")
     ap.add_argument("--max-segments", type=int, default=100)
+    ap.add_argument("--no-other-ir", action="store_true",
+                    help="skip the run of the other mixed-precision method (profiling runs)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--cpu-sample-iters", type=int, default=10,
                     help="iterations of the one-thread oracle sample")
@@ -376,7 +378,7 @@ def bench_diffusion(a, fsx, P):
     roofline["share_of_step"] = round(k1_ms * 1e-3 * ii.inner_iters_total / ir_s, 3)
     # the other mixed-precision method once (after a warm run), beside it
     other = "restart" if a.ir_method == "rr" else "rr"
-    if a.ir_jacobi == 2:
+    if a.ir_jacobi == 2 and not a.no_other_ir:
         solve_ir(fsx, ctx, A64, A32, bd, x, a, other)
         io_, dto = solve_ir(fsx, ctx, A64, A32, bd, x, a, other)
         log(f"{other}: {dto:.3f} s outer {io_.outer_iters} inner {io_.inner_iters_total} "
