@@ -674,14 +674,33 @@ def bench_batched(a, fsx, P):
     log(f"fp64: {t64:.3f} s, iters {it64.min()}..{it64.max()} (sum {int(it64.sum())}), "
         f"{gbs:.0f} GB/s algorithmic; ir: {tir:.3f} s, outer {oit.max()}, "
         f"inner sum {int(iit.sum())}")
+    # per system, the FP32 CG with FP64 residual replacement (fsx_batched_rr_cg)
+    rr = None
+    if a.ir_jacobi == 2:
+        seg_inner = fsx.Settings(maxiter=a.maxiter, rtol=a.segment_rtol)
+        (sg, sit, sst, _), trr = timed(
+            lambda: B.rr_cg(A32, bd, x, 1e-10, seg_inner, max_segments=a.max_segments),
+            "rr batched cg")
+        if not (sst == 1).all():
+            sys.exit(f"{int((sst != 1).sum())} systems did not reach 1e-10 (RR)")
+        relrr = true_rel()
+        xrr = x.get()
+        err_rr = float(np.max(np.sqrt(np.add.reduceat((xrr - x64) ** 2, rows[:-1])) /
+                              np.sqrt(np.add.reduceat(x64 * x64, rows[:-1]))))
+        log(f"rr: {trr:.3f} s, segments <= {sg.max()}, FP32 iterations sum {int(sit.sum())}")
+        rr = {"seconds": round(trr, 4), "method": IR_METHODS["rr"],
+              "segment_rtol": a.segment_rtol, "segments_max": int(sg.max()),
+              "inner_sum": int(sit.sum()), "max_true_rel_residual": float(relrr.max()),
+              "max_rel_err_vs_fp64": err_rr, "speedup_vs_fp64_cg": round(t64 / trr, 3)}
+    timed_s = rr["seconds"] if (rr and a.ir_method == "rr") else tir
     out = {
         "metric": "batched CG time-to-1e-10, 2048 diffusion systems (edges 8..48)",
-        "value": round(tir, 4),
+        "value": round(timed_s, 4),
         "unit": "s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(tir * 1e3, 2),
+        "ms_per_step": round(timed_s * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -703,6 +722,8 @@ def bench_batched(a, fsx, P):
                    "max_true_rel_residual": float(relir.max()),
                    "max_rel_err_vs_fp64": err_ir},
             "ir_speedup_vs_fp64_cg": round(t64 / tir, 3),
+            "rr": rr,
+            "value_is": "rr" if (rr and a.ir_method == "rr") else "ir",
         },
         "roofline": {"bound": "hbm", "kernel": "batched_cg_kernel<jacobi, fp64>",
                      "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -16,7 +16,10 @@
 // Refinement variant: per system, FP64 residual r = b - A x, inner FP32 CG
 // (FP32 values of the same layout, FP32 vectors, FP64 scalars and
 // reductions) on r/|r| to the inner tolerance, FP64 update x += |r| d, until
-// |r|/|b| <= rtol_outer.
+// |r|/|b| <= rtol_outer.  Residual-replacement variant (symmetrically scaled
+// FP32 matrix only): per system one FP32 CG whose recursive residual is
+// replaced by the true FP64 one after each segment, the search direction
+// kept (the per-system form of fsx_ir_cg_rr_solve).
 //
 // Row mapping inside a workgroup (chosen by the autotuner, fsx_batch_tune):
 //   0: sliced rows -- one row per lane on the matrix's sliced layout
@@ -87,6 +90,10 @@ struct batch_args {
 	// refinement
 	double rtol_outer;
 	int max_outer;
+	// residual replacement instead of restarts (a.sym set): the segment
+	// reduction; max_outer bounds the segments
+	int rr;
+	double seg_rtol;
 	// outputs
 	int * iters;
 	int * status;
@@ -391,6 +398,78 @@ __device__ void wg_cg(const batch_args & a, const V * sl_v, const V * v,
 	res_out = res;
 }
 
+// One segment of the unpreconditioned CG with residual replacement: from
+// r (recursive residual), p (search direction), rho = |r|^2, updates
+// r -= alpha w, then y += alpha p and p = beta p + r (beta = |r_new|^2 /
+// |r|^2, the two-reduction form), until |r| < tol or maxiter updates.  On
+// return r, p, y and rho hold the state the replacement continues from.
+template<class T, class V>
+__device__ int wg_cg_segment(const batch_args & a, const V * sl_v, const V * v,
+                             const V * dg_v, int64_t r0, int64_t n, T * y, T * r, T * p,
+                             T * w, double & rho, double tol, int maxiter, double * lds) {
+	const int tid = threadIdx.x, nt = blockDim.x;
+	double one[1];
+	int it = 0;
+	for (;;) {
+		one[0] = 0.0;
+		wg_spmv(a, sl_v, v, dg_v, r0, n, (const T *)p, [&](int64_t i, double s) {
+			const T wi = (T)s;
+			w[i] = wi;
+			one[0] += (double)wi * (double)p[i];
+		});
+		wg_sum<1>(one, lds);
+		const double alpha = rho / one[0];
+		one[0] = 0.0;
+		for (int64_t i0 = tid; i0 < n; i0 += (int64_t)kBatchRows * nt) {
+			T wv[kBatchRows], rv[kBatchRows];
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					wv[k] = w[i];
+					rv[k] = r[i];
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					const T ri = (T)((-alpha) * (double)wv[k] + (double)rv[k]);
+					r[i] = ri;
+					one[0] += (double)ri * (double)ri;
+				}
+			}
+		}
+		wg_sum<1>(one, lds);
+		const double beta = one[0] / rho;
+		rho = one[0];
+		for (int64_t i0 = tid; i0 < n; i0 += (int64_t)kBatchRows * nt) {
+			T pv[kBatchRows], rv[kBatchRows], yv[kBatchRows];
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					pv[k] = p[i];
+					rv[k] = r[i];
+					yv[k] = y[i];
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < kBatchRows; ++k) {
+				const int64_t i = i0 + (int64_t)k * nt;
+				if (i < n) {
+					y[i] = (T)(alpha * (double)pv[k] + (double)yv[k]);
+					p[i] = (T)(beta * (double)pv[k] + (double)rv[k]);
+				}
+			}
+		}
+		__syncthreads();
+		++it;
+		if (sqrt(rho) < tol || it >= maxiter) break;
+	}
+	return it;
+}
+
 template<bool JAC, bool IR>
 __global__ __launch_bounds__(kBatchMaxThreads) void batched_cg_kernel(batch_args a) {
 	__shared__ double lds[2 * 16];
@@ -417,6 +496,88 @@ __global__ __launch_bounds__(kBatchMaxThreads) void batched_cg_kernel(batch_args
 			                           a.r + r0, a.z + r0, a.p + r0, a.w + r0,
 			                           a.dinv64 + r0, a.maxiter, a.rtol,
 			                           a.use_zero_guess != 0, lds, iters, status, res);
+		}
+		else if (a.rr) {
+			// FP32 CG of this system with FP64 residual replacement
+			const double * b = a.b + r0;
+			double * x = a.x + r0;
+			double * r = a.r + r0;
+			const double * sc = a.sym + r0;
+			float * rh = a.fr + r0;
+			float * p = a.fp + r0;
+			float * y = a.fd + r0;
+			double acc[1] = {0.0};
+			for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc[0] += b[i] * b[i];
+			wg_sum<1>(acc, lds);
+			double bnorm = sqrt(acc[0]);
+			if (bnorm == 0.0) bnorm = 1.0;
+			// r = b - A64 x and |r|
+			auto residual = [&]() {
+				acc[0] = 0.0;
+				wg_spmv(a, a.sl_v64, a.v64, a.dg_v64, r0, n, (const double *)x,
+				        [&](int64_t i, double s) {
+					const double ri = b[i] - s;
+					r[i] = ri;
+					acc[0] += ri * ri;
+				});
+				wg_sum<1>(acc, lds);
+				return sqrt(acc[0]);
+			};
+			res = residual();
+			int seg = 0;
+			if (res / bnorm <= a.rtol_outer || res == 0.0)
+				status = FSX_CONVERGED_RTOL;
+			else {
+				const double c = res, inv = 1.0 / res;
+				// rh = (float)(S r / c), p = rh, y = 0
+				acc[0] = 0.0;
+				for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+					const float vi = (float)(sc[i] * (r[i] * inv));
+					rh[i] = vi;
+					p[i] = vi;
+					y[i] = 0.0f;
+					acc[0] += (double)vi * (double)vi;
+				}
+				wg_sum<1>(acc, lds);
+				double rho = acc[0];
+				for (;;) {
+					if (seg >= a.max_outer) {
+						status = FSX_DIVERGED_ITERS;
+						break;
+					}
+					if (!(rho > 0.0)) {
+						status = FSX_DIVERGED_BREAKDOWN;
+						break;
+					}
+					const double rh0 = sqrt(rho);
+					const double tol = fmax(a.seg_rtol * rh0,
+					                        0.5 * a.rtol_outer * bnorm * rh0 / res);
+					inner += wg_cg_segment<float, float>(a, a.sl_v32, a.v32, a.dg_v32, r0, n,
+					                                     y, rh, p, a.fw + r0, rho, tol,
+					                                     a.maxiter, lds);
+					++seg;
+					for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+						x[i] = x[i] + c * (sc[i] * (double)y[i]);
+					__syncthreads();
+					res = residual();
+					if (res / bnorm <= a.rtol_outer || res == 0.0) {
+						status = FSX_CONVERGED_RTOL;
+						break;
+					}
+					// the replacement: rh <- (float)(S r / c), p += rh_new - rh_old
+					acc[0] = 0.0;
+					for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+						const float vi = (float)(sc[i] * (r[i] * inv));
+						p[i] = (float)((double)p[i] + ((double)vi - (double)rh[i]));
+						rh[i] = vi;
+						y[i] = 0.0f;
+						acc[0] += (double)vi * (double)vi;
+					}
+					wg_sum<1>(acc, lds);
+					rho = acc[0];
+				}
+			}
+			iters = seg;
 		}
 		else {
 			// FP32-inner / FP64-outer refinement of this system
@@ -828,6 +989,67 @@ int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
 			else launch<false, true>(ctx, B, a);
 		}
 		copy_out(ctx, o, B->nsys, outer_iters, status, res, inner_iters);
+	});
+}
+
+int fsx_batched_rr_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
+                      const fsx_vec * b, fsx_vec * x, double rtol, int max_segments,
+                      const fsx_settings * inner, int * segments, int * inner_iters,
+                      int * status, double * res) {
+	return guard([&] {
+		FSX_REQUIRE(ctx && A32 && inner, "null argument");
+		check_batch(B);
+		check_vec(b, "b");
+		check_vec(x, "x");
+		check_same(b, x);
+		FSX_REQUIRE((int64_t)b->n == B->A->nrows && b->dt == FSX_F64,
+		            "b and x: FP64, one entry per batch row");
+		FSX_REQUIRE(b->ptr() != x->ptr() || b->n == 0, "b and x must differ");
+		FSX_REQUIRE(A32->vt == FSX_F32 && A32->idx == B->A->idx,
+		            "A32 must be fsx_csr_convert(A, FSX_F32) of the batch matrix holding "
+		            "fsx_csr_set_values_f32_sym_scaled values");
+		FSX_REQUIRE(B->mapping != 0 || A32->sl_vals, "A32 lacks the sliced layout");
+		FSX_REQUIRE(B->mapping != 2 || A32->dg_vals, "A32 lacks the slice-offset layout");
+		FSX_REQUIRE(inner->rtol > 0.0 && inner->rtol < 1.0,
+		            "the segment reduction inner->rtol must lie in (0, 1)");
+		FSX_REQUIRE(inner->maxiter >= 1, "inner->maxiter must be at least 1");
+		FSX_REQUIRE(max_segments >= 0, "max_segments must not be negative");
+		auto & W = work_for(B, true);
+		// S = D^{-1/2} of the FP64 batch matrix
+		if (!W.sym) W.sym = std::make_unique<dbuf>(std::max<int64_t>(1, B->A->nrows) * 8);
+		dbuf flag(sizeof(int));
+		FSX_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+		sym_scale_vector(ctx, B->A, (double *)W.sym->p, (int *)flag.p);
+		int bad = 0;
+		FSX_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+		FSX_HIP(hipStreamSynchronize(ctx->stream));
+		ctx->host_syncs++;
+		FSX_REQUIRE(!bad, "a diagonal entry of the batch matrix is missing or not positive");
+		batch_outputs o(B->nsys);
+		batch_args a = base_args(B, A32);
+		a.b = (const double *)b->ptr();
+		a.x = (double *)x->ptr();
+		a.sym = (const double *)W.sym->p;
+		a.r = (double *)W.r->p;
+		a.fr = (float *)W.fr->p;
+		a.fz = (float *)W.fz->p;
+		a.fp = (float *)W.fp->p;
+		a.fw = (float *)W.fw->p;
+		a.fd = (float *)W.fd->p;
+		a.fb = (float *)W.fb->p;
+		a.maxiter = inner->maxiter;
+		a.rtol = (double)inner->rtol;
+		a.use_zero_guess = 0;
+		a.rtol_outer = rtol;
+		a.max_outer = max_segments;
+		a.rr = 1;
+		a.seg_rtol = (double)inner->rtol;
+		a.iters = (int *)o.iters->p;
+		a.status = (int *)o.status->p;
+		a.res = (double *)o.res->p;
+		a.inner_iters = (int *)o.inner->p;
+		if (B->nsys > 0) launch<false, true>(ctx, B, a);
+		copy_out(ctx, o, B->nsys, segments, status, res, inner_iters);
 	});
 }
 

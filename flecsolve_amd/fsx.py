@@ -193,6 +193,8 @@ _SIGS = {
     "fsx_batched_cg": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(Settings), _P, _P, _P]),
     "fsx_batched_ir_cg": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, C.c_double, C.c_int,
                                     C.POINTER(Settings), _P, _P, _P, _P]),
+    "fsx_batched_rr_cg": (C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_int,
+                                    C.POINTER(Settings), _P, _P, _P, _P]),
     "fsx_mm_read_header": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int64)]),
@@ -977,6 +979,20 @@ class Batch:
                                       _ptr(outer), _ptr(inn), _ptr(st), _ptr(rs)),
               "fsx_batched_ir_cg")
         return outer, inn, st, rs
+
+    def rr_cg(self, A32: CSR, b: Vector, x: Vector, rtol: float, inner: Settings,
+              max_segments=100):
+        """Per system: FP32 CG on the symmetrically scaled A32 with FP64
+        residual replacement; x is the initial guess.  Returns (segments,
+        FP32 iterations, status, true |r|) per system."""
+        seg = np.zeros(self.nsys, np.int32)
+        inn = np.zeros(self.nsys, np.int32)
+        st = np.zeros(self.nsys, np.int32)
+        rs = np.zeros(self.nsys, np.float64)
+        check(lib().fsx_batched_rr_cg(self.ctx.h, self.h, A32.h, b.h, x.h, rtol,
+                                      int(max_segments), C.byref(inner), _ptr(seg),
+                                      _ptr(inn), _ptr(st), _ptr(rs)), "fsx_batched_rr_cg")
+        return seg, inn, st, rs
 
     def destroy(self):
         if self.h:

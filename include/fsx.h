@@ -605,6 +605,19 @@ int fsx_batched_ir_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
                       double rtol_outer, int max_outer, const fsx_settings * inner,
                       int * outer_iters, int * inner_iters, int * status,
                       double * res);
+/* fsx_batched_rr_cg: per system, fsx_ir_cg_rr_solve's FP32 CG with FP64
+ * residual replacement (new, no reference counterpart) in the same
+ * persistent launch: A32 = fsx_csr_convert(A, FSX_F32) holding
+ * fsx_csr_set_values_f32_sym_scaled values; segments of inner->rtol
+ * reduction (inner->maxiter iterations at most each), at most max_segments
+ * per system; x is the initial guess.  The FP32 iteration is the
+ * two-reduction form of the batched CG (beta = |r_new|^2 / |r|^2).
+ * Outputs per system: segments, FP32 iterations, status (converged_rtol,
+ * diverged_iters, diverged_breakdown) and the true FP64 |b - A x|. */
+int fsx_batched_rr_cg(fsx_ctx * ctx, fsx_batch * B, const fsx_csr * A32,
+                      const fsx_vec * b, fsx_vec * x, double rtol, int max_segments,
+                      const fsx_settings * inner, int * segments, int * inner_iters,
+                      int * status, double * res);
 
 /* ---- measurement helpers ------------------------------------------------
  * Average duration (ms) of `reps` back-to-back launches of the SpMV kernel

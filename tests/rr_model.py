@@ -13,7 +13,11 @@ between segments x += c S y (FP64), r = b - A x (FP64), the replacement
 rh = (float)(s (r / c)), p += rh_new - rh_old, y = 0.
 
 `keep_p=False` restarts the FP32 CG at every replacement (p = rh): the
-restarted scheme the replacement is measured against.  `low=np.float64`
+restarted scheme the replacement is measured against.
+`single_reduction=False` takes beta from the updated |r|^2 (the batched
+solver's two-reduction form) instead of the estimate; `acc64=True` sums each
+row of the FP32 product in FP64 before rounding it (the batched solver's
+product; the global FP32 product sums in FP32).  `low=np.float64`
 runs the same recurrences with FP64 vectors and matrix: then the iteration
 is CG with Jacobi preconditioning in FP64 (S A S y = S b, x = S y), which
 the tests hold against the oracle's CG + Dinv.
@@ -42,12 +46,13 @@ def _dot(a, b):
 
 
 def rr_solve(rp, ci, v, b, rtol, seg_rtol, maxiter, max_segments, x0=None,
-             keep_p=True, low=F32):
+             keep_p=True, low=F32, single_reduction=True, acc64=False):
     """Returns (x, segments, FP32 iterations, status, true |r|/|b|); status
     1 converged, 4 max_segments reached, 5 the FP32 residual vanished."""
     n = rp.size - 1
     A = sp.csr_matrix((v, ci, rp), shape=(n, n))
     s, Ah = scaled_matrix(rp, ci, v, low)
+    Ah64 = Ah.astype(np.float64) if acc64 else None
     x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
     bn = np.linalg.norm(b)
     bn = bn if bn > 0 else 1.0
@@ -72,15 +77,21 @@ def rr_solve(rp, ci, v, b, rtol, seg_rtol, maxiter, max_segments, x0=None,
         while True:
             if it > 0 and (np.sqrt(rho) < tol or it >= maxiter):
                 break
-            w = (Ah @ p).astype(low)
+            if acc64:
+                w = (Ah64 @ p.astype(np.float64)).astype(low)
+            else:
+                w = (Ah @ p).astype(low)
             pw, ww = _dot(w, p), _dot(w, w)
             alpha = rho / pw
-            beta = (alpha * alpha * ww - rho) / rho
             p64 = p.astype(np.float64)
             rh = ((-alpha) * w.astype(np.float64) + rh.astype(np.float64)).astype(low)
+            rho_new = _dot(rh, rh)
+            # single reduction: beta from the estimate alpha^2 w.w - rho, known
+            # before the update; two reductions: from the updated |r|^2
+            beta = ((alpha * alpha * ww - rho) if single_reduction else rho_new) / rho
             y = (alpha * p64 + y.astype(np.float64)).astype(low)
             p = (beta * p64 + rh.astype(np.float64)).astype(low)
-            rho = _dot(rh, rh)
+            rho = rho_new
             it += 1
         iters += it
         segs += 1

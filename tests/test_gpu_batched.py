@@ -124,6 +124,56 @@ def test_batched_ir_reaches_1e10(ctx, sigma, jacobi):
         assert outer[s] >= 1 and inner[s] >= outer[s]
 
 
+@pytest.mark.parametrize("sigma", [1.0, 2.5])
+def test_batched_rr_against_model(ctx, sigma):
+    """Per system, the FP32 CG with FP64 residual replacement
+    (fsx_batched_rr_cg): every system at true residual <= 1e-10, each
+    system's segment and FP32 iteration counts those of the CPU model in its
+    two-reduction form with FP64 row sums (tests/rr_model.py; +-1 / +-3; the
+    model with FP32 row sums is 5-7 iterations away on some systems, so the
+    bar separates the two products), fewer FP32 iterations
+    in total than the restarted refinement on the same FP32 matrix, and the
+    same x bit for bit under the sliced and slice-offset mappings (both sum
+    rows serially)."""
+    import rr_model as M
+    e, rows, rp, ci, v, b = make(32, 4, 14, sigma=sigma)
+    A = ctx.csr(rp, ci, v)
+    A.set_plan(2)   # the sliced layout too, for mapping 0
+    A.set_plan(3)
+    A32 = A.convert(fsx.F32)
+    A32.set_values_sym_scaled(A)
+    B = fsx.Batch(A, rows)
+    bd = ctx.vec(b)
+    inner = fsx.Settings(maxiter=2000, rtol=1e-2)
+    outs = []
+    # the same workgroup size (it sets the reduction tree), two row mappings
+    for cfg in ((256, 2, 1), (256, 0, 1)):
+        B.set_config(*cfg)
+        x = ctx.zeros(b.size)
+        outs.append((B.rr_cg(A32, bd, x, 1e-10, inner), x.get()))
+    (seg, its, st, rs), xg = outs[0]
+    np.testing.assert_array_equal(outs[1][1], xg)
+    np.testing.assert_array_equal(outs[1][0][1], its)
+    assert (st == 1).all()
+    S = sp.csr_matrix((v, ci, rp))
+    for k in range(rows.size - 1):
+        sl = slice(rows[k], rows[k + 1])
+        assert np.linalg.norm(b[sl] - S[sl] @ xg) / np.linalg.norm(b[sl]) <= 1.01e-10, k
+        assert abs(rs[k] - np.linalg.norm(b[sl] - S[sl] @ xg)) <= 1e-3 * rs[k] + 1e-300
+        _, msegs, mits, mst, _ = M.rr_solve(*local(rows, rp, ci, v, k), b[sl], 1e-10, 1e-2,
+                                            2000, 100, single_reduction=False, acc64=True)
+        assert mst == 1
+        assert abs(int(seg[k]) - msegs) <= 1, (k, seg[k], msegs)
+        assert abs(int(its[k]) - mits) <= 3, (k, its[k], mits)
+    x2 = ctx.zeros(b.size)
+    outer, inner_r, st2, _ = B.ir_cg(A32, bd, x2, 1e-10, fsx.Settings(maxiter=2000, rtol=1e-4),
+                                     jacobi=2)
+    assert (st2 == 1).all()
+    assert its.sum() < inner_r.sum()
+    with pytest.raises(fsx.FsxError):
+        B.rr_cg(A32, bd, x, 1e-10, fsx.Settings(maxiter=2000, rtol=1.0))
+
+
 def test_batched_edge_cases(ctx):
     e, rows, rp, ci, v, b = make(6, 2, 5)
     b[rows[2]:rows[3]] = 0.0       # zero right-hand side: converged at once

@@ -1,10 +1,11 @@
 """The CPU model of the FP32 CG with FP64 residual replacement
 (tests/rr_model.py, the checker of fsx_ir_cg_rr_solve), pinned on the CPU:
 
-  * run in FP64 its recurrences are CG with Jacobi preconditioning: after 30
-    iterations of one segment its x equals the oracle's CG + Dinv after 30
-    iterations (cg.hh:43-139) to 1e-9 -- a wrong alpha, beta, scaling S or
-    update changes x far beyond that;
+  * run in FP64 its recurrences (single- and two-reduction forms) are CG
+    with Jacobi preconditioning: after 30 iterations of one segment its x
+    equals the oracle's CG + Dinv after 30 iterations (cg.hh:43-139) to
+    1e-9 -- a wrong alpha, beta, scaling S or update changes x far beyond
+    that;
   * run in FP32 it reaches a true FP64 residual <= 1e-10 within a few
     iterations of the oracle's FP64 CG + Dinv, and restarting the FP32 CG at
     each replacement (keep_p=False) takes at least 20% more iterations: the
@@ -31,13 +32,15 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def test_fp64_model_is_jacobi_cg():
+@pytest.mark.parametrize("single_reduction", [True, False])
+def test_fp64_model_is_jacobi_cg(single_reduction):
     rp, ci, v, b = problem(16, 2.0)
     Ao = O.Csr(rp, ci, v)
     xo, io = O.cg(Ao, b, np.zeros_like(b), O.Settings(maxiter=30, rtol=1e-30, use_zero_guess=True),
                   P=O.prec(1, Ao.dinv()))[:2]
     assert io.reason == "diverged_iters"  # stopped at maxiter = 30
-    x, segs, its, st, _ = M.rr_solve(rp, ci, v, b, 1e-30, 1e-30, 30, 1, low=np.float64)
+    x, segs, its, st, _ = M.rr_solve(rp, ci, v, b, 1e-30, 1e-30, 30, 1, low=np.float64,
+                                     single_reduction=single_reduction)
     assert (segs, its, st) == (1, 30, 4)
     assert rel(x, xo) < 1e-9
 
