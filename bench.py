@@ -688,7 +688,9 @@ def bench_batched(a, fsx, P):
         err_rr = float(np.max(np.sqrt(np.add.reduceat((xrr - x64) ** 2, rows[:-1])) /
                               np.sqrt(np.add.reduceat(x64 * x64, rows[:-1]))))
         log(f"rr: {trr:.3f} s, segments <= {sg.max()}, FP32 iterations sum {int(sit.sum())}")
-        rr = {"seconds": round(trr, 4), "method": IR_METHODS["rr"],
+        rr = {"seconds": round(trr, 4),
+              "method": "per system: " + IR_METHODS["rr"].replace("fsx_ir_cg_rr_solve",
+                                                                  "fsx_batched_rr_cg"),
               "segment_rtol": a.segment_rtol, "segments_max": int(sg.max()),
               "inner_sum": int(sit.sum()), "max_true_rel_residual": float(relrr.max()),
               "max_rel_err_vs_fp64": err_rr, "speedup_vs_fp64_cg": round(t64 / trr, 3)}
