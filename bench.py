@@ -1,6 +1,7 @@
 """Benchmark: time to relative residual 1e-10 on 256^3 variable-coefficient
-diffusion, FP32-inner / FP64-outer refined CG vs plain FP64 CG (+Jacobi),
-with the SpMV kernel's fraction of the HBM roofline.
+diffusion, mixed precision (FP32 CG with FP64 residual replacement; the
+restarted FP32-inner / FP64-outer refinement beside it) vs plain FP64 CG
+(+Jacobi), with the SpMV kernel's fraction of the HBM roofline.
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--sigma 1.0]
                     [--no-cpu-baseline]
@@ -10,13 +11,16 @@ with the SpMV kernel's fraction of the HBM roofline.
 
 A step is one complete solve from x0 = 0 to |b - A x| / |b| <= 1e-10 with the
 matrix and right-hand side already resident in HBM.  The timed steps use
-the refined solver (the method the headline targets); plain FP64 CG + Jacobi
-is timed the same way beside it.  One JSON line is printed at the end.
+the mixed-precision solver (the method the headline targets; --ir-method
+chooses residual replacement, the default, or the restarted refinement);
+plain FP64 CG + Jacobi is timed the same way beside it.  One JSON line is
+printed at the end.
 
 `--workload batched`: 2048 independent lognormal-k diffusion systems with
 edge lengths uniform in 8..48, solved in one persistent launch (FP64 CG +
-Jacobi and FP32-inner/FP64-outer refinement, autotuned row mapping); a step
-is the whole batch solved to 1e-10 per system.
+Jacobi, the FP32-inner/FP64-outer refinement and the FP32 CG with FP64
+residual replacement per system, autotuned row mapping); a step is the
+whole batch solved to 1e-10 per system.
 
 `--workload adjoint`: backward-Euler heat on 128^3 (dt 1e-3, 50 steps, CG +
 Jacobi per step), loss |u_T - u_target|^2 and its gradient with respect to
@@ -849,6 +853,33 @@ def bench_adjoint(a, fsx, P):
                "stats": st_ir.as_dict()}
     log(f"refined solves: {t_ir:.3f} s, loss rel diff {refined['loss_rel_diff']:.2e}, "
         f"gradient rel diff {refined['grad_rel_diff']:.2e}")
+    # and with every solve an FP32 CG with FP64 residual replacement
+    # (fsx_be_adjoint_rr)
+    del prob_ir
+    prob_rr = HeatProblem(ctx, n=n, dt=1e-3, steps=a.heat_steps, ckpt=a.ckpt,
+                          rtol=a.heat_rtol, refine=True, method="rr",
+                          inner_rtol=a.segment_rtol)
+    tr_, res_rr = [], None
+    for i in range(a.warmup + a.steps):
+        ctx.sync()
+        t = time.perf_counter()
+        res_rr = prob_rr.loss_and_grad(theta)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        log(f"residual-replacement loss+gradient {'warmup' if i < a.warmup else 'step'} {i}: "
+            f"{dt:.3f} s")
+        if i >= a.warmup:
+            tr_.append(dt)
+    loss_rr, g_rr, _, st_rr = res_rr
+    t_rr = sum(tr_) / len(tr_)
+    refined_rr = {"seconds": round(t_rr, 4), "speedup_vs_cg": round(t_grad / t_rr, 3),
+                  "segment_rtol": a.segment_rtol,
+                  "loss_rel_diff": abs(loss_rr - loss) / abs(loss),
+                  "grad_rel_diff": float(np.linalg.norm(g_rr - g) / np.linalg.norm(g)),
+                  "stats": st_rr.as_dict()}
+    log(f"residual-replacement solves: {t_rr:.3f} s, loss rel diff "
+        f"{refined_rr['loss_rel_diff']:.2e}, gradient rel diff {refined_rr['grad_rel_diff']:.2e}")
+    del prob_rr
     iters = st["forward_iters"] + st["recompute_iters"] + st["adjoint_iters"]
     N, nnz = prob.N, prob.A.nnz
     gbs = iters * cg_iter_bytes(prob.A) / t_grad / 1e9
@@ -874,6 +905,7 @@ def bench_adjoint(a, fsx, P):
                          "max_rel_err": worst},
             "loss": loss,
             "refined_solves": refined,
+            "residual_replacement_solves": refined_rr,
         },
         "roofline": {"bound": "hbm",
                      "kernel": f"fused CG iteration (SpMV plan {prob.A.plan()[0]} + BLAS-1)",

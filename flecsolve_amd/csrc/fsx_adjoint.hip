@@ -187,13 +187,17 @@ void check_diffusion_pattern(const fsx_csr * A, int n) {
 	            "A does not have the 7-point pattern of an n^3 grid");
 }
 
-// Refined solves: FP32 inner CG on A32 = S A S (S = D^{-1/2}), FP64 outer
-// residual and update, to the outer tolerance s->rtol.
+// Refined solves on A32 = S A S (S = D^{-1/2}) to the outer tolerance
+// s->rtol: the restarted refinement (FP32 inner CG to inner.rtol, FP64
+// outer residual and update) or, with rr, the FP32 CG with FP64 residual
+// replacement (segments of inner.rtol reduction).
 struct refine {
 	const fsx_csr * A32;
 	fsx_settings inner;
+	bool rr = false;
 };
 constexpr int kRefineMaxOuter = 20;
+constexpr int kRefineMaxSegments = 100;
 
 // One backward-Euler step: x <- solution of A x = b from the guess x, by CG
 // (+ Jacobi) or, with R, by mixed-precision refinement (inner iterations
@@ -204,8 +208,12 @@ void be_step(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 	bool ok;
 	if (R) {
 		fsx_ir_info ii{};
-		call(fsx_ir_cg_solve(ctx, A, R->A32, 2, b, x, (double)s->rtol, kRefineMaxOuter,
-		                     &R->inner, &ii));
+		if (R->rr)
+			call(fsx_ir_cg_rr_solve(ctx, A, R->A32, b, x, (double)s->rtol, kRefineMaxSegments,
+			                        &R->inner, &ii));
+		else
+			call(fsx_ir_cg_solve(ctx, A, R->A32, 2, b, x, (double)s->rtol, kRefineMaxOuter,
+			                     &R->inner, &ii));
 		ok = ii.status == FSX_CONVERGED_RTOL;
 		iters += ii.inner_iters_total;
 	}
@@ -557,12 +565,16 @@ int fsx_be_adjoint(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * AT, const f
 	});
 }
 
-static refine refine_args(const fsx_csr * A, const fsx_csr * A32, const fsx_settings * inner) {
+static refine refine_args(const fsx_csr * A, const fsx_csr * A32, const fsx_settings * inner,
+                          bool rr = false) {
 	FSX_REQUIRE(A && A32 && inner, "null argument");
 	FSX_REQUIRE(A32->vt == FSX_F32 && A32->idx == A->idx,
 	            "A32 must be fsx_csr_convert(A, FSX_F32) holding "
 	            "fsx_csr_set_values_f32_sym_scaled values");
-	return refine{A32, *inner};
+	if (rr)
+		FSX_REQUIRE(inner->rtol > 0.0 && inner->rtol < 1.0 && inner->maxiter >= 1,
+		            "the segment reduction inner->rtol must lie in (0, 1), maxiter >= 1");
+	return refine{A32, *inner, rr};
 }
 
 int fsx_be_forward_ir(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
@@ -581,6 +593,26 @@ int fsx_be_adjoint_ir(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
 	return guard([&] {
 		const refine R = refine_args(A, A32, inner);
 		// A symmetric: the adjoint solves use A (and A32) themselves
+		be_adjoint_impl(ctx, A, nullptr, nullptr, u0, u_target, steps, ckpt, s, loss, G, uT,
+		                grad_u0, st, &R);
+	});
+}
+
+int fsx_be_forward_rr(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
+                      const fsx_vec * u0, int steps, const fsx_settings * s,
+                      const fsx_settings * inner, fsx_vec * uT, fsx_be_stats * st) {
+	return guard([&] {
+		const refine R = refine_args(A, A32, inner, true);
+		be_forward_impl(ctx, A, nullptr, u0, steps, s, uT, st, &R);
+	});
+}
+
+int fsx_be_adjoint_rr(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
+                      const fsx_vec * u0, const fsx_vec * u_target, int steps, int ckpt,
+                      const fsx_settings * s, const fsx_settings * inner, double * loss,
+                      fsx_vec * G, fsx_vec * uT, fsx_vec * grad_u0, fsx_be_stats * st) {
+	return guard([&] {
+		const refine R = refine_args(A, A32, inner, true);
 		be_adjoint_impl(ctx, A, nullptr, nullptr, u0, u_target, steps, ckpt, s, loss, G, uT,
 		                grad_u0, st, &R);
 	});

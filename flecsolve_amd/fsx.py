@@ -222,6 +222,11 @@ _SIGS = {
     "fsx_be_adjoint_ir": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.c_int,
                                     C.POINTER(Settings), C.POINTER(Settings),
                                     C.POINTER(C.c_double), _P, _P, _P, C.POINTER(BEStats)]),
+    "fsx_be_forward_rr": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(Settings),
+                                    C.POINTER(Settings), _P, C.POINTER(BEStats)]),
+    "fsx_be_adjoint_rr": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.c_int,
+                                    C.POINTER(Settings), C.POINTER(Settings),
+                                    C.POINTER(C.c_double), _P, _P, _P, C.POINTER(BEStats)]),
     "fsx_time_spmv": (C.c_int, [_P, _P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
     "fsx_time_copy": (C.c_int, [_P, _P, _P, C.c_int, C.POINTER(C.c_double)]),
 }
@@ -721,6 +726,33 @@ def be_adjoint_ir(A: CSR, A32: CSR, u0: Vector, u_target: Vector, steps: int, ck
     check(lib().fsx_be_adjoint_ir(A.ctx.h, A.h, A32.h, u0.h, u_target.h, steps, ckpt,
                                   C.byref(settings), C.byref(inner), C.byref(loss), G.h, uT.h,
                                   gu0.h, C.byref(st)), "fsx_be_adjoint_ir")
+    return loss.value, G, uT, gu0, st
+
+
+def be_forward_rr(A: CSR, A32: CSR, u0: Vector, steps: int, settings: Settings,
+                  inner: Settings):
+    """be_forward with each step solved by the FP32 CG with FP64 residual
+    replacement on A32 (inner.rtol: the segment reduction)."""
+    uT = Vector(A.ctx, A.nrows)
+    st = BEStats()
+    check(lib().fsx_be_forward_rr(A.ctx.h, A.h, A32.h, u0.h, steps, C.byref(settings),
+                                  C.byref(inner), uT.h, C.byref(st)), "fsx_be_forward_rr")
+    return uT, st
+
+
+def be_adjoint_rr(A: CSR, A32: CSR, u0: Vector, u_target: Vector, steps: int, ckpt: int,
+                  settings: Settings, inner: Settings):
+    """be_adjoint with every solve by the FP32 CG with FP64 residual
+    replacement on A32 (symmetric A).  Returns (loss, G, u^steps, dL/du^0,
+    stats)."""
+    G = Vector(A.ctx, A.nnz)
+    uT = Vector(A.ctx, A.nrows)
+    gu0 = Vector(A.ctx, A.nrows)
+    loss = C.c_double()
+    st = BEStats()
+    check(lib().fsx_be_adjoint_rr(A.ctx.h, A.h, A32.h, u0.h, u_target.h, steps, ckpt,
+                                  C.byref(settings), C.byref(inner), C.byref(loss), G.h, uT.h,
+                                  gu0.h, C.byref(st)), "fsx_be_adjoint_rr")
     return loss.value, G, uT, gu0, st
 
 

@@ -9,9 +9,11 @@ Each step is a CG + Jacobi solve from the previous state.  The loss is
 |u_T - u_target|^2, u_target the trajectory end of a second coefficient
 field; its gradient comes from the library's discrete adjoint
 (fsx_be_adjoint) chained to log k (fsx_diffusion3d_coeff_grad).  With
-refine=True every solve is a mixed-precision refinement on the
-symmetrically scaled FP32 copy of A (fsx_be_forward_ir / fsx_be_adjoint_ir)
-to the same tolerance.  Every
+refine=True every solve is a mixed-precision solve on the symmetrically
+scaled FP32 copy of A to the same tolerance: the restarted refinement
+(method="ir": fsx_be_forward_ir / fsx_be_adjoint_ir, inner tolerance 1e-4)
+or the FP32 CG with FP64 residual replacement (method="rr":
+fsx_be_forward_rr / fsx_be_adjoint_rr, segment reduction 1e-2).  Every
 solve, assembly and gradient kernel runs on the GPU through the C ABI.
 """
 from __future__ import annotations
@@ -28,9 +30,14 @@ class HeatProblem:
     def __init__(self, ctx: fsx.Context, n: int = 128, dt: float = 1e-3, steps: int = 50,
                  ckpt: int = 10, sigma: float = 1.0, ell: float = 8.0, bumps: int = 8,
                  rtol: float = 1e-12, maxiter: int = 5000, seeds: dict | None = None,
-                 refine: bool = False, inner_rtol: float = 1e-4):
+                 refine: bool = False, inner_rtol: float | None = None,
+                 method: str = "ir"):
         s = seeds or P.seeds()
-        self.refine = refine
+        if method not in ("ir", "rr"):
+            raise ValueError("method must be 'ir' or 'rr'")
+        self.refine, self.method = refine, method
+        if inner_rtol is None:
+            inner_rtol = 1e-4 if method == "ir" else 1e-2
         self.inner = fsx.Settings(maxiter=maxiter, rtol=inner_rtol)
         self.ctx, self.n, self.steps, self.ckpt = ctx, n, steps, ckpt
         self.gamma = dt * (n + 1) ** 2          # dt / h^2
@@ -58,8 +65,8 @@ class HeatProblem:
 
     def _forward(self):
         if self.refine:
-            return fsx.be_forward_ir(self.A, self.A32, self.u0, self.steps, self.settings,
-                                     self.inner)
+            fwd = fsx.be_forward_rr if self.method == "rr" else fsx.be_forward_ir
+            return fwd(self.A, self.A32, self.u0, self.steps, self.settings, self.inner)
         return fsx.be_forward(self.A, self.u0, self.steps, self.settings, self.dinv)
 
     def loss(self, theta: np.ndarray) -> float:
@@ -74,9 +81,9 @@ class HeatProblem:
         """(loss, dL/dtheta on the (n+2)^3 lattice, dL/du0, adjoint stats)."""
         self._set_k(np.exp(theta))
         if self.refine:
-            loss, G, uT, gu0, st = fsx.be_adjoint_ir(self.A, self.A32, self.u0, self.u_target,
-                                                     self.steps, self.ckpt, self.settings,
-                                                     self.inner)
+            adj = fsx.be_adjoint_rr if self.method == "rr" else fsx.be_adjoint_ir
+            loss, G, uT, gu0, st = adj(self.A, self.A32, self.u0, self.u_target, self.steps,
+                                       self.ckpt, self.settings, self.inner)
         else:
             loss, G, uT, gu0, st = fsx.be_adjoint(self.A, self.u0, self.u_target, self.steps,
                                                   self.ckpt, self.settings, self.dinv)

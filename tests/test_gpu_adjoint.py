@@ -7,7 +7,8 @@ spacing does not change the result; the gradient, wrapped as a
 torch.autograd.Function, passes torch.autograd.gradcheck on a small grid and
 matches central finite differences (relative error <= 1e-6) along seeded
 directions; the same with every solve a mixed-precision refinement on the
-symmetrically scaled FP32 matrix (fsx_be_forward_ir / fsx_be_adjoint_ir).
+symmetrically scaled FP32 matrix (fsx_be_forward_ir / fsx_be_adjoint_ir, and
+the residual-replacement fsx_be_forward_rr / fsx_be_adjoint_rr).
 """
 import numpy as np
 import pytest
@@ -206,21 +207,25 @@ def test_fd_directions_moderate_grid(ctx):
     assert worst <= 1e-6
 
 
-def test_refined_solves_match_cpu_adjoint(ctx):
-    """fsx_be_forward_ir / fsx_be_adjoint_ir: every solve a mixed-precision
-    refinement on the symmetrically scaled FP32 copy of A, to the same
-    tolerance; the trajectory, loss and gradients against the oracle's FP64
-    CG adjoint (both solve each step to a 1e-12 relative residual)."""
+@pytest.mark.parametrize("method", ["ir", "rr"])
+def test_refined_solves_match_cpu_adjoint(ctx, method):
+    """fsx_be_forward_ir / fsx_be_adjoint_ir (restarted refinement) and
+    fsx_be_forward_rr / fsx_be_adjoint_rr (FP32 CG with FP64 residual
+    replacement): every solve mixed-precision on the symmetrically scaled
+    FP32 copy of A, to the same tolerance; the trajectory, loss and
+    gradients against the oracle's FP64 CG adjoint (both solve each step to
+    a 1e-12 relative residual)."""
     n, steps = 10, 6
     k, u0, ut, (rp, ci, v), so = _small(ctx, n, steps)
     A = ctx.csr(rp, ci, v)
     A32 = A.convert(fsx.F32)
     A32.set_values_sym_scaled(A)
     s = fsx.Settings(maxiter=3000, rtol=1e-12)
-    inner = fsx.Settings(maxiter=3000, rtol=1e-4)
-    uT_f, stf = fsx.be_forward_ir(A, A32, ctx.vec(u0), steps, s, inner)
-    loss, G, uT, gu0, st = fsx.be_adjoint_ir(A, A32, ctx.vec(u0), ctx.vec(ut), steps, 4, s,
-                                             inner)
+    inner = fsx.Settings(maxiter=3000, rtol=1e-4 if method == "ir" else 1e-2)
+    fwd = fsx.be_forward_ir if method == "ir" else fsx.be_forward_rr
+    adj = fsx.be_adjoint_ir if method == "ir" else fsx.be_adjoint_rr
+    uT_f, stf = fwd(A, A32, ctx.vec(u0), steps, s, inner)
+    loss, G, uT, gu0, st = adj(A, A32, ctx.vec(u0), ctx.vec(ut), steps, 4, s, inner)
     Lo, Go, guo, uTo = O.be_adjoint(O.Csr(rp, ci, v), u0, ut, steps, so)
     assert stf.forward_solves == steps and stf.failed_step == -1
     assert rel(uT_f.get(), uTo) <= 1e-10 and rel(uT.get(), uTo) <= 1e-10
@@ -231,9 +236,10 @@ def test_refined_solves_match_cpu_adjoint(ctx):
     assert st.recompute_solves == 4
 
 
-def test_refined_fd_directions_moderate_grid(ctx):
+@pytest.mark.parametrize("method", ["ir", "rr"])
+def test_refined_fd_directions_moderate_grid(ctx, method):
     prob = HeatProblem(ctx, n=24, dt=1e-3, steps=10, ckpt=4, ell=4.0, rtol=1e-13,
-                       refine=True)
+                       refine=True, method=method)
     loss, g, _, st = prob.loss_and_grad(prob.theta0)
     seeds = P.seeds()
     eps = 1e-4
