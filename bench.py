@@ -80,7 +80,7 @@ def main():
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--inner-rtol", type=float, default=1e-4,
                     help="restarted refinement: each inner solve's tolerance")
-    ap.add_argument("--segment-rtol", type=float, default=1e-2,
+    ap.add_argument("--segment-rtol", type=float, default=3e-2,
                     help="residual replacement: reduction of the FP32 recursive "
                          "residual between two FP64 replacements")
     ap.add_argument("--ir-method", choices=["rr", "restart"], default="rr",

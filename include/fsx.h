@@ -573,7 +573,7 @@ int fsx_be_adjoint_ir(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
                       fsx_vec * G, fsx_vec * uT, fsx_vec * grad_u0, fsx_be_stats * stats);
 /* fsx_be_forward_rr / fsx_be_adjoint_rr: the same with every solve by
  * fsx_ir_cg_rr_solve (FP32 CG with FP64 residual replacement; inner->rtol
- * is the segment reduction, e.g. 1e-2).  New, no reference counterpart. */
+ * is the segment reduction, e.g. 3e-2).  New, no reference counterpart. */
 int fsx_be_forward_rr(fsx_ctx * ctx, const fsx_csr * A, const fsx_csr * A32,
                       const fsx_vec * u0, int steps, const fsx_settings * s,
                       const fsx_settings * inner, fsx_vec * uT, fsx_be_stats * stats);
