@@ -112,6 +112,18 @@ def build_cpp_tests() -> Path:
     return out
 
 
+def build_amg_setup_check() -> Path:
+    """UA-AMG setup on the host alone (tests/cpp/amg_setup_check.cpp over
+    flecsolve_amd/csrc/fsx_amg_setup.hpp, the code fsx_amg_create runs)."""
+    src = ROOT / "tests" / "cpp" / "amg_setup_check.cpp"
+    out = ROOT / "tests" / "cpp" / "amg_setup_check"
+    tmp = ROOT / "tests" / "cpp" / "amg_setup_check.tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", "-pthread",
+          "-I", str(CSRC), str(src), "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def build_bdf_checks() -> None:
     """Time-integrator check (tests/cpp/bdf_check.cpp) over host vectors (no
     device use) and over device vectors, and the heat example
@@ -144,6 +156,7 @@ def build_all(oracle: bool = True) -> None:
     build_fsx()
     build_cpp_tests()
     build_config_check()
+    build_amg_setup_check()
     build_bdf_checks()
 
 

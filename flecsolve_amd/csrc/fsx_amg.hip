@@ -46,6 +46,7 @@
 //    aggregate (create_aux, 280-303), and that is kept.
 // The priority queue is a bucket per priority with the candidates in
 // insertion order -- the order of the reference's multimap, in O(1) per step.
+#include "fsx_amg_setup.hpp"
 #include "fsx_internal.hpp"
 
 #include <algorithm>
@@ -55,278 +56,6 @@
 #include <vector>
 
 namespace fsx {
-
-// ---- host-side structures ---------------------------------------------------
-
-struct hcsr {
-	int64_t n = 0;
-	std::vector<int64_t> rp{0}, ci;
-	std::vector<double> v;
-};
-
-using aggregates = std::vector<std::vector<int64_t>>;
-constexpr int64_t kNoAgg = -1;
-
-// Candidates keyed by (priority, insertion sequence): one list per priority.
-class prospect {
-public:
-	explicit prospect(int64_t n) : prio_(n, -1), prev_(n, -1), next_(n, -1) {}
-	bool contains(int64_t e) const { return prio_[e] >= 0; }
-	bool empty() const { return count_ == 0; }
-	void add(int64_t e) { // before build(): priority 0
-		prio_[e] = 0;
-		++count_;
-	}
-	void bump(int64_t e) { ++prio_[e]; }
-	// append every candidate in ascending id (the multimap built from the
-	// id-ordered map)
-	void build() {
-		for (int64_t e = 0; e < (int64_t)prio_.size(); ++e)
-			if (prio_[e] >= 0) push_back(e);
-	}
-	int64_t pop() {
-		while (lowest_ < (int64_t)head_.size() && head_[lowest_] < 0) ++lowest_;
-		const int64_t e = head_[lowest_];
-		unlink(e);
-		prio_[e] = -1;
-		--count_;
-		return e;
-	}
-	void remove(int64_t e) {
-		unlink(e);
-		prio_[e] = -1;
-		--count_;
-	}
-	void decrement(int64_t e) {
-		if (!contains(e)) return;
-		unlink(e);
-		--prio_[e];
-		push_back(e);
-	}
-
-private:
-	void push_back(int64_t e) {
-		const int64_t p = prio_[e];
-		if (p >= (int64_t)head_.size()) {
-			head_.resize(p + 1, -1);
-			tail_.resize(p + 1, -1);
-		}
-		prev_[e] = tail_[p];
-		next_[e] = -1;
-		if (tail_[p] >= 0) next_[tail_[p]] = e;
-		else head_[p] = e;
-		tail_[p] = e;
-		lowest_ = std::min(lowest_, p);
-	}
-	void unlink(int64_t e) {
-		const int64_t p = prio_[e];
-		if (prev_[e] >= 0) next_[prev_[e]] = next_[e];
-		else head_[p] = next_[e];
-		if (next_[e] >= 0) prev_[next_[e]] = prev_[e];
-		else tail_[p] = prev_[e];
-		prev_[e] = next_[e] = -1;
-	}
-	std::vector<int64_t> prio_, prev_, next_, head_, tail_;
-	int64_t lowest_ = 0, count_ = 0;
-};
-
-// strong-neighbour sets: sorted, no duplicates (std::set order)
-struct soc_sets {
-	std::vector<int64_t> ptr{0}, idx;
-	bool has(int64_t r, int64_t c) const {
-		return std::binary_search(idx.begin() + ptr[r], idx.begin() + ptr[r + 1], c);
-	}
-};
-
-aggregates pairwise_agg(float beta, const hcsr & A, bool checkdd) {
-	const int64_t n = A.n;
-	prospect unmarked(n);
-	for (int64_t r = 0; r < n; ++r) {
-		if (checkdd) {
-			int offd_sum = 0;
-			double diag_val = 0.;
-			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) {
-				if (r != A.ci[off]) offd_sum += std::abs(A.v[off]);
-				else diag_val = A.v[off];
-			}
-			if (!(diag_val > 5 * offd_sum)) unmarked.add(r);
-		}
-		else
-			unmarked.add(r);
-	}
-	soc_sets soc;
-	soc.idx.reserve(A.ci.size());
-	std::vector<int64_t> tmp;
-	const int64_t nnz = (int64_t)A.v.size();
-	for (int64_t r = 0; r < n; ++r) {
-		// smallest stored value of the row; an empty row reads the next
-		// stored value, past the last the largest double (min_element's end)
-		double maxc;
-		if (A.rp[r] < A.rp[r + 1])
-			maxc = *std::min_element(A.v.begin() + A.rp[r], A.v.begin() + A.rp[r + 1]);
-		else
-			maxc = A.rp[r + 1] < nnz ? A.v[A.rp[r + 1]] : std::numeric_limits<double>::max();
-		tmp.clear();
-		for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) {
-			const int64_t c = A.ci[off];
-			if (r != c && (!checkdd || unmarked.contains(c)) && c < n &&
-			    A.v[off] < (beta * maxc))
-				tmp.push_back(c);
-		}
-		std::sort(tmp.begin(), tmp.end());
-		tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-		soc.idx.insert(soc.idx.end(), tmp.begin(), tmp.end());
-		soc.ptr.push_back((int64_t)soc.idx.size());
-	}
-	for (int64_t r = 0; r < n; ++r)
-		for (int64_t q = soc.ptr[r]; q < soc.ptr[r + 1]; ++q) unmarked.bump(soc.idx[q]);
-	unmarked.build();
-
-	aggregates agg;
-	auto update_priorities = [&](int64_t k) {
-		for (int64_t q = soc.ptr[k]; q < soc.ptr[k + 1]; ++q) unmarked.decrement(soc.idx[q]);
-	};
-	while (!unmarked.empty()) {
-		const int64_t sel = unmarked.pop();
-		double best = std::numeric_limits<double>::max();
-		int64_t pair = -1;
-		for (int64_t off = A.rp[sel]; off < A.rp[sel + 1]; ++off) {
-			const int64_t c = A.ci[off];
-			if (unmarked.contains(c) && A.v[off] < best) {
-				best = A.v[off];
-				pair = c;
-			}
-		}
-		if (pair >= 0 && soc.has(sel, pair)) {
-			unmarked.remove(pair);
-			update_priorities(pair);
-			agg.push_back({sel, pair});
-		}
-		else
-			agg.push_back({sel});
-		update_priorities(sel);
-	}
-	return agg;
-}
-
-// Rows of a coarse matrix: per coarse row, (column, value) pairs in the
-// traversal order; duplicates summed in that order, columns ascending (the
-// reference's std::map accumulation).
-static void emit_row(std::vector<std::pair<int64_t, double>> & acc, hcsr & C) {
-	std::stable_sort(acc.begin(), acc.end(),
-	                 [](const auto & a, const auto & b) { return a.first < b.first; });
-	size_t i = 0;
-	while (i < acc.size()) {
-		const int64_t key = acc[i].first;
-		double s = 0.;
-		for (; i < acc.size() && acc[i].first == key; ++i) s += acc[i].second;
-		C.ci.push_back(key);
-		C.v.push_back(s);
-	}
-	C.rp.push_back((int64_t)C.ci.size());
-}
-
-// create_aux: aggregate ids through a map that answers 0 for rows outside
-// every aggregate (coarsen.hh:280-303)
-hcsr create_aux(const hcsr & A, const aggregates & agg) {
-	std::vector<int64_t> aggt(A.n, 0);
-	for (size_t i = 0; i < agg.size(); ++i)
-		for (int64_t f : agg[i]) aggt[f] = (int64_t)i;
-	hcsr C;
-	C.n = (int64_t)agg.size();
-	std::vector<std::pair<int64_t, double>> acc;
-	for (size_t rc = 0; rc < agg.size(); ++rc) {
-		acc.clear();
-		for (int64_t r : agg[rc])
-			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off)
-				acc.emplace_back(aggt[A.ci[off]], A.v[off]);
-		emit_row(acc, C);
-	}
-	return C;
-}
-
-aggregates agg_union(const aggregates & a1, const aggregates & a2) {
-	aggregates a(a2.size());
-	for (size_t i = 0; i < a2.size(); ++i)
-		for (int64_t e2 : a2[i])
-			for (int64_t e1 : a1[e2]) a[i].push_back(e1);
-	return a;
-}
-
-aggregates multi_pairwise_agg(float beta, const hcsr & A, int passes) {
-	aggregates agg1 = pairwise_agg(beta, A, true);
-	hcsr aux = create_aux(A, agg1);
-	aggregates agg2 = pairwise_agg(beta, aux, false);
-	if (passes == 2) return agg_union(agg1, agg2);
-	hcsr aux1 = create_aux(aux, agg2);
-	aggregates agg3 = pairwise_agg(beta, aux1, false);
-	return agg_union(agg_union(agg1, agg2), agg3);
-}
-
-// coarsen_with_aggregates (coarsen.hh:455-488): aggt = kNoAgg outside
-// every aggregate, those columns dropped
-hcsr galerkin(const hcsr & A, const aggregates & agg, const std::vector<int64_t> & aggt) {
-	hcsr C;
-	C.n = (int64_t)agg.size();
-	std::vector<std::pair<int64_t, double>> acc;
-	for (size_t rc = 0; rc < agg.size(); ++rc) {
-		acc.clear();
-		for (int64_t r : agg[rc])
-			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off) {
-				const int64_t a = aggt[A.ci[off]];
-				if (a != kNoAgg) acc.emplace_back(a, A.v[off]);
-			}
-		emit_row(acc, C);
-	}
-	return C;
-}
-
-// Dense LU with partial pivoting (row of the largest |a_ik|, the first on
-// ties), row-major, in place; then the two triangular solves.
-struct dense_lu {
-	int64_t n = 0;
-	std::vector<double> a;
-	std::vector<int64_t> piv;
-	void factor(const hcsr & A) {
-		n = A.n;
-		a.assign((size_t)(n * n), 0.0);
-		for (int64_t r = 0; r < n; ++r)
-			for (int64_t off = A.rp[r]; off < A.rp[r + 1]; ++off)
-				a[(size_t)(r * n + A.ci[off])] = A.v[off];
-		piv.assign((size_t)n, 0);
-		for (int64_t k = 0; k < n; ++k) {
-			int64_t p = k;
-			double m = std::abs(a[(size_t)(k * n + k)]);
-			for (int64_t i = k + 1; i < n; ++i)
-				if (std::abs(a[(size_t)(i * n + k)]) > m) {
-					m = std::abs(a[(size_t)(i * n + k)]);
-					p = i;
-				}
-			piv[(size_t)k] = p;
-			if (p != k)
-				for (int64_t j = 0; j < n; ++j) std::swap(a[(size_t)(k * n + j)], a[(size_t)(p * n + j)]);
-			const double d = a[(size_t)(k * n + k)];
-			if (d == 0.0) continue;
-			for (int64_t i = k + 1; i < n; ++i) {
-				const double l = a[(size_t)(i * n + k)] / d;
-				a[(size_t)(i * n + k)] = l;
-				for (int64_t j = k + 1; j < n; ++j)
-					a[(size_t)(i * n + j)] = a[(size_t)(i * n + j)] - l * a[(size_t)(k * n + j)];
-			}
-		}
-	}
-	void solve(const double * b, double * x) const {
-		std::vector<double> y(b, b + n);
-		for (int64_t k = 0; k < n; ++k) std::swap(y[(size_t)k], y[(size_t)piv[(size_t)k]]);
-		for (int64_t i = 0; i < n; ++i)
-			for (int64_t j = 0; j < i; ++j) y[(size_t)i] = y[(size_t)i] - a[(size_t)(i * n + j)] * y[(size_t)j];
-		for (int64_t i = n - 1; i >= 0; --i) {
-			for (int64_t j = i + 1; j < n; ++j) y[(size_t)i] = y[(size_t)i] - a[(size_t)(i * n + j)] * y[(size_t)j];
-			y[(size_t)i] = y[(size_t)i] / a[(size_t)(i * n + i)];
-		}
-		std::copy(y.begin(), y.end(), x);
-	}
-};
 
 // ---- device kernels -----------------------------------------------------------
 
@@ -591,23 +320,23 @@ int fsx_amg_create(fsx_ctx * ctx, const fsx_csr * A, const fsx_amg_settings * s,
 		M->lv[0].nnz = (int64_t)H.ci.size();
 		// bound_solver::setup (ua.hh:110-139): coarsen until a level has
 		// fewer than min_coarse rows or max_levels levels exist
-		for (int i = 0; i + 1 < s->max_levels; ++i) {
+		const bool i32 = A->it == FSX_I32;
+		H = coarsen_levels(std::move(H), s->beta, s->pairwise_passes, s->max_levels,
+		                   s->min_coarse,
+		                   [&](int, const hcsr &, const aggregates & agg,
+		                       const std::vector<int64_t> & aggt, const hcsr & Cm) {
 			auto & F = M->lv.back();
-			aggregates agg = multi_pairwise_agg(s->beta, H, s->pairwise_passes);
-			F.aggt_host.assign((size_t)H.n, kNoAgg);
-			for (size_t a = 0; a < agg.size(); ++a)
-				for (int64_t f : agg[a]) F.aggt_host[(size_t)f] = (int64_t)a;
-			hcsr Cm = galerkin(H, agg, F.aggt_host);
+			F.aggt_host = aggt;
 			// each aggregate's fine rows, ascending (restrict::ave sums them
 			// in fine-row order)
 			std::vector<int64_t> mptr(agg.size() + 1, 0), midx;
+			midx.reserve(agg.idx.size());
 			for (size_t a = 0; a < agg.size(); ++a) {
-				std::vector<int64_t> m = agg[a];
-				std::sort(m.begin(), m.end());
-				midx.insert(midx.end(), m.begin(), m.end());
+				const size_t b0 = midx.size();
+				midx.insert(midx.end(), agg[a].begin(), agg[a].end());
+				std::sort(midx.begin() + (std::ptrdiff_t)b0, midx.end());
 				mptr[a + 1] = (int64_t)midx.size();
 			}
-			const bool i32 = F.A->it == FSX_I32;
 			F.aggt = i32 ? upload<int32_t>(ctx, F.aggt_host) : upload<int64_t>(ctx, F.aggt_host);
 			F.mptr = i32 ? upload<int32_t>(ctx, mptr) : upload<int64_t>(ctx, mptr);
 			F.midx = i32 ? upload<int32_t>(ctx, midx) : upload<int64_t>(ctx, midx);
@@ -628,9 +357,7 @@ int fsx_amg_create(fsx_ctx * ctx, const fsx_csr * A, const fsx_amg_settings * s,
 			rc(fsx_csr_plan(C.A, &kind, &lanes));
 			if (kind == 1) rc(fsx_csr_set_plan(C.A, 2, 0));
 			M->lv.push_back(std::move(C));
-			H = std::move(Cm);
-			if (H.n < s->min_coarse) break;
-		}
+		});
 		for (size_t l = 0; l < M->lv.size(); ++l) {
 			auto & L = M->lv[l];
 			if (l + 1 < M->lv.size() && s->smoother == 0) rc(fsx_gs_create(ctx, L.A, 1, &L.G));
