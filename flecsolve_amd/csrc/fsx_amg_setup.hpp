@@ -21,6 +21,8 @@
 #include <thread>
 #include <vector>
 
+#include "fsx_host_threads.hpp"
+
 namespace fsx {
 
 // ---- host-side structures ---------------------------------------------------
@@ -57,45 +59,6 @@ struct aggregates {
 		ptr.push_back((int64_t)idx.size());
 	}
 };
-
-// Host threads for the per-row parts of the setup: the hardware's, at most
-// kSetupThreads, and at most FSX_SETUP_THREADS when that is set.
-constexpr unsigned kSetupThreads = 16;
-
-inline unsigned setup_threads(int64_t n) {
-	unsigned t = std::max(1u, std::min(kSetupThreads, std::thread::hardware_concurrency()));
-	if (const char * e = std::getenv("FSX_SETUP_THREADS")) {
-		const long v = std::strtol(e, nullptr, 10);
-		if (v >= 1) t = std::min<unsigned>(t, (unsigned)v);
-	}
-	// at least 4096 rows per thread
-	return (unsigned)std::max<int64_t>(1, std::min<int64_t>(t, n / 4096));
-}
-
-// fn(t, lo, hi) on rows [lo, hi): contiguous blocks, block t on thread t.
-// An exception in a block is rethrown on the calling thread after the join.
-template<class F>
-void for_row_blocks(int64_t n, unsigned nt, F && fn) {
-	if (nt <= 1) {
-		fn(0u, (int64_t)0, n);
-		return;
-	}
-	std::vector<std::thread> th;
-	std::vector<std::exception_ptr> err(nt);
-	th.reserve(nt);
-	for (unsigned t = 0; t < nt; ++t)
-		th.emplace_back([&, t] {
-			try {
-				fn(t, n * t / nt, n * (t + 1) / nt);
-			}
-			catch (...) {
-				err[t] = std::current_exception();
-			}
-		});
-	for (auto & x : th) x.join();
-	for (auto & e : err)
-		if (e) std::rethrow_exception(e);
-}
 
 // Candidates keyed by (priority, insertion sequence): the order of the
 // reference's multimap.  After build() priorities only decrease, so each

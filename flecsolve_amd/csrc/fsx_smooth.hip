@@ -35,6 +35,7 @@
 // of that at once, then b and the x gathers, then stores: two dependent
 // steps instead of four through row ids and row pointers.  The copy's
 // values are refreshed from the matrix when its values change in place.
+#include "fsx_host_threads.hpp"
 #include "fsx_internal.hpp"
 
 #include <algorithm>
@@ -195,8 +196,11 @@ void build_dir(fsx_ctx * ctx, const std::vector<int64_t> & rp, const std::vector
 	std::vector<int32_t> slen(n);
 	std::vector<int64_t> dfrom(n, -1);
 	D.base.assign(nl, 0);
-	size_t nslots = 0;
-	for (size_t l = 0; l < nl; ++l) {
+	// levels are independent here: each level's width, then each level's
+	// slots, on host threads taking levels in turn
+	const unsigned nt = setup_threads(n);
+	std::vector<int64_t> width(nl, 0);
+	for_items(nl, nt, [&](size_t l) {
 		int64_t w = 0;
 		for (int64_t p = D.lev[l]; p < D.lev[l + 1]; ++p) {
 			const int64_t r = order[p];
@@ -204,12 +208,16 @@ void build_dir(fsx_ctx * ctx, const std::vector<int64_t> & rp, const std::vector
 			for (int64_t e = rp[r]; e < rp[r + 1]; ++e) len += ci[e] != r;
 			w = std::max(w, len);
 		}
+		width[l] = w;
+	});
+	size_t nslots = 0;
+	for (size_t l = 0; l < nl; ++l) {
 		D.base[l] = (int64_t)nslots;
-		nslots += (size_t)(w * (D.lev[l + 1] - D.lev[l]));
+		nslots += (size_t)(width[l] * (D.lev[l + 1] - D.lev[l]));
 	}
 	std::vector<I> scol(nslots, (I)-1);
 	std::vector<int64_t> vfrom(nslots, -1);
-	for (size_t l = 0; l < nl; ++l) {
+	for_items(nl, nt, [&](size_t l) {
 		const int64_t cnt = D.lev[l + 1] - D.lev[l];
 		for (int64_t p = D.lev[l]; p < D.lev[l + 1]; ++p) {
 			const int64_t r = order[p], t = p - D.lev[l];
@@ -235,7 +243,7 @@ void build_dir(fsx_ctx * ctx, const std::vector<int64_t> & rp, const std::vector
 					if (cm.of(ci[e]) != cr) put(e, false);
 			slen[p] = (int32_t)j;
 		}
-	}
+	});
 	D.nslots = nslots;
 	D.srow = upload(ctx, srow);
 	D.slen = upload(ctx, slen);
@@ -330,11 +338,13 @@ int fsx_gs_create(fsx_ctx * ctx, const fsx_csr * A, int ncolours, fsx_gs ** out)
 			FSX_HIP(hipStreamSynchronize(ctx->stream));
 			for (int64_t i = 0; i <= n; ++i)
 				rp[i] = isz == 4 ? ((int32_t *)hrp.data())[i] : ((int64_t *)hrp.data())[i];
-			for (int64_t e = 0; e < A->nnz; ++e)
-				ci[e] = isz == 4 ? ((int32_t *)hci.data())[e] : ((int64_t *)hci.data())[e];
+			for_row_blocks(A->nnz, setup_threads(A->nnz), [&](unsigned, int64_t lo, int64_t hi) {
+				for (int64_t e = lo; e < hi; ++e) {
+					ci[e] = isz == 4 ? ((int32_t *)hci.data())[e] : ((int64_t *)hci.data())[e];
+					FSX_REQUIRE(ci[e] >= 0 && ci[e] < n, "hybrid_gs: column index out of range");
+				}
+			});
 		}
-		for (int64_t e = 0; e < A->nnz; ++e)
-			FSX_REQUIRE(ci[e] >= 0 && ci[e] < n, "hybrid_gs: column index out of range");
 		auto G = std::make_unique<fsx_gs>();
 		G->ctx = ctx;
 		G->A = A;
