@@ -67,7 +67,11 @@
 // wave-uniform address select cost more issue time than the 8% of bytes it
 // saved (FP32 CG product 0.120 -> 0.127 ms, and the FP64 products slower
 // too); a branch per load split the merged scalar loads of offsets and
-// masks; not used.  Taking a wave's slices two at a time with every load of
+// masks; not used.  Taking x at the offsets -1 and +1 from the neighbouring
+// lanes of the aligned x[r] load (lane shifts, one broadcast load per slice
+// edge) instead of gathers that straddle a cache line: FP32 CG product
+// 0.124 -> 0.143 ms with DPP wave shifts, 0.148 ms with ds_bpermute; not
+// used.  Taking a wave's slices two at a time with every load of
 // both issued before the first gather (more loads in flight per wave)
 // raised the FP32 CG product's registers to 83 (0.141 ms) or, held to 64,
 // gave 0.125 ms against 0.118; not used.
