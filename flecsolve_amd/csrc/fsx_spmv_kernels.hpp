@@ -71,7 +71,10 @@
 // lanes of the aligned x[r] load (lane shifts, one broadcast load per slice
 // edge) instead of gathers that straddle a cache line: FP32 CG product
 // 0.124 -> 0.143 ms with DPP wave shifts, 0.148 ms with ds_bpermute; not
-// used.  Taking a wave's slices two at a time with every load of
+// used.  Issue order within a tile (same loads, same sums): the gathers
+// before the value loads 0.120 -> 0.125 ms, the -1/+1 gathers after the
+// value loads (so the x[r] lines they share are no longer pending)
+// 0.137 ms; values first, then the gathers in offset order, is kept.  Taking a wave's slices two at a time with every load of
 // both issued before the first gather (more loads in flight per wave)
 // raised the FP32 CG product's registers to 83 (0.141 ms) or, held to 64,
 // gave 0.125 ms against 0.118; not used.
