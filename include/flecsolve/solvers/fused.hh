@@ -16,7 +16,8 @@
 //   solve_info info = S.get().apply(b, x);
 //
 // ir::fused is the mixed-precision refinement (FP32 inner CG on the
-// symmetrically Jacobi-scaled matrix) with the same interface.
+// symmetrically Jacobi-scaled matrix) and ir::rr the FP32 CG with FP64
+// residual replacement on that matrix, with the same interface.
 //
 // The matrix is held by handle: values changed in place (fsx_csr_set_values,
 // fsx_csr_shift_scale) are seen by the next solve.
@@ -112,6 +113,49 @@ struct fused : op::base<> {
 	solver_settings settings, inner;
 	mat::device_csr A, A32;
 	int max_outer;
+};
+
+// FP32 CG with FP64 residual replacement (fsx_ir_cg_rr_solve; no reference
+// counterpart) as an operator, with ir::fused's interface: apply(b, x)
+// solves from x's current value until |b - A x| / |b| <= settings.rtol
+// (FP64), one FP32 CG on S A S whose recursive residual is replaced by the
+// true FP64 one after each segment of inner.rtol reduction (e.g. 3e-2; at
+// most inner.maxiter iterations a segment), the search direction kept.
+// solve_info: status, iters = FP32 iterations, restarts = segments,
+// res_norm_final = final relative residual.
+struct rr : op::base<> {
+	rr(const solver_settings & s, const solver_settings & inner, mat::device_csr A,
+	   int max_segments = 100)
+		: settings(s), inner(inner), A(std::move(A)), max_segments(max_segments) {
+		fsx_csr * a32 = nullptr;
+		fsx_check(fsx_csr_convert(this->A.context(), this->A.handle(), FSX_F32, &a32),
+		          "rr: FP32 copy");
+		A32 = mat::device_csr::adopt(this->A.context(), a32);
+		refresh();
+	}
+	void refresh() {
+		fsx_check(fsx_csr_set_values_f32_sym_scaled(A.context(), A32.handle(), A.handle()),
+		          "rr: scaled FP32 values");
+	}
+	template<class B, class X>
+	solve_info apply(const B & b, X & x) const {
+		fsx_settings in = detail::to_fsx(inner);
+		fsx_ir_info ii{};
+		fsx_check(fsx_ir_cg_rr_solve(A.context(), A.handle(), A32.handle(), b.data.get(),
+		                             x.data.get(), (double)settings.rtol, max_segments, &in,
+		                             &ii),
+		          "rr solve");
+		solve_info info;
+		info.status = static_cast<solve_info::stop_reason>(ii.status);
+		info.iters = ii.inner_iters_total;
+		info.restarts = ii.outer_iters;
+		info.rhs_norm = (float)ii.rhs_norm;
+		info.res_norm_final = (float)ii.rel_res_final;
+		return info;
+	}
+	solver_settings settings, inner;
+	mat::device_csr A, A32;
+	int max_segments;
 };
 
 }

@@ -307,6 +307,22 @@ int main(int argc, char ** argv) {
 		            "true rel res %.3e, |x_ir - x_cg| / |x_cg| %.2e\n",
 		            c1.iters, c2.restarts, c2.iters, rel2, dx);
 		fails += !c1.success() || !c2.success() || rel2 > 1.01e-10 || dx > 1e-6;
+		// FP32 CG with FP64 residual replacement behind the same interface
+		vec::device<double> y3(ctx, n * n);
+		solver_settings seg;
+		seg.maxiter = 3000;
+		seg.rtol = 3e-2f;
+		auto RR = op::make_shared<ir::rr>(cs, seg, Am);
+		y3.set_scalar(0.0);
+		solve_info c3 = RR.get().apply(rhs, y3);
+		A.residual(rhs, y3, r);
+		const double rel3 = r.l2norm().get() / rhs.l2norm().get();
+		y3.axpy(-1.0, y1, y3);
+		const double dx3 = y3.l2norm().get() / y1.l2norm().get();
+		std::printf("residual replacement: %d segments, %d iterations, true rel res %.3e, "
+		            "|x_rr - x_cg| / |x_cg| %.2e\n",
+		            c3.restarts, c3.iters, rel3, dx3);
+		fails += !c3.success() || rel3 > 1.01e-10 || dx3 > 1e-6 || c3.iters > c2.iters;
 	}
 	{
 		// the reference's parcsr test pattern (matrices/test/par_csr.cc:21-35):

@@ -2,8 +2,9 @@
 use theirs: tests/cpp/cg_dropin.cpp mirrors flecsolve/solvers/test/cg.cc with
 its A-norm / Trefethen-Bau diagnostic, plus GMRES(30)+Dinv, BiCGSTAB, CG with
 a hybrid Gauss-Seidel preconditioner, CG / GMRES / BiCGSTAB chosen by
-krylov_factory from config files, CG and the mixed-precision refinement
-through the fused solver policies, CG over the parallel CSR operator as
+krylov_factory from config files, CG, the mixed-precision refinement and
+the FP32 CG with FP64 residual replacement (ir::rr) through the fused solver
+policies, CG over the parallel CSR operator as
 flecsolve/matrices/test/par_csr.cc runs it (b = 0, x0 = 2; 1 and 4 colours),
 and CG on a Matrix Market file read through the matrix_market / coo headers
 (gpu)."""
