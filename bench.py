@@ -377,7 +377,18 @@ def bench_diffusion(a, fsx, P):
         if ii.status != 1:
             sys.exit("refinement did not reach 1e-10")
     ir_s = sum(times) / len(times)
+    # a caller holding b and x in host memory: one upload of b and one
+    # download of x around the solve (pageable numpy arrays; the matrix is
+    # uploaded once per matrix and not counted)
+    ctx.sync()
+    t = time.perf_counter()
+    b_up = ctx.vec(b)
+    ctx.sync()
+    t_up = time.perf_counter() - t
+    t = time.perf_counter()
     xir = x.get()
+    t_down = time.perf_counter() - t
+    del b_up
     rel_ir = true_rel_res(fsx, ctx, A64, bd, x)
     roofline["share_of_step"] = round(k1_ms * 1e-3 * ii.inner_iters_total / ir_s, 3)
     # the other mixed-precision method once (after a warm run), beside it
@@ -455,6 +466,10 @@ def bench_diffusion(a, fsx, P):
             "fp64_cg": {"seconds": round(cg_s, 4), "iters": info64.iters,
                         "true_rel_residual": rel64},
             "ir_speedup_vs_fp64_cg": round(cg_s / ir_s, 3),
+            "host_boundary": {"upload_b_s": round(t_up, 4), "download_x_s": round(t_down, 4),
+                              "pcie_inclusive_seconds": round(ir_s + t_up + t_down, 4),
+                              "note": "b uploaded and x downloaded around one solve "
+                                      "(pageable host arrays); value keeps them in HBM"},
             "ir_" + other: other_block,
             "generic_cg": generic,
         },
@@ -1062,6 +1077,29 @@ def bench_smoother(a, fsx, P):
         if i >= a.warmup:
             ts.append(dt)
     t_sweep = sum(ts) / len(ts)
+    # the one-colour stencil on a grid runs in 8^3 tiles, one launch per tile
+    # diagonal; the same sweep on the level copies (one launch per level),
+    # beside it
+    tiles = os.environ.get("FSX_GS_TILES", "1") != "0"
+    launches = 3 * ((n + 7) // 8) - 2 if tiles else lf
+    os.environ["FSX_GS_TILES"] = "0"
+    try:
+        G0 = fsx.HybridGS(A, ncolours=1, sweep=fsx.HybridGS.SYMMETRIC, nrelax=1)
+    finally:
+        if tiles:
+            del os.environ["FSX_GS_TILES"]
+    x0 = ctx.zeros(N)
+    G0.apply(bd, x0)
+    ctx.sync()
+    same_paths = bool(np.array_equal(x0.get(), first))
+    t = time.perf_counter()
+    for _ in range(10):
+        G0.apply(bd, x0)
+    ctx.sync()
+    t_levels = (time.perf_counter() - t) / 10
+    del G0, x0
+    log(f"level copies: {t_levels*1e3:.3f} ms per symmetric sweep; tiles and levels "
+        f"bit-identical: {same_paths}")
     # algorithmic bytes of one direction: the SpMV's (values, column ids,
     # row pointers, x read once, x written once) plus b read once
     per_dir = spmv_bytes(N, nnz, 8) + 8 * N
@@ -1082,12 +1120,17 @@ def bench_smoother(a, fsx, P):
         "config": {"workload": f"hybrid_gs_symmetric_{n}^3_lognormal_sigma{a.sigma:g}_1colour",
                    "rows": N, "nnz": nnz, "levels_forward": lf, "levels_backward": lb,
                    "analysis_seconds": round(t_an, 3),
-                   "us_per_level": round(t_sweep / (lf + lb) * 1e6, 2)},
+                   "path": "8^3 tiles in shared memory, one launch per tile diagonal"
+                           if tiles else "level copies, one launch per level",
+                   "launches_per_direction": launches,
+                   "us_per_launch": round(t_sweep / (2 * launches) * 1e6, 2),
+                   "level_copies_ms": round(t_levels * 1e3, 3),
+                   "tiles_equal_level_copies": same_paths},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": 2 * per_dir,
-                     "note": "a sweep is a chain of (levels) dependent launches, one per "
-                             "dependency level; latency, not bandwidth, bounds it"},
+                     "note": "a sweep is a chain of dependent launches (tile diagonals, or "
+                             "dependency levels); latency, not bandwidth, bounds it"},
     }
     if not a.no_cpu_baseline:
         from oracle import oracle as O

@@ -39,6 +39,7 @@
 #include "fsx_internal.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <map>
 #include <vector>
 
@@ -110,6 +111,112 @@ __global__ __launch_bounds__(256) void gs_snapshot_kernel(size_t n, const double
 	if (i < n) xold[i] = x[i];
 }
 
+// ---- stencil matrices on a grid: tiles relaxed in shared memory ------------
+//
+// A one-colour matrix whose every entry couples a row to itself or to a face
+// neighbour of its cell on an nx x ny x nz grid in natural order (offsets
+// +-1, +-nx, +-nx ny; columns ascending in each row) is swept in tiles of
+// kGridTile^3 cells.  A tile needs its faces' neighbours from the tiles
+// before it only, so tiles on one diagonal I + J + K run at once (face
+// neighbours always lie on the diagonals before and after: the halo holds
+// finished values from the one and untouched values from the other), one
+// launch per diagonal.  In a tile, one thread per cell; x for the tile and
+// a one-cell halo sits in shared memory and the cells are relaxed diagonal
+// by diagonal (i + j + k), a barrier between, each with the reference's
+// arithmetic in CSR order: the sequential sweep's result, bit for bit, in
+// nx/8 + ny/8 + nz/8 - 2 launches instead of nx + ny + nz - 2.
+constexpr int kGridTile = 8;
+constexpr int kGridHalo = kGridTile + 2;
+constexpr int kGridLds = kGridHalo * kGridHalo * kGridHalo;
+constexpr int kGridThreads = kGridTile * kGridTile * kGridTile;
+
+struct grid_desc {
+	int64_t nx, ny, nz, nxy; // grid; nxy = nx ny
+	int ti, tj, tk;          // tiles per dimension
+};
+
+// Entry slots in column order: 0 -nxy, 1 -nx, 2 -1, 3 the diagonal, 4 +1,
+// 5 +nx, 6 +nxy; gval[slot * n + r], bit slot of gmask[r] set when present.
+__host__ __device__ inline int grid_slot(int64_t o, const grid_desc & g) {
+	return o == -g.nxy ? 0 : o == -g.nx ? 1 : o == -1 ? 2 : o == 0 ? 3 : o == 1 ? 4
+	     : o == g.nx ? 5 : 6;
+}
+
+template<class I>
+__global__ __launch_bounds__(256) void gs_grid_fill_kernel(int64_t n, grid_desc g,
+                                                           const I * __restrict__ rp,
+                                                           const I * __restrict__ ci,
+                                                           const double * __restrict__ vals,
+                                                           double * __restrict__ gval,
+                                                           uint8_t * __restrict__ gmask) {
+	const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if (r >= n) return;
+	double v[7] = {0., 0., 0., 0., 0., 0., 0.};
+	unsigned m = 0;
+	for (I e = rp[r]; e < rp[r + 1]; ++e) {
+		const int sl = grid_slot((int64_t)ci[e] - r, g);
+		v[sl] = vals[e];
+		m |= 1u << sl;
+	}
+#pragma unroll
+	for (int k = 0; k < 7; ++k) gval[(int64_t)k * n + r] = v[k];
+	gmask[r] = (uint8_t)m;
+}
+
+template<bool FWD>
+__global__ __launch_bounds__(kGridThreads) void gs_tile_kernel(
+	grid_desc g, const int32_t * __restrict__ tiles, int64_t first,
+	const double * __restrict__ gval, const uint8_t * __restrict__ gmask, int64_t n,
+	const double * __restrict__ b, double * x) {
+	constexpr int T = kGridTile, H = kGridHalo;
+	__shared__ double xs[kGridLds];
+	const int tid = threadIdx.x;
+	const int tile = tiles[first + blockIdx.x];
+	const int64_t i0 = (int64_t)(tile % g.ti) * T;
+	const int64_t j0 = (int64_t)((tile / g.ti) % g.tj) * T;
+	const int64_t k0 = (int64_t)(tile / (g.ti * g.tj)) * T;
+	// x on the tile and its halo (cells outside the grid: never read)
+	for (int q = tid; q < kGridLds; q += kGridThreads) {
+		const int64_t gi = i0 + q % H - 1, gj = j0 + (q / H) % H - 1, gk = k0 + q / (H * H) - 1;
+		xs[q] = (gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz)
+		            ? x[gi + g.nx * (gj + g.ny * gk)]
+		            : 0.0;
+	}
+	const int li = tid % T, lj = (tid / T) % T, lk = tid / (T * T);
+	const int64_t gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+	const bool live = gi < g.nx && gj < g.ny && gk < g.nz;
+	const int64_t r = live ? gi + g.nx * (gj + g.ny * gk) : 0;
+	double v[7];
+	unsigned m = 0;
+	double br = 0.;
+#pragma unroll
+	for (int k = 0; k < 7; ++k) v[k] = 0.;
+	if (live) {
+#pragma unroll
+		for (int k = 0; k < 7; ++k) v[k] = gval[(int64_t)k * n + r];
+		m = gmask[r];
+		br = b[r];
+	}
+	const int me = (li + 1) + H * ((lj + 1) + H * (lk + 1));
+	const int mine = li + lj + lk;
+	__syncthreads();
+	for (int L = 0; L < 3 * T - 2; ++L) {
+		if (live && mine == (FWD ? L : 3 * T - 3 - L)) {
+			double rsum = 0.;
+			if (m & 1u) rsum += v[0] * xs[me - H * H];
+			if (m & 2u) rsum += v[1] * xs[me - H];
+			if (m & 4u) rsum += v[2] * xs[me - 1];
+			if (m & 16u) rsum += v[4] * xs[me + 1];
+			if (m & 32u) rsum += v[5] * xs[me + H];
+			if (m & 64u) rsum += v[6] * xs[me + H * H];
+			const double dinv = 1. / v[3];
+			xs[me] = dinv * (br - rsum);
+		}
+		__syncthreads();
+	}
+	if (live) x[r] = xs[me];
+}
+
 } // namespace fsx
 
 using namespace fsx;
@@ -128,6 +235,13 @@ struct fsx_gs {
 		size_t nslots = 0;
 	} fw, bw;
 	std::unique_ptr<dbuf> xold;
+	// stencil matrices on a grid (one colour): tiles swept by diagonal;
+	// tile_ids ordered by diagonal, diagonal d's at [dptr[d], dptr[d + 1]),
+	// the values by entry slot (gval) with their presence bits (gmask)
+	bool tiles = false;
+	grid_desc gd{};
+	std::vector<int64_t> dptr;
+	std::unique_ptr<dbuf> tile_ids, gval, gmask;
 	uint64_t version = 0; // of the matrix values the copies hold
 	struct graph {
 		hipGraph_t g = nullptr;
@@ -180,6 +294,85 @@ std::unique_ptr<dbuf> upload(fsx_ctx * ctx, const std::vector<T> & h) {
 		                       ctx->stream));
 	FSX_HIP(hipStreamSynchronize(ctx->stream)); // h may go out of scope
 	return d;
+}
+
+// The grid of a one-colour stencil matrix, if it is one: every entry's
+// offset c - r is 0 or +-1, +-nx, +-nx ny for a face neighbour of r's cell
+// (no wrap-around), columns strictly ascending in each row, and the grid
+// nx x ny x nz covers the rows.  The positive offsets give nx and nx ny
+// (one offset: a line, two: a plane).
+bool detect_grid(const std::vector<int64_t> & rp, const std::vector<int64_t> & ci,
+                 grid_desc & g) {
+	const int64_t n = (int64_t)rp.size() - 1;
+	if (n < 2) return false;
+	std::vector<int64_t> pos;
+	for (int64_t r = 0; r < n && pos.size() <= 3; ++r)
+		for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+			const int64_t o = ci[e] - r;
+			if (o > 0 && std::find(pos.begin(), pos.end(), o) == pos.end()) pos.push_back(o);
+		}
+	if (pos.empty() || pos.size() > 3) return false;
+	std::sort(pos.begin(), pos.end());
+	if (pos[0] != 1) return false;
+	g.nx = pos.size() >= 2 ? pos[1] : n;
+	g.nxy = pos.size() == 3 ? pos[2] : n;
+	if (g.nx < 2 || g.nxy % g.nx != 0 || n % g.nxy != 0) return false;
+	if (pos.size() == 3 && g.nxy / g.nx < 2) return false;
+	g.ny = g.nxy / g.nx;
+	g.nz = n / g.nxy;
+	const int64_t T = kGridTile;
+	if ((g.nx + T - 1) / T * ((g.ny + T - 1) / T) * ((g.nz + T - 1) / T) >
+	    std::numeric_limits<int32_t>::max())
+		return false;
+	g.ti = (int)((g.nx + T - 1) / T);
+	g.tj = (int)((g.ny + T - 1) / T);
+	g.tk = (int)((g.nz + T - 1) / T);
+	std::atomic<bool> ok{true};
+	for_row_blocks(n, setup_threads(n), [&](unsigned, int64_t lo, int64_t hi) {
+		for (int64_t r = lo; r < hi && ok.load(std::memory_order_relaxed); ++r) {
+			const int64_t i = r % g.nx, j = (r / g.nx) % g.ny, k = r / g.nxy;
+			int64_t prev = -1;
+			for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+				const int64_t c = ci[e], o = c - r;
+				bool good = c > prev;
+				if (o == 1) good = good && i + 1 < g.nx;
+				else if (o == -1) good = good && i >= 1;
+				else if (o == g.nx) good = good && j + 1 < g.ny;
+				else if (o == -g.nx) good = good && j >= 1;
+				else if (o == g.nxy) good = good && k + 1 < g.nz;
+				else if (o == -g.nxy) good = good && k >= 1;
+				else good = good && o == 0;
+				if (!good) {
+					ok = false;
+					break;
+				}
+				prev = c;
+			}
+		}
+	});
+	return ok;
+}
+
+// Tiles ordered by diagonal I + J + K.
+void build_tiles(fsx_ctx * ctx, fsx_gs & G) {
+	const grid_desc & g = G.gd;
+	const int nd = g.ti + g.tj + g.tk - 2;
+	std::vector<int64_t> cnt((size_t)nd + 1, 0);
+	for (int K = 0; K < g.tk; ++K)
+		for (int J = 0; J < g.tj; ++J)
+			for (int I = 0; I < g.ti; ++I) ++cnt[(size_t)(I + J + K) + 1];
+	G.dptr.assign((size_t)nd + 1, 0);
+	for (int d = 0; d < nd; ++d) G.dptr[(size_t)d + 1] = G.dptr[(size_t)d] + cnt[(size_t)d + 1];
+	std::vector<int64_t> fill(G.dptr.begin(), G.dptr.end() - 1);
+	std::vector<int32_t> ids((size_t)G.dptr.back());
+	for (int K = 0; K < g.tk; ++K)
+		for (int J = 0; J < g.tj; ++J)
+			for (int I = 0; I < g.ti; ++I)
+				ids[(size_t)fill[(size_t)(I + J + K)]++] = I + g.ti * (J + g.tj * K);
+	G.tile_ids = upload(ctx, ids);
+	const int64_t n = G.A->nrows;
+	G.gval = std::make_unique<dbuf>(std::max<int64_t>(1, n) * 7 * sizeof(double));
+	G.gmask = std::make_unique<dbuf>(std::max<int64_t>(1, n));
 }
 
 // The level-ordered copy of one sweep direction.  Entry order per row: the
@@ -257,6 +450,20 @@ void build_dir(fsx_ctx * ctx, const std::vector<int64_t> & rp, const std::vector
 void refresh_values(const fsx_gs & G) {
 	hipStream_t s = G.ctx->stream;
 	const double * vals = (const double *)G.A->vals->p;
+	if (G.tiles) {
+		const int64_t n = G.A->nrows;
+		const dim3 grid((unsigned)std::max<int64_t>(1, (n + 255) / 256));
+		if (G.A->it == FSX_I32)
+			hipLaunchKernelGGL((gs_grid_fill_kernel<int32_t>), grid, dim3(256), 0, s, n, G.gd,
+			                   (const int32_t *)G.A->rowptr(), (const int32_t *)G.A->colind(),
+			                   vals, (double *)G.gval->p, (uint8_t *)G.gmask->p);
+		else
+			hipLaunchKernelGGL((gs_grid_fill_kernel<int64_t>), grid, dim3(256), 0, s, n, G.gd,
+			                   (const int64_t *)G.A->rowptr(), (const int64_t *)G.A->colind(),
+			                   vals, (double *)G.gval->p, (uint8_t *)G.gmask->p);
+		FSX_HIP(hipGetLastError());
+		return;
+	}
 	for (const fsx_gs::dir * D : {&G.fw, &G.bw}) {
 		const size_t n = (size_t)G.A->nrows;
 		hipLaunchKernelGGL(gs_gather_kernel, dim3((unsigned)std::max<size_t>(1, (n + 255) / 256)),
@@ -275,6 +482,27 @@ void enqueue_sweep(const fsx_gs & G, bool fwd, const double * b, double * x) {
 	const fsx_csr * A = G.A;
 	hipStream_t s = G.ctx->stream;
 	const size_t n = (size_t)A->nrows;
+	if (G.tiles) {
+		// one launch per tile diagonal, in sweep order
+		const int nd = (int)G.dptr.size() - 1;
+		for (int q = 0; q < nd; ++q) {
+			const int d = fwd ? q : nd - 1 - q;
+			const int64_t first = G.dptr[(size_t)d];
+			const unsigned cnt = (unsigned)(G.dptr[(size_t)d + 1] - first);
+			if (fwd)
+				hipLaunchKernelGGL((gs_tile_kernel<true>), dim3(cnt), dim3(kGridThreads), 0, s,
+				                   G.gd, (const int32_t *)G.tile_ids->p, first,
+				                   (const double *)G.gval->p, (const uint8_t *)G.gmask->p,
+				                   (int64_t)n, b, x);
+			else
+				hipLaunchKernelGGL((gs_tile_kernel<false>), dim3(cnt), dim3(kGridThreads), 0, s,
+				                   G.gd, (const int32_t *)G.tile_ids->p, first,
+				                   (const double *)G.gval->p, (const uint8_t *)G.gmask->p,
+				                   (int64_t)n, b, x);
+		}
+		FSX_HIP(hipGetLastError());
+		return;
+	}
 	hipLaunchKernelGGL(gs_snapshot_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
 	                   n, (const double *)x, (double *)G.xold->p);
 	const fsx_gs::dir & D = fwd ? G.fw : G.bw;
@@ -354,11 +582,19 @@ int fsx_gs_create(fsx_ctx * ctx, const fsx_csr * A, int ncolours, fsx_gs ** out)
 			G->cm.m = n % ncolours;
 			G->cm.big = G->cm.m * (G->cm.q + 1);
 		}
+		// one colour on a grid: tiles (FSX_GS_TILES=0 keeps the level copies)
+		const char * te = getenv("FSX_GS_TILES");
+		G->tiles = ncolours == 1 && !(te && te[0] == '0') && detect_grid(rp, ci, G->gd);
 		for (bool fwd : {true, false}) {
 			fsx_gs::dir & D = fwd ? G->fw : G->bw;
-			if (isz == 4) build_dir<int32_t>(ctx, rp, ci, G->cm, ncolours > 1, fwd, D);
+			if (G->tiles) { // the level count only (fsx_gs_levels)
+				std::vector<int64_t> order;
+				schedule(rp, ci, G->cm, false, fwd, D.lev, order);
+			}
+			else if (isz == 4) build_dir<int32_t>(ctx, rp, ci, G->cm, ncolours > 1, fwd, D);
 			else build_dir<int64_t>(ctx, rp, ci, G->cm, ncolours > 1, fwd, D);
 		}
+		if (G->tiles) build_tiles(ctx, *G);
 		G->xold = std::make_unique<dbuf>(std::max<int64_t>(1, n) * sizeof(double));
 		refresh_values(*G);
 		G->version = A->vals_version;

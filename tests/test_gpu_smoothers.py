@@ -4,7 +4,10 @@ flecsolve/solvers/mg/gs.hh) against the oracle (gpu).
 Bar: bit-identical to the oracle's sequential sweep (the level-scheduled
 kernels do each row's arithmetic in the reference's order) for forward,
 backward and symmetric sweeps, one and several colours, int32 and int64
-indices, rows whose columns are not sorted; the launch-level count equals
+indices, rows whose columns are not sorted, and one-colour grid stencils
+swept in shared-memory tiles (with partial tiles, planes, lines and
+wrap-around couplings that fall back to the level copies); the level count
+equals
 the dependency depth the mathematics fixes (2n - 1 for the 5-point stencil
 on n^2, 3n - 2 for the 7-point stencil on n^3, 1 for a diagonal matrix).
 """
@@ -91,6 +94,54 @@ def test_graph_replay_and_value_update(ctx):
     x = ctx.vec(x0)
     G.apply(bd, x, sweep=2, nrelax=1)
     np.testing.assert_array_equal(x.get(), O.Csr(rp, ci, v2).hybrid_gs(b, x0, 2, 1))
+
+
+def grid_matrix(nx, ny, nz, seed, periodic_x=False, itype=np.int32):
+    """Random diagonally dominant 7-point matrix (5-point for nz = 1, 3-point
+    for ny = nz = 1) on an nx x ny x nz grid in natural order, columns
+    ascending; periodic_x adds the wrap-around couplings along x."""
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    rp, ci, v = [0], [], []
+    for r in range(n):
+        i, j, k = r % nx, (r // nx) % ny, r // (nx * ny)
+        cols = {r: 0.0}
+        for di, dj, dk in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            a, b2, c = i + di, j + dj, k + dk
+            if periodic_x and di and nx > 2:
+                a %= nx
+            if 0 <= a < nx and 0 <= b2 < ny and 0 <= c < nz:
+                cols[a + nx * (b2 + ny * c)] = -rng.random()
+        cols[r] = 7.0 + rng.random()
+        for c in sorted(cols):
+            ci.append(c)
+            v.append(cols[c])
+        rp.append(len(ci))
+    return np.array(rp, itype), np.array(ci, itype), np.array(v)
+
+
+@pytest.mark.parametrize("tiles", ["1", "0"])
+@pytest.mark.parametrize("dims,periodic,itype", [((21, 13, 9), False, np.int32),
+                                                 ((16, 8, 24), False, np.int64),
+                                                 ((37, 37, 1), False, np.int32),
+                                                 ((100, 1, 1), False, np.int32),
+                                                 ((12, 10, 6), True, np.int32)])
+def test_grid_tiles_bit_exact(ctx, monkeypatch, tiles, dims, periodic, itype):
+    """One-colour stencil matrices on a grid are swept in 8^3 tiles, one
+    launch per tile diagonal, cells relaxed in shared memory
+    (FSX_GS_TILES=0: the level copies instead): bit-identical to the oracle's
+    sequential sweep for grids that leave partial tiles in every dimension,
+    a plane, a line, int64 indices, and a matrix with wrap-around couplings
+    (not a grid stencil: the level copies either way); forward, backward and
+    symmetric, one and two repetitions."""
+    monkeypatch.setenv("FSX_GS_TILES", tiles)
+    rp, ci, v = grid_matrix(*dims, seed=sum(dims), periodic_x=periodic, itype=itype)
+    for sweep in (0, 1, 2):
+        for nrelax in (1, 2):
+            got, ref, G = run_both(ctx, rp, ci, v, sweep, nrelax, 1, seed=3 + sweep)
+            np.testing.assert_array_equal(got, ref)
+    if not periodic and dims[2] > 1:
+        assert G.levels() == (sum(dims) - 2, sum(dims) - 2)
 
 
 def test_errors(ctx):
