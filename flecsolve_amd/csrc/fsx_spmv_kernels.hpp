@@ -51,8 +51,12 @@
 //    FP64 0.202 / 0.209 ms; other shapes for the FP64 K1 (256- or
 //    1024-thread blocks one slice per wave, 256 or 512 threads striding over
 //    2-4 slices): 0.196-0.207 ms against 0.196 ms, steady state, for a plain
-//    product of 0.192 ms.  Rows keep their CSR order: bit-identical to the
-//    serial loop.
+//    product of 0.192 ms; x at -1 and +1 and the -1 mirror value taken from
+//    the neighbouring lanes (DPP wave shifts, one uniform load per slice
+//    edge; three of the fourteen loads a row issues): product 0.197 ->
+//    0.229 ms and CG K1 0.200 -> 0.513 ms (the reducing instance at 8 waves
+//    per SIMD).  Rows keep their CSR order: bit-identical to the serial
+//    loop.
 //
 // Workgroups are dealt to XCDs round-robin by block id.  Measured: keeping
 // that default order is faster than remapping blocks to give each XCD a
