@@ -55,42 +55,147 @@ struct colour_map {
 
 constexpr int kGsTile = 8;
 
-// One level: thread t relaxes the level's t-th row from the level-ordered
-// copy (entries loaded in tiles of kGsTile before use).
+// A row of a level in the level-ordered copy: its id, entry count, diagonal
+// and its first kRowW entries (tagged columns and values).  A level's ELL
+// block is `width` entries deep (padding: column tag -1, value 0), so all
+// kRowW loads go out at once, before the entry count arrives.
+constexpr int kRowW = 16;
+
 template<class I>
-__global__ __launch_bounds__(256) void gs_level_kernel(
-	const I * __restrict__ srow, const int32_t * __restrict__ slen,
-	const double * __restrict__ sdiag, const I * __restrict__ scol,
-	const double * __restrict__ sval, int64_t start, int64_t cnt, int64_t base,
-	const double * __restrict__ b, double * x, const double * __restrict__ xold) {
-	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if (t >= cnt) return;
+struct gs_row {
+	int64_t r;
+	int len;
+	double diag;
+	I c[kRowW];
+	double v[kRowW];
+};
+
+// Load row t of the level that starts at position `start` (cnt rows, ELL
+// block at `base`, `width` entries deep).
+template<class I>
+__device__ __forceinline__ void gs_load_row(gs_row<I> & m, const I * __restrict__ srow,
+                                            const int32_t * __restrict__ slen,
+                                            const double * __restrict__ sdiag,
+                                            const I * __restrict__ scol,
+                                            const double * __restrict__ sval, int64_t start,
+                                            int64_t t, int64_t cnt, int64_t base, int width) {
 	const int64_t p = start + t;
-	const int64_t r = srow[p];
-	const int len = slen[p];
-	const double diag = sdiag[p];
+	m.r = srow[p];
+	m.len = slen[p];
+	m.diag = sdiag[p];
 	const I * cp = scol + base + t;
 	const double * vp = sval + base + t;
-	const double br = b[r];
+#pragma unroll
+	for (int k = 0; k < kRowW; ++k)
+		if (k < width) {
+			m.c[k] = cp[k * cnt];
+			m.v[k] = vp[k * cnt];
+		}
+}
+
+// Relax the row: b and the x gathers, the sum in entry order (entries past
+// kRowW loaded in tiles of kGsTile), the update.
+template<class I>
+__device__ __forceinline__ void gs_finish_row(const gs_row<I> & m, const I * __restrict__ scol,
+                                              const double * __restrict__ sval, int64_t t,
+                                              int64_t cnt, int64_t base,
+                                              const double * __restrict__ b, double * x,
+                                              const double * __restrict__ xold) {
+	const double br = b[m.r];
+	double xv[kRowW];
+#pragma unroll
+	for (int k = 0; k < kRowW; ++k)
+		if (k < m.len)
+			xv[k] = (m.c[k] >= 0 ? x : xold)[m.c[k] >= 0 ? (int64_t)m.c[k] : -(int64_t)m.c[k] - 1];
 	double rsum = 0.;
-	for (int j0 = 0; j0 < len; j0 += kGsTile) {
+#pragma unroll
+	for (int k = 0; k < kRowW; ++k)
+		if (k < m.len) rsum += m.v[k] * xv[k];
+	const I * cp = scol + base + t;
+	const double * vp = sval + base + t;
+	for (int j0 = kRowW; j0 < m.len; j0 += kGsTile) {
 		I c[kGsTile];
-		double v[kGsTile], xv[kGsTile];
+		double v[kGsTile], xt[kGsTile];
 #pragma unroll
 		for (int k = 0; k < kGsTile; ++k) {
-			const int64_t j = min(j0 + k, len - 1);
+			const int64_t j = min(j0 + k, m.len - 1);
 			c[k] = cp[j * cnt];
 			v[k] = vp[j * cnt];
 		}
 #pragma unroll
 		for (int k = 0; k < kGsTile; ++k)
-			xv[k] = (c[k] >= 0 ? x : xold)[c[k] >= 0 ? (int64_t)c[k] : -(int64_t)c[k] - 1];
+			xt[k] = (c[k] >= 0 ? x : xold)[c[k] >= 0 ? (int64_t)c[k] : -(int64_t)c[k] - 1];
 #pragma unroll
 		for (int k = 0; k < kGsTile; ++k)
-			if (j0 + k < len) rsum += v[k] * xv[k];
+			if (j0 + k < m.len) rsum += v[k] * xt[k];
 	}
-	const double dinv = 1. / diag;
-	x[r] = dinv * (br - rsum);
+	const double dinv = 1. / m.diag;
+	x[m.r] = dinv * (br - rsum);
+}
+
+// One level: thread t relaxes the level's t-th row.
+template<class I>
+__global__ __launch_bounds__(256) void gs_level_kernel(
+	const I * __restrict__ srow, const int32_t * __restrict__ slen,
+	const double * __restrict__ sdiag, const I * __restrict__ scol,
+	const double * __restrict__ sval, int64_t start, int64_t cnt, int64_t base, int width,
+	const double * __restrict__ b, double * x, const double * __restrict__ xold) {
+	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if (t >= cnt) return;
+	gs_row<I> m;
+	gs_load_row<I>(m, srow, slen, sdiag, scol, sval, start, t, cnt, base, width);
+	gs_finish_row<I>(m, scol, sval, t, cnt, base, b, x, xold);
+}
+
+// A run of consecutive small levels (at most kRunThreads rows each) in one
+// workgroup: the levels one after another, a barrier between (it makes this
+// level's x stores visible to the whole workgroup before the next level
+// reads them), thread t relaxing each level's t-th row.  The next level's
+// row (id, count, diagonal, entries: nothing that depends on x) is loaded
+// while this level's x gathers are in flight.  Measured on the coarse
+// levels of a 256^3 aggregation hierarchy: ~1.7 us per level in a run
+// against ~2.4 us per launch of a small level in a captured graph; levels
+// of more than a few hundred rows keep a launch each (one workgroup would
+// relax them row after row).  Level offsets, ELL bases and widths of the
+// run are staged in shared memory first.
+constexpr int kRunThreads = 512;
+constexpr int kRunMaxLevels = 1024;
+constexpr int64_t kRunRowsDefault = 256;
+
+template<class I>
+__global__ __launch_bounds__(kRunThreads) void gs_run_kernel(
+	const I * __restrict__ srow, const int32_t * __restrict__ slen,
+	const double * __restrict__ sdiag, const I * __restrict__ scol,
+	const double * __restrict__ sval, const int64_t * __restrict__ lev,
+	const int64_t * __restrict__ lbase, const int32_t * __restrict__ lwidth, int64_t l0,
+	int nlev, const double * __restrict__ b, double * x, const double * __restrict__ xold) {
+	__shared__ int64_t s_lev[kRunMaxLevels + 1], s_base[kRunMaxLevels];
+	__shared__ int32_t s_w[kRunMaxLevels];
+	for (int i = threadIdx.x; i <= nlev; i += kRunThreads) {
+		s_lev[i] = lev[l0 + i];
+		if (i < nlev) {
+			s_base[i] = lbase[l0 + i];
+			s_w[i] = lwidth[l0 + i];
+		}
+	}
+	__syncthreads();
+	const int64_t t = threadIdx.x;
+	gs_row<I> cur, nxt;
+	if (t < s_lev[1] - s_lev[0])
+		gs_load_row<I>(cur, srow, slen, sdiag, scol, sval, s_lev[0], t, s_lev[1] - s_lev[0],
+		               s_base[0], s_w[0]);
+	for (int l = 0; l < nlev; ++l) {
+		const int64_t cnt = s_lev[l + 1] - s_lev[l];
+		if (l + 1 < nlev) {
+			const int64_t c1 = s_lev[l + 2] - s_lev[l + 1];
+			if (t < c1)
+				gs_load_row<I>(nxt, srow, slen, sdiag, scol, sval, s_lev[l + 1], t, c1,
+				               s_base[l + 1], s_w[l + 1]);
+		}
+		if (t < cnt) gs_finish_row<I>(cur, scol, sval, t, cnt, s_base[l], b, x, xold);
+		__syncthreads();
+		if (l + 1 < nlev) cur = nxt;
+	}
 }
 
 // Refresh the copy's values from the matrix (entry index per slot, -1 for
@@ -231,9 +336,13 @@ struct fsx_gs {
 	// values) with the matrix entry each value came from
 	struct dir {
 		std::vector<int64_t> lev, base;
-		std::unique_ptr<dbuf> srow, slen, sdiag, scol, sval, dfrom, vfrom;
+		std::vector<int32_t> width; // ELL depth of each level
+		std::unique_ptr<dbuf> srow, slen, sdiag, scol, sval, dfrom, vfrom, dlev, dbase, dwidth;
 		size_t nslots = 0;
 	} fw, bw;
+	// consecutive levels of at most this many rows (<= kRunThreads) run in
+	// one workgroup (gs_run_kernel); 0: a launch per level
+	int64_t run_rows = 0;
 	std::unique_ptr<dbuf> xold;
 	// stencil matrices on a grid (one colour): tiles swept by diagonal;
 	// tile_ids ordered by diagonal, diagonal d's at [dptr[d], dptr[d + 1]),
@@ -438,6 +547,14 @@ void build_dir(fsx_ctx * ctx, const std::vector<int64_t> & rp, const std::vector
 		}
 	});
 	D.nslots = nslots;
+	D.width.resize(nl);
+	for (size_t l = 0; l < nl; ++l) {
+		FSX_REQUIRE(width[l] <= std::numeric_limits<int32_t>::max(), "hybrid_gs: row too long");
+		D.width[l] = (int32_t)width[l];
+	}
+	D.dlev = upload(ctx, D.lev);
+	D.dbase = upload(ctx, D.base);
+	D.dwidth = upload(ctx, D.width);
 	D.srow = upload(ctx, srow);
 	D.slen = upload(ctx, slen);
 	D.scol = upload(ctx, scol);
@@ -507,13 +624,32 @@ void enqueue_sweep(const fsx_gs & G, bool fwd, const double * b, double * x) {
 	                   n, (const double *)x, (double *)G.xold->p);
 	const fsx_gs::dir & D = fwd ? G.fw : G.bw;
 	const double * xo = (const double *)G.xold->p;
-	for (size_t l = 0; l + 1 < D.lev.size(); ++l) {
+	const size_t nl = D.lev.size() - 1;
+	auto small = [&](size_t l) {
+		return G.run_rows > 0 && D.lev[l + 1] - D.lev[l] <= G.run_rows;
+	};
+	for (size_t l = 0; l < nl;) {
+		size_t e = l + 1; // a run: consecutive small levels, at most kRunMaxLevels
+		if (small(l))
+			while (e < nl && e - l < (size_t)kRunMaxLevels && small(e)) ++e;
+		if (e - l >= 2) {
+			hipLaunchKernelGGL((gs_run_kernel<I>), dim3(1), dim3(kRunThreads), 0, s,
+			                   (const I *)D.srow->p, (const int32_t *)D.slen->p,
+			                   (const double *)D.sdiag->p, (const I *)D.scol->p,
+			                   (const double *)D.sval->p, (const int64_t *)D.dlev->p,
+			                   (const int64_t *)D.dbase->p, (const int32_t *)D.dwidth->p, (int64_t)l,
+			                   (int)(e - l), b, x, xo);
+			l = e;
+			continue;
+		}
 		const int64_t cnt = D.lev[l + 1] - D.lev[l];
 		const unsigned g = (unsigned)((cnt + 255) / 256);
 		hipLaunchKernelGGL((gs_level_kernel<I>), dim3(g), dim3(256), 0, s,
 		                   (const I *)D.srow->p, (const int32_t *)D.slen->p,
 		                   (const double *)D.sdiag->p, (const I *)D.scol->p,
-		                   (const double *)D.sval->p, D.lev[l], cnt, D.base[l], b, x, xo);
+		                   (const double *)D.sval->p, D.lev[l], cnt, D.base[l], D.width[l], b, x,
+		                   xo);
+		++l;
 	}
 	FSX_HIP(hipGetLastError());
 }
@@ -585,6 +721,9 @@ int fsx_gs_create(fsx_ctx * ctx, const fsx_csr * A, int ncolours, fsx_gs ** out)
 		// one colour on a grid: tiles (FSX_GS_TILES=0 keeps the level copies)
 		const char * te = getenv("FSX_GS_TILES");
 		G->tiles = ncolours == 1 && !(te && te[0] == '0') && detect_grid(rp, ci, G->gd);
+		const char * re = getenv("FSX_GS_RUN_ROWS");
+		G->run_rows = std::min<int64_t>(kRunThreads,
+		                                re ? std::max<int64_t>(0, atoll(re)) : kRunRowsDefault);
 		for (bool fwd : {true, false}) {
 			fsx_gs::dir & D = fwd ? G->fw : G->bw;
 			if (G->tiles) { // the level count only (fsx_gs_levels)

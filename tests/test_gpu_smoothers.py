@@ -144,6 +144,46 @@ def test_grid_tiles_bit_exact(ctx, monkeypatch, tiles, dims, periodic, itype):
         assert G.levels() == (sum(dims) - 2, sum(dims) - 2)
 
 
+def pairs_matrix(n, seed):
+    """Rows 2i and 2i+1 coupled, nothing else: two levels of n/2 rows each."""
+    rng = np.random.default_rng(seed)
+    rp, ci, v = [0], [], []
+    for r in range(n):
+        o = r ^ 1
+        cols = sorted({r, o})
+        for c in cols:
+            ci.append(c)
+            v.append(4.0 + rng.random() if c == r else -rng.random())
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+
+
+@pytest.mark.parametrize("run_rows", ["0", "1", "7", "512"])
+def test_level_runs_bit_exact(ctx, monkeypatch, run_rows):
+    """Consecutive levels of at most FSX_GS_RUN_ROWS rows (at most 512) run
+    in one workgroup with a barrier between levels, the next level's rows
+    loaded ahead (0: a launch per level, 1: only one-row levels grouped, 512:
+    every level that fits, runs split every 1024 levels): bit-identical to
+    the oracle's sequential sweep on random rows (one and three colours; up
+    to ~50 entries a row, past the 16 loaded at once), a chain of 2500
+    one-row levels, a 3D stencil, two levels of 500 rows and two levels of
+    3000 rows (a launch each)."""
+    monkeypatch.setenv("FSX_GS_TILES", "0")
+    monkeypatch.setenv("FSX_GS_RUN_ROWS", run_rows)
+    rr = random_csr(500, 0.02, 17)[:3]
+    dense = random_csr(300, 0.1, 19)[:3]
+    assert np.diff(dense[0]).max() > 16 + 8
+    cases = [(rr, 1), (rr, 3), (dense, 1), (dense, 2), (grid_matrix(2500, 1, 1, seed=5), 1),
+             (P.diffusion3d(12, P.lognormal_field(12, 1.0, 4.0, 8)), 1),
+             (pairs_matrix(1000, 6), 1), (pairs_matrix(6000, 6), 1)]
+    for (rp, ci, v), ncolours in cases:
+        for sweep in (0, 1, 2):
+            for nrelax in (1, 2):
+                got, ref, G = run_both(ctx, rp, ci, v, sweep, nrelax, ncolours, seed=7 + sweep)
+                np.testing.assert_array_equal(got, ref)
+    assert G.levels() == (2, 2)
+
+
 def test_errors(ctx):
     rp, ci, v = P.poisson2d(8)
     A = ctx.csr(rp, ci, v)
