@@ -29,6 +29,23 @@ inline unsigned setup_threads(int64_t n) {
 	return (unsigned)std::max<int64_t>(1, std::min<int64_t>(t, n / 4096));
 }
 
+// body(t) on threads t = 0 .. nt-1, all joined before returning (also when
+// starting a thread fails: the ones started are joined, then the error is
+// rethrown).  body must not throw.
+template<class B>
+void run_threads(unsigned nt, B && body) {
+	std::vector<std::thread> th;
+	th.reserve(nt);
+	try {
+		for (unsigned t = 0; t < nt; ++t) th.emplace_back([&body, t] { body(t); });
+	}
+	catch (...) {
+		for (auto & x : th) x.join();
+		throw;
+	}
+	for (auto & x : th) x.join();
+}
+
 // fn(t, lo, hi) on rows [lo, hi): contiguous blocks, block t on thread t.
 // An exception in a block is rethrown on the calling thread after the join.
 template<class F>
@@ -37,19 +54,15 @@ void for_row_blocks(int64_t n, unsigned nt, F && fn) {
 		fn(0u, (int64_t)0, n);
 		return;
 	}
-	std::vector<std::thread> th;
 	std::vector<std::exception_ptr> err(nt);
-	th.reserve(nt);
-	for (unsigned t = 0; t < nt; ++t)
-		th.emplace_back([&, t] {
-			try {
-				fn(t, n * t / nt, n * (t + 1) / nt);
-			}
-			catch (...) {
-				err[t] = std::current_exception();
-			}
-		});
-	for (auto & x : th) x.join();
+	run_threads(nt, [&](unsigned t) {
+		try {
+			fn(t, n * t / nt, n * (t + 1) / nt);
+		}
+		catch (...) {
+			err[t] = std::current_exception();
+		}
+	});
 	for (auto & e : err)
 		if (e) std::rethrow_exception(e);
 }
@@ -64,19 +77,15 @@ void for_items(size_t count, unsigned nt, F && fn) {
 		return;
 	}
 	std::atomic<size_t> next{0};
-	std::vector<std::thread> th;
 	std::vector<std::exception_ptr> err(nt);
-	th.reserve(nt);
-	for (unsigned t = 0; t < nt; ++t)
-		th.emplace_back([&, t] {
-			try {
-				for (size_t i; (i = next.fetch_add(1)) < count;) fn(i);
-			}
-			catch (...) {
-				err[t] = std::current_exception();
-			}
-		});
-	for (auto & x : th) x.join();
+	run_threads(nt, [&](unsigned t) {
+		try {
+			for (size_t i; (i = next.fetch_add(1)) < count;) fn(i);
+		}
+		catch (...) {
+			err[t] = std::current_exception();
+		}
+	});
 	for (auto & e : err)
 		if (e) std::rethrow_exception(e);
 }
