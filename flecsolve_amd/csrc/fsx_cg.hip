@@ -1559,9 +1559,12 @@ static void bcg_run_chunks(fsx_ctx * ctx, const bcg_bufs & B, bcg_state & st_hos
 //   G0  v = A P b_k (right; P = D^-1 enters as a scaled gather
 //       0 + dinv[c] b_k[c], bitwise the z the reference forms) or
 //       v = P A b_k (left); partials of v.b_0
-//   then for j = 0..k: a one-block kernel stores h_jk, and
-//   Gj  v = v - h_jk b_j; partials of v.b_{j+1} (j < k) or |v|^2 (j = k)
-//   GN  h_{k+1,k} = |v|; b_{k+1} = v / |v| (v if |v| = 0)
+//   F0  a one-block kernel stores h_0k
+//   then for j = 0..k:
+//   Gj  v = v - h_jk b_j; partials of v.b_{j+1} (j < k) or |v|^2 (j = k);
+//       for j > 0 each block first sums G(j-1)'s partials into h_jk itself
+//   GN  h_{k+1,k} = |v| from Gk's partials (each block); b_{k+1} = v / |v|
+//       (v if |v| = 0)
 //   GR  one thread: previous rotations on column k, the new rotation, the
 //       rhs update, |dw_{k+1}| test, iteration count, end of cycle.
 // A cycle's m steps are captured in one graph; the host runs back_solve,
@@ -1619,38 +1622,78 @@ struct fin_gm_h {
 	}
 };
 
-// v = v - h b_j; partials of v.b_next (or |v|^2 when b_next is null)
+// rows per thread of the Gram-Schmidt passes (fewer blocks: fewer partials
+// for every block of the next pass to sum)
+constexpr int kGmPerThread = 8;
+
+// v = v - h_jk b_j; partials of v.b_next (or |v|^2 when b_next is null)
+// into `partials`.  h_jk is *hjk (j = 0: stored by the product's finalize)
+// or, when `prev` is given, the sum of the previous pass's nprev partials,
+// which every block forms in the same order while its vector loads are in
+// flight (block 0 stores it at *hjk for the rotations): no finalize launch
+// between passes.
 __global__ __launch_bounds__(kUpdThreads) void gm_mgs_kernel(
 	size_t n, double * __restrict__ v, const double * __restrict__ bj,
-	const double * __restrict__ bnext, const double * hjk, double * partials,
-	const gm_state * st) {
+	const double * __restrict__ bnext, const double * __restrict__ prev, unsigned nprev,
+	double * hjk, double * __restrict__ partials, const gm_state * st) {
 	if (st->done) return;
-	const double h = *hjk;
 	const unsigned blk = blockIdx.x;
-	const size_t base = (size_t)blk * kUpdThreads * kUpdPerThread + threadIdx.x;
-	double acc = 0.0;
+	const size_t base = (size_t)blk * kUpdThreads * kGmPerThread + threadIdx.x;
+	double bv[kGmPerThread], vv[kGmPerThread], nv[kGmPerThread];
 #pragma unroll
-	for (int j = 0; j < kUpdPerThread; ++j) {
+	for (int j = 0; j < kGmPerThread; ++j) {
 		const size_t i = base + (size_t)j * kUpdThreads;
 		if (i < n) {
-			const double vi = (-h) * bj[i] + v[i]; // v.axpy(-h_jk, basis[j], v)
+			bv[j] = bj[i];
+			vv[j] = v[i];
+			nv[j] = bnext ? bnext[i] : 0.0;
+		}
+	}
+	double h;
+	if (prev) {
+		double t[1];
+		sum_partials<1, kUpdThreads>(prev, nprev, t);
+		h = t[0];
+		if (blk == 0 && threadIdx.x == 0) *hjk = h;
+	}
+	else
+		h = *hjk;
+	double acc = 0.0;
+#pragma unroll
+	for (int j = 0; j < kGmPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) {
+			const double vi = (-h) * bv[j] + vv[j]; // v.axpy(-h_jk, basis[j], v)
 			v[i] = vi;
-			acc += bnext ? vi * bnext[i] : vi * vi;
+			acc += bnext ? vi * nv[j] : vi * vi;
 		}
 	}
 	double d[1] = {acc};
 	store_block_partials<1>(d, partials, blk);
 }
 
-// b_{k+1} = v scaled by 1/|v| (gmres.hh:140-146)
-__global__ __launch_bounds__(256) void gm_scale_kernel(size_t n, const double * __restrict__ v,
-                                                       double * __restrict__ bnext,
-                                                       const double * hnorm,
-                                                       const gm_state * st) {
+// h_{k+1,k} = |v| from the last pass's nprev partials (every block, same
+// order; block 0 stores it) and b_{k+1} = v scaled by 1/|v| (gmres.hh:140-146)
+__global__ __launch_bounds__(kUpdThreads) void gm_scale_kernel(
+	size_t n, const double * __restrict__ v, double * __restrict__ bnext,
+	const double * __restrict__ prev, unsigned nprev, double * hnorm, const gm_state * st) {
 	if (st->done) return;
-	const double vn = *hnorm;
-	const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-	if (i < n) bnext[i] = vn != 0.0 ? (1.0 / vn) * v[i] : v[i];
+	const size_t base = (size_t)blockIdx.x * kUpdThreads * kGmPerThread + threadIdx.x;
+	double vv[kGmPerThread];
+#pragma unroll
+	for (int j = 0; j < kGmPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) vv[j] = v[i];
+	}
+	double t[1];
+	sum_partials<1, kUpdThreads>(prev, nprev, t);
+	const double vn = sqrt(t[0]);
+	if (blockIdx.x == 0 && threadIdx.x == 0) *hnorm = vn;
+#pragma unroll
+	for (int j = 0; j < kGmPerThread; ++j) {
+		const size_t i = base + (size_t)j * kUpdThreads;
+		if (i < n) bnext[i] = vn != 0.0 ? (1.0 / vn) * vv[j] : vv[j];
+	}
 }
 
 // Givens rotations of column k, |dw_{k+1}| test (gmres.hh:148-208, 282-331)
@@ -1750,9 +1793,11 @@ template<class V, class I>
 static void gm_enqueue_cycle(fsx_ctx * ctx, const gm_bufs & B, gm_state * st) {
 	hipStream_t s = ctx->stream;
 	double * part = ctx->ws.partials;
-	const size_t per = (size_t)kUpdThreads * kUpdPerThread;
+	const size_t per = (size_t)kUpdThreads * kGmPerThread;
 	const unsigned gu = (unsigned)((B.n + per - 1) / per);
-	const unsigned g1 = (unsigned)((B.n + 255) / 256);
+	// the passes' partials alternate between two buffers, so that a pass
+	// never overwrites the ones its blocks are still summing
+	double * pp[2] = {part, part + gu};
 	for (int k = 0; k < B.m; ++k) {
 		const double * bk = B.basis[k];
 		// G0: v = A P b_k / P A b_k, partials of v.b_0
@@ -1768,26 +1813,18 @@ static void gm_enqueue_cycle(fsx_ctx * ctx, const gm_bufs & B, gm_state * st) {
 			epi_gm_v<double, double, false> e{B.v, B.basis[0], nullptr, part, st};
 			spmv_launch_typed<decltype(e), double, V, I, double>(s, B.A, bk, e);
 		}
-		for (int j = 0; j <= k; ++j) {
-			// h_jk from the partials just written (the product's, then MGS's)
-			const fin_gm_h fh{st, B.H, j + B.ld * k, false};
-			if (j == 0)
-				spmv_finalize<1>(s, B.A, part, fh);
-			else
-				hipLaunchKernelGGL((finalize_kernel<1, fin_gm_h>), dim3(1), dim3(kFinThreads),
-				                   0, s, (const double *)part, gu, fh);
+		// h_0k from the product's partials
+		spmv_finalize<1>(s, B.A, part, fin_gm_h{st, B.H, B.ld * k, false});
+		for (int j = 0; j <= k; ++j)
 			hipLaunchKernelGGL(gm_mgs_kernel, dim3(gu), dim3(kUpdThreads), 0, s, B.n, B.v,
 			                   (const double *)B.basis[j],
 			                   j < k ? (const double *)B.basis[j + 1] : nullptr,
-			                   (const double *)(B.H + j + (size_t)B.ld * k), part,
-			                   (const gm_state *)st);
-		}
+			                   j > 0 ? (const double *)pp[(j - 1) & 1] : nullptr, gu,
+			                   B.H + j + (size_t)B.ld * k, pp[j & 1], (const gm_state *)st);
 		// GN: h_{k+1,k} = |v|; b_{k+1} = v / |v|
-		hipLaunchKernelGGL((finalize_kernel<1, fin_gm_h>), dim3(1), dim3(kFinThreads), 0, s,
-		                   (const double *)part, gu, fin_gm_h{st, B.H, k + 1 + B.ld * k, true});
-		hipLaunchKernelGGL(gm_scale_kernel, dim3(g1), dim3(256), 0, s, B.n, (const double *)B.v,
-		                   B.basis[k + 1], (const double *)(B.H + k + 1 + (size_t)B.ld * k),
-		                   (const gm_state *)st);
+		hipLaunchKernelGGL(gm_scale_kernel, dim3(gu), dim3(kUpdThreads), 0, s, B.n,
+		                   (const double *)B.v, B.basis[k + 1], (const double *)pp[k & 1], gu,
+		                   B.H + k + 1 + (size_t)B.ld * k, (const gm_state *)st);
 		// GR
 		hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(64), 0, s, B.H, B.ld, B.cs, B.sn,
 		                   B.dw, k, st);
@@ -2515,8 +2552,8 @@ int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 		gm_state * st = (gm_state *)cache.statebuf->p;
 		gm_state * pinned = (gm_state *)ctx->host_scalars;
 		static_assert(sizeof(gm_state) <= sizeof(cg_state) * 4, "");
-		const size_t per = (size_t)kUpdThreads * kUpdPerThread;
-		ensure_partials(ctx, std::max<size_t>(spmv_partials_size(A, 1), (n + per - 1) / per));
+		const size_t per = (size_t)kUpdThreads * kGmPerThread;
+		ensure_partials(ctx, std::max<size_t>(spmv_partials_size(A, 1), 2 * ((n + per - 1) / per)));
 		gm_state sh{};
 		sh.tol = tol;
 		sh.m = max_dim;
@@ -2608,7 +2645,7 @@ int fsx_gmres_solve(fsx_ctx * ctx, const fsx_csr * A, const fsx_vec * dinv,
 				if (ge) FSX_HIP(hipGraphLaunch(ge, ctx->stream));
 				else gm_enqueue_cycle<V, I>(ctx, B, st);
 			});
-			ctx->launches += (size_t)max_dim * (max_dim + 7);
+			ctx->launches += (size_t)max_dim * (max_dim + 9) / 2;
 			FSX_HIP(hipMemcpyAsync(pinned, st, sizeof(gm_state), hipMemcpyDeviceToHost,
 			                       ctx->stream));
 			FSX_HIP(hipStreamSynchronize(ctx->stream));

@@ -335,32 +335,33 @@ __device__ __forceinline__ void store_block_partials(const double (&d)[NV],
 
 constexpr int kFinThreads = 1024;
 
-// Sum of nblk*NV partials by one block of kFinThreads threads.  Thread t
-// owns blocks t, t + kFinThreads, ...; it keeps kFinUnroll independent
-// accumulators so that many loads are in flight (the reads are latency
-// bound), folds them in a fixed order, then a wave tree and the waves in
-// order.  The order depends only on nblk: deterministic.  Result valid in
-// every thread.
+// Sum of nblk*NV partials by one block of NT threads (kFinThreads unless
+// stated).  Thread t owns blocks t, t + NT, ...; it keeps kFinUnroll
+// independent accumulators so that many loads are in flight (the reads are
+// latency bound), folds them in a fixed order, then a wave tree and the
+// waves in order.  The order depends only on nblk and NT: deterministic, so
+// every block that sums the same partials gets the same bits.  Result valid
+// in every thread.
 constexpr int kFinUnroll = 8;
 
-template<int NV>
+template<int NV, int NT = kFinThreads>
 __device__ __forceinline__ void sum_partials(const double * partials,
                                              unsigned nblk, double (&tot)[NV]) {
-	__shared__ double red[NV][kFinThreads / 64];
+	__shared__ double red[NV][NT / 64];
 	double acc[NV][kFinUnroll];
 #pragma unroll
 	for (int i = 0; i < NV; ++i)
 #pragma unroll
 		for (int u = 0; u < kFinUnroll; ++u) acc[i][u] = 0.0;
 	unsigned b = threadIdx.x;
-	for (; b + (kFinUnroll - 1) * kFinThreads < nblk; b += kFinUnroll * kFinThreads) {
+	for (; b + (kFinUnroll - 1) * NT < nblk; b += kFinUnroll * NT) {
 #pragma unroll
 		for (int u = 0; u < kFinUnroll; ++u)
 #pragma unroll
 			for (int i = 0; i < NV; ++i)
-				acc[i][u] += partials[(size_t)(b + u * kFinThreads) * NV + i];
+				acc[i][u] += partials[(size_t)(b + u * NT) * NV + i];
 	}
-	for (; b < nblk; b += kFinThreads)
+	for (; b < nblk; b += NT)
 #pragma unroll
 		for (int i = 0; i < NV; ++i) acc[i][0] += partials[(size_t)b * NV + i];
 #pragma unroll
@@ -375,7 +376,7 @@ __device__ __forceinline__ void sum_partials(const double * partials,
 #pragma unroll
 	for (int i = 0; i < NV; ++i) {
 		double s = 0.0;
-		for (int w = 0; w < kFinThreads / 64; ++w) s += red[i][w];
+		for (int w = 0; w < NT / 64; ++w) s += red[i][w];
 		tot[i] = s;
 	}
 }
